@@ -1,0 +1,167 @@
+/*
+ * libvipe_hip — C ABI of the MI355X (gfx950) SLAM inner loop.
+ *
+ * Plain pointers and sizes only.  Every compute entry point takes the HIP
+ * stream to launch on (a hipStream_t passed as void*; NULL = default stream),
+ * never synchronises unless stated, and returns VH_OK or an error code whose
+ * text is available from vh_last_error().  Tensors are dense row-major
+ * (C-contiguous) buffers in device memory unless the argument says host.
+ *
+ * Each declaration names the reference interface it replaces
+ * (nv-tlabs/vipe, paths relative to its source tree).
+ */
+#ifndef VIPE_HIP_H
+#define VIPE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status / dtype / placement codes ---------------------------------- */
+#define VH_OK 0
+#define VH_ERR_ARG 1         /* bad argument: shape, range, null pointer      */
+#define VH_ERR_HIP 2         /* HIP runtime error (text in vh_last_error)     */
+#define VH_ERR_UNSUPPORTED 3 /* dtype / size outside what a kernel handles    */
+#define VH_ERR_UNDEFINED 4   /* reference returns an undefined tensor here    */
+
+#define VH_F16 1
+#define VH_F32 2
+#define VH_F64 3
+#define VH_BF16 4
+#define VH_FP8E4M3 5
+
+#define VH_DEVICE 0 /* pointers are device memory, kernels on `stream` */
+#define VH_HOST 1   /* pointers are host memory, computed on the CPU   */
+
+/* Text of the last error raised on the calling thread. */
+const char* vh_last_error(void);
+/* ABI version; bumped when a signature changes. */
+int vh_abi_version(void);
+/* Stamp kernel: writes `value` into out[0..n) on `stream` (plumbing probe). */
+int vh_fill_i32(int32_t* out, int64_t n, int32_t value, void* stream);
+
+/* ---- lietorch_ext (csrc/lietorch_ext/lietorch.cpp:23-335) ---------------
+ * group: 1 SO3 [qx,qy,qz,qw], 2 RxSO3 [q,s], 3 SE3 [t,q], 4 Sim3 [t,q,s]
+ *        (vipe/ext/lietorch/groups.py:244,260,276,298).
+ * dtype: VH_F32 or VH_F64.  place: VH_DEVICE or VH_HOST (the reference's
+ * lietorch_cpu.cpp path for CPU tensors).  n = number of elements.
+ * Embedding N and tangent K per group: (4,3) (5,4) (7,6) (8,7).
+ * Gradients w.r.t. group elements are K-vectors written at stride N into
+ * buffers the caller zero-fills; upstream gradients of group outputs are read
+ * at stride N (lietorch_gpu.cu:38-46, 95-98).  An unknown group id or dtype
+ * returns VH_ERR_UNDEFINED (the reference returns an undefined tensor,
+ * dispatch.h:40-50).
+ */
+int vh_lie_expm(int group, int dtype, int64_t n, const void* a, void* X, int place, void* stream);          /* lietorch.cpp:23  expm          */
+int vh_lie_expm_backward(int group, int dtype, int64_t n, const void* grad, const void* a, void* da, int place, void* stream); /* :36 expm_backward */
+int vh_lie_logm(int group, int dtype, int64_t n, const void* X, void* a, int place, void* stream);          /* :50  logm          */
+int vh_lie_logm_backward(int group, int dtype, int64_t n, const void* grad, const void* X, void* dX, int place, void* stream); /* :63 logm_backward */
+int vh_lie_inv(int group, int dtype, int64_t n, const void* X, void* Y, int place, void* stream);           /* :78  inv           */
+int vh_lie_inv_backward(int group, int dtype, int64_t n, const void* grad, const void* X, void* dX, int place, void* stream); /* :91 inv_backward */
+int vh_lie_mul(int group, int dtype, int64_t n, const void* X, const void* Y, void* Z, int place, void* stream); /* :108 mul       */
+int vh_lie_mul_backward(int group, int dtype, int64_t n, const void* grad, const void* X, const void* Y, void* dX, void* dY, int place, void* stream); /* :123 */
+int vh_lie_adj(int group, int dtype, int64_t n, const void* X, const void* a, void* b, int place, void* stream); /* :139 adj        */
+int vh_lie_adj_backward(int group, int dtype, int64_t n, const void* grad, const void* X, const void* a, void* dX, void* da, int place, void* stream); /* :154 */
+int vh_lie_adjT(int group, int dtype, int64_t n, const void* X, const void* a, void* b, int place, void* stream); /* :170 adjT      */
+int vh_lie_adjT_backward(int group, int dtype, int64_t n, const void* grad, const void* X, const void* a, void* dX, void* da, int place, void* stream); /* :185 */
+int vh_lie_act(int group, int dtype, int64_t n, const void* X, const void* p, void* q, int place, void* stream); /* :201 act (3-pt) */
+int vh_lie_act_backward(int group, int dtype, int64_t n, const void* grad, const void* X, const void* p, void* dX, void* dp, int place, void* stream); /* :216 */
+int vh_lie_act4(int group, int dtype, int64_t n, const void* X, const void* p, void* q, int place, void* stream); /* :232 act4 */
+int vh_lie_act4_backward(int group, int dtype, int64_t n, const void* grad, const void* X, const void* p, void* dX, void* dp, int place, void* stream); /* :247 */
+int vh_lie_as_matrix(int group, int dtype, int64_t n, const void* X, void* T, int place, void* stream);     /* :277 as_matrix (row-major 4x4) */
+int vh_lie_projector(int group, int dtype, int64_t n, const void* X, void* Pm, int place, void* stream);    /* :263 projector (row-major NxN) */
+int vh_lie_Jinv(int group, int dtype, int64_t n, const void* X, const void* a, void* b, int place, void* stream); /* :290 Jinv */
+
+/* ---- droid_net_ext correlation (csrc/droid_net_ext/droid.cpp:24-82) ----- */
+
+/* corr_index_forward (droid.cpp:25, correlation_kernels.cu:22-66, 115-136).
+ * volume [B,H1,W1,H2,W2] (dtype F16/F32/F64), coords [B,2,H1,W1] f32,
+ * corr out [B,2r+1,2r+1,H1,W1] (dtype of volume), fully written. */
+int vh_corr_index_forward(int dtype, int64_t B, int H1, int W1, int H2, int W2, const void* volume,
+                          const float* coords, int radius, void* corr, void* stream);
+/* corr_index_backward (droid.cpp:32, correlation_kernels.cu:68-113, 138-158).
+ * volume_grad [B,H1,W1,H2,W2] out, fully written (zeros outside the taps). */
+int vh_corr_index_backward(int dtype, int64_t B, int H1, int W1, int H2, int W2, const float* coords,
+                           const void* corr_grad, int radius, void* volume_grad, void* stream);
+
+/* All-pairs correlation volume and its avg-pool pyramid: the CorrBlock
+ * constructor (vipe/slam/networks/droid_net.py:56-69, 94-102).
+ * fmap1, fmap2 [E,C,H,W] in `in_dtype` (F32, F16, BF16); levels[l] is the
+ * level-l output [E,H,W,H>>l,W>>l] in `out_dtype` (F32 or F16); values are
+ * (f1/4).(f2/4) summed over C (FP32 accumulation), level l>0 = 2x2 mean of
+ * level l-1 (floor of odd sizes). */
+int vh_corr_build(int in_dtype, int out_dtype, int64_t E, int C, int H, int W, const void* fmap1,
+                  const void* fmap2, int num_levels, void* const* levels, void* stream);
+
+/* Mixed-precision build: fmap1/fmap2 f32 [E,C,H,W] are quantised per
+ * (edge, channel) to FP8 e4m3 (scale = amax/448) and contracted on MFMA with
+ * FP32 accumulation; output as vh_corr_build.  Needs a workspace of
+ * vh_corr_build_fp8_workspace(E,C,H,W) bytes. */
+size_t vh_corr_build_fp8_workspace(int64_t E, int C, int H, int W);
+int vh_corr_build_fp8(int out_dtype, int64_t E, int C, int H, int W, const float* fmap1, const float* fmap2,
+                      int num_levels, void* const* levels, void* workspace, void* stream);
+
+/* altcorr_forward (droid.cpp:41, altcorr_kernel.cu:26-138, 368-392).
+ * fmap1 [B,H1,W1,C], fmap2 [B,H2,W2,C] (F16/F32/F64), coords [B,N,H1,W1,2] f32,
+ * corr out [B,N,(2r+1)^2,H1,W1] in the fmap dtype, fully written. */
+int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
+                       const void* fmap2, const float* coords, int radius, void* corr, void* stream);
+/* altcorr_backward (droid.cpp:62, altcorr_kernel.cu:242-366, 426-455); F32 only
+ * like the reference.  fmap1_grad [B,H1,W1,C], fmap2_grad [B,H2,W2,C] out, fully
+ * written; coords_grad is not produced (the reference returns zeros). */
+int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
+                        const void* fmap2, const float* coords, const void* corr_grad, int radius, void* fmap1_grad,
+                        void* fmap2_grad, void* stream);
+/* altcorr_index_forward (droid.cpp:50, altcorr_kernel.cu:140-240, 394-424).
+ * fmap1 [B,Nf,H,W,C]; levels[l] [B,Nf,Hl[l],Wl[l],C]; coords [B,M,H,W,2] f32;
+ * ii, jj [M] int64 (device); corr out [B,M,L*(2r+1)^2,H,W] f32, fully written. */
+int vh_altcorr_index_forward(int dtype, int64_t B, int Nf, int M, int H, int W, int C, const void* fmap1,
+                             int num_levels, const void* const* levels, const int* Hl, const int* Wl,
+                             const float* coords, const int64_t* ii, const int64_t* jj, int radius, void* corr,
+                             void* stream);
+
+/* ---- slam_ext dense bundle adjustment (csrc/slam_ext/slam.cpp:41-55,
+ *      geom_kernels.cu:1626-1847) ---------------------------------------- */
+
+typedef struct vh_ba_plan vh_ba_plan;
+
+/* Builds the graph bookkeeping of ba_cuda_impl once per edge list:
+ * kx = unique(cat(arange(t0,t1), ii)) and its inverse, the per-depth-frame
+ * edge lists, the Schur row pairs and the block-assembly lists
+ * (geom_kernels.cu:1648-1655, 1437-1468).  ii/jj are HOST arrays.  Allocates
+ * and uploads device index tables (synchronously).  Returns NULL on error. */
+vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W, int t0,
+                              int t1);
+void vh_ba_plan_destroy(vh_ba_plan* plan);
+/* Number of depth frames |kx| and the frame ids (host copy, kx_out[|kx|]). */
+int vh_ba_plan_num_depth_frames(const vh_ba_plan* plan);
+int vh_ba_plan_depth_frames(const vh_ba_plan* plan, int64_t* kx_out);
+/* Device workspace bytes vh_ba_run needs for this plan. */
+size_t vh_ba_workspace_size(const vh_ba_plan* plan);
+
+#define VH_BA_MOTION_ONLY 1
+#define VH_BA_OPTIMIZE_INTRINSICS 2
+#define VH_BA_COMPUTE_ENERGY 4
+
+/* ba / ba_extended (slam.cpp:41-51).  All pointers device, f32 unless noted:
+ * poses [N,7] (in place), disps [N,H,W] (in place), intrinsics [4]
+ * (fx,fy,cx,cy; in place when optimising intrinsics), disps_sens [N,H,W],
+ * targets/weights [E,2,H,W], eta [|kx|,H,W], depth_active [N].
+ * Outputs: dx [P,6] (P = t1-t0), dz [|kx|,H*W] (untouched when motion only),
+ * energy [iterations] when VH_BA_COMPUTE_ENERGY.  The reduced camera system
+ * is factorised in FP64 on the device; a non-positive pivot gives dx = 0
+ * (geom_kernels.cu:1283-1285).  `workspace` holds vh_ba_workspace_size bytes. */
+int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
+              const float* targets, const float* weights, const float* eta, const float* depth_active,
+              int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
+              float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* workspace,
+              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VIPE_HIP_H */

@@ -1,0 +1,25 @@
+"""FP64 CPU oracle for the DROID-style SLAM inner loop.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the ``vipe_amd`` product path imports,
+calls or links anything in this package.  It is imported only by ``tests/``,
+by ``__graft_entry__.smoke()`` (as the checker) and by the ``cpu_baseline``
+leg of ``bench.py``.
+
+Every function restates, from scratch, the arithmetic of one reference
+function and cites the reference file:line it follows (paths relative to the
+nv-tlabs/vipe source tree).  How each function is pinned independently of
+its own formula (closed forms, group axioms, finite differences, brute-force
+dense solves, ``torch.nn.functional`` equivalents) is listed in the module
+docstrings and exercised by ``tests/test_oracle_*.py``.
+
+Modules
+-------
+lie   : SO3 / RxSO3 / SE3 / Sim3 group maths and the 13 lietorch ops with
+        their backward formulas (csrc/lietorch_ext/*.h, lietorch_gpu.cu).
+corr  : all-pairs correlation volume, avg-pool pyramid, corr_index
+        forward/backward, altcorr forward/backward/indexed
+        (vipe/slam/networks/droid_net.py, csrc/droid_net_ext/*.cu).
+ba    : slam_ext.ba / ba_extended dense bundle adjustment with the reference
+        bookkeeping (csrc/slam_ext/geom_kernels.cu:1626-1847) and a
+        brute-force dense Gauss-Newton solve used to pin it.
+"""

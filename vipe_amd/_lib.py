@@ -1,0 +1,141 @@
+"""ctypes binding of libvipe_hip.so (the C ABI declared in include/vipe_hip.h).
+
+The library is built in-tree by ``python -m vipe_amd.build``.  There is no
+fallback: if the shared object is missing or fails to load, every op raises.
+``torch`` is imported first so the HIP runtime that torch ships is the one the
+library binds to (both carry the soname libamdhip64.so.7).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import torch  # noqa: F401  (loads the HIP runtime before the library)
+
+LIB_PATH = Path(os.environ.get("VIPE_HIP_LIB", Path(__file__).resolve().parent / "_lib" / "libvipe_hip.so"))
+
+VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_UNSUPPORTED, VH_ERR_UNDEFINED = 0, 1, 2, 3, 4
+VH_F16, VH_F32, VH_F64, VH_BF16, VH_FP8E4M3 = 1, 2, 3, 4, 5
+VH_DEVICE, VH_HOST = 0, 1
+
+vp = C.c_void_p
+i32 = C.c_int
+i64 = C.c_int64
+f32 = C.c_float
+sz = C.c_size_t
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "vh_last_error": (C.c_char_p, []),
+    "vh_abi_version": (i32, []),
+    "vh_fill_i32": (i32, [vp, i64, i32, vp]),
+    # lietorch
+    "vh_lie_expm": (i32, [i32, i32, i64, vp, vp, i32, vp]),
+    "vh_lie_expm_backward": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_logm": (i32, [i32, i32, i64, vp, vp, i32, vp]),
+    "vh_lie_logm_backward": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_inv": (i32, [i32, i32, i64, vp, vp, i32, vp]),
+    "vh_lie_inv_backward": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_mul": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_mul_backward": (i32, [i32, i32, i64, vp, vp, vp, vp, vp, i32, vp]),
+    "vh_lie_adj": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_adj_backward": (i32, [i32, i32, i64, vp, vp, vp, vp, vp, i32, vp]),
+    "vh_lie_adjT": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_adjT_backward": (i32, [i32, i32, i64, vp, vp, vp, vp, vp, i32, vp]),
+    "vh_lie_act": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_act_backward": (i32, [i32, i32, i64, vp, vp, vp, vp, vp, i32, vp]),
+    "vh_lie_act4": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    "vh_lie_act4_backward": (i32, [i32, i32, i64, vp, vp, vp, vp, vp, i32, vp]),
+    "vh_lie_as_matrix": (i32, [i32, i32, i64, vp, vp, i32, vp]),
+    "vh_lie_projector": (i32, [i32, i32, i64, vp, vp, i32, vp]),
+    "vh_lie_Jinv": (i32, [i32, i32, i64, vp, vp, vp, i32, vp]),
+    # correlation
+    "vh_corr_index_forward": (i32, [i32, i64, i32, i32, i32, i32, vp, vp, i32, vp, vp]),
+    "vh_corr_index_backward": (i32, [i32, i64, i32, i32, i32, i32, vp, vp, i32, vp, vp]),
+    "vh_corr_build": (i32, [i32, i32, i64, i32, i32, i32, vp, vp, i32, vp, vp]),
+    "vh_corr_build_fp8_workspace": (sz, [i64, i32, i32, i32]),
+    "vh_corr_build_fp8": (i32, [i32, i64, i32, i32, i32, vp, vp, i32, vp, vp, vp]),
+    "vh_altcorr_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp]),
+    "vh_altcorr_backward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp]),
+    "vh_altcorr_index_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, i32,
+                                       vp, vp]),
+    # bundle adjustment
+    "vh_ba_plan_create": (vp, [vp, vp, i32, i32, i32, i32, i32, i32]),
+    "vh_ba_plan_destroy": (None, [vp]),
+    "vh_ba_plan_num_depth_frames": (i32, [vp]),
+    "vh_ba_plan_depth_frames": (i32, [vp, vp]),
+    "vh_ba_workspace_size": (sz, [vp]),
+    "vh_ba_run": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, vp, vp, vp, vp,
+                        vp]),
+}
+
+_lib = None
+_load_error = None
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def load():
+    """Load (once) and return the ctypes library; raises if it is unavailable."""
+    global _lib, _load_error
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"libvipe_hip.so not found at {LIB_PATH}; build it with `python -m vipe_amd.build` "
+            "(there is no non-native fallback)."
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    missing = []
+    for name, (res, args) in _SIGS.items():
+        try:
+            fn = getattr(lib, name)
+        except AttributeError:
+            missing.append(name)
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    if missing:
+        raise RuntimeError(f"libvipe_hip.so is missing symbols: {missing}")
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().vh_last_error().decode(errors="replace")
+
+
+class VHUndefined(Exception):
+    """The reference op would return an undefined tensor (None in Python)."""
+
+
+def check(rc: int, what: str):
+    if rc == VH_OK:
+        return
+    if rc == VH_ERR_UNDEFINED:
+        raise VHUndefined(last_error())
+    raise RuntimeError(f"{what}: {last_error()}")
+
+
+def ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def stream_of(t):
+    if t.device.type == "cpu":
+        return None
+    return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def place_of(t):
+    return VH_HOST if t.device.type == "cpu" else VH_DEVICE
+
+
+DTYPE_CODE = {torch.float16: VH_F16, torch.float32: VH_F32, torch.float64: VH_F64, torch.bfloat16: VH_BF16}
+
+
+def dtype_code(t):
+    return DTYPE_CODE.get(t.dtype, -1)

@@ -1,0 +1,6 @@
+"""Drop-in for ``vipe.ext`` on MI355X: ``slam_ext``, ``droid_net_ext`` and
+``lietorch_ext`` with the reference's function names and signatures
+(vipe/ext/__init__.py:39-45), backed by libvipe_hip.so."""
+from . import droid_net_ext, lietorch_ext, slam_ext  # noqa: F401
+
+__all__ = ["slam_ext", "droid_net_ext", "lietorch_ext"]

@@ -1,0 +1,177 @@
+"""Seeded synthetic frame graphs for the SLAM inner loop.
+
+One generator, CPU ``torch.Generator`` so inputs are bit-identical on every
+machine.  Value distributions (per the workload definitions this package is
+measured on):
+
+* poses: SE3 ``exp([t ~ N(0, 0.1), w ~ N(0, 0.05 rad)])`` per keyframe
+  (Sim3 adds ``log s ~ N(0, 0.02)``), stored ``[tx,ty,tz,qx,qy,qz,qw(,s)]``;
+* inverse depth ``U[0.2, 1.0]``; pinhole intrinsics at 1/8 resolution;
+* features ``N(0,1)/sqrt(C)`` float32, ``[N, C, H, W]``;
+* BA targets = reprojection of every edge + ``N(0, 0.5 px)``, weights ``U[0,1]``.
+
+Edge lists: window edges ``(i, i+d)`` ordered by ``|d|`` then ``i`` then sign,
+truncated to ``n_edges - n_random``, followed by ``n_random`` long-range edges
+(source round-robin over frames, target uniform with ``|i-j| > window``,
+no duplicates).  For 8 frames / 24 edges / window 2 / 4 random this keeps all
+14 ``|d|=1`` edges and 6 of the ``|d|=2`` edges; for 64 frames / 384 edges /
+window 3 / 2 random per frame it keeps every ``|d|<=2`` edge, 6 ``|d|=3`` edges
+and 128 random edges, so every frame has out-degree 4..7.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import torch
+
+
+@dataclass
+class FrameGraph:
+    poses: torch.Tensor        # [N, 7] (SE3) or [N, 8] (Sim3) float32
+    disps: torch.Tensor        # [N, H, W] float32
+    disps_sens: torch.Tensor   # [N, H, W] float32 (zeros: no depth sensor)
+    intrinsics: torch.Tensor   # [4] float32 (fx, fy, cx, cy)
+    ii: torch.Tensor           # [E] int64
+    jj: torch.Tensor           # [E] int64
+    targets: torch.Tensor      # [E, 2, H, W] float32
+    weights: torch.Tensor      # [E, 2, H, W] float32
+    eta: torch.Tensor          # [|kx|, H, W] float32 for the window [t0, t1)
+    fmaps: torch.Tensor        # [N, C, H, W] float32
+    coords: torch.Tensor       # [E, H, W, 2] float32 reprojected pixel coordinates
+    t0: int
+    t1: int
+
+
+# name -> (frames, edges, H, W, fx, fy, cx, cy, window, n_random, group)
+CONFIGS = {
+    # 8 keyframes, 24 edges, 48x64 (1/8 of 384x512)
+    "kf8_e24_48x64": dict(n_frames=8, n_edges=24, H=48, W=64, intr=(60.0, 60.0, 32.0, 24.0),
+                          window=2, n_random=4, sim3=False),
+    # 64 keyframes, 384 edges, 64x80 (1/8 of 512x640)
+    "kf64_e384_64x80": dict(n_frames=64, n_edges=384, H=64, W=80, intr=(80.0, 80.0, 40.0, 32.0),
+                            window=3, n_random=128, sim3=False),
+    # Sim3 poses at the 64-keyframe size
+    "kf64_e384_64x80_sim3": dict(n_frames=64, n_edges=384, H=64, W=80, intr=(80.0, 80.0, 40.0, 32.0),
+                                 window=3, n_random=128, sim3=True),
+    # 32 keyframes, 192 edges, 48x64
+    "kf32_e192_48x64": dict(n_frames=32, n_edges=192, H=48, W=64, intr=(60.0, 60.0, 32.0, 24.0),
+                            window=3, n_random=64, sim3=False),
+}
+
+
+def make_edges(n_frames: int, n_edges: int, window: int, n_random: int, gen: torch.Generator):
+    cands = []
+    for d in range(1, window + 1):
+        for i in range(n_frames):
+            for sgn in (1, -1):
+                j = i + sgn * d
+                if 0 <= j < n_frames:
+                    cands.append((i, j))
+    n_win = n_edges - n_random
+    if n_win > len(cands):
+        raise ValueError("not enough window edges for the requested edge count")
+    edges = cands[:n_win]
+    seen = set(edges)
+    k = 0
+    guard = 0
+    while len(edges) < n_edges:
+        i = k % n_frames
+        j = int(torch.randint(0, n_frames, (1,), generator=gen))
+        guard += 1
+        if guard > 100000:
+            raise ValueError("cannot place random edges")
+        if abs(i - j) <= window or (i, j) in seen:
+            continue
+        edges.append((i, j))
+        seen.add((i, j))
+        k += 1
+    ii = torch.tensor([e[0] for e in edges], dtype=torch.long)
+    jj = torch.tensor([e[1] for e in edges], dtype=torch.long)
+    return ii, jj
+
+
+def _quat_exp(phi: torch.Tensor) -> torch.Tensor:
+    th = phi.norm(dim=-1, keepdim=True)
+    ths = torch.where(th > 1e-12, th, torch.ones_like(th))
+    imag = torch.where(th > 1e-12, torch.sin(0.5 * ths) / ths, 0.5 * torch.ones_like(th))
+    return torch.cat([imag * phi, torch.cos(0.5 * th)], -1)
+
+
+def _qrot(q, p):
+    v, w = q[..., :3], q[..., 3:]
+    uv = 2.0 * torch.cross(v, p, dim=-1)
+    return p + w * uv + torch.cross(v, uv, dim=-1)
+
+
+def _qmul(a, b):
+    av, aw = a[..., :3], a[..., 3:]
+    bv, bw = b[..., :3], b[..., 3:]
+    return torch.cat([aw * bv + bw * av + torch.cross(av, bv, dim=-1), aw * bw - (av * bv).sum(-1, keepdim=True)], -1)
+
+
+def reproject(poses, disps, intrinsics, ii, jj):
+    """Pixel coordinates of frame ii's pixels in frame jj (float64 maths):
+    X = [(u-cx)/fx, (v-cy)/fy, 1, d]; Xj = Tj Ti^-1 X; returns [E, H, W, 2] and z."""
+    P = poses.double()
+    N, H, W = disps.shape
+    fx, fy, cx, cy = [float(v) for v in intrinsics]
+    qi, qj = P[ii, 3:7], P[jj, 3:7]
+    ti, tj = P[ii, :3], P[jj, :3]
+    si = P[ii, 7] if P.shape[1] == 8 else torch.ones(ii.shape[0], dtype=torch.float64)
+    sj = P[jj, 7] if P.shape[1] == 8 else torch.ones(ii.shape[0], dtype=torch.float64)
+    qinv = torch.cat([-qi[:, :3], qi[:, 3:]], -1)
+    qij = _qmul(qj, qinv)
+    sij = sj / si
+    tij = tj - sij[:, None] * _qrot(qij, ti)
+    vv, uu = torch.meshgrid(torch.arange(H, dtype=torch.float64), torch.arange(W, dtype=torch.float64), indexing="ij")
+    X = torch.stack([(uu - cx) / fx, (vv - cy) / fy, torch.ones_like(uu)], -1)  # [H,W,3]
+    d = disps.double()[ii]  # [E,H,W]
+    Xr = sij[:, None, None, None] * _qrot(qij[:, None, None, :].expand(-1, H, W, -1), X.expand(ii.shape[0], H, W, 3))
+    Xj = Xr + tij[:, None, None, :] * d[..., None]
+    z = Xj[..., 2]
+    zs = torch.where(z.abs() > 1e-9, z, torch.full_like(z, 1e-9))
+    u = fx * Xj[..., 0] / zs + cx
+    v = fy * Xj[..., 1] / zs + cy
+    return torch.stack([u, v], -1), z
+
+
+def make_graph(name: str = "kf8_e24_48x64", seed: int = 0, C: int = 128, with_fmaps: bool = True,
+               t0: int = 1, t1: int | None = None) -> FrameGraph:
+    cfg = CONFIGS[name]
+    gen = torch.Generator().manual_seed(seed)
+    N, E, H, W = cfg["n_frames"], cfg["n_edges"], cfg["H"], cfg["W"]
+    t1 = N if t1 is None else t1
+    ii, jj = make_edges(N, E, cfg["window"], cfg["n_random"], gen)
+    tau = torch.randn(N, 3, generator=gen, dtype=torch.float64) * 0.1
+    phi = torch.randn(N, 3, generator=gen, dtype=torch.float64) * 0.05
+    q = _quat_exp(phi)
+    # SE3 exp translation: V(phi) tau
+    th = phi.norm(dim=-1, keepdim=True)
+    a = (1 - torch.cos(th)) / (th * th)
+    b = (th - torch.sin(th)) / (th ** 3)
+    c1 = torch.cross(phi, tau, dim=-1)
+    t = tau + a * c1 + b * torch.cross(phi, c1, dim=-1)
+    if cfg["sim3"]:
+        s = torch.exp(torch.randn(N, 1, generator=gen, dtype=torch.float64) * 0.02)
+        poses = torch.cat([t, q, s], -1)
+    else:
+        poses = torch.cat([t, q], -1)
+    disps = 0.2 + 0.8 * torch.rand(N, H, W, generator=gen, dtype=torch.float64)
+    intr = torch.tensor(cfg["intr"], dtype=torch.float64)
+    coords, _ = reproject(poses, disps, intr, ii, jj)
+    noise = torch.randn(E, H, W, 2, generator=gen, dtype=torch.float64) * 0.5
+    targets = (coords + noise).permute(0, 3, 1, 2).contiguous()
+    weights = torch.rand(E, 2, H, W, generator=gen, dtype=torch.float64)
+    kx = torch.unique(torch.cat([torch.arange(t0, t1), ii]))
+    eta = 0.2 * (0.05 + 0.1 * torch.rand(kx.shape[0], H, W, generator=gen, dtype=torch.float64)) + 1e-7
+    if with_fmaps:
+        fmaps = torch.randn(N, C, H, W, generator=gen, dtype=torch.float32) / math.sqrt(C)
+    else:
+        fmaps = torch.zeros(0)
+    return FrameGraph(
+        poses=poses.float().contiguous(), disps=disps.float().contiguous(),
+        disps_sens=torch.zeros(N, H, W), intrinsics=intr.float(), ii=ii, jj=jj,
+        targets=targets.float().contiguous(), weights=weights.float().contiguous(),
+        eta=eta.float().contiguous(), fmaps=fmaps, coords=coords.float().contiguous(), t0=t0, t1=t1,
+    )
