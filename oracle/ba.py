@@ -342,13 +342,15 @@ def ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, t0, 
 # ----------------------------------------------------------------------------
 
 def dense_gn_step(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active,
-                  t0, t1, lm, ep, alpha):
+                  t0, t1, lm, ep, alpha, return_system=False):
     """One GN step over [poses in window (6P), all depth pixels of kx (K*HW)] with the explicit
     Jacobian J of every residual row, the same damping of the pose block and the same depth
     prior / eta terms, solved with torch.linalg.solve.  Returns (dx [P,6], dz [K,HW]).
 
     Independent of the Schur assembly: it forms the full normal equations from an explicit
-    stacked Jacobian.  Without intrinsics; depth_active must be all ones."""
+    stacked Jacobian.  Without intrinsics; depth_active must be all ones.  The pose damping
+    is applied to the diagonal of H (the Schur path damps H - S), so the two agree for
+    lm = 0."""
     E = ii.shape[0]
     N, H, W = disps.shape
     HW = H * W
@@ -399,4 +401,6 @@ def dense_gn_step(poses, disps, intrinsics, disps_sens, targets, weights, eta, i
     Hm[idx, idx] += (eta.to(F64).reshape(K, HW) + m * alpha).reshape(-1)
     g[idx] -= (m * alpha * (dk - dsk)).reshape(-1)
     x = torch.linalg.solve(Hm, g)
+    if return_system:
+        return x[:6 * P].reshape(P, 6), x[6 * P:].reshape(K, HW), Hm, g
     return x[:6 * P].reshape(P, 6), x[6 * P:].reshape(K, HW)

@@ -41,7 +41,8 @@ def corr_pyramid(corr: torch.Tensor, num_levels: int = 4):
     out = []
     for i in range(num_levels):
         out.append(c.reshape(B * N, H1, W1, c.shape[-2], c.shape[-1]))
-        c = F.avg_pool2d(c, 2, stride=2)
+        if i + 1 < num_levels:
+            c = F.avg_pool2d(c, 2, stride=2)
     return out
 
 
@@ -189,7 +190,8 @@ def altcorr_pyramid(fmaps: torch.Tensor, num_levels: int = 4):
     out = []
     for i in range(num_levels):
         out.append(f.permute(0, 2, 3, 1).reshape(B, N, f.shape[-2], f.shape[-1], C))
-        f = F.avg_pool2d(f, 2, stride=2)
+        if i + 1 < num_levels:
+            f = F.avg_pool2d(f, 2, stride=2)
     return out
 
 

@@ -1,0 +1,84 @@
+"""gfx950 correlation volume, pyramid and lookups vs the FP64 oracle."""
+import math
+
+import pytest
+import torch
+
+from oracle import corr as OC
+
+pytestmark = pytest.mark.gpu
+
+
+def _coords(B, H, W, H2, W2, g, spread=3.0):
+    yy, xx = torch.meshgrid(torch.arange(H, dtype=torch.float32), torch.arange(W, dtype=torch.float32), indexing="ij")
+    base = torch.stack([xx * (W2 / W), yy * (H2 / H)], 0)[None].repeat(B, 1, 1, 1)
+    return (base + spread * torch.randn(B, 2, H, W, generator=g)).contiguous()
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.float64, 1e-12), (torch.float16, 2e-3)])
+def test_corr_index_forward_backward(dtype, tol):
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(0)
+    B, H1, W1, H2, W2, r = 3, 6, 7, 9, 11, 3
+    vol = torch.randn(B, H1, W1, H2, W2, generator=g).to(dtype)
+    coords = _coords(B, H1, W1, H2, W2, g)
+    (out,) = D.corr_index_forward(vol.cuda(), coords.cuda(), r)
+    ref = OC.corr_index_forward(vol.double(), coords.double(), r)
+    assert out.shape == ref.shape and out.dtype == dtype
+    assert (out.double().cpu() - ref).abs().max().item() <= tol * 8
+    cg = torch.randn(ref.shape, generator=g).to(dtype)
+    (vg,) = D.corr_index_backward(vol.cuda(), coords.cuda(), cg.cuda(), r)
+    vref = OC.corr_index_backward(vol.shape, coords.double(), cg.double(), r)
+    assert (vg.double().cpu() - vref).abs().max().item() <= tol * 8
+
+
+@pytest.mark.parametrize("in_dt,out_dt,tol", [(torch.float32, torch.float32, 2e-5), (torch.float16, torch.float16, 3e-3),
+                                              (torch.bfloat16, torch.float32, 2e-2)])
+def test_corr_build_and_pyramid(in_dt, out_dt, tol):
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(1)
+    E, C, H, W = 3, 128, 12, 16
+    f1 = (torch.randn(E, C, H, W, generator=g) / math.sqrt(C))
+    f2 = (torch.randn(E, C, H, W, generator=g) / math.sqrt(C))
+    levels = D.corr_build(f1.to(in_dt).cuda(), f2.to(in_dt).cuda(), 4, out_dtype=out_dt)
+    ref = OC.corr_pyramid(OC.corr_volume(f1.to(in_dt).double()[None], f2.to(in_dt).double()[None]), 4)
+    for lv, rv in zip(levels, ref):
+        assert lv.shape == rv.shape
+        err = (lv.double().cpu() - rv).abs().max().item()
+        assert err <= tol, err
+
+
+def test_altcorr_forward_backward_and_indexed():
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(2)
+    B, N, H, W, C, r = 2, 3, 8, 10, 64, 3
+    f1 = torch.randn(B, H, W, C, generator=g) / math.sqrt(C)
+    f2 = torch.randn(B, H, W, C, generator=g) / math.sqrt(C)
+    coords = _coords(B * N, H, W, H, W, g).view(B, N, 2, H, W).permute(0, 1, 3, 4, 2).contiguous()
+    (out,) = D.altcorr_forward(f1.cuda(), f2.cuda(), coords.cuda(), r)
+    ref = OC.altcorr_forward(f1.double(), f2.double(), coords.double(), r)
+    assert (out.double().cpu() - ref).abs().max().item() < 1e-5
+    cg = torch.randn(ref.shape, generator=g)
+    g1, g2, gc = D.altcorr_backward(f1.cuda(), f2.cuda(), coords.cuda(), cg.cuda(), r)
+    r1, r2, _ = OC.altcorr_backward(f1.double(), f2.double(), coords.double(), cg.double(), r)
+    assert (g1.double().cpu() - r1).abs().max().item() < 1e-4
+    assert (g2.double().cpu() - r2).abs().max().item() < 1e-4
+    assert torch.count_nonzero(gc) == 0
+
+    # indexed multi-level lookup (test_altcorr_indexed.py construction, FP16 features)
+    Nf, M = 5, 4
+    fmaps = torch.randn(1, Nf, C, H * 2, W * 2, generator=g).half()
+    pyr = OC.altcorr_pyramid(fmaps.double(), 4)
+    pyr16 = [p.half().contiguous() for p in pyr]
+    ii = torch.tensor([0, 1, 2, 3])
+    jj = torch.tensor([1, 2, 3, 4])
+    yy, xx = torch.meshgrid(torch.arange(2 * H), torch.arange(2 * W), indexing="ij")
+    cc = torch.stack([xx, yy], -1).float()[None, None].repeat(1, M, 1, 1, 1)
+    cc = cc + 0.4 * torch.randn(cc.shape, generator=g)
+    (idx,) = D.altcorr_index_forward(pyr16[0].cuda(), [p.cuda() for p in pyr16], cc.cuda(), ii.cuda(), jj.cuda(), r)
+    iref = OC.altcorr_index_forward(pyr16[0].double(), [p.double() for p in pyr16], cc.double(), ii, jj, r)
+    assert idx.dtype == torch.float32
+    assert (idx.double().cpu() - iref).abs().max().item() < 2e-4

@@ -1,0 +1,150 @@
+"""Correlation volume classes of the DROID update loop on gfx950.
+
+Mirrors vipe/slam/networks/droid_net.py:32-196 (same class names, constructor
+arguments, call signatures and output layouts):
+
+* ``CorrSampler`` / ``CorrLayer``: autograd Functions over the
+  ``droid_net_ext`` lookup ops (droid_net.py:32-45, 105-118).
+* ``CorrBlock``: all-pairs volume + 4-level pyramid, looked up at radius r
+  (droid_net.py:48-102).  The volume and its pooled levels are built by one
+  MFMA kernel plus pooling kernels (``droid_net_ext.corr_build``) instead of
+  ``torch.matmul`` + ``F.avg_pool2d``; the values are the same,
+  ``(f1/4).(f2/4)`` summed over channels.
+* ``AltCorrBlock``: the memory-light on-the-fly variant (droid_net.py:121-196).
+"""
+from __future__ import annotations
+
+import torch
+import torch.nn.functional as F
+
+from .ext import droid_net_ext
+
+
+class CorrSampler(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, volume, coords, radius):
+        ctx.save_for_backward(volume, coords)
+        ctx.radius = radius
+        (corr,) = droid_net_ext.corr_index_forward(volume, coords, radius)
+        return corr
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        volume, coords = ctx.saved_tensors
+        grad_output = grad_output.contiguous()
+        (grad_volume,) = droid_net_ext.corr_index_backward(volume, coords, grad_output, ctx.radius)
+        return grad_volume, None, None
+
+
+class CorrBlock:
+    """All-pairs correlation pyramid of two feature maps ``[B, N, C, H, W]``.
+
+    ``volume_dtype`` selects the storage of the pyramid (float32, or float16
+    as the reference produces under autocast); by default it follows the
+    feature dtype.
+    """
+
+    def __init__(self, fmap1, fmap2, num_levels=4, radius=3, volume_dtype=None):
+        self.num_levels = num_levels
+        self.radius = radius
+        B, N, C, H, W = fmap1.shape
+        f1 = fmap1.reshape(B * N, C, H, W).contiguous()
+        f2 = fmap2.reshape(B * N, C, H, W).contiguous()
+        levels = droid_net_ext.corr_build(f1, f2, num_levels, out_dtype=volume_dtype)
+        self.corr_pyramid = levels  # level i: [B*N, H, W, H/2^i, W/2^i]
+
+    def __call__(self, coords):
+        out_pyramid = []
+        batch, num, ht, wd, _ = coords.shape
+        coords = coords.permute(0, 1, 4, 2, 3)
+        coords = coords.contiguous().view(batch * num, 2, ht, wd)
+        for i in range(self.num_levels):
+            corr = CorrSampler.apply(self.corr_pyramid[i], coords / 2**i, self.radius)
+            out_pyramid.append(corr.view(batch, num, -1, ht, wd))
+        return torch.cat(out_pyramid, dim=2)
+
+    def cat(self, other):
+        for i in range(self.num_levels):
+            self.corr_pyramid[i] = torch.cat([self.corr_pyramid[i], other.corr_pyramid[i]], 0)
+        return self
+
+    def __getitem__(self, index):
+        for i in range(self.num_levels):
+            self.corr_pyramid[i] = self.corr_pyramid[i][index]
+        return self
+
+    @staticmethod
+    def corr(fmap1, fmap2):
+        """all-pairs correlation [B, N, H, W, H, W] (level 0 only)."""
+        batch, num, dim, ht, wd = fmap1.shape
+        (lvl0,) = droid_net_ext.corr_build(fmap1.reshape(batch * num, dim, ht, wd).contiguous(),
+                                           fmap2.reshape(batch * num, dim, ht, wd).contiguous(), 1)
+        return lvl0.view(batch, num, ht, wd, ht, wd)
+
+
+class CorrLayer(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, fmap1, fmap2, coords, r):
+        ctx.r = r
+        ctx.save_for_backward(fmap1, fmap2, coords)
+        (corr,) = droid_net_ext.altcorr_forward(fmap1, fmap2, coords, ctx.r)
+        return corr
+
+    @staticmethod
+    def backward(ctx, grad_corr):
+        fmap1, fmap2, coords = ctx.saved_tensors
+        grad_corr = grad_corr.contiguous()
+        fmap1_grad, fmap2_grad, coords_grad = droid_net_ext.altcorr_backward(fmap1, fmap2, coords, grad_corr, ctx.r)
+        return fmap1_grad, fmap2_grad, coords_grad, None
+
+
+class AltCorrBlock:
+    """On-the-fly correlation over an NHWC feature pyramid (fmaps/4, avg-pooled)."""
+
+    def __init__(self, fmaps, num_levels=4, radius=3):
+        self.num_levels = num_levels
+        self.radius = radius
+        B, N, C, H, W = fmaps.shape
+        fmaps = fmaps.view(B * N, C, H, W) / 4.0
+        self.pyramid = []
+        for i in range(self.num_levels):
+            sz = (B, N, H // 2**i, W // 2**i, C)
+            fmap_lvl = fmaps.permute(0, 2, 3, 1).contiguous()
+            self.pyramid.append(fmap_lvl.view(*sz))
+            fmaps = F.avg_pool2d(fmaps, 2, stride=2)
+
+    def corr_fn_reference(self, coords, ii, jj):
+        B, N, H, W, S, _ = coords.shape
+        coords = coords.permute(0, 1, 4, 2, 3, 5)
+        corr_list = []
+        for i in range(self.num_levels):
+            fmap1_i = self.pyramid[0][:, ii]
+            fmap2_i = self.pyramid[i][:, jj]
+            coords_i = (coords / 2**i).reshape(B * N, S, H, W, 2).contiguous()
+            fmap1_i = fmap1_i.reshape((B * N,) + fmap1_i.shape[2:])
+            fmap2_i = fmap2_i.reshape((B * N,) + fmap2_i.shape[2:])
+            corr = CorrLayer.apply(fmap1_i.float().contiguous(), fmap2_i.float().contiguous(), coords_i, self.radius)
+            corr = corr.view(B, N, S, -1, H, W).permute(0, 1, 3, 4, 5, 2)
+            corr_list.append(corr)
+        return torch.cat(corr_list, dim=2)
+
+    def corr_fn_indexed(self, coords, ii, jj):
+        if coords.shape[-2] != 1:
+            return self.corr_fn_reference(coords, ii, jj)
+        coords = coords.squeeze(dim=-2).contiguous()
+        (corr,) = droid_net_ext.altcorr_index_forward(self.pyramid[0], self.pyramid, coords, ii.contiguous(),
+                                                      jj.contiguous(), self.radius)
+        return corr.unsqueeze(dim=-1)
+
+    def corr_fn(self, coords, ii, jj):
+        return self.corr_fn_indexed(coords, ii, jj)
+
+    def __call__(self, coords, ii, jj):
+        squeeze_output = False
+        if len(coords.shape) == 5:
+            coords = coords.unsqueeze(dim=-2)
+            squeeze_output = True
+        corr = self.corr_fn(coords, ii, jj)
+        if squeeze_output:
+            corr = corr.squeeze(dim=-1)
+        return corr.contiguous()

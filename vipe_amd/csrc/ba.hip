@@ -1,0 +1,1255 @@
+// Dense DROID bundle adjustment on gfx950 (slam_ext.ba / ba_extended).
+//
+// Semantics: csrc/slam_ext/geom_kernels.cu:178-524 (per-edge terms),
+// :1197-1424 (block system, damping, zeros on failed factorisation),
+// :1426-1624 (Schur over rows that share a depth frame), :1179-1195
+// (back-substitution that skips rows whose local pose index is <= 0),
+// :974-1041 (retractions), :1626-1820 (iteration order).
+//
+// Per Gauss-Newton iteration, all on the device, no host round trip:
+//   1. ba_frame_kernel   one workgroup per (depth frame k, pixel tile).  For
+//      every edge leaving k it evaluates residuals and Jacobians per pixel and
+//      accumulates (a) the edge's j-side pose terms Hjj, vj (+ intrinsics
+//      terms, energy) - the i-side follows from Ji = -Adj(Tij)^T Jj; (b) the
+//      depth terms C, w; (c) the E rows (self row + one per edge) in LDS, and
+//      from them the frame's Schur Gram sum_p Q E_a E_b^T, plus
+//      [w, Ef] as a pseudo row so the Schur right-hand side and the
+//      intrinsics coupling come out of the same Gram.  Writes per-tile
+//      partials and Q, w per pixel.  E is never written to memory.
+//   2. ba_finalize_kernel  sums partials over tiles in FP64, forms
+//      Hii = A^T Hjj A, Hij = -A^T Hjj, vi = -A^T vj (A = Adj(Tij)).
+//   3. ba_assemble_kernel  dense lower-triangular reduced system H - S in
+//      FP64 from host-built contribution lists (fixed order: deterministic),
+//      damping diag += ep + lm * diag.
+//   4. chol_*               blocked right-looking FP64 Cholesky (64 x 64 blocks);
+//      a non-positive pivot raises a flag and the solve returns zeros.
+//   5. ba_backsub_kernel   recomputes E rows from weights and geometry,
+//      dz = Q (w - E^T dx) - Q Ef df, disps += dz.
+//   6. ba_retract_kernel   T <- Exp(dx) T in FP64, intrinsics += df * scale.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "vh_common.hpp"
+
+#define MIN_DEPTH 0.25f
+
+namespace vh {
+namespace ba {
+
+constexpr int kThreads = 256;
+constexpr int kMaxDeg = 20;   // edges leaving one depth frame
+constexpr int kMaxRext = 22;  // self row + edges + pseudo row
+constexpr int kNV = 40;       // per-edge partial stride (floats)
+constexpr int kEDS = 136;     // per-edge finalised record (doubles)
+constexpr int kCB = 64;       // Cholesky block size
+constexpr int kLdsBudget = 56 * 1024;  // dynamic E-row tile; static LDS stays under 8 KB
+
+// per-edge partial layout (floats)
+//   [0,21)  Hjj upper triangle, index(n,m) n>=m: n(n+1)/2 + m
+//   [21,27) vj   27 energy  28 Hff  29 vf  [32,38) Fj
+// finalised edge record (doubles)
+//   [0,36) Hii  [36,72) Hij  [72,108) Hjj  [108,114) vi  [114,120) vj
+//   [120,126) Fi [126,132) Fj  132 Hff  133 vf  134 energy
+enum { E_HII = 0, E_HIJ = 36, E_HJJ = 72, E_VI = 108, E_VJ = 114, E_FI = 120, E_FJ = 126, E_HFF = 132, E_VF = 133,
+       E_EN = 134 };
+
+// contribution item kinds for the assembly lists
+enum { K_HII = 0, K_HJJ = 1, K_HIJ = 2, K_HIJT = 3, K_G = 4, K_GT = 5, K_VI = 6, K_VJ = 7, K_ROW = 8 };
+
+struct DevPlan {
+    int E, N, H, W, HW, t0, t1, P, K;
+    int TP, SUB, T, Rext_max, GT;
+    int n, ld, nb, nlb;  // system size, leading dim, blocks, lower pose blocks
+    int intr;            // system includes the intrinsics scalar (set per run)
+    const int* kx;       // [K] frame id of slot k
+    const int* fr_ptr;   // [K+1] CSR into fr_edges
+    const int* fr_edges; // [E]
+    const int* fr_self;  // [K]
+    const int* gram_off; // [K+1] float offsets of each frame's Gram blocks
+    const int* ii;       // [E]
+    const int* jj;       // [E]
+    const int* blk_ab;   // [nlb*2]
+    const int* blk_ptr;  // [nlb+1]
+    const int* blk_items;// packed (kind << 27 | index)
+    const int* rhs_ptr;  // [P+1]
+    const int* rhs_items;
+    const int* pseudo_off; // [K] gram float offset of the (pseudo, pseudo) block
+};
+
+struct Work {
+    float* edge_part;  // [T][E][kNV]
+    float* gram_part;  // [T][GT]
+    float* fen_part;   // [T][K]
+    float* Qb;         // [K][HW]
+    float* Wb;         // [K][HW]
+    double* ed;        // [E][kEDS]
+    double* gd;        // [GT]
+    double* fen;       // [K]
+    double* A;         // [ld][ld]
+    double* b;         // [ld]
+    double* x;         // [ld]
+    double* Ld;        // [nb][64][64] factored diagonal blocks
+    int* flag;         // [4]
+};
+
+struct RunArgs {
+    float* poses;
+    float* disps;
+    float* intrinsics;
+    const float* disps_sens;
+    const float* targets;
+    const float* weights;
+    const float* eta;
+    const float* depth_active;
+    float lm, ep, alpha, ilm, iep, iscale;
+    float* dx;
+    float* dz;
+    float* energy;
+    int itr;
+    int compute_energy;
+};
+
+__host__ __device__ inline int pk(int kind, int idx) { return (kind << 27) | idx; }
+
+// Relative pose Tij = Tj * Ti^-1 as rotation matrix + translation (relSE3,
+// geom_kernels.cu:101-114); stereo edges use (t = (-0.1,0,0), R = I) (:231-241).
+__device__ __forceinline__ void edge_transform(const float* __restrict__ poses, int i, int j, float* R, float* t) {
+    if (i == j) {
+        R[0] = 1; R[1] = 0; R[2] = 0; R[3] = 0; R[4] = 1; R[5] = 0; R[6] = 0; R[7] = 0; R[8] = 1;
+        t[0] = -0.1f; t[1] = 0.f; t[2] = 0.f;
+        return;
+    }
+    const float* Pi = poses + 7 * i;
+    const float* Pj = poses + 7 * j;
+    const float qi[4] = {Pi[3], Pi[4], Pi[5], Pi[6]};
+    const float qj[4] = {Pj[3], Pj[4], Pj[5], Pj[6]};
+    float q[4];
+    q[0] = -qj[3] * qi[0] + qj[0] * qi[3] - qj[1] * qi[2] + qj[2] * qi[1];
+    q[1] = -qj[3] * qi[1] + qj[1] * qi[3] - qj[2] * qi[0] + qj[0] * qi[2];
+    q[2] = -qj[3] * qi[2] + qj[2] * qi[3] - qj[0] * qi[1] + qj[1] * qi[0];
+    q[3] = qj[3] * qi[3] + qj[0] * qi[0] + qj[1] * qi[1] + qj[2] * qi[2];
+    const float x = q[0], y = q[1], z = q[2], w = q[3];
+    R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - w * z);     R[2] = 2 * (x * z + w * y);
+    R[3] = 2 * (x * y + w * z);     R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - w * x);
+    R[6] = 2 * (x * z - w * y);     R[7] = 2 * (y * z + w * x);     R[8] = 1 - 2 * (x * x + y * y);
+    const float ti[3] = {Pi[0], Pi[1], Pi[2]};
+    t[0] = Pj[0] - (R[0] * ti[0] + R[1] * ti[1] + R[2] * ti[2]);
+    t[1] = Pj[1] - (R[3] * ti[0] + R[4] * ti[1] + R[5] * ti[2]);
+    t[2] = Pj[2] - (R[6] * ti[0] + R[7] * ti[1] + R[8] * ti[2]);
+}
+
+// Per-pixel terms of one edge (projective_transform_kernel :298-435).
+struct PixTerms {
+    float ru, rv, wu, wv, Jzu, Jzv, Jfu, Jfv;
+    float Ju[6], Jv[6];
+};
+
+template <bool INTR, bool RES>
+__device__ __forceinline__ void pixel_terms(const float* R, const float* t, float X0, float Y0, float d,
+                                            float fx, float fy, float cx, float cy, float wgt_u, float wgt_v,
+                                            float tgt_u, float tgt_v, float iscale, PixTerms& o) {
+    const float Xj0 = R[0] * X0 + R[1] * Y0 + R[2] + t[0] * d;
+    const float Xj1 = R[3] * X0 + R[4] * Y0 + R[5] + t[1] * d;
+    const float Xj2 = R[6] * X0 + R[7] * Y0 + R[8] + t[2] * d;
+    const bool ok = !(Xj2 < MIN_DEPTH);
+    const float dinv = ok ? 1.0f / Xj2 : 0.0f;
+    const float d2 = dinv * dinv;
+    const float x = Xj0, y = Xj1, h = d;
+    o.wu = ok ? 0.001f * wgt_u : 0.0f;
+    o.wv = ok ? 0.001f * wgt_v : 0.0f;
+    if (RES) {
+        o.ru = tgt_u - (fx * dinv * x + cx);
+        o.rv = tgt_v - (fy * dinv * y + cy);
+    }
+    o.Ju[0] = fx * (h * dinv);
+    o.Ju[1] = 0.f;
+    o.Ju[2] = fx * (-x * h * d2);
+    o.Ju[3] = fx * (-x * y * d2);
+    o.Ju[4] = fx * (1 + x * x * d2);
+    o.Ju[5] = fx * (-y * dinv);
+    o.Jv[0] = 0.f;
+    o.Jv[1] = fy * (h * dinv);
+    o.Jv[2] = fy * (-y * h * d2);
+    o.Jv[3] = fy * (-1 - y * y * d2);
+    o.Jv[4] = fy * (x * y * d2);
+    o.Jv[5] = fy * (x * dinv);
+    o.Jzu = fx * (t[0] * dinv - t[2] * (x * d2));
+    o.Jzv = fy * (t[1] * dinv - t[2] * (y * d2));
+    if (INTR) {
+        // dXi/df = (-X0/fx, -Y0/fy, 0, 0) mapped by R only (:325-331)
+        const float a0 = -X0 / fx, a1 = -Y0 / fy;
+        const float e0 = R[0] * a0 + R[1] * a1;
+        const float e1 = R[3] * a0 + R[4] * a1;
+        const float e2 = R[6] * a0 + R[7] * a1;
+        o.Jfu = iscale * (fx * dinv * e0 - fx * x * d2 * e2 + x * dinv);
+        o.Jfv = iscale * (fy * dinv * e1 - fy * y * d2 * e2 + y * dinv);
+    } else {
+        o.Jfu = o.Jfv = 0.f;
+    }
+}
+
+// -Adj(T)^T e for Adj = [[R, [t]x R], [0, R]]:  top = -R^T e_top,
+// bottom = -(R^T (e_top x t) + R^T e_bot)   (adjSE3, geom_kernels.cu:81-99).
+__device__ __forceinline__ void neg_adjT(const float* R, const float* t, const float* e, float* o) {
+    const float c0 = e[1] * t[2] - e[2] * t[1];
+    const float c1 = e[2] * t[0] - e[0] * t[2];
+    const float c2 = e[0] * t[1] - e[1] * t[0];
+    const float b0 = e[3] + c0, b1 = e[4] + c1, b2 = e[5] + c2;
+    o[0] = -(R[0] * e[0] + R[3] * e[1] + R[6] * e[2]);
+    o[1] = -(R[1] * e[0] + R[4] * e[1] + R[7] * e[2]);
+    o[2] = -(R[2] * e[0] + R[5] * e[1] + R[8] * e[2]);
+    o[3] = -(R[0] * b0 + R[3] * b1 + R[6] * b2);
+    o[4] = -(R[1] * b0 + R[4] * b1 + R[7] * b2);
+    o[5] = -(R[2] * b0 + R[5] * b1 + R[8] * b2);
+}
+
+// Sum of 32 values over the 64 lanes of a wave with 32 exchanges: after the
+// halving steps lane l holds the partial of value (l & 31); a last step adds
+// the other half-wave.  Returns the total of value (lane & 31).
+__device__ __forceinline__ float wave_sum32(float (&v)[32]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int s = 16; s >= 1; s >>= 1) {
+        const bool hi = (lane & s) != 0;
+#pragma unroll
+        for (int i = 0; i < s; i++) {
+            const float send = hi ? v[i] : v[i + s];
+            const float keep = hi ? v[i + s] : v[i];
+            v[i] = keep + __shfl_xor(send, s, 64);
+        }
+    }
+    return v[0] + __shfl_xor(v[0], 32, 64);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+    return v;
+}
+
+// Decode Gram block b of a frame with Rext rows into (ra, rb), ra <= rb.
+__device__ __forceinline__ void gram_rows(int b, int Rext, int& ra, int& rb) {
+    int a = 0;
+    while (b >= Rext - a) {
+        b -= Rext - a;
+        a++;
+    }
+    ra = a;
+    rb = a + b;
+}
+
+// ---------------------------------------------------------------------------
+// 1. frame kernel
+// ---------------------------------------------------------------------------
+template <bool INTR, bool MOTION>
+__global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a, Work w) {
+    extern __shared__ float dyn[];  // rows [Rext_max*6][TP+1]; reused for the Gram reduction
+    __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo
+    __shared__ float sQ[256];
+    __shared__ float sRed[4][kNV];
+    __shared__ float sAcc[kMaxDeg][kNV];
+    __shared__ float sFen[4];
+
+    const int k = blockIdx.y;
+    const int tile = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int TP = p.TP;
+    const int ldr = TP + 1;
+    const int kf = p.kx[k];
+    const int e0 = p.fr_ptr[k];
+    const int deg = p.fr_ptr[k + 1] - e0;
+    const int has_self = p.fr_self[k];
+    const int R = has_self + deg;
+    const int Rext = R + 1;
+    const int NB = Rext * (Rext + 1) / 2;
+    const int NS = kThreads / NB;
+    const int HW = p.HW;
+    const float act = a.depth_active[kf];
+    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+
+    if (tid < deg) {
+        const int e = p.fr_edges[e0 + tid];
+        float Rm[9], t[3];
+        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+        for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+        sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
+        sEP[tid][13] = __int_as_float(e);
+    }
+    for (int i = tid; i < kMaxDeg * kNV; i += kThreads) (&sAcc[0][0])[i] = 0.f;
+    if (tid < 4) sFen[tid] = 0.f;
+
+    // Gram thread mapping (fixed for all sub-tiles)
+    int gra = 0, grb = 0;
+    const int gslice = (NB > 0) ? tid / NB : 0;
+    const bool gactive = !MOTION && (gslice < NS);
+    if (gactive) gram_rows(tid % NB, Rext, gra, grb);
+    float G[36];
+#pragma unroll
+    for (int v = 0; v < 36; v++) G[v] = 0.f;
+    float fen_acc = 0.f;
+    __syncthreads();
+
+    for (int sub = 0; sub < p.SUB; sub++) {
+        const int pix = (tile * p.SUB + sub) * TP + tid;
+        const bool valid = (tid < TP) && (pix < HW);
+        const int py = valid ? pix / p.W : 0;
+        const int px = valid ? pix - py * p.W : 0;
+        const float X0 = ((float)px - cx) / fx;
+        const float Y0 = ((float)py - cy) / fy;
+        const float d = valid ? a.disps[(int64_t)kf * HW + pix] : 0.f;
+        float Csum = 0.f, wsum = 0.f, Ef = 0.f;
+        float es[6] = {0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < deg; s++) {
+            const float* Rm = sEP[s];
+            const float* t = sEP[s] + 9;
+            const bool stereo = sEP[s][12] != 0.f;
+            const int e = __float_as_int(sEP[s][13]);
+            float v[32];
+#pragma unroll
+            for (int i = 0; i < 32; i++) v[i] = 0.f;
+            float fj[6] = {0, 0, 0, 0, 0, 0};
+            float er[6] = {0, 0, 0, 0, 0, 0};
+            if (valid) {
+                const int64_t eb = (int64_t)e * 2 * HW + pix;
+                PixTerms T;
+                pixel_terms<INTR, true>(Rm, t, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
+                                        a.targets[eb], a.targets[eb + HW], a.iscale, T);
+                const float wdu = T.wu * act, wdv = T.wv * act;
+                const float pu = stereo ? 0.f : T.wu;
+                const float pv = stereo ? 0.f : T.wv;
+                // pose j-side terms
+                int l = 0;
+#pragma unroll
+                for (int n = 0; n < 6; n++)
+#pragma unroll
+                    for (int m = 0; m <= n; m++) {
+                        v[l] = pu * T.Ju[n] * T.Ju[m] + pv * T.Jv[n] * T.Jv[m];
+                        l++;
+                    }
+#pragma unroll
+                for (int n = 0; n < 6; n++) v[21 + n] = pu * T.ru * T.Ju[n] + pv * T.rv * T.Jv[n];
+                v[27] = T.wu * T.ru * T.ru + T.wv * T.rv * T.rv;
+                if (INTR) {
+                    v[28] = pu * T.Jfu * T.Jfu + pv * T.Jfv * T.Jfv;
+                    v[29] = pu * T.ru * T.Jfu + pv * T.rv * T.Jfv;
+#pragma unroll
+                    for (int n = 0; n < 6; n++) fj[n] = pu * T.Ju[n] * T.Jfu + pv * T.Jv[n] * T.Jfv;
+                }
+                if (!MOTION) {
+                    Csum += wdu * T.Jzu * T.Jzu + wdv * T.Jzv * T.Jzv;
+                    wsum += wdu * T.ru * T.Jzu + wdv * T.rv * T.Jzv;
+                    if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
+                    const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
+#pragma unroll
+                    for (int n = 0; n < 6; n++) er[n] = cu * T.Ju[n] + cv * T.Jv[n];
+                    if (has_self) {
+                        float o[6];
+                        neg_adjT(Rm, t, er, o);
+#pragma unroll
+                        for (int n = 0; n < 6; n++) es[n] += o[n];
+                    }
+                }
+            }
+            if (!MOTION && tid < TP) {
+#pragma unroll
+                for (int n = 0; n < 6; n++) dyn[((has_self + s) * 6 + n) * ldr + tid] = er[n];
+            }
+            // block reduction of this edge's partials
+            const float tot = wave_sum32(v);
+            if (lane < 32) sRed[wave][lane] = tot;
+            if (INTR) {
+#pragma unroll
+                for (int n = 0; n < 6; n++) {
+                    const float f = wave_sum(fj[n]);
+                    if (lane == 0) sRed[wave][32 + n] = f;
+                }
+            }
+            __syncthreads();
+            if (tid < (INTR ? 38 : 28)) sAcc[s][tid] += sRed[0][tid] + sRed[1][tid] + sRed[2][tid] + sRed[3][tid];
+            __syncthreads();
+        }
+        if (!MOTION) {
+            float Q = 0.f, wt = 0.f, Eft = 0.f;
+            if (valid) {
+                const float ds = a.disps_sens[(int64_t)kf * HW + pix];
+                const float m = (ds > 0.f ? 1.f : 0.f) * act;
+                const float C = act * (Csum + a.eta[(int64_t)k * HW + pix] + m * a.alpha) + (1.f - act);
+                wt = act * (wsum - m * a.alpha * (d - ds));
+                Q = 1.f / C;
+                Eft = Ef * act;
+                fen_acc += m * a.alpha * (d - ds) * (d - ds);
+                w.Qb[(int64_t)k * HW + pix] = Q;
+                w.Wb[(int64_t)k * HW + pix] = wt;
+            }
+            if (tid < TP) {
+                if (has_self) {
+#pragma unroll
+                    for (int n = 0; n < 6; n++) dyn[n * ldr + tid] = es[n];
+                }
+                dyn[(R * 6 + 0) * ldr + tid] = wt;
+                dyn[(R * 6 + 1) * ldr + tid] = Eft;
+#pragma unroll
+                for (int n = 2; n < 6; n++) dyn[(R * 6 + n) * ldr + tid] = 0.f;
+                sQ[tid] = Q;
+            }
+            __syncthreads();
+            if (gactive) {
+                const float* ra = dyn + gra * 6 * ldr;
+                const float* rb = dyn + grb * 6 * ldr;
+                for (int q = gslice; q < TP; q += NS) {
+                    const float qq = sQ[q];
+                    float av[6], bv[6];
+#pragma unroll
+                    for (int n = 0; n < 6; n++) {
+                        av[n] = ra[n * ldr + q] * qq;
+                        bv[n] = rb[n * ldr + q];
+                    }
+#pragma unroll
+                    for (int n = 0; n < 6; n++)
+#pragma unroll
+                        for (int m = 0; m < 6; m++) G[n * 6 + m] += av[n] * bv[m];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // per-edge partials
+    for (int i = tid; i < deg * kNV; i += kThreads) {
+        const int s = i / kNV, v = i % kNV;
+        const int e = p.fr_edges[e0 + s];
+        w.edge_part[((int64_t)tile * p.E + e) * kNV + v] = sAcc[s][v];
+    }
+    if (!MOTION) {
+        // depth-prior energy
+        const float f = wave_sum(fen_acc);
+        if (lane == 0) sFen[wave] = f;
+        // cross-slice Gram reduction through LDS
+        float* red = dyn;  // [NS][NB][36]
+        if (gactive) {
+            const int b = tid % NB;
+#pragma unroll
+            for (int v = 0; v < 36; v++) red[(gslice * NB + b) * 36 + v] = G[v];
+        }
+        __syncthreads();
+        if (tid == 0) w.fen_part[(int64_t)tile * p.K + k] = sFen[0] + sFen[1] + sFen[2] + sFen[3];
+        const int nvals = NB * 36;
+        float* out = w.gram_part + (int64_t)tile * p.GT + p.gram_off[k];
+        for (int i = tid; i < nvals; i += kThreads) {
+            float s = 0.f;
+            for (int sl = 0; sl < NS; sl++) s += red[sl * nvals + i];
+            out[i] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. finalize: tile sums in FP64, i-side edge blocks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArgs a, Work w, int motion) {
+    __shared__ double s[64];
+    __shared__ double Ad[36];
+    const int tid = threadIdx.x;
+    const int bid = blockIdx.x;
+    if (bid < p.E) {
+        const int e = bid;
+        if (tid < kNV) {
+            double acc = 0.0;
+            for (int t = 0; t < p.T; t++) acc += (double)w.edge_part[((int64_t)t * p.E + e) * kNV + tid];
+            s[tid] = acc;
+        }
+        if (tid == 0) {
+            float Rm[9], t[3];
+            edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+            // Adj = [[R, [t]x R], [0, R]]
+            double Rd[9], td[3];
+            for (int i = 0; i < 9; i++) Rd[i] = Rm[i];
+            for (int i = 0; i < 3; i++) td[i] = t[i];
+            const double tx[9] = {0, -td[2], td[1], td[2], 0, -td[0], -td[1], td[0], 0};
+            for (int i = 0; i < 36; i++) Ad[i] = 0.0;
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    Ad[r * 6 + c] = Rd[r * 3 + c];
+                    Ad[(r + 3) * 6 + c + 3] = Rd[r * 3 + c];
+                    double v = 0;
+                    for (int q = 0; q < 3; q++) v += tx[r * 3 + q] * Rd[q * 3 + c];
+                    Ad[r * 6 + c + 3] = v;
+                }
+        }
+        __syncthreads();
+        double* rec = w.ed + (int64_t)e * kEDS;
+        if (tid < 36) {
+            const int i = tid / 6, j = tid % 6;
+            const int n = i > j ? i : j, m = i > j ? j : i;
+            const double hjj = s[n * (n + 1) / 2 + m];
+            rec[E_HJJ + tid] = hjj;
+            // (A^T Hjj)[i][j] and (A^T Hjj A)[i][j]
+            double aTh = 0.0, aTha = 0.0;
+            for (int q = 0; q < 6; q++) {
+                const int nq = q > j ? q : j, mq = q > j ? j : q;
+                aTh += Ad[q * 6 + i] * s[nq * (nq + 1) / 2 + mq];
+            }
+            for (int q = 0; q < 6; q++) {
+                double row = 0.0;  // (A^T Hjj)[i][q]
+                for (int r = 0; r < 6; r++) {
+                    const int nr = r > q ? r : q, mr = r > q ? q : r;
+                    row += Ad[r * 6 + i] * s[nr * (nr + 1) / 2 + mr];
+                }
+                aTha += row * Ad[q * 6 + j];
+            }
+            rec[E_HIJ + tid] = -aTh;
+            rec[E_HII + tid] = aTha;
+        } else if (tid >= 64 && tid < 70) {
+            const int i = tid - 64;
+            double vi = 0.0, fi = 0.0;
+            for (int q = 0; q < 6; q++) {
+                vi += Ad[q * 6 + i] * s[21 + q];
+                fi += Ad[q * 6 + i] * s[32 + q];
+            }
+            rec[E_VI + i] = -vi;
+            rec[E_VJ + i] = s[21 + i];
+            rec[E_FI + i] = -fi;
+            rec[E_FJ + i] = s[32 + i];
+        } else if (tid == 70) {
+            rec[E_HFF] = s[28];
+            rec[E_VF] = s[29];
+            rec[E_EN] = s[27];
+        }
+        return;
+    }
+    if (motion) return;
+    const int k = bid - p.E;
+    const int off = p.gram_off[k];
+    const int nv = p.gram_off[k + 1] - off;
+    for (int i = tid; i < nv; i += kThreads) {
+        double acc = 0.0;
+        for (int t = 0; t < p.T; t++) acc += (double)w.gram_part[(int64_t)t * p.GT + off + i];
+        w.gd[off + i] = acc;
+    }
+    if (tid == 0) {
+        double acc = 0.0;
+        for (int t = 0; t < p.T; t++) acc += (double)w.fen_part[(int64_t)t * p.K + k];
+        w.fen[k] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. assemble the dense lower triangle of H - S (+ intrinsics) and b - v
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, Work w, int motion) {
+    const int tid = threadIdx.x;
+    const int bid = blockIdx.x;
+    const int ld = p.ld;
+    const int n6 = 6 * p.P;
+    if (bid < p.nlb) {
+        if (tid >= 36) return;
+        const int ab = p.blk_ab[2 * bid], bb = p.blk_ab[2 * bid + 1];
+        const int i = tid / 6, j = tid % 6;
+        double s = 0.0;
+        for (int it = p.blk_ptr[bid]; it < p.blk_ptr[bid + 1]; it++) {
+            const int item = p.blk_items[it];
+            const int kind = item >> 27, idx = item & ((1 << 27) - 1);
+            switch (kind) {
+                case K_HII: s += w.ed[(int64_t)idx * kEDS + E_HII + i * 6 + j]; break;
+                case K_HJJ: s += w.ed[(int64_t)idx * kEDS + E_HJJ + i * 6 + j]; break;
+                case K_HIJ: s += w.ed[(int64_t)idx * kEDS + E_HIJ + i * 6 + j]; break;
+                case K_HIJT: s += w.ed[(int64_t)idx * kEDS + E_HIJ + j * 6 + i]; break;
+                case K_G: s -= w.gd[idx + i * 6 + j]; break;
+                case K_GT: s -= w.gd[idx + j * 6 + i]; break;
+                default: break;
+            }
+        }
+        if (ab == bb && i == j) s += (double)a.ep + (double)a.lm * s;
+        w.A[(int64_t)(6 * ab + i) * ld + 6 * bb + j] = s;
+        return;
+    }
+    const int r = bid - p.nlb;
+    if (r < p.P) {
+        if (tid >= 6) return;
+        double sb = 0.0, sf = 0.0;
+        for (int it = p.rhs_ptr[r]; it < p.rhs_ptr[r + 1]; it++) {
+            const int item = p.rhs_items[it];
+            const int kind = item >> 27, idx = item & ((1 << 27) - 1);
+            if (kind == K_VI) {
+                sb += w.ed[(int64_t)idx * kEDS + E_VI + tid];
+                sf += w.ed[(int64_t)idx * kEDS + E_FI + tid];
+            } else if (kind == K_VJ) {
+                sb += w.ed[(int64_t)idx * kEDS + E_VJ + tid];
+                sf += w.ed[(int64_t)idx * kEDS + E_FJ + tid];
+            } else if (kind == K_ROW && !motion) {
+                sb -= w.gd[idx + tid * 6 + 0];
+                sf -= w.gd[idx + tid * 6 + 1];
+            }
+        }
+        w.b[6 * r + tid] = sb;
+        if (p.intr) w.A[(int64_t)n6 * ld + 6 * r + tid] = sf;
+        return;
+    }
+    if (r == p.P) {
+        // intrinsics scalar, energy, padding, failure flag
+        if (tid == 0) {
+            double hff = 0.0, vf = 0.0, en = 0.0;
+            for (int e = 0; e < p.E; e++) {
+                hff += w.ed[(int64_t)e * kEDS + E_HFF];
+                vf += w.ed[(int64_t)e * kEDS + E_VF];
+                en += w.ed[(int64_t)e * kEDS + E_EN];
+            }
+            if (!motion) {
+                for (int k = 0; k < p.K; k++) {
+                    const int po = p.pseudo_off[k];
+                    hff -= w.gd[po + 1 * 6 + 1];
+                    vf -= w.gd[po + 0 * 6 + 1];
+                    en += w.fen[k];
+                }
+            }
+            if (p.intr) {
+                w.A[(int64_t)n6 * ld + n6] = hff + (double)a.iep + (double)a.ilm * hff;
+                w.b[n6] = vf;
+            }
+            if (a.compute_energy) a.energy[a.itr] = (float)en;
+            w.flag[0] = 0;
+        }
+        return;
+    }
+    // padding rows n..ld-1: identity
+    const int prow = p.n + (r - p.P - 1);
+    if (prow < ld) {
+        for (int c = tid; c < ld; c += 64) w.A[(int64_t)prow * ld + c] = (c == prow) ? 1.0 : 0.0;
+        if (tid == 0) w.b[prow] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. blocked FP64 Cholesky (lower), 64 x 64 blocks
+// ---------------------------------------------------------------------------
+// Factor the diagonal block kb in LDS (one wave, lane = row, left-looking);
+// workgroup r > 0 then solves its block row X L^T = A_{kb+r,kb}.
+// The factored diagonal block goes to Ldiag[kb] (not back into A): sibling
+// workgroups of the same launch are still reading the original A_kk.
+__global__ __launch_bounds__(64) void chol_panel_kernel(double* __restrict__ A, double* __restrict__ Ldiag, int ld,
+                                                        int kb, int* flag) {
+    __shared__ double L[kCB][kCB + 1];
+    __shared__ double X[kCB][kCB + 1];
+    const int lane = threadIdx.x;
+    const int c0 = kb * kCB;
+    for (int c = 0; c < kCB; c++) L[lane][c] = A[(int64_t)(c0 + lane) * ld + c0 + c];
+    __syncthreads();
+    for (int j = 0; j < kCB; j++) {
+        double s = 0.0;
+        if (lane >= j) {
+            s = L[lane][j];
+            for (int m = 0; m < j; m++) s -= L[lane][m] * L[j][m];
+        }
+        const double dj = __shfl(s, j, 64);
+        if (lane == 0 && dj <= 0.0) atomicExch(flag, 1);
+        const double ljj = sqrt(dj);
+        __syncthreads();
+        if (lane > j) L[lane][j] = s / ljj;
+        if (lane == j) L[j][j] = ljj;
+        __syncthreads();
+    }
+    const int rb = blockIdx.x;
+    if (rb == 0) {
+        double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
+        for (int c = 0; c < kCB; c++) Lk[lane * kCB + c] = (c <= lane) ? L[lane][c] : 0.0;
+        return;
+    }
+    const int r0 = c0 + rb * kCB;
+    for (int c = 0; c < kCB; c++) X[lane][c] = A[(int64_t)(r0 + lane) * ld + c0 + c];
+    __syncthreads();
+    for (int j = 0; j < kCB; j++) {
+        double s = X[lane][j];
+        for (int m = 0; m < j; m++) s -= X[lane][m] * L[j][m];
+        X[lane][j] = s / L[j][j];
+    }
+    for (int c = 0; c < kCB; c++) A[(int64_t)(r0 + lane) * ld + c0 + c] = X[lane][c];
+}
+
+// Trailing update A_ij -= L_ik L_jk^T for kb < j <= i.
+__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int ld, int kb, int nb) {
+    __shared__ double Li[kCB][kCB + 1];
+    __shared__ double Lj[kCB][kCB + 1];
+    // decode lower-triangular trailing block index -> (bi, bj)
+    int idx = blockIdx.x;
+    int bi = kb + 1, bj = kb + 1;
+    {
+        int row = 0;
+        while (idx > row) {
+            idx -= row + 1;
+            row++;
+        }
+        bi = kb + 1 + row;
+        bj = kb + 1 + idx;
+    }
+    const int tid = threadIdx.x;
+    const int c0 = kb * kCB;
+    for (int q = tid; q < kCB * kCB; q += 256) {
+        const int r = q / kCB, c = q % kCB;
+        Li[r][c] = A[(int64_t)(bi * kCB + r) * ld + c0 + c];
+        Lj[r][c] = A[(int64_t)(bj * kCB + r) * ld + c0 + c];
+    }
+    __syncthreads();
+    const int tr = (tid / 16) * 4, tc = (tid % 16) * 4;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int m = 0; m < kCB; m++) {
+        double xi[4], xj[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            xi[i] = Li[tr + i][m];
+            xj[i] = Lj[tc + i][m];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] += xi[i] * xj[j];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double* d = A + (int64_t)(bi * kCB + tr + i) * ld + bj * kCB + tc + j;
+            *d -= acc[i][j];
+        }
+}
+
+// Forward and backward triangular solves with L (one wave), zeros when the
+// factorisation failed.  Diagonal blocks come from Ldiag, the rest from A.
+// Writes x (FP64) and dx [P,6] (f32).
+constexpr int kMaxLd = 2048;
+__global__ __launch_bounds__(64) void chol_solve_kernel(const double* __restrict__ A,
+                                                        const double* __restrict__ Ldiag, int ld, int nb,
+                                                        const double* __restrict__ b, double* __restrict__ x,
+                                                        const int* flag, int P, float* dx) {
+    __shared__ double y[kMaxLd];
+    __shared__ double xs[kMaxLd];
+    const int lane = threadIdx.x;
+    const bool failed = flag[0] != 0;
+    if (failed) {
+        for (int i = lane; i < ld; i += 64) xs[i] = 0.0;
+    } else {
+        for (int kb = 0; kb < nb; kb++) {
+            const double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
+            const int r = kb * kCB + lane;
+            double s = b[r];
+            for (int c = 0; c < kb * kCB; c++) s -= A[(int64_t)r * ld + c] * y[c];
+            for (int j = 0; j < kCB; j++) {
+                const double yj = __shfl(s, j, 64) / Lk[j * kCB + j];
+                if (lane > j) s -= Lk[lane * kCB + j] * yj;
+                if (lane == j) y[r] = yj;
+            }
+            __syncthreads();
+        }
+        for (int kb = nb - 1; kb >= 0; kb--) {
+            const double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
+            const int c = kb * kCB + lane;
+            double s = y[c];
+            for (int rr = (kb + 1) * kCB; rr < nb * kCB; rr++) s -= A[(int64_t)rr * ld + c] * xs[rr];
+            for (int j = kCB - 1; j >= 0; j--) {
+                const double xj = __shfl(s, j, 64) / Lk[j * kCB + j];
+                if (lane < j) s -= Lk[j * kCB + lane] * xj;
+                if (lane == j) xs[kb * kCB + j] = xj;
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    for (int i = lane; i < ld; i += 64) x[i] = xs[i];
+    for (int i = lane; i < 6 * P; i += 64) dx[i] = (float)xs[i];
+}
+
+// ---------------------------------------------------------------------------
+// 5. back-substitution: dz = Q (w - sum_r E_r . dx_r) - Q Ef df
+// ---------------------------------------------------------------------------
+template <bool INTR>
+__global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs a, Work w) {
+    __shared__ float sEP[kMaxDeg][16];
+    __shared__ float sU[kMaxDeg][6];
+    const int k = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int kf = p.kx[k];
+    const int e0 = p.fr_ptr[k];
+    const int deg = p.fr_ptr[k + 1] - e0;
+    const int has_self = p.fr_self[k];
+    const float act = a.depth_active[kf];
+    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+    const int HW = p.HW;
+    if (tid < deg) {
+        const int e = p.fr_edges[e0 + tid];
+        float Rm[9], t[3];
+        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+        for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+        // u = dx[jj] (if 1 <= jj-t0 < P) - Adj(Tij) dx[k] (self row, if 1 <= k-t0 < P)
+        double u[6] = {0, 0, 0, 0, 0, 0};
+        const int lj = p.jj[e] - p.t0;
+        if (lj >= 1 && lj < p.P)
+            for (int n = 0; n < 6; n++) u[n] += w.x[6 * lj + n];
+        const int lk = kf - p.t0;
+        if (has_self && lk >= 1 && lk < p.P) {
+            const double* v = w.x + 6 * lk;
+            double rv[3], rw[3];
+            for (int r = 0; r < 3; r++) {
+                rv[r] = Rm[3 * r] * v[0] + Rm[3 * r + 1] * v[1] + Rm[3 * r + 2] * v[2];
+                rw[r] = Rm[3 * r] * v[3] + Rm[3 * r + 1] * v[4] + Rm[3 * r + 2] * v[5];
+            }
+            // top = R v_tau + t x (R v_phi), bottom = R v_phi
+            u[0] -= rv[0] + (t[1] * rw[2] - t[2] * rw[1]);
+            u[1] -= rv[1] + (t[2] * rw[0] - t[0] * rw[2]);
+            u[2] -= rv[2] + (t[0] * rw[1] - t[1] * rw[0]);
+            u[3] -= rw[0];
+            u[4] -= rw[1];
+            u[5] -= rw[2];
+        }
+        for (int n = 0; n < 6; n++) sU[tid][n] = (float)u[n];
+    }
+    __syncthreads();
+    const double df = INTR ? w.x[6 * p.P] : 0.0;
+    const int pix = blockIdx.x * kThreads + tid;
+    if (pix >= HW) return;
+    const int py = pix / p.W, px = pix - py * p.W;
+    const float X0 = ((float)px - cx) / fx;
+    const float Y0 = ((float)py - cy) / fy;
+    const int64_t dpos = (int64_t)kf * HW + pix;
+    const float d = a.disps[dpos];
+    float dot = 0.f, Ef = 0.f;
+    for (int s = 0; s < deg; s++) {
+        const int e = p.fr_edges[e0 + s];
+        const int64_t eb = (int64_t)e * 2 * HW + pix;
+        PixTerms T;
+        pixel_terms<INTR, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
+                                 0.f, 0.f, a.iscale, T);
+        const float wdu = T.wu * act, wdv = T.wv * act;
+        const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
+        float s6 = 0.f;
+#pragma unroll
+        for (int n = 0; n < 6; n++) s6 += (cu * T.Ju[n] + cv * T.Jv[n]) * sU[s][n];
+        dot += s6;
+        if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
+    }
+    const float Q = w.Qb[(int64_t)k * HW + pix];
+    const float wt = w.Wb[(int64_t)k * HW + pix];
+    float dz = Q * (wt - dot);
+    if (INTR) dz = dz - Q * (Ef * act) * (float)df;
+    a.dz[(int64_t)k * HW + pix] = dz;
+    a.disps[dpos] = d + dz;
+}
+
+// ---------------------------------------------------------------------------
+// 6. retraction T <- Exp(dx) T in FP64 (expSE3/retrSE3 semantics,
+//    geom_kernels.cu:116-177, :974-1023), intrinsics update (:1038-1041)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void ba_retract_kernel(float* poses, float* intrinsics, const double* x, int t0,
+                                                        int P, int intr, float iscale) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (intr && i == 0) {
+        const double df = x[6 * P] * (double)iscale;
+        intrinsics[0] = (float)((double)intrinsics[0] + df);
+        intrinsics[1] = (float)((double)intrinsics[1] + df);
+    }
+    if (i >= P) return;
+    const double* xi = x + 6 * i;
+    const double tau[3] = {xi[0], xi[1], xi[2]};
+    const double phi[3] = {xi[3], xi[4], xi[5]};
+    const double th2 = phi[0] * phi[0] + phi[1] * phi[1] + phi[2] * phi[2];
+    const double th = sqrt(th2);
+    double imag, real;
+    if (th2 < 1e-8) {
+        imag = 0.5 - th2 / 48.0 + th2 * th2 / 3840.0;
+        real = 1.0 - th2 / 8.0 + th2 * th2 / 384.0;
+    } else {
+        imag = sin(0.5 * th) / th;
+        real = cos(0.5 * th);
+    }
+    const double dq[4] = {imag * phi[0], imag * phi[1], imag * phi[2], real};
+    double dt[3] = {tau[0], tau[1], tau[2]};
+    if (th > 1e-4) {
+        const double ca = (1 - cos(th)) / th2;
+        const double cb = (th - sin(th)) / (th * th2);
+        double c1[3] = {phi[1] * tau[2] - phi[2] * tau[1], phi[2] * tau[0] - phi[0] * tau[2],
+                        phi[0] * tau[1] - phi[1] * tau[0]};
+        double c2[3] = {phi[1] * c1[2] - phi[2] * c1[1], phi[2] * c1[0] - phi[0] * c1[2],
+                        phi[0] * c1[1] - phi[1] * c1[0]};
+        for (int r = 0; r < 3; r++) dt[r] += ca * c1[r] + cb * c2[r];
+    }
+    float* T = poses + 7 * (t0 + i);
+    const double t[3] = {T[0], T[1], T[2]};
+    const double q[4] = {T[3], T[4], T[5], T[6]};
+    double q1[4];
+    q1[0] = dq[3] * q[0] + dq[0] * q[3] + dq[1] * q[2] - dq[2] * q[1];
+    q1[1] = dq[3] * q[1] + dq[1] * q[3] + dq[2] * q[0] - dq[0] * q[2];
+    q1[2] = dq[3] * q[2] + dq[2] * q[3] + dq[0] * q[1] - dq[1] * q[0];
+    q1[3] = dq[3] * q[3] - dq[0] * q[0] - dq[1] * q[1] - dq[2] * q[2];
+    // actSO3(dq, t) with the uv form
+    double uv[3] = {2.0 * (dq[1] * t[2] - dq[2] * t[1]), 2.0 * (dq[2] * t[0] - dq[0] * t[2]),
+                    2.0 * (dq[0] * t[1] - dq[1] * t[0])};
+    double t1[3];
+    t1[0] = t[0] + dq[3] * uv[0] + (dq[1] * uv[2] - dq[2] * uv[1]) + dt[0];
+    t1[1] = t[1] + dq[3] * uv[1] + (dq[2] * uv[0] - dq[0] * uv[2]) + dt[1];
+    t1[2] = t[2] + dq[3] * uv[2] + (dq[0] * uv[1] - dq[1] * uv[0]) + dt[2];
+    for (int r = 0; r < 3; r++) T[r] = (float)t1[r];
+    for (int r = 0; r < 4; r++) T[3 + r] = (float)q1[r];
+}
+
+}  // namespace ba
+}  // namespace vh
+
+// ---------------------------------------------------------------------------
+// host: plan construction and the run loop
+// ---------------------------------------------------------------------------
+using namespace vh;
+using namespace vh::ba;
+
+struct vh_ba_plan {
+    DevPlan d;
+    std::vector<int64_t> kx;
+    void* dev = nullptr;  // one allocation holding every index table
+    int deg_max = 0;
+    size_t ws_bytes = 0;
+    // workspace offsets
+    size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_Ld, o_flag;
+    // plan variants: the system size depends on whether intrinsics are optimised
+    int P;
+};
+
+namespace {
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" {
+
+vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W, int t0,
+                              int t1) {
+    if (E < 0 || N <= 0 || H <= 0 || W <= 0 || t0 < 0 || t1 <= t0 || t1 > N) {
+        fail(VH_ERR_ARG, "ba_plan_create: bad sizes (E=%d N=%d H=%d W=%d t0=%d t1=%d)", E, N, H, W, t0, t1);
+        return nullptr;
+    }
+    for (int e = 0; e < E; e++) {
+        if (ii_host[e] < 0 || ii_host[e] >= N || jj_host[e] < 0 || jj_host[e] >= N) {
+            fail(VH_ERR_ARG, "ba_plan_create: edge %d (%lld -> %lld) outside [0, %d)", e, (long long)ii_host[e],
+                 (long long)jj_host[e], N);
+            return nullptr;
+        }
+    }
+    vh_ba_plan* pl = new (std::nothrow) vh_ba_plan();
+    if (!pl) {
+        fail(VH_ERR_ARG, "ba_plan_create: out of host memory");
+        return nullptr;
+    }
+    const int P = t1 - t0;
+    const int HW = H * W;
+    pl->P = P;
+    if (6 * P + 1 > kMaxLd) {
+        delete pl;
+        fail(VH_ERR_UNSUPPORTED, "ba_plan_create: %d poses in the window (limit %d)", P, (kMaxLd - 1) / 6);
+        return nullptr;
+    }
+    // kx = unique(cat(arange(t0,t1), ii)), sorted (geom_kernels.cu:1648-1652)
+    std::vector<int64_t> cand;
+    for (int t = t0; t < t1; t++) cand.push_back(t);
+    for (int e = 0; e < E; e++) cand.push_back(ii_host[e]);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    pl->kx = cand;
+    const int K = (int)cand.size();
+    std::vector<int> slot(N, -1);
+    for (int k = 0; k < K; k++) slot[cand[k]] = k;
+    // per-frame edge lists in edge order
+    std::vector<int> fr_ptr(K + 1, 0), fr_edges(E), fr_self(K);
+    for (int e = 0; e < E; e++) fr_ptr[slot[ii_host[e]] + 1]++;
+    for (int k = 0; k < K; k++) fr_ptr[k + 1] += fr_ptr[k];
+    {
+        std::vector<int> fill(fr_ptr.begin(), fr_ptr.end() - 1);
+        for (int e = 0; e < E; e++) fr_edges[fill[slot[ii_host[e]]]++] = e;
+    }
+    int deg_max = 0, Rext_max = 1;
+    for (int k = 0; k < K; k++) {
+        fr_self[k] = (cand[k] >= t0 && cand[k] < t1) ? 1 : 0;
+        const int deg = fr_ptr[k + 1] - fr_ptr[k];
+        deg_max = std::max(deg_max, deg);
+        Rext_max = std::max(Rext_max, fr_self[k] + deg + 1);
+    }
+    if (deg_max > kMaxDeg || Rext_max > kMaxRext) {
+        delete pl;
+        fail(VH_ERR_UNSUPPORTED, "ba_plan_create: a depth frame has %d outgoing edges (limit %d)", deg_max, kMaxDeg);
+        return nullptr;
+    }
+    // Gram storage offsets: frame k holds Rext_k (Rext_k+1)/2 blocks of 36
+    std::vector<int> gram_off(K + 1, 0), pseudo_off(K);
+    for (int k = 0; k < K; k++) {
+        const int Rext = fr_self[k] + (fr_ptr[k + 1] - fr_ptr[k]) + 1;
+        gram_off[k + 1] = gram_off[k] + Rext * (Rext + 1) / 2 * 36;
+    }
+    auto gblock = [&](int k, int ra, int rb) {  // ra <= rb
+        const int Rext = fr_self[k] + (fr_ptr[k + 1] - fr_ptr[k]) + 1;
+        const int b = ra * Rext - ra * (ra - 1) / 2 + (rb - ra);
+        return gram_off[k] + b * 36;
+    };
+    for (int k = 0; k < K; k++) {
+        const int Rext = fr_self[k] + (fr_ptr[k + 1] - fr_ptr[k]) + 1;
+        pseudo_off[k] = gblock(k, Rext - 1, Rext - 1);
+    }
+    // tiles: TP pixels per sub-tile, sized so the E rows fit the LDS budget
+    int TP = 256;
+    while (TP > 64 && (size_t)Rext_max * 6 * (TP + 1) * 4 > kLdsBudget) TP /= 2;
+    const int SUB = 2;
+    const int T = (HW + TP * SUB - 1) / (TP * SUB);
+    // lower pose blocks and their contribution lists (fixed order)
+    std::vector<std::vector<int>> items((size_t)P * (P + 1) / 2);
+    auto lidx = [&](int a, int b) { return a * (a + 1) / 2 + b; };  // a >= b
+    for (int e = 0; e < E; e++) {
+        const int li = (int)ii_host[e] - t0, lj = (int)jj_host[e] - t0;
+        if (ii_host[e] == jj_host[e]) continue;  // stereo: no pose terms
+        const bool iin = li >= 0 && li < P, jin = lj >= 0 && lj < P;
+        if (iin) items[lidx(li, li)].push_back(pk(K_HII, e));
+        if (jin) items[lidx(lj, lj)].push_back(pk(K_HJJ, e));
+        if (iin && jin) {
+            if (li > lj) items[lidx(li, lj)].push_back(pk(K_HIJ, e));
+            else items[lidx(lj, li)].push_back(pk(K_HIJT, e));
+        }
+    }
+    std::vector<std::vector<int>> ritems(P);
+    for (int k = 0; k < K; k++) {
+        const int deg = fr_ptr[k + 1] - fr_ptr[k];
+        const int R = fr_self[k] + deg;
+        std::vector<int> rpose(R);
+        for (int r = 0; r < R; r++) {
+            if (fr_self[k] && r == 0) rpose[r] = (int)cand[k] - t0;
+            else rpose[r] = (int)jj_host[fr_edges[fr_ptr[k] + r - fr_self[k]]] - t0;
+        }
+        for (int ra = 0; ra < R; ra++) {
+            const int pa = rpose[ra];
+            if (pa < 0 || pa >= P) continue;
+            for (int rb = ra; rb < R; rb++) {
+                const int pb = rpose[rb];
+                if (pb < 0 || pb >= P) continue;
+                const int g = gblock(k, ra, rb);
+                if (pa > pb) items[lidx(pa, pb)].push_back(pk(K_G, g));
+                else if (pa < pb) items[lidx(pb, pa)].push_back(pk(K_GT, g));
+                else {
+                    items[lidx(pa, pa)].push_back(pk(K_G, g));
+                    if (ra != rb) items[lidx(pa, pa)].push_back(pk(K_GT, g));
+                }
+            }
+        }
+    }
+    // right-hand side / intrinsics column per pose, ordered by edge then by row
+    for (int e = 0; e < E; e++) {
+        const int li = (int)ii_host[e] - t0, lj = (int)jj_host[e] - t0;
+        if (li >= 0 && li < P) ritems[li].push_back(pk(K_VI, e));
+        if (lj >= 0 && lj < P) ritems[lj].push_back(pk(K_VJ, e));
+    }
+    for (int k = 0; k < K; k++) {
+        const int deg = fr_ptr[k + 1] - fr_ptr[k];
+        const int R = fr_self[k] + deg;
+        for (int r = 0; r < R; r++) {
+            int pr;
+            if (fr_self[k] && r == 0) pr = (int)cand[k] - t0;
+            else pr = (int)jj_host[fr_edges[fr_ptr[k] + r - fr_self[k]]] - t0;
+            if (pr < 0 || pr >= P) continue;
+            ritems[pr].push_back(pk(K_ROW, gblock(k, r, R)));
+        }
+    }
+    const int nlb = P * (P + 1) / 2;
+    std::vector<int> blk_ab(2 * nlb), blk_ptr(nlb + 1, 0), blk_items;
+    for (int a = 0, q = 0; a < P; a++)
+        for (int b = 0; b <= a; b++, q++) {
+            blk_ab[2 * q] = a;
+            blk_ab[2 * q + 1] = b;
+        }
+    for (int q = 0; q < nlb; q++) {
+        blk_ptr[q + 1] = blk_ptr[q] + (int)items[q].size();
+        blk_items.insert(blk_items.end(), items[q].begin(), items[q].end());
+    }
+    std::vector<int> rhs_ptr(P + 1, 0), rhs_items;
+    for (int a = 0; a < P; a++) {
+        rhs_ptr[a + 1] = rhs_ptr[a] + (int)ritems[a].size();
+        rhs_items.insert(rhs_items.end(), ritems[a].begin(), ritems[a].end());
+    }
+    std::vector<int> kx32(K), ii32(E), jj32(E);
+    for (int k = 0; k < K; k++) kx32[k] = (int)cand[k];
+    for (int e = 0; e < E; e++) {
+        ii32[e] = (int)ii_host[e];
+        jj32[e] = (int)jj_host[e];
+    }
+    // upload all tables in one allocation
+    std::vector<std::pair<const std::vector<int>*, const int**>> tabs = {
+        {&kx32, &pl->d.kx}, {&fr_ptr, &pl->d.fr_ptr}, {&fr_edges, &pl->d.fr_edges}, {&fr_self, &pl->d.fr_self},
+        {&gram_off, &pl->d.gram_off}, {&ii32, &pl->d.ii}, {&jj32, &pl->d.jj}, {&blk_ab, &pl->d.blk_ab},
+        {&blk_ptr, &pl->d.blk_ptr}, {&blk_items, &pl->d.blk_items}, {&rhs_ptr, &pl->d.rhs_ptr},
+        {&rhs_items, &pl->d.rhs_items}, {&pseudo_off, &pl->d.pseudo_off}};
+    size_t total = 0;
+    std::vector<size_t> offs;
+    for (auto& t : tabs) {
+        offs.push_back(total);
+        total += align256(std::max<size_t>(1, t.first->size()) * sizeof(int));
+    }
+    std::vector<char> host(total, 0);
+    for (size_t i = 0; i < tabs.size(); i++)
+        if (!tabs[i].first->empty()) memcpy(host.data() + offs[i], tabs[i].first->data(), tabs[i].first->size() * 4);
+    hipError_t err = hipMalloc(&pl->dev, total);
+    if (err != hipSuccess) {
+        hip_fail(err, "ba_plan_create hipMalloc");
+        delete pl;
+        return nullptr;
+    }
+    err = hipMemcpy(pl->dev, host.data(), total, hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        hip_fail(err, "ba_plan_create hipMemcpy");
+        (void)hipFree(pl->dev);
+        delete pl;
+        return nullptr;
+    }
+    for (size_t i = 0; i < tabs.size(); i++) *tabs[i].second = (const int*)((char*)pl->dev + offs[i]);
+
+    DevPlan& d = pl->d;
+    d.E = E; d.N = N; d.H = H; d.W = W; d.HW = HW; d.t0 = t0; d.t1 = t1; d.P = P; d.K = K;
+    d.TP = TP; d.SUB = SUB; d.T = T; d.Rext_max = Rext_max; d.GT = gram_off[K];
+    d.nlb = nlb;
+    const int nmax = 6 * P + 1;
+    d.ld = (nmax + kCB - 1) / kCB * kCB;
+    d.nb = d.ld / kCB;
+    d.n = 6 * P;
+    d.intr = 0;
+    pl->deg_max = deg_max;
+    // workspace layout
+    size_t o = 0;
+    pl->o_edge = o; o += align256((size_t)T * E * kNV * 4);
+    pl->o_gram = o; o += align256((size_t)T * d.GT * 4);
+    pl->o_fen = o; o += align256((size_t)T * K * 4);
+    pl->o_Q = o; o += align256((size_t)K * HW * 4);
+    pl->o_W = o; o += align256((size_t)K * HW * 4);
+    pl->o_ed = o; o += align256((size_t)std::max(E, 1) * kEDS * 8);
+    pl->o_gd = o; o += align256((size_t)std::max(d.GT, 1) * 8);
+    pl->o_fenk = o; o += align256((size_t)K * 8);
+    pl->o_A = o; o += align256((size_t)d.ld * d.ld * 8);
+    pl->o_b = o; o += align256((size_t)d.ld * 8);
+    pl->o_x = o; o += align256((size_t)d.ld * 8);
+    pl->o_Ld = o; o += align256((size_t)d.ld * kCB * 8);
+    pl->o_flag = o; o += 256;
+    pl->ws_bytes = o;
+    return pl;
+}
+
+void vh_ba_plan_destroy(vh_ba_plan* plan) {
+    if (!plan) return;
+    if (plan->dev) (void)hipFree(plan->dev);
+    delete plan;
+}
+
+int vh_ba_plan_num_depth_frames(const vh_ba_plan* plan) { return plan ? (int)plan->kx.size() : -1; }
+
+int vh_ba_plan_depth_frames(const vh_ba_plan* plan, int64_t* kx_out) {
+    if (!plan || !kx_out) return fail(VH_ERR_ARG, "ba_plan_depth_frames: null argument");
+    for (size_t i = 0; i < plan->kx.size(); i++) kx_out[i] = plan->kx[i];
+    return VH_OK;
+}
+
+size_t vh_ba_workspace_size(const vh_ba_plan* plan) { return plan ? plan->ws_bytes : 0; }
+
+int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
+              const float* targets, const float* weights, const float* eta, const float* depth_active,
+              int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
+              float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* workspace,
+              void* stream) {
+    if (!plan) return fail(VH_ERR_ARG, "ba_run: null plan");
+    if (iterations < 0) return fail(VH_ERR_ARG, "ba_run: negative iterations");
+    if (iterations == 0) return VH_OK;
+    if (!poses || !disps || !intrinsics || !disps_sens || !targets || !weights || !depth_active || !dx ||
+        !workspace)
+        return fail(VH_ERR_ARG, "ba_run: null tensor pointer");
+    const bool motion = (flags & VH_BA_MOTION_ONLY) != 0;
+    const bool intr = (flags & VH_BA_OPTIMIZE_INTRINSICS) != 0;
+    const bool cen = (flags & VH_BA_COMPUTE_ENERGY) != 0;
+    if (!motion && (!eta || !dz)) return fail(VH_ERR_ARG, "ba_run: eta and dz are required unless motion only");
+    if (cen && !energy) return fail(VH_ERR_ARG, "ba_run: energy output required");
+    hipStream_t s = (hipStream_t)stream;
+    DevPlan d = plan->d;
+    d.intr = intr ? 1 : 0;
+    d.n = 6 * d.P + (intr ? 1 : 0);
+    char* ws = (char*)workspace;
+    Work w;
+    w.edge_part = (float*)(ws + plan->o_edge);
+    w.gram_part = (float*)(ws + plan->o_gram);
+    w.fen_part = (float*)(ws + plan->o_fen);
+    w.Qb = (float*)(ws + plan->o_Q);
+    w.Wb = (float*)(ws + plan->o_W);
+    w.ed = (double*)(ws + plan->o_ed);
+    w.gd = (double*)(ws + plan->o_gd);
+    w.fen = (double*)(ws + plan->o_fenk);
+    w.A = (double*)(ws + plan->o_A);
+    w.b = (double*)(ws + plan->o_b);
+    w.x = (double*)(ws + plan->o_x);
+    w.Ld = (double*)(ws + plan->o_Ld);
+    w.flag = (int*)(ws + plan->o_flag);
+    RunArgs a;
+    a.poses = poses; a.disps = disps; a.intrinsics = intrinsics; a.disps_sens = disps_sens;
+    a.targets = targets; a.weights = weights; a.eta = eta; a.depth_active = depth_active;
+    a.lm = lm; a.ep = ep; a.alpha = alpha; a.ilm = intrinsics_lm; a.iep = intrinsics_ep;
+    a.iscale = intrinsics_scale; a.dx = dx; a.dz = dz; a.energy = energy; a.compute_energy = cen ? 1 : 0;
+    const size_t lds = std::max((size_t)d.Rext_max * 6 * (d.TP + 1), (size_t)kThreads * 36) * 4;
+    const dim3 fgrid(d.T, d.K);
+    const int npad = d.ld - d.n;
+    for (int itr = 0; itr < iterations; itr++) {
+        a.itr = itr;
+        if (d.E > 0 && d.K > 0) {
+            if (motion) {
+                if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                else hipLaunchKernelGGL((ba_frame_kernel<false, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
+            } else {
+                if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                else hipLaunchKernelGGL((ba_frame_kernel<false, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
+            }
+            VH_LAUNCH_CHECK("ba_frame_kernel");
+            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + (motion ? 0 : d.K)), dim3(kThreads), 0, s, d, a, w,
+                               motion ? 1 : 0);
+            VH_LAUNCH_CHECK("ba_finalize_kernel");
+        } else if (!motion && d.K > 0) {
+            // no edges: every depth frame still has its prior / damping terms
+            hipLaunchKernelGGL((ba_frame_kernel<false, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
+            VH_LAUNCH_CHECK("ba_frame_kernel");
+            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + d.K), dim3(kThreads), 0, s, d, a, w, 0);
+            VH_LAUNCH_CHECK("ba_finalize_kernel");
+        }
+        hipLaunchKernelGGL(ba_assemble_kernel, dim3(d.nlb + d.P + 1 + npad), dim3(64), 0, s, d, a, w,
+                           motion ? 1 : 0);
+        VH_LAUNCH_CHECK("ba_assemble_kernel");
+        for (int kb = 0; kb < d.nb; kb++) {
+            hipLaunchKernelGGL(chol_panel_kernel, dim3(d.nb - kb), dim3(64), 0, s, w.A, w.Ld, d.ld, kb, w.flag);
+            VH_LAUNCH_CHECK("chol_panel_kernel");
+            const int rem = d.nb - kb - 1;
+            if (rem > 0) {
+                hipLaunchKernelGGL(chol_update_kernel, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, w.A, d.ld, kb,
+                                   d.nb);
+                VH_LAUNCH_CHECK("chol_update_kernel");
+            }
+        }
+        hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(64), 0, s, w.A, w.Ld, d.ld, d.nb, w.b, w.x, w.flag, d.P,
+                           dx);
+        VH_LAUNCH_CHECK("chol_solve_kernel");
+        if (!motion && d.K > 0) {
+            const dim3 bgrid((d.HW + kThreads - 1) / kThreads, d.K);
+            if (intr) hipLaunchKernelGGL(ba_backsub_kernel<true>, bgrid, dim3(kThreads), 0, s, d, a, w);
+            else hipLaunchKernelGGL(ba_backsub_kernel<false>, bgrid, dim3(kThreads), 0, s, d, a, w);
+            VH_LAUNCH_CHECK("ba_backsub_kernel");
+        }
+        hipLaunchKernelGGL(ba_retract_kernel, dim3((d.P + 63) / 64), dim3(64), 0, s, poses, intrinsics, w.x, d.t0, d.P,
+                           intr ? 1 : 0, intrinsics_scale);
+        VH_LAUNCH_CHECK("ba_retract_kernel");
+    }
+    return VH_OK;
+}
+
+}  // extern "C"

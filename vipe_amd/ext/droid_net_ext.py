@@ -1,0 +1,153 @@
+"""Drop-in for ``vipe.ext.droid_net_ext`` (csrc/droid_net_ext/droid.cpp:24-82).
+
+Same names, positional arguments and list returns as the reference binding:
+
+    corr_index_forward(volume, coords, radius)               -> [corr]
+    corr_index_backward(volume, coords, corr_grad, radius)    -> [volume_grad]
+    altcorr_forward(fmap1, fmap2, coords, radius)             -> [corr]
+    altcorr_backward(fmap1, fmap2, coords, corr_grad, radius) -> [fmap1_grad, fmap2_grad, coords_grad]
+    altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius) -> [corr]
+
+Non-contiguous inputs raise ``RuntimeError("<name> must be contiguous")``
+(droid.cpp:11-12).  The ops run on gfx950 on the current stream and need
+device tensors, as the reference's CUDA-only kernels do.
+
+``corr_build`` is this package's all-pairs volume + pyramid op behind
+``vipe_amd.corr.CorrBlock`` (the reference does the same with torch.matmul and
+avg_pool2d, vipe/slam/networks/droid_net.py:56-69, 94-102).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from .. import _lib
+
+
+def _contig(**named):
+    for k, v in named.items():
+        if not v.is_contiguous():
+            raise RuntimeError(f"{k} must be contiguous")
+
+
+def _gpu(*ts):
+    for t in ts:
+        if t.device.type != "cuda":
+            raise RuntimeError("droid_net_ext ops run on the GPU; got a tensor on " + str(t.device))
+
+
+def _code(t, allowed):
+    if t.dtype not in allowed:
+        raise RuntimeError(f"unsupported dtype {t.dtype}")
+    return _lib.DTYPE_CODE[t.dtype]
+
+
+_VOL_DT = (torch.float16, torch.float32, torch.float64)
+
+
+def corr_index_forward(volume, coords, radius):
+    _contig(volume=volume, coords=coords)
+    _gpu(volume, coords)
+    if coords.dtype != torch.float32:
+        raise RuntimeError("coords must be float32")
+    B, H1, W1, H2, W2 = volume.shape
+    rd = 2 * int(radius) + 1
+    corr = torch.empty((B, rd, rd, H1, W1), dtype=volume.dtype, device=volume.device)
+    with torch.cuda.device(volume.device):
+        rc = _lib.load().vh_corr_index_forward(_code(volume, _VOL_DT), B, H1, W1, H2, W2, _lib.ptr(volume),
+                                               _lib.ptr(coords), int(radius), _lib.ptr(corr), _lib.stream_of(volume))
+    _lib.check(rc, "corr_index_forward")
+    return [corr]
+
+
+def corr_index_backward(volume, coords, corr_grad, radius):
+    _contig(volume=volume, coords=coords, corr_grad=corr_grad)
+    _gpu(volume, coords, corr_grad)
+    B, H1, W1, H2, W2 = volume.shape
+    volume_grad = torch.zeros_like(volume)
+    g = corr_grad if corr_grad.dtype == volume.dtype else corr_grad.to(volume.dtype).contiguous()
+    with torch.cuda.device(volume.device):
+        rc = _lib.load().vh_corr_index_backward(_code(volume, _VOL_DT), B, H1, W1, H2, W2, _lib.ptr(coords),
+                                                _lib.ptr(g), int(radius), _lib.ptr(volume_grad),
+                                                _lib.stream_of(volume))
+    _lib.check(rc, "corr_index_backward")
+    return [volume_grad]
+
+
+def altcorr_forward(fmap1, fmap2, coords, radius):
+    _contig(fmap1=fmap1, fmap2=fmap2, coords=coords)
+    _gpu(fmap1, fmap2, coords)
+    B, N, H1, W1, _ = coords.shape
+    _, H2, W2, Cc = fmap2.shape
+    rd = 2 * int(radius) + 1
+    corr = torch.empty((B, N, rd * rd, H1, W1), dtype=fmap1.dtype, device=fmap1.device)
+    with torch.cuda.device(fmap1.device):
+        rc = _lib.load().vh_altcorr_forward(_code(fmap1, (torch.float16, torch.float32)), B, N, H1, W1, H2, W2,
+                                            fmap1.shape[3], _lib.ptr(fmap1), _lib.ptr(fmap2), _lib.ptr(coords),
+                                            int(radius), _lib.ptr(corr), _lib.stream_of(fmap1))
+    _lib.check(rc, "altcorr_forward")
+    return [corr]
+
+
+def altcorr_backward(fmap1, fmap2, coords, corr_grad, radius):
+    _contig(fmap1=fmap1, fmap2=fmap2, coords=coords, corr_grad=corr_grad)
+    _gpu(fmap1, fmap2, coords, corr_grad)
+    B, N, H1, W1, _ = coords.shape
+    _, H2, W2, Cc = fmap2.shape
+    fmap1_grad = torch.zeros((B, H1, W1, fmap1.shape[3]), dtype=fmap1.dtype, device=fmap1.device)
+    fmap2_grad = torch.zeros((B, H2, W2, Cc), dtype=fmap1.dtype, device=fmap1.device)
+    coords_grad = torch.zeros((B, N, H1, W1, 2), dtype=fmap1.dtype, device=fmap1.device)
+    with torch.cuda.device(fmap1.device):
+        rc = _lib.load().vh_altcorr_backward(_code(fmap1, (torch.float32,)), B, N, H1, W1, H2, W2, fmap1.shape[3],
+                                             _lib.ptr(fmap1), _lib.ptr(fmap2), _lib.ptr(coords), _lib.ptr(corr_grad),
+                                             int(radius), _lib.ptr(fmap1_grad), _lib.ptr(fmap2_grad),
+                                             _lib.stream_of(fmap1))
+    _lib.check(rc, "altcorr_backward")
+    return [fmap1_grad, fmap2_grad, coords_grad]
+
+
+def altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius):
+    _contig(fmap1=fmap1, coords=coords, ii=ii, jj=jj)
+    for f in fmap2_pyramid:
+        _contig(fmap2=f)
+    _gpu(fmap1, coords, ii, jj, *fmap2_pyramid)
+    B, Nf, H, W, Cc = fmap1.shape
+    _, M, _, _, _ = coords.shape
+    L = len(fmap2_pyramid)
+    rd = 2 * int(radius) + 1
+    corr = torch.empty((B, M, L * rd * rd, H, W), dtype=torch.float32, device=fmap1.device)
+    lv = (C.c_void_p * max(L, 1))(*[f.data_ptr() for f in fmap2_pyramid])
+    Hl = (C.c_int * max(L, 1))(*[f.shape[2] for f in fmap2_pyramid])
+    Wl = (C.c_int * max(L, 1))(*[f.shape[3] for f in fmap2_pyramid])
+    iil = ii if ii.dtype == torch.int64 else ii.long().contiguous()
+    jjl = jj if jj.dtype == torch.int64 else jj.long().contiguous()
+    with torch.cuda.device(fmap1.device):
+        rc = _lib.load().vh_altcorr_index_forward(_code(fmap1, (torch.float16, torch.float32)), B, Nf, M, H, W, Cc,
+                                                  _lib.ptr(fmap1), L, lv, Hl, Wl, _lib.ptr(coords), _lib.ptr(iil),
+                                                  _lib.ptr(jjl), int(radius), _lib.ptr(corr), _lib.stream_of(fmap1))
+    _lib.check(rc, "altcorr_index_forward")
+    return [corr]
+
+
+def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None):
+    """All-pairs correlation volume pyramid of CorrBlock.  fmap1, fmap2 [E,C,H,W]
+    (f32 / f16 / bf16) -> list of level-l volumes [E,H,W,H>>l,W>>l] in out_dtype
+    (default: f16 for 16-bit inputs, f32 for f32 inputs)."""
+    _contig(fmap1=fmap1, fmap2=fmap2)
+    _gpu(fmap1, fmap2)
+    E, Cc, H, W = fmap1.shape
+    if out_dtype is None:
+        out_dtype = torch.float32 if fmap1.dtype == torch.float32 else torch.float16
+    levels = []
+    h, w = H, W
+    for _ in range(num_levels):
+        levels.append(torch.empty((E, H, W, h, w), dtype=out_dtype, device=fmap1.device))
+        h, w = h // 2, w // 2
+    lv = (C.c_void_p * num_levels)(*[t.data_ptr() for t in levels])
+    with torch.cuda.device(fmap1.device):
+        rc = _lib.load().vh_corr_build(_code(fmap1, (torch.float32, torch.float16, torch.bfloat16)),
+                                       _lib.DTYPE_CODE[out_dtype], E, Cc, H, W, _lib.ptr(fmap1), _lib.ptr(fmap2),
+                                       num_levels, lv, _lib.stream_of(fmap1))
+    _lib.check(rc, "corr_build")
+    return levels

@@ -1,0 +1,133 @@
+"""Drop-in for ``vipe.ext.slam_ext`` bundle adjustment (csrc/slam_ext/slam.cpp:41-51).
+
+    ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj,
+       t0, t1, iterations, lm, ep, motion_only, alpha=0.05) -> [dx, dz, energy]
+    ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj,
+                depth_active, t0, t1, iterations, lm, ep, motion_only, alpha,
+                optimize_intrinsics, intrinsics_lm, intrinsics_ep, intrinsics_scale,
+                compute_energy=False) -> [dx, dz, energy]
+
+``poses`` [N,7], ``disps`` [N,H,W] and (when optimising intrinsics)
+``intrinsics`` [4] are updated in place; ``dx`` is [t1-t0, 6] and ``dz``
+[|kx|, H*W] from the last iteration (``None`` when motion only or when no
+iteration ran, as the reference returns undefined tensors), ``energy`` has
+``iterations`` entries when requested and is empty otherwise
+(geom_kernels.cu:1626-1847).  Non-contiguous inputs raise
+``RuntimeError("<name> must be contiguous")`` (geom_kernels.cu:30-31,
+1634-1642).
+
+The whole solve runs on the device: per-edge Jacobians and Hessian blocks,
+the Schur complement, the FP64 Cholesky of the reduced camera system and the
+retractions.  The edge list is copied to the host once per call to build (or
+fetch from a cache) the index bookkeeping.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from collections import OrderedDict
+
+import torch
+
+from .. import _lib
+
+_PLANS: "OrderedDict[tuple, _Plan]" = OrderedDict()
+_PLAN_CACHE_SIZE = 32
+
+
+class _Plan:
+    def __init__(self, ii_cpu, jj_cpu, N, H, W, t0, t1):
+        lib = _lib.load()
+        ii_c = ii_cpu.to(torch.int64).contiguous()
+        jj_c = jj_cpu.to(torch.int64).contiguous()
+        h = lib.vh_ba_plan_create(C.c_void_p(ii_c.data_ptr()), C.c_void_p(jj_c.data_ptr()), int(ii_c.numel()),
+                                  int(N), int(H), int(W), int(t0), int(t1))
+        if not h:
+            raise RuntimeError(f"slam_ext.ba: {_lib.last_error()}")
+        self.h = C.c_void_p(h)
+        self.K = lib.vh_ba_plan_num_depth_frames(self.h)
+        self.ws_bytes = int(lib.vh_ba_workspace_size(self.h))
+        kx = torch.empty(self.K, dtype=torch.int64)
+        lib.vh_ba_plan_depth_frames(self.h, C.c_void_p(kx.data_ptr()))
+        self.kx = kx
+
+    def __del__(self):
+        try:
+            _lib.load().vh_ba_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+def get_plan(ii, jj, N, H, W, t0, t1) -> _Plan:
+    ii_cpu = ii.detach().to("cpu", torch.int64)
+    jj_cpu = jj.detach().to("cpu", torch.int64)
+    key = (ii_cpu.numpy().tobytes(), jj_cpu.numpy().tobytes(), int(N), int(H), int(W), int(t0), int(t1))
+    plan = _PLANS.get(key)
+    if plan is None:
+        plan = _Plan(ii_cpu, jj_cpu, N, H, W, t0, t1)
+        _PLANS[key] = plan
+        while len(_PLANS) > _PLAN_CACHE_SIZE:
+            _PLANS.popitem(last=False)
+    else:
+        _PLANS.move_to_end(key)
+    return plan
+
+
+def _check(**named):
+    for k, v in named.items():
+        if not v.is_contiguous():
+            raise RuntimeError(f"{k} must be contiguous")
+
+
+def _f32(name, t):
+    if t.dtype != torch.float32:
+        raise RuntimeError(f"{name} must be float32 (got {t.dtype})")
+
+
+def ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active, t0, t1,
+                iterations, lm, ep, motion_only, alpha, optimize_intrinsics, intrinsics_lm, intrinsics_ep,
+                intrinsics_scale, compute_energy=False):
+    _check(targets=targets, weights=weights, poses=poses, disps=disps, intrinsics=intrinsics,
+           disps_sens=disps_sens, ii=ii, jj=jj, depth_active=depth_active)
+    for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics), ("disps_sens", disps_sens),
+                    ("targets", targets), ("weights", weights), ("depth_active", depth_active)):
+        _f32(name, t)
+        if t.device.type != "cuda":
+            raise RuntimeError(f"slam_ext.ba runs on the GPU; {name} is on {t.device}")
+    N, H, W = disps.shape
+    E = ii.shape[0]
+    t0, t1, iterations = int(t0), int(t1), int(iterations)
+    if poses.shape != (N, 7):
+        raise RuntimeError(f"poses must be [N,7] SE3 (got {tuple(poses.shape)})")
+    if targets.shape != (E, 2, H, W) or weights.shape != (E, 2, H, W):
+        raise RuntimeError("targets/weights must be [E,2,H,W]")
+    plan = get_plan(ii, jj, N, H, W, t0, t1)
+    P = t1 - t0
+    dev = poses.device
+    eta_c = eta.contiguous().to(torch.float32) if not motion_only else eta
+    if not motion_only and eta_c.numel() != plan.K * H * W:
+        raise RuntimeError(f"eta must hold |kx| = {plan.K} maps of {H}x{W}")
+    energy = torch.empty((iterations if compute_energy else 0,), dtype=torch.float32, device=dev)
+    if iterations <= 0:
+        return [None, None, energy]
+    dx = torch.empty((P, 6), dtype=torch.float32, device=dev)
+    dz = None if motion_only else torch.empty((plan.K, H * W), dtype=torch.float32, device=dev)
+    ws = torch.empty((max(plan.ws_bytes, 1),), dtype=torch.uint8, device=dev)
+    flags = (1 if motion_only else 0) | (2 if optimize_intrinsics else 0) | (4 if compute_energy else 0)
+    lib = _lib.load()
+    with torch.cuda.device(dev):
+        rc = lib.vh_ba_run(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(disps_sens),
+                           _lib.ptr(targets), _lib.ptr(weights), None if motion_only else _lib.ptr(eta_c),
+                           _lib.ptr(depth_active), iterations, float(lm), float(ep), float(alpha),
+                           float(intrinsics_lm), float(intrinsics_ep), float(intrinsics_scale), flags, _lib.ptr(dx),
+                           _lib.ptr(dz), _lib.ptr(energy) if compute_energy else None, _lib.ptr(ws),
+                           _lib.stream_of(poses))
+    _lib.check(rc, "slam_ext.ba")
+    return [dx, dz, energy]
+
+
+def ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, t0, t1, iterations, lm, ep,
+       motion_only, alpha=0.05):
+    """Legacy entry: depth_active = 1, no intrinsics, no energy (geom_kernels.cu:1822-1830)."""
+    depth_active = torch.ones(disps.shape[0], dtype=disps.dtype, device=disps.device)
+    return ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active, t0, t1,
+                       iterations, lm, ep, motion_only, alpha, False, 1e-6, 1e-6, 1.0, False)
