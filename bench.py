@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--mode", choices=["fp32", "mixed"], default="fp32")
     ap.add_argument("--workload", default="kf64_e384_64x80")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-edges", type=int, default=8, help="edges of the bounded CPU lookup sample")
+    ap.add_argument("--cpu-edges", type=int, default=32, help="edges of the bounded CPU lookup sample")
     return ap.parse_args()
 
 

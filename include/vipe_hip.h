@@ -83,9 +83,18 @@ int vh_lie_Jinv(int group, int dtype, int64_t n, const void* X, const void* a, v
 int vh_corr_index_forward(int dtype, int64_t B, int H1, int W1, int H2, int W2, const void* volume,
                           const float* coords, int radius, void* corr, void* stream);
 /* corr_index_backward (droid.cpp:32, correlation_kernels.cu:68-113, 138-158).
- * volume_grad [B,H1,W1,H2,W2] out, fully written (zeros outside the taps). */
+ * volume_grad [B,H1,W1,H2,W2] must be zero-filled by the caller (the reference
+ * allocates it with zeros_like); the kernel writes the gradient of every tap. */
 int vh_corr_index_backward(int dtype, int64_t B, int H1, int W1, int H2, int W2, const float* coords,
                            const void* corr_grad, int radius, void* volume_grad, void* stream);
+
+/* Fused pyramid lookup: CorrBlock.__call__ (vipe/slam/networks/droid_net.py:71-84,
+ * i.e. the per-level corr_index_forward calls plus their concatenation).
+ * levels[l] [B,H1,W1,Hl[l],Wl[l]] (dtype F16/F32/F64), coords [B,H1,W1,2] f32 at
+ * level-0 resolution (level l samples at coords / 2^l), out [B, L*(2r+1)^2, H1, W1]
+ * in the volume dtype, channel l*(2r+1)^2 + i*(2r+1) + j, fully written. */
+int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels, const void* const* levels,
+                           const int* Hl, const int* Wl, const float* coords, int radius, void* out, void* stream);
 
 /* All-pairs correlation volume and its avg-pool pyramid: the CorrBlock
  * constructor (vipe/slam/networks/droid_net.py:56-69, 94-102).

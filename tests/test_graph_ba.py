@@ -1,0 +1,144 @@
+"""Caller-side fused BA (GraphBuffer._fused_ba restatement): the reference's
+error messages (tests/test_graph_buffer_fused_ba.py:277-324) on CPU, and the
+fixture of that test file solved on the GPU against the FP64 oracle."""
+from types import SimpleNamespace
+
+import pytest
+import torch
+
+from oracle import ba as OB
+from vipe_amd import graph_ba
+from vipe_amd.synth import reproject
+
+
+class FixtureSparseTracks:
+    """tests/test_graph_buffer_fused_ba.py:24-50: a constant-velocity track target."""
+
+    def __init__(self, enabled):
+        self.enabled = enabled
+
+    def compute_dense_disp_target_weight(self, source_view_inds, source_frame_inds, target_view_inds,
+                                         target_frame_inds, image_size, dense_disp_size):
+        h, w = dense_disp_size
+        yy, xx = torch.meshgrid(torch.arange(h, device=source_frame_inds.device, dtype=torch.float32),
+                                torch.arange(w, device=source_frame_inds.device, dtype=torch.float32), indexing="ij")
+        coords = torch.stack([xx, yy], dim=-1)
+        dt = (target_frame_inds - source_frame_inds).to(torch.float32).view(-1, 1, 1, 1)
+        target = coords.unsqueeze(0) + torch.cat([0.03 * dt, -0.02 * dt], dim=-1)
+        weight = torch.full_like(target, 0.07)
+        weight = weight * (source_frame_inds != target_frame_inds).to(torch.float32).view(-1, 1, 1, 1)
+        return target, weight
+
+
+def make_buffer(device, tracks=False, robust=None):
+    """tests/test_graph_buffer_fused_ba.py:70-106 construction (CPU RNG)."""
+    torch.manual_seed(20260519)
+    height, width, n_frames, size = 32, 40, 5, 6
+    h8, w8 = height // 8, width // 8
+    poses = torch.zeros(size, 7)
+    poses[:, 6] = 1.0
+    poses[:n_frames, 0] = torch.linspace(0.0, 0.08, n_frames)
+    poses[:n_frames, 1] = torch.tensor([0.0, 0.01, -0.005, 0.015, 0.02])
+    poses[:n_frames, 2] = torch.linspace(0.0, 0.025, n_frames)
+    yy, xx = torch.meshgrid(torch.arange(h8, dtype=torch.float32), torch.arange(w8, dtype=torch.float32), indexing="ij")
+    base = 0.72 + 0.015 * xx + 0.02 * yy
+    disps = torch.full((size, 1, h8, w8), 0.8)
+    for f in range(n_frames):
+        disps[f, 0] = base + 0.025 * f
+    disps_sens = torch.zeros(size, 1, h8, w8)
+    disps_sens[2, 0] = disps[2, 0] * 1.015
+    disps_sens[4, 0] = disps[4, 0] * 0.985
+    intr = torch.zeros(1, 4)
+    intr[0, :4] = torch.tensor([38.0, 38.0, 19.5, 15.5])
+    buf = SimpleNamespace(
+        poses=poses.to(device), disps=disps.to(device), disps_sens=disps_sens.to(device), intrinsics=intr.to(device),
+        tstamp=torch.arange(size).to(device), rig=torch.tensor([[0, 0, 0, 0, 0, 0, 1.0]]).to(device), n_views=1,
+        camera_type="pinhole", height=height, width=width, device=device,
+        ba_config={"dense_disp_alpha": 0.001, "intrinsics_damping_scale": 1.0, "robust_kernel": robust},
+        sparse_tracks=FixtureSparseTracks(tracks))
+    return buf
+
+
+def make_inputs(buf):
+    """tests/test_graph_buffer_fused_ba.py:109-139 (targets from the reprojection)."""
+    ii = torch.tensor([1, 2, 3, 4, 0, 3])
+    jj = torch.tensor([0, 0, 1, 2, 2, 3])
+    h8, w8 = buf.height // 8, buf.width // 8
+    coords, _ = reproject(buf.poses.cpu(), buf.disps.cpu()[:, 0], buf.intrinsics.cpu()[0] / 8.0, ii, jj)
+    target = coords.float().reshape(6, h8 * w8, 2)
+    yy, xx = torch.meshgrid(torch.arange(h8, dtype=torch.float32), torch.arange(w8, dtype=torch.float32), indexing="ij")
+    delta = torch.stack([0.01 * torch.sin(0.7 * xx + 0.2 * yy), -0.012 * torch.cos(0.3 * xx - 0.5 * yy)], -1)
+    edge = torch.linspace(-0.015, 0.02, 6).view(-1, 1, 1)
+    target = target + delta.reshape(1, -1, 2) + torch.cat([edge, -0.5 * edge], dim=-1)
+    weight = (0.2 + 0.03 * (xx + yy).reshape(1, -1, 1)).repeat(6, 1, 2).contiguous()
+    weight[-1].zero_()
+    damping = 0.08 + 0.01 * torch.arange(6, dtype=torch.float32).view(-1, 1, 1) + 0.002 * xx + 0.001 * yy
+    dev = buf.poses.device
+    return target.contiguous().to(dev), weight.to(dev), damping.contiguous().to(dev), ii.to(dev), jj.to(dev)
+
+
+def _call(buf, inp, **kw):
+    target, weight, damping, ii, jj = inp
+    args = dict(t0=1, t1=5, n_iters=1, pose_damping=1e-3, pose_ep=0.1, motion_only=False, limited_disp=False,
+                optimize_intrinsics=False, optimize_rig_rotation=False, verbose=False)
+    args.update(kw)
+    return graph_ba.bundle_adjustment(buf, target.clone(), weight.clone(), damping.clone(), ii.clone(), jj.clone(),
+                                      **args)
+
+
+def test_errors_instead_of_falling_back():
+    buf = make_buffer("cpu")
+    inp = make_inputs(buf)
+    with pytest.raises(RuntimeError, match="does not support optimizing rig rotation"):
+        _call(buf, inp, optimize_rig_rotation=True)
+    with pytest.raises(RuntimeError, match="the active optimization range is empty"):
+        _call(buf, inp, t0=3, t1=3)
+    buf2 = make_buffer("cpu", robust="huber")
+    with pytest.raises(RuntimeError, match="robust kernels are not supported"):
+        _call(buf2, inp)
+    buf3 = make_buffer("cpu")
+    buf3.n_views = 2
+    with pytest.raises(RuntimeError, match="single-view pinhole-camera"):
+        _call(buf3, inp)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tracks,limited,intr,motion", [(True, True, True, False), (False, False, False, True),
+                                                        (False, False, False, False)])
+def test_fixture_matches_oracle(tracks, limited, intr, motion):
+    buf = make_buffer("cuda", tracks=tracks)
+    inp = make_inputs(buf)
+    d0 = buf.disps.clone()
+    ref = make_buffer("cpu", tracks=tracks)
+    _call(buf, inp, limited_disp=limited, optimize_intrinsics=intr, motion_only=motion)
+    # independent preparation for the oracle, per buffer.py:408-467
+    target, weight, damping, ii, jj = [t.cpu() for t in inp]
+    m = ii != jj
+    ii, jj, target, weight = ii[m], jj[m], target[m], weight[m]
+    if tracks:
+        st, sw = ref.sparse_tracks.compute_dense_disp_target_weight(ii * 0, ref.tstamp[ii], jj * 0, ref.tstamp[jj],
+                                                                    (32, 40), (4, 5))
+        st, sw = st.flatten(1, 2), sw.flatten(1, 2)
+        cw = weight + sw
+        target = torch.where(cw > 0, (weight * target + sw * st) / cw.clamp_min(1e-12), target)
+        weight = cw
+    target = target.view(-1, 4, 5, 2).permute(0, 3, 1, 2).contiguous()
+    weight = weight.view(-1, 4, 5, 2).permute(0, 3, 1, 2).contiguous()
+    kx = torch.unique(torch.cat([torch.arange(1, 5), ii]))
+    eta = 0.2 * damping[kx] + 1e-7
+    da = torch.ones(6)
+    if limited:
+        da.zero_()
+        da[1:5] = 1.0
+    intrinsics = ref.intrinsics[0, :4] / 8.0
+    disps = ref.disps[:, 0].clone()
+    OB.ba_extended(ref.poses, disps, intrinsics, ref.disps_sens[:, 0], target, weight, eta, ii, jj, da, 1, 5, 1,
+                   1e-3, 0.1, motion, 0.001, intr, 1e-6, 1e-6, 0.125, False)
+    disps = disps.clamp(min=0.001)
+    torch.testing.assert_close(buf.poses[:5].cpu().double(), ref.poses[:5].double(), atol=2e-5, rtol=1e-5)
+    torch.testing.assert_close(buf.disps[:5, 0].cpu().double(), disps[:5].double(), atol=2e-4, rtol=2e-4)
+    if intr:
+        torch.testing.assert_close(buf.intrinsics[0, :4].cpu().double(), (intrinsics * 8.0).double(),
+                                   atol=2e-4, rtol=2e-4)
+    if motion:
+        assert torch.equal(buf.disps, d0)

@@ -53,6 +53,7 @@ _SIGS = {
     # correlation
     "vh_corr_index_forward": (i32, [i32, i64, i32, i32, i32, i32, vp, vp, i32, vp, vp]),
     "vh_corr_index_backward": (i32, [i32, i64, i32, i32, i32, i32, vp, vp, i32, vp, vp]),
+    "vh_corr_lookup_pyramid": (i32, [i32, i64, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]),
     "vh_corr_build": (i32, [i32, i32, i64, i32, i32, i32, vp, vp, i32, vp, vp]),
     "vh_corr_build_fp8_workspace": (sz, [i64, i32, i32, i32]),
     "vh_corr_build_fp8": (i32, [i32, i64, i32, i32, i32, vp, vp, i32, vp, vp, vp]),

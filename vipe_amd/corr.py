@@ -36,6 +36,33 @@ class CorrSampler(torch.autograd.Function):
         return grad_volume, None, None
 
 
+class CorrPyramidSampler(torch.autograd.Function):
+    """All levels of one CorrBlock lookup in one kernel; backward scatters each
+    level's output gradient into its volume (corr_index_backward)."""
+
+    @staticmethod
+    def forward(ctx, coords, radius, *levels):
+        ctx.radius = radius
+        ctx.save_for_backward(coords, *levels)
+        return droid_net_ext.corr_lookup_pyramid(list(levels), coords, radius)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        coords, *levels = ctx.saved_tensors
+        rd = 2 * ctx.radius + 1
+        B, H, W, _ = coords.shape
+        cxy = coords.permute(0, 3, 1, 2).contiguous()
+        grads = []
+        for i, vol in enumerate(levels):
+            if not ctx.needs_input_grad[2 + i]:
+                grads.append(None)
+                continue
+            g = grad_output[:, i * rd * rd:(i + 1) * rd * rd].reshape(B, rd, rd, H, W).contiguous()
+            (gv,) = droid_net_ext.corr_index_backward(vol, (cxy / 2**i).contiguous(), g, ctx.radius)
+            grads.append(gv)
+        return (None, None, *grads)
+
+
 class CorrBlock:
     """All-pairs correlation pyramid of two feature maps ``[B, N, C, H, W]``.
 
@@ -54,6 +81,15 @@ class CorrBlock:
         self.corr_pyramid = levels  # level i: [B*N, H, W, H/2^i, W/2^i]
 
     def __call__(self, coords):
+        """coords [B, N, H, W, 2] -> [B, N, L*(2r+1)^2, H, W] (droid_net.py:71-84), all
+        levels in one fused kernel reading the coordinates in place."""
+        batch, num, ht, wd, _ = coords.shape
+        c = coords.reshape(batch * num, ht, wd, 2).contiguous().float()
+        out = CorrPyramidSampler.apply(c, self.radius, *self.corr_pyramid)
+        return out.view(batch, num, -1, ht, wd)
+
+    def lookup_per_level(self, coords):
+        """The reference's per-level form: one corr_index_forward per level + concat."""
         out_pyramid = []
         batch, num, ht, wd, _ = coords.shape
         coords = coords.permute(0, 1, 4, 2, 3)

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void corr_index_fwd_kernel(const T* __restrict
     const float x0 = coords[(b * 2 + 0) * HW1 + yx];
     const float y0 = coords[(b * 2 + 1) * HW1 + yx];
     const float fx0 = floorf(x0), fy0 = floorf(y0);
-    const A dx = (A)(x0 - fx0), dy = (A)(y0 - fy0);
+    const float dx = x0 - fx0, dy = y0 - fy0;
     const int xb = (int)fx0 - R, yb = (int)fy0 - R;
     const T* plane = vol + pix * (int64_t)H2 * W2;
     A acc[RD][RD];
@@ -72,7 +72,9 @@ __global__ __launch_bounds__(256) void corr_index_fwd_kernel(const T* __restrict
     for (int i = 0; i < RD; i++)
 #pragma unroll
         for (int j = 0; j < RD; j++) acc[i][j] = 0;
-    const A w00 = (1 - dx) * (1 - dy), w10 = dx * (1 - dy), w01 = (1 - dx) * dy, w11 = dx * dy;
+    // bilinear weights are float products, as the reference forms them (scalar_t(dx * dy))
+    const A w00 = (A)((1.f - dx) * (1.f - dy)), w10 = (A)(dx * (1.f - dy)), w01 = (A)((1.f - dx) * dy),
+            w11 = (A)(dx * dy);
 #pragma unroll
     for (int j = 0; j < RD + 1; j++) {  // tap row (y offset)
         const int y1 = yb + j;
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(256) void corr_index_fwd_kernel_rt(const T* __restr
     const float x0 = coords[(b * 2 + 0) * HW1 + yx];
     const float y0 = coords[(b * 2 + 1) * HW1 + yx];
     const float fx0 = floorf(x0), fy0 = floorf(y0);
-    const A dx = (A)(x0 - fx0), dy = (A)(y0 - fy0);
+    const float dx = x0 - fx0, dy = y0 - fy0;
     const int xb = (int)fx0 - r, yb = (int)fy0 - r;
     const T* plane = vol + pix * (int64_t)H2 * W2;
     T* o = out + b * rd * rd * HW1 + yx;
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(256) void corr_index_fwd_kernel_rt(const T* __restr
                 for (int di = 0; di < 2; di++) {
                     const int x1 = xb + i + di, y1 = yb + j + dj;
                     if (x1 < 0 || x1 >= W2 || y1 < 0 || y1 >= H2) continue;
-                    const A w = (di ? dx : 1 - dx) * (dj ? dy : 1 - dy);
+                    const A w = (A)((di ? dx : 1.f - dx) * (dj ? dy : 1.f - dy));
                     v += w * load_as<A>(plane + (int64_t)y1 * W2 + x1);
                 }
             o[(i * rd + j) * HW1] = store_from<T>(v);
@@ -146,7 +148,9 @@ __global__ __launch_bounds__(256) void corr_index_bwd_kernel(const float* __rest
     const float x0 = coords[(b * 2 + 0) * HW1 + yx];
     const float y0 = coords[(b * 2 + 1) * HW1 + yx];
     const float fx0 = floorf(x0), fy0 = floorf(y0);
-    const A dx = (A)(x0 - fx0), dy = (A)(y0 - fy0);
+    const float dx = x0 - fx0, dy = y0 - fy0;
+    const A w11 = (A)(dx * dy), w10 = (A)(dx * (1.f - dy)), w01 = (A)((1.f - dx) * dy),
+            w00 = (A)((1.f - dx) * (1.f - dy));
     const int xb = (int)fx0 - r, yb = (int)fy0 - r;
     T* plane = vgrad + pix * (int64_t)H2 * W2;
     const T* g = gout + b * rd * rd * HW1 + yx;
@@ -157,10 +161,10 @@ __global__ __launch_bounds__(256) void corr_index_bwd_kernel(const float* __rest
             const int x1 = xb + i;
             if (x1 < 0 || x1 >= W2) continue;
             A s = 0;
-            if (i > 0 && j > 0) s += load_as<A>(g + ((i - 1) * rd + (j - 1)) * HW1) * dx * dy;
-            if (i > 0 && j < rd) s += load_as<A>(g + ((i - 1) * rd + j) * HW1) * dx * (1 - dy);
-            if (i < rd && j > 0) s += load_as<A>(g + (i * rd + (j - 1)) * HW1) * (1 - dx) * dy;
-            if (i < rd && j < rd) s += load_as<A>(g + (i * rd + j) * HW1) * (1 - dx) * (1 - dy);
+            if (i > 0 && j > 0) s += load_as<A>(g + ((i - 1) * rd + (j - 1)) * HW1) * w11;
+            if (i > 0 && j < rd) s += load_as<A>(g + ((i - 1) * rd + j) * HW1) * w10;
+            if (i < rd && j > 0) s += load_as<A>(g + (i * rd + (j - 1)) * HW1) * w01;
+            if (i < rd && j < rd) s += load_as<A>(g + (i * rd + j) * HW1) * w00;
             plane[(int64_t)y1 * W2 + x1] = store_from<T>(s);
         }
     }
@@ -190,6 +194,91 @@ int corr_index_bwd_launch(int64_t B, int H1, int W1, int H2, int W2, const float
     hipLaunchKernelGGL(corr_index_bwd_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, s, coords, (const T*)g, (T*)vg,
                        B, H1, W1, H2, W2, r);
     VH_LAUNCH_CHECK("corr_index_backward");
+    return VH_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Fused pyramid lookup (CorrBlock.__call__, droid_net.py:71-84): one lane per
+// source pixel handles every level, reading coords in [B,H,W,2] layout,
+// scaling them by 2^-l in-register and writing channel l*(2r+1)^2 + i*(2r+1)+j
+// of one [B, L*(2r+1)^2, H, W] output - no coordinate copies, no concat.
+// ---------------------------------------------------------------------------
+struct LevelPtrs {
+    const void* p[8];
+    int h[8];
+    int w[8];
+};
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, int L, const float* __restrict__ coords,
+                                                                  T* __restrict__ out, int64_t B, int H1, int W1) {
+    using A = typename Acc<T>::type;
+    constexpr int RD = 2 * R + 1;
+    const int64_t HW1 = (int64_t)H1 * W1;
+    const int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (pix >= B * HW1) return;
+    const int64_t b = pix / HW1;
+    const int64_t yx = pix - b * HW1;
+    const float cxf = coords[pix * 2 + 0];
+    const float cyf = coords[pix * 2 + 1];
+    T* o = out + b * (int64_t)L * RD * RD * HW1 + yx;
+    for (int l = 0; l < L; l++) {
+        const float sc = 1.0f / (float)(1 << l);
+        const float x0 = cxf * sc, y0 = cyf * sc;
+        const int H2 = lv.h[l], W2 = lv.w[l];
+        const T* plane = (const T*)lv.p[l] + pix * (int64_t)H2 * W2;
+        const float fx0 = floorf(x0), fy0 = floorf(y0);
+        const float dx = x0 - fx0, dy = y0 - fy0;
+        const int xb = (int)fx0 - R, yb = (int)fy0 - R;
+        const A w00 = (A)((1.f - dx) * (1.f - dy)), w10 = (A)(dx * (1.f - dy)), w01 = (A)((1.f - dx) * dy),
+                w11 = (A)(dx * dy);
+        A acc[RD][RD];
+#pragma unroll
+        for (int i = 0; i < RD; i++)
+#pragma unroll
+            for (int j = 0; j < RD; j++) acc[i][j] = 0;
+#pragma unroll
+        for (int j = 0; j < RD + 1; j++) {
+            const int y1 = yb + j;
+            const bool yok = (y1 >= 0) && (y1 < H2);
+            A row[RD + 1];
+#pragma unroll
+            for (int i = 0; i < RD + 1; i++) {
+                const int x1 = xb + i;
+                row[i] = (yok && x1 >= 0 && x1 < W2) ? load_as<A>(plane + (int64_t)y1 * W2 + x1) : A(0);
+            }
+#pragma unroll
+            for (int i = 0; i < RD + 1; i++) {
+                const A s = row[i];
+                if (i > 0 && j > 0) acc[i - 1][j - 1] += s * w11;
+                if (i > 0 && j < RD) acc[i - 1][j] += s * w10;
+                if (i < RD && j > 0) acc[i][j - 1] += s * w01;
+                if (i < RD && j < RD) acc[i][j] += s * w00;
+            }
+        }
+        T* ol = o + (int64_t)l * RD * RD * HW1;
+#pragma unroll
+        for (int i = 0; i < RD; i++)
+#pragma unroll
+            for (int j = 0; j < RD; j++) ol[(i * RD + j) * HW1] = store_from<T>(acc[i][j]);
+    }
+}
+
+template <typename T>
+int corr_lookup_pyramid_launch(int64_t B, int H1, int W1, int L, const void* const* levels, const int* Hl,
+                               const int* Wl, const float* coords, void* out, hipStream_t s) {
+    LevelPtrs lv;
+    for (int l = 0; l < L; l++) {
+        lv.p[l] = levels[l];
+        lv.h[l] = Hl[l];
+        lv.w[l] = Wl[l];
+    }
+    const int64_t n = B * H1 * (int64_t)W1;
+    if (n == 0) return VH_OK;
+    hipLaunchKernelGGL((corr_lookup_pyramid_kernel<T, 3>), dim3(ceil_div(n, 256)), dim3(256), 0, s, lv, L, coords,
+                       (T*)out, B, H1, W1);
+    VH_LAUNCH_CHECK("corr_lookup_pyramid");
     return VH_OK;
 }
 
@@ -527,6 +616,20 @@ int vh_corr_index_backward(int dtype, int64_t B, int H1, int W1, int H2, int W2,
         case VH_F32: return corr_index_bwd_launch<float>(B, H1, W1, H2, W2, coords, corr_grad, radius, volume_grad, s);
         case VH_F64: return corr_index_bwd_launch<double>(B, H1, W1, H2, W2, coords, corr_grad, radius, volume_grad, s);
         default: return fail(VH_ERR_UNSUPPORTED, "corr_index_backward: dtype %d", dtype);
+    }
+}
+
+int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels, const void* const* levels,
+                           const int* Hl, const int* Wl, const float* coords, int radius, void* out, void* stream) {
+    if (B < 0 || H1 < 0 || W1 < 0) return fail(VH_ERR_ARG, "corr_lookup_pyramid: negative size");
+    if (num_levels < 1 || num_levels > 8) return fail(VH_ERR_ARG, "corr_lookup_pyramid: 1..8 levels");
+    if (radius != 3) return fail(VH_ERR_UNSUPPORTED, "corr_lookup_pyramid: radius %d (built for r=3)", radius);
+    hipStream_t s = (hipStream_t)stream;
+    switch (dtype) {
+        case VH_F16: return corr_lookup_pyramid_launch<__half>(B, H1, W1, num_levels, levels, Hl, Wl, coords, out, s);
+        case VH_F32: return corr_lookup_pyramid_launch<float>(B, H1, W1, num_levels, levels, Hl, Wl, coords, out, s);
+        case VH_F64: return corr_lookup_pyramid_launch<double>(B, H1, W1, num_levels, levels, Hl, Wl, coords, out, s);
+        default: return fail(VH_ERR_UNSUPPORTED, "corr_lookup_pyramid: dtype %d", dtype);
     }
 }
 

@@ -130,6 +130,30 @@ def altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius):
     return [corr]
 
 
+def corr_lookup_pyramid(levels, coords, radius):
+    """Fused CorrBlock lookup: levels[l] [B,H1,W1,Hl,Wl], coords [B,H1,W1,2] (level-0
+    pixels) -> [B, L*(2r+1)^2, H1, W1] in the volume dtype."""
+    _contig(coords=coords)
+    for lvl in levels:
+        _contig(volume=lvl)
+    _gpu(coords, *levels)
+    if coords.dtype != torch.float32:
+        raise RuntimeError("coords must be float32")
+    B, H1, W1 = levels[0].shape[:3]
+    L = len(levels)
+    rd = 2 * int(radius) + 1
+    out = torch.empty((B, L * rd * rd, H1, W1), dtype=levels[0].dtype, device=coords.device)
+    lv = (C.c_void_p * L)(*[t.data_ptr() for t in levels])
+    Hl = (C.c_int * L)(*[t.shape[3] for t in levels])
+    Wl = (C.c_int * L)(*[t.shape[4] for t in levels])
+    with torch.cuda.device(coords.device):
+        rc = _lib.load().vh_corr_lookup_pyramid(_code(levels[0], _VOL_DT), B, H1, W1, L, lv, Hl, Wl,
+                                                _lib.ptr(coords), int(radius), _lib.ptr(out),
+                                                _lib.stream_of(coords))
+    _lib.check(rc, "corr_lookup_pyramid")
+    return out
+
+
 def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None):
     """All-pairs correlation volume pyramid of CorrBlock.  fmap1, fmap2 [E,C,H,W]
     (f32 / f16 / bf16) -> list of level-l volumes [E,H,W,H>>l,W>>l] in out_dtype

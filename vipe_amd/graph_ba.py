@@ -1,0 +1,133 @@
+"""Caller side of the fused BA: GraphBuffer._fused_ba / bundle_adjustment.
+
+Restates the argument preparation of vipe/slam/components/buffer.py:364-489
+(and the fused branch of :501-566) on top of ``slam_ext.ba_extended``:
+
+* rejects layouts the fused kernels do not cover with the reference's
+  messages (multi-view or non-pinhole buffers, rig rotation, empty window,
+  mismatched edge counts, non-default flow weights, robust kernels);
+* drops self edges ii == jj; merges sparse-track targets and weights into the
+  dense ones (weighted mean where the combined weight is positive);
+* ``[E, H*W, 2] -> [E, 2, H, W]``; eta = 0.2 * damping[kx] + 1e-7 with
+  kx = unique(cat(arange(t0,t1), ii)); pinhole intrinsics at 1/8 resolution
+  with intrinsics_scale = 1/8; depth_active = 1, or the [t0, t1) mask for
+  limited_disp;
+* writes optimised intrinsics back at full resolution and clamps disparities
+  at 0.001 afterwards.
+
+``buffer`` is any object with the GraphBuffer attributes used here:
+``poses [B,7]``, ``disps [B,V,h,w]``, ``disps_sens [B,V,h,w]``,
+``intrinsics [V,>=4]``, ``tstamp [B]``, ``rig [V,7]``, ``n_views``,
+``camera_type`` (``"pinhole"``), ``height``, ``width``, ``device``,
+``ba_config`` (mapping with ``dense_disp_alpha``, optional
+``intrinsics_damping_scale``, ``robust_kernel``) and ``sparse_tracks``
+(``None`` or an object with ``enabled`` and
+``compute_dense_disp_target_weight``).
+"""
+from __future__ import annotations
+
+import logging
+
+import torch
+from einops import rearrange
+
+from .ext import slam_ext
+
+logger = logging.getLogger(__name__)
+
+
+def _cfg(cfg, key, default=None):
+    if cfg is None:
+        return default
+    if isinstance(cfg, dict):
+        return cfg.get(key, default)
+    getter = getattr(cfg, "get", None)
+    if getter is not None:
+        return getter(key, default)
+    return getattr(cfg, key, default)
+
+
+def _fail(reason: str):
+    raise RuntimeError(f"Fused BA is enabled, but {reason}. Set ba.fused=false to use the generic BA solver.")
+
+
+def fused_ba(buffer, target, weight, disp_damping, ii, jj, t0, t1, n_iters, pose_damping, pose_ep, motion_only,
+             limited_disp, optimize_intrinsics, optimize_rig_rotation, weight_dense_disp=0.001,
+             weight_tracks=0.001, verbose=False):
+    """GraphBuffer._fused_ba (buffer.py:364-489).  Returns the per-iteration energy
+    (empty unless ``verbose``)."""
+    camera = str(getattr(buffer.camera_type, "name", buffer.camera_type)).lower()
+    if buffer.n_views != 1 or camera != "pinhole":
+        _fail("it only supports single-view pinhole-camera graph buffers")
+    if optimize_rig_rotation:
+        _fail("it does not support optimizing rig rotation")
+    if t0 >= t1:
+        _fail("the active optimization range is empty")
+    if target.shape[0] != ii.shape[0] or weight.shape[0] != ii.shape[0]:
+        _fail("target/weight edge counts do not match the graph edges")
+    if weight_dense_disp != 0.001 or weight_tracks != 0.001:
+        _fail("the configured dense-flow or sparse-track weights are unsupported")
+
+    edge_mask = ii != jj
+    if not torch.any(edge_mask):
+        _fail("the graph contains no non-self edges")
+    if not torch.all(edge_mask):
+        ii, jj = ii[edge_mask], jj[edge_mask]
+        target, weight = target[edge_mask], weight[edge_mask]
+
+    identity_rig = torch.as_tensor([0, 0, 0, 0, 0, 0, 1], dtype=buffer.rig.dtype, device=buffer.rig.device)
+    if not torch.allclose(buffer.rig[0], identity_rig):
+        _fail("it only supports identity rig extrinsics")
+
+    ht, wd = buffer.height // 8, buffer.width // 8
+    tracks = getattr(buffer, "sparse_tracks", None)
+    if tracks is not None and getattr(tracks, "enabled", False):
+        view_inds = torch.zeros_like(ii)
+        s_target, s_weight = tracks.compute_dense_disp_target_weight(
+            source_view_inds=view_inds, source_frame_inds=buffer.tstamp[ii], target_view_inds=view_inds,
+            target_frame_inds=buffer.tstamp[jj], image_size=(buffer.height, buffer.width), dense_disp_size=(ht, wd))
+        s_target = s_target.flatten(1, 2)
+        s_weight = s_weight.flatten(1, 2)
+        combined = weight + s_weight
+        target = torch.where(combined > 0, (weight * target + s_weight * s_target) / combined.clamp_min(1e-12), target)
+        weight = combined
+
+    target = rearrange(target, "k (h w) c -> k c h w", h=ht, w=wd, c=2).contiguous()
+    weight = rearrange(weight, "k (h w) c -> k c h w", h=ht, w=wd, c=2).contiguous()
+    device = buffer.poses.device
+    kx = torch.unique(torch.cat([torch.arange(t0, t1, device=device), ii]))
+    eta = (0.2 * disp_damping[kx] + 1e-7).contiguous()
+    intrinsics_scale = 1.0 / 8.0
+    intrinsics = (buffer.intrinsics[0, :4] * intrinsics_scale).contiguous()
+    depth_active = torch.ones(buffer.disps.shape[0], dtype=buffer.disps.dtype, device=device)
+    if limited_disp:
+        depth_active.zero_()
+        depth_active[t0:t1] = 1.0
+    damp_scale = float(_cfg(buffer.ba_config, "intrinsics_damping_scale", 1.0))
+    disps = buffer.disps[:, 0]
+    if not disps.is_contiguous():
+        raise RuntimeError("disps must be contiguous")
+    _, _, energy = slam_ext.ba_extended(
+        buffer.poses, disps, intrinsics, buffer.disps_sens[:, 0].contiguous(), target, weight, eta,
+        ii.contiguous(), jj.contiguous(), depth_active.contiguous(), t0, t1, n_iters, pose_damping, pose_ep,
+        motion_only, float(_cfg(buffer.ba_config, "dense_disp_alpha")), bool(optimize_intrinsics),
+        1e-6 * damp_scale, 1e-6 * damp_scale, intrinsics_scale, verbose)
+    if verbose:
+        logger.info(f"BA iters = {n_iters}, energy: {energy[0].item()} -> {energy[-1].item()}")
+    if optimize_intrinsics:
+        buffer.intrinsics[0, :4] = intrinsics / intrinsics_scale
+    return energy
+
+
+def bundle_adjustment(buffer, target, weight, disp_damping, ii, jj, t0, t1, n_iters, pose_damping, pose_ep,
+                      motion_only, limited_disp, optimize_intrinsics, optimize_rig_rotation, verbose=False):
+    """Fused branch of GraphBuffer.bundle_adjustment (buffer.py:501-566)."""
+    if t0 > t1:
+        raise ValueError("t0 must not exceed t1")
+    if _cfg(buffer.ba_config, "robust_kernel", None) is not None:
+        raise RuntimeError("Fused BA is enabled, but robust kernels are not supported. "
+                           "Set ba.fused=false to use the generic BA solver.")
+    energy = fused_ba(buffer, target, weight, disp_damping, ii, jj, t0, t1, n_iters, pose_damping, pose_ep,
+                      motion_only, limited_disp, optimize_intrinsics, optimize_rig_rotation, 0.001, 0.001, verbose)
+    buffer.disps.clamp_(min=0.001)
+    return energy
