@@ -151,3 +151,41 @@ def test_failed_factorisation_gives_zero_update():
     dx, dz, _ = slam_ext.ba(poses, disps, intr, ds, tg, wt, eta, ii, jj, 1, 4, 1, 0.0, -1.0, True)
     assert torch.count_nonzero(dx) == 0
     assert torch.equal(poses, p0)
+
+
+def test_motion_only_ignores_stale_workspace():
+    """Regression: motion-only BA must not read Schur terms left in a reused
+    workspace by an earlier full solve."""
+    from vipe_amd.ext import slam_ext
+
+    g = make_graph("kf8_e24_48x64", with_fmaps=False)
+    d = _cuda(g)
+    N = g.disps.shape[0]
+    da = torch.ones(N, device="cuda")
+    for it in range(3):
+        # full solve first (fills the Gram buffer), then motion-only on fresh copies
+        slam_ext.ba_extended(d["poses"].clone(), d["disps"].clone(), d["intrinsics"].clone(), d["disps_sens"],
+                             d["targets"], d["weights"], d["eta"], d["ii"], d["jj"], da, g.t0, g.t1, 1, 1e-4, 0.1,
+                             False, 0.05, False, 1e-6, 1e-6, 1.0)
+        p = d["poses"].clone()
+        slam_ext.ba_extended(p, d["disps"].clone(), d["intrinsics"].clone(), d["disps_sens"], d["targets"],
+                             d["weights"], d["eta"], d["ii"], d["jj"], da, g.t0, g.t1, 1, 1e-4, 0.1, True, 0.05,
+                             False, 1e-6, 1e-6, 1.0)
+        rp = g.poses.clone()
+        OB.ba_extended(rp, g.disps.clone(), g.intrinsics.clone(), g.disps_sens, g.targets, g.weights, g.eta, g.ii,
+                       g.jj, torch.ones(N), g.t0, g.t1, 1, 1e-4, 0.1, True, 0.05, False, 1e-6, 1e-6, 1.0)
+        assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
+
+
+def test_ba_full_size_fp32_meets_accuracy_targets():
+    """64 KF / 384 edges / 64x80, 2 GN iterations: FP32 device BA vs the FP64
+    oracle within 1e-4 px reprojection and 1e-5 relative pose error."""
+    g, mine, ref = _run_pair("kf64_e384_64x80", 2)
+    p, d, it, dx, dz, en = mine
+    rp, rd, rit, rdx, rdz, ren = ref
+    px = _reproj_err(g, p, d, rp, rd, g.intrinsics)
+    rel_pose = ((p.double() - rp.double()).norm(dim=1) / rp.double().norm(dim=1)).max().item()
+    print(f"[full-size BA] reprojection err {px:.3e} px, relative pose err {rel_pose:.3e}")
+    assert px < 1e-4
+    assert rel_pose < 1e-5
+    assert torch.allclose(en.double(), ren, rtol=1e-5)

@@ -208,21 +208,28 @@ __device__ __forceinline__ void neg_adjT(const float* R, const float* t, const f
     o[5] = -(R[2] * b0 + R[5] * b1 + R[8] * b2);
 }
 
-// Sum of 32 values over the 64 lanes of a wave with 32 exchanges: after the
-// halving steps lane l holds the partial of value (l & 31); a last step adds
-// the other half-wave.  Returns the total of value (lane & 31).
+// Sum of 32 values over the 64 lanes of a wave with 32 exchanges: each
+// halving step S keeps half of the live values (the half selected by lane bit
+// S) and adds the partner lane's other half, so after S = 16..1 lane l holds the
+// partial of value (l & 31); a last step adds the other half-wave.  The step
+// width is a template constant so every array index is static.
+template <int S>
+__device__ __forceinline__ void halve_step(float (&v)[32], bool hi) {
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        const float send = hi ? v[i] : v[i + S];
+        const float keep = hi ? v[i + S] : v[i];
+        v[i] = keep + __shfl_xor(send, S, 64);
+    }
+}
+
 __device__ __forceinline__ float wave_sum32(float (&v)[32]) {
     const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int s = 16; s >= 1; s >>= 1) {
-        const bool hi = (lane & s) != 0;
-#pragma unroll
-        for (int i = 0; i < s; i++) {
-            const float send = hi ? v[i] : v[i + s];
-            const float keep = hi ? v[i + s] : v[i];
-            v[i] = keep + __shfl_xor(send, s, 64);
-        }
-    }
+    halve_step<16>(v, (lane & 16) != 0);
+    halve_step<8>(v, (lane & 8) != 0);
+    halve_step<4>(v, (lane & 4) != 0);
+    halve_step<2>(v, (lane & 2) != 0);
+    halve_step<1>(v, (lane & 1) != 0);
     return v[0] + __shfl_xor(v[0], 32, 64);
 }
 
@@ -561,8 +568,10 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
                 case K_HJJ: s += w.ed[(int64_t)idx * kEDS + E_HJJ + i * 6 + j]; break;
                 case K_HIJ: s += w.ed[(int64_t)idx * kEDS + E_HIJ + i * 6 + j]; break;
                 case K_HIJT: s += w.ed[(int64_t)idx * kEDS + E_HIJ + j * 6 + i]; break;
-                case K_G: s -= w.gd[idx + i * 6 + j]; break;
-                case K_GT: s -= w.gd[idx + j * 6 + i]; break;
+                // Schur terms exist only when the depth system is solved; in motion-only
+                // mode the Gram buffer is never written.
+                case K_G: if (!motion) s -= w.gd[idx + i * 6 + j]; break;
+                case K_GT: if (!motion) s -= w.gd[idx + j * 6 + i]; break;
                 default: break;
             }
         }
@@ -593,22 +602,27 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
         return;
     }
     if (r == p.P) {
-        // intrinsics scalar, energy, padding, failure flag
+        // intrinsics scalar, energy, failure flag: sums split over the wave
+        double hff = 0.0, vf = 0.0, en = 0.0;
+        for (int e = tid; e < p.E; e += 64) {
+            hff += w.ed[(int64_t)e * kEDS + E_HFF];
+            vf += w.ed[(int64_t)e * kEDS + E_VF];
+            en += w.ed[(int64_t)e * kEDS + E_EN];
+        }
+        if (!motion) {
+            for (int k = tid; k < p.K; k += 64) {
+                const int po = p.pseudo_off[k];
+                hff -= w.gd[po + 1 * 6 + 1];
+                vf -= w.gd[po + 0 * 6 + 1];
+                en += w.fen[k];
+            }
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            hff += __shfl_xor(hff, o, 64);
+            vf += __shfl_xor(vf, o, 64);
+            en += __shfl_xor(en, o, 64);
+        }
         if (tid == 0) {
-            double hff = 0.0, vf = 0.0, en = 0.0;
-            for (int e = 0; e < p.E; e++) {
-                hff += w.ed[(int64_t)e * kEDS + E_HFF];
-                vf += w.ed[(int64_t)e * kEDS + E_VF];
-                en += w.ed[(int64_t)e * kEDS + E_EN];
-            }
-            if (!motion) {
-                for (int k = 0; k < p.K; k++) {
-                    const int po = p.pseudo_off[k];
-                    hff -= w.gd[po + 1 * 6 + 1];
-                    vf -= w.gd[po + 0 * 6 + 1];
-                    en += w.fen[k];
-                }
-            }
             if (p.intr) {
                 w.A[(int64_t)n6 * ld + n6] = hff + (double)a.iep + (double)a.ilm * hff;
                 w.b[n6] = vf;
@@ -629,47 +643,55 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
 // ---------------------------------------------------------------------------
 // 4. blocked FP64 Cholesky (lower), 64 x 64 blocks
 // ---------------------------------------------------------------------------
-// Factor the diagonal block kb in LDS (one wave, lane = row, left-looking);
-// workgroup r > 0 then solves its block row X L^T = A_{kb+r,kb}.
-// The factored diagonal block goes to Ldiag[kb] (not back into A): sibling
-// workgroups of the same launch are still reading the original A_kk.
-__global__ __launch_bounds__(64) void chol_panel_kernel(double* __restrict__ A, double* __restrict__ Ldiag, int ld,
-                                                        int kb, int* flag) {
+// Factor the diagonal block kb in LDS (right-looking, 256 lanes: lane & 63 is
+// the row, lane >> 6 picks every 4th column) and, in workgroup r > 0, solve
+// the block row X L^T = A_{kb+r,kb} the same way.  The factored diagonal block
+// goes to Ldiag[kb] (not back into A): sibling workgroups of the same launch are
+// still reading the original A_kk.
+__global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A, double* __restrict__ Ldiag, int ld,
+                                                         int kb, int* flag) {
     __shared__ double L[kCB][kCB + 1];
     __shared__ double X[kCB][kCB + 1];
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int r = tid & 63, cg = tid >> 6;
     const int c0 = kb * kCB;
-    for (int c = 0; c < kCB; c++) L[lane][c] = A[(int64_t)(c0 + lane) * ld + c0 + c];
-    __syncthreads();
+    for (int q = tid; q < kCB * kCB; q += 256) L[q / kCB][q % kCB] = A[(int64_t)(c0 + q / kCB) * ld + c0 + q % kCB];
     for (int j = 0; j < kCB; j++) {
-        double s = 0.0;
-        if (lane >= j) {
-            s = L[lane][j];
-            for (int m = 0; m < j; m++) s -= L[lane][m] * L[j][m];
+        __syncthreads();
+        const double d = L[j][j];
+        if (tid == 0 && d <= 0.0) atomicExch(flag, 1);
+        const double ljj = sqrt(d);
+        const double inv = 1.0 / ljj;
+        __syncthreads();
+        if (tid > j && tid < kCB) L[tid][j] *= inv;
+        if (tid == j) L[j][j] = ljj;
+        __syncthreads();
+        if (r > j) {
+            const double lrj = L[r][j];
+            for (int c = j + 1 + cg; c <= r; c += 4) L[r][c] -= lrj * L[c][j];
         }
-        const double dj = __shfl(s, j, 64);
-        if (lane == 0 && dj <= 0.0) atomicExch(flag, 1);
-        const double ljj = sqrt(dj);
-        __syncthreads();
-        if (lane > j) L[lane][j] = s / ljj;
-        if (lane == j) L[j][j] = ljj;
-        __syncthreads();
     }
+    __syncthreads();
     const int rb = blockIdx.x;
     if (rb == 0) {
         double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
-        for (int c = 0; c < kCB; c++) Lk[lane * kCB + c] = (c <= lane) ? L[lane][c] : 0.0;
+        for (int q = tid; q < kCB * kCB; q += 256) {
+            const int i = q / kCB, c = q % kCB;
+            Lk[q] = (c <= i) ? L[i][c] : 0.0;
+        }
         return;
     }
     const int r0 = c0 + rb * kCB;
-    for (int c = 0; c < kCB; c++) X[lane][c] = A[(int64_t)(r0 + lane) * ld + c0 + c];
-    __syncthreads();
+    for (int q = tid; q < kCB * kCB; q += 256) X[q / kCB][q % kCB] = A[(int64_t)(r0 + q / kCB) * ld + c0 + q % kCB];
     for (int j = 0; j < kCB; j++) {
-        double s = X[lane][j];
-        for (int m = 0; m < j; m++) s -= X[lane][m] * L[j][m];
-        X[lane][j] = s / L[j][j];
+        __syncthreads();
+        if (tid < kCB) X[tid][j] /= L[j][j];
+        __syncthreads();
+        const double xj = X[r][j];
+        for (int c = j + 1 + cg; c < kCB; c += 4) X[r][c] -= xj * L[c][j];
     }
-    for (int c = 0; c < kCB; c++) A[(int64_t)(r0 + lane) * ld + c0 + c] = X[lane][c];
+    __syncthreads();
+    for (int q = tid; q < kCB * kCB; q += 256) A[(int64_t)(r0 + q / kCB) * ld + c0 + q % kCB] = X[q / kCB][q % kCB];
 }
 
 // Trailing update A_ij -= L_ik L_jk^T for kb < j <= i.
@@ -723,49 +745,65 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
         }
 }
 
-// Forward and backward triangular solves with L (one wave), zeros when the
-// factorisation failed.  Diagonal blocks come from Ldiag, the rest from A.
-// Writes x (FP64) and dx [P,6] (f32).
+// Forward and backward triangular solves with L, zeros when the factorisation
+// failed.  Per 64-row block: the off-diagonal product is split over the four
+// waves (coalesced along the block's columns in the backward pass), then wave
+// 0 solves the diagonal block held in LDS.  Writes x (FP64) and dx [P,6] (f32).
 constexpr int kMaxLd = 2048;
-__global__ __launch_bounds__(64) void chol_solve_kernel(const double* __restrict__ A,
-                                                        const double* __restrict__ Ldiag, int ld, int nb,
-                                                        const double* __restrict__ b, double* __restrict__ x,
-                                                        const int* flag, int P, float* dx) {
+__global__ __launch_bounds__(256) void chol_solve_kernel(const double* __restrict__ A,
+                                                         const double* __restrict__ Ldiag, int ld, int nb,
+                                                         const double* __restrict__ b, double* __restrict__ x,
+                                                         const int* flag, int P, float* dx) {
     __shared__ double y[kMaxLd];
     __shared__ double xs[kMaxLd];
-    const int lane = threadIdx.x;
+    __shared__ double Lk[kCB][kCB + 1];
+    __shared__ double part[4][kCB];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, g = tid >> 6;
     const bool failed = flag[0] != 0;
     if (failed) {
-        for (int i = lane; i < ld; i += 64) xs[i] = 0.0;
+        for (int i = tid; i < ld; i += 256) xs[i] = 0.0;
     } else {
         for (int kb = 0; kb < nb; kb++) {
-            const double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
+            const double* Ld = Ldiag + (int64_t)kb * kCB * kCB;
+            for (int q = tid; q < kCB * kCB; q += 256) Lk[q / kCB][q % kCB] = Ld[q];
             const int r = kb * kCB + lane;
-            double s = b[r];
-            for (int c = 0; c < kb * kCB; c++) s -= A[(int64_t)r * ld + c] * y[c];
-            for (int j = 0; j < kCB; j++) {
-                const double yj = __shfl(s, j, 64) / Lk[j * kCB + j];
-                if (lane > j) s -= Lk[lane * kCB + j] * yj;
-                if (lane == j) y[r] = yj;
+            double s = 0.0;
+            for (int c = g; c < kb * kCB; c += 4) s += A[(int64_t)r * ld + c] * y[c];
+            part[g][lane] = s;
+            __syncthreads();
+            if (g == 0) {
+                double v = b[r] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+                for (int j = 0; j < kCB; j++) {
+                    const double yj = __shfl(v, j, 64) / Lk[j][j];
+                    if (lane > j) v -= Lk[lane][j] * yj;
+                    if (lane == j) y[r] = yj;
+                }
             }
             __syncthreads();
         }
         for (int kb = nb - 1; kb >= 0; kb--) {
-            const double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
+            const double* Ld = Ldiag + (int64_t)kb * kCB * kCB;
+            for (int q = tid; q < kCB * kCB; q += 256) Lk[q / kCB][q % kCB] = Ld[q];
             const int c = kb * kCB + lane;
-            double s = y[c];
-            for (int rr = (kb + 1) * kCB; rr < nb * kCB; rr++) s -= A[(int64_t)rr * ld + c] * xs[rr];
-            for (int j = kCB - 1; j >= 0; j--) {
-                const double xj = __shfl(s, j, 64) / Lk[j * kCB + j];
-                if (lane < j) s -= Lk[j * kCB + lane] * xj;
-                if (lane == j) xs[kb * kCB + j] = xj;
+            double s = 0.0;
+            for (int rr = (kb + 1) * kCB + g; rr < nb * kCB; rr += 4) s += A[(int64_t)rr * ld + c] * xs[rr];
+            part[g][lane] = s;
+            __syncthreads();
+            if (g == 0) {
+                double v = y[c] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+                for (int j = kCB - 1; j >= 0; j--) {
+                    const double xj = __shfl(v, j, 64) / Lk[j][j];
+                    if (lane < j) v -= Lk[j][lane] * xj;
+                    if (lane == j) xs[kb * kCB + j] = xj;
+                }
             }
             __syncthreads();
         }
     }
     __syncthreads();
-    for (int i = lane; i < ld; i += 64) x[i] = xs[i];
-    for (int i = lane; i < 6 * P; i += 64) dx[i] = (float)xs[i];
+    for (int i = tid; i < ld; i += 256) x[i] = xs[i];
+    for (int i = tid; i < 6 * P; i += 256) dx[i] = (float)xs[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1227,7 +1265,7 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
                            motion ? 1 : 0);
         VH_LAUNCH_CHECK("ba_assemble_kernel");
         for (int kb = 0; kb < d.nb; kb++) {
-            hipLaunchKernelGGL(chol_panel_kernel, dim3(d.nb - kb), dim3(64), 0, s, w.A, w.Ld, d.ld, kb, w.flag);
+            hipLaunchKernelGGL(chol_panel_kernel, dim3(d.nb - kb), dim3(256), 0, s, w.A, w.Ld, d.ld, kb, w.flag);
             VH_LAUNCH_CHECK("chol_panel_kernel");
             const int rem = d.nb - kb - 1;
             if (rem > 0) {
@@ -1236,7 +1274,7 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
                 VH_LAUNCH_CHECK("chol_update_kernel");
             }
         }
-        hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(64), 0, s, w.A, w.Ld, d.ld, d.nb, w.b, w.x, w.flag, d.P,
+        hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), 0, s, w.A, w.Ld, d.ld, d.nb, w.b, w.x, w.flag, d.P,
                            dx);
         VH_LAUNCH_CHECK("chol_solve_kernel");
         if (!motion && d.K > 0) {
