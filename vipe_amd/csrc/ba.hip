@@ -94,6 +94,7 @@ struct Work {
     double* b;         // [ld]
     double* x;         // [ld]
     double* Ld;        // [nb][64][64] factored diagonal blocks
+    double* Li;        // [nb][64][64] L_kk^-T of each diagonal block
     int* flag;         // [4]
 };
 
@@ -643,55 +644,76 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
 // ---------------------------------------------------------------------------
 // 4. blocked FP64 Cholesky (lower), 64 x 64 blocks
 // ---------------------------------------------------------------------------
-// Factor the diagonal block kb in LDS (right-looking, 256 lanes: lane & 63 is
-// the row, lane >> 6 picks every 4th column) and, in workgroup r > 0, solve
-// the block row X L^T = A_{kb+r,kb} the same way.  The factored diagonal block
-// goes to Ldiag[kb] (not back into A): sibling workgroups of the same launch are
-// still reading the original A_kk.
-__global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A, double* __restrict__ Ldiag, int ld,
-                                                         int kb, int* flag) {
-    __shared__ double L[kCB][kCB + 1];
-    __shared__ double X[kCB][kCB + 1];
-    const int tid = threadIdx.x;
-    const int r = tid & 63, cg = tid >> 6;
+// Panel step kb of the blocked Cholesky; one wave per workgroup, lane = row.
+// Every workgroup factors the diagonal block A_kk right-looking with its row
+// in registers (the pivot column is broadcast through LDS; entries above the
+// diagonal are left as scratch and never read).  Workgroup 0 stores L_kk in
+// Ldiag[kb] and X = L_kk^-T (the same triangular solve applied to the identity)
+// in LinvT[kb]; workgroup r > 0 solves its block row X L_kk^T = A_{kb+r,kb} in
+// place.  A_kk itself is never overwritten: sibling workgroups of the same
+// launch are still reading it.
+__global__ __launch_bounds__(64) void chol_panel_kernel(double* __restrict__ A, double* __restrict__ Ldiag,
+                                                        double* __restrict__ LinvT, int ld, int kb, int* flag) {
+    __shared__ double col[kCB];
+    __shared__ double LT[kCB][kCB];  // LT[j][c] = L[c][j]
+    __shared__ double invd[kCB];
+    const int lane = threadIdx.x;
     const int c0 = kb * kCB;
-    for (int q = tid; q < kCB * kCB; q += 256) L[q / kCB][q % kCB] = A[(int64_t)(c0 + q / kCB) * ld + c0 + q % kCB];
+    double row[kCB];
+    const double* arow = A + (int64_t)(c0 + lane) * ld + c0;
+#pragma unroll
+    for (int c = 0; c < kCB; c++) row[c] = arow[c];
+    bool bad = false;
+#pragma unroll
     for (int j = 0; j < kCB; j++) {
+        col[lane] = row[j];
         __syncthreads();
-        const double d = L[j][j];
-        if (tid == 0 && d <= 0.0) atomicExch(flag, 1);
-        const double ljj = sqrt(d);
-        const double inv = 1.0 / ljj;
+        const double d = col[j];
+        bad |= (d <= 0.0);  // a NaN pivot propagates, like the reference's LLT
+        const double inv = 1.0 / sqrt(d);
+        const double lrj = row[j] * inv;
+        row[j] = lrj;
+#pragma unroll
+        for (int c = j + 1; c < kCB; c++) row[c] -= lrj * (col[c] * inv);
         __syncthreads();
-        if (tid > j && tid < kCB) L[tid][j] *= inv;
-        if (tid == j) L[j][j] = ljj;
-        __syncthreads();
-        if (r > j) {
-            const double lrj = L[r][j];
-            for (int c = j + 1 + cg; c <= r; c += 4) L[r][c] -= lrj * L[c][j];
-        }
     }
+    if (lane == 0 && bad) atomicExch(flag, 1);
+    // transpose L into LDS for broadcast reads; upper entries are zero
+#pragma unroll
+    for (int c = 0; c < kCB; c++) LT[c][lane] = (c <= lane) ? row[c] : 0.0;
+    __syncthreads();
+    invd[lane] = 1.0 / LT[lane][lane];
     __syncthreads();
     const int rb = blockIdx.x;
+    // x L^T = a for one row per lane: right-looking over columns
+    double x[kCB];
     if (rb == 0) {
         double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
-        for (int q = tid; q < kCB * kCB; q += 256) {
-            const int i = q / kCB, c = q % kCB;
-            Lk[q] = (c <= i) ? L[i][c] : 0.0;
-        }
-        return;
+#pragma unroll
+        for (int c = 0; c < kCB; c++) Lk[lane * kCB + c] = LT[c][lane];
+#pragma unroll
+        for (int c = 0; c < kCB; c++) x[c] = (c == lane) ? 1.0 : 0.0;
+    } else {
+        const double* xr = A + (int64_t)(c0 + rb * kCB + lane) * ld + c0;
+#pragma unroll
+        for (int c = 0; c < kCB; c++) x[c] = xr[c];
     }
-    const int r0 = c0 + rb * kCB;
-    for (int q = tid; q < kCB * kCB; q += 256) X[q / kCB][q % kCB] = A[(int64_t)(r0 + q / kCB) * ld + c0 + q % kCB];
+#pragma unroll
     for (int j = 0; j < kCB; j++) {
-        __syncthreads();
-        if (tid < kCB) X[tid][j] /= L[j][j];
-        __syncthreads();
-        const double xj = X[r][j];
-        for (int c = j + 1 + cg; c < kCB; c += 4) X[r][c] -= xj * L[c][j];
+        const double xj = x[j] * invd[j];
+        x[j] = xj;
+#pragma unroll
+        for (int c = j + 1; c < kCB; c++) x[c] -= xj * LT[j][c];
     }
-    __syncthreads();
-    for (int q = tid; q < kCB * kCB; q += 256) A[(int64_t)(r0 + q / kCB) * ld + c0 + q % kCB] = X[q / kCB][q % kCB];
+    if (rb == 0) {
+        double* Xk = LinvT + (int64_t)kb * kCB * kCB;
+#pragma unroll
+        for (int c = 0; c < kCB; c++) Xk[lane * kCB + c] = x[c];
+    } else {
+        double* xr = A + (int64_t)(c0 + rb * kCB + lane) * ld + c0;
+#pragma unroll
+        for (int c = 0; c < kCB; c++) xr[c] = x[c];
+    }
 }
 
 // Trailing update A_ij -= L_ik L_jk^T for kb < j <= i.
@@ -746,58 +768,65 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
 }
 
 // Forward and backward triangular solves with L, zeros when the factorisation
-// failed.  Per 64-row block: the off-diagonal product is split over the four
-// waves (coalesced along the block's columns in the backward pass), then wave
-// 0 solves the diagonal block held in LDS.  Writes x (FP64) and dx [P,6] (f32).
+// failed.  With X_k = L_kk^-T from the panel step every block is two GEMVs:
+//   forward   t = b_k - L_k,<k y_<k ;  y_k = X_k^T t
+//   backward  t = y_k - L_>k,k^T x_>k ;  x_k = X_k t
+// The off-diagonal products read A coalesced (lanes along a row in the
+// forward pass, along the block's columns in the backward pass).
+// Writes x (FP64) and dx [P,6] (f32).
 constexpr int kMaxLd = 2048;
 __global__ __launch_bounds__(256) void chol_solve_kernel(const double* __restrict__ A,
-                                                         const double* __restrict__ Ldiag, int ld, int nb,
+                                                         const double* __restrict__ LinvT, int ld, int nb,
                                                          const double* __restrict__ b, double* __restrict__ x,
                                                          const int* flag, int P, float* dx) {
     __shared__ double y[kMaxLd];
     __shared__ double xs[kMaxLd];
-    __shared__ double Lk[kCB][kCB + 1];
+    __shared__ double Xk[kCB][kCB + 1];
+    __shared__ double t[kCB];
     __shared__ double part[4][kCB];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, g = tid >> 6;
+    const int lane = tid & 63, wv = tid >> 6;
     const bool failed = flag[0] != 0;
     if (failed) {
         for (int i = tid; i < ld; i += 256) xs[i] = 0.0;
     } else {
         for (int kb = 0; kb < nb; kb++) {
-            const double* Ld = Ldiag + (int64_t)kb * kCB * kCB;
-            for (int q = tid; q < kCB * kCB; q += 256) Lk[q / kCB][q % kCB] = Ld[q];
-            const int r = kb * kCB + lane;
-            double s = 0.0;
-            for (int c = g; c < kb * kCB; c += 4) s += A[(int64_t)r * ld + c] * y[c];
-            part[g][lane] = s;
-            __syncthreads();
-            if (g == 0) {
-                double v = b[r] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-                for (int j = 0; j < kCB; j++) {
-                    const double yj = __shfl(v, j, 64) / Lk[j][j];
-                    if (lane > j) v -= Lk[lane][j] * yj;
-                    if (lane == j) y[r] = yj;
-                }
+            const double* X = LinvT + (int64_t)kb * kCB * kCB;
+            for (int q = tid; q < kCB * kCB; q += 256) Xk[q / kCB][q % kCB] = X[q];
+            // t = b_k - L_k,<k y : wave wv takes rows wv, wv+4, ...; lanes along the row
+            for (int rr = wv; rr < kCB; rr += 4) {
+                const int r = kb * kCB + rr;
+                double s = 0.0;
+                for (int c = lane; c < kb * kCB; c += 64) s += A[(int64_t)r * ld + c] * y[c];
+                for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+                if (lane == 0) t[rr] = b[r] - s;
             }
+            __syncthreads();
+            // y_k = X^T t : y[i] = sum_c X[c][i] t[c], waves split c
+            double s = 0.0;
+            for (int c = wv; c < kCB; c += 4) s += Xk[c][lane] * t[c];
+            part[wv][lane] = s;
+            __syncthreads();
+            if (wv == 0) y[kb * kCB + lane] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
             __syncthreads();
         }
         for (int kb = nb - 1; kb >= 0; kb--) {
-            const double* Ld = Ldiag + (int64_t)kb * kCB * kCB;
-            for (int q = tid; q < kCB * kCB; q += 256) Lk[q / kCB][q % kCB] = Ld[q];
+            const double* X = LinvT + (int64_t)kb * kCB * kCB;
+            for (int q = tid; q < kCB * kCB; q += 256) Xk[q / kCB][q % kCB] = X[q];
+            // t = y_k - L_>k,k^T x : lanes along the block's columns, waves split rows
             const int c = kb * kCB + lane;
             double s = 0.0;
-            for (int rr = (kb + 1) * kCB + g; rr < nb * kCB; rr += 4) s += A[(int64_t)rr * ld + c] * xs[rr];
-            part[g][lane] = s;
+            for (int rr = (kb + 1) * kCB + wv; rr < nb * kCB; rr += 4) s += A[(int64_t)rr * ld + c] * xs[rr];
+            part[wv][lane] = s;
             __syncthreads();
-            if (g == 0) {
-                double v = y[c] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-                for (int j = kCB - 1; j >= 0; j--) {
-                    const double xj = __shfl(v, j, 64) / Lk[j][j];
-                    if (lane < j) v -= Lk[j][lane] * xj;
-                    if (lane == j) xs[kb * kCB + j] = xj;
-                }
-            }
+            if (wv == 0) t[lane] = y[c] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
+            __syncthreads();
+            // x_k = X t : x[i] = sum_c X[i][c] t[c], waves split c
+            double s2 = 0.0;
+            for (int cc = wv; cc < kCB; cc += 4) s2 += Xk[lane][cc] * t[cc];
+            part[wv][lane] = s2;
+            __syncthreads();
+            if (wv == 0) xs[kb * kCB + lane] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
             __syncthreads();
         }
     }
@@ -955,7 +984,7 @@ struct vh_ba_plan {
     int deg_max = 0;
     size_t ws_bytes = 0;
     // workspace offsets
-    size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_Ld, o_flag;
+    size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_Ld, o_Li, o_flag;
     // plan variants: the system size depends on whether intrinsics are optimised
     int P;
 };
@@ -1176,6 +1205,7 @@ vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, in
     pl->o_b = o; o += align256((size_t)d.ld * 8);
     pl->o_x = o; o += align256((size_t)d.ld * 8);
     pl->o_Ld = o; o += align256((size_t)d.ld * kCB * 8);
+    pl->o_Li = o; o += align256((size_t)d.ld * kCB * 8);
     pl->o_flag = o; o += 256;
     pl->ws_bytes = o;
     return pl;
@@ -1231,6 +1261,7 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
     w.b = (double*)(ws + plan->o_b);
     w.x = (double*)(ws + plan->o_x);
     w.Ld = (double*)(ws + plan->o_Ld);
+    w.Li = (double*)(ws + plan->o_Li);
     w.flag = (int*)(ws + plan->o_flag);
     RunArgs a;
     a.poses = poses; a.disps = disps; a.intrinsics = intrinsics; a.disps_sens = disps_sens;
@@ -1265,7 +1296,7 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
                            motion ? 1 : 0);
         VH_LAUNCH_CHECK("ba_assemble_kernel");
         for (int kb = 0; kb < d.nb; kb++) {
-            hipLaunchKernelGGL(chol_panel_kernel, dim3(d.nb - kb), dim3(256), 0, s, w.A, w.Ld, d.ld, kb, w.flag);
+            hipLaunchKernelGGL(chol_panel_kernel, dim3(d.nb - kb), dim3(64), 0, s, w.A, w.Ld, w.Li, d.ld, kb, w.flag);
             VH_LAUNCH_CHECK("chol_panel_kernel");
             const int rem = d.nb - kb - 1;
             if (rem > 0) {
@@ -1274,7 +1305,7 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
                 VH_LAUNCH_CHECK("chol_update_kernel");
             }
         }
-        hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), 0, s, w.A, w.Ld, d.ld, d.nb, w.b, w.x, w.flag, d.P,
+        hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.x, w.flag, d.P,
                            dx);
         VH_LAUNCH_CHECK("chol_solve_kernel");
         if (!motion && d.K > 0) {
