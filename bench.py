@@ -58,6 +58,23 @@ def ba_bytes(E, N, H, W):
     return E * 4 * H * W * 4 + N * H * W * 4 * 4
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 summaries
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration); None when no
+    summary covers it."""
+    import glob
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    for fn in sorted(glob.glob(os.path.join(here, "profiles", "*.md"))):
+        with open(fn) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_dispatch ") and ":" in line:
+                    name, val = line[len("traffic_bytes_per_dispatch "):].rsplit(":", 1)
+                    if name.strip() == kernel:
+                        return float(val)
+    return None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -130,6 +147,7 @@ def main():
     elem = 4 if vol_dtype == torch.float32 else 2
     lk_alg = lookup_bytes(E, H, W, 4, elem)
     lk_gbs = lk_alg / (lk_ms * 1e-3)
+    lk_kernel = f"corr_lookup_pyramid_kernel<{'float' if elem == 4 else '__half'}, 3>"
 
     # BA-only timing (same inputs, separate region)
     d["poses"].copy_(poses0)
@@ -171,13 +189,13 @@ def main():
         "corr_build_tflops": round(2.0 * E * (H * W) ** 2 * 128 / (build_ms * 1e-3) / 1e12, 2),
         "wall_s": round(wall, 3),
         "roofline": {
-            "kernel": "corr_index_fwd (4 pyramid levels of one lookup)",
+            "kernel": lk_kernel,
             "bound": "hbm",
             "achieved": round(lk_gbs / 1e9, 1),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(lk_gbs / HBM_PEAK, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(lk_kernel),
         },
     }
     if not a.no_cpu_baseline:

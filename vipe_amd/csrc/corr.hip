@@ -50,11 +50,52 @@ __device__ __forceinline__ double store_from<double, double>(double v) { return 
 // (2r+2)^2 taps of its own plane and writes its (2r+1)^2 outputs, channel
 // stride H1*W1 (coalesced across x).
 // ---------------------------------------------------------------------------
+// Output column j (y offset) needs only tap rows j and j+1, so each lane
+// streams its 2r+2 tap rows keeping two of them live and stores the 2r+1
+// outputs of a column as soon as both rows are in: ~40 live registers instead
+// of (2r+1)^2 accumulators, which keeps occupancy high for the scattered loads.
+template <typename T, typename A, int R>
+__device__ __forceinline__ void load_tap_row(const T* __restrict__ plane, int y1, int xb, int H2, int W2,
+                                             A (&row)[2 * R + 2]) {
+    const bool yok = (y1 >= 0) && (y1 < H2);
+    const T* rp = plane + (int64_t)(yok ? y1 : 0) * W2;
+#pragma unroll
+    for (int i = 0; i < 2 * R + 2; i++) {
+        const int x1 = xb + i;
+        row[i] = (yok && x1 >= 0 && x1 < W2) ? load_as<A>(rp + x1) : A(0);
+    }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void lookup_level(const T* __restrict__ plane, int H2, int W2, float x0, float y0,
+                                             T* __restrict__ o, int64_t ostride) {
+    using A = typename Acc<T>::type;
+    constexpr int RD = 2 * R + 1;
+    const float fx0 = floorf(x0), fy0 = floorf(y0);
+    const float dx = x0 - fx0, dy = y0 - fy0;
+    const int xb = (int)fx0 - R, yb = (int)fy0 - R;
+    // bilinear weights are float products, as the reference forms them (scalar_t(dx * dy))
+    const A w00 = (A)((1.f - dx) * (1.f - dy)), w10 = (A)(dx * (1.f - dy)), w01 = (A)((1.f - dx) * dy),
+            w11 = (A)(dx * dy);
+    A r0[RD + 1], r1[RD + 1];
+    load_tap_row<T, A, R>(plane, yb, xb, H2, W2, r0);
+#pragma unroll
+    for (int j = 0; j < RD; j++) {
+        A (&top)[RD + 1] = (j & 1) ? r1 : r0;
+        A (&bot)[RD + 1] = (j & 1) ? r0 : r1;
+        load_tap_row<T, A, R>(plane, yb + j + 1, xb, H2, W2, bot);
+#pragma unroll
+        for (int i = 0; i < RD; i++) {
+            const A v = top[i] * w00 + top[i + 1] * w10 + bot[i] * w01 + bot[i + 1] * w11;
+            o[(int64_t)(i * RD + j) * ostride] = store_from<T>(v);
+        }
+    }
+}
+
 template <typename T, int R>
 __global__ __launch_bounds__(256) void corr_index_fwd_kernel(const T* __restrict__ vol,
                                                              const float* __restrict__ coords, T* __restrict__ out,
                                                              int64_t B, int H1, int W1, int H2, int W2) {
-    using A = typename Acc<T>::type;
     constexpr int RD = 2 * R + 1;
     const int64_t HW1 = (int64_t)H1 * W1;
     const int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -63,38 +104,7 @@ __global__ __launch_bounds__(256) void corr_index_fwd_kernel(const T* __restrict
     const int64_t yx = pix - b * HW1;
     const float x0 = coords[(b * 2 + 0) * HW1 + yx];
     const float y0 = coords[(b * 2 + 1) * HW1 + yx];
-    const float fx0 = floorf(x0), fy0 = floorf(y0);
-    const float dx = x0 - fx0, dy = y0 - fy0;
-    const int xb = (int)fx0 - R, yb = (int)fy0 - R;
-    const T* plane = vol + pix * (int64_t)H2 * W2;
-    A acc[RD][RD];
-#pragma unroll
-    for (int i = 0; i < RD; i++)
-#pragma unroll
-        for (int j = 0; j < RD; j++) acc[i][j] = 0;
-    // bilinear weights are float products, as the reference forms them (scalar_t(dx * dy))
-    const A w00 = (A)((1.f - dx) * (1.f - dy)), w10 = (A)(dx * (1.f - dy)), w01 = (A)((1.f - dx) * dy),
-            w11 = (A)(dx * dy);
-#pragma unroll
-    for (int j = 0; j < RD + 1; j++) {  // tap row (y offset)
-        const int y1 = yb + j;
-        const bool yok = (y1 >= 0) && (y1 < H2);
-#pragma unroll
-        for (int i = 0; i < RD + 1; i++) {  // tap column (x offset)
-            const int x1 = xb + i;
-            A s = 0;
-            if (yok && x1 >= 0 && x1 < W2) s = load_as<A>(plane + (int64_t)y1 * W2 + x1);
-            if (i > 0 && j > 0) acc[i - 1][j - 1] += s * w11;
-            if (i > 0 && j < RD) acc[i - 1][j] += s * w10;
-            if (i < RD && j > 0) acc[i][j - 1] += s * w01;
-            if (i < RD && j < RD) acc[i][j] += s * w00;
-        }
-    }
-    T* o = out + b * RD * RD * HW1 + yx;
-#pragma unroll
-    for (int i = 0; i < RD; i++)
-#pragma unroll
-        for (int j = 0; j < RD; j++) o[(i * RD + j) * HW1] = store_from<T>(acc[i][j]);
+    lookup_level<T, R>(vol + pix * (int64_t)H2 * W2, H2, W2, x0, y0, out + b * RD * RD * HW1 + yx, HW1);
 }
 
 // Generic-radius variant (any r): same maths, runtime loops.
@@ -198,12 +208,10 @@ int corr_index_bwd_launch(int64_t B, int H1, int W1, int H2, int W2, const float
 }
 
 
-// ---------------------------------------------------------------------------
 // Fused pyramid lookup (CorrBlock.__call__, droid_net.py:71-84): one lane per
 // source pixel handles every level, reading coords in [B,H,W,2] layout,
 // scaling them by 2^-l in-register and writing channel l*(2r+1)^2 + i*(2r+1)+j
 // of one [B, L*(2r+1)^2, H, W] output - no coordinate copies, no concat.
-// ---------------------------------------------------------------------------
 struct LevelPtrs {
     const void* p[8];
     int h[8];
@@ -213,7 +221,6 @@ struct LevelPtrs {
 template <typename T, int R>
 __global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, int L, const float* __restrict__ coords,
                                                                   T* __restrict__ out, int64_t B, int H1, int W1) {
-    using A = typename Acc<T>::type;
     constexpr int RD = 2 * R + 1;
     const int64_t HW1 = (int64_t)H1 * W1;
     const int64_t pix = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -223,45 +230,12 @@ __global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, 
     const float cxf = coords[pix * 2 + 0];
     const float cyf = coords[pix * 2 + 1];
     T* o = out + b * (int64_t)L * RD * RD * HW1 + yx;
+#pragma unroll 1
     for (int l = 0; l < L; l++) {
         const float sc = 1.0f / (float)(1 << l);
-        const float x0 = cxf * sc, y0 = cyf * sc;
         const int H2 = lv.h[l], W2 = lv.w[l];
         const T* plane = (const T*)lv.p[l] + pix * (int64_t)H2 * W2;
-        const float fx0 = floorf(x0), fy0 = floorf(y0);
-        const float dx = x0 - fx0, dy = y0 - fy0;
-        const int xb = (int)fx0 - R, yb = (int)fy0 - R;
-        const A w00 = (A)((1.f - dx) * (1.f - dy)), w10 = (A)(dx * (1.f - dy)), w01 = (A)((1.f - dx) * dy),
-                w11 = (A)(dx * dy);
-        A acc[RD][RD];
-#pragma unroll
-        for (int i = 0; i < RD; i++)
-#pragma unroll
-            for (int j = 0; j < RD; j++) acc[i][j] = 0;
-#pragma unroll
-        for (int j = 0; j < RD + 1; j++) {
-            const int y1 = yb + j;
-            const bool yok = (y1 >= 0) && (y1 < H2);
-            A row[RD + 1];
-#pragma unroll
-            for (int i = 0; i < RD + 1; i++) {
-                const int x1 = xb + i;
-                row[i] = (yok && x1 >= 0 && x1 < W2) ? load_as<A>(plane + (int64_t)y1 * W2 + x1) : A(0);
-            }
-#pragma unroll
-            for (int i = 0; i < RD + 1; i++) {
-                const A s = row[i];
-                if (i > 0 && j > 0) acc[i - 1][j - 1] += s * w11;
-                if (i > 0 && j < RD) acc[i - 1][j] += s * w10;
-                if (i < RD && j > 0) acc[i][j - 1] += s * w01;
-                if (i < RD && j < RD) acc[i][j] += s * w00;
-            }
-        }
-        T* ol = o + (int64_t)l * RD * RD * HW1;
-#pragma unroll
-        for (int i = 0; i < RD; i++)
-#pragma unroll
-            for (int j = 0; j < RD; j++) ol[(i * RD + j) * HW1] = store_from<T>(acc[i][j]);
+        lookup_level<T, R>(plane, H2, W2, cxf * sc, cyf * sc, o + (int64_t)l * RD * RD * HW1, HW1);
     }
 }
 

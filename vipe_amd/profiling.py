@@ -1,0 +1,121 @@
+"""rocprofv3 helpers for the kernel summaries.
+
+    python3 -m vipe_amd.profiling copy [--mib 1024]
+        A plain device-to-device copy of known size (3 launches).  Profiled with
+        ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE``, it gives the factor between
+        the counters' KiB and real bytes on this GPU.
+
+    python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
+        --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
+        Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
+        a markdown summary: per-kernel calls / average / total time, and for the
+        named kernels the HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE,
+        each scaled by the copy calibration.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import sys
+from collections import defaultdict
+
+
+def _copy(mib: int) -> None:
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy: {n * 4} bytes read and {n * 4} bytes written per launch", flush=True)
+
+
+def _find(d: str, suffix: str) -> str:
+    hits = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not hits:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    return hits[0]
+
+
+def kernel_stats(d: str):
+    rows = []
+    with open(_find(d, "kernel_stats.csv")) as f:
+        for r in csv.DictReader(f):
+            rows.append((r["Name"], int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]),
+                         float(r["Percentage"])))
+    return rows
+
+
+def counter_per_kernel(d: str, counter: str):
+    """Mean counter value per dispatch, grouped by kernel name."""
+    acc = defaultdict(list)
+    with open(_find(d, "counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def _short(name: str, width: int = 90) -> str:
+    return name if len(name) <= width else name[: width - 3] + "..."
+
+
+def _summarize(a) -> None:
+    out = []
+    stats = kernel_stats(a.stats)
+    total = sum(r[2] for r in stats)
+    out.append("| kernel | calls | avg (us) | total (ms) | % |")
+    out.append("|---|---:|---:|---:|---:|")
+    for name, calls, tot, avg, pct in sorted(stats, key=lambda r: -r[2])[: a.top]:
+        out.append(f"| `{_short(name)}` | {calls} | {avg / 1e3:.2f} | {tot / 1e6:.3f} | {pct:.1f} |")
+    out.append(f"\nTotal kernel time: {total / 1e6:.3f} ms\n")
+    if a.fetch and a.write and a.cfetch and a.cwrite:
+        copy_bytes = a.copy_mib * 1024 * 1024
+        cf = counter_per_kernel(a.cfetch, "FETCH_SIZE")
+        cw = counter_per_kernel(a.cwrite, "WRITE_SIZE")
+        ck = max(cf, key=cf.get)
+        f_scale = copy_bytes / cf[ck]
+        w_scale = copy_bytes / cw[max(cw, key=cw.get)]
+        out.append(f"Copy calibration ({a.copy_mib} MiB copy, `{_short(ck, 60)}`): FETCH_SIZE reads "
+                   f"{cf[ck]:.0f}, WRITE_SIZE {cw[max(cw, key=cw.get)]:.0f} per dispatch; "
+                   f"bytes per counter unit: fetch {f_scale:.2f}, write {w_scale:.2f}.\n")
+        fe = counter_per_kernel(a.fetch, "FETCH_SIZE")
+        wr = counter_per_kernel(a.write, "WRITE_SIZE")
+        out.append("| kernel | FETCH_SIZE/dispatch | WRITE_SIZE/dispatch | HBM bytes/dispatch (scaled) |")
+        out.append("|---|---:|---:|---:|")
+        for k in sorted(fe, key=lambda k: -fe[k]):
+            if a.kernel and not any(s in k for s in a.kernel):
+                continue
+            byts = fe[k] * f_scale + wr.get(k, 0.0) * w_scale
+            out.append(f"| `{_short(k)}` | {fe[k]:.0f} | {wr.get(k, 0.0):.0f} | {byts:.4e} |")
+            print(f"traffic_bytes_per_dispatch[{k.split('(')[0]}] = {byts:.6e}", file=sys.stderr)
+    print("\n".join(out))
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("copy")
+    c.add_argument("--mib", type=int, default=1024)
+    s = sub.add_parser("summarize")
+    s.add_argument("--stats", required=True)
+    s.add_argument("--fetch")
+    s.add_argument("--write")
+    s.add_argument("--cfetch")
+    s.add_argument("--cwrite")
+    s.add_argument("--copy-mib", type=int, default=1024)
+    s.add_argument("--kernel", action="append", default=[])
+    s.add_argument("--top", type=int, default=25)
+    a = ap.parse_args(argv)
+    if a.cmd == "copy":
+        _copy(a.mib)
+    else:
+        _summarize(a)
+
+
+if __name__ == "__main__":
+    main()
