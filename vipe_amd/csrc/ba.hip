@@ -47,6 +47,7 @@ constexpr int kMaxRext = 22;  // self row + edges + pseudo row
 constexpr int kNV = 40;       // per-edge partial stride (floats)
 constexpr int kEDS = 136;     // per-edge finalised record (doubles)
 constexpr int kCB = 64;       // Cholesky block size
+constexpr int kMaxLd = 2048;  // leading dimension of the reduced system (6P+1 rounded up to kCB)
 constexpr int kLdsBudget = 56 * 1024;  // dynamic E-row tile; static LDS stays under 8 KB
 
 // per-edge partial layout (floats)
@@ -93,8 +94,8 @@ struct Work {
     double* A;         // [ld][ld]
     double* b;         // [ld]
     double* x;         // [ld]
-    double* Ld;        // [nb][64][64] factored diagonal blocks
     double* Li;        // [nb][64][64] L_kk^-T of each diagonal block
+    double* y;         // [ld] forward-substitution result
     int* flag;         // [4]
 };
 
@@ -642,197 +643,382 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
 }
 
 // ---------------------------------------------------------------------------
-// 4. blocked FP64 Cholesky (lower), 64 x 64 blocks
+// 4. blocked FP64 Cholesky (lower), 64 x 64 blocks, right-looking
 // ---------------------------------------------------------------------------
-// Panel step kb of the blocked Cholesky; one wave per workgroup, lane = row.
-// Every workgroup factors the diagonal block A_kk right-looking with its row
-// in registers (the pivot column is broadcast through LDS; entries above the
-// diagonal are left as scratch and never read).  Workgroup 0 stores L_kk in
-// Ldiag[kb] and X = L_kk^-T (the same triangular solve applied to the identity)
-// in LinvT[kb]; workgroup r > 0 solves its block row X L_kk^T = A_{kb+r,kb} in
-// place.  A_kk itself is never overwritten: sibling workgroups of the same
-// launch are still reading it.
-__global__ __launch_bounds__(64) void chol_panel_kernel(double* __restrict__ A, double* __restrict__ Ldiag,
-                                                        double* __restrict__ LinvT, int ld, int kb, int* flag) {
-    __shared__ double col[kCB];
-    __shared__ double LT[kCB][kCB];  // LT[j][c] = L[c][j]
-    __shared__ double invd[kCB];
-    const int lane = threadIdx.x;
-    const int c0 = kb * kCB;
-    double row[kCB];
-    const double* arow = A + (int64_t)(c0 + lane) * ld + c0;
+// Launches per solve: chol_factor_kernel for block 0, then chol_step_kernel for
+// kb = 0 .. nb-2, then chol_backsolve_kernel.  Step kb has one workgroup per
+// trailing lower block (i, j), kb < j <= i, which
+//   forms L_ik = A_ik X_k and L_jk = A_jk X_k        (X_k = L_kk^-T)
+//   updates A_ij -= L_ik L_jk^T
+// and, on the diagonal (i == j), stores L_ik^T in the unused upper block (kb, i)
+// and carries the right-hand side: y_k = X_k^T b'_k, b'_i -= L_ik y_k.  The
+// workgroup of block (kb+1, kb+1) then factors its freshly updated block, so
+// the next step's X is ready when the launch ends.  The forward substitution
+// is done when the factorisation is; chol_backsolve_kernel runs L^T x = y.
+// Entries above the diagonal of A are scratch: the assembly never writes a
+// result there, and only the stored L_ik^T blocks are read back.
+
+// 1/sqrt(d): hardware estimate and two Newton steps (within about 1 ulp for
+// normal positive d; zero, negative and NaN pivots are flagged by the caller).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
-    for (int c = 0; c < kCB; c++) row[c] = arow[c];
+    for (int it = 0; it < 2; it++) {
+        const double e = fma(-d * y, y, 1.0);  // 1 - d y^2
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+
+constexpr int kSB = 16;  // sub-block of the diagonal-block inverse
+
+struct FactorLds {
+    double col[2][kCB];
+    double invd[kCB];
+    double T[3][kSB][kSB + 1];
+};
+
+// Factor the 64 x 64 block whose lower triangle is in S (256 threads) and
+// write X = L^-T (row-major) to Xout.  Wave q keeps columns [16q, 16q+16) of
+// every row in registers (lane = row); at pivot j the owning wave publishes
+// column j through LDS (double-buffered, one barrier per pivot).  Then
+// M = L^-1 by 16 x 16 blocks: M_qq by substitution (wave q), and
+// M_ts = -M_tt sum_{q=s}^{t-1} L_tq M_qs by block diagonals.  S receives L and
+// M is scratch.
+__device__ __forceinline__ void factor_block(double (*S)[kCB + 1], double (*M)[kCB + 1], FactorLds& F, double* __restrict__ Xout,
+                             int* flag) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    double v[kSB];
+#pragma unroll
+    for (int m = 0; m < kSB; m++) {
+        const int c = kSB * wv + m;
+        v[m] = (c <= lane) ? S[lane][c] : 0.0;
+    }
     bool bad = false;
+#pragma unroll 1
+    for (int qj = 0; qj < kCB / kSB; qj++)
 #pragma unroll
-    for (int j = 0; j < kCB; j++) {
-        col[lane] = row[j];
+    for (int jj = 0; jj < kSB; jj++) {
+        const int j = qj * kSB + jj;
+        double* cb = F.col[jj & 1];
+        if (wv == qj) cb[lane] = v[jj];
         __syncthreads();
-        const double d = col[j];
+        double cv[kSB];
+#pragma unroll
+        for (int m = 0; m < kSB; m++) cv[m] = cb[kSB * wv + m];
+        const double d = cb[j];
+        const double arj = cb[lane];
         bad |= (d <= 0.0);  // a NaN pivot propagates, like the reference's LLT
-        const double inv = 1.0 / sqrt(d);
-        const double lrj = row[j] * inv;
-        row[j] = lrj;
+        const double inv = rsqrt_nr(d);
+        const double s = arj * (inv * inv);  // A'[r][j] / d
+        // A'[r][c] -= A'[r][j] A'[c][j] / d for the columns c > j this wave holds:
+        // all of them right of the pivot's wave, those after jj in the pivot's wave.
+        // Multiplying by zero elsewhere leaves the (finite) entries unchanged.
+        const double sa = (wv > qj) ? s : 0.0;
+        const double sb = (wv >= qj) ? s : 0.0;
 #pragma unroll
-        for (int c = j + 1; c < kCB; c++) row[c] -= lrj * (col[c] * inv);
+        for (int m = 0; m < kSB; m++) v[m] -= ((m > jj) ? sb : sa) * cv[m];
+        v[jj] *= (wv == qj) ? inv : 1.0;
+        if (tid == 0) F.invd[j] = inv;
+    }
+    if (tid == 0 && bad) atomicExch(flag, 1);
+#pragma unroll
+    for (int m = 0; m < kSB; m++) {
+        const int c = kSB * wv + m;
+        S[lane][c] = (c <= lane) ? v[m] : 0.0;
+    }
+    __syncthreads();
+    {
+        // wave q: row r of L_qq^-T solves x L_qq^T = e_r; M_qq = (L_qq^-T)^T
+        const int o = kSB * wv, r = lane & (kSB - 1);
+        double x[kSB];
+#pragma unroll
+        for (int c = 0; c < kSB; c++) x[c] = (c == r) ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = 0; j < kSB; j++) {
+            const double xj = x[j] * F.invd[o + j];
+            x[j] = xj;
+#pragma unroll
+            for (int c = j + 1; c < kSB; c++) x[c] -= xj * S[o + c][o + j];
+        }
+        if (lane < kSB) {
+#pragma unroll
+            for (int c = 0; c < kSB; c++) M[o + c][o + r] = x[c];
+        }
+    }
+    __syncthreads();
+    const int rr = tid >> 4, cc = tid & 15;
+#pragma unroll
+    for (int dd = 1; dd < 4; dd++) {
+#pragma unroll
+        for (int bq = 0; bq < 4 - dd; bq++) {
+            const int bi = bq + dd, bj = bq;
+            double acc = 0.0;
+#pragma unroll
+            for (int k = bj; k < bi; k++)
+#pragma unroll
+                for (int m = 0; m < kSB; m++) acc += S[bi * kSB + rr][k * kSB + m] * M[k * kSB + m][bj * kSB + cc];
+            F.T[bq][rr][cc] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int bq = 0; bq < 4 - dd; bq++) {
+            const int bi = bq + dd, bj = bq;
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < kSB; m++) acc += M[bi * kSB + rr][bi * kSB + m] * F.T[bq][m][cc];
+            M[bi * kSB + rr][bj * kSB + cc] = -acc;
+        }
         __syncthreads();
     }
-    if (lane == 0 && bad) atomicExch(flag, 1);
-    // transpose L into LDS for broadcast reads; upper entries are zero
 #pragma unroll
-    for (int c = 0; c < kCB; c++) LT[c][lane] = (c <= lane) ? row[c] : 0.0;
-    __syncthreads();
-    invd[lane] = 1.0 / LT[lane][lane];
-    __syncthreads();
-    const int rb = blockIdx.x;
-    // x L^T = a for one row per lane: right-looking over columns
-    double x[kCB];
-    if (rb == 0) {
-        double* Lk = Ldiag + (int64_t)kb * kCB * kCB;
-#pragma unroll
-        for (int c = 0; c < kCB; c++) Lk[lane * kCB + c] = LT[c][lane];
-#pragma unroll
-        for (int c = 0; c < kCB; c++) x[c] = (c == lane) ? 1.0 : 0.0;
-    } else {
-        const double* xr = A + (int64_t)(c0 + rb * kCB + lane) * ld + c0;
-#pragma unroll
-        for (int c = 0; c < kCB; c++) x[c] = xr[c];
-    }
-#pragma unroll
-    for (int j = 0; j < kCB; j++) {
-        const double xj = x[j] * invd[j];
-        x[j] = xj;
-#pragma unroll
-        for (int c = j + 1; c < kCB; c++) x[c] -= xj * LT[j][c];
-    }
-    if (rb == 0) {
-        double* Xk = LinvT + (int64_t)kb * kCB * kCB;
-#pragma unroll
-        for (int c = 0; c < kCB; c++) Xk[lane * kCB + c] = x[c];
-    } else {
-        double* xr = A + (int64_t)(c0 + rb * kCB + lane) * ld + c0;
-#pragma unroll
-        for (int c = 0; c < kCB; c++) xr[c] = x[c];
+    for (int q = 0; q < 16; q++) {
+        const int e = tid + q * 256;
+        const int r = e >> 6, c = e & 63;
+        Xout[e] = (c >= r) ? M[c][r] : 0.0;
     }
 }
 
-// Trailing update A_ij -= L_ik L_jk^T for kb < j <= i.
-__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int ld, int kb, int nb) {
-    __shared__ double Li[kCB][kCB + 1];
-    __shared__ double Lj[kCB][kCB + 1];
-    // decode lower-triangular trailing block index -> (bi, bj)
-    int idx = blockIdx.x;
-    int bi = kb + 1, bj = kb + 1;
-    {
-        int row = 0;
-        while (idx > row) {
-            idx -= row + 1;
-            row++;
-        }
-        bi = kb + 1 + row;
-        bj = kb + 1 + idx;
+// Load a 64 x 64 FP64 block (row stride ld) into LDS, all 16 loads per thread
+// in flight before the first store.  TRANS stores it transposed.
+template <bool TRANS>
+__device__ __forceinline__ void load_block(const double* __restrict__ src, int64_t ld, double (*dst)[kCB + 1],
+                                           int tid) {
+    double tmp[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = tid + q * 256;
+        tmp[q] = src[(int64_t)(e >> 6) * ld + (e & 63)];
     }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int e = tid + q * 256;
+        if (TRANS) dst[e & 63][e >> 6] = tmp[q];
+        else dst[e >> 6][e & 63] = tmp[q];
+    }
+}
+
+// 4 x 4 outputs per thread of C = P^T Q with P, Q in LDS: C[r][c] = sum_m P[m][r] Q[m][c]
+__device__ __forceinline__ void gemm_tn_4x4(const double (*P)[kCB + 1], const double (*Q)[kCB + 1], int tr, int tc,
+                                            double (&acc)[4][4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+#pragma unroll 8
+    for (int m = 0; m < kCB; m++) {
+        double a[4], q[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            a[i] = P[m][tr + i];
+            q[i] = Q[m][tc + i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] += a[i] * q[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void chol_factor_kernel(const double* __restrict__ A, double* __restrict__ LinvT,
+                                                          int ld, int kb, int* flag) {
+    __shared__ double S[kCB][kCB + 1];
+    __shared__ double M[kCB][kCB + 1];
+    __shared__ FactorLds F;
+    load_block<false>(A + (int64_t)kb * kCB * ld + kb * kCB, ld, S, threadIdx.x);
+    __syncthreads();
+    factor_block(S, M, F, LinvT + (int64_t)kb * kCB * kCB, flag);
+}
+
+__global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, double* __restrict__ LinvT,
+                                                        double* __restrict__ b, double* __restrict__ y, int ld, int kb,
+                                                        int* flag) {
+    __shared__ double R0[kCB][kCB + 1];  // X_k (R0[m][c] = X[m][c]); then the block to factor
+    __shared__ double R1[kCB][kCB + 1];  // A_ik^T, then L_ik; then factorisation scratch
+    __shared__ double R2[kCB][kCB + 1];  // A_jk^T, then L_jk
+    __shared__ FactorLds F;
+    __shared__ double yk[kCB];
+    int idx = blockIdx.x, row = 0;
+    while (idx > row) {
+        idx -= row + 1;
+        row++;
+    }
+    const int bi = kb + 1 + row, bj = kb + 1 + idx;
+    const bool diag = (bi == bj);
+    const bool next = diag && bi == kb + 1;
     const int tid = threadIdx.x;
     const int c0 = kb * kCB;
-    for (int q = tid; q < kCB * kCB; q += 256) {
-        const int r = q / kCB, c = q % kCB;
-        Li[r][c] = A[(int64_t)(bi * kCB + r) * ld + c0 + c];
-        Lj[r][c] = A[(int64_t)(bj * kCB + r) * ld + c0 + c];
+    load_block<false>(LinvT + (int64_t)kb * kCB * kCB, kCB, R0, tid);
+    load_block<true>(A + (int64_t)bi * kCB * ld + c0, ld, R1, tid);
+    if (!diag) load_block<true>(A + (int64_t)bj * kCB * ld + c0, ld, R2, tid);
+    const int tr = (tid >> 4) * 4, tc = (tid & 15) * 4;
+    double* Ab = A + (int64_t)(bi * kCB + tr) * ld + bj * kCB + tc;
+    double2 cur[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        cur[i][0] = *reinterpret_cast<const double2*>(Ab + (int64_t)i * ld);
+        cur[i][1] = *reinterpret_cast<const double2*>(Ab + (int64_t)i * ld + 2);
     }
     __syncthreads();
-    const int tr = (tid / 16) * 4, tc = (tid % 16) * 4;
+    double li[4][4], lj[4][4];
+    gemm_tn_4x4(R1, R0, tr, tc, li);
+    if (!diag) gemm_tn_4x4(R2, R0, tr, tc, lj);
+    if (diag && tid < kCB) {
+        double s = 0.0;
+#pragma unroll 8
+        for (int m = 0; m < kCB; m++) s += b[c0 + m] * R0[m][tid];
+        yk[tid] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            R1[tr + i][tc + j] = li[i][j];
+            if (!diag) R2[tr + i][tc + j] = lj[i][j];
+        }
+    if (diag) {
+        // L_ik^T into the upper block (kb, bi): rows of it are columns of L_ik
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double* u = A + (int64_t)(c0 + tc + j) * ld + bi * kCB + tr;
+            *reinterpret_cast<double2*>(u) = make_double2(li[0][j], li[1][j]);
+            *reinterpret_cast<double2*>(u + 2) = make_double2(li[2][j], li[3][j]);
+        }
+        if (next && tid < kCB) y[c0 + tid] = yk[tid];
+    }
+    __syncthreads();
+    double (*Lb)[kCB + 1] = diag ? R1 : R2;
     double acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+#pragma unroll 8
     for (int m = 0; m < kCB; m++) {
         double xi[4], xj[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            xi[i] = Li[tr + i][m];
-            xj[i] = Lj[tc + i][m];
+            xi[i] = R1[tr + i][m];
+            xj[i] = Lb[tc + i][m];
         }
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) acc[i][j] += xi[i] * xj[j];
     }
+    if (diag && tid < kCB) {
+        double s = 0.0;
+#pragma unroll 8
+        for (int m = 0; m < kCB; m++) s += R1[tid][m] * yk[m];
+        b[bi * kCB + tid] -= s;
+    }
+    if (!next) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            double* d = A + (int64_t)(bi * kCB + tr + i) * ld + bj * kCB + tc + j;
-            *d -= acc[i][j];
+        for (int i = 0; i < 4; i++) {
+            *reinterpret_cast<double2*>(Ab + (int64_t)i * ld) =
+                make_double2(cur[i][0].x - acc[i][0], cur[i][0].y - acc[i][1]);
+            *reinterpret_cast<double2*>(Ab + (int64_t)i * ld + 2) =
+                make_double2(cur[i][1].x - acc[i][2], cur[i][1].y - acc[i][3]);
         }
+        return;
+    }
+    // this workgroup owns the next diagonal block: factor it straight from registers
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        R0[tr + i][tc + 0] = cur[i][0].x - acc[i][0];
+        R0[tr + i][tc + 1] = cur[i][0].y - acc[i][1];
+        R0[tr + i][tc + 2] = cur[i][1].x - acc[i][2];
+        R0[tr + i][tc + 3] = cur[i][1].y - acc[i][3];
+    }
+    __syncthreads();
+    factor_block(R0, R1, F, LinvT + (int64_t)(kb + 1) * kCB * kCB, flag);
 }
 
-// Forward and backward triangular solves with L, zeros when the factorisation
-// failed.  With X_k = L_kk^-T from the panel step every block is two GEMVs:
-//   forward   t = b_k - L_k,<k y_<k ;  y_k = X_k^T t
-//   backward  t = y_k - L_>k,k^T x_>k ;  x_k = X_k t
-// The off-diagonal products read A coalesced (lanes along a row in the
-// forward pass, along the block's columns in the backward pass).
+// Backward substitution L^T x = y, zeros when the factorisation failed.
+//   last block:  y = X^T b'            x = X y
+//   block k:     t = y_k - U_k x_>k    x_k = X_k t
+// where row c of U_k is row kb*64+c of A right of the diagonal block (the
+// stored L_ik^T blocks).  16 waves; lanes run along rows (coalesced).
 // Writes x (FP64) and dx [P,6] (f32).
-constexpr int kMaxLd = 2048;
-__global__ __launch_bounds__(256) void chol_solve_kernel(const double* __restrict__ A,
-                                                         const double* __restrict__ LinvT, int ld, int nb,
-                                                         const double* __restrict__ b, double* __restrict__ x,
-                                                         const int* flag, int P, float* dx) {
-    __shared__ double y[kMaxLd];
+__global__ __launch_bounds__(1024) void chol_backsolve_kernel(const double* __restrict__ A,
+                                                              const double* __restrict__ LinvT, int ld, int nb,
+                                                              const double* __restrict__ b,
+                                                              const double* __restrict__ y, double* __restrict__ x,
+                                                              const int* flag, int P, float* dx) {
     __shared__ double xs[kMaxLd];
     __shared__ double Xk[kCB][kCB + 1];
     __shared__ double t[kCB];
-    __shared__ double part[4][kCB];
+    __shared__ double part[16][kCB];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
-    const bool failed = flag[0] != 0;
-    if (failed) {
-        for (int i = tid; i < ld; i += 256) xs[i] = 0.0;
-    } else {
-        for (int kb = 0; kb < nb; kb++) {
-            const double* X = LinvT + (int64_t)kb * kCB * kCB;
-            for (int q = tid; q < kCB * kCB; q += 256) Xk[q / kCB][q % kCB] = X[q];
-            // t = b_k - L_k,<k y : wave wv takes rows wv, wv+4, ...; lanes along the row
-            for (int rr = wv; rr < kCB; rr += 4) {
-                const int r = kb * kCB + rr;
-                double s = 0.0;
-                for (int c = lane; c < kb * kCB; c += 64) s += A[(int64_t)r * ld + c] * y[c];
-                for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-                if (lane == 0) t[rr] = b[r] - s;
-            }
-            __syncthreads();
-            // y_k = X^T t : y[i] = sum_c X[c][i] t[c], waves split c
-            double s = 0.0;
-            for (int c = wv; c < kCB; c += 4) s += Xk[c][lane] * t[c];
-            part[wv][lane] = s;
-            __syncthreads();
-            if (wv == 0) y[kb * kCB + lane] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-            __syncthreads();
-        }
-        for (int kb = nb - 1; kb >= 0; kb--) {
-            const double* X = LinvT + (int64_t)kb * kCB * kCB;
-            for (int q = tid; q < kCB * kCB; q += 256) Xk[q / kCB][q % kCB] = X[q];
-            // t = y_k - L_>k,k^T x : lanes along the block's columns, waves split rows
-            const int c = kb * kCB + lane;
-            double s = 0.0;
-            for (int rr = (kb + 1) * kCB + wv; rr < nb * kCB; rr += 4) s += A[(int64_t)rr * ld + c] * xs[rr];
-            part[wv][lane] = s;
-            __syncthreads();
-            if (wv == 0) t[lane] = y[c] - (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
-            __syncthreads();
-            // x_k = X t : x[i] = sum_c X[i][c] t[c], waves split c
-            double s2 = 0.0;
-            for (int cc = wv; cc < kCB; cc += 4) s2 += Xk[lane][cc] * t[cc];
-            part[wv][lane] = s2;
-            __syncthreads();
-            if (wv == 0) xs[kb * kCB + lane] = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-            __syncthreads();
-        }
+    if (flag[0] != 0) {
+        for (int i = tid; i < ld; i += 1024) x[i] = 0.0;
+        for (int i = tid; i < 6 * P; i += 1024) dx[i] = 0.f;
+        return;
     }
-    __syncthreads();
-    for (int i = tid; i < ld; i += 256) x[i] = xs[i];
-    for (int i = tid; i < 6 * P; i += 256) dx[i] = (float)xs[i];
+    for (int kb = nb - 1; kb >= 0; kb--) {
+        const int c0 = kb * kCB;
+        const double* X = LinvT + (int64_t)kb * kCB * kCB;
+        double xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) xv[q] = X[tid + q * 1024];
+        if (kb == nb - 1) {
+            // y_k = X^T b': wave wv takes m in [4 wv, 4 wv + 4)
+            double s = 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) s += b[c0 + 4 * wv + u] * X[(4 * wv + u) * kCB + lane];
+            part[wv][lane] = s;
+        } else {
+            // row c = wv + 16 u of U_k, lanes along its nr columns
+            const int r0 = c0 + kCB, nr = (nb - kb - 1) * kCB;
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int q = lane; q < nr; q += 64) {
+                const double xq = xs[r0 + q];
+#pragma unroll
+                for (int u = 0; u < 4; u++) s[u] += A[(int64_t)(c0 + wv + 16 * u) * ld + r0 + q] * xq;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                double a = s[u];
+                for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+                s[u] = a;
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) t[wv + 16 * u] = y[c0 + wv + 16 * u] - s[u];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * 1024;
+            Xk[e >> 6][e & 63] = xv[q];
+        }
+        __syncthreads();
+        if (kb == nb - 1 && wv == 0) {
+            double a = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; u++) a += part[u][lane];
+            t[lane] = a;
+        }
+        __syncthreads();
+        // x_k[r] = sum_c X[r][c] t[c]: wave wv takes c in [4 wv, 4 wv + 4)
+        double s2 = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) s2 += Xk[lane][4 * wv + u] * t[4 * wv + u];
+        part[wv][lane] = s2;
+        __syncthreads();
+        if (wv == 0) {
+            double a = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; u++) a += part[u][lane];
+            xs[c0 + lane] = a;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < ld; i += 1024) x[i] = xs[i];
+    for (int i = tid; i < 6 * P; i += 1024) dx[i] = (float)xs[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -984,7 +1170,7 @@ struct vh_ba_plan {
     int deg_max = 0;
     size_t ws_bytes = 0;
     // workspace offsets
-    size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_Ld, o_Li, o_flag;
+    size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_y, o_Li, o_flag;
     // plan variants: the system size depends on whether intrinsics are optimised
     int P;
 };
@@ -1204,8 +1390,8 @@ vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, in
     pl->o_A = o; o += align256((size_t)d.ld * d.ld * 8);
     pl->o_b = o; o += align256((size_t)d.ld * 8);
     pl->o_x = o; o += align256((size_t)d.ld * 8);
-    pl->o_Ld = o; o += align256((size_t)d.ld * kCB * 8);
     pl->o_Li = o; o += align256((size_t)d.ld * kCB * 8);
+    pl->o_y = o; o += align256((size_t)d.ld * 8);
     pl->o_flag = o; o += 256;
     pl->ws_bytes = o;
     return pl;
@@ -1260,8 +1446,8 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
     w.A = (double*)(ws + plan->o_A);
     w.b = (double*)(ws + plan->o_b);
     w.x = (double*)(ws + plan->o_x);
-    w.Ld = (double*)(ws + plan->o_Ld);
     w.Li = (double*)(ws + plan->o_Li);
+    w.y = (double*)(ws + plan->o_y);
     w.flag = (int*)(ws + plan->o_flag);
     RunArgs a;
     a.poses = poses; a.disps = disps; a.intrinsics = intrinsics; a.disps_sens = disps_sens;
@@ -1295,19 +1481,17 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
         hipLaunchKernelGGL(ba_assemble_kernel, dim3(d.nlb + d.P + 1 + npad), dim3(64), 0, s, d, a, w,
                            motion ? 1 : 0);
         VH_LAUNCH_CHECK("ba_assemble_kernel");
-        for (int kb = 0; kb < d.nb; kb++) {
-            hipLaunchKernelGGL(chol_panel_kernel, dim3(d.nb - kb), dim3(64), 0, s, w.A, w.Ld, w.Li, d.ld, kb, w.flag);
-            VH_LAUNCH_CHECK("chol_panel_kernel");
+        hipLaunchKernelGGL(chol_factor_kernel, dim3(1), dim3(256), 0, s, w.A, w.Li, d.ld, 0, w.flag);
+        VH_LAUNCH_CHECK("chol_factor_kernel");
+        for (int kb = 0; kb + 1 < d.nb; kb++) {
             const int rem = d.nb - kb - 1;
-            if (rem > 0) {
-                hipLaunchKernelGGL(chol_update_kernel, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, w.A, d.ld, kb,
-                                   d.nb);
-                VH_LAUNCH_CHECK("chol_update_kernel");
-            }
+            hipLaunchKernelGGL(chol_step_kernel, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, w.A, w.Li, w.b, w.y, d.ld,
+                               kb, w.flag);
+            VH_LAUNCH_CHECK("chol_step_kernel");
         }
-        hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(256), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.x, w.flag, d.P,
-                           dx);
-        VH_LAUNCH_CHECK("chol_solve_kernel");
+        hipLaunchKernelGGL(chol_backsolve_kernel, dim3(1), dim3(1024), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.y, w.x,
+                           w.flag, d.P, dx);
+        VH_LAUNCH_CHECK("chol_backsolve_kernel");
         if (!motion && d.K > 0) {
             const dim3 bgrid((d.HW + kThreads - 1) / kThreads, d.K);
             if (intr) hipLaunchKernelGGL(ba_backsub_kernel<true>, bgrid, dim3(kThreads), 0, s, d, a, w);
