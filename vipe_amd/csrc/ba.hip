@@ -658,8 +658,9 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
 // Entries above the diagonal of A are scratch: the assembly never writes a
 // result there, and only the stored L_ik^T blocks are read back.
 
-// 1/sqrt(d): hardware estimate and two Newton steps (within about 1 ulp for
-// normal positive d; zero, negative and NaN pivots are flagged by the caller).
+// 1/sqrt(d): hardware estimate (about 2^-24) and two Newton steps (within
+// about 1 ulp for normal positive d; zero, negative and NaN pivots are flagged
+// by the caller).
 __device__ __forceinline__ double rsqrt_nr(double d) {
     double y = __builtin_amdgcn_rsq(d);
 #pragma unroll
@@ -678,69 +679,104 @@ struct FactorLds {
     double T[3][kSB][kSB + 1];
 };
 
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Factor the 64 x 64 block whose lower triangle is in S (256 threads) and
-// write X = L^-T (row-major) to Xout.  Wave q keeps columns [16q, 16q+16) of
-// every row in registers (lane = row); at pivot j the owning wave publishes
-// column j through LDS (double-buffered, one barrier per pivot).  Then
-// M = L^-1 by 16 x 16 blocks: M_qq by substitution (wave q), and
-// M_ts = -M_tt sum_{q=s}^{t-1} L_tq M_qs by block diagonals.  S receives L and
-// M is scratch.
-__device__ __forceinline__ void factor_block(double (*S)[kCB + 1], double (*M)[kCB + 1], FactorLds& F, double* __restrict__ Xout,
-                             int* flag) {
+// write X = L^-T (row-major) to Xout.  S receives L; M is scratch.
+//   per 16-column panel s: wave 0 factors rows o..63 of the panel with row r in
+//     lane r.  The pivot chain never leaves registers: the next pivot is lane
+//     (o+j+1)'s own entry after the step's update, read with readlane; the
+//     other columns of the step are broadcast through LDS, which a single
+//     wavefront reads back in program order without a barrier.  Then all
+//     waves apply the trailing update A_tt -= L_ts L_ts^T with 4 x 4 tiles.
+//   M = L^-1 by 16 x 16 blocks: M_qq by substitution (wave q), then
+//     M_ts = -M_tt sum_{q=s}^{t-1} L_tq M_qs by block diagonals.
+__device__ __forceinline__ void factor_block(double (*S)[kCB + 1], double (*M)[kCB + 1], FactorLds& F,
+                                             double* __restrict__ Xout, int* flag) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    double v[kSB];
-#pragma unroll
-    for (int m = 0; m < kSB; m++) {
-        const int c = kSB * wv + m;
-        v[m] = (c <= lane) ? S[lane][c] : 0.0;
-    }
     bool bad = false;
-#pragma unroll 1
-    for (int qj = 0; qj < kCB / kSB; qj++)
 #pragma unroll
-    for (int jj = 0; jj < kSB; jj++) {
-        const int j = qj * kSB + jj;
-        double* cb = F.col[jj & 1];
-        if (wv == qj) cb[lane] = v[jj];
+    for (int sb = 0; sb < kCB / kSB; sb++) {
+        const int o = sb * kSB;
+        if (wv == 0) {
+            double v[kSB];
+#pragma unroll
+            for (int m = 0; m < kSB; m++) v[m] = (lane >= o + m) ? S[lane][o + m] : 0.0;
+            double dn = readlane_f64(v[0], o);
+#pragma unroll
+            for (int j = 0; j < kSB; j++) {
+                double* cb = F.col[j & 1];
+                cb[lane] = v[j];  // column o+j below the pivot (rows above it are ignored)
+                const double d = dn;
+                bad |= (d <= 0.0);  // a NaN pivot propagates, like the reference's LLT
+                const double inv = rsqrt_nr(d);
+                const double s = v[j] * (inv * inv);  // A'[r][o+j] / d
+                // next pivot: row o+j+1 updates its own diagonal with its own entry
+                if (j + 1 < kSB) dn = readlane_f64(v[j + 1] - s * v[j], o + j + 1);
+#pragma unroll
+                for (int c = j + 1; c < kSB; c++) v[c] -= s * cb[o + c];
+                v[j] *= inv;
+                if (lane == 0) F.invd[o + j] = inv;
+            }
+#pragma unroll
+            for (int m = 0; m < kSB; m++)
+                if (lane >= o + m) S[lane][o + m] = v[m];
+        }
         __syncthreads();
-        double cv[kSB];
+        const int t0 = o + kSB, nt4 = (kCB - t0) / 4;
+        if (nt4 > 0 && tid < nt4 * nt4) {
+            const int tr = t0 + 4 * (tid / nt4), tc = t0 + 4 * (tid % nt4);
+            if (tc <= tr) {
+                double acc[4][4];
 #pragma unroll
-        for (int m = 0; m < kSB; m++) cv[m] = cb[kSB * wv + m];
-        const double d = cb[j];
-        const double arj = cb[lane];
-        bad |= (d <= 0.0);  // a NaN pivot propagates, like the reference's LLT
-        const double inv = rsqrt_nr(d);
-        const double s = arj * (inv * inv);  // A'[r][j] / d
-        // A'[r][c] -= A'[r][j] A'[c][j] / d for the columns c > j this wave holds:
-        // all of them right of the pivot's wave, those after jj in the pivot's wave.
-        // Multiplying by zero elsewhere leaves the (finite) entries unchanged.
-        const double sa = (wv > qj) ? s : 0.0;
-        const double sb = (wv >= qj) ? s : 0.0;
+                for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int m = 0; m < kSB; m++) v[m] -= ((m > jj) ? sb : sa) * cv[m];
-        v[jj] *= (wv == qj) ? inv : 1.0;
-        if (tid == 0) F.invd[j] = inv;
+                    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+#pragma unroll
+                for (int m = 0; m < kSB; m++) {
+                    double a[4], bb[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        a[i] = S[tr + i][o + m];
+                        bb[i] = S[tc + i][o + m];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * bb[j];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) S[tr + i][tc + j] -= acc[i][j];
+            }
+        }
+        __syncthreads();
     }
     if (tid == 0 && bad) atomicExch(flag, 1);
-#pragma unroll
-    for (int m = 0; m < kSB; m++) {
-        const int c = kSB * wv + m;
-        S[lane][c] = (c <= lane) ? v[m] : 0.0;
-    }
-    __syncthreads();
     {
         // wave q: row r of L_qq^-T solves x L_qq^T = e_r; M_qq = (L_qq^-T)^T
         const int o = kSB * wv, r = lane & (kSB - 1);
+        double lq[kSB][kSB];  // L_qq[c][j], broadcast reads
         double x[kSB];
 #pragma unroll
-        for (int c = 0; c < kSB; c++) x[c] = (c == r) ? 1.0 : 0.0;
+        for (int c = 0; c < kSB; c++) {
+            x[c] = (c == r) ? 1.0 : 0.0;
+#pragma unroll
+            for (int j = 0; j < c; j++) lq[c][j] = S[o + c][o + j];
+        }
 #pragma unroll
         for (int j = 0; j < kSB; j++) {
             const double xj = x[j] * F.invd[o + j];
             x[j] = xj;
 #pragma unroll
-            for (int c = j + 1; c < kSB; c++) x[c] -= xj * S[o + c][o + j];
+            for (int c = j + 1; c < kSB; c++) x[c] -= xj * lq[c][j];
         }
         if (lane < kSB) {
 #pragma unroll
@@ -754,21 +790,22 @@ __device__ __forceinline__ void factor_block(double (*S)[kCB + 1], double (*M)[k
 #pragma unroll
         for (int bq = 0; bq < 4 - dd; bq++) {
             const int bi = bq + dd, bj = bq;
-            double acc = 0.0;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int k = bj; k < bi; k++)
 #pragma unroll
-                for (int m = 0; m < kSB; m++) acc += S[bi * kSB + rr][k * kSB + m] * M[k * kSB + m][bj * kSB + cc];
-            F.T[bq][rr][cc] = acc;
+                for (int m = 0; m < kSB; m++)
+                    acc[m & 3] += S[bi * kSB + rr][k * kSB + m] * M[k * kSB + m][bj * kSB + cc];
+            F.T[bq][rr][cc] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         __syncthreads();
 #pragma unroll
         for (int bq = 0; bq < 4 - dd; bq++) {
             const int bi = bq + dd, bj = bq;
-            double acc = 0.0;
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int m = 0; m < kSB; m++) acc += M[bi * kSB + rr][bi * kSB + m] * F.T[bq][m][cc];
-            M[bi * kSB + rr][bj * kSB + cc] = -acc;
+            for (int m = 0; m < kSB; m++) acc[m & 3] += M[bi * kSB + rr][bi * kSB + m] * F.T[bq][m][cc];
+            M[bi * kSB + rr][bj * kSB + cc] = -((acc[0] + acc[1]) + (acc[2] + acc[3]));
         }
         __syncthreads();
     }
