@@ -97,21 +97,20 @@ int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels,
                            const int* Hl, const int* Wl, const float* coords, int radius, void* out, void* stream);
 
 /* All-pairs correlation volume and its avg-pool pyramid: the CorrBlock
- * constructor (vipe/slam/networks/droid_net.py:56-69, 94-102).
- * fmap1, fmap2 [E,C,H,W] in `in_dtype` (F32, F16, BF16); levels[l] is the
- * level-l output [E,H,W,H>>l,W>>l] in `out_dtype` (F32 or F16); values are
- * (f1/4).(f2/4) summed over C (FP32 accumulation), level l>0 = 2x2 mean of
- * level l-1 (floor of odd sizes). */
-int vh_corr_build(int in_dtype, int out_dtype, int64_t E, int C, int H, int W, const void* fmap1,
-                  const void* fmap2, int num_levels, void* const* levels, void* stream);
-
-/* Mixed-precision build: fmap1/fmap2 f32 [E,C,H,W] are quantised per
- * (edge, channel) to FP8 e4m3 (scale = amax/448) and contracted on MFMA with
- * FP32 accumulation; output as vh_corr_build.  Needs a workspace of
- * vh_corr_build_fp8_workspace(E,C,H,W) bytes. */
-size_t vh_corr_build_fp8_workspace(int64_t E, int C, int H, int W);
-int vh_corr_build_fp8(int out_dtype, int64_t E, int C, int H, int W, const float* fmap1, const float* fmap2,
-                      int num_levels, void* const* levels, void* workspace, void* stream);
+ * constructor (vipe/slam/networks/droid_net.py:56-69, 94-102), which the
+ * reference forms with torch.matmul + F.avg_pool2d (no native counterpart).
+ * fmap1, fmap2 [E,C,H,W] in `in_dtype` (F32, F16, BF16), C <= 448; levels[l]
+ * is the level-l output [E,H,W,H>>l,W>>l] in `out_dtype` (F32 or F16).
+ * Values are (f1/4).(f2/4) summed over C with FP32 accumulation; level l > 0
+ * is the 2x2 mean of level l-1 as stored (floor of odd sizes).
+ * `mma_dtype` selects the contraction: F32, F16, BF16, or FP8E4M3 (each
+ * channel balanced between the two maps by sqrt(amax1/amax2), then one e4m3
+ * scale per pixel row; the scales are applied to the FP32 accumulator).
+ * The caller provides vh_corr_build_workspace(mma_dtype, E, C, H, W) bytes of
+ * device memory (channels-last operand copies and scales). */
+size_t vh_corr_build_workspace(int mma_dtype, int64_t E, int C, int H, int W);
+int vh_corr_build(int in_dtype, int mma_dtype, int out_dtype, int64_t E, int C, int H, int W, const void* fmap1,
+                  const void* fmap2, int num_levels, void* const* levels, void* workspace, void* stream);
 
 /* altcorr_forward (droid.cpp:41, altcorr_kernel.cu:26-138, 368-392).
  * fmap1 [B,H1,W1,C], fmap2 [B,H2,W2,C] (F16/F32/F64), coords [B,N,H1,W1,2] f32,

@@ -68,16 +68,19 @@ class CorrBlock:
 
     ``volume_dtype`` selects the storage of the pyramid (float32, or float16
     as the reference produces under autocast); by default it follows the
-    feature dtype.
+    feature dtype.  ``contraction_dtype`` selects the MFMA operand type of the
+    volume build (default: the feature dtype; float16, bfloat16 or
+    float8_e4m3fn for the mixed-precision forms, all accumulating in FP32).
     """
 
-    def __init__(self, fmap1, fmap2, num_levels=4, radius=3, volume_dtype=None):
+    def __init__(self, fmap1, fmap2, num_levels=4, radius=3, volume_dtype=None, contraction_dtype=None):
         self.num_levels = num_levels
         self.radius = radius
         B, N, C, H, W = fmap1.shape
         f1 = fmap1.reshape(B * N, C, H, W).contiguous()
         f2 = fmap2.reshape(B * N, C, H, W).contiguous()
-        levels = droid_net_ext.corr_build(f1, f2, num_levels, out_dtype=volume_dtype)
+        levels = droid_net_ext.corr_build(f1, f2, num_levels, out_dtype=volume_dtype,
+                                          contraction_dtype=contraction_dtype)
         self.corr_pyramid = levels  # level i: [B*N, H, W, H/2^i, W/2^i]
 
     def __call__(self, coords):

@@ -545,111 +545,436 @@ __global__ __launch_bounds__(128) void altcorr_bwd_kernel(const float* __restric
 }
 
 // ---------------------------------------------------------------------------
-// All-pairs volume (level 0) on MFMA + 2x2 mean pyramid.
+// All-pairs volume and pyramid (CorrBlock, droid_net.py:56-69, 94-102).
 //
-// Level 0 for one edge is a [HW x C] x [C x HW] product.  A workgroup of
-// 256 lanes (4 waves) computes a 64 x 64 output tile: each wave a 32 x 32
-// sub-tile with v_mfma_f32_32x32x8 (f16 / bf16 inputs, K = 8 per issue) or
-// v_mfma_f32_32x32x2 (f32 inputs).  Operands are staged through LDS in
-// K-slices of 32 channels.  The 1/16 scale is applied in the epilogue.
+// Level 0 of edge e is O[m][n] = sum_c f1[c][m] f2[c][n] / 16 over the HW
+// source pixels m and HW target pixels n; level l > 0 is the 2x2 mean of
+// level l-1 over the target dimensions (floor of odd sizes), each level
+// rounded to the output type before the next is pooled, as avg_pool2d over
+// a stored pyramid does.
+//
+//   1. corr_to_cl_kernel       features -> channels-last [E][HW][Cp] in the
+//                              contraction type (zero channel padding).  For
+//                              FP8 each channel is first balanced between the
+//                              two maps (f1 / a_c, f2 * a_c, a_c = sqrt of the
+//                              ratio of their per-edge amax), then each pixel
+//                              row gets its own scale amax/448 (e4m3).
+//   2. corr_build_kernel       one workgroup per (edge, 128 source pixels, band
+//                              of 8 target rows); it walks the band's 8 x 16
+//                              target patches left to right, so that whole
+//                              output row segments come from one CU.  Each
+//                              patch is a 128 x 128 tile: 4 waves x 2 x 2 tiles
+//                              of 32 x 32 on MFMA (v_mfma_f32_32x32x16_{f16,
+//                              bf16,fp8_fp8} or 32x32x2f32) with operands staged
+//                              in LDS 128 channels at a time (A once per band
+//                              when C <= 128).  The epilogue keeps the level-0
+//                              tile in LDS and writes level 0 and the pooled
+//                              levels 1-3 of the patch directly: the pyramid
+//                              costs no second pass over level 0.
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kTM = 64, kTN = 64, kTK = 32;
+constexpr int kBM = 128;            // source pixels per tile
+constexpr int kPY = 8, kPX = 16;    // target patch per tile
+constexpr int kBN = kPY * kPX;      // 128 target pixels
+constexpr int kKC = 128;            // channels staged per pass
+constexpr int kOS = kBN + 4;        // FP32 tile row stride in LDS
+constexpr float kFp8Max = 448.f;    // e4m3 (OCP) largest finite value
 
-template <typename Tin>
-struct Mfma;
-
-template <>
-struct Mfma<__half> {
-    static constexpr int KSTEP = 8;  // per issue
-    __device__ static f32x16 run(const __half* a, const __half* b, f32x16 c) {
-        f16x4 av, bv;
-        for (int i = 0; i < 4; i++) {
-            av[i] = (_Float16)__half2float(a[i]);
-            bv[i] = (_Float16)__half2float(b[i]);
-        }
-        return __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
-    }
-};
-
-template <>
-struct Mfma<__hip_bfloat16> {
-    static constexpr int KSTEP = 8;
-    __device__ static f32x16 run(const __hip_bfloat16* a, const __hip_bfloat16* b, f32x16 c) {
-        bf16x4 av, bv;
-        for (int i = 0; i < 4; i++) {
-            av[i] = *reinterpret_cast<const short*>(&a[i]);
-            bv[i] = *reinterpret_cast<const short*>(&b[i]);
-        }
-        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, c, 0, 0, 0);
-    }
-};
-
-template <>
-struct Mfma<float> {
-    static constexpr int KSTEP = 2;
-    __device__ static f32x16 run(const float* a, const float* b, f32x16 c) {
+// Contraction types: element type in LDS, K per MFMA issue (KI), elements per
+// lane per issue (KI / 2 lanes groups).
+struct MmaF32 {
+    using T = float;
+    static constexpr int KI = 2;
+    __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x2f32(a[0], b[0], c, 0, 0, 0);
     }
 };
-
-// LDS tiles: As[k][m], Bs[k][n] with K-slice kTK.  MFMA 32x32xK operand layout:
-// lane l supplies row (l % 32) of A and column (l % 32) of B for the K-group
-// (l / 32): f16/bf16 K=8 -> elements k = 4*(l/32) .. +3; f32 K=2 -> k = l/32.
-// Accumulator layout: lane l holds column (l % 32), rows
-// 8*(v/4) + 4*(l/32) + (v%4) for v = 0..15.
-template <typename Tin, typename Tout>
-__global__ __launch_bounds__(256) void corr_gemm_kernel(const Tin* __restrict__ f1, const Tin* __restrict__ f2,
-                                                        Tout* __restrict__ out, int C, int HW) {
-    constexpr int KS = Mfma<Tin>::KSTEP;
-    constexpr int PER = KS / 2;  // operand elements per lane per issue
-    __shared__ Tin As[kTK][kTM + 4];
-    __shared__ Tin Bs[kTK][kTN + 4];
-    const int e = blockIdx.z;
-    const int m0 = blockIdx.y * kTM;
-    const int n0 = blockIdx.x * kTN;
-    const int tid = threadIdx.x;
-    const int wave = tid / 64, lane = tid % 64;
-    const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
-    const Tin* A = f1 + (int64_t)e * C * HW;  // [C][HW] : A[k][m]
-    const Tin* Bm = f2 + (int64_t)e * C * HW; // [C][HW] : B[k][n]
-    f32x16 acc;
-    for (int v = 0; v < 16; v++) acc[v] = 0.f;
-    for (int k0 = 0; k0 < C; k0 += kTK) {
-        // stage K-slice: kTK x 64 for A and B, coalesced along HW
-        for (int idx = tid; idx < kTK * kTM; idx += 256) {
-            const int k = idx / kTM, m = idx % kTM;
-            const int kk = k0 + k;
-            As[k][m] = (kk < C && m0 + m < HW) ? A[(int64_t)kk * HW + m0 + m] : from_f<Tin>(0.f);
-            Bs[k][m] = (kk < C && n0 + m < HW) ? Bm[(int64_t)kk * HW + n0 + m] : from_f<Tin>(0.f);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < kTK; ks += KS) {
-            Tin a[PER > 0 ? PER * 2 : 1], b[PER > 0 ? PER * 2 : 1];
-            const int kb = ks + (lane / 32) * (KS / 2);
-#pragma unroll
-            for (int i = 0; i < KS / 2; i++) {
-                a[i] = As[kb + i][wm + lane % 32];
-                b[i] = Bs[kb + i][wn + lane % 32];
-            }
-            acc = Mfma<Tin>::run(a, b, acc);
-        }
-        __syncthreads();
+struct MmaF16 {
+    using T = _Float16;
+    static constexpr int KI = 16;
+    __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(*reinterpret_cast<const f16x8*>(a),
+                                                       *reinterpret_cast<const f16x8*>(b), c, 0, 0, 0);
     }
-    Tout* O = out + (int64_t)e * HW * HW;
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int row = m0 + wm + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-        const int col = n0 + wn + lane % 32;
-        if (row < HW && col < HW) O[(int64_t)row * HW + col] = from_f<Tout>(acc[v] * 0.0625f);
+};
+struct MmaBF16 {
+    using T = __bf16;
+    static constexpr int KI = 16;
+    __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8*>(a),
+                                                        *reinterpret_cast<const bf16x8*>(b), c, 0, 0, 0);
+    }
+};
+struct MmaFP8 {
+    using T = unsigned char;  // e4m3
+    static constexpr int KI = 16;
+    __device__ static f32x16 mma(const T* a, const T* b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(*reinterpret_cast<const long*>(a),
+                                                           *reinterpret_cast<const long*>(b), c, 0, 0, 0);
+    }
+};
+
+template <typename Mma>
+__device__ __forceinline__ typename Mma::T to_mma(float v) { return (typename Mma::T)v; }
+
+// Per-(edge, channel) amax of both maps over the pixels (FP8 balancing).
+template <typename Ts>
+__global__ __launch_bounds__(256) void corr_chan_amax_kernel(const Ts* __restrict__ f1, const Ts* __restrict__ f2,
+                                                             float* __restrict__ am1, float* __restrict__ am2,
+                                                             int HW) {
+    __shared__ float red[2][4];
+    const int64_t ec = blockIdx.x;
+    const Ts* p1 = f1 + ec * HW;
+    const Ts* p2 = f2 + ec * HW;
+    float a = 0.f, b = 0.f;
+    for (int i = threadIdx.x; i < HW; i += 256) {
+        a = fmaxf(a, fabsf(to_f(p1[i])));
+        b = fmaxf(b, fabsf(to_f(p2[i])));
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        a = fmaxf(a, __shfl_xor(a, o, 64));
+        b = fmaxf(b, __shfl_xor(b, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        am1[ec] = fmaxf(fmaxf(red[0][0], red[0][1]), fmaxf(red[0][2], red[0][3]));
+        am2[ec] = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
     }
 }
 
-// 2x2 mean over the last two dims of [planes, H, W] -> [planes, H/2, W/2].
+// Channels-last copy of one map in the contraction type: 32 pixels per
+// workgroup, [C][32] staged in LDS as FP32.  side = 0 for fmap1 (divide by the
+// balance factor), 1 for fmap2 (multiply); FP8 only.
+template <typename Ts, typename Mma, bool FP8>
+__global__ __launch_bounds__(256) void corr_to_cl_kernel(const Ts* __restrict__ f, typename Mma::T* __restrict__ out,
+                                                         const float* __restrict__ am1, const float* __restrict__ am2,
+                                                         float* __restrict__ pscale, int C, int Cp, int HW, int side) {
+    using T = typename Mma::T;
+    extern __shared__ float tile[];  // [Cp][33]
+    __shared__ float sc[32];
+    const int nblk = (HW + 31) / 32;
+    const int e = blockIdx.x / nblk;
+    const int p0 = (blockIdx.x % nblk) * 32;
+    const int tid = threadIdx.x;
+    const Ts* fe = f + (int64_t)e * C * HW;
+    for (int i = tid; i < Cp * 32; i += 256) {
+        const int c = i >> 5, px = i & 31;
+        float v = 0.f;
+        if (c < C && p0 + px < HW) {
+            v = to_f(fe[(int64_t)c * HW + p0 + px]);
+            if (FP8) {
+                const float a1 = am1[(int64_t)e * C + c], a2 = am2[(int64_t)e * C + c];
+                const float bal = (a1 > 0.f && a2 > 0.f) ? sqrtf(a1 / a2) : 1.f;
+                v = side == 0 ? v / bal : v * bal;
+            }
+        }
+        tile[c * 33 + px] = v;
+    }
+    __syncthreads();
+    if (FP8) {
+        // per-pixel amax over channels: 8 lanes per pixel
+        const int px = tid >> 3, part = tid & 7;
+        float a = 0.f;
+        for (int c = part; c < Cp; c += 8) a = fmaxf(a, fabsf(tile[c * 33 + px]));
+        for (int o = 4; o >= 1; o >>= 1) a = fmaxf(a, __shfl_xor(a, o, 64));
+        if (part == 0) {
+            const float s = a > 0.f ? a / kFp8Max : 1.f;
+            sc[px] = s;
+            if (p0 + px < HW) pscale[(int64_t)e * HW + p0 + px] = s;
+        }
+        __syncthreads();
+        // four channels per lane, packed into one dword
+        for (int i = tid; i < 32 * (Cp / 4); i += 256) {
+            const int px = i / (Cp / 4), c4 = (i % (Cp / 4)) * 4;
+            if (p0 + px >= HW) continue;
+            const float inv = 1.f / sc[px];
+            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(tile[c4 * 33 + px] * inv, tile[(c4 + 1) * 33 + px] * inv, 0, false);
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(tile[(c4 + 2) * 33 + px] * inv, tile[(c4 + 3) * 33 + px] * inv, pk, true);
+            *reinterpret_cast<int*>(out + ((int64_t)e * HW + p0 + px) * Cp + c4) = pk;
+        }
+    } else {
+        for (int i = tid; i < 32 * Cp; i += 256) {
+            const int px = i / Cp, c = i % Cp;
+            if (p0 + px < HW) out[((int64_t)e * HW + p0 + px) * Cp + c] = to_mma<Mma>(tile[c * 33 + px]);
+        }
+    }
+}
+
+template <typename Tout>
+__device__ __forceinline__ float round_to(float v) { return to_f(from_f<Tout>(v)); }
+
+// Four consecutive LDS values as float (one 8- or 16-byte read).
+__device__ __forceinline__ void load4(const float* p, float* v) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+}
+__device__ __forceinline__ void load4(const __half* p, float* v) {
+    const uint2 t = *reinterpret_cast<const uint2*>(p);
+    const float2 a = __half22float2(*reinterpret_cast<const __half2*>(&t.x));
+    const float2 b = __half22float2(*reinterpret_cast<const __half2*>(&t.y));
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+
+// Store n <= V consecutive values; one vector store when all V are valid and
+// the row layout keeps the vector aligned (vec), element stores otherwise.
+template <typename Tout, int V>
+__device__ __forceinline__ void store_run(Tout* dst, const float (&v)[V], int n, bool vec);
+template <>
+__device__ __forceinline__ void store_run<float, 4>(float* dst, const float (&v)[4], int n, bool vec) {
+    if (vec && n == 4) {
+        *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        for (int i = 0; i < n; i++) dst[i] = v[i];
+    }
+}
+template <>
+__device__ __forceinline__ void store_run<float, 2>(float* dst, const float (&v)[2], int n, bool vec) {
+    if (vec && n == 2) {
+        *reinterpret_cast<float2*>(dst) = make_float2(v[0], v[1]);
+    } else {
+        for (int i = 0; i < n; i++) dst[i] = v[i];
+    }
+}
+template <>
+__device__ __forceinline__ void store_run<__half, 4>(__half* dst, const float (&v)[4], int n, bool vec) {
+    if (vec && n == 4) {
+        const __half2 a = __floats2half2_rn(v[0], v[1]), b = __floats2half2_rn(v[2], v[3]);
+        uint2 u;
+        u.x = *reinterpret_cast<const unsigned int*>(&a);
+        u.y = *reinterpret_cast<const unsigned int*>(&b);
+        *reinterpret_cast<uint2*>(dst) = u;
+    } else {
+        for (int i = 0; i < n; i++) dst[i] = __float2half(v[i]);
+    }
+}
+template <>
+__device__ __forceinline__ void store_run<__half, 2>(__half* dst, const float (&v)[2], int n, bool vec) {
+    if (vec && n == 2) {
+        *reinterpret_cast<__half2*>(dst) = __floats2half2_rn(v[0], v[1]);
+    } else {
+        for (int i = 0; i < n; i++) dst[i] = __float2half(v[i]);
+    }
+}
+
+// Workgroup barrier that orders LDS only.  The build kernel never reads back
+// what it writes to global memory, so its stores need not drain at each
+// barrier (__syncthreads would wait for them).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <typename Mma, typename Tout, bool FP8>
+__global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* __restrict__ A,
+                                                         const typename Mma::T* __restrict__ B,
+                                                         const float* __restrict__ s1, const float* __restrict__ s2,
+                                                         Tout* __restrict__ o0, Tout* __restrict__ o1,
+                                                         Tout* __restrict__ o2, Tout* __restrict__ o3, int L, int Cp,
+                                                         int H, int W) {
+    using T = typename Mma::T;
+    constexpr int RS = kKC + 16 / (int)sizeof(T);  // operand row stride (elements): 16 B of padding
+    constexpr int AB = kBM * RS * (int)sizeof(T);  // bytes of one staged operand
+    constexpr int OB = kBM * kOS * (int)sizeof(Tout);
+    __shared__ __align__(16) unsigned char smem[AB + (AB > OB ? AB : OB)];
+    __shared__ __align__(16) Tout P1[kBM][kPY / 2 * kPX / 2];  // pooled level 1 of the patch
+    __shared__ __align__(16) Tout P2[kBM][kPY / 4 * kPX / 4];  // pooled level 2
+    __shared__ float ss1[kBM], ss2[kBN];
+    T* As = reinterpret_cast<T*>(smem);
+    T* Bs = reinterpret_cast<T*>(smem + AB);
+    Tout* Os = reinterpret_cast<Tout*>(smem + AB);  // level-0 tile in the output type, over Bs
+    const int HW = H * W;
+    const int npx = (W + kPX - 1) / kPX, npy = (H + kPY - 1) / kPY;
+    const int nm = (HW + kBM - 1) / kBM;
+    int64_t bid = blockIdx.x;
+    const int ty = (int)(bid % npy);
+    bid /= npy;
+    const int mt = (int)(bid % nm);
+    const int e = (int)(bid / nm);
+    const int y0 = ty * kPY;
+    const int m0 = mt * kBM;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+    const T* Ae = A + (int64_t)e * HW * Cp;
+    const T* Be = B + (int64_t)e * HW * Cp;
+    const int64_t ob = (int64_t)e * HW;
+    constexpr int CH = 16 / (int)sizeof(T);  // elements per 16-byte chunk
+    constexpr int CPR = kKC / CH;             // chunks per full row
+    constexpr int NQ = kBM * CPR / 256;       // chunks per lane and operand
+    constexpr int PER = Mma::KI / 2;
+    if (FP8 && tid < kBM) ss1[tid] = (m0 + tid < HW) ? s1[(int64_t)e * HW + m0 + tid] : 0.f;
+    const bool single_pass = Cp <= kKC;
+    // Stage rows [0, kBM) of one operand: rows are source pixels (A) or the
+    // patch's target pixels (B).  Loads go to registers first (all of a group
+    // in flight), then to LDS; B for the next patch is loaded before the
+    // current patch's epilogue so that its latency hides behind the stores.
+    auto load_rows = [&](uint4 (&v)[NQ], bool isA, int x0, int c0, int cpr) {
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int i = tid + u * 256;
+            const int r = i / CPR, q = i % CPR;
+            v[u] = make_uint4(0, 0, 0, 0);
+            if (q < cpr) {
+                if (isA) {
+                    if (m0 + r < HW) v[u] = *reinterpret_cast<const uint4*>(Ae + (int64_t)(m0 + r) * Cp + c0 + q * CH);
+                } else {
+                    const int y = y0 + r / kPX, x = x0 + r % kPX;
+                    if (y < H && x < W)
+                        v[u] = *reinterpret_cast<const uint4*>(Be + (int64_t)(y * W + x) * Cp + c0 + q * CH);
+                }
+            }
+        }
+    };
+    auto store_rows = [&](T* dst, const uint4 (&v)[NQ]) {
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int i = tid + u * 256;
+            *reinterpret_cast<uint4*>(dst + (i / CPR) * RS + (i % CPR) * CH) = v[u];
+        }
+    };
+    uint4 breg[NQ];
+    if (single_pass) {
+        load_rows(breg, true, 0, 0, Cp / CH);
+        store_rows(As, breg);
+        load_rows(breg, false, 0, 0, Cp / CH);
+    }
+    for (int tx = 0; tx < npx; tx++) {
+        const int x0 = tx * kPX;
+        if (FP8 && tid < kBN) {
+            const int y = y0 + tid / kPX, x = x0 + tid % kPX;
+            ss2[tid] = (y < H && x < W) ? s2[(int64_t)e * HW + y * W + x] : 0.f;
+        }
+        f32x16 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
+        for (int c0 = 0; c0 < Cp; c0 += kKC) {
+            const int kc = min(kKC, Cp - c0);
+            if (!single_pass) {
+                load_rows(breg, true, 0, c0, kc / CH);
+                store_rows(As, breg);
+                load_rows(breg, false, x0, c0, kc / CH);
+            }
+            store_rows(Bs, breg);  // single pass: loaded during the previous patch
+            lds_barrier();
+            for (int kk = 0; kk < kc; kk += Mma::KI) {
+                const int k = kk + PER * (lane >> 5);
+                const T* a0 = As + (wm + (lane & 31)) * RS + k;
+                const T* a1 = a0 + 32 * RS;
+                const T* b0 = Bs + (wn + (lane & 31)) * RS + k;
+                const T* b1 = b0 + 32 * RS;
+                acc[0][0] = Mma::mma(a0, b0, acc[0][0]);
+                acc[0][1] = Mma::mma(a0, b1, acc[0][1]);
+                acc[1][0] = Mma::mma(a1, b0, acc[1][0]);
+                acc[1][1] = Mma::mma(a1, b1, acc[1][1]);
+            }
+            lds_barrier();
+        }
+        if (single_pass && tx + 1 < npx) load_rows(breg, false, x0 + kPX, 0, Cp / CH);
+        // level-0 tile, rounded to the output type, over Bs: lane holds column
+        // (lane % 32) and rows 8 (v / 4) + 4 (lane / 32) + (v % 4) of each sub-tile
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int n = wn + 32 * j + (lane & 31);
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int m = wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    float val = acc[i][j][v] * 0.0625f;
+                    if (FP8) val *= ss1[m] * ss2[n];
+                    Os[m * kOS + n] = from_f<Tout>(val);
+                }
+            }
+        lds_barrier();
+        const bool vec0 = (W % 4) == 0;
+        {
+            // level 0: 4 target columns per store
+#pragma unroll 4
+            for (int q = 0; q < kBM * kBN / 4 / 256; q++) {
+                const int idx = tid + q * 256;
+                const int px = (idx & 3) * 4, py = (idx >> 2) & (kPY - 1), ml = idx >> 5;
+                const int m = m0 + ml, y = y0 + py, x = x0 + px;
+                if (m >= HW || y >= H || x >= W) continue;
+                float vv[4];
+                load4(Os + ml * kOS + py * kPX + px, vv);
+                store_run<Tout, 4>(o0 + (ob + m) * HW + y * W + x, vv, min(4, W - x), vec0);
+            }
+        }
+        if (L >= 2) {
+            // level 1 of the patch (4 x 8), once each, from rounded level 0
+            const int Hl = H / 2, Wl = W / 2;
+            const int64_t HWl = (int64_t)Hl * Wl;
+            const bool vec = (Wl % 4) == 0;
+#pragma unroll
+            for (int q = 0; q < kBM * 8 / 256; q++) {
+                const int idx = tid + q * 256;
+                const int hx = idx & 1, py = (idx >> 1) & 3, ml = idx >> 3;  // four L1 columns 4 hx ..
+                float r0[8], r1[8];
+                load4(Os + ml * kOS + (2 * py) * kPX + 8 * hx, r0);
+                load4(Os + ml * kOS + (2 * py) * kPX + 8 * hx + 4, r0 + 4);
+                load4(Os + ml * kOS + (2 * py + 1) * kPX + 8 * hx, r1);
+                load4(Os + ml * kOS + (2 * py + 1) * kPX + 8 * hx + 4, r1 + 4);
+                float vv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
+                    P1[ml][py * 8 + 4 * hx + u] = from_f<Tout>(vv[u]);
+                }
+                const int m = m0 + ml, y = y0 / 2 + py, x = x0 / 2 + 4 * hx;
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o1 + (ob + m) * HWl + y * Wl + x, vv, min(4, Wl - x), vec);
+            }
+        }
+        if (L >= 3) {
+            lds_barrier();
+            // level 2 (2 x 4) from level 1
+            const int Hl = H / 4, Wl = W / 4;
+            const int64_t HWl = (int64_t)Hl * Wl;
+            const bool vec = (Wl % 4) == 0;
+            const int py = tid & 1, ml = tid >> 1;
+            float r0[8], r1[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                r0[u] = to_f(P1[ml][(2 * py) * 8 + u]);
+                r1[u] = to_f(P1[ml][(2 * py + 1) * 8 + u]);
+            }
+            float vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
+                P2[ml][py * 4 + u] = from_f<Tout>(vv[u]);
+            }
+            const int m = m0 + ml, y = y0 / 4 + py, x = x0 / 4;
+            if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o2 + (ob + m) * HWl + y * Wl + x, vv, min(4, Wl - x), vec);
+        }
+        if (L >= 4) {
+            lds_barrier();
+            // level 3 (1 x 2) from level 2
+            if (tid < kBM) {
+                const int Hl = H / 8, Wl = W / 8;
+                const int64_t HWl = (int64_t)Hl * Wl;
+                const bool vec = (Wl % 2) == 0;
+                float vv[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+                    vv[u] = round_to<Tout>((((to_f(P2[tid][2 * u]) + to_f(P2[tid][2 * u + 1])) + to_f(P2[tid][4 + 2 * u])) +
+                                            to_f(P2[tid][4 + 2 * u + 1])) * 0.25f);
+                const int m = m0 + tid, y = y0 / 8, x = x0 / 8;
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 2>(o3 + (ob + m) * HWl + y * Wl + x, vv, min(2, Wl - x), vec);
+            }
+        }
+        lds_barrier();  // the next patch stages over Os
+    }
+}
+
+// 2x2 mean over the last two dims of [planes, H, W] -> [planes, H/2, W/2]
+// (levels beyond the fused four).
 template <typename T>
 __global__ __launch_bounds__(256) void pool2x2_kernel(const T* __restrict__ in, T* __restrict__ out, int64_t planes,
                                                       int H, int W) {
@@ -660,33 +985,88 @@ __global__ __launch_bounds__(256) void pool2x2_kernel(const T* __restrict__ in, 
         const int r = (int)(i - p * Ho * Wo);
         const int y = r / Wo, x = r % Wo;
         const T* s = in + p * H * (int64_t)W + (int64_t)(2 * y) * W + 2 * x;
-        const float v = 0.25f * ((to_f(s[0]) + to_f(s[1])) + (to_f(s[W]) + to_f(s[W + 1])));
+        const float v = (((to_f(s[0]) + to_f(s[1])) + to_f(s[W])) + to_f(s[W + 1])) * 0.25f;
         out[i] = from_f<T>(v);
     }
 }
 
-template <typename Tin, typename Tout>
-int corr_build_launch(int64_t E, int C, int H, int W, const void* f1, const void* f2, int L, void* const* levels,
-                      hipStream_t s) {
-    const int HW = H * W;
-    dim3 grid(ceil_div(HW, kTN), ceil_div(HW, kTM), (unsigned)E);
-    hipLaunchKernelGGL((corr_gemm_kernel<Tin, Tout>), grid, dim3(256), 0, s, (const Tin*)f1, (const Tin*)f2,
-                       (Tout*)levels[0], C, HW);
-    VH_LAUNCH_CHECK("corr_build gemm");
-    int h = H, w = W;
-    for (int l = 1; l < L; l++) {
-        const int64_t planes = E * (int64_t)HW;
-        const int64_t n = planes * (h / 2) * (int64_t)(w / 2);
+inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+inline int chan_pad(int C) { return (C + 15) / 16 * 16; }
+
+size_t corr_build_ws(int mma, int64_t E, int C, int H, int W) {
+    const int Cp = chan_pad(C);
+    const size_t el = mma == VH_F32 ? 4 : mma == VH_FP8E4M3 ? 1 : 2;
+    const size_t HW = (size_t)H * W;
+    size_t b = 2 * align256(E * HW * Cp * el);
+    if (mma == VH_FP8E4M3) b += 2 * align256(E * HW * 4) + 2 * align256((size_t)E * C * 4);
+    return b;
+}
+
+template <typename Ts, typename Mma, typename Tout>
+int corr_build_run(int64_t E, int C, int H, int W, const Ts* f1, const Ts* f2, int L, void* const* levels,
+                   void* ws, hipStream_t s) {
+    constexpr bool FP8 = std::is_same<Mma, MmaFP8>::value;
+    using T = typename Mma::T;
+    const int Cp = chan_pad(C);
+    const int64_t HW = (int64_t)H * W;
+    unsigned char* w = (unsigned char*)ws;
+    T* A = (T*)w;
+    w += align256(E * HW * Cp * sizeof(T));
+    T* B = (T*)w;
+    w += align256(E * HW * Cp * sizeof(T));
+    float *ps1 = nullptr, *ps2 = nullptr, *am1 = nullptr, *am2 = nullptr;
+    if (FP8) {
+        ps1 = (float*)w; w += align256(E * HW * 4);
+        ps2 = (float*)w; w += align256(E * HW * 4);
+        am1 = (float*)w; w += align256((size_t)E * C * 4);
+        am2 = (float*)w; w += align256((size_t)E * C * 4);
+        hipLaunchKernelGGL(corr_chan_amax_kernel<Ts>, dim3((unsigned)(E * C)), dim3(256), 0, s, f1, f2, am1, am2,
+                           (int)HW);
+        VH_LAUNCH_CHECK("corr_build amax");
+    }
+    const unsigned cl_blocks = (unsigned)(E * ((HW + 31) / 32));
+    const size_t cl_lds = (size_t)Cp * 33 * 4;
+    hipLaunchKernelGGL((corr_to_cl_kernel<Ts, Mma, FP8>), dim3(cl_blocks), dim3(256), cl_lds, s, f1, A, am1, am2,
+                       ps1, C, Cp, (int)HW, 0);
+    VH_LAUNCH_CHECK("corr_build channels-last");
+    hipLaunchKernelGGL((corr_to_cl_kernel<Ts, Mma, FP8>), dim3(cl_blocks), dim3(256), cl_lds, s, f2, B, am1, am2,
+                       ps2, C, Cp, (int)HW, 1);
+    VH_LAUNCH_CHECK("corr_build channels-last");
+    const int64_t nblk = E * ((HW + kBM - 1) / kBM) * (int64_t)((H + kPY - 1) / kPY);
+    if (nblk > 0x7fffffffll) return fail(VH_ERR_UNSUPPORTED, "corr_build: %lld tiles", (long long)nblk);
+    Tout* lv[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int l = 0; l < L && l < 4; l++) lv[l] = (Tout*)levels[l];
+    hipLaunchKernelGGL((corr_build_kernel<Mma, Tout, FP8>), dim3((unsigned)nblk), dim3(256), 0, s, A, B, ps1, ps2,
+                       lv[0], lv[1], lv[2], lv[3], L, Cp, H, W);
+    VH_LAUNCH_CHECK("corr_build");
+    int h = H >> 3, wdt = W >> 3;
+    for (int l = 4; l < L; l++) {
+        const int64_t planes = E * HW;
+        const int64_t n = planes * (h / 2) * (int64_t)(wdt / 2);
         int64_t blocks = (n + 255) / 256;
         if (blocks > 1 << 20) blocks = 1 << 20;
         if (n > 0)
-            hipLaunchKernelGGL(pool2x2_kernel<Tout>, dim3((unsigned)blocks), dim3(256), 0, s,
-                               (const Tout*)levels[l - 1], (Tout*)levels[l], planes, h, w);
+            hipLaunchKernelGGL(pool2x2_kernel<Tout>, dim3((unsigned)blocks), dim3(256), 0, s, (const Tout*)levels[l - 1],
+                               (Tout*)levels[l], planes, h, wdt);
         VH_LAUNCH_CHECK("corr_build pool");
         h /= 2;
-        w /= 2;
+        wdt /= 2;
     }
     return VH_OK;
+}
+
+template <typename Ts, typename Tout>
+int corr_build_dispatch(int mma, int64_t E, int C, int H, int W, const void* f1, const void* f2, int L,
+                        void* const* levels, void* ws, hipStream_t s) {
+    const Ts* a = (const Ts*)f1;
+    const Ts* b = (const Ts*)f2;
+    switch (mma) {
+        case VH_F32: return corr_build_run<Ts, MmaF32, Tout>(E, C, H, W, a, b, L, levels, ws, s);
+        case VH_F16: return corr_build_run<Ts, MmaF16, Tout>(E, C, H, W, a, b, L, levels, ws, s);
+        case VH_BF16: return corr_build_run<Ts, MmaBF16, Tout>(E, C, H, W, a, b, L, levels, ws, s);
+        case VH_FP8E4M3: return corr_build_run<Ts, MmaFP8, Tout>(E, C, H, W, a, b, L, levels, ws, s);
+        default: return fail(VH_ERR_UNSUPPORTED, "corr_build: contraction dtype %d", mma);
+    }
 }
 
 }  // namespace
@@ -736,31 +1116,29 @@ int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels,
     }
 }
 
-int vh_corr_build(int in_dtype, int out_dtype, int64_t E, int C, int H, int W, const void* fmap1, const void* fmap2,
-                  int num_levels, void* const* levels, void* stream) {
-    if (E < 0 || C <= 0 || H <= 0 || W <= 0 || num_levels < 1)
-        return fail(VH_ERR_ARG, "corr_build: bad sizes");
-    if (E == 0) return VH_OK;
-    hipStream_t s = (hipStream_t)stream;
-    if (out_dtype == VH_F32) {
-        if (in_dtype == VH_F32) return corr_build_launch<float, float>(E, C, H, W, fmap1, fmap2, num_levels, levels, s);
-        if (in_dtype == VH_F16) return corr_build_launch<__half, float>(E, C, H, W, fmap1, fmap2, num_levels, levels, s);
-        if (in_dtype == VH_BF16)
-            return corr_build_launch<__hip_bfloat16, float>(E, C, H, W, fmap1, fmap2, num_levels, levels, s);
-    } else if (out_dtype == VH_F16) {
-        if (in_dtype == VH_F32) return corr_build_launch<float, __half>(E, C, H, W, fmap1, fmap2, num_levels, levels, s);
-        if (in_dtype == VH_F16) return corr_build_launch<__half, __half>(E, C, H, W, fmap1, fmap2, num_levels, levels, s);
-        if (in_dtype == VH_BF16)
-            return corr_build_launch<__hip_bfloat16, __half>(E, C, H, W, fmap1, fmap2, num_levels, levels, s);
-    }
-    return fail(VH_ERR_UNSUPPORTED, "corr_build: dtypes in=%d out=%d", in_dtype, out_dtype);
+size_t vh_corr_build_workspace(int mma_dtype, int64_t E, int C, int H, int W) {
+    if (E < 0 || C <= 0 || H <= 0 || W <= 0) return 0;
+    return corr_build_ws(mma_dtype, E, C, H, W);
 }
 
-size_t vh_corr_build_fp8_workspace(int64_t E, int C, int H, int W) { return 0; }
-
-int vh_corr_build_fp8(int out_dtype, int64_t E, int C, int H, int W, const float* fmap1, const float* fmap2,
-                      int num_levels, void* const* levels, void* workspace, void* stream) {
-    return fail(VH_ERR_UNSUPPORTED, "corr_build_fp8: not built yet");
+int vh_corr_build(int in_dtype, int mma_dtype, int out_dtype, int64_t E, int C, int H, int W, const void* fmap1,
+                  const void* fmap2, int num_levels, void* const* levels, void* workspace, void* stream) {
+    if (E < 0 || C <= 0 || H <= 0 || W <= 0 || num_levels < 1)
+        return fail(VH_ERR_ARG, "corr_build: bad sizes");
+    if (C > 448) return fail(VH_ERR_UNSUPPORTED, "corr_build: %d channels (at most 448)", C);
+    if (E == 0) return VH_OK;
+    if (!workspace) return fail(VH_ERR_ARG, "corr_build: null workspace");
+    hipStream_t s = (hipStream_t)stream;
+    if (out_dtype == VH_F32) {
+        if (in_dtype == VH_F32) return corr_build_dispatch<float, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+        if (in_dtype == VH_F16) return corr_build_dispatch<__half, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+        if (in_dtype == VH_BF16) return corr_build_dispatch<__hip_bfloat16, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+    } else if (out_dtype == VH_F16) {
+        if (in_dtype == VH_F32) return corr_build_dispatch<float, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+        if (in_dtype == VH_F16) return corr_build_dispatch<__half, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+        if (in_dtype == VH_BF16) return corr_build_dispatch<__hip_bfloat16, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+    }
+    return fail(VH_ERR_UNSUPPORTED, "corr_build: dtypes in=%d out=%d", in_dtype, out_dtype);
 }
 
 int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,

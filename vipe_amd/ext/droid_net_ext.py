@@ -154,24 +154,37 @@ def corr_lookup_pyramid(levels, coords, radius):
     return out
 
 
-def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None):
+_MMA_CODE = {torch.float32: _lib.VH_F32, torch.float16: _lib.VH_F16, torch.bfloat16: _lib.VH_BF16,
+             torch.float8_e4m3fn: _lib.VH_FP8E4M3}
+
+
+def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None, contraction_dtype=None):
     """All-pairs correlation volume pyramid of CorrBlock.  fmap1, fmap2 [E,C,H,W]
     (f32 / f16 / bf16) -> list of level-l volumes [E,H,W,H>>l,W>>l] in out_dtype
-    (default: f16 for 16-bit inputs, f32 for f32 inputs)."""
+    (default: f16 for 16-bit inputs, f32 for f32 inputs).  contraction_dtype
+    selects the MFMA operand type (default: the input dtype); float8_e4m3fn
+    balances each channel between the two maps and scales each pixel row into
+    e4m3, accumulating in FP32."""
     _contig(fmap1=fmap1, fmap2=fmap2)
     _gpu(fmap1, fmap2)
+    if fmap2.shape != fmap1.shape or fmap2.dtype != fmap1.dtype:
+        raise RuntimeError("fmap1 and fmap2 must have the same shape and dtype")
     E, Cc, H, W = fmap1.shape
     if out_dtype is None:
         out_dtype = torch.float32 if fmap1.dtype == torch.float32 else torch.float16
+    mma = _MMA_CODE.get(contraction_dtype if contraction_dtype is not None else fmap1.dtype, -1)
     levels = []
     h, w = H, W
     for _ in range(num_levels):
         levels.append(torch.empty((E, H, W, h, w), dtype=out_dtype, device=fmap1.device))
         h, w = h // 2, w // 2
+    lib = _lib.load()
+    ws = torch.empty((max(int(lib.vh_corr_build_workspace(mma, E, Cc, H, W)), 1),), dtype=torch.uint8,
+                     device=fmap1.device)
     lv = (C.c_void_p * num_levels)(*[t.data_ptr() for t in levels])
     with torch.cuda.device(fmap1.device):
-        rc = _lib.load().vh_corr_build(_code(fmap1, (torch.float32, torch.float16, torch.bfloat16)),
-                                       _lib.DTYPE_CODE[out_dtype], E, Cc, H, W, _lib.ptr(fmap1), _lib.ptr(fmap2),
-                                       num_levels, lv, _lib.stream_of(fmap1))
+        rc = lib.vh_corr_build(_code(fmap1, (torch.float32, torch.float16, torch.bfloat16)), mma,
+                               _lib.DTYPE_CODE.get(out_dtype, -1), E, Cc, H, W, _lib.ptr(fmap1), _lib.ptr(fmap2),
+                               num_levels, lv, _lib.ptr(ws), _lib.stream_of(fmap1))
     _lib.check(rc, "corr_build")
     return levels
