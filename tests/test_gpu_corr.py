@@ -133,3 +133,45 @@ def test_fused_pyramid_lookup_matches_per_level_and_oracle():
     (blk.lookup_per_level(coords.cuda()) * go).sum().backward()
     for a, b in zip(g_fused, lv):
         assert torch.equal(a, b.grad)
+
+
+def test_corr_lookup_gradients_to_both_feature_maps():
+    """CorrBlock lookup backward to fmap1 and fmap2 against FP64 autograd through
+    the oracle's volume + avg_pool2d + bilinear lookup, and against central
+    differences (FP64, eps=1e-6) of the oracle on 64 random directions."""
+    from vipe_amd.corr import CorrBlock
+
+    g = torch.Generator().manual_seed(7)
+    B, N, C, H, W = 1, 3, 32, 12, 16
+    f1 = torch.randn(B, N, C, H, W, generator=g) / math.sqrt(C)
+    f2 = torch.randn(B, N, C, H, W, generator=g) / math.sqrt(C)
+    coords = torch.rand(B, N, H, W, 2, generator=g) * torch.tensor([W + 2.0, H + 2.0]) - 1.0
+    go = torch.randn(B, N, 4 * 49, H, W, generator=g)
+
+    a1, a2 = f1.cuda().requires_grad_(True), f2.cuda().requires_grad_(True)
+    out = CorrBlock(a1, a2, num_levels=4, radius=3)(coords.cuda())
+    (out * go.cuda()).sum().backward()
+    d1, d2 = a1.grad.double().cpu(), a2.grad.double().cpu()
+
+    def loss(x1, x2):
+        o = OC.corr_block_lookup(OC.corr_pyramid(OC.corr_volume(x1, x2), 4), coords.double(), 3)
+        return (o * go.double()).sum()
+
+    r1, r2 = f1.double().requires_grad_(True), f2.double().requires_grad_(True)
+    loss(r1, r2).backward()
+    scale = max(r1.grad.abs().max().item(), r2.grad.abs().max().item())
+    assert (d1 - r1.grad).abs().max().item() < 1e-5 * scale
+    assert (d2 - r2.grad).abs().max().item() < 1e-5 * scale
+
+    eps = 1e-6
+    f1d, f2d = f1.double(), f2.double()
+    errs = []
+    for k in range(64):
+        v1 = torch.randn(f1.shape, generator=g, dtype=torch.float64)
+        v2 = torch.randn(f2.shape, generator=g, dtype=torch.float64)
+        with torch.no_grad():
+            fd = (loss(f1d + eps * v1, f2d + eps * v2) - loss(f1d - eps * v1, f2d - eps * v2)).item() / (2 * eps)
+        an = ((d1 * v1).sum() + (d2 * v2).sum()).item()
+        errs.append(abs(fd - an) / max(abs(fd), 1e-3))
+    print(f"[corr feature grads] max rel directional error over 64 directions: {max(errs):.3e}")
+    assert max(errs) < 1e-4

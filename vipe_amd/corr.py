@@ -63,6 +63,53 @@ class CorrPyramidSampler(torch.autograd.Function):
         return (None, None, *grads)
 
 
+class CorrLookupFeatures(torch.autograd.Function):
+    """Fused pyramid lookup whose backward returns the gradients of both
+    feature maps (CorrBlock built from ``fmap1``, ``fmap2`` [E, C, H, W]).
+
+    By linearity of the pooling, level l of the lookup equals the on-the-fly
+    correlation of fmap1/4 with avgpool^l(fmap2)/4 sampled at coords/2^l
+    (droid_net.py:56-102, altcorr_kernel.cu:242-366), so each level's feature
+    gradients come from the altcorr backward kernel; the level-l gradient of
+    fmap2 is spread back over its 2^l x 2^l block with weight 4^-l (the
+    adjoint of the repeated 2x2 mean; floor-dropped rows and columns get 0).
+    """
+
+    @staticmethod
+    def forward(ctx, coords, radius, fmap1, fmap2, *levels):
+        ctx.radius = radius
+        ctx.num_levels = len(levels)
+        ctx.save_for_backward(coords, fmap1, fmap2)
+        return droid_net_ext.corr_lookup_pyramid(list(levels), coords, radius)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        coords, f1, f2 = ctx.saved_tensors
+        E, Cc, H, W = f1.shape
+        r, L = ctx.radius, ctx.num_levels
+        rd2 = (2 * r + 1) ** 2
+        g = grad_output.float()
+        F1 = (f1.float() / 4.0).permute(0, 2, 3, 1).contiguous()
+        df1 = torch.zeros((E, H, W, Cc), dtype=torch.float32, device=f1.device)
+        df2 = torch.zeros((E, Cc, H, W), dtype=torch.float32, device=f1.device)
+        f2l = f2.float()
+        for lvl in range(L):
+            if lvl > 0:
+                f2l = F.avg_pool2d(f2l, 2, stride=2)
+            F2 = (f2l / 4.0).permute(0, 2, 3, 1).contiguous()
+            gl = g[:, lvl * rd2:(lvl + 1) * rd2].reshape(E, 1, rd2, H, W).contiguous()
+            cl = (coords / 2 ** lvl).reshape(E, 1, H, W, 2).contiguous()
+            g1, g2, _ = droid_net_ext.altcorr_backward(F1, F2, cl, gl, r)
+            df1 += g1
+            s = 2 ** lvl
+            up = (g2.permute(0, 3, 1, 2) / 4.0 ** lvl).repeat_interleave(s, 2).repeat_interleave(s, 3)
+            df2[:, :, :up.shape[2], :up.shape[3]] += up
+        df1 = (df1 / 4.0).permute(0, 3, 1, 2).to(f1.dtype)
+        df2 = (df2 / 4.0).to(f2.dtype)
+        return (None, None, df1 if ctx.needs_input_grad[2] else None, df2 if ctx.needs_input_grad[3] else None,
+                *([None] * L))
+
+
 class CorrBlock:
     """All-pairs correlation pyramid of two feature maps ``[B, N, C, H, W]``.
 
@@ -79,16 +126,21 @@ class CorrBlock:
         B, N, C, H, W = fmap1.shape
         f1 = fmap1.reshape(B * N, C, H, W).contiguous()
         f2 = fmap2.reshape(B * N, C, H, W).contiguous()
-        levels = droid_net_ext.corr_build(f1, f2, num_levels, out_dtype=volume_dtype,
+        levels = droid_net_ext.corr_build(f1.detach(), f2.detach(), num_levels, out_dtype=volume_dtype,
                                           contraction_dtype=contraction_dtype)
         self.corr_pyramid = levels  # level i: [B*N, H, W, H/2^i, W/2^i]
+        # kept for gradients to the features (the pyramid itself is built without autograd)
+        self._fmaps = (f1, f2) if (f1.requires_grad or f2.requires_grad) else None
 
     def __call__(self, coords):
         """coords [B, N, H, W, 2] -> [B, N, L*(2r+1)^2, H, W] (droid_net.py:71-84), all
         levels in one fused kernel reading the coordinates in place."""
         batch, num, ht, wd, _ = coords.shape
         c = coords.reshape(batch * num, ht, wd, 2).contiguous().float()
-        out = CorrPyramidSampler.apply(c, self.radius, *self.corr_pyramid)
+        if self._fmaps is not None and torch.is_grad_enabled():
+            out = CorrLookupFeatures.apply(c, self.radius, self._fmaps[0], self._fmaps[1], *self.corr_pyramid)
+        else:
+            out = CorrPyramidSampler.apply(c, self.radius, *self.corr_pyramid)
         return out.view(batch, num, -1, ht, wd)
 
     def lookup_per_level(self, coords):
@@ -103,11 +155,13 @@ class CorrBlock:
         return torch.cat(out_pyramid, dim=2)
 
     def cat(self, other):
+        self._fmaps = None
         for i in range(self.num_levels):
             self.corr_pyramid[i] = torch.cat([self.corr_pyramid[i], other.corr_pyramid[i]], 0)
         return self
 
     def __getitem__(self, index):
+        self._fmaps = None
         for i in range(self.num_levels):
             self.corr_pyramid[i] = self.corr_pyramid[i][index]
         return self
