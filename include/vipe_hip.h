@@ -168,6 +168,22 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
               float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* workspace,
               void* stream);
 
+/* Implicit adjoint of one Gauss-Newton iteration of vh_ba_run (no counterpart
+ * in the reference, whose BA layer has no backward).  Call it after a
+ * vh_ba_run with iterations = 1 on the same plan and workspace, with nothing
+ * else run on that workspace in between: it reuses that iteration's Cholesky
+ * factor, Q, w and dx.  poses, disps, intrinsics are the values BEFORE that
+ * iteration (vh_ba_run updates them in place); targets, weights, depth_active,
+ * lm and flags are the ones the iteration used.  grad_dx [P,6] and grad_dz
+ * [|kx|,H*W] (unused when motion only) are dL/d(dx), dL/d(dz); grad_targets and
+ * grad_weights [E,2,H,W] receive dL/d(targets), dL/d(weights) (fully written).
+ * Intrinsics optimisation is not supported.  With lm != 0 every non-stereo
+ * edge of a depth frame must reach a different pose. */
+int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps, const float* intrinsics,
+                  const float* targets, const float* weights, const float* depth_active, float lm, int flags,
+                  const float* grad_dx, const float* grad_dz, float* grad_targets, float* grad_weights,
+                  void* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

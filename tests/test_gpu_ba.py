@@ -189,3 +189,53 @@ def test_ba_full_size_fp32_meets_accuracy_targets():
     assert px < 1e-4
     assert rel_pose < 1e-5
     assert torch.allclose(en.double(), ren, rtol=1e-5)
+
+
+@pytest.mark.parametrize("motion_only", [False, True])
+def test_ba_implicit_adjoint_against_fp64_central_differences(motion_only):
+    """dL/d(targets, weights) of one BA iteration through the implicit adjoint
+    (forward Cholesky factor reused) against central differences (FP64 oracle,
+    eps = 1e-6) on 64 random directions; L is the squared error of (dx, dz)
+    to a random ground truth."""
+    from vipe_amd.ba_layer import ba_iteration
+
+    g = make_graph("kf8_e24_48x64", with_fmaps=False)
+    N = g.disps.shape[0]
+    da = torch.ones(N)
+    ds = g.disps.clone() * 1.01
+    ds[:, ::2] = 0.0
+    lm, ep, alpha = 1e-4, 0.1, 0.05
+    gen = torch.Generator().manual_seed(11)
+    d = _cuda(g)
+    tg = d["targets"].clone().requires_grad_(True)
+    wt = d["weights"].clone().requires_grad_(True)
+    dx, dz = ba_iteration(tg, wt, d["poses"], d["disps"], d["intrinsics"], ds.cuda(), d["eta"], d["ii"], d["jj"],
+                          da.cuda(), g.t0, g.t1, lm, ep, motion_only, alpha)
+    Y = dx.detach().cpu().double() + 0.01 * torch.randn(dx.shape, generator=gen, dtype=torch.float64)
+    loss = ((dx - Y.float().cuda()) ** 2).sum()
+    if not motion_only:
+        Z = dz.detach().cpu().double() + 0.01 * torch.randn(dz.shape, generator=gen, dtype=torch.float64)
+        loss = loss + ((dz - Z.float().cuda()) ** 2).sum()
+    loss.backward()
+    gt_, gw_ = tg.grad.double().cpu(), wt.grad.double().cpu()
+
+    def loss64(t64, w64):
+        rdx, rdz, _ = OB.ba_extended(g.poses.clone(), g.disps.clone(), g.intrinsics.clone(), ds, t64, w64, g.eta,
+                                     g.ii, g.jj, da, g.t0, g.t1, 1, lm, ep, motion_only, alpha, False, 1e-6, 1e-6,
+                                     1.0, False)
+        val = ((rdx - Y) ** 2).sum()
+        if not motion_only:
+            val = val + ((rdz - Z) ** 2).sum()
+        return val.item()
+
+    t64, w64 = g.targets.double(), g.weights.double()
+    eps = 1e-6
+    errs = []
+    for k in range(64):
+        vt = torch.randn(t64.shape, generator=gen, dtype=torch.float64)
+        vw = torch.randn(w64.shape, generator=gen, dtype=torch.float64)
+        fd = (loss64(t64 + eps * vt, w64 + eps * vw) - loss64(t64 - eps * vt, w64 - eps * vw)) / (2 * eps)
+        an = ((gt_ * vt).sum() + (gw_ * vw).sum()).item()
+        errs.append(abs(fd - an) / max(abs(fd), 1e-6))
+    print(f"[BA adjoint motion_only={motion_only}] max rel directional error over 64 directions: {max(errs):.3e}")
+    assert max(errs) < 2e-3

@@ -66,6 +66,7 @@ _SIGS = {
     "vh_ba_plan_num_depth_frames": (i32, [vp]),
     "vh_ba_plan_depth_frames": (i32, [vp, vp]),
     "vh_ba_workspace_size": (sz, [vp]),
+    "vh_ba_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, f32, i32, vp, vp, vp, vp, vp, vp]),
     "vh_ba_run": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, vp, vp, vp, vp,
                         vp]),
 }

@@ -149,6 +149,7 @@ __device__ __forceinline__ void edge_transform(const float* __restrict__ poses, 
 struct PixTerms {
     float ru, rv, wu, wv, Jzu, Jzv, Jfu, Jfv;
     float Ju[6], Jv[6];
+    bool ok;  // depth in front of MIN_DEPTH: the residual carries weight
 };
 
 template <bool INTR, bool RES>
@@ -159,6 +160,7 @@ __device__ __forceinline__ void pixel_terms(const float* R, const float* t, floa
     const float Xj1 = R[3] * X0 + R[4] * Y0 + R[5] + t[1] * d;
     const float Xj2 = R[6] * X0 + R[7] * Y0 + R[8] + t[2] * d;
     const bool ok = !(Xj2 < MIN_DEPTH);
+    o.ok = ok;
     const float dinv = ok ? 1.0f / Xj2 : 0.0f;
     const float d2 = dinv * dinv;
     const float x = Xj0, y = Xj1, h = d;
@@ -1191,6 +1193,393 @@ __global__ __launch_bounds__(64) void ba_retract_kernel(float* poses, float* int
     for (int r = 0; r < 4; r++) T[3 + r] = (float)q1[r];
 }
 
+// ---------------------------------------------------------------------------
+// 7. implicit adjoint of one Gauss-Newton iteration (no counterpart in the
+//    reference, whose BA has no backward)
+// ---------------------------------------------------------------------------
+// Forward of one iteration, per depth pixel z and residual row (e, c):
+//   pose terms  H_pp += pw J^T J,  b_p += pw r J        pw = stereo ? 0 : w
+//   depth terms C += a^2 w Jz^2, b_z += a^2 w r Jz, E_row += a^2 w Jz J
+//   (w = 0.001 weight on valid depth, a = depth_active of the frame)
+//   S = H_pp - E Q E^T damped by ep + lm diag;  y = S^-1 (b_p - E Q b_z)
+//   dz = Q (b_z - E~^T y)   (E~: E without rows of local pose index <= 0)
+// Given gy = dL/dy and gz = dL/dz the adjoint is
+//   ybar = gy - E~ (Q gz),  mu = S^-1 ybar  (the forward's Cholesky factor)
+//   dL/db_p = mu,  dL/db_z = Q (gz - E^T mu)
+//   dL/dS_raw = -mu y^T - lm diag(mu * y)
+//   dL/dQ = gz (b_z - E~^T y) - (E^T mu) b_z + (E^T mu)(E^T y) + lm sum_i mu_i y_i E_i^2
+//   dL/dE = Q [-mu b_z + (E^T y) mu + (E^T mu) y + 2 lm (mu*y) E] - mask * y Q gz
+// and per row dL/dtarget = dL/dr, dL/dweight = 0.001 valid dL/dw with
+//   dL/dr = pw J.mu + a^2 w Jz dL/db_z
+//   dL/dw = ns [r J.mu - (J.mu)(J.y) - lm sum_i mu_i y_i J_i^2]
+//         + a^2 [Jz J.dL/dE + Jz^2 dL/dC + r Jz dL/db_z]
+// where J.v applies the row to pose vectors: v[jj] - Adj(Tij) v[kf].
+// The lm terms take the pose rows of one frame as distinct apart from stereo
+// edges, which the host checks when it builds the plan.
+
+// Zero a double array.
+__global__ void zero_f64_kernel(double* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+struct AdjArgs {
+    const float* gdx;  // [P][6]
+    const float* gdz;  // [K][HW]
+    float* gt;         // [E][2][HW]
+    float* gw;         // [E][2][HW]
+    double* esum;      // [E][6]
+    double* ybar;      // [ld]
+    double* mu;        // [ld]
+    float lm;
+};
+
+// E row (j side) of edge s at one pixel: a^2 (wu Jzu Ju + wv Jzv Jv).
+__device__ __forceinline__ void e_row(const PixTerms& T, float a2, float* er) {
+#pragma unroll
+    for (int n = 0; n < 6; n++) er[n] = a2 * (T.wu * T.Jzu * T.Ju[n] + T.wv * T.Jzv * T.Jv[n]);
+}
+
+// Pass 1: esum[e] = sum_p er_e(p) Q(p) gz(p).
+__global__ __launch_bounds__(kThreads) void ba_adj_rhs_kernel(DevPlan p, RunArgs a, Work w, AdjArgs g) {
+    __shared__ float sEP[kMaxDeg][16];
+    __shared__ float sRed[4][6];
+    const int k = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kf = p.kx[k];
+    const int e0 = p.fr_ptr[k];
+    const int deg = p.fr_ptr[k + 1] - e0;
+    const float act = a.depth_active[kf];
+    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+    const int HW = p.HW;
+    if (tid < deg) {
+        const int e = p.fr_edges[e0 + tid];
+        float Rm[9], t[3];
+        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+        for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+    }
+    __syncthreads();
+    const int pix = blockIdx.x * kThreads + tid;
+    const bool valid = pix < HW;
+    const int py = valid ? pix / p.W : 0, px = valid ? pix - py * p.W : 0;
+    const float X0 = ((float)px - cx) / fx, Y0 = ((float)py - cy) / fy;
+    const float d = valid ? a.disps[(int64_t)kf * HW + pix] : 0.f;
+    const float tz = valid ? w.Qb[(int64_t)k * HW + pix] * g.gdz[(int64_t)k * HW + pix] : 0.f;
+    for (int s = 0; s < deg; s++) {
+        const int e = p.fr_edges[e0 + s];
+        float v[6] = {0, 0, 0, 0, 0, 0};
+        if (valid) {
+            const int64_t eb = (int64_t)e * 2 * HW + pix;
+            PixTerms T;
+            pixel_terms<false, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
+                                      0.f, 0.f, 1.f, T);
+            float er[6];
+            e_row(T, act * act, er);
+#pragma unroll
+            for (int n = 0; n < 6; n++) v[n] = er[n] * tz;
+        }
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const float sum = wave_sum(v[n]);
+            if (lane == 0) sRed[wave][n] = sum;
+        }
+        __syncthreads();
+        if (tid < 6)
+            atomicAdd(g.esum + (int64_t)e * 6 + tid,
+                      (double)sRed[0][tid] + (double)sRed[1][tid] + (double)sRed[2][tid] + (double)sRed[3][tid]);
+        __syncthreads();
+    }
+}
+
+// Pass 2: ybar = gy - E~ t on the pose entries (quirk mask: local index >= 1),
+// zeros on the padding.  One thread per entry of the system.
+__global__ __launch_bounds__(256) void ba_adj_pose_kernel(DevPlan p, RunArgs a, AdjArgs g, int motion) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.ld) return;
+    if (i >= 6 * p.P) {
+        g.ybar[i] = 0.0;
+        return;
+    }
+    const int l = i / 6, n = i % 6;
+    double acc = g.gdx[i];
+    if (!motion && l >= 1) {
+        for (int e = 0; e < p.E; e++) {
+            const int lj = p.jj[e] - p.t0, li = p.ii[e] - p.t0;
+            if (lj == l) acc -= g.esum[(int64_t)e * 6 + n];
+            if (li == l) {
+                // kf side: E_kf = -Adj^T er, so -E~ t adds Adj^T esum
+                float Rm[9], t[3];
+                edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+                float es[6], o[6];
+                for (int q = 0; q < 6; q++) es[q] = (float)g.esum[(int64_t)e * 6 + q];
+                neg_adjT(Rm, t, es, o);
+                acc -= (double)o[n];
+            }
+        }
+    }
+    g.ybar[i] = acc;
+}
+
+// x = (L L^T)^-1 rhs with the stored factor (X_k = L_kk^-T in LinvT, L_ik^T in
+// the upper blocks of A); zeros when the factorisation failed.
+//   forward   z_k = X_k^T (rhs_k - sum_{j<k} L_kj z_j)
+//   backward  x_k = X_k (z_k - sum_{i>k} L_ik^T x_i)
+__global__ __launch_bounds__(1024) void chol_solve_full_kernel(const double* __restrict__ A,
+                                                               const double* __restrict__ LinvT, int ld, int nb,
+                                                               const double* __restrict__ rhs,
+                                                               double* __restrict__ out, const int* flag) {
+    __shared__ double zs[kMaxLd];
+    __shared__ double xs[kMaxLd];
+    __shared__ double Xk[kCB][kCB + 1];
+    __shared__ double t[kCB];
+    __shared__ double part[16][kCB];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
+    if (flag[0] != 0) {
+        for (int i = tid; i < ld; i += 1024) out[i] = 0.0;
+        return;
+    }
+    for (int kb = 0; kb < nb; kb++) {
+        const int c0 = kb * kCB;
+        const double* X = LinvT + (int64_t)kb * kCB * kCB;
+        double xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) xv[q] = X[tid + q * 1024];
+        // s[r] = sum_{q < c0} A[q][c0 + r] z[q]: lanes along r, waves split q
+        double s = 0.0;
+        for (int q = wv; q < c0; q += 16) s += A[(int64_t)q * ld + c0 + lane] * zs[q];
+        part[wv][lane] = s;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * 1024;
+            Xk[e >> 6][e & 63] = xv[q];
+        }
+        __syncthreads();
+        if (wv == 0) {
+            double acc = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; u++) acc += part[u][lane];
+            t[lane] = rhs[c0 + lane] - acc;
+        }
+        __syncthreads();
+        // z_k[c] = sum_m X[m][c] t[m]: wave wv takes m in [4 wv, 4 wv + 4)
+        double s2 = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) s2 += Xk[4 * wv + u][lane] * t[4 * wv + u];
+        part[wv][lane] = s2;
+        __syncthreads();
+        if (wv == 0) {
+            double acc = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; u++) acc += part[u][lane];
+            zs[c0 + lane] = acc;
+        }
+        __syncthreads();
+    }
+    for (int kb = nb - 1; kb >= 0; kb--) {
+        const int c0 = kb * kCB;
+        const double* X = LinvT + (int64_t)kb * kCB * kCB;
+        double xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) xv[q] = X[tid + q * 1024];
+        const int r0 = c0 + kCB, nr = (nb - kb - 1) * kCB;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = lane; q < nr; q += 64) {
+            const double xq = xs[r0 + q];
+#pragma unroll
+            for (int u = 0; u < 4; u++) s[u] += A[(int64_t)(c0 + wv + 16 * u) * ld + r0 + q] * xq;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            double v = s[u];
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            s[u] = v;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) t[wv + 16 * u] = zs[c0 + wv + 16 * u] - s[u];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * 1024;
+            Xk[e >> 6][e & 63] = xv[q];
+        }
+        __syncthreads();
+        double s2 = 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) s2 += Xk[lane][4 * wv + u] * t[4 * wv + u];
+        part[wv][lane] = s2;
+        __syncthreads();
+        if (wv == 0) {
+            double acc = 0.0;
+#pragma unroll
+            for (int u = 0; u < 16; u++) acc += part[u][lane];
+            xs[c0 + lane] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < ld; i += 1024) out[i] = xs[i];
+}
+
+// Pass 3: per-pixel aggregates and per-row gradients.
+template <bool MOTION>
+__global__ __launch_bounds__(kThreads) void ba_adj_grad_kernel(DevPlan p, RunArgs a, Work w, AdjArgs g) {
+    __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo(12)
+    __shared__ float sUm[kMaxDeg][6], sUy[kMaxDeg][6], sUq[kMaxDeg][6], sMj[kMaxDeg][6];
+    __shared__ float sMk[6];
+    const int k = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int kf = p.kx[k];
+    const int e0 = p.fr_ptr[k];
+    const int deg = p.fr_ptr[k + 1] - e0;
+    const int has_self = p.fr_self[k];
+    const int lk = kf - p.t0;
+    const float act = a.depth_active[kf];
+    const float a2 = act * act;
+    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+    const int HW = p.HW;
+    const float lm = g.lm;
+    const double* y = w.x;
+    const double* mu = g.mu;
+    if (tid < deg) {
+        const int e = p.fr_edges[e0 + tid];
+        float Rm[9], t[3];
+        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+        for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+        const bool stereo = p.ii[e] == p.jj[e];
+        sEP[tid][12] = stereo ? 1.f : 0.f;
+        // u(v) = v[jj] - Adj(Tij) v[kf] over the window, and with the back-substitution mask
+        const int lj = p.jj[e] - p.t0;
+        const bool jv = lj >= 0 && lj < p.P, kv = has_self && lk >= 0 && lk < p.P;
+        double um[6] = {0, 0, 0, 0, 0, 0}, uy[6] = {0, 0, 0, 0, 0, 0}, uq[6] = {0, 0, 0, 0, 0, 0};
+        for (int n = 0; n < 6; n++) {
+            if (jv) {
+                um[n] = mu[6 * lj + n];
+                uy[n] = y[6 * lj + n];
+                if (lj >= 1) uq[n] = y[6 * lj + n];
+            }
+            sMj[tid][n] = (jv && !stereo) ? (float)(mu[6 * lj + n] * y[6 * lj + n]) : 0.f;
+        }
+        if (kv) {
+            const double* vm = mu + 6 * lk;
+            const double* vy = y + 6 * lk;
+            double am[6], ay[6];
+            for (int r = 0; r < 3; r++) {
+                const double rvm = Rm[3 * r] * vm[0] + Rm[3 * r + 1] * vm[1] + Rm[3 * r + 2] * vm[2];
+                const double rwm = Rm[3 * r] * vm[3] + Rm[3 * r + 1] * vm[4] + Rm[3 * r + 2] * vm[5];
+                const double rvy = Rm[3 * r] * vy[0] + Rm[3 * r + 1] * vy[1] + Rm[3 * r + 2] * vy[2];
+                const double rwy = Rm[3 * r] * vy[3] + Rm[3 * r + 1] * vy[4] + Rm[3 * r + 2] * vy[5];
+                am[r] = rvm; am[3 + r] = rwm;
+                ay[r] = rvy; ay[3 + r] = rwy;
+            }
+            // Adj v = [R v_tau + t x (R v_phi), R v_phi]
+            double adm[6] = {am[0] + (t[1] * am[5] - t[2] * am[4]), am[1] + (t[2] * am[3] - t[0] * am[5]),
+                             am[2] + (t[0] * am[4] - t[1] * am[3]), am[3], am[4], am[5]};
+            double ady[6] = {ay[0] + (t[1] * ay[5] - t[2] * ay[4]), ay[1] + (t[2] * ay[3] - t[0] * ay[5]),
+                             ay[2] + (t[0] * ay[4] - t[1] * ay[3]), ay[3], ay[4], ay[5]};
+            for (int n = 0; n < 6; n++) {
+                um[n] -= adm[n];
+                uy[n] -= ady[n];
+                if (lk >= 1) uq[n] -= ady[n];
+            }
+        }
+        for (int n = 0; n < 6; n++) {
+            sUm[tid][n] = (float)um[n];
+            sUy[tid][n] = (float)uy[n];
+            sUq[tid][n] = (float)uq[n];
+        }
+    }
+    if (tid < 6) {
+        const bool kv = has_self && lk >= 0 && lk < p.P;
+        sMk[tid] = kv ? (float)(mu[6 * lk + tid] * y[6 * lk + tid]) : 0.f;
+    }
+    __syncthreads();
+    const bool kv = has_self && lk >= 0 && lk < p.P;
+    const int pix = blockIdx.x * kThreads + tid;
+    if (pix >= HW) return;
+    const int py = pix / p.W, px = pix - py * p.W;
+    const float X0 = ((float)px - cx) / fx, Y0 = ((float)py - cy) / fy;
+    const float d = a.disps[(int64_t)kf * HW + pix];
+    float Q = 0.f, bz = 0.f, lz = 0.f, tz = 0.f, bbz = 0.f, Cbar = 0.f, Ey = 0.f, Emu = 0.f;
+    float Ekf[6] = {0, 0, 0, 0, 0, 0};
+    if (!MOTION) {
+        Q = w.Qb[(int64_t)k * HW + pix];
+        bz = w.Wb[(int64_t)k * HW + pix];
+        lz = g.gdz[(int64_t)k * HW + pix];
+        tz = Q * lz;
+        float Eyq = 0.f, lmq = 0.f;
+        for (int s = 0; s < deg; s++) {
+            const int e = p.fr_edges[e0 + s];
+            const int64_t eb = (int64_t)e * 2 * HW + pix;
+            PixTerms T;
+            pixel_terms<false, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
+                                      0.f, 0.f, 1.f, T);
+            float er[6];
+            e_row(T, a2, er);
+            float o[6];
+            neg_adjT(sEP[s], sEP[s] + 9, er, o);
+            const bool stereo = sEP[s][12] != 0.f;
+#pragma unroll
+            for (int n = 0; n < 6; n++) {
+                Ey += er[n] * sUy[s][n];
+                Emu += er[n] * sUm[s][n];
+                Eyq += er[n] * sUq[s][n];
+                lmq += sMj[s][n] * er[n] * er[n];
+                Ekf[n] += (kv ? o[n] : 0.f) + (stereo && kv ? er[n] : 0.f);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 6; n++) lmq += sMk[n] * Ekf[n] * Ekf[n];
+        bbz = Q * (lz - Emu);
+        const float Qbar = lz * (bz - Eyq) - Emu * bz + Emu * Ey + lm * lmq;
+        Cbar = -Q * Q * Qbar;
+    }
+    for (int s = 0; s < deg; s++) {
+        const int e = p.fr_edges[e0 + s];
+        const int64_t eb = (int64_t)e * 2 * HW + pix;
+        const float* Rm = sEP[s];
+        const float* tt = sEP[s] + 9;
+        const bool stereo = sEP[s][12] != 0.f;
+        PixTerms T;
+        pixel_terms<false, true>(Rm, tt, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW], a.targets[eb],
+                                 a.targets[eb + HW], 1.f, T);
+        float er[6];
+        e_row(T, a2, er);
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const float* J = c == 0 ? T.Ju : T.Jv;
+            const float Jz = c == 0 ? T.Jzu : T.Jzv;
+            const float wr = c == 0 ? T.wu : T.wv;
+            const float r = c == 0 ? T.ru : T.rv;
+            float Ji[6];
+            neg_adjT(Rm, tt, J, Ji);
+            float Jm = 0.f, Jy = 0.f, Jq = 0.f, lmp = 0.f, lmr = 0.f;
+#pragma unroll
+            for (int n = 0; n < 6; n++) {
+                Jm += J[n] * sUm[s][n];
+                Jy += J[n] * sUy[s][n];
+                Jq += J[n] * sUq[s][n];
+                const float jk = (kv ? Ji[n] : 0.f) + (stereo && kv ? J[n] : 0.f);  // row at pose kf
+                lmp += sMj[s][n] * J[n] * J[n] + sMk[n] * (kv ? Ji[n] * Ji[n] : 0.f);
+                lmr += J[n] * sMj[s][n] * er[n] + jk * sMk[n] * Ekf[n];
+            }
+            const float pw = stereo ? 0.f : wr;
+            float rbar = pw * Jm;
+            float wbar = stereo ? 0.f : (r * Jm - Jm * Jy - lm * lmp);
+            if (!MOTION) {
+                rbar += a2 * wr * Jz * bbz;
+                wbar += a2 * (Jz * Q * (Jm * Ey + Jy * Emu + 2.f * lm * lmr) - Jz * Jm * Q * bz - Jz * Jq * tz +
+                              Jz * Jz * Cbar + r * Jz * bbz);
+            }
+            // w = 0.001 weight where the depth test passes (pixel_terms)
+            g.gt[eb + c * HW] = rbar;
+            g.gw[eb + c * HW] = T.ok ? 0.001f * wbar : 0.f;
+        }
+    }
+}
+
 }  // namespace ba
 }  // namespace vh
 
@@ -1208,8 +1597,12 @@ struct vh_ba_plan {
     size_t ws_bytes = 0;
     // workspace offsets
     size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_y, o_Li, o_flag;
+    size_t o_esum, o_ybar, o_mu;  // adjoint
     // plan variants: the system size depends on whether intrinsics are optimised
     int P;
+    // every non-stereo edge of a depth frame reaches a different pose (the
+    // adjoint's damping terms rely on it)
+    bool adj_distinct = true;
 };
 
 namespace {
@@ -1266,6 +1659,13 @@ vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, in
     }
     int deg_max = 0, Rext_max = 1;
     for (int k = 0; k < K; k++) {
+        std::vector<int64_t> js;
+        for (int q = fr_ptr[k]; q < fr_ptr[k + 1]; q++) {
+            const int e = fr_edges[q];
+            if (ii_host[e] != jj_host[e]) js.push_back(jj_host[e]);
+        }
+        std::sort(js.begin(), js.end());
+        if (std::adjacent_find(js.begin(), js.end()) != js.end()) pl->adj_distinct = false;
         fr_self[k] = (cand[k] >= t0 && cand[k] < t1) ? 1 : 0;
         const int deg = fr_ptr[k + 1] - fr_ptr[k];
         deg_max = std::max(deg_max, deg);
@@ -1430,6 +1830,9 @@ vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, in
     pl->o_Li = o; o += align256((size_t)d.ld * kCB * 8);
     pl->o_y = o; o += align256((size_t)d.ld * 8);
     pl->o_flag = o; o += 256;
+    pl->o_esum = o; o += align256((size_t)std::max(E, 1) * 6 * 8);
+    pl->o_ybar = o; o += align256((size_t)d.ld * 8);
+    pl->o_mu = o; o += align256((size_t)d.ld * 8);
     pl->ws_bytes = o;
     return pl;
 }
@@ -1538,6 +1941,68 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
         hipLaunchKernelGGL(ba_retract_kernel, dim3((d.P + 63) / 64), dim3(64), 0, s, poses, intrinsics, w.x, d.t0, d.P,
                            intr ? 1 : 0, intrinsics_scale);
         VH_LAUNCH_CHECK("ba_retract_kernel");
+    }
+    return VH_OK;
+}
+
+
+int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps, const float* intrinsics,
+                  const float* targets, const float* weights, const float* depth_active, float lm, int flags,
+                  const float* grad_dx, const float* grad_dz, float* grad_targets, float* grad_weights,
+                  void* workspace, void* stream) {
+    if (!plan) return fail(VH_ERR_ARG, "ba_adjoint: null plan");
+    const bool motion = (flags & VH_BA_MOTION_ONLY) != 0;
+    if (flags & VH_BA_OPTIMIZE_INTRINSICS) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: intrinsics optimisation");
+    if (!poses || !disps || !intrinsics || !targets || !weights || !depth_active || !grad_dx || !workspace ||
+        (!motion && !grad_dz) || (plan->d.E > 0 && (!grad_targets || !grad_weights)))
+        return fail(VH_ERR_ARG, "ba_adjoint: null tensor pointer");
+    if (lm != 0.f && !plan->adj_distinct)
+        return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: a depth frame has two edges to the same pose (needs lm = 0)");
+    hipStream_t s = (hipStream_t)stream;
+    DevPlan d = plan->d;
+    d.intr = 0;
+    d.n = 6 * d.P;
+    char* ws = (char*)workspace;
+    Work w{};
+    w.Qb = (float*)(ws + plan->o_Q);
+    w.Wb = (float*)(ws + plan->o_W);
+    w.A = (double*)(ws + plan->o_A);
+    w.x = (double*)(ws + plan->o_x);
+    w.Li = (double*)(ws + plan->o_Li);
+    w.flag = (int*)(ws + plan->o_flag);
+    RunArgs a{};
+    a.poses = const_cast<float*>(poses);
+    a.disps = const_cast<float*>(disps);
+    a.intrinsics = const_cast<float*>(intrinsics);
+    a.targets = targets;
+    a.weights = weights;
+    a.depth_active = depth_active;
+    AdjArgs g;
+    g.gdx = grad_dx;
+    g.gdz = grad_dz;
+    g.gt = grad_targets;
+    g.gw = grad_weights;
+    g.esum = (double*)(ws + plan->o_esum);
+    g.ybar = (double*)(ws + plan->o_ybar);
+    g.mu = (double*)(ws + plan->o_mu);
+    g.lm = lm;
+    const dim3 pgrid((d.HW + kThreads - 1) / kThreads, d.K);
+    if (!motion && d.E > 0 && d.K > 0) {
+        hipLaunchKernelGGL(zero_f64_kernel, dim3((d.E * 6 + 255) / 256), dim3(256), 0, s, g.esum, d.E * 6);
+        VH_LAUNCH_CHECK("ba_adjoint zero");
+        hipLaunchKernelGGL(ba_adj_rhs_kernel, pgrid, dim3(kThreads), 0, s, d, a, w, g);
+        VH_LAUNCH_CHECK("ba_adj_rhs_kernel");
+    }
+    hipLaunchKernelGGL(ba_adj_pose_kernel, dim3((d.ld + 255) / 256), dim3(256), 0, s, d, a, g,
+                       (motion || d.E == 0) ? 1 : 0);
+    VH_LAUNCH_CHECK("ba_adj_pose_kernel");
+    hipLaunchKernelGGL(chol_solve_full_kernel, dim3(1), dim3(1024), 0, s, w.A, w.Li, d.ld, d.nb, g.ybar, g.mu,
+                       w.flag);
+    VH_LAUNCH_CHECK("chol_solve_full_kernel");
+    if (d.E > 0 && d.K > 0) {
+        if (motion) hipLaunchKernelGGL(ba_adj_grad_kernel<true>, pgrid, dim3(kThreads), 0, s, d, a, w, g);
+        else hipLaunchKernelGGL(ba_adj_grad_kernel<false>, pgrid, dim3(kThreads), 0, s, d, a, w, g);
+        VH_LAUNCH_CHECK("ba_adj_grad_kernel");
     }
     return VH_OK;
 }

@@ -103,6 +103,17 @@ def _f32(name, t):
 def ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active, t0, t1,
                 iterations, lm, ep, motion_only, alpha, optimize_intrinsics, intrinsics_lm, intrinsics_ep,
                 intrinsics_scale, compute_energy=False):
+    dx, dz, energy, _, _ = _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj,
+                                        depth_active, t0, t1, iterations, lm, ep, motion_only, alpha,
+                                        optimize_intrinsics, intrinsics_lm, intrinsics_ep, intrinsics_scale,
+                                        compute_energy)
+    return [dx, dz, energy]
+
+
+def _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active, t0, t1,
+                 iterations, lm, ep, motion_only, alpha, optimize_intrinsics, intrinsics_lm, intrinsics_ep,
+                 intrinsics_scale, compute_energy=False):
+    """ba_extended that also returns the plan and the workspace of the run."""
     _check(targets=targets, weights=weights, poses=poses, disps=disps, intrinsics=intrinsics,
            disps_sens=disps_sens, ii=ii, jj=jj, depth_active=depth_active)
     for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics), ("disps_sens", disps_sens),
@@ -125,7 +136,7 @@ def ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii,
         raise RuntimeError(f"eta must hold |kx| = {plan.K} maps of {H}x{W}")
     energy = torch.empty((iterations if compute_energy else 0,), dtype=torch.float32, device=dev)
     if iterations <= 0:
-        return [None, None, energy]
+        return None, None, energy, plan, None
     dx = torch.empty((P, 6), dtype=torch.float32, device=dev)
     dz = None if motion_only else torch.empty((plan.K, H * W), dtype=torch.float32, device=dev)
     ws = torch.empty((max(plan.ws_bytes, 1),), dtype=torch.uint8, device=dev)
@@ -139,7 +150,28 @@ def ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii,
                            _lib.ptr(dz), _lib.ptr(energy) if compute_energy else None, _lib.ptr(ws),
                            _lib.stream_of(poses))
     _lib.check(rc, "slam_ext.ba")
-    return [dx, dz, energy]
+    return dx, dz, energy, plan, ws
+
+
+def ba_adjoint(plan, ws, poses, disps, intrinsics, targets, weights, depth_active, lm, motion_only, grad_dx,
+               grad_dz):
+    """Implicit adjoint of the single iteration that just ran on (plan, ws):
+    returns [dL/dtargets, dL/dweights] ([E,2,H,W] float32) from dL/d(dx) [P,6]
+    and dL/d(dz) [|kx|,H*W].  poses / disps / intrinsics are the values before
+    that iteration.  This package's extension: the reference BA has no backward."""
+    E = targets.shape[0]
+    gt = torch.empty_like(targets)
+    gw = torch.empty_like(weights)
+    gdx = grad_dx.contiguous().float()
+    gdz = None if motion_only else grad_dz.contiguous().float()
+    lib = _lib.load()
+    with torch.cuda.device(poses.device):
+        rc = lib.vh_ba_adjoint(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(targets),
+                               _lib.ptr(weights), _lib.ptr(depth_active), float(lm), 1 if motion_only else 0,
+                               _lib.ptr(gdx), _lib.ptr(gdz), _lib.ptr(gt) if E else None,
+                               _lib.ptr(gw) if E else None, _lib.ptr(ws), _lib.stream_of(poses))
+    _lib.check(rc, "slam_ext.ba_adjoint")
+    return [gt, gw]
 
 
 def ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, t0, t1, iterations, lm, ep,
