@@ -33,6 +33,11 @@ extern "C" {
 #define VH_BF16 4
 #define VH_FP8E4M3 5
 
+/* volume plane layouts (vh_corr_build / vh_corr_lookup_pyramid) */
+#define VH_LAYOUT_PLANAR 0 /* [Hl][Wl] row-major: the reference's */
+#define VH_LAYOUT_TILED 1  /* padded to [ceil(Hl/4)*4][ceil(Wl/tw)*tw], stored as 4 x tw tiles of 64 bytes
+                            * (tw = 4 for F32, 8 for F16), tiles row-major, padding zero */
+
 #define VH_DEVICE 0 /* pointers are device memory, kernels on `stream` */
 #define VH_HOST 1   /* pointers are host memory, computed on the CPU   */
 
@@ -90,11 +95,13 @@ int vh_corr_index_backward(int dtype, int64_t B, int H1, int W1, int H2, int W2,
 
 /* Fused pyramid lookup: CorrBlock.__call__ (vipe/slam/networks/droid_net.py:71-84,
  * i.e. the per-level corr_index_forward calls plus their concatenation).
- * levels[l] [B,H1,W1,Hl[l],Wl[l]] (dtype F16/F32/F64), coords [B,H1,W1,2] f32 at
- * level-0 resolution (level l samples at coords / 2^l), out [B, L*(2r+1)^2, H1, W1]
- * in the volume dtype, channel l*(2r+1)^2 + i*(2r+1) + j, fully written. */
+ * levels[l] [B,H1,W1,<plane of Hl[l] x Wl[l]>] in `layout` (dtype F16/F32/F64;
+ * tiled: F16/F32), coords [B,H1,W1,2] f32 at level-0 resolution (level l
+ * samples at coords / 2^l), out [B, L*(2r+1)^2, H1, W1] in the volume dtype,
+ * channel l*(2r+1)^2 + i*(2r+1) + j, fully written. */
 int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels, const void* const* levels,
-                           const int* Hl, const int* Wl, const float* coords, int radius, void* out, void* stream);
+                           const int* Hl, const int* Wl, int layout, const float* coords, int radius, void* out,
+                           void* stream);
 
 /* All-pairs correlation volume and its avg-pool pyramid: the CorrBlock
  * constructor (vipe/slam/networks/droid_net.py:56-69, 94-102), which the
@@ -106,11 +113,13 @@ int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels,
  * `mma_dtype` selects the contraction: F32, F16, BF16, or FP8E4M3 (each
  * channel balanced between the two maps by sqrt(amax1/amax2), then one e4m3
  * scale per pixel row; the scales are applied to the FP32 accumulator).
- * The caller provides vh_corr_build_workspace(mma_dtype, E, C, H, W) bytes of
- * device memory (channels-last operand copies and scales). */
+ * `layout` VH_LAYOUT_PLANAR or VH_LAYOUT_TILED (at most 4 levels) sets the
+ * plane layout of every level.  The caller provides
+ * vh_corr_build_workspace(mma_dtype, E, C, H, W) bytes of device memory
+ * (channels-last operand copies and scales). */
 size_t vh_corr_build_workspace(int mma_dtype, int64_t E, int C, int H, int W);
 int vh_corr_build(int in_dtype, int mma_dtype, int out_dtype, int64_t E, int C, int H, int W, const void* fmap1,
-                  const void* fmap2, int num_levels, void* const* levels, void* workspace, void* stream);
+                  const void* fmap2, int num_levels, void* const* levels, int layout, void* workspace, void* stream);
 
 /* altcorr_forward (droid.cpp:41, altcorr_kernel.cu:26-138, 368-392).
  * fmap1 [B,H1,W1,C], fmap2 [B,H2,W2,C] (F16/F32/F64), coords [B,N,H1,W1,2] f32,

@@ -108,24 +108,28 @@ def test_altcorr_forward_backward_and_indexed():
     assert (idx.double().cpu() - iref).abs().max().item() < 2e-4
 
 
-def test_fused_pyramid_lookup_matches_per_level_and_oracle():
+@pytest.mark.parametrize("layout", ["planar", "tiled"])
+@pytest.mark.parametrize("vol_dt,tol", [(torch.float32, 1e-5), (torch.float16, 2e-3)])
+@pytest.mark.parametrize("H,W", [(12, 16), (13, 18)])
+def test_fused_pyramid_lookup_matches_per_level_and_oracle(layout, vol_dt, tol, H, W):
     from vipe_amd.corr import CorrBlock
 
     g = torch.Generator().manual_seed(5)
-    B, N, C, H, W = 1, 3, 32, 12, 16
+    B, N, C = 1, 3, 32
     f1 = torch.randn(B, N, C, H, W, generator=g) / math.sqrt(C)
     f2 = torch.randn(B, N, C, H, W, generator=g) / math.sqrt(C)
     coords = torch.rand(B, N, H, W, 2, generator=g) * torch.tensor([W + 4.0, H + 4.0]) - 2.0
-    blk = CorrBlock(f1.cuda(), f2.cuda(), num_levels=4, radius=3)
+    blk = CorrBlock(f1.cuda(), f2.cuda(), num_levels=4, radius=3, volume_dtype=vol_dt, layout=layout)
+    assert blk.layout == layout
     fused = blk(coords.cuda())
     per_level = blk.lookup_per_level(coords.cuda())
     assert torch.equal(fused, per_level)
     ref = OC.corr_block_lookup(OC.corr_pyramid(OC.corr_volume(f1.double(), f2.double()), 4), coords.double(), 3)
-    assert (fused.double().cpu() - ref).abs().max().item() < 1e-5
+    assert (fused.double().cpu() - ref).abs().max().item() < tol
     # backward through the fused op equals backward through the per-level op
     lv = [t.detach().clone().requires_grad_(True) for t in blk.corr_pyramid]
     blk.corr_pyramid = lv
-    go = torch.randn(fused.shape, generator=g).cuda()
+    go = torch.randn(fused.shape, generator=g).to(vol_dt).cuda()
     (blk(coords.cuda()) * go).sum().backward()
     g_fused = [t.grad.clone() for t in lv]
     for t in lv:
@@ -133,6 +137,21 @@ def test_fused_pyramid_lookup_matches_per_level_and_oracle():
     (blk.lookup_per_level(coords.cuda()) * go).sum().backward()
     for a, b in zip(g_fused, lv):
         assert torch.equal(a, b.grad)
+
+
+@pytest.mark.parametrize("out_dt", [torch.float32, torch.float16])
+@pytest.mark.parametrize("E,C,H,W", [(2, 40, 13, 18), (1, 128, 24, 40)])
+def test_tiled_build_equals_planar_build(out_dt, E, C, H, W):
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(3)
+    f1 = (torch.randn(E, C, H, W, generator=g) / math.sqrt(C)).cuda()
+    f2 = (torch.randn(E, C, H, W, generator=g) / math.sqrt(C)).cuda()
+    planar = D.corr_build(f1, f2, 4, out_dtype=out_dt)
+    tiled = D.corr_build(f1, f2, 4, out_dtype=out_dt, layout="tiled")
+    for lvl, (p, t) in enumerate(zip(planar, tiled)):
+        assert torch.equal(D.untile(t, H >> lvl, W >> lvl), p)
+        assert torch.equal(t, D.tile(p))  # padding is zero
 
 
 def test_corr_lookup_gradients_to_both_feature_maps():
@@ -172,6 +191,9 @@ def test_corr_lookup_gradients_to_both_feature_maps():
         with torch.no_grad():
             fd = (loss(f1d + eps * v1, f2d + eps * v2) - loss(f1d - eps * v1, f2d - eps * v2)).item() / (2 * eps)
         an = ((d1 * v1).sum() + (d2 * v2).sum()).item()
-        errs.append(abs(fd - an) / max(abs(fd), 1e-3))
-    print(f"[corr feature grads] max rel directional error over 64 directions: {max(errs):.3e}")
-    assert max(errs) < 1e-4
+        # error relative to |grad| |v| (the bound of any directional derivative):
+        # FP32 gradients with atomically accumulated fmap2 terms sit near 1e-7
+        gv = (d1.norm() ** 2 + d2.norm() ** 2).sqrt().item() * (v1.norm() ** 2 + v2.norm() ** 2).sqrt().item()
+        errs.append(abs(fd - an) / gv)
+    print(f"[corr feature grads] max normalised directional error over 64 directions: {max(errs):.3e}")
+    assert max(errs) < 1e-5

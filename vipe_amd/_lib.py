@@ -17,6 +17,7 @@ LIB_PATH = Path(os.environ.get("VIPE_HIP_LIB", Path(__file__).resolve().parent /
 
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_UNSUPPORTED, VH_ERR_UNDEFINED = 0, 1, 2, 3, 4
 VH_F16, VH_F32, VH_F64, VH_BF16, VH_FP8E4M3 = 1, 2, 3, 4, 5
+VH_LAYOUT_PLANAR, VH_LAYOUT_TILED = 0, 1
 VH_DEVICE, VH_HOST = 0, 1
 
 vp = C.c_void_p
@@ -53,9 +54,9 @@ _SIGS = {
     # correlation
     "vh_corr_index_forward": (i32, [i32, i64, i32, i32, i32, i32, vp, vp, i32, vp, vp]),
     "vh_corr_index_backward": (i32, [i32, i64, i32, i32, i32, i32, vp, vp, i32, vp, vp]),
-    "vh_corr_lookup_pyramid": (i32, [i32, i64, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp]),
+    "vh_corr_lookup_pyramid": (i32, [i32, i64, i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp]),
     "vh_corr_build_workspace": (sz, [i32, i64, i32, i32, i32]),
-    "vh_corr_build": (i32, [i32, i32, i32, i64, i32, i32, i32, vp, vp, i32, vp, vp, vp]),
+    "vh_corr_build": (i32, [i32, i32, i32, i64, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp]),
     "vh_altcorr_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp]),
     "vh_altcorr_backward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp]),
     "vh_altcorr_index_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, i32,

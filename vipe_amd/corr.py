@@ -38,13 +38,14 @@ class CorrSampler(torch.autograd.Function):
 
 class CorrPyramidSampler(torch.autograd.Function):
     """All levels of one CorrBlock lookup in one kernel; backward scatters each
-    level's output gradient into its volume (corr_index_backward)."""
+    level's output gradient into its volume (corr_index_backward, re-tiled for
+    tiled pyramids)."""
 
     @staticmethod
-    def forward(ctx, coords, radius, *levels):
-        ctx.radius = radius
+    def forward(ctx, coords, radius, layout, level_dims, *levels):
+        ctx.radius, ctx.layout, ctx.level_dims = radius, layout, level_dims
         ctx.save_for_backward(coords, *levels)
-        return droid_net_ext.corr_lookup_pyramid(list(levels), coords, radius)
+        return droid_net_ext.corr_lookup_pyramid(list(levels), coords, radius, level_dims, layout)
 
     @staticmethod
     def backward(ctx, grad_output):
@@ -54,13 +55,14 @@ class CorrPyramidSampler(torch.autograd.Function):
         cxy = coords.permute(0, 3, 1, 2).contiguous()
         grads = []
         for i, vol in enumerate(levels):
-            if not ctx.needs_input_grad[2 + i]:
+            if not ctx.needs_input_grad[4 + i]:
                 grads.append(None)
                 continue
             g = grad_output[:, i * rd * rd:(i + 1) * rd * rd].reshape(B, rd, rd, H, W).contiguous()
-            (gv,) = droid_net_ext.corr_index_backward(vol, (cxy / 2**i).contiguous(), g, ctx.radius)
-            grads.append(gv)
-        return (None, None, *grads)
+            planar = vol if ctx.layout != "tiled" else droid_net_ext.untile(vol, *ctx.level_dims[i])
+            (gv,) = droid_net_ext.corr_index_backward(planar, (cxy / 2**i).contiguous(), g, ctx.radius)
+            grads.append(gv if ctx.layout != "tiled" else droid_net_ext.tile(gv))
+        return (None, None, None, None, *grads)
 
 
 class CorrLookupFeatures(torch.autograd.Function):
@@ -76,11 +78,11 @@ class CorrLookupFeatures(torch.autograd.Function):
     """
 
     @staticmethod
-    def forward(ctx, coords, radius, fmap1, fmap2, *levels):
+    def forward(ctx, coords, radius, layout, level_dims, fmap1, fmap2, *levels):
         ctx.radius = radius
         ctx.num_levels = len(levels)
         ctx.save_for_backward(coords, fmap1, fmap2)
-        return droid_net_ext.corr_lookup_pyramid(list(levels), coords, radius)
+        return droid_net_ext.corr_lookup_pyramid(list(levels), coords, radius, level_dims, layout)
 
     @staticmethod
     def backward(ctx, grad_output):
@@ -106,8 +108,8 @@ class CorrLookupFeatures(torch.autograd.Function):
             df2[:, :, :up.shape[2], :up.shape[3]] += up
         df1 = (df1 / 4.0).permute(0, 3, 1, 2).to(f1.dtype)
         df2 = (df2 / 4.0).to(f2.dtype)
-        return (None, None, df1 if ctx.needs_input_grad[2] else None, df2 if ctx.needs_input_grad[3] else None,
-                *([None] * L))
+        return (None, None, None, None, df1 if ctx.needs_input_grad[4] else None,
+                df2 if ctx.needs_input_grad[5] else None, *([None] * L))
 
 
 class CorrBlock:
@@ -118,17 +120,26 @@ class CorrBlock:
     feature dtype.  ``contraction_dtype`` selects the MFMA operand type of the
     volume build (default: the feature dtype; float16, bfloat16 or
     float8_e4m3fn for the mixed-precision forms, all accumulating in FP32).
+    ``layout`` "tiled" (default) keeps each level's planes as 64-byte tiles,
+    which the build writes and the lookup reads in whole sectors; "planar"
+    keeps the reference's [B*N, H, W, h, w] levels.  Callers only build,
+    look up, ``cat`` and index along the edge dimension, which both support.
     """
 
-    def __init__(self, fmap1, fmap2, num_levels=4, radius=3, volume_dtype=None, contraction_dtype=None):
+    def __init__(self, fmap1, fmap2, num_levels=4, radius=3, volume_dtype=None, contraction_dtype=None,
+                 layout="tiled"):
         self.num_levels = num_levels
         self.radius = radius
         B, N, C, H, W = fmap1.shape
         f1 = fmap1.reshape(B * N, C, H, W).contiguous()
         f2 = fmap2.reshape(B * N, C, H, W).contiguous()
+        if layout == "tiled" and num_levels > 4:
+            layout = "planar"
         levels = droid_net_ext.corr_build(f1.detach(), f2.detach(), num_levels, out_dtype=volume_dtype,
-                                          contraction_dtype=contraction_dtype)
-        self.corr_pyramid = levels  # level i: [B*N, H, W, H/2^i, W/2^i]
+                                          contraction_dtype=contraction_dtype, layout=layout)
+        self.layout = layout
+        self.level_dims = [(H >> i, W >> i) for i in range(num_levels)]
+        self.corr_pyramid = levels  # level i: [B*N, H, W, H/2^i, W/2^i] (planar) or [B*N, H, W, Hp*Wp]
         # kept for gradients to the features (the pyramid itself is built without autograd)
         self._fmaps = (f1, f2) if (f1.requires_grad or f2.requires_grad) else None
 
@@ -138,9 +149,10 @@ class CorrBlock:
         batch, num, ht, wd, _ = coords.shape
         c = coords.reshape(batch * num, ht, wd, 2).contiguous().float()
         if self._fmaps is not None and torch.is_grad_enabled():
-            out = CorrLookupFeatures.apply(c, self.radius, self._fmaps[0], self._fmaps[1], *self.corr_pyramid)
+            out = CorrLookupFeatures.apply(c, self.radius, self.layout, self.level_dims, self._fmaps[0],
+                                           self._fmaps[1], *self.corr_pyramid)
         else:
-            out = CorrPyramidSampler.apply(c, self.radius, *self.corr_pyramid)
+            out = CorrPyramidSampler.apply(c, self.radius, self.layout, self.level_dims, *self.corr_pyramid)
         return out.view(batch, num, -1, ht, wd)
 
     def lookup_per_level(self, coords):
@@ -150,7 +162,10 @@ class CorrBlock:
         coords = coords.permute(0, 1, 4, 2, 3)
         coords = coords.contiguous().view(batch * num, 2, ht, wd)
         for i in range(self.num_levels):
-            corr = CorrSampler.apply(self.corr_pyramid[i], coords / 2**i, self.radius)
+            vol = self.corr_pyramid[i]
+            if self.layout == "tiled":
+                vol = droid_net_ext.untile(vol, *self.level_dims[i])
+            corr = CorrSampler.apply(vol, coords / 2**i, self.radius)
             out_pyramid.append(corr.view(batch, num, -1, ht, wd))
         return torch.cat(out_pyramid, dim=2)
 
@@ -171,7 +186,7 @@ class CorrBlock:
         """all-pairs correlation [B, N, H, W, H, W] (level 0 only)."""
         batch, num, dim, ht, wd = fmap1.shape
         (lvl0,) = droid_net_ext.corr_build(fmap1.reshape(batch * num, dim, ht, wd).contiguous(),
-                                           fmap2.reshape(batch * num, dim, ht, wd).contiguous(), 1)
+                                           fmap2.reshape(batch * num, dim, ht, wd).contiguous(), 1, layout="planar")
         return lvl0.view(batch, num, ht, wd, ht, wd)
 
 

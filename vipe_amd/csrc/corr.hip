@@ -95,6 +95,34 @@ __device__ __forceinline__ void lookup_level(const T* __restrict__ plane, int H2
     }
 }
 
+// Volume plane layouts.  Planar: [H2][W2] row-major (the reference's).
+// Tiled: the plane padded to Hp = ceil(H2/4)*4 rows and Wp = ceil(W2/tw)*tw
+// columns and stored as [Hp/4][Wp/tw] tiles of 4 rows x tw columns, one
+// 64-byte tile each (tw = 4 for f32, 8 for f16); the padding holds zeros.  A
+// lookup window then touches whole 64-byte sectors, and the build writes
+// whole tiles.
+struct LevelGeom {
+    int Hl, Wl, Hp, Wp, tw;
+    int tiled;
+    int64_t plane;  // elements per source pixel
+    __host__ __device__ int64_t at(int64_t pm, int y, int x) const {
+        if (!tiled) return pm * plane + (int64_t)y * Wl + x;
+        return pm * plane + ((int64_t)(y >> 2) * (Wp / tw) + x / tw) * (4 * tw) + (y & 3) * tw + (x % tw);
+    }
+};
+
+inline LevelGeom level_geom(int Hl, int Wl, int elem, bool tiled) {
+    LevelGeom g;
+    g.Hl = Hl;
+    g.Wl = Wl;
+    g.tiled = tiled ? 1 : 0;
+    g.tw = 16 / elem;
+    g.Hp = tiled ? (Hl + 3) / 4 * 4 : Hl;
+    g.Wp = tiled ? (Wl + g.tw - 1) / g.tw * g.tw : Wl;
+    g.plane = (int64_t)g.Hp * g.Wp;
+    return g;
+}
+
 // Vector form of one lookup: each tap row comes from NL aligned loads of VEC
 // elements starting at xa = floor(xb / VEC) * VEC; the row is then shifted by
 // xb - xa in registers.  Needs W2 % VEC == 0 and a plane base aligned to
@@ -130,14 +158,18 @@ __device__ __forceinline__ void unpack_vec<__half, 2>(const unsigned int& v, flo
     e[0] = f.x; e[1] = f.y;
 }
 
-template <typename T, int R, int VEC>
+template <typename T, int R, int VEC, bool TILED = false>
 __device__ __forceinline__ void load_tap_row_vec(const T* __restrict__ plane, int y1, int xb, int H2, int W2,
                                                  float (&row)[2 * R + 2]) {
     using V = typename VecOf<T, VEC>::type;
     constexpr int NT = 2 * R + 2;
     constexpr int NL = (NT + VEC - 1 + VEC - 1) / VEC;  // vectors covering NT taps at any shift < VEC
+    constexpr int TW = 16 / (int)sizeof(T);              // tile width (tiled layout)
+    // tiled: H2, W2 are the padded sizes, whose extra rows and columns hold zeros
     const bool yok = (y1 >= 0) && (y1 < H2);
-    const T* rp = plane + (int64_t)(yok ? y1 : 0) * W2;
+    const int yy = yok ? y1 : 0;
+    const T* rp = TILED ? plane + (int64_t)(yy >> 2) * (W2 / TW) * (4 * TW) + (yy & 3) * TW
+                        : plane + (int64_t)yy * W2;
     const int xa = xb & ~(VEC - 1);
     float g[NL][VEC];
 #pragma unroll
@@ -145,7 +177,8 @@ __device__ __forceinline__ void load_tap_row_vec(const T* __restrict__ plane, in
         // out-of-plane vectors read the (aligned, valid) plane start and are zeroed
         const int x = xa + q * VEC;
         const bool ok = yok && x >= 0 && x < W2;
-        const V v = *reinterpret_cast<const V*>(ok ? rp + x : plane);
+        const T* src = TILED ? rp + (x / TW) * (4 * TW) + (x % TW) : rp + x;
+        const V v = *reinterpret_cast<const V*>(ok ? src : plane);
         unpack_vec<T, VEC>(v, g[q]);
 #pragma unroll
         for (int e = 0; e < VEC; e++) g[q][e] = ok ? g[q][e] : 0.f;
@@ -174,7 +207,7 @@ __device__ __forceinline__ void load_tap_row_vec(const T* __restrict__ plane, in
     }
 }
 
-template <typename T, int R, int VEC>
+template <typename T, int R, int VEC, bool TILED = false>
 __device__ __forceinline__ void lookup_level_vec(const T* __restrict__ plane, int H2, int W2, float x0, float y0,
                                                  T* __restrict__ o, int64_t ostride) {
     constexpr int RD = 2 * R + 1;
@@ -183,12 +216,12 @@ __device__ __forceinline__ void lookup_level_vec(const T* __restrict__ plane, in
     const int xb = (int)fx0 - R, yb = (int)fy0 - R;
     const float w00 = (1.f - dx) * (1.f - dy), w10 = dx * (1.f - dy), w01 = (1.f - dx) * dy, w11 = dx * dy;
     float r0[RD + 1], r1[RD + 1];
-    load_tap_row_vec<T, R, VEC>(plane, yb, xb, H2, W2, r0);
+    load_tap_row_vec<T, R, VEC, TILED>(plane, yb, xb, H2, W2, r0);
 #pragma unroll
     for (int j = 0; j < RD; j++) {
         float (&top)[RD + 1] = (j & 1) ? r1 : r0;
         float (&bot)[RD + 1] = (j & 1) ? r0 : r1;
-        load_tap_row_vec<T, R, VEC>(plane, yb + j + 1, xb, H2, W2, bot);
+        load_tap_row_vec<T, R, VEC, TILED>(plane, yb + j + 1, xb, H2, W2, bot);
 #pragma unroll
         for (int i = 0; i < RD; i++) {
             const float v = top[i] * w00 + top[i + 1] * w10 + bot[i] * w01 + bot[i + 1] * w11;
@@ -203,6 +236,8 @@ template <typename T, int R>
 __device__ __forceinline__ void lookup_level_any(const T* __restrict__ plane, int H2, int W2, float x0, float y0,
                                                  T* __restrict__ o, int64_t ostride, int vec) {
     if constexpr (std::is_same<T, float>::value || std::is_same<T, __half>::value) {
+        // vec == 0: tiled plane (H2, W2 padded)
+        if (vec == 0) return lookup_level_vec<T, R, 4, true>(plane, H2, W2, x0, y0, o, ostride);
         if (vec == 4) return lookup_level_vec<T, R, 4>(plane, H2, W2, x0, y0, o, ostride);
         if (vec == 2) return lookup_level_vec<T, R, 2>(plane, H2, W2, x0, y0, o, ostride);
     }
@@ -346,7 +381,11 @@ struct LevelPtrs {
     int vec[8];
 };
 
-template <typename T, int R>
+// TILED: every level is tiled (only that path is compiled).  The output
+// offsets of the 49 channels are computed once and stay live across levels;
+// the resulting ~150 VGPRs (3 waves per SIMD) measured faster than forcing 7-8
+// waves, where the extra scattered gathers in flight slowed the lookup down.
+template <typename T, int R, bool TILED>
 __global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, int L, const float* __restrict__ coords,
                                                                   T* __restrict__ out, int64_t B, int H1, int W1) {
     constexpr int RD = 2 * R + 1;
@@ -363,24 +402,42 @@ __global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, 
         const float sc = 1.0f / (float)(1 << l);
         const int H2 = lv.h[l], W2 = lv.w[l];
         const T* plane = (const T*)lv.p[l] + pix * (int64_t)H2 * W2;
-        lookup_level_any<T, R>(plane, H2, W2, cxf * sc, cyf * sc, o + (int64_t)l * RD * RD * HW1, HW1, lv.vec[l]);
+        if constexpr (TILED)
+            lookup_level_vec<T, R, 4, true>(plane, H2, W2, cxf * sc, cyf * sc, o + (int64_t)l * RD * RD * HW1, HW1);
+        else
+            lookup_level_any<T, R>(plane, H2, W2, cxf * sc, cyf * sc, o + (int64_t)l * RD * RD * HW1, HW1, lv.vec[l]);
     }
 }
 
 template <typename T>
 int corr_lookup_pyramid_launch(int64_t B, int H1, int W1, int L, const void* const* levels, const int* Hl,
-                               const int* Wl, const float* coords, void* out, hipStream_t s) {
+                               const int* Wl, bool tiled, const float* coords, void* out, hipStream_t s) {
     LevelPtrs lv;
     for (int l = 0; l < L; l++) {
         lv.p[l] = levels[l];
-        lv.h[l] = Hl[l];
-        lv.w[l] = Wl[l];
-        lv.vec[l] = lookup_vec_width(levels[l], Hl[l], Wl[l], (int)sizeof(T));
+        if (tiled) {
+            const LevelGeom g = level_geom(Hl[l], Wl[l], (int)sizeof(T), true);
+            lv.h[l] = g.Hp;
+            lv.w[l] = g.Wp;
+            lv.vec[l] = 0;
+        } else {
+            lv.h[l] = Hl[l];
+            lv.w[l] = Wl[l];
+            lv.vec[l] = lookup_vec_width(levels[l], Hl[l], Wl[l], (int)sizeof(T));
+        }
     }
     const int64_t n = B * H1 * (int64_t)W1;
     if (n == 0) return VH_OK;
-    hipLaunchKernelGGL((corr_lookup_pyramid_kernel<T, 3>), dim3(ceil_div(n, 256)), dim3(256), 0, s, lv, L, coords,
-                       (T*)out, B, H1, W1);
+    if constexpr (std::is_same<T, float>::value || std::is_same<T, __half>::value) {
+        if (tiled) {
+            hipLaunchKernelGGL((corr_lookup_pyramid_kernel<T, 3, true>), dim3(ceil_div(n, 256)), dim3(256), 0, s, lv,
+                               L, coords, (T*)out, B, H1, W1);
+            VH_LAUNCH_CHECK("corr_lookup_pyramid");
+            return VH_OK;
+        }
+    }
+    hipLaunchKernelGGL((corr_lookup_pyramid_kernel<T, 3, false>), dim3(ceil_div(n, 256)), dim3(256), 0, s, lv, L,
+                       coords, (T*)out, B, H1, W1);
     VH_LAUNCH_CHECK("corr_lookup_pyramid");
     return VH_OK;
 }
@@ -768,13 +825,17 @@ __device__ __forceinline__ void store_run<__half, 2>(__half* dst, const float (&
 // barrier (__syncthreads would wait for them).
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+struct LevelGeoms {
+    LevelGeom g[4];
+};
+
 template <typename Mma, typename Tout, bool FP8>
 __global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* __restrict__ A,
                                                          const typename Mma::T* __restrict__ B,
                                                          const float* __restrict__ s1, const float* __restrict__ s2,
                                                          Tout* __restrict__ o0, Tout* __restrict__ o1,
                                                          Tout* __restrict__ o2, Tout* __restrict__ o3, int L, int Cp,
-                                                         int H, int W) {
+                                                         int H, int W, LevelGeoms lg) {
     using T = typename Mma::T;
     constexpr int RS = kKC + 16 / (int)sizeof(T);  // operand row stride (elements): 16 B of padding
     constexpr int AB = kBM * RS * (int)sizeof(T);  // bytes of one staged operand
@@ -893,7 +954,7 @@ __global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* 
                 }
             }
         lds_barrier();
-        const bool vec0 = (W % 4) == 0;
+        const bool vec0 = lg.g[0].tiled || (W % 4) == 0;
         {
             // level 0: 4 target columns per store
 #pragma unroll 4
@@ -904,14 +965,13 @@ __global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* 
                 if (m >= HW || y >= H || x >= W) continue;
                 float vv[4];
                 load4(Os + ml * kOS + py * kPX + px, vv);
-                store_run<Tout, 4>(o0 + (ob + m) * HW + y * W + x, vv, min(4, W - x), vec0);
+                store_run<Tout, 4>(o0 + lg.g[0].at(ob + m, y, x), vv, min(4, W - x), vec0);
             }
         }
         if (L >= 2) {
             // level 1 of the patch (4 x 8), once each, from rounded level 0
             const int Hl = H / 2, Wl = W / 2;
-            const int64_t HWl = (int64_t)Hl * Wl;
-            const bool vec = (Wl % 4) == 0;
+            const bool vec = lg.g[1].tiled || (Wl % 4) == 0;
 #pragma unroll
             for (int q = 0; q < kBM * 8 / 256; q++) {
                 const int idx = tid + q * 256;
@@ -928,15 +988,14 @@ __global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* 
                     P1[ml][py * 8 + 4 * hx + u] = from_f<Tout>(vv[u]);
                 }
                 const int m = m0 + ml, y = y0 / 2 + py, x = x0 / 2 + 4 * hx;
-                if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o1 + (ob + m) * HWl + y * Wl + x, vv, min(4, Wl - x), vec);
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o1 + lg.g[1].at(ob + m, y, x), vv, min(4, Wl - x), vec);
             }
         }
         if (L >= 3) {
             lds_barrier();
             // level 2 (2 x 4) from level 1
             const int Hl = H / 4, Wl = W / 4;
-            const int64_t HWl = (int64_t)Hl * Wl;
-            const bool vec = (Wl % 4) == 0;
+            const bool vec = lg.g[2].tiled || (Wl % 4) == 0;
             const int py = tid & 1, ml = tid >> 1;
             float r0[8], r1[8];
 #pragma unroll
@@ -951,22 +1010,21 @@ __global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* 
                 P2[ml][py * 4 + u] = from_f<Tout>(vv[u]);
             }
             const int m = m0 + ml, y = y0 / 4 + py, x = x0 / 4;
-            if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o2 + (ob + m) * HWl + y * Wl + x, vv, min(4, Wl - x), vec);
+            if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o2 + lg.g[2].at(ob + m, y, x), vv, min(4, Wl - x), vec);
         }
         if (L >= 4) {
             lds_barrier();
             // level 3 (1 x 2) from level 2
             if (tid < kBM) {
                 const int Hl = H / 8, Wl = W / 8;
-                const int64_t HWl = (int64_t)Hl * Wl;
-                const bool vec = (Wl % 2) == 0;
+                const bool vec = lg.g[3].tiled || (Wl % 2) == 0;
                 float vv[2];
 #pragma unroll
                 for (int u = 0; u < 2; u++)
                     vv[u] = round_to<Tout>((((to_f(P2[tid][2 * u]) + to_f(P2[tid][2 * u + 1])) + to_f(P2[tid][4 + 2 * u])) +
                                             to_f(P2[tid][4 + 2 * u + 1])) * 0.25f);
                 const int m = m0 + tid, y = y0 / 8, x = x0 / 8;
-                if (m < HW && y < Hl && x < Wl) store_run<Tout, 2>(o3 + (ob + m) * HWl + y * Wl + x, vv, min(2, Wl - x), vec);
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 2>(o3 + lg.g[3].at(ob + m, y, x), vv, min(2, Wl - x), vec);
             }
         }
         lds_barrier();  // the next patch stages over Os
@@ -1002,9 +1060,15 @@ size_t corr_build_ws(int mma, int64_t E, int C, int H, int W) {
     return b;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void zero_fill_kernel(T* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = from_f<T>(0.f);
+}
+
 template <typename Ts, typename Mma, typename Tout>
 int corr_build_run(int64_t E, int C, int H, int W, const Ts* f1, const Ts* f2, int L, void* const* levels,
-                   void* ws, hipStream_t s) {
+                   bool tiled, void* ws, hipStream_t s) {
     constexpr bool FP8 = std::is_same<Mma, MmaFP8>::value;
     using T = typename Mma::T;
     const int Cp = chan_pad(C);
@@ -1035,9 +1099,23 @@ int corr_build_run(int64_t E, int C, int H, int W, const Ts* f1, const Ts* f2, i
     const int64_t nblk = E * ((HW + kBM - 1) / kBM) * (int64_t)((H + kPY - 1) / kPY);
     if (nblk > 0x7fffffffll) return fail(VH_ERR_UNSUPPORTED, "corr_build: %lld tiles", (long long)nblk);
     Tout* lv[4] = {nullptr, nullptr, nullptr, nullptr};
-    for (int l = 0; l < L && l < 4; l++) lv[l] = (Tout*)levels[l];
+    LevelGeoms lg;
+    for (int l = 0; l < 4; l++) {
+        lg.g[l] = level_geom(H >> l, W >> l, (int)sizeof(Tout), tiled);
+        if (l < L) {
+            lv[l] = (Tout*)levels[l];
+            // tiled padding must read as zero; the tiles' valid part is written below
+            const int64_t n = E * HW * lg.g[l].plane;
+            if (tiled && (lg.g[l].Hp != lg.g[l].Hl || lg.g[l].Wp != lg.g[l].Wl) && n > 0) {
+                int64_t blocks = (n + 255) / 256;
+                if (blocks > 1 << 20) blocks = 1 << 20;
+                hipLaunchKernelGGL(zero_fill_kernel<Tout>, dim3((unsigned)blocks), dim3(256), 0, s, lv[l], n);
+                VH_LAUNCH_CHECK("corr_build pad");
+            }
+        }
+    }
     hipLaunchKernelGGL((corr_build_kernel<Mma, Tout, FP8>), dim3((unsigned)nblk), dim3(256), 0, s, A, B, ps1, ps2,
-                       lv[0], lv[1], lv[2], lv[3], L, Cp, H, W);
+                       lv[0], lv[1], lv[2], lv[3], L, Cp, H, W, lg);
     VH_LAUNCH_CHECK("corr_build");
     int h = H >> 3, wdt = W >> 3;
     for (int l = 4; l < L; l++) {
@@ -1057,14 +1135,15 @@ int corr_build_run(int64_t E, int C, int H, int W, const Ts* f1, const Ts* f2, i
 
 template <typename Ts, typename Tout>
 int corr_build_dispatch(int mma, int64_t E, int C, int H, int W, const void* f1, const void* f2, int L,
-                        void* const* levels, void* ws, hipStream_t s) {
+                        void* const* levels, bool tiled, void* ws, hipStream_t s) {
     const Ts* a = (const Ts*)f1;
     const Ts* b = (const Ts*)f2;
+    if (tiled && L > 4) return fail(VH_ERR_UNSUPPORTED, "corr_build: tiled pyramids have at most 4 levels");
     switch (mma) {
-        case VH_F32: return corr_build_run<Ts, MmaF32, Tout>(E, C, H, W, a, b, L, levels, ws, s);
-        case VH_F16: return corr_build_run<Ts, MmaF16, Tout>(E, C, H, W, a, b, L, levels, ws, s);
-        case VH_BF16: return corr_build_run<Ts, MmaBF16, Tout>(E, C, H, W, a, b, L, levels, ws, s);
-        case VH_FP8E4M3: return corr_build_run<Ts, MmaFP8, Tout>(E, C, H, W, a, b, L, levels, ws, s);
+        case VH_F32: return corr_build_run<Ts, MmaF32, Tout>(E, C, H, W, a, b, L, levels, tiled, ws, s);
+        case VH_F16: return corr_build_run<Ts, MmaF16, Tout>(E, C, H, W, a, b, L, levels, tiled, ws, s);
+        case VH_BF16: return corr_build_run<Ts, MmaBF16, Tout>(E, C, H, W, a, b, L, levels, tiled, ws, s);
+        case VH_FP8E4M3: return corr_build_run<Ts, MmaFP8, Tout>(E, C, H, W, a, b, L, levels, tiled, ws, s);
         default: return fail(VH_ERR_UNSUPPORTED, "corr_build: contraction dtype %d", mma);
     }
 }
@@ -1103,15 +1182,19 @@ int vh_corr_index_backward(int dtype, int64_t B, int H1, int W1, int H2, int W2,
 }
 
 int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels, const void* const* levels,
-                           const int* Hl, const int* Wl, const float* coords, int radius, void* out, void* stream) {
+                           const int* Hl, const int* Wl, int layout, const float* coords, int radius, void* out,
+                           void* stream) {
     if (B < 0 || H1 < 0 || W1 < 0) return fail(VH_ERR_ARG, "corr_lookup_pyramid: negative size");
     if (num_levels < 1 || num_levels > 8) return fail(VH_ERR_ARG, "corr_lookup_pyramid: 1..8 levels");
     if (radius != 3) return fail(VH_ERR_UNSUPPORTED, "corr_lookup_pyramid: radius %d (built for r=3)", radius);
+    const bool tiled = layout == VH_LAYOUT_TILED;
+    if (layout != VH_LAYOUT_PLANAR && !tiled) return fail(VH_ERR_ARG, "corr_lookup_pyramid: layout %d", layout);
+    if (tiled && dtype == VH_F64) return fail(VH_ERR_UNSUPPORTED, "corr_lookup_pyramid: tiled f64 volume");
     hipStream_t s = (hipStream_t)stream;
     switch (dtype) {
-        case VH_F16: return corr_lookup_pyramid_launch<__half>(B, H1, W1, num_levels, levels, Hl, Wl, coords, out, s);
-        case VH_F32: return corr_lookup_pyramid_launch<float>(B, H1, W1, num_levels, levels, Hl, Wl, coords, out, s);
-        case VH_F64: return corr_lookup_pyramid_launch<double>(B, H1, W1, num_levels, levels, Hl, Wl, coords, out, s);
+        case VH_F16: return corr_lookup_pyramid_launch<__half>(B, H1, W1, num_levels, levels, Hl, Wl, tiled, coords, out, s);
+        case VH_F32: return corr_lookup_pyramid_launch<float>(B, H1, W1, num_levels, levels, Hl, Wl, tiled, coords, out, s);
+        case VH_F64: return corr_lookup_pyramid_launch<double>(B, H1, W1, num_levels, levels, Hl, Wl, tiled, coords, out, s);
         default: return fail(VH_ERR_UNSUPPORTED, "corr_lookup_pyramid: dtype %d", dtype);
     }
 }
@@ -1122,21 +1205,23 @@ size_t vh_corr_build_workspace(int mma_dtype, int64_t E, int C, int H, int W) {
 }
 
 int vh_corr_build(int in_dtype, int mma_dtype, int out_dtype, int64_t E, int C, int H, int W, const void* fmap1,
-                  const void* fmap2, int num_levels, void* const* levels, void* workspace, void* stream) {
+                  const void* fmap2, int num_levels, void* const* levels, int layout, void* workspace,
+                  void* stream) {
     if (E < 0 || C <= 0 || H <= 0 || W <= 0 || num_levels < 1)
         return fail(VH_ERR_ARG, "corr_build: bad sizes");
     if (C > 448) return fail(VH_ERR_UNSUPPORTED, "corr_build: %d channels (at most 448)", C);
     if (E == 0) return VH_OK;
     if (!workspace) return fail(VH_ERR_ARG, "corr_build: null workspace");
+    if (layout != VH_LAYOUT_PLANAR && layout != VH_LAYOUT_TILED) return fail(VH_ERR_ARG, "corr_build: layout %d", layout);
     hipStream_t s = (hipStream_t)stream;
     if (out_dtype == VH_F32) {
-        if (in_dtype == VH_F32) return corr_build_dispatch<float, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
-        if (in_dtype == VH_F16) return corr_build_dispatch<__half, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
-        if (in_dtype == VH_BF16) return corr_build_dispatch<__hip_bfloat16, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+        if (in_dtype == VH_F32) return corr_build_dispatch<float, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, layout == VH_LAYOUT_TILED, workspace, s);
+        if (in_dtype == VH_F16) return corr_build_dispatch<__half, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, layout == VH_LAYOUT_TILED, workspace, s);
+        if (in_dtype == VH_BF16) return corr_build_dispatch<__hip_bfloat16, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, layout == VH_LAYOUT_TILED, workspace, s);
     } else if (out_dtype == VH_F16) {
-        if (in_dtype == VH_F32) return corr_build_dispatch<float, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
-        if (in_dtype == VH_F16) return corr_build_dispatch<__half, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
-        if (in_dtype == VH_BF16) return corr_build_dispatch<__hip_bfloat16, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, workspace, s);
+        if (in_dtype == VH_F32) return corr_build_dispatch<float, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, layout == VH_LAYOUT_TILED, workspace, s);
+        if (in_dtype == VH_F16) return corr_build_dispatch<__half, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, layout == VH_LAYOUT_TILED, workspace, s);
+        if (in_dtype == VH_BF16) return corr_build_dispatch<__hip_bfloat16, __half>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, layout == VH_LAYOUT_TILED, workspace, s);
     }
     return fail(VH_ERR_UNSUPPORTED, "corr_build: dtypes in=%d out=%d", in_dtype, out_dtype);
 }

@@ -130,24 +130,54 @@ def altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius):
     return [corr]
 
 
-def corr_lookup_pyramid(levels, coords, radius):
-    """Fused CorrBlock lookup: levels[l] [B,H1,W1,Hl,Wl], coords [B,H1,W1,2] (level-0
-    pixels) -> [B, L*(2r+1)^2, H1, W1] in the volume dtype."""
+def tiled_plane(Hl, Wl, dtype):
+    """(Hp, Wp, tw) of a tiled volume plane: 4-row x tw-column tiles of 64 bytes."""
+    tw = 16 // torch.empty((), dtype=dtype).element_size()
+    return (Hl + 3) // 4 * 4, (Wl + tw - 1) // tw * tw, tw
+
+
+def untile(level, Hl, Wl):
+    """Tiled level [B,H1,W1,Hp*Wp] -> planar [B,H1,W1,Hl,Wl] (copy)."""
+    B, H1, W1, _ = level.shape
+    Hp, Wp, tw = tiled_plane(Hl, Wl, level.dtype)
+    v = level.view(B, H1, W1, Hp // 4, Wp // tw, 4, tw).permute(0, 1, 2, 3, 5, 4, 6)
+    return v.reshape(B, H1, W1, Hp, Wp)[..., :Hl, :Wl].contiguous()
+
+
+def tile(volume):
+    """Planar [B,H1,W1,Hl,Wl] -> tiled [B,H1,W1,Hp*Wp] with zero padding (copy)."""
+    B, H1, W1, Hl, Wl = volume.shape
+    Hp, Wp, tw = tiled_plane(Hl, Wl, volume.dtype)
+    pad = torch.zeros((B, H1, W1, Hp, Wp), dtype=volume.dtype, device=volume.device)
+    pad[..., :Hl, :Wl] = volume
+    v = pad.view(B, H1, W1, Hp // 4, 4, Wp // tw, tw).permute(0, 1, 2, 3, 5, 4, 6)
+    return v.reshape(B, H1, W1, Hp * Wp).contiguous()
+
+
+def corr_lookup_pyramid(levels, coords, radius, level_dims=None, layout="planar"):
+    """Fused CorrBlock lookup: levels[l] [B,H1,W1,Hl,Wl] (planar) or [B,H1,W1,Hp*Wp]
+    (tiled, with level_dims[l] = (Hl, Wl)), coords [B,H1,W1,2] (level-0 pixels)
+    -> [B, L*(2r+1)^2, H1, W1] in the volume dtype."""
     _contig(coords=coords)
     for lvl in levels:
         _contig(volume=lvl)
     _gpu(coords, *levels)
     if coords.dtype != torch.float32:
         raise RuntimeError("coords must be float32")
+    tiled = layout == "tiled"
+    if tiled and level_dims is None:
+        raise RuntimeError("tiled levels need level_dims")
     B, H1, W1 = levels[0].shape[:3]
     L = len(levels)
     rd = 2 * int(radius) + 1
     out = torch.empty((B, L * rd * rd, H1, W1), dtype=levels[0].dtype, device=coords.device)
     lv = (C.c_void_p * L)(*[t.data_ptr() for t in levels])
-    Hl = (C.c_int * L)(*[t.shape[3] for t in levels])
-    Wl = (C.c_int * L)(*[t.shape[4] for t in levels])
+    dims = level_dims if tiled else [(t.shape[3], t.shape[4]) for t in levels]
+    Hl = (C.c_int * L)(*[d[0] for d in dims])
+    Wl = (C.c_int * L)(*[d[1] for d in dims])
     with torch.cuda.device(coords.device):
         rc = _lib.load().vh_corr_lookup_pyramid(_code(levels[0], _VOL_DT), B, H1, W1, L, lv, Hl, Wl,
+                                                _lib.VH_LAYOUT_TILED if tiled else _lib.VH_LAYOUT_PLANAR,
                                                 _lib.ptr(coords), int(radius), _lib.ptr(out),
                                                 _lib.stream_of(coords))
     _lib.check(rc, "corr_lookup_pyramid")
@@ -158,13 +188,14 @@ _MMA_CODE = {torch.float32: _lib.VH_F32, torch.float16: _lib.VH_F16, torch.bfloa
              torch.float8_e4m3fn: _lib.VH_FP8E4M3}
 
 
-def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None, contraction_dtype=None):
+def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None, contraction_dtype=None, layout="planar"):
     """All-pairs correlation volume pyramid of CorrBlock.  fmap1, fmap2 [E,C,H,W]
     (f32 / f16 / bf16) -> list of level-l volumes [E,H,W,H>>l,W>>l] in out_dtype
     (default: f16 for 16-bit inputs, f32 for f32 inputs).  contraction_dtype
     selects the MFMA operand type (default: the input dtype); float8_e4m3fn
     balances each channel between the two maps and scales each pixel row into
-    e4m3, accumulating in FP32."""
+    e4m3, accumulating in FP32.  layout "tiled" stores each level's planes as
+    64-byte tiles, [E,H,W,Hp*Wp] (see tiled_plane / untile)."""
     _contig(fmap1=fmap1, fmap2=fmap2)
     _gpu(fmap1, fmap2)
     if fmap2.shape != fmap1.shape or fmap2.dtype != fmap1.dtype:
@@ -173,10 +204,15 @@ def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None, contraction_dtype=Non
     if out_dtype is None:
         out_dtype = torch.float32 if fmap1.dtype == torch.float32 else torch.float16
     mma = _MMA_CODE.get(contraction_dtype if contraction_dtype is not None else fmap1.dtype, -1)
+    tiled = layout == "tiled"
     levels = []
     h, w = H, W
     for _ in range(num_levels):
-        levels.append(torch.empty((E, H, W, h, w), dtype=out_dtype, device=fmap1.device))
+        if tiled:
+            Hp, Wp, _ = tiled_plane(h, w, out_dtype)
+            levels.append(torch.empty((E, H, W, Hp * Wp), dtype=out_dtype, device=fmap1.device))
+        else:
+            levels.append(torch.empty((E, H, W, h, w), dtype=out_dtype, device=fmap1.device))
         h, w = h // 2, w // 2
     lib = _lib.load()
     ws = torch.empty((max(int(lib.vh_corr_build_workspace(mma, E, Cc, H, W)), 1),), dtype=torch.uint8,
@@ -185,6 +221,7 @@ def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None, contraction_dtype=Non
     with torch.cuda.device(fmap1.device):
         rc = lib.vh_corr_build(_code(fmap1, (torch.float32, torch.float16, torch.bfloat16)), mma,
                                _lib.DTYPE_CODE.get(out_dtype, -1), E, Cc, H, W, _lib.ptr(fmap1), _lib.ptr(fmap2),
-                               num_levels, lv, _lib.ptr(ws), _lib.stream_of(fmap1))
+                               num_levels, lv, _lib.VH_LAYOUT_TILED if tiled else _lib.VH_LAYOUT_PLANAR,
+                               _lib.ptr(ws), _lib.stream_of(fmap1))
     _lib.check(rc, "corr_build")
     return levels
