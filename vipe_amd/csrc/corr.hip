@@ -101,6 +101,8 @@ __device__ __forceinline__ void lookup_level(const T* __restrict__ plane, int H2
 // 64-byte tile each (tw = 4 for f32, 8 for f16); the padding holds zeros.  A
 // lookup window then touches whole 64-byte sectors, and the build writes
 // whole tiles.
+constexpr int kTileBytes = 64;  // tile size in bytes (measured against 128: equal for f32, faster for f16)
+
 struct LevelGeom {
     int Hl, Wl, Hp, Wp, tw;
     int tiled;
@@ -116,7 +118,7 @@ inline LevelGeom level_geom(int Hl, int Wl, int elem, bool tiled) {
     g.Hl = Hl;
     g.Wl = Wl;
     g.tiled = tiled ? 1 : 0;
-    g.tw = 16 / elem;
+    g.tw = kTileBytes / 4 / elem;
     g.Hp = tiled ? (Hl + 3) / 4 * 4 : Hl;
     g.Wp = tiled ? (Wl + g.tw - 1) / g.tw * g.tw : Wl;
     g.plane = (int64_t)g.Hp * g.Wp;
@@ -164,7 +166,7 @@ __device__ __forceinline__ void load_tap_row_vec(const T* __restrict__ plane, in
     using V = typename VecOf<T, VEC>::type;
     constexpr int NT = 2 * R + 2;
     constexpr int NL = (NT + VEC - 1 + VEC - 1) / VEC;  // vectors covering NT taps at any shift < VEC
-    constexpr int TW = 16 / (int)sizeof(T);              // tile width (tiled layout)
+    constexpr int TW = kTileBytes / 4 / (int)sizeof(T);  // tile width (tiled layout)
     // tiled: H2, W2 are the padded sizes, whose extra rows and columns hold zeros
     const bool yok = (y1 >= 0) && (y1 < H2);
     const int yy = yok ? y1 : 0;

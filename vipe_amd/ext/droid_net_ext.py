@@ -130,9 +130,12 @@ def altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius):
     return [corr]
 
 
+TILE_BYTES = 64  # matches kTileBytes in corr.hip
+
+
 def tiled_plane(Hl, Wl, dtype):
     """(Hp, Wp, tw) of a tiled volume plane: 4-row x tw-column tiles of 64 bytes."""
-    tw = 16 // torch.empty((), dtype=dtype).element_size()
+    tw = TILE_BYTES // 4 // torch.empty((), dtype=dtype).element_size()
     return (Hl + 3) // 4 * 4, (Wl + tw - 1) // tw * tw, tw
 
 
