@@ -48,6 +48,7 @@ constexpr int kNV = 40;       // per-edge partial stride (floats)
 constexpr int kEDS = 136;     // per-edge finalised record (doubles)
 constexpr int kCB = 64;       // Cholesky block size
 constexpr int kMaxLd = 2048;  // leading dimension of the reduced system (6P+1 rounded up to kCB)
+constexpr int kGramSplit = 4; // finalize workgroups per depth frame
 constexpr int kLdsBudget = 56 * 1024;  // dynamic E-row tile; static LDS stays under 8 KB
 
 // per-edge partial layout (floats)
@@ -472,9 +473,15 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
     if (bid < p.E) {
         const int e = bid;
         if (tid < kNV) {
-            double acc = 0.0;
-            for (int t = 0; t < p.T; t++) acc += (double)w.edge_part[((int64_t)t * p.E + e) * kNV + tid];
-            s[tid] = acc;
+            // four independent chains keep the tile loads in flight
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            int t = 0;
+            for (; t + 4 <= p.T; t += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) acc[u] += (double)w.edge_part[((int64_t)(t + u) * p.E + e) * kNV + tid];
+            }
+            for (; t < p.T; t++) acc[0] += (double)w.edge_part[((int64_t)t * p.E + e) * kNV + tid];
+            s[tid] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         if (tid == 0) {
             float Rm[9], t[3];
@@ -536,15 +543,25 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
         return;
     }
     if (motion) return;
-    const int k = bid - p.E;
+    // kGramSplit workgroups per depth frame, each a contiguous share of its Gram
+    const int k = (bid - p.E) / kGramSplit, part = (bid - p.E) % kGramSplit;
     const int off = p.gram_off[k];
     const int nv = p.gram_off[k + 1] - off;
-    for (int i = tid; i < nv; i += kThreads) {
-        double acc = 0.0;
-        for (int t = 0; t < p.T; t++) acc += (double)w.gram_part[(int64_t)t * p.GT + off + i];
-        w.gd[off + i] = acc;
+    const int per = (nv + kGramSplit - 1) / kGramSplit;
+    const int i0 = part * per, i1 = min(nv, i0 + per);
+    for (int i = i0 + tid; i < i1; i += 4 * kThreads) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < p.T; t++) {
+            const float* gp = w.gram_part + (int64_t)t * p.GT + off;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (i + u * kThreads < i1) acc[u] += (double)gp[i + u * kThreads];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (i + u * kThreads < i1) w.gd[off + i + u * kThreads] = acc[u];
     }
-    if (tid == 0) {
+    if (part == 0 && tid == 0) {
         double acc = 0.0;
         for (int t = 0; t < p.T; t++) acc += (double)w.fen_part[(int64_t)t * p.K + k];
         w.fen[k] = acc;
@@ -1908,14 +1925,14 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
                 else hipLaunchKernelGGL((ba_frame_kernel<false, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
             }
             VH_LAUNCH_CHECK("ba_frame_kernel");
-            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + (motion ? 0 : d.K)), dim3(kThreads), 0, s, d, a, w,
-                               motion ? 1 : 0);
+            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + (motion ? 0 : d.K * kGramSplit)), dim3(kThreads), 0, s,
+                               d, a, w, motion ? 1 : 0);
             VH_LAUNCH_CHECK("ba_finalize_kernel");
         } else if (!motion && d.K > 0) {
             // no edges: every depth frame still has its prior / damping terms
             hipLaunchKernelGGL((ba_frame_kernel<false, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
             VH_LAUNCH_CHECK("ba_frame_kernel");
-            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + d.K), dim3(kThreads), 0, s, d, a, w, 0);
+            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + d.K * kGramSplit), dim3(kThreads), 0, s, d, a, w, 0);
             VH_LAUNCH_CHECK("ba_finalize_kernel");
         }
         hipLaunchKernelGGL(ba_assemble_kernel, dim3(d.nlb + d.P + 1 + npad), dim3(64), 0, s, d, a, w,
