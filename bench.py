@@ -153,7 +153,8 @@ def main():
     elem = 4 if vol_dtype == torch.float32 else 2
     lk_alg = lookup_bytes(E, H, W, 4, elem)
     lk_gbs = lk_alg / (lk_ms * 1e-3)
-    lk_kernel = f"corr_lookup_pyramid_kernel<{'float' if elem == 4 else '__half'}, 3, {str(corr.layout == 'tiled').lower()}>"
+    lk_kernel = (f"corr_lookup_tiled_kernel<{'float' if elem == 4 else '__half'}, 3>" if corr.layout == "tiled"
+                 else f"corr_lookup_pyramid_kernel<{'float' if elem == 4 else '__half'}, 3, false>")
 
     # BA-only timing (same inputs, separate region)
     d["poses"].copy_(poses0)

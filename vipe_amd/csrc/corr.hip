@@ -12,6 +12,7 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -411,6 +412,138 @@ __global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, 
     }
 }
 
+// Tiled pyramid lookup through buffer resources.  One lane per source pixel.
+// Per level a lane issues every load of its (2r+2)-row window before it uses
+// any of them, so the four rows that share a 64-byte tile are fetched back to
+// back; read row by row, a tile could leave L2 between its rows and be fetched
+// again.  The load resource starts at the wave's first plane and a lane's
+// plane is lane * plane_bytes into it; a vector outside the plane gets an
+// offset past the resource's range, which reads zeros without touching memory.
+// The outputs go through a second resource: one 32-bit lane offset, and the
+// channel offset c * H1 * W1 in a scalar register.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint32_t kBufRange = 0x80000000u;    // num_records of both resources (bytes)
+constexpr uint32_t kBufInvalid = 0x80000000u;  // offset of an out-of-plane vector: past kBufRange
+constexpr int kBufWord3 = 0x00020000;          // word 3 of a gfx9 raw buffer resource
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)kBufRange, kBufWord3);
+}
+
+// the 2 (f32) or 4 (f16) elements of one dword as floats
+template <typename T> struct TapDword;
+template <> struct TapDword<float> {
+    static constexpr int n = 1;
+    static __device__ __forceinline__ void unpack(uint32_t u, float* e) { e[0] = __uint_as_float(u); }
+    static __device__ __forceinline__ void store(float v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo, so, 0);
+    }
+};
+template <> struct TapDword<__half> {
+    static constexpr int n = 2;
+    static __device__ __forceinline__ void unpack(uint32_t u, float* e) {
+        const float2 f = __half22float2(*reinterpret_cast<const __half2*>(&u));
+        e[0] = f.x;
+        e[1] = f.y;
+    }
+    static __device__ __forceinline__ void store(float v, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+        __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half(v)), r, vo, so, 0);
+    }
+};
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void corr_lookup_tiled_kernel(LevelPtrs lv, int L, const float* __restrict__ coords,
+                                                                T* __restrict__ out, int64_t B, int H1, int W1) {
+    constexpr int RD = 2 * R + 1, NT = 2 * R + 2;
+    constexpr int TW = kTileBytes / 4 / (int)sizeof(T);  // elements of one 16-byte tile row
+    constexpr int LOG_TW = TW == 4 ? 2 : 3;
+    constexpr int NL = (NT + TW - 1 + TW - 1) / TW;      // 16-byte vectors per tap row at any shift
+    constexpr int NE = NL * TW;                          // elements loaded per tap row
+    const int64_t HW1 = (int64_t)H1 * W1;
+    const int lane = threadIdx.x & 63;
+    const int64_t pix0 = (int64_t)blockIdx.x * 256 + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63);
+    const int64_t pix = pix0 + lane;
+    if (pix >= B * HW1) return;
+    const int64_t b0 = pix0 / HW1;
+    const int64_t b = pix / HW1;
+    const int64_t yx = pix - b * HW1;
+    const float cx = coords[pix * 2 + 0], cy = coords[pix * 2 + 1];
+    const __amdgpu_buffer_rsrc_t ro = buf_rsrc(out + b0 * (int64_t)L * RD * RD * HW1);
+    const uint32_t ovo = (uint32_t)(((b - b0) * (int64_t)L * RD * RD * HW1 + yx) * (int64_t)sizeof(T));
+#pragma unroll 1
+    for (int l = 0; l < L; l++) {
+        const float sc = 1.0f / (float)(1 << l);
+        const int Hp = lv.h[l], Wp = lv.w[l], ntx = Wp >> LOG_TW;
+        const int64_t plane = (int64_t)Hp * Wp;
+        const __amdgpu_buffer_rsrc_t rl = buf_rsrc((const T*)lv.p[l] + pix0 * plane);
+        const uint32_t lvo = (uint32_t)(lane * plane * (int64_t)sizeof(T));
+        const float x0 = cx * sc, y0 = cy * sc;
+        const float fx0 = floorf(x0), fy0 = floorf(y0);
+        const float dx = x0 - fx0, dy = y0 - fy0;
+        const int xb = (int)fx0 - R, yb = (int)fy0 - R;
+        const int xt = xb >> LOG_TW, sh = xb - (xt << LOG_TW);  // first tile column, shift 0 .. TW-1
+        u32x4 g[NT][NL];
+#pragma unroll
+        for (int r = 0; r < NT; r++) {
+            const int y = yb + r;
+            const bool yok = (unsigned)y < (unsigned)Hp;
+            const uint32_t rowoff = lvo + (uint32_t)(((y >> 2) * ntx) * kTileBytes + (y & 3) * 16);
+#pragma unroll
+            for (int q = 0; q < NL; q++) {
+                const bool ok = yok && (unsigned)(xt + q) < (unsigned)ntx;
+                g[r][q] = __builtin_amdgcn_raw_buffer_load_b128(rl, ok ? rowoff + (uint32_t)(xt + q) * kTileBytes
+                                                                       : kBufInvalid, 0, 0);
+            }
+        }
+        const float w00 = (1.f - dx) * (1.f - dy), w10 = dx * (1.f - dy), w01 = (1.f - dx) * dy, w11 = dx * dy;
+        float top[NT], bot[NT];
+#pragma unroll
+        for (int r = 0; r < NT; r++) {
+            float e[NE];
+#pragma unroll
+            for (int q = 0; q < NL; q++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) TapDword<T>::unpack(g[r][q][k], e + (q * 4 + k) * TapDword<T>::n);
+            // taps e[sh .. sh+NT-1] by select stages of 1, 2 (and 4); the empty asm
+            // keeps per-lane selects rather than a dynamically indexed array
+            float h[NE];
+#pragma unroll
+            for (int k = 0; k < NE; k++) h[k] = e[k];
+#pragma unroll
+            for (int st = 1; st < TW; st <<= 1) {
+#pragma unroll
+                for (int k = 0; k + st < NE; k++) {
+                    float a = h[k], bb = h[k + st];
+                    asm("" : "+v"(a), "+v"(bb));
+                    h[k] = (sh & st) ? bb : a;
+                }
+            }
+            float(&cur)[NT] = (r & 1) ? bot : top;
+#pragma unroll
+            for (int i = 0; i < NT; i++) cur[i] = h[i];
+            if (r == 0) continue;
+            // output row j = r - 1 from tap rows r - 1 and r
+            const int j = r - 1;
+            const float(&t0)[NT] = (j & 1) ? bot : top;
+            const float(&t1)[NT] = (j & 1) ? top : bot;
+#pragma unroll
+            for (int i = 0; i < RD; i++) {
+                const float v = t0[i] * w00 + t0[i + 1] * w10 + t1[i] * w01 + t1[i + 1] * w11;
+                TapDword<T>::store(v, ro, ovo, (uint32_t)(((l * RD + i) * RD + j) * HW1 * (int64_t)sizeof(T)));
+            }
+        }
+    }
+}
+
+// Whether corr_lookup_tiled_kernel's 32-bit buffer offsets cover this shape: a
+// wave's 64 planes of the largest level, and the outputs of the batches a wave
+// can span.
+inline bool lookup_tiled_offsets_fit(int64_t B, int64_t HW1, int L, int rd2, int64_t plane0_bytes, int elem) {
+    const int64_t nb = std::min<int64_t>(B, 64 / std::max<int64_t>(HW1, 1) + 2);
+    return 64 * plane0_bytes < (int64_t)kBufRange - kTileBytes &&
+           nb * L * rd2 * HW1 * elem < (int64_t)kBufRange;
+}
+
 template <typename T>
 int corr_lookup_pyramid_launch(int64_t B, int H1, int W1, int L, const void* const* levels, const int* Hl,
                                const int* Wl, bool tiled, const float* coords, void* out, hipStream_t s) {
@@ -432,8 +565,13 @@ int corr_lookup_pyramid_launch(int64_t B, int H1, int W1, int L, const void* con
     if (n == 0) return VH_OK;
     if constexpr (std::is_same<T, float>::value || std::is_same<T, __half>::value) {
         if (tiled) {
-            hipLaunchKernelGGL((corr_lookup_pyramid_kernel<T, 3, true>), dim3(ceil_div(n, 256)), dim3(256), 0, s, lv,
-                               L, coords, (T*)out, B, H1, W1);
+            if (lookup_tiled_offsets_fit(B, (int64_t)H1 * W1, L, 49, (int64_t)lv.h[0] * lv.w[0] * (int)sizeof(T),
+                                         (int)sizeof(T)))
+                hipLaunchKernelGGL((corr_lookup_tiled_kernel<T, 3>), dim3(ceil_div(n, 256)), dim3(256), 0, s, lv, L,
+                                   coords, (T*)out, B, H1, W1);
+            else
+                hipLaunchKernelGGL((corr_lookup_pyramid_kernel<T, 3, true>), dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                                   lv, L, coords, (T*)out, B, H1, W1);
             VH_LAUNCH_CHECK("corr_lookup_pyramid");
             return VH_OK;
         }
