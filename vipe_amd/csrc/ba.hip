@@ -705,129 +705,189 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4f64 takes A[i][k] from lane i + 16 k and B[k][j] from lane
+// j + 16 k, and leaves D[i][j] in register i / 4 of lane j + 16 (i % 4).
+// One wave: acc += P Q over k < K (a multiple of 4) for a 16 x 16 tile, with
+// P[i][k] at p[i * ps + k] and Q[k][j] at q[k * qs + j], or at q[j * qs + k]
+// when QT.
+template <bool QT>
+__device__ __forceinline__ f64x4 mfma_tile16(const double* p, int ps, const double* q, int qs, int K, int lane,
+                                             f64x4 acc) {
+    const int i = lane & 15, k = lane >> 4;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const double a = p[i * ps + k0 + k];
+        const double b = QT ? q[i * qs + k0 + k] : q[(k0 + k) * qs + i];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Wave 0: factor the 16-column panel at column o (rows o..63, row r in lane r).
+// The pivot chain never leaves registers: the next pivot is lane (o+j+1)'s own
+// entry after the step's update, read with readlane; the other columns of the
+// step are broadcast through LDS, which a single wavefront reads back in
+// program order without a barrier.  Writes L's panel to S and 1/L_jj to invd.
+__device__ __forceinline__ void factor_panel(double (*S)[kCB + 1], FactorLds& F, int o, int lane, bool& bad) {
+    double v[kSB];
+#pragma unroll
+    for (int m = 0; m < kSB; m++) v[m] = (lane >= o + m) ? S[lane][o + m] : 0.0;
+    double dn = readlane_f64(v[0], o);
+#pragma unroll
+    for (int j = 0; j < kSB; j++) {
+        double* cb = F.col[j & 1];
+        cb[lane] = v[j];  // column o+j below the pivot (rows above it are ignored)
+        const double d = dn;
+        bad |= (d <= 0.0);  // a NaN pivot propagates, like the reference's LLT
+        const double inv = rsqrt_nr(d);
+        const double sc = v[j] * (inv * inv);  // A'[r][o+j] / d
+        // next pivot: row o+j+1 updates its own diagonal with its own entry
+        if (j + 1 < kSB) dn = readlane_f64(v[j + 1] - sc * v[j], o + j + 1);
+#pragma unroll
+        for (int c = j + 1; c < kSB; c++) v[c] -= sc * cb[o + c];
+        v[j] *= inv;
+        if (lane == 0) F.invd[o + j] = inv;
+    }
+#pragma unroll
+    for (int m = 0; m < kSB; m++)
+        if (lane >= o + m) S[lane][o + m] = v[m];
+}
+
+// S[r][c] -= sum_{m in [p0, p0+16)} S[r][m] S[c][m] over 4 x 4 tiles with
+// c in [c_lo, c_hi) and r in [c_lo, 64), tiles wholly above the diagonal
+// skipped; thread t of nt.
+__device__ __forceinline__ void panel_update(double (*S)[kCB + 1], int p0, int c_lo, int c_hi, int t, int nt) {
+    const int ntc = (c_hi - c_lo) / 4, ntr = (kCB - c_lo) / 4;
+    for (int task = t; task < ntc * ntr; task += nt) {
+        const int tc = c_lo + 4 * (task % ntc), tr = c_lo + 4 * (task / ntc);
+        if (tr < tc) continue;
+        double acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+#pragma unroll
+        for (int m = 0; m < kSB; m++) {
+            double a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                a[i] = S[tr + i][p0 + m];
+                bb[i] = S[tc + i][p0 + m];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * bb[j];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) S[tr + i][tc + j] -= acc[i][j];
+    }
+}
+
+// One wave: M_tt = L_tt^-1 for the 16 x 16 diagonal block at o.  Lane r < 16
+// solves L_tt x = e_r by forward substitution (column r of M_tt).
+__device__ __forceinline__ void diag_inverse(const double (*S)[kCB + 1], double (*M)[kCB + 1], const FactorLds& F,
+                                             int o, int lane) {
+    const int r = lane & (kSB - 1);
+    double lq[kSB][kSB];  // L_tt[c][j], broadcast reads
+    double x[kSB];
+#pragma unroll
+    for (int c = 0; c < kSB; c++) {
+        x[c] = (c == r) ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = 0; j < c; j++) lq[c][j] = S[o + c][o + j];
+    }
+#pragma unroll
+    for (int j = 0; j < kSB; j++) {
+        const double xj = x[j] * F.invd[o + j];
+        x[j] = xj;
+#pragma unroll
+        for (int c = j + 1; c < kSB; c++) x[c] -= xj * lq[c][j];
+    }
+    if (lane < kSB) {
+#pragma unroll
+        for (int c = 0; c < kSB; c++) M[o + c][o + r] = x[c];
+    }
+}
+
+// T_q = sum_{p=q}^{t-1} L_tp M_pq for q < t (16 x 16 blocks), thread u of nu.
+__device__ __forceinline__ void inverse_row_partial(const double (*S)[kCB + 1], const double (*M)[kCB + 1],
+                                                    FactorLds& F, int t, int u, int nu) {
+    for (int e = u; e < t * kSB * kSB; e += nu) {
+        const int q = e >> 8, i = (e >> 4) & 15, j = e & 15;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = kSB * q; k < kSB * t; k += 4) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) acc[m] += S[kSB * t + i][k + m] * M[k + m][kSB * q + j];
+        }
+        F.T[q][i][j] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    }
+}
+
 // Factor the 64 x 64 block whose lower triangle is in S (256 threads) and
 // write X = L^-T (row-major) to Xout.  S receives L; M is scratch.
-//   per 16-column panel s: wave 0 factors rows o..63 of the panel with row r in
-//     lane r.  The pivot chain never leaves registers: the next pivot is lane
-//     (o+j+1)'s own entry after the step's update, read with readlane; the
-//     other columns of the step are broadcast through LDS, which a single
-//     wavefront reads back in program order without a barrier.  Then all
-//     waves apply the trailing update A_tt -= L_ts L_ts^T with 4 x 4 tiles.
-//   M = L^-1 by 16 x 16 blocks: M_qq by substitution (wave q), then
-//     M_ts = -M_tt sum_{q=s}^{t-1} L_tq M_qs by block diagonals.
+// Four rounds, one per 16-column panel s (o = 16 s):
+//   A: wave 0 factors panel s.  Meanwhile, for t = s - 1, wave 1 forms the
+//      diagonal inverse M_tt, and waves 2-3 apply panel t's update to the
+//      columns right of panel s and form T_q = sum_{p=q}^{t-1} L_tp M_pq.
+//   B: one 16 x 16 matrix-core tile per wave: panel s's update of panel s+1's
+//      columns (3 - s tiles), and block row t of M = L^-1: M_tq = -M_tt T_q.
+// A last round does block row 3 of M, then X = M^T is stored.  Entries above
+// the diagonal of S are scratch (the diagonal tiles update them too).
 __device__ __forceinline__ void factor_block(double (*S)[kCB + 1], double (*M)[kCB + 1], FactorLds& F,
                                              double* __restrict__ Xout, int* flag) {
+    constexpr int LS = kCB + 1, LT = kSB + 1;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    const int ti = lane & 15, tk = lane >> 4;                 // tile element (4 r + tk, ti) in register r
+    const f64x4 zero = {0.0, 0.0, 0.0, 0.0};
     bool bad = false;
 #pragma unroll
-    for (int sb = 0; sb < kCB / kSB; sb++) {
-        const int o = sb * kSB;
+    for (int s = 0; s < kCB / kSB; s++) {
+        const int o = s * kSB, t = s - 1;
         if (wv == 0) {
-            double v[kSB];
-#pragma unroll
-            for (int m = 0; m < kSB; m++) v[m] = (lane >= o + m) ? S[lane][o + m] : 0.0;
-            double dn = readlane_f64(v[0], o);
-#pragma unroll
-            for (int j = 0; j < kSB; j++) {
-                double* cb = F.col[j & 1];
-                cb[lane] = v[j];  // column o+j below the pivot (rows above it are ignored)
-                const double d = dn;
-                bad |= (d <= 0.0);  // a NaN pivot propagates, like the reference's LLT
-                const double inv = rsqrt_nr(d);
-                const double s = v[j] * (inv * inv);  // A'[r][o+j] / d
-                // next pivot: row o+j+1 updates its own diagonal with its own entry
-                if (j + 1 < kSB) dn = readlane_f64(v[j + 1] - s * v[j], o + j + 1);
-#pragma unroll
-                for (int c = j + 1; c < kSB; c++) v[c] -= s * cb[o + c];
-                v[j] *= inv;
-                if (lane == 0) F.invd[o + j] = inv;
-            }
-#pragma unroll
-            for (int m = 0; m < kSB; m++)
-                if (lane >= o + m) S[lane][o + m] = v[m];
+            factor_panel(S, F, o, lane, bad);
+        } else if (wv == 1) {
+            if (t >= 0) diag_inverse(S, M, F, kSB * t, lane);
+        } else {
+            if (t >= 0 && o + kSB < kCB) panel_update(S, kSB * t, o + kSB, kCB, tid - 128, 128);
+            if (t >= 1) inverse_row_partial(S, M, F, t, tid - 128, 128);
         }
         __syncthreads();
-        const int t0 = o + kSB, nt4 = (kCB - t0) / 4;
-        if (nt4 > 0 && tid < nt4 * nt4) {
-            const int tr = t0 + 4 * (tid / nt4), tc = t0 + 4 * (tid % nt4);
-            if (tc <= tr) {
-                double acc[4][4];
+        const int npu = (s + 1 < kCB / kSB) ? kCB / kSB - 1 - s : 0;
+        if (wv < npu) {
+            const int r0 = o + kSB * (wv + 1), c0 = o + kSB;
+            const f64x4 acc = mfma_tile16<true>(&S[r0][o], LS, &S[c0][o], LS, kSB, lane, zero);
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+            for (int r = 0; r < 4; r++) S[r0 + 4 * r + tk][c0 + ti] -= acc[r];
+        } else if (wv - npu < t) {
+            const int q = wv - npu;
+            const f64x4 acc = mfma_tile16<false>(&M[kSB * t][kSB * t], LS, &F.T[q][0][0], LT, kSB, lane, zero);
 #pragma unroll
-                    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-#pragma unroll
-                for (int m = 0; m < kSB; m++) {
-                    double a[4], bb[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        a[i] = S[tr + i][o + m];
-                        bb[i] = S[tc + i][o + m];
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * bb[j];
-                }
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) S[tr + i][tc + j] -= acc[i][j];
-            }
+            for (int r = 0; r < 4; r++) M[kSB * t + 4 * r + tk][kSB * q + ti] = -acc[r];
         }
         __syncthreads();
     }
     if (tid == 0 && bad) atomicExch(flag, 1);
-    {
-        // wave q: row r of L_qq^-T solves x L_qq^T = e_r; M_qq = (L_qq^-T)^T
-        const int o = kSB * wv, r = lane & (kSB - 1);
-        double lq[kSB][kSB];  // L_qq[c][j], broadcast reads
-        double x[kSB];
+    constexpr int tl = kCB - kSB;  // last block row
+    if (wv == 1) {
+        diag_inverse(S, M, F, tl, lane);
+    } else {
+        const int q = wv == 0 ? 0 : wv - 1;
+        const f64x4 acc = mfma_tile16<false>(&S[tl][kSB * q], LS, &M[kSB * q][kSB * q], LS, tl - kSB * q, lane, zero);
 #pragma unroll
-        for (int c = 0; c < kSB; c++) {
-            x[c] = (c == r) ? 1.0 : 0.0;
-#pragma unroll
-            for (int j = 0; j < c; j++) lq[c][j] = S[o + c][o + j];
-        }
-#pragma unroll
-        for (int j = 0; j < kSB; j++) {
-            const double xj = x[j] * F.invd[o + j];
-            x[j] = xj;
-#pragma unroll
-            for (int c = j + 1; c < kSB; c++) x[c] -= xj * lq[c][j];
-        }
-        if (lane < kSB) {
-#pragma unroll
-            for (int c = 0; c < kSB; c++) M[o + c][o + r] = x[c];
-        }
+        for (int r = 0; r < 4; r++) F.T[q][4 * r + tk][ti] = acc[r];
     }
     __syncthreads();
-    const int rr = tid >> 4, cc = tid & 15;
+    if (wv < 3) {
+        const f64x4 acc = mfma_tile16<false>(&M[tl][tl], LS, &F.T[wv][0][0], LT, kSB, lane, zero);
 #pragma unroll
-    for (int dd = 1; dd < 4; dd++) {
-#pragma unroll
-        for (int bq = 0; bq < 4 - dd; bq++) {
-            const int bi = bq + dd, bj = bq;
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int k = bj; k < bi; k++)
-#pragma unroll
-                for (int m = 0; m < kSB; m++)
-                    acc[m & 3] += S[bi * kSB + rr][k * kSB + m] * M[k * kSB + m][bj * kSB + cc];
-            F.T[bq][rr][cc] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int bq = 0; bq < 4 - dd; bq++) {
-            const int bi = bq + dd, bj = bq;
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int m = 0; m < kSB; m++) acc[m & 3] += M[bi * kSB + rr][bi * kSB + m] * F.T[bq][m][cc];
-            M[bi * kSB + rr][bj * kSB + cc] = -((acc[0] + acc[1]) + (acc[2] + acc[3]));
-        }
-        __syncthreads();
+        for (int r = 0; r < 4; r++) M[tl + 4 * r + tk][kSB * wv + ti] = -acc[r];
     }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const int e = tid + q * 256;
@@ -855,25 +915,28 @@ __device__ __forceinline__ void load_block(const double* __restrict__ src, int64
     }
 }
 
-// 4 x 4 outputs per thread of C = P^T Q with P, Q in LDS: C[r][c] = sum_m P[m][r] Q[m][c]
-__device__ __forceinline__ void gemm_tn_4x4(const double (*P)[kCB + 1], const double (*Q)[kCB + 1], int tr, int tc,
-                                            double (&acc)[4][4]) {
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-#pragma unroll 8
-    for (int m = 0; m < kCB; m++) {
-        double a[4], q[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            a[i] = P[m][tr + i];
-            q[i] = Q[m][tc + i];
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] += a[i] * q[j];
+// 64 x 64 x 64 FP64 products on the matrix cores (layout at mfma_tile16).
+// Wave w owns the 32 x 32 quadrant (w / 2, w % 2) of the result as 2 x 2 tiles.
+
+// row / column of element r of tile (ta, tb) of quadrant (qr, qc) in this lane
+__device__ __forceinline__ int quad_row(int qr, int ta, int r, int lane) { return 32 * qr + 16 * ta + 4 * r + (lane >> 4); }
+__device__ __forceinline__ int quad_col(int qc, int tb, int lane) { return 32 * qc + 16 * tb + (lane & 15); }
+
+// acc += P Q (BT false: Q[m][c]) or P Q^T (BT true: Q[c][m]) over the wave's
+// quadrant, with P[r][m] and Q in LDS.
+template <bool BT>
+__device__ __forceinline__ void mfma_quad_64(const double (*P)[kCB + 1], const double (*Q)[kCB + 1], int qr, int qc,
+                                             int lane, f64x4 (&acc)[2][2]) {
+    const int i = lane & 15, k = lane >> 4;
+#pragma unroll 4
+    for (int m0 = 0; m0 < kCB; m0 += 4) {
+        const double a0 = P[32 * qr + i][m0 + k], a1 = P[32 * qr + 16 + i][m0 + k];
+        const double b0 = BT ? Q[32 * qc + i][m0 + k] : Q[m0 + k][32 * qc + i];
+        const double b1 = BT ? Q[32 * qc + 16 + i][m0 + k] : Q[m0 + k][32 * qc + 16 + i];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
 }
 
@@ -891,10 +954,11 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
                                                         double* __restrict__ b, double* __restrict__ y, int ld, int kb,
                                                         int* flag) {
     __shared__ double R0[kCB][kCB + 1];  // X_k (R0[m][c] = X[m][c]); then the block to factor
-    __shared__ double R1[kCB][kCB + 1];  // A_ik^T, then L_ik; then factorisation scratch
-    __shared__ double R2[kCB][kCB + 1];  // A_jk^T, then L_jk
+    __shared__ double R1[kCB][kCB + 1];  // A_ik, then L_ik; then factorisation scratch
+    __shared__ double R2[kCB][kCB + 1];  // A_jk, then L_jk
     __shared__ FactorLds F;
     __shared__ double yk[kCB];
+    __shared__ double bk[kCB];
     int idx = blockIdx.x, row = 0;
     while (idx > row) {
         idx -= row + 1;
@@ -903,91 +967,82 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
     const int bi = kb + 1 + row, bj = kb + 1 + idx;
     const bool diag = (bi == bj);
     const bool next = diag && bi == kb + 1;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), qr = wv >> 1, qc = wv & 1;
     const int c0 = kb * kCB;
     load_block<false>(LinvT + (int64_t)kb * kCB * kCB, kCB, R0, tid);
-    load_block<true>(A + (int64_t)bi * kCB * ld + c0, ld, R1, tid);
-    if (!diag) load_block<true>(A + (int64_t)bj * kCB * ld + c0, ld, R2, tid);
-    const int tr = (tid >> 4) * 4, tc = (tid & 15) * 4;
-    double* Ab = A + (int64_t)(bi * kCB + tr) * ld + bj * kCB + tc;
-    double2 cur[4][2];
+    load_block<false>(A + (int64_t)bi * kCB * ld + c0, ld, R1, tid);
+    if (!diag) load_block<false>(A + (int64_t)bj * kCB * ld + c0, ld, R2, tid);
+    if (diag && tid < kCB) bk[tid] = b[c0 + tid];
+    double* Ab = A + (int64_t)bi * kCB * ld + bj * kCB;
+    double cur[2][2][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        cur[i][0] = *reinterpret_cast<const double2*>(Ab + (int64_t)i * ld);
-        cur[i][1] = *reinterpret_cast<const double2*>(Ab + (int64_t)i * ld + 2);
-    }
+    for (int ta = 0; ta < 2; ta++)
+#pragma unroll
+        for (int tb = 0; tb < 2; tb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                cur[ta][tb][r] = Ab[(int64_t)quad_row(qr, ta, r, lane) * ld + quad_col(qc, tb, lane)];
     __syncthreads();
-    double li[4][4], lj[4][4];
-    gemm_tn_4x4(R1, R0, tr, tc, li);
-    if (!diag) gemm_tn_4x4(R2, R0, tr, tc, lj);
+    f64x4 li[2][2], lj[2][2], acc[2][2];
+#pragma unroll
+    for (int ta = 0; ta < 2; ta++)
+#pragma unroll
+        for (int tb = 0; tb < 2; tb++) li[ta][tb] = lj[ta][tb] = acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
+    mfma_quad_64<false>(R1, R0, qr, qc, lane, li);  // L_ik = A_ik X_k
+    if (!diag) mfma_quad_64<false>(R2, R0, qr, qc, lane, lj);
     if (diag && tid < kCB) {
-        double s = 0.0;
-#pragma unroll 8
-        for (int m = 0; m < kCB; m++) s += b[c0 + m] * R0[m][tid];
-        yk[tid] = s;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < kCB; m++) s[m & 3] += bk[m] * R0[m][tid];
+        yk[tid] = (s[0] + s[1]) + (s[2] + s[3]);
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int ta = 0; ta < 2; ta++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            R1[tr + i][tc + j] = li[i][j];
-            if (!diag) R2[tr + i][tc + j] = lj[i][j];
-        }
+        for (int tb = 0; tb < 2; tb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rr = quad_row(qr, ta, r, lane), cc = quad_col(qc, tb, lane);
+                R1[rr][cc] = li[ta][tb][r];
+                if (!diag) R2[rr][cc] = lj[ta][tb][r];
+            }
+    if (next && tid < kCB) y[c0 + tid] = yk[tid];
+    __syncthreads();
     if (diag) {
-        // L_ik^T into the upper block (kb, bi): rows of it are columns of L_ik
+        // L_ik^T into the upper block (kb, bi): row c of it is column c of L_ik
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            double* u = A + (int64_t)(c0 + tc + j) * ld + bi * kCB + tr;
-            *reinterpret_cast<double2*>(u) = make_double2(li[0][j], li[1][j]);
-            *reinterpret_cast<double2*>(u + 2) = make_double2(li[2][j], li[3][j]);
+        for (int q = 0; q < 16; q++) {
+            const int e = tid + q * 256;
+            A[(int64_t)(c0 + (e >> 6)) * ld + bi * kCB + (e & 63)] = R1[e & 63][e >> 6];
         }
-        if (next && tid < kCB) y[c0 + tid] = yk[tid];
-    }
-    __syncthreads();
-    double (*Lb)[kCB + 1] = diag ? R1 : R2;
-    double acc[4][4];
+        if (tid < kCB) {
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-#pragma unroll 8
-    for (int m = 0; m < kCB; m++) {
-        double xi[4], xj[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            xi[i] = R1[tr + i][m];
-            xj[i] = Lb[tc + i][m];
+            for (int m = 0; m < kCB; m++) s[m & 3] += R1[tid][m] * yk[m];
+            b[bi * kCB + tid] -= (s[0] + s[1]) + (s[2] + s[3]);
         }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] += xi[i] * xj[j];
     }
-    if (diag && tid < kCB) {
-        double s = 0.0;
-#pragma unroll 8
-        for (int m = 0; m < kCB; m++) s += R1[tid][m] * yk[m];
-        b[bi * kCB + tid] -= s;
-    }
+    mfma_quad_64<true>(R1, diag ? R1 : R2, qr, qc, lane, acc);  // L_ik L_jk^T
     if (!next) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            *reinterpret_cast<double2*>(Ab + (int64_t)i * ld) =
-                make_double2(cur[i][0].x - acc[i][0], cur[i][0].y - acc[i][1]);
-            *reinterpret_cast<double2*>(Ab + (int64_t)i * ld + 2) =
-                make_double2(cur[i][1].x - acc[i][2], cur[i][1].y - acc[i][3]);
-        }
+        for (int ta = 0; ta < 2; ta++)
+#pragma unroll
+            for (int tb = 0; tb < 2; tb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    Ab[(int64_t)quad_row(qr, ta, r, lane) * ld + quad_col(qc, tb, lane)] = cur[ta][tb][r] - acc[ta][tb][r];
         return;
     }
     // this workgroup owns the next diagonal block: factor it straight from registers
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        R0[tr + i][tc + 0] = cur[i][0].x - acc[i][0];
-        R0[tr + i][tc + 1] = cur[i][0].y - acc[i][1];
-        R0[tr + i][tc + 2] = cur[i][1].x - acc[i][2];
-        R0[tr + i][tc + 3] = cur[i][1].y - acc[i][3];
-    }
+    for (int ta = 0; ta < 2; ta++)
+#pragma unroll
+        for (int tb = 0; tb < 2; tb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                R0[quad_row(qr, ta, r, lane)][quad_col(qc, tb, lane)] = cur[ta][tb][r] - acc[ta][tb][r];
     __syncthreads();
     factor_block(R0, R1, F, LinvT + (int64_t)(kb + 1) * kCB * kCB, flag);
 }
