@@ -1051,85 +1051,149 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
 //   last block:  y = X^T b'            x = X y
 //   block k:     t = y_k - U_k x_>k    x_k = X_k t
 // where row c of U_k is row kb*64+c of A right of the diagonal block (the
-// stored L_ik^T blocks).  16 waves; lanes run along rows (coalesced).
-// Writes x (FP64) and dx [P,6] (f32).
-__global__ __launch_bounds__(1024) void chol_backsolve_kernel(const double* __restrict__ A,
-                                                              const double* __restrict__ LinvT, int ld, int nb,
-                                                              const double* __restrict__ b,
-                                                              const double* __restrict__ y, double* __restrict__ x,
-                                                              const int* flag, int P, float* dx) {
+// stored L_ik^T blocks).  16 waves; wave w takes rows w + 16 u of U_k with its
+// lanes along the columns (coalesced).  The loads of block k-1 (X, the first
+// kPre column chunks of U, y) are issued before block k is computed, so only
+// the first block waits for memory.
+constexpr int kBT = 1024;           // threads of the backward substitution
+constexpr int kBW = kBT / 64;       // its waves
+constexpr int kBR = kCB / kBW;      // rows of a block per wave
+constexpr int kPre = 3;             // 128-column chunks of a U row held in registers (ld <= 448)
+
+// Operands of block kb of the backward substitution for thread (wv, lane), all
+// as 16-byte pairs: X_k entries (2 tid + 2 kBT q, +1) in row-major order,
+// U_k[wv + kBW u][2 lane + 128 c, +1] for c < kPre, and y_k of the row whose sum
+// this lane ends up holding (back_row).
+struct BackOps {
+    double2 x[kBR / 2], u[kBR][kPre];
+    double y;
+};
+
+// After the halving reduction, lane l holds row wv + kBW back_row(l) of the block.
+__device__ __forceinline__ int back_row(int lane) { return 2 * ((lane >> 5) & 1) + ((lane >> 4) & 1); }
+
+__device__ __forceinline__ void back_fetch(BackOps& o, const double* __restrict__ A, const double* __restrict__ LinvT,
+                                           const double* __restrict__ y, int ld, int nb, int kb, int tid) {
+    const int lane = tid & 63, wv = tid >> 6;
+    const double2* X = reinterpret_cast<const double2*>(LinvT + (int64_t)kb * kCB * kCB);
+#pragma unroll
+    for (int q = 0; q < kBR / 2; q++) o.x[q] = X[tid + q * kBT];
+    const int c0 = kb * kCB, r0 = c0 + kCB, nr = (nb - kb - 1) * kCB;
+    o.y = y[c0 + wv + kBW * back_row(lane)];
+#pragma unroll
+    for (int u = 0; u < kBR; u++) {
+        const double2* row = reinterpret_cast<const double2*>(A + (int64_t)(c0 + wv + kBW * u) * ld + r0);
+#pragma unroll
+        for (int c = 0; c < kPre; c++) {
+            const int col = 2 * lane + 128 * c;
+            o.u[u][c] = (col < nr) ? row[lane + 64 * c] : make_double2(0.0, 0.0);
+        }
+    }
+}
+
+// Sum of 4 per-lane values over the wave with 7 exchanges: the xor-32 step
+// keeps rows {0,1} or {2,3}, the xor-16 step one of those, and the last four
+// steps sum within groups of 16 lanes.  Lane l ends with the sum of row
+// back_row(l).
+__device__ __forceinline__ double back_reduce4(const double (&s)[4], int lane) {
+    const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
+    const double k0 = h32 ? s[2] : s[0], k1 = h32 ? s[3] : s[1];
+    const double g0 = h32 ? s[0] : s[2], g1 = h32 ? s[1] : s[3];
+    const double a0 = k0 + __shfl_xor(g0, 32, 64);
+    const double a1 = k1 + __shfl_xor(g1, 32, 64);
+    double v = (h16 ? a1 : a0) + __shfl_xor(h16 ? a0 : a1, 16, 64);
+#pragma unroll
+    for (int sh = 8; sh >= 1; sh >>= 1) v += __shfl_xor(v, sh, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __restrict__ A,
+                                                             const double* __restrict__ LinvT, int ld, int nb,
+                                                             const double* __restrict__ b,
+                                                             const double* __restrict__ y, double* __restrict__ x,
+                                                             const int* flag, int P, float* dx) {
+    static_assert(kBR == 4, "back_reduce4 sums 4 rows per wave");
+    static_assert(kBT * 2 * (kBR / 2) == kCB * kCB, "X_k pairs cover the block");
     __shared__ double xs[kMaxLd];
     __shared__ double Xk[kCB][kCB + 1];
     __shared__ double t[kCB];
-    __shared__ double part[16][kCB];
+    __shared__ double part[kBW][kCB];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     if (flag[0] != 0) {
-        for (int i = tid; i < ld; i += 1024) x[i] = 0.0;
-        for (int i = tid; i < 6 * P; i += 1024) dx[i] = 0.f;
+        for (int i = tid; i < ld; i += kBT) x[i] = 0.0;
+        for (int i = tid; i < 6 * P; i += kBT) dx[i] = 0.f;
         return;
     }
+    // The operands of block kb-1 are requested as soon as those of block kb
+    // are consumed, so their latency overlaps block kb's reductions.
+    BackOps o;
+    back_fetch(o, A, LinvT, y, ld, nb, nb - 1, tid);
     for (int kb = nb - 1; kb >= 0; kb--) {
         const int c0 = kb * kCB;
-        const double* X = LinvT + (int64_t)kb * kCB * kCB;
-        double xv[4];
+        const bool last = (kb == nb - 1);
+        double s[kBR];
 #pragma unroll
-        for (int q = 0; q < 4; q++) xv[q] = X[tid + q * 1024];
-        if (kb == nb - 1) {
-            // y_k = X^T b': wave wv takes m in [4 wv, 4 wv + 4)
-            double s = 0.0;
-#pragma unroll
-            for (int u = 0; u < 4; u++) s += b[c0 + 4 * wv + u] * X[(4 * wv + u) * kCB + lane];
-            part[wv][lane] = s;
-        } else {
-            // row c = wv + 16 u of U_k, lanes along its nr columns
+        for (int u = 0; u < kBR; u++) s[u] = 0.0;
+        if (!last) {
             const int r0 = c0 + kCB, nr = (nb - kb - 1) * kCB;
-            double s[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int q = lane; q < nr; q += 64) {
-                const double xq = xs[r0 + q];
 #pragma unroll
-                for (int u = 0; u < 4; u++) s[u] += A[(int64_t)(c0 + wv + 16 * u) * ld + r0 + q] * xq;
+            for (int c = 0; c < kPre; c++) {
+                const int col = 2 * lane + 128 * c;
+                const double x0 = (col < nr) ? xs[r0 + col] : 0.0, x1 = (col < nr) ? xs[r0 + col + 1] : 0.0;
+#pragma unroll
+                for (int u = 0; u < kBR; u++) s[u] += o.u[u][c].x * x0 + o.u[u][c].y * x1;
             }
+            for (int col = lane + 128 * kPre; col < nr; col += 64) {
+                const double xq = xs[r0 + col];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                double a = s[u];
-                for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
-                s[u] = a;
-            }
-            if (lane == 0) {
-#pragma unroll
-                for (int u = 0; u < 4; u++) t[wv + 16 * u] = y[c0 + wv + 16 * u] - s[u];
+                for (int u = 0; u < kBR; u++) s[u] += A[(int64_t)(c0 + wv + kBW * u) * ld + r0 + col] * xq;
             }
         }
+        const double yc = o.y;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int e = tid + q * 1024;
-            Xk[e >> 6][e & 63] = xv[q];
+        for (int q = 0; q < kBR / 2; q++) {
+            const int e = 2 * (tid + q * kBT);
+            Xk[e >> 6][e & 63] = o.x[q].x;
+            Xk[e >> 6][(e & 63) + 1] = o.x[q].y;
+        }
+        if (kb > 0) back_fetch(o, A, LinvT, y, ld, nb, kb - 1, tid);
+        if (!last) {
+            const double r = back_reduce4(s, lane);
+            if ((lane & 15) == 0) t[wv + kBW * back_row(lane)] = yc - r;
         }
         __syncthreads();
-        if (kb == nb - 1 && wv == 0) {
+        if (last) {
+            // y_k = X^T b': wave wv takes rows m in [kBR wv, kBR wv + kBR) of column lane
             double a = 0.0;
 #pragma unroll
-            for (int u = 0; u < 16; u++) a += part[u][lane];
-            t[lane] = a;
+            for (int u = 0; u < kBR; u++) a += b[c0 + kBR * wv + u] * Xk[kBR * wv + u][lane];
+            part[wv][lane] = a;
+            __syncthreads();
+            if (wv == 0) {
+                double a2 = 0.0;
+#pragma unroll
+                for (int u = 0; u < kBW; u++) a2 += part[u][lane];
+                t[lane] = a2;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        // x_k[r] = sum_c X[r][c] t[c]: wave wv takes c in [4 wv, 4 wv + 4)
+        // x_k[r] = sum_c X[r][c] t[c]: wave wv takes c in [kBR wv, kBR wv + kBR)
         double s2 = 0.0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) s2 += Xk[lane][4 * wv + u] * t[4 * wv + u];
+        for (int u = 0; u < kBR; u++) s2 += Xk[lane][kBR * wv + u] * t[kBR * wv + u];
         part[wv][lane] = s2;
         __syncthreads();
         if (wv == 0) {
             double a = 0.0;
 #pragma unroll
-            for (int u = 0; u < 16; u++) a += part[u][lane];
+            for (int u = 0; u < kBW; u++) a += part[u][lane];
             xs[c0 + lane] = a;
         }
         __syncthreads();
     }
-    for (int i = tid; i < ld; i += 1024) x[i] = xs[i];
-    for (int i = tid; i < 6 * P; i += 1024) dx[i] = (float)xs[i];
+    for (int i = tid; i < ld; i += kBT) x[i] = xs[i];
+    for (int i = tid; i < 6 * P; i += kBT) dx[i] = (float)xs[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -2001,7 +2065,7 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
                                kb, w.flag);
             VH_LAUNCH_CHECK("chol_step_kernel");
         }
-        hipLaunchKernelGGL(chol_backsolve_kernel, dim3(1), dim3(1024), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.y, w.x,
+        hipLaunchKernelGGL(chol_backsolve_kernel, dim3(1), dim3(kBT), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.y, w.x,
                            w.flag, d.P, dx);
         VH_LAUNCH_CHECK("chol_backsolve_kernel");
         if (!motion && d.K > 0) {
