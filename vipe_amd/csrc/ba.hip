@@ -836,9 +836,17 @@ __device__ __forceinline__ void inverse_row_partial(const double (*S)[kCB + 1], 
 //   B: one 16 x 16 matrix-core tile per wave: panel s's update of panel s+1's
 //      columns (3 - s tiles), and block row t of M = L^-1: M_tq = -M_tt T_q.
 // A last round does block row 3 of M, then X = M^T is stored.  Entries above
-// the diagonal of S are scratch (the diagonal tiles update them too).
+// the diagonal of S are scratch (the diagonal tiles update them too).  In
+// every round, waves 1-3 then run hook(round, wave, lane): in round 0 the
+// caller may use it to finish columns 16..63 of S while panel 0 is factored,
+// later for work of its own.
+struct NoHook {
+    __device__ void operator()(int, int, int) const {}
+};
+
+template <typename Hook = NoHook>
 __device__ __forceinline__ void factor_block(double (*S)[kCB + 1], double (*M)[kCB + 1], FactorLds& F,
-                                             double* __restrict__ Xout, int* flag) {
+                                             double* __restrict__ Xout, int* flag, const Hook& hook = Hook()) {
     constexpr int LS = kCB + 1, LT = kSB + 1;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -850,11 +858,14 @@ __device__ __forceinline__ void factor_block(double (*S)[kCB + 1], double (*M)[k
         const int o = s * kSB, t = s - 1;
         if (wv == 0) {
             factor_panel(S, F, o, lane, bad);
-        } else if (wv == 1) {
-            if (t >= 0) diag_inverse(S, M, F, kSB * t, lane);
         } else {
-            if (t >= 0 && o + kSB < kCB) panel_update(S, kSB * t, o + kSB, kCB, tid - 128, 128);
-            if (t >= 1) inverse_row_partial(S, M, F, t, tid - 128, 128);
+            if (wv == 1) {
+                if (t >= 0) diag_inverse(S, M, F, kSB * t, lane);
+            } else {
+                if (t >= 0 && o + kSB < kCB) panel_update(S, kSB * t, o + kSB, kCB, tid - 128, 128);
+                if (t >= 1) inverse_row_partial(S, M, F, t, tid - 128, 128);
+            }
+            hook(s, wv, lane);
         }
         __syncthreads();
         const int npu = (s + 1 < kCB / kSB) ? kCB / kSB - 1 - s : 0;
@@ -950,12 +961,26 @@ __global__ __launch_bounds__(256) void chol_factor_kernel(const double* __restri
     factor_block(S, M, F, LinvT + (int64_t)kb * kCB * kCB, flag);
 }
 
+// R[16 a + i][16 c + j] -= sum_m L[16 a + i][m] L[16 c + j][m] for one 16 x 16
+// tile (a, c) of a diagonal block (one wave; L and R in LDS).
+__device__ __forceinline__ void syrk_tile(double (*R)[kCB + 1], const double (*L)[kCB + 1], int a, int c, int lane) {
+    const f64x4 acc = mfma_tile16<true>(&L[16 * a][0], kCB + 1, &L[16 * c][0], kCB + 1, kCB, lane,
+                                        f64x4{0.0, 0.0, 0.0, 0.0});
+    const int ti = lane & 15, tk = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; r++) R[16 * a + 4 * r + tk][16 * c + ti] -= acc[r];
+}
+
+// lower 16 x 16 tiles of a 64 x 64 block, column 0 first: (a, c) = (kTileA[n], kTileC[n])
+__constant__ const int kTileA[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+__constant__ const int kTileC[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+
 __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, double* __restrict__ LinvT,
                                                         double* __restrict__ b, double* __restrict__ y, int ld, int kb,
                                                         int* flag) {
-    __shared__ double R0[kCB][kCB + 1];  // X_k (R0[m][c] = X[m][c]); then the block to factor
-    __shared__ double R1[kCB][kCB + 1];  // A_ik, then L_ik; then factorisation scratch
-    __shared__ double R2[kCB][kCB + 1];  // A_jk, then L_jk
+    __shared__ double R0[kCB][kCB + 1];  // X_k (R0[m][c] = X[m][c]); then factorisation scratch
+    __shared__ double R1[kCB][kCB + 1];  // A_ik, then L_ik
+    __shared__ double R2[kCB][kCB + 1];  // A_jk, then L_jk (off-diagonal); A_ii, updated, factored (diagonal)
     __shared__ FactorLds F;
     __shared__ double yk[kCB];
     __shared__ double bk[kCB];
@@ -970,25 +995,27 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), qr = wv >> 1, qc = wv & 1;
     const int c0 = kb * kCB;
+    double* Ab = A + (int64_t)bi * kCB * ld + bj * kCB;
     load_block<false>(LinvT + (int64_t)kb * kCB * kCB, kCB, R0, tid);
     load_block<false>(A + (int64_t)bi * kCB * ld + c0, ld, R1, tid);
-    if (!diag) load_block<false>(A + (int64_t)bj * kCB * ld + c0, ld, R2, tid);
+    load_block<false>(diag ? Ab : A + (int64_t)bj * kCB * ld + c0, ld, R2, tid);
     if (diag && tid < kCB) bk[tid] = b[c0 + tid];
-    double* Ab = A + (int64_t)bi * kCB * ld + bj * kCB;
     double cur[2][2][4];
+    if (!diag) {
 #pragma unroll
-    for (int ta = 0; ta < 2; ta++)
+        for (int ta = 0; ta < 2; ta++)
 #pragma unroll
-        for (int tb = 0; tb < 2; tb++)
+            for (int tb = 0; tb < 2; tb++)
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                cur[ta][tb][r] = Ab[(int64_t)quad_row(qr, ta, r, lane) * ld + quad_col(qc, tb, lane)];
+                for (int r = 0; r < 4; r++)
+                    cur[ta][tb][r] = Ab[(int64_t)quad_row(qr, ta, r, lane) * ld + quad_col(qc, tb, lane)];
+    }
     __syncthreads();
-    f64x4 li[2][2], lj[2][2], acc[2][2];
+    f64x4 li[2][2], lj[2][2];
 #pragma unroll
     for (int ta = 0; ta < 2; ta++)
 #pragma unroll
-        for (int tb = 0; tb < 2; tb++) li[ta][tb] = lj[ta][tb] = acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int tb = 0; tb < 2; tb++) li[ta][tb] = lj[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
     mfma_quad_64<false>(R1, R0, qr, qc, lane, li);  // L_ik = A_ik X_k
     if (!diag) mfma_quad_64<false>(R2, R0, qr, qc, lane, lj);
     if (diag && tid < kCB) {
@@ -1010,22 +1037,13 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
             }
     if (next && tid < kCB) y[c0 + tid] = yk[tid];
     __syncthreads();
-    if (diag) {
-        // L_ik^T into the upper block (kb, bi): row c of it is column c of L_ik
+    if (!diag) {
+        f64x4 acc[2][2];
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int e = tid + q * 256;
-            A[(int64_t)(c0 + (e >> 6)) * ld + bi * kCB + (e & 63)] = R1[e & 63][e >> 6];
-        }
-        if (tid < kCB) {
-            double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int ta = 0; ta < 2; ta++)
 #pragma unroll
-            for (int m = 0; m < kCB; m++) s[m & 3] += R1[tid][m] * yk[m];
-            b[bi * kCB + tid] -= (s[0] + s[1]) + (s[2] + s[3]);
-        }
-    }
-    mfma_quad_64<true>(R1, diag ? R1 : R2, qr, qc, lane, acc);  // L_ik L_jk^T
-    if (!next) {
+            for (int tb = 0; tb < 2; tb++) acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
+        mfma_quad_64<true>(R1, R2, qr, qc, lane, acc);  // L_ik L_jk^T
 #pragma unroll
         for (int ta = 0; ta < 2; ta++)
 #pragma unroll
@@ -1035,16 +1053,43 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
                     Ab[(int64_t)quad_row(qr, ta, r, lane) * ld + quad_col(qc, tb, lane)] = cur[ta][tb][r] - acc[ta][tb][r];
         return;
     }
-    // this workgroup owns the next diagonal block: factor it straight from registers
+    // diagonal block: L_ik^T goes into the upper block (kb, bi) (row c of it is
+    // column c of L_ik) and the right-hand side b'_i -= L_ik y_k; thread u of nu
+    auto store_lt_and_rhs = [&](int u, int nu) {
+        for (int e = u; e < kCB * kCB; e += nu)
+            A[(int64_t)(c0 + (e >> 6)) * ld + bi * kCB + (e & 63)] = R1[e & 63][e >> 6];
+        if (u < kCB) {
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int ta = 0; ta < 2; ta++)
+            for (int m = 0; m < kCB; m++) s[m & 3] += R1[u][m] * yk[m];
+            b[bi * kCB + u] -= (s[0] + s[1]) + (s[2] + s[3]);
+        }
+    };
+    // A_ii -= L_ik L_ik^T on the lower tiles, column 0 first (one tile per wave)
+    syrk_tile(R2, R1, wv, 0, lane);
+    if (!next) {
+        store_lt_and_rhs(tid, 256);
+        for (int n = 4 + wv; n < 10; n += 4) syrk_tile(R2, R1, kTileA[n], kTileC[n], lane);
+        __syncthreads();
 #pragma unroll
-        for (int tb = 0; tb < 2; tb++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                R0[quad_row(qr, ta, r, lane)][quad_col(qc, tb, lane)] = cur[ta][tb][r] - acc[ta][tb][r];
+        for (int q = 0; q < 16; q++) {
+            const int e = tid + q * 256;
+            Ab[(int64_t)(e >> 6) * ld + (e & 63)] = R2[e >> 6][e & 63];
+        }
+        return;
+    }
+    // this workgroup owns the next diagonal block: factor it while waves 1-3
+    // finish its remaining six tiles (columns 16..63) during panel 0, and the
+    // L_ik^T store and the right-hand side during panel 1
     __syncthreads();
-    factor_block(R0, R1, F, LinvT + (int64_t)(kb + 1) * kCB * kCB, flag);
+    factor_block(R2, R0, F, LinvT + (int64_t)(kb + 1) * kCB * kCB, flag, [&](int round, int w, int ln) {
+        if (round == 0) {
+            syrk_tile(R2, R1, kTileA[2 + 2 * w], kTileC[2 + 2 * w], ln);
+            syrk_tile(R2, R1, kTileA[3 + 2 * w], kTileC[3 + 2 * w], ln);
+        } else if (round == 1) {
+            store_lt_and_rhs(64 * (w - 1) + ln, 192);
+        }
+    });
 }
 
 // Backward substitution L^T x = y, zeros when the factorisation failed.
