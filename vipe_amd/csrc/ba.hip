@@ -49,7 +49,7 @@ constexpr int kEDS = 136;     // per-edge finalised record (doubles)
 constexpr int kCB = 64;       // Cholesky block size
 constexpr int kMaxLd = 2048;  // leading dimension of the reduced system (6P+1 rounded up to kCB)
 constexpr int kGramSplit = 4; // finalize workgroups per depth frame
-constexpr int kLdsBudget = 56 * 1024;  // dynamic E-row tile; static LDS stays under 8 KB
+constexpr int kLdsBudget = 64 * 1024;  // dynamic E-row tile; with ~15 KB static, two frame workgroups fit a CU
 
 // per-edge partial layout (floats)
 //   [0,21)  Hjj upper triangle, index(n,m) n>=m: n(n+1)/2 + m
@@ -263,8 +263,7 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
     extern __shared__ float dyn[];  // rows [Rext_max*6][TP+1]; reused for the Gram reduction
     __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo
     __shared__ float sQ[256];
-    __shared__ float sRed[4][kNV];
-    __shared__ float sAcc[kMaxDeg][kNV];
+    __shared__ float sAcc[4][kMaxDeg][kNV];  // per-wave edge partials (each wave adds into its own slot)
     __shared__ float sFen[4];
 
     const int k = blockIdx.y;
@@ -294,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
         sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
         sEP[tid][13] = __int_as_float(e);
     }
-    for (int i = tid; i < kMaxDeg * kNV; i += kThreads) (&sAcc[0][0])[i] = 0.f;
+    for (int i = tid; i < 4 * kMaxDeg * kNV; i += kThreads) (&sAcc[0][0][0])[i] = 0.f;
     if (tid < 4) sFen[tid] = 0.f;
 
     // Gram thread mapping (fixed for all sub-tiles)
@@ -308,6 +307,13 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
     float fen_acc = 0.f;
     __syncthreads();
 
+    // weights (u, v) and targets (u, v) of edge slot s at this lane's pixel
+    auto edge_in = [&](int s, int pix, bool valid, float4& o) {
+        if (valid && s < deg) {
+            const int64_t eb = (int64_t)__float_as_int(sEP[s][13]) * 2 * HW + pix;
+            o = make_float4(a.weights[eb], a.weights[eb + HW], a.targets[eb], a.targets[eb + HW]);
+        }
+    };
     for (int sub = 0; sub < p.SUB; sub++) {
         const int pix = (tile * p.SUB + sub) * TP + tid;
         const bool valid = (tid < TP) && (pix < HW);
@@ -316,23 +322,29 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
         const float X0 = ((float)px - cx) / fx;
         const float Y0 = ((float)py - cy) / fy;
         const float d = valid ? a.disps[(int64_t)kf * HW + pix] : 0.f;
+        const float ds = (!MOTION && valid) ? a.disps_sens[(int64_t)kf * HW + pix] : 0.f;
+        const float et = (!MOTION && valid) ? a.eta[(int64_t)k * HW + pix] : 0.f;
+        // the inputs of the next two edges are in flight while an edge is reduced
+        float4 in1 = make_float4(0.f, 0.f, 0.f, 0.f), in2 = in1;
+        edge_in(0, pix, valid, in1);
+        edge_in(1, pix, valid, in2);
         float Csum = 0.f, wsum = 0.f, Ef = 0.f;
         float es[6] = {0, 0, 0, 0, 0, 0};
         for (int s = 0; s < deg; s++) {
             const float* Rm = sEP[s];
             const float* t = sEP[s] + 9;
             const bool stereo = sEP[s][12] != 0.f;
-            const int e = __float_as_int(sEP[s][13]);
+            const float4 in = in1;
+            in1 = in2;
+            edge_in(s + 2, pix, valid, in2);
             float v[32];
 #pragma unroll
             for (int i = 0; i < 32; i++) v[i] = 0.f;
             float fj[6] = {0, 0, 0, 0, 0, 0};
             float er[6] = {0, 0, 0, 0, 0, 0};
             if (valid) {
-                const int64_t eb = (int64_t)e * 2 * HW + pix;
                 PixTerms T;
-                pixel_terms<INTR, true>(Rm, t, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
-                                        a.targets[eb], a.targets[eb + HW], a.iscale, T);
+                pixel_terms<INTR, true>(Rm, t, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale, T);
                 const float wdu = T.wu * act, wdv = T.wv * act;
                 const float pu = stereo ? 0.f : T.wu;
                 const float pv = stereo ? 0.f : T.wv;
@@ -373,26 +385,24 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
 #pragma unroll
                 for (int n = 0; n < 6; n++) dyn[((has_self + s) * 6 + n) * ldr + tid] = er[n];
             }
-            // block reduction of this edge's partials
+            // this wave's sums of the edge's partials; its LDS slot is its own, and
+            // a wavefront's LDS accesses complete in order, so no barrier is needed
             const float tot = wave_sum32(v);
-            if (lane < 32) sRed[wave][lane] = tot;
+            if (lane < 28) sAcc[wave][s][lane] += tot;
             if (INTR) {
+                if (lane >= 28 && lane < 30) sAcc[wave][s][lane] += tot;
 #pragma unroll
                 for (int n = 0; n < 6; n++) {
                     const float f = wave_sum(fj[n]);
-                    if (lane == 0) sRed[wave][32 + n] = f;
+                    if (lane == 0) sAcc[wave][s][32 + n] += f;
                 }
             }
-            __syncthreads();
-            if (tid < (INTR ? 38 : 28)) sAcc[s][tid] += sRed[0][tid] + sRed[1][tid] + sRed[2][tid] + sRed[3][tid];
-            __syncthreads();
         }
         if (!MOTION) {
             float Q = 0.f, wt = 0.f, Eft = 0.f;
             if (valid) {
-                const float ds = a.disps_sens[(int64_t)kf * HW + pix];
                 const float m = (ds > 0.f ? 1.f : 0.f) * act;
-                const float C = act * (Csum + a.eta[(int64_t)k * HW + pix] + m * a.alpha) + (1.f - act);
+                const float C = act * (Csum + et + m * a.alpha) + (1.f - act);
                 wt = act * (wsum - m * a.alpha * (d - ds));
                 Q = 1.f / C;
                 Eft = Ef * act;
@@ -433,11 +443,12 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
         }
     }
 
-    // per-edge partials
+    // per-edge partials (the sum of the four waves' slots)
+    __syncthreads();
     for (int i = tid; i < deg * kNV; i += kThreads) {
         const int s = i / kNV, v = i % kNV;
         const int e = p.fr_edges[e0 + s];
-        w.edge_part[((int64_t)tile * p.E + e) * kNV + v] = sAcc[s][v];
+        w.edge_part[((int64_t)tile * p.E + e) * kNV + v] = (sAcc[0][s][v] + sAcc[1][s][v]) + (sAcc[2][s][v] + sAcc[3][s][v]);
     }
     if (!MOTION) {
         // depth-prior energy
@@ -1875,7 +1886,7 @@ vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, in
     // tiles: TP pixels per sub-tile, sized so the E rows fit the LDS budget
     int TP = 256;
     while (TP > 64 && (size_t)Rext_max * 6 * (TP + 1) * 4 > kLdsBudget) TP /= 2;
-    const int SUB = 2;
+    const int SUB = std::max(1, 256 / TP);  // 256-pixel tiles
     const int T = (HW + TP * SUB - 1) / (TP * SUB);
     // lower pose blocks and their contribution lists (fixed order)
     std::vector<std::vector<int>> items((size_t)P * (P + 1) / 2);
