@@ -87,13 +87,31 @@ def _summarize(a) -> None:
         wr = counter_per_kernel(a.write, "WRITE_SIZE")
         out.append("| kernel | FETCH_SIZE/dispatch | WRITE_SIZE/dispatch | HBM bytes/dispatch (scaled) |")
         out.append("|---|---:|---:|---:|")
+        lines = []
         for k in sorted(fe, key=lambda k: -fe[k]):
             if a.kernel and not any(s in k for s in a.kernel):
                 continue
             byts = fe[k] * f_scale + wr.get(k, 0.0) * w_scale
             out.append(f"| `{_short(k)}` | {fe[k]:.0f} | {wr.get(k, 0.0):.0f} | {byts:.4e} |")
-            print(f"traffic_bytes_per_dispatch[{k.split('(')[0]}] = {byts:.6e}", file=sys.stderr)
+            lines.append(f"traffic_bytes_per_dispatch {kernel_short_name(k)}: {byts:.4e}")
+        out.append("")
+        out.extend(lines)
     print("\n".join(out))
+
+
+def kernel_short_name(name: str) -> str:
+    """`void vh::(anonymous namespace)::foo<float, 3>(args)` -> `foo<float, 3>`."""
+    for prefix in ("void ", "vh::(anonymous namespace)::", "vh::ba::", "vh::"):
+        name = name.replace(prefix, "")
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i]
+    return name
 
 
 def main(argv=None) -> None:
