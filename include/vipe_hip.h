@@ -152,6 +152,14 @@ typedef struct vh_ba_plan vh_ba_plan;
  * and uploads device index tables (synchronously).  Returns NULL on error. */
 vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W, int t0,
                               int t1);
+/* vh_ba_plan_create with the frame kernel's pixel tile chosen by the caller
+ * (no counterpart in the reference; for autotuners that time the candidates):
+ * pixel_tile is 64, 128 or 256 pixels per pass, or 0 for the automatic choice.
+ * Returns NULL when the tile's E rows do not fit the workgroup's LDS. */
+vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W,
+                                    int t0, int t1, int pixel_tile);
+/* Pixels per pass of the plan's frame kernel. */
+int vh_ba_plan_pixel_tile(const vh_ba_plan* plan);
 void vh_ba_plan_destroy(vh_ba_plan* plan);
 /* Number of depth frames |kx| and the frame ids (host copy, kx_out[|kx|]). */
 int vh_ba_plan_num_depth_frames(const vh_ba_plan* plan);

@@ -63,6 +63,8 @@ _SIGS = {
                                        vp, vp]),
     # bundle adjustment
     "vh_ba_plan_create": (vp, [vp, vp, i32, i32, i32, i32, i32, i32]),
+    "vh_ba_plan_create_tiled": (vp, [vp, vp, i32, i32, i32, i32, i32, i32, i32]),
+    "vh_ba_plan_pixel_tile": (i32, [vp]),
     "vh_ba_plan_destroy": (None, [vp]),
     "vh_ba_plan_num_depth_frames": (i32, [vp]),
     "vh_ba_plan_depth_frames": (i32, [vp, vp]),

@@ -50,6 +50,7 @@ constexpr int kCB = 64;       // Cholesky block size
 constexpr int kMaxLd = 2048;  // leading dimension of the reduced system (6P+1 rounded up to kCB)
 constexpr int kGramSplit = 4; // finalize workgroups per depth frame
 constexpr int kLdsBudget = 64 * 1024;  // dynamic E-row tile; with ~15 KB static, two frame workgroups fit a CU
+constexpr int kLdsMax = 128 * 1024;    // largest dynamic E-row tile a caller may ask for (one workgroup per CU)
 
 // per-edge partial layout (floats)
 //   [0,21)  Hjj upper triangle, index(n,m) n>=m: n(n+1)/2 + m
@@ -1807,6 +1808,15 @@ extern "C" {
 
 vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W, int t0,
                               int t1) {
+    return vh_ba_plan_create_tiled(ii_host, jj_host, E, N, H, W, t0, t1, 0);
+}
+
+vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W,
+                                    int t0, int t1, int pixel_tile) {
+    if (pixel_tile != 0 && pixel_tile != 64 && pixel_tile != 128 && pixel_tile != 256) {
+        fail(VH_ERR_ARG, "ba_plan_create: pixel_tile %d is not 0, 64, 128 or 256", pixel_tile);
+        return nullptr;
+    }
     if (E < 0 || N <= 0 || H <= 0 || W <= 0 || t0 < 0 || t1 <= t0 || t1 > N) {
         fail(VH_ERR_ARG, "ba_plan_create: bad sizes (E=%d N=%d H=%d W=%d t0=%d t1=%d)", E, N, H, W, t0, t1);
         return nullptr;
@@ -1884,8 +1894,18 @@ vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, in
         pseudo_off[k] = gblock(k, Rext - 1, Rext - 1);
     }
     // tiles: TP pixels per sub-tile, sized so the E rows fit the LDS budget
+    // unless the caller chose TP (then it must fit the workgroup's LDS)
     int TP = 256;
     while (TP > 64 && (size_t)Rext_max * 6 * (TP + 1) * 4 > kLdsBudget) TP /= 2;
+    if (pixel_tile != 0) {
+        if ((size_t)Rext_max * 6 * (pixel_tile + 1) * 4 > kLdsMax) {
+            delete pl;
+            fail(VH_ERR_UNSUPPORTED, "ba_plan_create: %d-pixel tiles need %zu B of LDS for %d E rows (limit %d)",
+                 pixel_tile, (size_t)Rext_max * 6 * (pixel_tile + 1) * 4, Rext_max * 6, kLdsMax);
+            return nullptr;
+        }
+        TP = pixel_tile;
+    }
     const int SUB = std::max(1, 256 / TP);  // 256-pixel tiles
     const int T = (HW + TP * SUB - 1) / (TP * SUB);
     // lower pose blocks and their contribution lists (fixed order)
@@ -2044,6 +2064,8 @@ int vh_ba_plan_depth_frames(const vh_ba_plan* plan, int64_t* kx_out) {
 }
 
 size_t vh_ba_workspace_size(const vh_ba_plan* plan) { return plan ? plan->ws_bytes : 0; }
+
+int vh_ba_plan_pixel_tile(const vh_ba_plan* plan) { return plan ? plan->d.TP : -1; }
 
 int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
               const float* targets, const float* weights, const float* eta, const float* depth_active,

@@ -37,17 +37,19 @@ _PLAN_CACHE_SIZE = 32
 
 
 class _Plan:
-    def __init__(self, ii_cpu, jj_cpu, N, H, W, t0, t1):
+    def __init__(self, ii_cpu, jj_cpu, N, H, W, t0, t1, pixel_tile=0):
         lib = _lib.load()
         ii_c = ii_cpu.to(torch.int64).contiguous()
         jj_c = jj_cpu.to(torch.int64).contiguous()
-        h = lib.vh_ba_plan_create(C.c_void_p(ii_c.data_ptr()), C.c_void_p(jj_c.data_ptr()), int(ii_c.numel()),
-                                  int(N), int(H), int(W), int(t0), int(t1))
+        h = lib.vh_ba_plan_create_tiled(C.c_void_p(ii_c.data_ptr()), C.c_void_p(jj_c.data_ptr()),
+                                        int(ii_c.numel()), int(N), int(H), int(W), int(t0), int(t1),
+                                        int(pixel_tile))
         if not h:
             raise RuntimeError(f"slam_ext.ba: {_lib.last_error()}")
         self.h = C.c_void_p(h)
         self.K = lib.vh_ba_plan_num_depth_frames(self.h)
         self.ws_bytes = int(lib.vh_ba_workspace_size(self.h))
+        self.pixel_tile = int(lib.vh_ba_plan_pixel_tile(self.h))
         kx = torch.empty(self.K, dtype=torch.int64)
         lib.vh_ba_plan_depth_frames(self.h, C.c_void_p(kx.data_ptr()))
         self.kx = kx
@@ -62,10 +64,12 @@ class _Plan:
 _LAST = {"ii": None, "jj": None, "iv": -1, "jv": -1, "key": None}
 
 
-def get_plan(ii, jj, N, H, W, t0, t1) -> _Plan:
+def get_plan(ii, jj, N, H, W, t0, t1, pixel_tile=0) -> _Plan:
+    """The (cached) plan of an edge list; pixel_tile 0 lets the library choose
+    the frame kernel's pixels per pass, 64 / 128 / 256 force it."""
     # Fast path: the same (unmodified) edge tensors as the previous call reuse the
     # plan without copying them to the host.
-    dims = (int(N), int(H), int(W), int(t0), int(t1))
+    dims = (int(N), int(H), int(W), int(t0), int(t1), int(pixel_tile))
     if (_LAST["ii"] is not None and _LAST["ii"]() is ii and _LAST["jj"]() is jj and ii._version == _LAST["iv"]
             and jj._version == _LAST["jv"] and _LAST["key"] is not None and _LAST["key"][2:] == dims):
         plan = _PLANS.get(_LAST["key"])
@@ -73,10 +77,10 @@ def get_plan(ii, jj, N, H, W, t0, t1) -> _Plan:
             return plan
     ii_cpu = ii.detach().to("cpu", torch.int64)
     jj_cpu = jj.detach().to("cpu", torch.int64)
-    key = (ii_cpu.numpy().tobytes(), jj_cpu.numpy().tobytes(), int(N), int(H), int(W), int(t0), int(t1))
+    key = (ii_cpu.numpy().tobytes(), jj_cpu.numpy().tobytes()) + dims
     plan = _PLANS.get(key)
     if plan is None:
-        plan = _Plan(ii_cpu, jj_cpu, N, H, W, t0, t1)
+        plan = _Plan(ii_cpu, jj_cpu, N, H, W, t0, t1, pixel_tile)
         _PLANS[key] = plan
         while len(_PLANS) > _PLAN_CACHE_SIZE:
             _PLANS.popitem(last=False)
@@ -151,6 +155,22 @@ def _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii
                            _lib.stream_of(poses))
     _lib.check(rc, "slam_ext.ba")
     return dx, dz, energy, plan, ws
+
+
+def run_planned(plan, ws, poses, disps, intrinsics, disps_sens, targets, weights, eta, depth_active, iterations,
+                lm, ep, alpha, dx, dz, motion_only=False):
+    """``iterations`` GN iterations of ``ba_extended`` on a prepared plan with
+    caller-allocated workspace and outputs (dx [P,6], dz [|kx|,H*W] or None).
+    It allocates nothing, copies nothing to the host and launches on torch's
+    current stream only, so it may be captured into a graph.  The inputs must be
+    contiguous float32 device tensors of the plan's shapes (not re-checked)."""
+    flags = 1 if motion_only else 0
+    rc = _lib.load().vh_ba_run(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(disps_sens),
+                               _lib.ptr(targets), _lib.ptr(weights), None if motion_only else _lib.ptr(eta),
+                               _lib.ptr(depth_active), int(iterations), float(lm), float(ep), float(alpha), 1e-6,
+                               1e-6, 1.0, flags, _lib.ptr(dx), None if motion_only else _lib.ptr(dz), None,
+                               _lib.ptr(ws), _lib.stream_of(poses))
+    _lib.check(rc, "slam_ext.ba")
 
 
 def ba_adjoint(plan, ws, poses, disps, intrinsics, targets, weights, depth_active, lm, motion_only, grad_dx,

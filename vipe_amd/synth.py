@@ -138,7 +138,54 @@ def reproject(poses, disps, intrinsics, ii, jj):
 
 def make_graph(name: str = "kf8_e24_48x64", seed: int = 0, C: int = 128, with_fmaps: bool = True,
                t0: int = 1, t1: int | None = None) -> FrameGraph:
-    cfg = CONFIGS[name]
+    return make_graph_from(CONFIGS[name], seed, C, with_fmaps, t0, t1)
+
+
+# Batched uneven workload: window sizes, out-degrees and resolutions of the clips
+BATCH_WINDOWS = (8, 16, 24, 48, 64, 96)
+BATCH_DEGREES = (2, 12)  # inclusive range of edges per frame
+# (H, W, fx, fy, cx, cy): 1/8 of 384x512, 512x640, and a 2:1 grid of 320x640
+BATCH_RESOLUTIONS = ((48, 64, 60.0, 60.0, 32.0, 24.0), (64, 80, 80.0, 80.0, 40.0, 32.0),
+                     (40, 80, 40.0, 40.0, 40.0, 20.0))
+
+
+def batch_configs(n_clips: int = 16, seed: int = 0, windows=BATCH_WINDOWS, resolutions=BATCH_RESOLUTIONS):
+    """Per-clip graph configurations of the batched uneven workload: the window
+    size is drawn from ``windows``, the out-degree uniformly from 2..12 (at most
+    frames-1), the resolution from ``resolutions``; a quarter of each frame's
+    edges (rounded down) are long-range, the rest a +-window band."""
+    gen = torch.Generator().manual_seed(seed)
+    out = []
+    for _ in range(n_clips):
+        n = windows[int(torch.randint(len(windows), (1,), generator=gen))]
+        deg = int(torch.randint(BATCH_DEGREES[0], BATCH_DEGREES[1] + 1, (1,), generator=gen))
+        deg = min(deg, n - 1)
+        H, W, fx, fy, cx, cy = resolutions[int(torch.randint(len(resolutions), (1,), generator=gen))]
+        n_rand_pf = deg // 4
+        window = max(1, (deg - n_rand_pf + 1) // 2)
+        # every source frame must have n_rand_pf targets farther than the window
+        worst = min(max(0, i - window) + max(0, n - 1 - i - window) for i in range(n))
+        if worst < n_rand_pf:
+            n_rand_pf = 0
+            window = min(max(1, (deg + 1) // 2), n - 1)
+        band = deg - n_rand_pf
+        n_cands = sum(2 * (n - d) for d in range(1, window + 1))
+        n_win = min(n * band, n_cands)
+        n_random = n * n_rand_pf
+        out.append(dict(n_frames=n, n_edges=n_win + n_random, H=H, W=W, intr=(fx, fy, cx, cy), window=window,
+                        n_random=n_random, sim3=False))
+    return out
+
+
+def make_batch(n_clips: int = 16, seed: int = 0, windows=BATCH_WINDOWS, resolutions=BATCH_RESOLUTIONS):
+    """Independent clips of the batched uneven workload (no feature maps),
+    clip c seeded with ``seed * 1000 + c``; each optimises poses [1, N)."""
+    return [make_graph_from(cfg, seed * 1000 + c, with_fmaps=False)
+            for c, cfg in enumerate(batch_configs(n_clips, seed, windows, resolutions))]
+
+
+def make_graph_from(cfg: dict, seed: int = 0, C: int = 128, with_fmaps: bool = True, t0: int = 1,
+                    t1: int | None = None) -> FrameGraph:
     gen = torch.Generator().manual_seed(seed)
     N, E, H, W = cfg["n_frames"], cfg["n_edges"], cfg["H"], cfg["W"]
     t1 = N if t1 is None else t1
