@@ -480,61 +480,78 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
 __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArgs a, Work w, int motion) {
     __shared__ double s[64];
     __shared__ double Ad[36];
+    __shared__ double RT[12];  // R (row-major) and t of the edge's relative pose
+    __shared__ double Tm[36];  // Adj^T Hjj
+    __shared__ double tpart[kThreads / kNV][kNV];  // per-slice tile sums of an edge
     const int tid = threadIdx.x;
     const int bid = blockIdx.x;
     if (bid < p.E) {
         const int e = bid;
-        if (tid < kNV) {
-            // four independent chains keep the tile loads in flight
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-            int t = 0;
-            for (; t + 4 <= p.T; t += 4) {
+        // thread (slice, v) sums tiles slice, slice + 6, ... of value v; the loads
+        // of a slice are issued together (8 in flight), then the 6 slices are summed
+        constexpr int kSl = kThreads / kNV;  // 6
+        const int sl = tid / kNV, v = tid % kNV;
+        if (sl < kSl) {
+            double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int t0 = sl; t0 < p.T; t0 += 8 * kSl) {
 #pragma unroll
-                for (int u = 0; u < 4; u++) acc[u] += (double)w.edge_part[((int64_t)(t + u) * p.E + e) * kNV + tid];
+                for (int u = 0; u < 8; u++) {
+                    const int t = t0 + u * kSl;
+                    if (t < p.T) acc[u] += (double)w.edge_part[((int64_t)t * p.E + e) * kNV + v];
+                }
             }
-            for (; t < p.T; t++) acc[0] += (double)w.edge_part[((int64_t)t * p.E + e) * kNV + tid];
-            s[tid] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-        }
-        if (tid == 0) {
+            tpart[sl][v] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        } else if (tid == kThreads - 1) {
+            // the relative pose, while the other threads sum the tiles
             float Rm[9], t[3];
             edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
-            // Adj = [[R, [t]x R], [0, R]]
-            double Rd[9], td[3];
-            for (int i = 0; i < 9; i++) Rd[i] = Rm[i];
-            for (int i = 0; i < 3; i++) td[i] = t[i];
-            const double tx[9] = {0, -td[2], td[1], td[2], 0, -td[0], -td[1], td[0], 0};
-            for (int i = 0; i < 36; i++) Ad[i] = 0.0;
-            for (int r = 0; r < 3; r++)
-                for (int c = 0; c < 3; c++) {
-                    Ad[r * 6 + c] = Rd[r * 3 + c];
-                    Ad[(r + 3) * 6 + c + 3] = Rd[r * 3 + c];
-                    double v = 0;
-                    for (int q = 0; q < 3; q++) v += tx[r * 3 + q] * Rd[q * 3 + c];
-                    Ad[r * 6 + c + 3] = v;
-                }
+            for (int i = 0; i < 9; i++) RT[i] = Rm[i];
+            for (int i = 0; i < 3; i++) RT[9 + i] = t[i];
+        }
+        __syncthreads();
+        if (tid < kNV) {
+            double a2 = 0.0;
+#pragma unroll
+            for (int u = 0; u < kSl; u++) a2 += tpart[u][tid];
+            s[tid] = a2;
+        }
+        // Adj = [[R, [t]x R], [0, R]], one entry per thread
+        if (tid < 36) {
+            const int r = tid / 6, c = tid % 6;
+            double v = 0.0;
+            if (r < 3 && c < 3) {
+                v = RT[r * 3 + c];
+            } else if (r >= 3 && c >= 3) {
+                v = RT[(r - 3) * 3 + (c - 3)];
+            } else if (r < 3) {  // ([t]x R)[r][c-3]
+                const double* td = RT + 9;
+                const double tx[3] = {r == 0 ? 0.0 : (r == 1 ? td[2] : -td[1]),
+                                      r == 0 ? -td[2] : (r == 1 ? 0.0 : td[0]),
+                                      r == 0 ? td[1] : (r == 1 ? -td[0] : 0.0)};
+                for (int q = 0; q < 3; q++) v += tx[q] * RT[q * 3 + (c - 3)];
+            }
+            Ad[tid] = v;
         }
         __syncthreads();
         double* rec = w.ed + (int64_t)e * kEDS;
+        // T = Adj^T Hjj (the i-j block is -T), then Hii = T Adj: two 6-term steps
         if (tid < 36) {
             const int i = tid / 6, j = tid % 6;
             const int n = i > j ? i : j, m = i > j ? j : i;
-            const double hjj = s[n * (n + 1) / 2 + m];
-            rec[E_HJJ + tid] = hjj;
-            // (A^T Hjj)[i][j] and (A^T Hjj A)[i][j]
-            double aTh = 0.0, aTha = 0.0;
+            rec[E_HJJ + tid] = s[n * (n + 1) / 2 + m];
+            double aTh = 0.0;
             for (int q = 0; q < 6; q++) {
                 const int nq = q > j ? q : j, mq = q > j ? j : q;
                 aTh += Ad[q * 6 + i] * s[nq * (nq + 1) / 2 + mq];
             }
-            for (int q = 0; q < 6; q++) {
-                double row = 0.0;  // (A^T Hjj)[i][q]
-                for (int r = 0; r < 6; r++) {
-                    const int nr = r > q ? r : q, mr = r > q ? q : r;
-                    row += Ad[r * 6 + i] * s[nr * (nr + 1) / 2 + mr];
-                }
-                aTha += row * Ad[q * 6 + j];
-            }
+            Tm[tid] = aTh;
             rec[E_HIJ + tid] = -aTh;
+        }
+        __syncthreads();
+        if (tid < 36) {
+            const int i = tid / 6, j = tid % 6;
+            double aTha = 0.0;
+            for (int q = 0; q < 6; q++) aTha += Tm[i * 6 + q] * Ad[q * 6 + j];
             rec[E_HII + tid] = aTha;
         } else if (tid >= 64 && tid < 70) {
             const int i = tid - 64;
@@ -561,17 +578,15 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
     const int nv = p.gram_off[k + 1] - off;
     const int per = (nv + kGramSplit - 1) / kGramSplit;
     const int i0 = part * per, i1 = min(nv, i0 + per);
-    for (int i = i0 + tid; i < i1; i += 4 * kThreads) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int t = 0; t < p.T; t++) {
-            const float* gp = w.gram_part + (int64_t)t * p.GT + off;
+    for (int i = i0 + tid; i < i1; i += kThreads) {
+        // 8 tiles' loads in flight per round, 8 partial sums
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int t0 = 0; t0 < p.T; t0 += 8) {
 #pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (i + u * kThreads < i1) acc[u] += (double)gp[i + u * kThreads];
+            for (int u = 0; u < 8; u++)
+                if (t0 + u < p.T) acc[u] += (double)w.gram_part[(int64_t)(t0 + u) * p.GT + off + i];
         }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (i + u * kThreads < i1) w.gd[off + i + u * kThreads] = acc[u];
+        w.gd[off + i] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     }
     if (part == 0 && tid == 0) {
         double acc = 0.0;
@@ -583,35 +598,52 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
 // ---------------------------------------------------------------------------
 // 3. assemble the dense lower triangle of H - S (+ intrinsics) and b - v
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, Work w, int motion) {
+// Signed contribution of one assembly-list item to entry (i, j) of its block,
+// branch-free so that the loads of several items can be in flight together.
+__device__ __forceinline__ double block_item(const Work& w, int item, int i, int j, int motion) {
+    const int kind = item >> 27, idx = item & ((1 << 27) - 1);
+    const bool gram = kind >= K_G;
+    const bool tr = (kind == K_HIJT) || (kind == K_GT);
+    const int ij = tr ? j * 6 + i : i * 6 + j;
+    const int base = kind == K_HII ? E_HII : (kind == K_HJJ ? E_HJJ : E_HIJ);
+    const double* src = gram ? w.gd + idx : w.ed + (int64_t)idx * kEDS + base;
+    // Schur terms exist only when the depth system is solved; in motion-only
+    // mode the Gram buffer is never written.
+    if (gram && motion) return 0.0;
+    const double v = src[ij];
+    return gram ? -v : v;
+}
+
+constexpr int kAsmBlocks = kThreads / 36;  // pose blocks per assembly workgroup (36 threads each)
+
+__global__ __launch_bounds__(kThreads) void ba_assemble_kernel(DevPlan p, RunArgs a, Work w, int motion) {
     const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
+    const int nbw = (p.nlb + kAsmBlocks - 1) / kAsmBlocks;  // workgroups of pose blocks
     const int ld = p.ld;
     const int n6 = 6 * p.P;
-    if (bid < p.nlb) {
-        if (tid >= 36) return;
-        const int ab = p.blk_ab[2 * bid], bb = p.blk_ab[2 * bid + 1];
-        const int i = tid / 6, j = tid % 6;
-        double s = 0.0;
-        for (int it = p.blk_ptr[bid]; it < p.blk_ptr[bid + 1]; it++) {
-            const int item = p.blk_items[it];
-            const int kind = item >> 27, idx = item & ((1 << 27) - 1);
-            switch (kind) {
-                case K_HII: s += w.ed[(int64_t)idx * kEDS + E_HII + i * 6 + j]; break;
-                case K_HJJ: s += w.ed[(int64_t)idx * kEDS + E_HJJ + i * 6 + j]; break;
-                case K_HIJ: s += w.ed[(int64_t)idx * kEDS + E_HIJ + i * 6 + j]; break;
-                case K_HIJT: s += w.ed[(int64_t)idx * kEDS + E_HIJ + j * 6 + i]; break;
-                // Schur terms exist only when the depth system is solved; in motion-only
-                // mode the Gram buffer is never written.
-                case K_G: if (!motion) s -= w.gd[idx + i * 6 + j]; break;
-                case K_GT: if (!motion) s -= w.gd[idx + j * 6 + i]; break;
-                default: break;
-            }
+    if ((int)blockIdx.x < nbw) {
+        // thread (g, o): entry o of pose block 7 * blockIdx + g; its list's item
+        // loads are issued four at a time
+        const int g = tid / 36, o = tid % 36;
+        const int blk = blockIdx.x * kAsmBlocks + g;
+        if (g >= kAsmBlocks || blk >= p.nlb) return;
+        const int ab = p.blk_ab[2 * blk], bb = p.blk_ab[2 * blk + 1];
+        const int i = o / 6, j = o % 6;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const int it0 = p.blk_ptr[blk], it1 = p.blk_ptr[blk + 1];
+        int it = it0;
+        for (; it + 4 <= it1; it += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] += block_item(w, p.blk_items[it + u], i, j, motion);
         }
+        for (; it < it1; it++) acc[0] += block_item(w, p.blk_items[it], i, j, motion);
+        double s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         if (ab == bb && i == j) s += (double)a.ep + (double)a.lm * s;
         w.A[(int64_t)(6 * ab + i) * ld + 6 * bb + j] = s;
         return;
     }
+    const int bid = blockIdx.x - nbw + p.nlb;  // the remaining roles keep their former numbering
+    if (tid >= 64) return;
     const int r = bid - p.nlb;
     if (r < p.P) {
         if (tid >= 6) return;
@@ -2132,8 +2164,8 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
             hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + d.K * kGramSplit), dim3(kThreads), 0, s, d, a, w, 0);
             VH_LAUNCH_CHECK("ba_finalize_kernel");
         }
-        hipLaunchKernelGGL(ba_assemble_kernel, dim3(d.nlb + d.P + 1 + npad), dim3(64), 0, s, d, a, w,
-                           motion ? 1 : 0);
+        hipLaunchKernelGGL(ba_assemble_kernel, dim3((d.nlb + kAsmBlocks - 1) / kAsmBlocks + d.P + 1 + npad),
+                           dim3(kThreads), 0, s, d, a, w, motion ? 1 : 0);
         VH_LAUNCH_CHECK("ba_assemble_kernel");
         hipLaunchKernelGGL(chol_factor_kernel, dim3(1), dim3(256), 0, s, w.A, w.Li, d.ld, 0, w.flag);
         VH_LAUNCH_CHECK("chol_factor_kernel");
