@@ -201,6 +201,32 @@ int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps
                   const float* grad_dx, const float* grad_dz, float* grad_targets, float* grad_weights,
                   void* workspace, void* stream);
 
+/* ---- slam_ext geometry siblings (csrc/slam_ext/slam.cpp:52-55,
+ *      geom_kernels.cu:526-953, 1849-1950) ---------------------------------
+ * All pointers device; poses [N,7] (t, q xyzw), disps [N,H,W] f32, index
+ * tensors int64.  An out-of-range frame index gives NaN (iproj, projmap,
+ * frame_distance) or a zero count (depth_filter) for that item. */
+
+/* iproj_cuda (geom_kernels.cu:1931-1950): points [N,H,W,3] = T_n applied to
+ * the back-projected pixel, divided by its disparity. */
+int vh_geom_iproj(const float* poses, const float* disps, const float* intrinsics, int N, int H, int W, float* points,
+                  void* stream);
+/* projmap_cuda (geom_kernels.cu:1879-1903): coords [E,H,W,3] (third channel
+ * zero), valid [E,H,W] for edges ii -> jj; intrinsics [4]. */
+int vh_geom_projmap(const float* poses, const float* disps, const float* intrinsics, const int64_t* ii,
+                    const int64_t* jj, int E, int N, int H, int W, float* coords, float* valid, void* stream);
+/* frame_distance_cuda (geom_kernels.cu:1849-1877): dist [num] for frame pairs
+ * (pi, pj), disparity maps di and intrinsics rows (qi, qj) of intrinsics
+ * [n_intrinsics,4]; beta weighs full motion against translation only. */
+int vh_geom_frame_distance(const float* poses, const float* disps, const float* intrinsics, int n_intrinsics,
+                           const int64_t* pi, const int64_t* pj, const int64_t* qi, const int64_t* qj,
+                           const int64_t* di, int num, int N, int H, int W, float beta, float* dist, void* stream);
+/* depth_filter_cuda (geom_kernels.cu:1905-1929): counter [num,H,W] of the six
+ * neighbour frames of inds[b] that agree with each pixel's depth within
+ * thresh[b]; intrinsics [4]. */
+int vh_geom_depth_filter(const float* poses, const float* disps, const float* intrinsics, const int64_t* inds,
+                         const float* thresh, int num, int N, int H, int W, float* counter, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,11 @@ _SIGS = {
     "vh_ba_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, f32, i32, vp, vp, vp, vp, vp, vp]),
     "vh_ba_run": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, vp, vp, vp, vp,
                         vp]),
+    # geometry siblings
+    "vh_geom_iproj": (i32, [vp, vp, vp, i32, i32, i32, vp, vp]),
+    "vh_geom_projmap": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp]),
+    "vh_geom_frame_distance": (i32, [vp, vp, vp, i32, vp, vp, vp, vp, vp, i32, i32, i32, i32, f32, vp, vp]),
+    "vh_geom_depth_filter": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp]),
 }
 
 _lib = None

@@ -200,3 +200,96 @@ def ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, t0, 
     depth_active = torch.ones(disps.shape[0], dtype=disps.dtype, device=disps.device)
     return ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active, t0, t1,
                        iterations, lm, ep, motion_only, alpha, False, 1e-6, 1e-6, 1.0, False)
+
+
+# ---------------------------------------------------------------------------
+# geometry siblings (slam.cpp:52-55; geom_kernels.cu:526-953, 1849-1950)
+# ---------------------------------------------------------------------------
+def _geom_inputs(fn, **named):
+    _check(**named)
+    for k, v in named.items():
+        if v.device.type != "cuda":
+            raise RuntimeError(f"slam_ext.{fn} runs on the GPU; {k} is on {v.device}")
+
+
+def _i64(name, t):
+    if t.dtype != torch.int64:
+        raise RuntimeError(f"{name} must be int64 (got {t.dtype})")
+
+
+def iproj(poses, disps, intrinsics):
+    """points [N,H,W,3]: each pixel back-projected with its disparity and mapped
+    by its frame's pose, divided by the disparity (iproj_kernel, :887-953)."""
+    _geom_inputs("iproj", poses=poses, disps=disps, intrinsics=intrinsics)
+    for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics)):
+        _f32(name, t)
+    N, H, W = disps.shape
+    points = torch.empty((N, H, W, 3), dtype=torch.float32, device=disps.device)
+    with torch.cuda.device(disps.device):
+        rc = _lib.load().vh_geom_iproj(_lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), N, H, W,
+                                       _lib.ptr(points), _lib.stream_of(disps))
+    _lib.check(rc, "slam_ext.iproj")
+    return points
+
+
+def projmap(poses, disps, intrinsics, ii, jj):
+    """[coords [E,H,W,3], valid [E,H,W,1]]: pixels of frame ii[e] reprojected
+    into frame jj[e]; coordinates stay (u, v) for points behind z = 0.01, the
+    third channel is zero, valid = z > MIN_DEPTH (projmap_kernel, :526-611)."""
+    _geom_inputs("projmap", poses=poses, disps=disps, intrinsics=intrinsics, ii=ii, jj=jj)
+    for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics)):
+        _f32(name, t)
+    _i64("ii", ii)
+    _i64("jj", jj)
+    N, H, W = disps.shape
+    E = ii.shape[0]
+    coords = torch.empty((E, H, W, 3), dtype=torch.float32, device=disps.device)
+    valid = torch.empty((E, H, W, 1), dtype=torch.float32, device=disps.device)
+    with torch.cuda.device(disps.device):
+        rc = _lib.load().vh_geom_projmap(_lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(ii),
+                                         _lib.ptr(jj), E, N, H, W, _lib.ptr(coords), _lib.ptr(valid),
+                                         _lib.stream_of(disps))
+    _lib.check(rc, "slam_ext.projmap")
+    return [coords, valid]
+
+
+def frame_distance(poses, disps, intrinsics, pi, pj, qi, qj, di, beta):
+    """dist [num] between frames pi[b] -> pj[b] using the disparity map di[b]
+    and intrinsics rows qi[b], qj[b] of intrinsics [R,4]; 1000 when less than
+    75 % of the beta-weighted pixel terms are valid (frame_distance_kernel,
+    :613-768)."""
+    _geom_inputs("frame_distance", poses=poses, disps=disps, intrinsics=intrinsics, pi=pi, pj=pj, qi=qi, qj=qj,
+                 di=di)
+    for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics)):
+        _f32(name, t)
+    for name, t in (("pi", pi), ("pj", pj), ("qi", qi), ("qj", qj), ("di", di)):
+        _i64(name, t)
+    N, H, W = disps.shape
+    num = pi.shape[0]
+    dist = torch.empty((num,), dtype=torch.float32, device=disps.device)
+    with torch.cuda.device(disps.device):
+        rc = _lib.load().vh_geom_frame_distance(_lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics),
+                                                int(intrinsics.shape[0]), _lib.ptr(pi), _lib.ptr(pj), _lib.ptr(qi),
+                                                _lib.ptr(qj), _lib.ptr(di), num, N, H, W, float(beta),
+                                                _lib.ptr(dist), _lib.stream_of(disps))
+    _lib.check(rc, "slam_ext.frame_distance")
+    return dist
+
+
+def depth_filter(poses, disps, intrinsics, ix, thresh):
+    """counter [num,H,W]: how many of the frames ix[b]-3 .. ix[b]+3 (excluding
+    ix[b], within [0, N)) see each pixel of frame ix[b] at a consistent depth
+    (depth_filter_kernel, :770-885)."""
+    _geom_inputs("depth_filter", poses=poses, disps=disps, intrinsics=intrinsics, ix=ix, thresh=thresh)
+    for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics), ("thresh", thresh)):
+        _f32(name, t)
+    _i64("ix", ix)
+    N, H, W = disps.shape
+    num = ix.shape[0]
+    counter = torch.empty((num, H, W), dtype=torch.float32, device=disps.device)
+    with torch.cuda.device(disps.device):
+        rc = _lib.load().vh_geom_depth_filter(_lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(ix),
+                                              _lib.ptr(thresh), num, N, H, W, _lib.ptr(counter),
+                                              _lib.stream_of(disps))
+    _lib.check(rc, "slam_ext.depth_filter")
+    return counter
