@@ -5,6 +5,9 @@
         ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE``, it gives the factor between
         the counters' KiB and real bytes on this GPU.
 
+    python3 -m vipe_amd.profiling build [--contraction f32|f16|bf16|fp8]
+        Two correlation-pyramid builds of the bench workload (for counters).
+
     python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
         --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
         Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
@@ -32,6 +35,28 @@ def _copy(mib: int) -> None:
         b.copy_(a)
     torch.cuda.synchronize()
     print(f"copy: {n * 4} bytes read and {n * 4} bytes written per launch", flush=True)
+
+
+def _build(contraction: str, workload: str) -> None:
+    """Two pyramid builds (one warm-up) of the bench workload's 384 edges with the
+    given MFMA contraction type; output levels in f32 (f32 contraction) or f16."""
+    import torch
+
+    from . import synth
+    from .corr import CorrBlock
+
+    dev = torch.device("cuda", 0)
+    g = synth.make_graph(workload, seed=0)
+    fm = g.fmaps.to(dev)
+    f1 = fm[g.ii.to(dev)].contiguous()[None]
+    f2 = fm[g.jj.to(dev)].contiguous()[None]
+    ctype = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "fp8": "fp8"}[contraction]
+    out = torch.float32 if contraction == "f32" else torch.float16
+    for _ in range(2):
+        c = CorrBlock(f1, f2, num_levels=4, radius=3, volume_dtype=out, contraction_dtype=ctype)
+        del c
+        torch.cuda.synchronize()
+    print(f"build: {g.ii.numel()} edges, contraction {contraction}, levels {out}", flush=True)
 
 
 def _find(d: str, suffix: str) -> str:
@@ -119,6 +144,9 @@ def main(argv=None) -> None:
     sub = ap.add_subparsers(dest="cmd", required=True)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
+    bl = sub.add_parser("build")
+    bl.add_argument("--contraction", choices=["f32", "f16", "bf16", "fp8"], default="f32")
+    bl.add_argument("--workload", default="kf64_e384_64x80")
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch")
@@ -131,6 +159,8 @@ def main(argv=None) -> None:
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         _copy(a.mib)
+    elif a.cmd == "build":
+        _build(a.contraction, a.workload)
     else:
         _summarize(a)
 
