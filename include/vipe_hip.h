@@ -227,6 +227,24 @@ int vh_geom_frame_distance(const float* poses, const float* disps, const float* 
 int vh_geom_depth_filter(const float* poses, const float* disps, const float* intrinsics, const int64_t* inds,
                          const float* thresh, int num, int N, int H, int W, float* counter, void* stream);
 
+/* ---- corr_ext: spatial correlation sampler ---------------------------------
+ * Replaces correlation_sample_forward / correlation_sample_backward
+ * (csrc/corr_ext/correlation_sampler.cpp:41-85, kernels in
+ * correlation_cuda_kernel.cu:27-286).  in1, in2: [N,C,H,W] contiguous, same
+ * shape, dtype VH_F32 / VH_F64 / VH_F16 / VH_BF16 (the sum accumulates in FP32,
+ * FP64 for VH_F64).  out: [N,patchH,patchW,oH,oW] with
+ * oH = (H + 2 padH - ((kH-1) dilH + 1)) / dH + 1 (vh_spatial_corr_out_size).
+ * Every output / gradient element is written (no zero-fill needed). */
+int vh_spatial_corr_out_size(int H, int W, int kH, int kW, int padH, int padW, int dilH, int dilW, int dH, int dW,
+                             int* oH, int* oW);
+int vh_spatial_corr_forward(int dtype, const void* in1, const void* in2, int N, int C, int H, int W, int kH, int kW,
+                            int patchH, int patchW, int padH, int padW, int dilH, int dilW, int dpH, int dpW, int dH,
+                            int dW, void* out, void* stream);
+/* grad_out: [N,patchH,patchW,oH,oW] contiguous; grad1, grad2: [N,C,H,W]. */
+int vh_spatial_corr_backward(int dtype, const void* in1, const void* in2, const void* grad_out, int N, int C, int H,
+                             int W, int kH, int kW, int patchH, int patchW, int padH, int padW, int dilH, int dilW,
+                             int dpH, int dpW, int dH, int dW, void* grad1, void* grad2, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,4 +22,8 @@ corr  : all-pairs correlation volume, avg-pool pyramid, corr_index
 ba    : slam_ext.ba / ba_extended dense bundle adjustment with the reference
         bookkeeping (csrc/slam_ext/geom_kernels.cu:1626-1847) and a
         brute-force dense Gauss-Newton solve used to pin it.
+geom  : slam_ext geometry siblings iproj, projmap, frame_distance,
+        depth_filter (csrc/slam_ext/geom_kernels.cu:526-953).
+spatial_corr : corr_ext spatial correlation sampler forward/backward
+        (csrc/corr_ext/correlation_cuda_kernel.cu).
 """
