@@ -32,6 +32,11 @@ extern "C" {
 #define VH_F64 3
 #define VH_BF16 4
 #define VH_FP8E4M3 5
+#define VH_U8 6
+#define VH_I8 7
+#define VH_I16 8
+#define VH_I32 9
+#define VH_I64 10
 
 /* volume plane layouts (vh_corr_build / vh_corr_lookup_pyramid) */
 #define VH_LAYOUT_PLANAR 0 /* [Hl][Wl] row-major: the reference's */
@@ -244,6 +249,22 @@ int vh_spatial_corr_forward(int dtype, const void* in1, const void* in2, int N, 
 int vh_spatial_corr_backward(int dtype, const void* in1, const void* in2, const void* grad_out, int N, int C, int H,
                              int W, int kH, int kW, int patchH, int patchW, int padH, int padW, int dilH, int dilW,
                              int dpH, int dpW, int dH, int dW, void* grad1, void* grad2, void* stream);
+
+/* ---- scatter_ext -----------------------------------------------------------
+ * Replaces scatter_cuda (csrc/scatter_ext/cuda/scatter_cuda.cu:56-129), the
+ * kernel behind scatter_sum / scatter_mul / scatter_mean / scatter_min /
+ * scatter_max (scatter.cpp:54-237).  reduce: 0 sum, 1 mul, 2 min, 3 max (mean
+ * is a sum divided by a count, composed by the caller as the reference does).
+ * dtype: VH_F32 / VH_F64 / VH_F16 / VH_BF16 / VH_U8 / VH_I8 / VH_I16 / VH_I32 /
+ * VH_I64.  src: contiguous [B,E,K] (the axes before, at and after dim);
+ * index: int64, element (b,e,k) at b*sB + e*sE + k*sK; out: contiguous
+ * [B,N,K].  out_given = 0: out is initialised here (identity; min / max
+ * entries left at the identity become 0); 1: out's values take part.
+ * arg_out [B,N,K] int64 (min / max only, NULL otherwise): the smallest e whose
+ * src equals the result, E if none.  Indices outside [0,N) are skipped. */
+int vh_scatter(int reduce, int dtype, const void* src, const int64_t* index, int64_t B, int64_t E, int64_t K,
+               int64_t sB, int64_t sE, int64_t sK, int64_t N, void* out, int64_t* arg_out, int out_given,
+               void* stream);
 
 #ifdef __cplusplus
 }

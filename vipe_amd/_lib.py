@@ -17,6 +17,7 @@ LIB_PATH = Path(os.environ.get("VIPE_HIP_LIB", Path(__file__).resolve().parent /
 
 VH_OK, VH_ERR_ARG, VH_ERR_HIP, VH_ERR_UNSUPPORTED, VH_ERR_UNDEFINED = 0, 1, 2, 3, 4
 VH_F16, VH_F32, VH_F64, VH_BF16, VH_FP8E4M3 = 1, 2, 3, 4, 5
+VH_U8, VH_I8, VH_I16, VH_I32, VH_I64 = 6, 7, 8, 9, 10
 VH_LAYOUT_PLANAR, VH_LAYOUT_TILED = 0, 1
 VH_DEVICE, VH_HOST = 0, 1
 
@@ -81,6 +82,8 @@ _SIGS = {
     "vh_spatial_corr_out_size": (i32, [i32] * 10 + [C.POINTER(i32), C.POINTER(i32)]),
     "vh_spatial_corr_forward": (i32, [i32, vp, vp] + [i32] * 16 + [vp, vp]),
     "vh_spatial_corr_backward": (i32, [i32, vp, vp, vp] + [i32] * 16 + [vp, vp, vp]),
+    # scatter_ext
+    "vh_scatter": (i32, [i32, i32, vp, vp] + [i64] * 7 + [vp, vp, i32, vp]),
 }
 
 _lib = None
@@ -147,7 +150,8 @@ def place_of(t):
     return VH_HOST if t.device.type == "cpu" else VH_DEVICE
 
 
-DTYPE_CODE = {torch.float16: VH_F16, torch.float32: VH_F32, torch.float64: VH_F64, torch.bfloat16: VH_BF16}
+DTYPE_CODE = {torch.float16: VH_F16, torch.float32: VH_F32, torch.float64: VH_F64, torch.bfloat16: VH_BF16,
+              torch.uint8: VH_U8, torch.int8: VH_I8, torch.int16: VH_I16, torch.int32: VH_I32, torch.int64: VH_I64}
 
 
 def dtype_code(t):
