@@ -261,10 +261,15 @@ int vh_spatial_corr_backward(int dtype, const void* in1, const void* in2, const 
  * [B,N,K].  out_given = 0: out is initialised here (identity; min / max
  * entries left at the identity become 0); 1: out's values take part.
  * arg_out [B,N,K] int64 (min / max only, NULL otherwise): the smallest e whose
- * src equals the result, E if none.  Indices outside [0,N) are skipped. */
+ * src equals the result, E if none.  Indices outside [0,N) are skipped.
+ * workspace: device memory of vh_scatter_workspace_size(...) bytes; when that
+ * size is non-zero (index independent of b and k, K >= 32, N <= 15360,
+ * E <= 65536) the reduction runs atomic-free over a stable sort of the index
+ * and gives the same bits on every run; with no workspace it uses atomics. */
+size_t vh_scatter_workspace_size(int64_t B, int64_t E, int64_t K, int64_t sB, int64_t sE, int64_t sK, int64_t N);
 int vh_scatter(int reduce, int dtype, const void* src, const int64_t* index, int64_t B, int64_t E, int64_t K,
                int64_t sB, int64_t sE, int64_t sK, int64_t N, void* out, int64_t* arg_out, int out_given,
-               void* stream);
+               void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -49,13 +49,14 @@ def _check(reduce, dtype, got, ref, src_np, idx, dim, dim_size, out_np=None):
     assert np.all(np.abs(got - ref) <= bound + 1e-300), np.abs(got - ref).max()
 
 
+@pytest.mark.parametrize("K", [17, 40, 36])   # 17: atomics; 40, 36: the segmented path (36: 8-wide types unvectorised)
 @pytest.mark.parametrize("reduce", ["sum", "mul", "min", "max", "mean"])
 @pytest.mark.parametrize("dtype", FLOATS + INTS)
-def test_broadcast_index_matches_oracle(reduce, dtype):
+def test_broadcast_index_matches_oracle(reduce, dtype, K):
     from vipe_amd.ext import scatter
 
-    g = torch.Generator().manual_seed(zlib.crc32(f"{reduce}{dtype}".encode()) % 1000)
-    src = _src((3, 40, 17), dtype, g, reduce)
+    g = torch.Generator().manual_seed(zlib.crc32(f"{reduce}{dtype}{K}".encode()) % 1000)
+    src = _src((3, 40, K), dtype, g, reduce)
     idx = torch.randint(0, 9, (40,), generator=g)
     fn = getattr(scatter, f"scatter_{reduce}")
     res = fn(src.cuda(), idx.cuda(), 1, None, 11)
@@ -64,7 +65,7 @@ def test_broadcast_index_matches_oracle(reduce, dtype):
     if reduce in ("min", "max"):
         res, got_arg = res
         assert np.array_equal(got_arg.cpu().numpy(), arg)
-    assert res.dtype == dtype and tuple(res.shape) == (3, 11, 17)
+    assert res.dtype == dtype and tuple(res.shape) == (3, 11, K)
     _check(reduce, dtype, res, ref, src_np.astype(np.float64) if dtype.is_floating_point else src_np,
            idx.numpy(), 1, 11)
 
@@ -138,6 +139,36 @@ def test_min_max_edge_cases_are_deterministic():
     r1 = scatter_ext.scatter_max(big, bidx, 0)
     r2 = scatter_ext.scatter_max(big, bidx, 0)
     assert torch.equal(r1[0], r2[0]) and torch.equal(r1[1], r2[1])
+
+
+@pytest.mark.parametrize("reduce", ["min", "max", "sum"])
+def test_segmented_path_equals_atomic_path(reduce):
+    """The same reduction through a broadcast 1-D index (segmented path) and
+    through the materialised index (atomics): min / max / arg_out bit-equal,
+    including -0.0 / +0.0, NaN, the identity value and a given out holding NaN;
+    the segmented sum is bit-identical from run to run."""
+    from vipe_amd.ext import scatter_ext
+
+    nan = float("nan")
+    fmax = torch.finfo(torch.float32).max
+    col = torch.tensor([3.0, -0.0, 0.0, nan, 5.0, 5.0, fmax, -1.0, 2.0, -7.0])
+    idx = torch.tensor([0, 1, 1, 2, 3, 3, 4, 6, 0, 6], device="cuda")
+    g = torch.Generator().manual_seed(1)
+    src = (col[:, None] * torch.ones(1, 64)).clone()
+    src[8:] = torch.randn(2, 64, generator=g)
+    src = src.cuda()
+    full = idx[:, None].expand(10, 64).contiguous()
+    fn = getattr(scatter_ext, f"scatter_{reduce}")
+    for out0 in (None, torch.full((8, 64), nan, device="cuda")):
+        a = fn(src, idx, 0, None if out0 is None else out0.clone(), 8)
+        b = fn(src, full, 0, None if out0 is None else out0.clone(), 8)
+        if reduce == "sum":
+            assert torch.allclose(a, b, equal_nan=True, atol=1e-6)
+            r1, r2 = fn(src, idx, 0, None, 8), fn(src, idx, 0, None, 8)
+            assert torch.equal(r1.view(torch.int32), r2.view(torch.int32))   # bits: bucket 2 holds a NaN
+        else:
+            assert torch.equal(a[0].view(torch.int32), b[0].view(torch.int32))
+            assert torch.equal(a[1], b[1])
 
 
 def test_out_of_range_indices_are_skipped_and_empty_src():

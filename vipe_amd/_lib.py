@@ -83,7 +83,8 @@ _SIGS = {
     "vh_spatial_corr_forward": (i32, [i32, vp, vp] + [i32] * 16 + [vp, vp]),
     "vh_spatial_corr_backward": (i32, [i32, vp, vp, vp] + [i32] * 16 + [vp, vp, vp]),
     # scatter_ext
-    "vh_scatter": (i32, [i32, i32, vp, vp] + [i64] * 7 + [vp, vp, i32, vp]),
+    "vh_scatter_workspace_size": (sz, [i64] * 7),
+    "vh_scatter": (i32, [i32, i32, vp, vp] + [i64] * 7 + [vp, vp, i32, vp, sz, vp]),
 }
 
 _lib = None

@@ -20,6 +20,10 @@
 //    none): an atomic min over e, so ties resolve deterministically.
 // Integer sums and products wrap (mod 2^bits) and are exact in any order;
 // float sums and products are exact up to the order of the atomics.
+//
+// When the index depends on e alone (a 1-D index broadcast over the other
+// axes, the common case) and K >= 32, a segmented path replaces the atomics:
+// see scatter_segment_sort_kernel / scatter_segment_gather_kernel.
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -272,10 +276,229 @@ unsigned grid_for(int64_t n) {
     return (unsigned)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
 }
 
+// ---- segmented path: index depends on e only --------------------------------
+// A stable counting sort of the E indices gives, per output slot n, the list of
+// source positions e in ascending order (start[n] .. start[n+1] into order[]).
+// The gather kernel then reduces each output element over its list in that
+// order: no atomics, each src element read once, each out element written once,
+// the same result on every run, and 8 / 16-bit floats reduced in FP32 with one
+// final rounding.
+constexpr int kSegMaxN = 15360;   // LDS counters (60 KiB)
+constexpr int kSegMaxE = 1 << 16;   // the placement pass is one wave
+constexpr int kSegMinK = 32;      // lanes along k share one list
+
+__global__ __launch_bounds__(256) void scatter_segment_sort_kernel(const int64_t* __restrict__ index, int64_t sE,
+                                                                   int E, int N, int* __restrict__ start,
+                                                                   int* __restrict__ order) {
+    __shared__ int cnt[kSegMaxN];
+    __shared__ int tsum[256];
+    const int tid = threadIdx.x;
+    for (int n = tid; n < N; n += 256) cnt[n] = 0;
+    __syncthreads();
+    for (int e = tid; e < E; e += 256) {
+        const int64_t n = index[(int64_t)e * sE];
+        if (n >= 0 && n < N) atomicAdd(&cnt[n], 1);
+    }
+    __syncthreads();
+    // exclusive scan: each thread a contiguous chunk, then a serial pass over the 256 chunk sums
+    const int per = (N + 255) / 256;
+    const int lo = min(N, tid * per), hi = min(N, lo + per);
+    int sum = 0;
+    for (int n = lo; n < hi; n++) sum += cnt[n];
+    tsum[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int i = 0; i < 256; i++) {
+            const int v = tsum[i];
+            tsum[i] = run;
+            run += v;
+        }
+        start[N] = run;
+    }
+    __syncthreads();
+    int run = tsum[tid];
+    for (int n = lo; n < hi; n++) {
+        const int c = cnt[n];
+        start[n] = run;
+        cnt[n] = run;  // becomes the placement cursor
+        run += c;
+    }
+    __syncthreads();
+    // stable placement by one wave: a lane's slot is its bucket's cursor plus
+    // the number of lower lanes in the same bucket; the highest such lane
+    // advances the cursor.  One wave's LDS reads and writes complete in order.
+    if (tid < 64) {
+        const int lane = tid;
+        for (int base = 0; base < E; base += 64) {
+            const int e = base + lane;
+            int64_t nn = e < E ? index[(int64_t)e * sE] : -1;
+            const int n = (nn >= 0 && nn < N) ? (int)nn : -1;
+            int rank = 0;
+            bool higher = false;
+            for (int j = 0; j < 64; j++) {
+                const int nj = __shfl(n, j, 64);
+                if (nj == n) {
+                    rank += j < lane;
+                    higher |= j > lane;
+                }
+            }
+            if (n >= 0) {
+                const int pos = cnt[n] + rank;
+                order[pos] = e;
+                if (!higher) cnt[n] = pos + 1;
+            }
+        }
+    }
+}
+
+// Per-element running state of one output element of the gather.
+template <typename T, int OP> struct SegAcc {
+    using Cv = typename Val<T>::C;
+    Cv acc;
+    __device__ void init(T v0) { acc = Val<T>::get(v0); }
+    __device__ void add(T v, int) {
+        if constexpr (OP == kSum) acc = acc + Val<T>::get(v); else acc = acc * Val<T>::get(v);
+    }
+    __device__ T finish(int, int64_t&, int64_t) const { return Val<T>::put(acc); }
+};
+// min / max on order-preserving keys: the same result as the atomic path; the
+// arg is the smallest e whose value equals the running extremum (e ascends).
+template <typename T, int OP> struct SegArg {
+    using U = Bits<T>;
+    static constexpr bool MAX = OP == kMax;
+    U best;
+    int64_t a;
+    __device__ void init(T v0) {
+        best = encode(v0);
+        if (is_nan(v0)) best = MAX ? (U)0 : (U)~(U)0;
+        a = -1;
+    }
+    __device__ void add(T v, int e) {
+        if (is_nan(v)) return;
+        const U key = encode(v);
+        const bool beats = MAX ? key > best : key < best;
+        const bool same = Val<T>::get(v) == Val<T>::get(decode<T>(best));
+        if (beats) {
+            if (!same || a < 0) a = e;
+            best = key;
+        } else if (same && a < 0) {
+            a = e;
+        }
+    }
+    __device__ T finish(int out_given, int64_t& arg, int64_t E) const {
+        T r = decode<T>(best);
+        arg = a < 0 ? E : a;
+        if (!out_given && bits_of(r) == bits_of(identity<T, OP>())) {
+            r = Val<T>::put(0);
+            arg = E;
+        }
+        return r;
+    }
+};
+
+template <typename T, int VEC> struct alignas(sizeof(T) * VEC) VecT {
+    T v[VEC];
+};
+
+// One lane per VEC consecutive output elements along k; the segment's source
+// rows are read four positions at a time so that four vector loads are in
+// flight per lane.  VEC > 1 needs K % VEC == 0 and src / out aligned to
+// VEC * sizeof(T) (checked at launch).
+template <typename T, int OP, int VEC>
+__global__ __launch_bounds__(256) void scatter_segment_gather_kernel(const T* __restrict__ src,
+                                                                     const int* __restrict__ start,
+                                                                     const int* __restrict__ order,
+                                                                     T* __restrict__ out, int64_t* __restrict__ arg,
+                                                                     Geom g, int out_given) {
+    using State = std::conditional_t<OP == kMin || OP == kMax, SegArg<T, OP>, SegAcc<T, OP>>;
+    using V = VecT<T, VEC>;
+    const int64_t KV = g.K / VEC;
+    const int64_t n_vec = g.B * g.N * KV;
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n_vec; o += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t kv = o % KV;
+        const int64_t bn = o / KV;
+        const int n = (int)(bn % g.N);
+        const int64_t b = bn / g.N;
+        const T* sp = src + b * g.E * g.K + kv * VEC;
+        const int64_t ob = bn * g.K + kv * VEC;
+        const int s0 = start[n], s1 = start[n + 1];
+        State st[VEC];
+        if (out_given) {
+            const V o0 = *reinterpret_cast<const V*>(out + ob);
+#pragma unroll
+            for (int i = 0; i < VEC; i++) st[i].init(o0.v[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < VEC; i++) st[i].init(identity<T, OP>());
+        }
+        int j = s0;
+        for (; j + 4 <= s1; j += 4) {
+            int e[4];
+            V v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) e[u] = order[j + u];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = *reinterpret_cast<const V*>(sp + (int64_t)e[u] * g.K);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int i = 0; i < VEC; i++) st[i].add(v[u].v[i], e[u]);
+        }
+        for (; j < s1; j++) {
+            const int e = order[j];
+            const V v = *reinterpret_cast<const V*>(sp + (int64_t)e * g.K);
+#pragma unroll
+            for (int i = 0; i < VEC; i++) st[i].add(v.v[i], e);
+        }
+        V r;
+        int64_t a[VEC];
+#pragma unroll
+        for (int i = 0; i < VEC; i++) r.v[i] = st[i].finish(out_given, a[i], g.E);
+        *reinterpret_cast<V*>(out + ob) = r;
+        if constexpr (OP == kMin || OP == kMax) {
+#pragma unroll
+            for (int i = 0; i < VEC; i++) arg[ob + i] = a[i];
+        }
+    }
+}
+
+bool segment_path(const Geom& g) {
+    return (g.B == 1 || g.sB == 0) && (g.K == 1 || g.sK == 0) && g.K >= kSegMinK && g.N >= 1 && g.N <= kSegMaxN &&
+           g.E >= 1 && g.E <= kSegMaxE;
+}
+
+size_t segment_workspace(const Geom& g) {
+    return segment_path(g) ? (size_t)(g.N + 1 + g.E) * sizeof(int) : 0;
+}
+
 template <typename T, int OP>
-int run_op(const void* src, const int64_t* index, const Geom& g, void* out, int64_t* arg, int out_given,
-           hipStream_t s) {
+int run_segmented(const void* src, const int64_t* index, const Geom& g, void* out, int64_t* arg, int out_given,
+                  void* ws, hipStream_t s) {
+    int* start = (int*)ws;
+    int* order = start + g.N + 1;
+    hipLaunchKernelGGL(scatter_segment_sort_kernel, dim3(1), dim3(256), 0, s, index, g.sE, (int)g.E, (int)g.N,
+                       start, order);
+    VH_LAUNCH_CHECK("scatter segment sort");
+    constexpr int VEC = 16 / sizeof(T) < 8 ? 16 / sizeof(T) : 8;   // 16-byte loads, at most 8 lanes of state
+    const bool vec_ok = g.K % VEC == 0 && reinterpret_cast<uintptr_t>(src) % (VEC * sizeof(T)) == 0 &&
+                        reinterpret_cast<uintptr_t>(out) % (VEC * sizeof(T)) == 0;
+    if (vec_ok)
+        hipLaunchKernelGGL((scatter_segment_gather_kernel<T, OP, VEC>), dim3(grid_for(g.B * g.N * (g.K / VEC))),
+                           dim3(256), 0, s, (const T*)src, start, order, (T*)out, arg, g, out_given);
+    else
+        hipLaunchKernelGGL((scatter_segment_gather_kernel<T, OP, 1>), dim3(grid_for(g.B * g.N * g.K)), dim3(256), 0,
+                           s, (const T*)src, start, order, (T*)out, arg, g, out_given);
+    VH_LAUNCH_CHECK("scatter segment gather");
+    return VH_OK;
+}
+
+template <typename T, int OP>
+int run_op(const void* src, const int64_t* index, const Geom& g, void* out, int64_t* arg, int out_given, void* ws,
+           size_t ws_bytes, hipStream_t s) {
     const int64_t n_src = g.B * g.E * g.K, n_out = g.B * g.N * g.K;
+    if (n_src > 0 && n_out > 0 && segment_path(g) && ws && ws_bytes >= segment_workspace(g))
+        return run_segmented<T, OP>(src, index, g, out, arg, out_given, ws, s);
     T* o = (T*)out;
     if (n_out > 0 && (OP == kMin || OP == kMax || !out_given)) {
         hipLaunchKernelGGL((scatter_prepare_kernel<T, OP>), dim3(grid_for(n_out)), dim3(256), 0, s, o, arg, n_out,
@@ -312,12 +535,12 @@ int run_op(const void* src, const int64_t* index, const Geom& g, void* out, int6
 
 template <typename T>
 int run_typed(int reduce, const void* src, const int64_t* index, const Geom& g, void* out, int64_t* arg,
-              int out_given, hipStream_t s) {
+              int out_given, void* ws, size_t ws_bytes, hipStream_t s) {
     switch (reduce) {
-        case kSum: return run_op<T, kSum>(src, index, g, out, arg, out_given, s);
-        case kMul: return run_op<T, kMul>(src, index, g, out, arg, out_given, s);
-        case kMin: return run_op<T, kMin>(src, index, g, out, arg, out_given, s);
-        case kMax: return run_op<T, kMax>(src, index, g, out, arg, out_given, s);
+        case kSum: return run_op<T, kSum>(src, index, g, out, arg, out_given, ws, ws_bytes, s);
+        case kMul: return run_op<T, kMul>(src, index, g, out, arg, out_given, ws, ws_bytes, s);
+        case kMin: return run_op<T, kMin>(src, index, g, out, arg, out_given, ws, ws_bytes, s);
+        case kMax: return run_op<T, kMax>(src, index, g, out, arg, out_given, ws, ws_bytes, s);
         default: return fail(VH_ERR_ARG, "scatter: unknown reduction %d", reduce);
     }
 }
@@ -327,9 +550,15 @@ int run_typed(int reduce, const void* src, const int64_t* index, const Geom& g, 
 
 using namespace vh;
 
+extern "C" size_t vh_scatter_workspace_size(int64_t B, int64_t E, int64_t K, int64_t sB, int64_t sE, int64_t sK,
+                                             int64_t N) {
+    if (B < 0 || E < 0 || K < 0 || N < 0) return 0;
+    return segment_workspace(Geom{B, E, K, N, sB, sE, sK});
+}
+
 extern "C" int vh_scatter(int reduce, int dtype, const void* src, const int64_t* index, int64_t B, int64_t E,
                           int64_t K, int64_t sB, int64_t sE, int64_t sK, int64_t N, void* out, int64_t* arg_out,
-                          int out_given, void* stream) {
+                          int out_given, void* workspace, size_t workspace_bytes, void* stream) {
     if (B < 0 || E < 0 || K < 0 || N < 0) return fail(VH_ERR_ARG, "scatter: negative size");
     if (reduce < kSum || reduce > kMax) return fail(VH_ERR_ARG, "scatter: unknown reduction %d", reduce);
     const int64_t n_src = B * E * K, n_out = B * N * K;
@@ -339,15 +568,15 @@ extern "C" int vh_scatter(int reduce, int dtype, const void* src, const int64_t*
     const Geom g{B, E, K, N, sB, sE, sK};
     hipStream_t s = (hipStream_t)stream;
     switch (dtype) {
-        case VH_F32: return run_typed<float>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_F64: return run_typed<double>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_F16: return run_typed<__half>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_BF16: return run_typed<__hip_bfloat16>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_U8: return run_typed<uint8_t>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_I8: return run_typed<int8_t>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_I16: return run_typed<int16_t>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_I32: return run_typed<int32_t>(reduce, src, index, g, out, arg_out, out_given, s);
-        case VH_I64: return run_typed<int64_t>(reduce, src, index, g, out, arg_out, out_given, s);
+        case VH_F32: return run_typed<float>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_F64: return run_typed<double>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_F16: return run_typed<__half>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_BF16: return run_typed<__hip_bfloat16>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_U8: return run_typed<uint8_t>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_I8: return run_typed<int8_t>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_I16: return run_typed<int16_t>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_I32: return run_typed<int32_t>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
+        case VH_I64: return run_typed<int64_t>(reduce, src, index, g, out, arg_out, out_given, workspace, workspace_bytes, s);
         default: return fail(VH_ERR_UNSUPPORTED, "scatter: dtype %d", dtype);
     }
 }

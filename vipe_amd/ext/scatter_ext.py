@@ -13,6 +13,10 @@ Semantics kept from the reference (cuda/scatter_cuda.cu:56-129):
   * min / max also return ``arg_out`` (int64), the source position along
     ``dim`` of the result, ``src.size(dim)`` where there is none; NaN sources
     are ignored.
+A 1-D index broadcast over the other axes (K >= 32 trailing elements) runs
+atomic-free over a stable sort of the index: the same bits on every run, sums
+and products taken in ascending source order, 8 / 16-bit floats accumulated in
+FP32 and rounded once.  Other index shapes use atomics.
 Differences, by design: ties in ``arg_out`` resolve to the smallest position
 (the reference keeps whichever write lands last); min / max values do not
 depend on the order of the atomics; indices outside ``[0, out.size(dim))``
@@ -114,9 +118,12 @@ def _scatter(src, index, dim, out, dim_size, reduce):
     K = src.numel() // (B * E)
     N = out.size(dim)
     index, sB, sE, sK = _index_strides(index, dim)
+    lib = _lib.load()
+    ws_bytes = lib.vh_scatter_workspace_size(B, E, K, sB, sE, sK, N)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=src.device) if ws_bytes else None
     with torch.cuda.device(src.device):
-        rc = _lib.load().vh_scatter(reduce, code, _lib.ptr(src), _lib.ptr(index), B, E, K, sB, sE, sK, N,
-                                    _lib.ptr(out), _lib.ptr(arg_out), int(out_given), _lib.stream_of(src))
+        rc = lib.vh_scatter(reduce, code, _lib.ptr(src), _lib.ptr(index), B, E, K, sB, sE, sK, N, _lib.ptr(out),
+                            _lib.ptr(arg_out), int(out_given), _lib.ptr(ws), ws_bytes, _lib.stream_of(src))
     _lib.check(rc, "scatter")
     return out, arg_out
 
