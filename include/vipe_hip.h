@@ -242,9 +242,14 @@ int vh_geom_depth_filter(const float* poses, const float* disps, const float* in
  * Every output / gradient element is written (no zero-fill needed). */
 int vh_spatial_corr_out_size(int H, int W, int kH, int kW, int padH, int padW, int dilH, int dilW, int dH, int dW,
                              int* oH, int* oW);
+/* Bytes of device workspace that let vh_spatial_corr_forward use the matrix
+ * cores (kernel 1, stride 1, F32 / F16 / BF16, (patchW-1)*dpW <= 48): two
+ * channels-last copies of the inputs.  0 when that path does not apply. */
+size_t vh_spatial_corr_workspace_size(int dtype, int N, int C, int H, int W, int kH, int kW, int patchH, int patchW,
+                                      int padH, int padW, int dilH, int dilW, int dpH, int dpW, int dH, int dW);
 int vh_spatial_corr_forward(int dtype, const void* in1, const void* in2, int N, int C, int H, int W, int kH, int kW,
                             int patchH, int patchW, int padH, int padW, int dilH, int dilW, int dpH, int dpW, int dH,
-                            int dW, void* out, void* stream);
+                            int dW, void* out, void* workspace, size_t workspace_bytes, void* stream);
 /* grad_out: [N,patchH,patchW,oH,oW] contiguous; grad1, grad2: [N,C,H,W]. */
 int vh_spatial_corr_backward(int dtype, const void* in1, const void* in2, const void* grad_out, int N, int C, int H,
                              int W, int kH, int kW, int patchH, int patchW, int padH, int padW, int dilH, int dilW,

@@ -20,6 +20,9 @@ CASES = [
     (3, 2, 3, 5, 2, 0, 2, 1, 2, 1, 2, 1),
     (2, 3, 4, 2, 1, 2, 1, 2, 3, 2, 1, 3),    # even patches, dilation_patch > 1
     (5, 1, 1, 3, 0, 3, 1, 1, 1, 1, 3, 2),
+    (1, 1, 5, 7, 2, 3, 1, 1, 1, 3, 1, 1),    # kernel 1 with padding, a 3-tile band
+    (1, 1, 3, 13, 0, 0, 1, 1, 2, 4, 1, 1),   # kernel 1, a 4-tile band ((13-1)*4 = 48)
+    (1, 1, 2, 4, 1, 0, 2, 1, 3, 2, 1, 1),    # kernel 1, even patches
 ]
 TOL = {torch.float32: 2e-6, torch.float16: 2 ** -11 + 2e-6, torch.bfloat16: 2 ** -8 + 2e-6}
 

@@ -80,7 +80,8 @@ _SIGS = {
     "vh_geom_depth_filter": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp]),
     # corr_ext spatial correlation sampler
     "vh_spatial_corr_out_size": (i32, [i32] * 10 + [C.POINTER(i32), C.POINTER(i32)]),
-    "vh_spatial_corr_forward": (i32, [i32, vp, vp] + [i32] * 16 + [vp, vp]),
+    "vh_spatial_corr_workspace_size": (sz, [i32] * 17),
+    "vh_spatial_corr_forward": (i32, [i32, vp, vp] + [i32] * 16 + [vp, vp, sz, vp]),
     "vh_spatial_corr_backward": (i32, [i32, vp, vp, vp] + [i32] * 16 + [vp, vp, vp]),
     # scatter_ext
     "vh_scatter_workspace_size": (sz, [i64] * 7),

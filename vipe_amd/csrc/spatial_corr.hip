@@ -8,8 +8,10 @@
 // are [N,C,H,W]; the output is [N,patchH,patchW,oH,oW] with
 // oH = (H + 2 padH - ((kH-1) dilH + 1)) / dH + 1 (same for W).
 //
-// gfx950 layout: one lane per output element, consecutive lanes on
-// consecutive output columns, so every channel step reads two coalesced rows;
+// Kernel 1 / stride 1 (the local-attention form) runs on the matrix cores (see
+// spatial_corr_mfma_kernel) when the caller passes the workspace for the
+// channels-last copies.  Otherwise, one lane per output element, consecutive
+// lanes on consecutive output columns, so every channel step reads two coalesced rows;
 // the channel sum accumulates in FP32 (FP64 for f64 inputs).  The gradients are
 // gathers: one lane per input element sums the output gradients that read it,
 // so there are no atomics and the result is deterministic.  For odd patch
@@ -141,6 +143,205 @@ __global__ __launch_bounds__(64) void spatial_corr_bwd_kernel(const T* __restric
     grad[((int64_t)n * a.C + c) * HW + (int64_t)y * a.W + x] = store_acc<T>(acc);
 }
 
+
+// ---- matrix-core path: kernel 1, stride 1 (the local-attention form) ----------
+// For one output row h and shift row ph the outputs over (w, pw) are the band
+// x' = x1(w) + sw(pw) of the product S = A^T B, A = in1 row y1 (C x W),
+// B = in2 row y1 + sh (C x W).  A wave computes S for 16 positions w and
+// 16 * NT columns x' (NT = ceil((16 + (patchW-1) dpW) / 16)) on the matrix cores,
+// and writes the band's diagonals to out[.., ph, pw, h, w].  The inputs are first
+// copied channels-last with the channels padded to 32, so one 16-byte load per
+// lane feeds four v_mfma_f32_16x16x4f32 or one v_mfma_f32_16x16x32_{f16,bf16}.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// channel padding of the channels-last copies: a multiple of 128 bytes, so that
+// the mfma kernel's count of 64-byte steps is even
+constexpr int kClPadBytes = 128;
+constexpr int kMaxNT = 4;
+constexpr uint32_t kScRange = 0x80000000u;    // num_records of the operand resources (bytes)
+constexpr uint32_t kScInvalid = 0x80000000u;  // offset of an off-map pixel: past kScRange
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sc_rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)kScRange, 0x00020000);
+}
+
+struct alignas(16) Q16 {
+    uint32_t w[4];
+};
+
+__device__ __forceinline__ Q16 sc_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(Q16, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+template <typename T> struct Mfma16;
+template <> struct Mfma16<float> {
+    __device__ static f32x4 run(const Q16& a, const Q16& b, f32x4 acc) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.w[q]),
+                                                       __builtin_bit_cast(float, b.w[q]), acc, 0, 0, 0);
+        return acc;
+    }
+};
+template <> struct Mfma16<__half> {
+    __device__ static f32x4 run(const Q16& a, const Q16& b, f32x4 acc) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), acc,
+                                                      0, 0, 0);
+    }
+};
+template <> struct Mfma16<__hip_bfloat16> {
+    __device__ static f32x4 run(const Q16& a, const Q16& b, f32x4 acc) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                       acc, 0, 0, 0);
+    }
+};
+
+// [N,C,H,W] -> [N,H*W,Cp] with zeros for c >= C; 32 channels x 64 pixels per block.
+template <typename T>
+__global__ __launch_bounds__(256) void spatial_corr_to_cl_kernel(const T* __restrict__ in, T* __restrict__ out, int C,
+                                                                 int HW, int Cp) {
+    __shared__ T tile[32][65];
+    const int p0 = blockIdx.x * 64, c0 = blockIdx.y * 32, n = blockIdx.z;
+    const int tid = threadIdx.x;
+    const T* src = in + (int64_t)n * C * HW;
+    for (int i = tid; i < 32 * 64; i += 256) {
+        const int c = i / 64, p = i % 64;
+        const int cc = c0 + c, pp = p0 + p;
+        tile[c][p] = (cc < C && pp < HW) ? src[(int64_t)cc * HW + pp] : from_acc<T>(0.f);
+    }
+    __syncthreads();
+    T* dst = out + (int64_t)n * HW * Cp;
+    for (int i = tid; i < 32 * 64; i += 256) {
+        const int p = i / 32, c = i % 32;
+        const int pp = p0 + p;
+        if (pp < HW) dst[(int64_t)pp * Cp + c0 + c] = tile[c][p];
+    }
+}
+
+// grid (ceil(oW/64), oH * patchH, N), 4 waves; wave v covers w0 = 64 bx + 16 v.
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void spatial_corr_mfma_kernel(const T* __restrict__ a_cl, const T* __restrict__ b_cl,
+                                                                T* __restrict__ out, SCArgs a, int Cp) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w0 = blockIdx.x * 64 + wave * 16;
+    if (w0 >= a.oW) return;   // no barriers below
+    const int h = blockIdx.y / a.patchH, ph = blockIdx.y % a.patchH;
+    const int n = blockIdx.z;
+    const int radH = a.dpH * (a.patchH - 1) / 2, radW = a.dpW * (a.patchW - 1) / 2;
+    const int y1 = h - a.padH, y2 = y1 + ph * a.dpH - radH;
+    const int g = lane >> 4, c16 = lane & 15;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (y1 >= 0 && y1 < a.H && y2 >= 0 && y2 < a.H) {
+        // Operands come through buffer resources based at the channels-last
+        // copies: a lane whose pixel lies off the map gets an offset past the
+        // resource's range and reads zeros without a memory access.  At each
+        // step the four lane groups take the four 16-byte chunks of the next 64
+        // bytes of a pixel's channels (the same channel-to-k map for both
+        // operands), so the four groups of a pixel read 64 contiguous bytes.
+        const int x1 = w0 + c16 - a.padW;                 // A row i = c16
+        const int xb = w0 - a.padW - radW;                // x' of tile 0, column 0
+        const __amdgpu_buffer_rsrc_t ra = sc_rsrc(a_cl), rb = sc_rsrc(b_cl);
+        const uint32_t lane_off = (uint32_t)g * 16u;
+        const uint32_t aoff = (x1 >= 0 && x1 < a.W)
+                                  ? (uint32_t)((((int64_t)n * a.H + y1) * a.W + x1) * Cp * (int64_t)sizeof(T)) + lane_off
+                                  : kScInvalid;
+        uint32_t boff[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int x2 = xb + 16 * t + c16;
+            boff[t] = (x2 >= 0 && x2 < a.W)
+                          ? (uint32_t)((((int64_t)n * a.H + y2) * a.W + x2) * Cp * (int64_t)sizeof(T)) + lane_off
+                          : kScInvalid;
+        }
+        const int steps = Cp * (int)sizeof(T) / 64;   // 64 bytes of channels per step and pixel; even
+        // two register sets in turn: the loads of one step are in flight while
+        // the other step's products run
+        Q16 a0 = sc_load(ra, aoff), b0[NT], a1, b1[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) b0[t] = sc_load(rb, boff[t]);
+        for (int s = 0; s < steps; s += 2) {
+            const uint32_t o1 = 64u * (uint32_t)(s + 1);
+            a1 = sc_load(ra, aoff + o1);
+#pragma unroll
+            for (int t = 0; t < NT; t++) b1[t] = sc_load(rb, boff[t] + o1);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[t] = Mfma16<T>::run(a0, b0[t], acc[t]);
+            // the step after next, or this step again at the end (in range, unused)
+            const uint32_t o2 = 64u * (uint32_t)(s + 2 < steps ? s + 2 : s);
+            a0 = sc_load(ra, aoff + o2);
+#pragma unroll
+            for (int t = 0; t < NT; t++) b0[t] = sc_load(rb, boff[t] + o2);
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[t] = Mfma16<T>::run(a1, b1[t], acc[t]);
+        }
+    }
+    // D[row][col] of tile t sits in lane col + 16 (row / 4), register row % 4.
+    const int64_t plane = (int64_t)a.oH * a.oW;
+    T* ob = out + (((int64_t)n * a.patchH + ph) * a.patchW) * plane + (int64_t)h * a.oW;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = 4 * g + r;
+            const int diff = 16 * t + c16 - row;   // = pw * dpW
+            const int w = w0 + row;
+            if (diff < 0 || diff % a.dpW || w >= a.oW) continue;
+            const int pw = diff / a.dpW;
+            if (pw < a.patchW) ob[(int64_t)pw * plane + w] = from_acc<T>(acc[t][r]);
+        }
+    }
+}
+
+int cl_channels(int C, size_t elem) {
+    const int pad = kClPadBytes / (int)elem;
+    return (C + pad - 1) / pad * pad;
+}
+
+bool mfma_path(int dtype, const SCArgs& a) {
+    if (dtype != VH_F32 && dtype != VH_F16 && dtype != VH_BF16) return false;
+    if (a.kH != 1 || a.kW != 1 || a.dH != 1 || a.dW != 1) return false;
+    const int nt = (16 + (a.patchW - 1) * a.dpW + 15) / 16;
+    const size_t elem = dtype == VH_F32 ? 4 : 2;
+    const int64_t copy_bytes = (int64_t)a.N * a.H * a.W * cl_channels(a.C, elem) * (int64_t)elem;
+    return nt <= kMaxNT && (int64_t)a.oH * a.patchH <= 65535 && a.N <= 65535 &&
+           copy_bytes + 64 < (int64_t)kScRange;
+}
+
+size_t mfma_workspace(int dtype, const SCArgs& a) {
+    if (!mfma_path(dtype, a)) return 0;
+    const size_t elem = dtype == VH_F32 ? 4 : 2;
+    const size_t one = (size_t)a.N * a.H * a.W * cl_channels(a.C, elem) * elem;
+    return 2 * ((one + 255) / 256 * 256);
+}
+
+template <typename T>
+int fwd_mfma_launch(const void* in1, const void* in2, void* out, const SCArgs& a, void* ws, hipStream_t s) {
+    const int Cp = cl_channels(a.C, sizeof(T));
+    const int HW = a.H * a.W;
+    const size_t one = ((size_t)a.N * HW * Cp * sizeof(T) + 255) / 256 * 256;
+    T* a_cl = (T*)ws;
+    T* b_cl = (T*)((char*)ws + one);
+    const dim3 tg((unsigned)((HW + 63) / 64), (unsigned)(Cp / 32), (unsigned)a.N);
+    hipLaunchKernelGGL(spatial_corr_to_cl_kernel<T>, tg, dim3(256), 0, s, (const T*)in1, a_cl, a.C, HW, Cp);
+    VH_LAUNCH_CHECK("spatial_corr channels-last copy");
+    hipLaunchKernelGGL(spatial_corr_to_cl_kernel<T>, tg, dim3(256), 0, s, (const T*)in2, b_cl, a.C, HW, Cp);
+    VH_LAUNCH_CHECK("spatial_corr channels-last copy");
+    const dim3 grid((unsigned)((a.oW + 63) / 64), (unsigned)(a.oH * a.patchH), (unsigned)a.N);
+    const int nt = (16 + (a.patchW - 1) * a.dpW + 15) / 16;
+    switch (nt) {
+        case 1: hipLaunchKernelGGL((spatial_corr_mfma_kernel<T, 1>), grid, dim3(256), 0, s, a_cl, b_cl, (T*)out, a, Cp); break;
+        case 2: hipLaunchKernelGGL((spatial_corr_mfma_kernel<T, 2>), grid, dim3(256), 0, s, a_cl, b_cl, (T*)out, a, Cp); break;
+        case 3: hipLaunchKernelGGL((spatial_corr_mfma_kernel<T, 3>), grid, dim3(256), 0, s, a_cl, b_cl, (T*)out, a, Cp); break;
+        default: hipLaunchKernelGGL((spatial_corr_mfma_kernel<T, 4>), grid, dim3(256), 0, s, a_cl, b_cl, (T*)out, a, Cp); break;
+    }
+    VH_LAUNCH_CHECK("spatial_corr_forward (matrix cores)");
+    return VH_OK;
+}
+
 int check_args(const SCArgs& a, const char* what) {
     if (a.N < 0 || a.C <= 0 || a.H <= 0 || a.W <= 0 || a.kH <= 0 || a.kW <= 0 || a.patchH <= 0 || a.patchW <= 0 ||
         a.padH < 0 || a.padW < 0 || a.dilH <= 0 || a.dilW <= 0 || a.dpH <= 0 || a.dpW <= 0 || a.dH <= 0 ||
@@ -199,14 +400,30 @@ int vh_spatial_corr_out_size(int H, int W, int kH, int kW, int padH, int padW, i
     return VH_OK;
 }
 
+size_t vh_spatial_corr_workspace_size(int dtype, int N, int C, int H, int W, int kH, int kW, int patchH, int patchW,
+                                      int padH, int padW, int dilH, int dilW, int dpH, int dpW, int dH, int dW) {
+    const SCArgs a = make_args(N, C, H, W, kH, kW, patchH, patchW, padH, padW, dilH, dilW, dpH, dpW, dH, dW);
+    if (check_args(a, "spatial_corr_workspace_size") != VH_OK) return 0;
+    return mfma_workspace(dtype, a);
+}
+
 int vh_spatial_corr_forward(int dtype, const void* in1, const void* in2, int N, int C, int H, int W, int kH, int kW,
                             int patchH, int patchW, int padH, int padW, int dilH, int dilW, int dpH, int dpW, int dH,
-                            int dW, void* out, void* stream) {
+                            int dW, void* out, void* workspace, size_t workspace_bytes, void* stream) {
     const SCArgs a = make_args(N, C, H, W, kH, kW, patchH, patchW, padH, padW, dilH, dilW, dpH, dpW, dH, dW);
     if (int rc = check_args(a, "spatial_corr_forward")) return rc;
     if (N == 0) return VH_OK;
     if (!in1 || !in2 || !out) return fail(VH_ERR_ARG, "spatial_corr_forward: null pointer");
     hipStream_t s = (hipStream_t)stream;
+    const size_t need = mfma_workspace(dtype, a);
+    if (need > 0 && workspace && workspace_bytes >= need) {
+        switch (dtype) {
+            case VH_F32: return fwd_mfma_launch<float>(in1, in2, out, a, workspace, s);
+            case VH_F16: return fwd_mfma_launch<__half>(in1, in2, out, a, workspace, s);
+            case VH_BF16: return fwd_mfma_launch<__hip_bfloat16>(in1, in2, out, a, workspace, s);
+            default: break;
+        }
+    }
     switch (dtype) {
         case VH_F32: return fwd_launch<float>(in1, in2, out, a, s);
         case VH_F64: return fwd_launch<double>(in1, in2, out, a, s);

@@ -11,7 +11,9 @@ The same checks as the reference raise RuntimeError with its messages
 ("input1 must be a CUDA tensor", "input1 must be contiguous", "input1 is not
 on same device as input2").  The reference's CPU path has no counterpart
 here: host tensors raise.  Sums accumulate in FP32 (FP64 for float64 inputs),
-where the reference accumulates half inputs in half.
+where the reference accumulates half inputs in half.  Kernel size 1 with
+stride 1 (the local-attention form) runs on the matrix cores over
+channels-last copies of the inputs (a workspace allocated per call).
 """
 from __future__ import annotations
 
@@ -72,11 +74,15 @@ def forward(input1, input2, kH, kW, patchH, patchW, padH, padW, dilationH, dilat
     out = torch.empty((N, patchH, patchW, oH, oW), dtype=input1.dtype, device=input1.device)
     if out.numel() == 0:
         return out
+    lib = _lib.load()
+    code = _dtype(input1)
+    ints = (N, Cc, H, W, kH, kW, patchH, patchW, padH, padW, dilationH, dilationW, dilation_patchH, dilation_patchW,
+            dH, dW)
+    ws_bytes = lib.vh_spatial_corr_workspace_size(code, *ints)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=input1.device) if ws_bytes else None
     with torch.cuda.device(input1.device):
-        rc = _lib.load().vh_spatial_corr_forward(_dtype(input1), _lib.ptr(input1), _lib.ptr(input2), N, Cc, H, W,
-                                                 kH, kW, patchH, patchW, padH, padW, dilationH, dilationW,
-                                                 dilation_patchH, dilation_patchW, dH, dW, _lib.ptr(out),
-                                                 _lib.stream_of(input1))
+        rc = lib.vh_spatial_corr_forward(code, _lib.ptr(input1), _lib.ptr(input2), *ints, _lib.ptr(out),
+                                         _lib.ptr(ws), ws_bytes, _lib.stream_of(input1))
     _lib.check(rc, "corr_ext.forward")
     return out
 
