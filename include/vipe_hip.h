@@ -250,10 +250,17 @@ size_t vh_spatial_corr_workspace_size(int dtype, int N, int C, int H, int W, int
 int vh_spatial_corr_forward(int dtype, const void* in1, const void* in2, int N, int C, int H, int W, int kH, int kW,
                             int patchH, int patchW, int padH, int padW, int dilH, int dilW, int dpH, int dpW, int dH,
                             int dW, void* out, void* workspace, size_t workspace_bytes, void* stream);
+/* Bytes of device workspace that let vh_spatial_corr_backward use the matrix
+ * cores (kernel 1, stride 1, F32 / F16 / BF16, (patchW-1)*dpW <= 48): two
+ * channels-last FP32 copies of the inputs.  0 when that path does not apply. */
+size_t vh_spatial_corr_backward_workspace_size(int dtype, int N, int C, int H, int W, int kH, int kW, int patchH,
+                                               int patchW, int padH, int padW, int dilH, int dilW, int dpH, int dpW,
+                                               int dH, int dW);
 /* grad_out: [N,patchH,patchW,oH,oW] contiguous; grad1, grad2: [N,C,H,W]. */
 int vh_spatial_corr_backward(int dtype, const void* in1, const void* in2, const void* grad_out, int N, int C, int H,
                              int W, int kH, int kW, int patchH, int patchW, int padH, int padW, int dilH, int dilW,
-                             int dpH, int dpW, int dH, int dW, void* grad1, void* grad2, void* stream);
+                             int dpH, int dpW, int dH, int dW, void* grad1, void* grad2, void* workspace,
+                             size_t workspace_bytes, void* stream);
 
 /* ---- scatter_ext -----------------------------------------------------------
  * Replaces scatter_cuda (csrc/scatter_ext/cuda/scatter_cuda.cu:56-129), the

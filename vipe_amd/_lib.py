@@ -82,7 +82,8 @@ _SIGS = {
     "vh_spatial_corr_out_size": (i32, [i32] * 10 + [C.POINTER(i32), C.POINTER(i32)]),
     "vh_spatial_corr_workspace_size": (sz, [i32] * 17),
     "vh_spatial_corr_forward": (i32, [i32, vp, vp] + [i32] * 16 + [vp, vp, sz, vp]),
-    "vh_spatial_corr_backward": (i32, [i32, vp, vp, vp] + [i32] * 16 + [vp, vp, vp]),
+    "vh_spatial_corr_backward_workspace_size": (sz, [i32] * 17),
+    "vh_spatial_corr_backward": (i32, [i32, vp, vp, vp] + [i32] * 16 + [vp, vp, vp, sz, vp]),
     # scatter_ext
     "vh_scatter_workspace_size": (sz, [i64] * 7),
     "vh_scatter": (i32, [i32, i32, vp, vp] + [i64] * 7 + [vp, vp, i32, vp, sz, vp]),

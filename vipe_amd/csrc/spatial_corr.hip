@@ -198,21 +198,22 @@ template <> struct Mfma16<__hip_bfloat16> {
     }
 };
 
-// [N,C,H,W] -> [N,H*W,Cp] with zeros for c >= C; 32 channels x 64 pixels per block.
-template <typename T>
-__global__ __launch_bounds__(256) void spatial_corr_to_cl_kernel(const T* __restrict__ in, T* __restrict__ out, int C,
+// [N,C,H,W] -> [N,H*W,Cp] with zeros for c >= C, converted to TO; 32 channels
+// x 64 pixels per block (Cp a multiple of 32).
+template <typename T, typename TO = T>
+__global__ __launch_bounds__(256) void spatial_corr_to_cl_kernel(const T* __restrict__ in, TO* __restrict__ out, int C,
                                                                  int HW, int Cp) {
-    __shared__ T tile[32][65];
+    __shared__ TO tile[32][65];
     const int p0 = blockIdx.x * 64, c0 = blockIdx.y * 32, n = blockIdx.z;
     const int tid = threadIdx.x;
     const T* src = in + (int64_t)n * C * HW;
     for (int i = tid; i < 32 * 64; i += 256) {
         const int c = i / 64, p = i % 64;
         const int cc = c0 + c, pp = p0 + p;
-        tile[c][p] = (cc < C && pp < HW) ? src[(int64_t)cc * HW + pp] : from_acc<T>(0.f);
+        tile[c][p] = from_acc<TO>((cc < C && pp < HW) ? to_f<T>(src[(int64_t)cc * HW + pp]) : 0.f);
     }
     __syncthreads();
-    T* dst = out + (int64_t)n * HW * Cp;
+    TO* dst = out + (int64_t)n * HW * Cp;
     for (int i = tid; i < 32 * 64; i += 256) {
         const int p = i / 32, c = i % 32;
         const int pp = p0 + p;
@@ -342,6 +343,158 @@ int fwd_mfma_launch(const void* in1, const void* in2, void* out, const SCArgs& a
     return VH_OK;
 }
 
+
+// ---- matrix-core backward: kernel 1, stride 1 ----------------------------------
+// With the backward's patch centre rb = ((patch-1)/2) dp, for one row y and
+// shift row ph:
+//   grad1[c][y][x]   = sum_x2 G1[x][x2] in2[c][y + sh][x2],
+//                      G1[x][x2] = gout[ph][pw][y + padH][x + padW], pw dpW = x2 - x + rbW
+//   grad2[c][y2][x2] = sum_x1 G2[x2][x1] in1[c][y2 - sh][x1],
+//                      G2[x2][x1] = gout[ph][pw][y2 - sh + padH][x1 + padW], pw dpW = x2 - x1 + rbW
+// a banded product over 16 + (patchW-1) dpW columns.  A wave owns 16 output
+// positions x and up to kBwdCT tiles of 16 channels; per shift row it gathers its
+// band of G once (the A operand, zero off the band) and multiplies it with
+// channels-last FP32 copies of the other input (B) on v_mfma_f32_16x16x4f32,
+// accumulating over all shift rows in registers.  Each lane then writes four
+// consecutive positions of one channel: no atomics, one writer per element.
+constexpr int kBwdCT = 4;     // channel tiles of 16 per wave
+constexpr int kBwdKS = 16;    // at most 64 band columns (16 steps of 4)
+
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ uint32_t buf_u16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+template <typename T> __device__ __forceinline__ float load_g(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ float load_g<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return buf_f32(r, off);
+}
+template <> __device__ __forceinline__ float load_g<__half>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __half2float(__builtin_bit_cast(__half, (unsigned short)buf_u16(r, off)));
+}
+template <> __device__ __forceinline__ float load_g<__hip_bfloat16>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __uint_as_float(buf_u16(r, off) << 16);
+}
+
+// grid (ceil(ceil(W/16) / 4), H, N * ceil(Cp / (16 kBwdCT))), 4 waves
+template <typename T, bool SECOND>
+__global__ __launch_bounds__(256) void spatial_corr_bwd_mfma_kernel(const T* __restrict__ gout,
+                                                                    const float* __restrict__ other_cl,
+                                                                    T* __restrict__ grad, SCArgs a, int Cp) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int x0 = (blockIdx.x * 4 + wave) * 16;
+    if (x0 >= a.W) return;   // no barriers below
+    const int y = blockIdx.y;
+    const int ncg = (Cp + 16 * kBwdCT - 1) / (16 * kBwdCT);
+    const int n = blockIdx.z / ncg, cg = blockIdx.z % ncg;
+    const int nct = min(kBwdCT, Cp / 16 - cg * kBwdCT);
+    const int i16 = lane & 15, g = lane >> 4;
+    const int rbH = (a.patchH - 1) / 2 * a.dpH, rbW = (a.patchW - 1) / 2 * a.dpW;
+    const int span = (a.patchW - 1) * a.dpW;
+    const int ksteps = (16 + span + 3) / 4;
+    const int kb = SECOND ? x0 + rbW - span : x0 - rbW;   // band column of k = 0
+    const int64_t plane = (int64_t)a.oH * a.oW;
+    const __amdgpu_buffer_rsrc_t rg = sc_rsrc(gout + (int64_t)n * a.patchH * a.patchW * plane);
+    const __amdgpu_buffer_rsrc_t ro = sc_rsrc(other_cl + (int64_t)n * a.H * a.W * Cp);
+    f32x4 acc[kBwdCT];
+#pragma unroll
+    for (int t = 0; t < kBwdCT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ph = 0; ph < a.patchH; ph++) {
+        const int sh = ph * a.dpH - rbH;
+        const int yo = SECOND ? y - sh : y + sh;          // row of the other input
+        if (yo < 0 || yo >= a.H) continue;
+        const int hrow = (SECOND ? yo : y) + a.padH;      // output row of gout
+        // A: G[i][k] for k = 4 s + g
+        float af[kBwdKS];
+#pragma unroll
+        for (int st = 0; st < kBwdKS; st++) {
+            af[st] = 0.f;
+            if (st < ksteps) {
+                const int kk = 4 * st + g;
+                const int xk = kb + kk;                    // x2 (first) / x1 (second)
+                const int xo = x0 + i16;                   // x (first) / x2 (second)
+                const int pwd = SECOND ? span + i16 - kk : kk - i16;
+                const int gcol = (SECOND ? xk : xo) + a.padW;
+                const bool ok = pwd >= 0 && pwd % a.dpW == 0 && pwd / a.dpW < a.patchW && xo < a.W &&
+                                (!SECOND || (xk >= 0 && xk < a.W));
+                const int64_t off = ((int64_t)(ph * a.patchW + (ok ? pwd / a.dpW : 0)) * plane +
+                                     (int64_t)hrow * a.oW + (ok ? gcol : 0)) * (int64_t)sizeof(T);
+                af[st] = load_g<T>(rg, ok ? (uint32_t)off : kScInvalid);
+            }
+        }
+        // band step outer, channel tiles inner: kBwdCT independent accumulations
+#pragma unroll
+        for (int st = 0; st < kBwdKS; st++) {
+            if (st >= ksteps) break;
+            const int xk = kb + 4 * st + g;
+            const bool ok = xk >= 0 && xk < a.W;
+            const uint32_t base = (uint32_t)((((int64_t)yo * a.W + (ok ? xk : 0)) * Cp + cg * kBwdCT * 16 + i16) * 4);
+            float bv[kBwdCT];
+#pragma unroll
+            for (int t = 0; t < kBwdCT; t++) bv[t] = buf_f32(ro, ok && t < nct ? base + 64u * t : kScInvalid);
+#pragma unroll
+            for (int t = 0; t < kBwdCT; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[st], bv[t], acc[t], 0, 0, 0);
+        }
+    }
+    // D[row = x][col = c]: lane c + 16 (row / 4), register row % 4
+    const int64_t HW = (int64_t)a.H * a.W;
+#pragma unroll
+    for (int t = 0; t < kBwdCT; t++) {
+        if (t >= nct) break;
+        const int c = (cg * kBwdCT + t) * 16 + i16;
+        if (c >= a.C) continue;
+        T* gp = grad + ((int64_t)n * a.C + c) * HW + (int64_t)y * a.W;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int x = x0 + 4 * g + r;
+            if (x < a.W) gp[x] = from_acc<T>(acc[t][r]);
+        }
+    }
+}
+
+bool bwd_mfma_path(int dtype, const SCArgs& a) {
+    if (dtype != VH_F32 && dtype != VH_F16 && dtype != VH_BF16) return false;
+    if (a.kH != 1 || a.kW != 1 || a.dH != 1 || a.dW != 1) return false;
+    if ((a.patchW - 1) * a.dpW > 4 * kBwdKS - 16) return false;
+    const int Cp = cl_channels(a.C, 4);
+    const int64_t copy_bytes = (int64_t)a.N * a.H * a.W * Cp * 4;
+    const int64_t gout_bytes = (int64_t)a.patchH * a.patchW * a.oH * a.oW * (dtype == VH_F32 ? 4 : 2);
+    const int64_t ncg = (Cp + 16 * kBwdCT - 1) / (16 * kBwdCT);
+    return copy_bytes / a.N + 64 < (int64_t)kScRange && gout_bytes + 64 < (int64_t)kScRange &&
+           (int64_t)a.N * ncg <= 65535 && a.H <= 65535;
+}
+
+size_t bwd_mfma_workspace(int dtype, const SCArgs& a) {
+    if (!bwd_mfma_path(dtype, a)) return 0;
+    const size_t one = (size_t)a.N * a.H * a.W * cl_channels(a.C, 4) * 4;
+    return 2 * ((one + 255) / 256 * 256);
+}
+
+template <typename T>
+int bwd_mfma_launch(const void* in1, const void* in2, const void* gout, void* g1, void* g2, const SCArgs& a, void* ws,
+                    hipStream_t s) {
+    const int Cp = cl_channels(a.C, 4);
+    const int HW = a.H * a.W;
+    const size_t one = ((size_t)a.N * HW * Cp * 4 + 255) / 256 * 256;
+    float* in1_cl = (float*)ws;
+    float* in2_cl = (float*)((char*)ws + one);
+    const dim3 tg((unsigned)((HW + 63) / 64), (unsigned)(Cp / 32), (unsigned)a.N);
+    hipLaunchKernelGGL((spatial_corr_to_cl_kernel<T, float>), tg, dim3(256), 0, s, (const T*)in1, in1_cl, a.C, HW, Cp);
+    VH_LAUNCH_CHECK("spatial_corr channels-last copy");
+    hipLaunchKernelGGL((spatial_corr_to_cl_kernel<T, float>), tg, dim3(256), 0, s, (const T*)in2, in2_cl, a.C, HW, Cp);
+    VH_LAUNCH_CHECK("spatial_corr channels-last copy");
+    const int ncg = (Cp + 16 * kBwdCT - 1) / (16 * kBwdCT);
+    const dim3 grid((unsigned)(((a.W + 15) / 16 + 3) / 4), (unsigned)a.H, (unsigned)(a.N * ncg));
+    hipLaunchKernelGGL((spatial_corr_bwd_mfma_kernel<T, false>), grid, dim3(256), 0, s, (const T*)gout,
+                       (const float*)in2_cl, (T*)g1, a, Cp);
+    VH_LAUNCH_CHECK("spatial_corr_backward input1 (matrix cores)");
+    hipLaunchKernelGGL((spatial_corr_bwd_mfma_kernel<T, true>), grid, dim3(256), 0, s, (const T*)gout,
+                       (const float*)in1_cl, (T*)g2, a, Cp);
+    VH_LAUNCH_CHECK("spatial_corr_backward input2 (matrix cores)");
+    return VH_OK;
+}
+
 int check_args(const SCArgs& a, const char* what) {
     if (a.N < 0 || a.C <= 0 || a.H <= 0 || a.W <= 0 || a.kH <= 0 || a.kW <= 0 || a.patchH <= 0 || a.patchW <= 0 ||
         a.padH < 0 || a.padW < 0 || a.dilH <= 0 || a.dilW <= 0 || a.dpH <= 0 || a.dpW <= 0 || a.dH <= 0 ||
@@ -433,14 +586,32 @@ int vh_spatial_corr_forward(int dtype, const void* in1, const void* in2, int N, 
     }
 }
 
+size_t vh_spatial_corr_backward_workspace_size(int dtype, int N, int C, int H, int W, int kH, int kW, int patchH,
+                                               int patchW, int padH, int padW, int dilH, int dilW, int dpH, int dpW,
+                                               int dH, int dW) {
+    const SCArgs a = make_args(N, C, H, W, kH, kW, patchH, patchW, padH, padW, dilH, dilW, dpH, dpW, dH, dW);
+    if (check_args(a, "spatial_corr_backward_workspace_size") != VH_OK) return 0;
+    return bwd_mfma_workspace(dtype, a);
+}
+
 int vh_spatial_corr_backward(int dtype, const void* in1, const void* in2, const void* grad_out, int N, int C, int H,
                              int W, int kH, int kW, int patchH, int patchW, int padH, int padW, int dilH, int dilW,
-                             int dpH, int dpW, int dH, int dW, void* grad1, void* grad2, void* stream) {
+                             int dpH, int dpW, int dH, int dW, void* grad1, void* grad2, void* workspace,
+                             size_t workspace_bytes, void* stream) {
     const SCArgs a = make_args(N, C, H, W, kH, kW, patchH, patchW, padH, padW, dilH, dilW, dpH, dpW, dH, dW);
     if (int rc = check_args(a, "spatial_corr_backward")) return rc;
     if (N == 0) return VH_OK;
     if (!in1 || !in2 || !grad_out || !grad1 || !grad2) return fail(VH_ERR_ARG, "spatial_corr_backward: null pointer");
     hipStream_t s = (hipStream_t)stream;
+    const size_t need = bwd_mfma_workspace(dtype, a);
+    if (need > 0 && workspace && workspace_bytes >= need) {
+        switch (dtype) {
+            case VH_F32: return bwd_mfma_launch<float>(in1, in2, grad_out, grad1, grad2, a, workspace, s);
+            case VH_F16: return bwd_mfma_launch<__half>(in1, in2, grad_out, grad1, grad2, a, workspace, s);
+            case VH_BF16: return bwd_mfma_launch<__hip_bfloat16>(in1, in2, grad_out, grad1, grad2, a, workspace, s);
+            default: break;
+        }
+    }
     switch (dtype) {
         case VH_F32: return bwd_launch<float>(in1, in2, grad_out, grad1, grad2, a, s);
         case VH_F64: return bwd_launch<double>(in1, in2, grad_out, grad1, grad2, a, s);

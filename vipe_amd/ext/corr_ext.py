@@ -12,8 +12,9 @@ The same checks as the reference raise RuntimeError with its messages
 on same device as input2").  The reference's CPU path has no counterpart
 here: host tensors raise.  Sums accumulate in FP32 (FP64 for float64 inputs),
 where the reference accumulates half inputs in half.  Kernel size 1 with
-stride 1 (the local-attention form) runs on the matrix cores over
-channels-last copies of the inputs (a workspace allocated per call).
+stride 1 (the local-attention form) runs on the matrix cores, forward and
+backward, over channels-last copies of the inputs (a workspace allocated per
+call).
 """
 from __future__ import annotations
 
@@ -105,10 +106,14 @@ def backward(input1, input2, grad_output, kH, kW, patchH, patchW, padH, padW, di
     g2 = torch.empty_like(input2)
     if g1.numel() == 0:
         return [g1, g2]
+    lib = _lib.load()
+    code = _dtype(input1)
+    ints = (N, Cc, H, W, kH, kW, patchH, patchW, padH, padW, dilationH, dilationW, dilation_patchH, dilation_patchW,
+            dH, dW)
+    ws_bytes = lib.vh_spatial_corr_backward_workspace_size(code, *ints)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=input1.device) if ws_bytes else None
     with torch.cuda.device(input1.device):
-        rc = _lib.load().vh_spatial_corr_backward(_dtype(input1), _lib.ptr(input1), _lib.ptr(input2), _lib.ptr(go),
-                                                  N, Cc, H, W, kH, kW, patchH, patchW, padH, padW, dilationH,
-                                                  dilationW, dilation_patchH, dilation_patchW, dH, dW,
-                                                  _lib.ptr(g1), _lib.ptr(g2), _lib.stream_of(input1))
+        rc = lib.vh_spatial_corr_backward(code, _lib.ptr(input1), _lib.ptr(input2), _lib.ptr(go), *ints,
+                                          _lib.ptr(g1), _lib.ptr(g2), _lib.ptr(ws), ws_bytes, _lib.stream_of(input1))
     _lib.check(rc, "corr_ext.backward")
     return [g1, g2]
