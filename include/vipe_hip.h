@@ -275,7 +275,7 @@ int vh_spatial_corr_backward(int dtype, const void* in1, const void* in2, const 
  * arg_out [B,N,K] int64 (min / max only, NULL otherwise): the smallest e whose
  * src equals the result, E if none.  Indices outside [0,N) are skipped.
  * workspace: device memory of vh_scatter_workspace_size(...) bytes; when that
- * size is non-zero (index independent of b and k, K >= 32, N <= 15360,
+ * size is non-zero (index independent of b and k, K >= 32, N <= 12000,
  * E <= 65536) the reduction runs atomic-free over a stable sort of the index
  * and gives the same bits on every run; with no workspace it uses atomics. */
 size_t vh_scatter_workspace_size(int64_t B, int64_t E, int64_t K, int64_t sB, int64_t sE, int64_t sK, int64_t N);

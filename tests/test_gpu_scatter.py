@@ -171,6 +171,26 @@ def test_segmented_path_equals_atomic_path(reduce):
             assert torch.equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("reduce", ["sum", "max"])
+def test_segmented_path_many_sources(reduce):
+    """E = 5000 sources: the placement pass reads the first 4096 indices from
+    its LDS copy and the rest from memory; out-of-range indices are skipped."""
+    from vipe_amd.ext import scatter_ext
+
+    g = torch.Generator().manual_seed(8)
+    src = torch.randn(2, 5000, 32, generator=g)
+    idx = torch.randint(-3, 300, (5000,), generator=g)
+    fn = getattr(scatter_ext, f"scatter_{reduce}")
+    res = fn(src.cuda(), idx.cuda(), 1, None, 300)
+    ref, arg = OSc.scatter(src.numpy(), idx.numpy(), 1, None, 300, reduce)
+    if reduce == "max":
+        assert np.array_equal(res[0].cpu().numpy(), ref.astype(np.float32))
+        assert np.array_equal(res[1].cpu().numpy(), arg)
+    else:
+        mag, _ = OSc.scatter(np.abs(src.numpy()), idx.numpy(), 1, None, 300, "sum")
+        assert np.all(np.abs(res.cpu().numpy() - ref) <= 64 * 2 ** -24 * mag + 1e-30)
+
+
 def test_out_of_range_indices_are_skipped_and_empty_src():
     from vipe_amd.ext import scatter_ext
 

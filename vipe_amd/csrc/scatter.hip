@@ -283,7 +283,8 @@ unsigned grid_for(int64_t n) {
 // order: no atomics, each src element read once, each out element written once,
 // the same result on every run, and 8 / 16-bit floats reduced in FP32 with one
 // final rounding.
-constexpr int kSegMaxN = 15360;   // LDS counters (60 KiB)
+constexpr int kSegMaxN = 12000;   // LDS counters (47 KiB)
+constexpr int kSegIdxCache = 4096;   // indices kept in LDS for the placement pass (16 KiB)
 constexpr int kSegMaxE = 1 << 16;   // the placement pass is one wave
 constexpr int kSegMinK = 32;      // lanes along k share one list
 
@@ -291,33 +292,37 @@ __global__ __launch_bounds__(256) void scatter_segment_sort_kernel(const int64_t
                                                                    int E, int N, int* __restrict__ start,
                                                                    int* __restrict__ order) {
     __shared__ int cnt[kSegMaxN];
-    __shared__ int tsum[256];
+    __shared__ int sidx[kSegIdxCache];
+    __shared__ int tsum[4];
     const int tid = threadIdx.x;
     for (int n = tid; n < N; n += 256) cnt[n] = 0;
     __syncthreads();
     for (int e = tid; e < E; e += 256) {
         const int64_t n = index[(int64_t)e * sE];
-        if (n >= 0 && n < N) atomicAdd(&cnt[n], 1);
+        const bool ok = n >= 0 && n < N;
+        if (ok) atomicAdd(&cnt[n], 1);
+        if (e < kSegIdxCache) sidx[e] = ok ? (int)n : -1;
     }
     __syncthreads();
-    // exclusive scan: each thread a contiguous chunk, then a serial pass over the 256 chunk sums
+    // exclusive scan: each thread sums a contiguous chunk; the 256 chunk sums
+    // are scanned within each wave by shuffles, then across the four waves
     const int per = (N + 255) / 256;
     const int lo = min(N, tid * per), hi = min(N, lo + per);
     int sum = 0;
     for (int n = lo; n < hi; n++) sum += cnt[n];
-    tsum[tid] = sum;
-    __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int i = 0; i < 256; i++) {
-            const int v = tsum[i];
-            tsum[i] = run;
-            run += v;
-        }
-        start[N] = run;
+    const int lane = tid & 63, wv = tid >> 6;
+    int inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += v;
     }
+    if (lane == 63) tsum[wv] = inc;
     __syncthreads();
-    int run = tsum[tid];
+    int wbase = 0;
+    for (int w = 0; w < wv; w++) wbase += tsum[w];
+    if (tid == 255) start[N] = wbase + inc;
+    int run = wbase + inc - sum;
     for (int n = lo; n < hi; n++) {
         const int c = cnt[n];
         start[n] = run;
@@ -325,23 +330,35 @@ __global__ __launch_bounds__(256) void scatter_segment_sort_kernel(const int64_t
         run += c;
     }
     __syncthreads();
-    // stable placement by one wave: a lane's slot is its bucket's cursor plus
-    // the number of lower lanes in the same bucket; the highest such lane
-    // advances the cursor.  One wave's LDS reads and writes complete in order.
+    // stable placement by one wave, 64 positions at a time: a lane's slot is its
+    // bucket's cursor plus the number of lower lanes in the same bucket; the
+    // highest such lane advances the cursor.  One wave's LDS reads and writes
+    // complete in order.
     if (tid < 64) {
         const int lane = tid;
         for (int base = 0; base < E; base += 64) {
             const int e = base + lane;
-            int64_t nn = e < E ? index[(int64_t)e * sE] : -1;
-            const int n = (nn >= 0 && nn < N) ? (int)nn : -1;
+            int n = -1;
+            if (e < E && e < kSegIdxCache) {
+                n = sidx[e];
+            } else if (e < E) {
+                const int64_t nn = index[(int64_t)e * sE];
+                n = (nn >= 0 && nn < N) ? (int)nn : -1;
+            }
+            // one vote per distinct bucket in the chunk: the lanes that share the
+            // first remaining lane's bucket, their ranks by popcount
             int rank = 0;
             bool higher = false;
-            for (int j = 0; j < 64; j++) {
-                const int nj = __shfl(n, j, 64);
-                if (nj == n) {
-                    rank += j < lane;
-                    higher |= j > lane;
+            unsigned long long remaining = __ballot(n >= 0);
+            while (remaining) {
+                const int leader = __ffsll((long long)remaining) - 1;
+                const int nl = __builtin_amdgcn_readlane(n, leader);
+                const unsigned long long same = __ballot(n == nl);
+                if (n == nl) {
+                    rank = __popcll(same & ((1ull << lane) - 1ull));
+                    higher = lane < 63 && (same >> (lane + 1)) != 0ull;
                 }
+                remaining &= ~same;
             }
             if (n >= 0) {
                 const int pos = cnt[n] + rank;

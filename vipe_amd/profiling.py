@@ -8,6 +8,14 @@
     python3 -m vipe_amd.profiling build [--contraction f32|f16|bf16|fp8]
         Two correlation-pyramid builds of the bench workload (for counters).
 
+    python3 -m vipe_amd.profiling spatial_corr [--dtype f32|f16|bf16] [--iters 5]
+        Forward and backward of the spatial correlation sampler on the
+        local-attention shape: [8, 256, 30, 54] inputs, kernel 1, a 15 x 15 patch.
+
+    python3 -m vipe_amd.profiling scatter [--dtype f32|f16|bf16] [--iters 5]
+        scatter_sum and scatter_max of [1, 384, 128 * 48 * 64] sources into 64
+        slots along dim 1 by a 1-D index (the per-frame aggregation shape).
+
     python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
         --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
         Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
@@ -85,6 +93,43 @@ def counter_per_kernel(d: str, counter: str):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
+_DT = {"f32": "float32", "f16": "float16", "bf16": "bfloat16"}
+
+
+def _spatial_corr(dtype: str, iters: int) -> None:
+    import torch
+
+    from .ext import corr_ext
+
+    dt = getattr(torch, _DT[dtype])
+    g = torch.Generator(device="cuda").manual_seed(0)
+    q = torch.randn(8, 256, 30, 54, device="cuda", generator=g).to(dt)
+    k = torch.randn(8, 256, 30, 54, device="cuda", generator=g).to(dt)
+    args = (1, 1, 15, 15, 0, 0, 1, 1, 1, 1, 1, 1)
+    go = torch.randn(8, 15, 15, 30, 54, device="cuda", generator=g).to(dt)
+    for _ in range(iters):
+        corr_ext.forward(q, k, *args)
+        corr_ext.backward(q, k, go, *args)
+    torch.cuda.synchronize()
+    print(f"spatial_corr: {iters} forward + backward, [8, 256, 30, 54] {dtype}, kernel 1, patch 15", flush=True)
+
+
+def _scatter(dtype: str, iters: int) -> None:
+    import torch
+
+    from .ext import scatter_ext
+
+    dt = getattr(torch, _DT[dtype])
+    g = torch.Generator(device="cuda").manual_seed(0)
+    src = torch.randn(1, 384, 128 * 48 * 64, device="cuda", generator=g).to(dt)
+    idx = torch.randint(0, 64, (384,), device="cuda", generator=g)
+    for _ in range(iters):
+        scatter_ext.scatter_sum(src, idx, 1, None, 64)
+        scatter_ext.scatter_max(src, idx, 1, None, 64)
+    torch.cuda.synchronize()
+    print(f"scatter: {iters} x (sum, max), src [1, 384, 393216] {dtype} -> 64 slots", flush=True)
+
+
 def _short(name: str, width: int = 90) -> str:
     return name if len(name) <= width else name[: width - 3] + "..."
 
@@ -147,6 +192,10 @@ def main(argv=None) -> None:
     bl = sub.add_parser("build")
     bl.add_argument("--contraction", choices=["f32", "f16", "bf16", "fp8"], default="f32")
     bl.add_argument("--workload", default="kf64_e384_64x80")
+    for name in ("spatial_corr", "scatter"):
+        w = sub.add_parser(name)
+        w.add_argument("--dtype", choices=list(_DT), default="f32")
+        w.add_argument("--iters", type=int, default=5)
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch")
@@ -161,6 +210,10 @@ def main(argv=None) -> None:
         _copy(a.mib)
     elif a.cmd == "build":
         _build(a.contraction, a.workload)
+    elif a.cmd == "spatial_corr":
+        _spatial_corr(a.dtype, a.iters)
+    elif a.cmd == "scatter":
+        _scatter(a.dtype, a.iters)
     else:
         _summarize(a)
 
