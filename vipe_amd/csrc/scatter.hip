@@ -285,6 +285,7 @@ unsigned grid_for(int64_t n) {
 // final rounding.
 constexpr int kSegMaxN = 12000;   // LDS counters (47 KiB)
 constexpr int kSegIdxCache = 4096;   // indices kept in LDS for the placement pass (16 KiB)
+constexpr int kSegParE = 1024;       // up to this many sources, every thread places (O(E^2 / 256) reads)
 constexpr int kSegMaxE = 1 << 16;   // the placement pass is one wave
 constexpr int kSegMinK = 32;      // lanes along k share one list
 
@@ -330,12 +331,24 @@ __global__ __launch_bounds__(256) void scatter_segment_sort_kernel(const int64_t
         run += c;
     }
     __syncthreads();
+    if (E <= kSegParE) {
+        // all threads: the rank of e in its bucket is the number of earlier
+        // positions with the same index (one broadcast LDS read per position)
+        for (int e = tid; e < E; e += 256) {
+            const int n = sidx[e];
+            if (n < 0) continue;
+            int rank = 0;
+#pragma unroll 8
+            for (int j = 0; j < e; j++) rank += sidx[j] == n;
+            order[cnt[n] + rank] = e;   // cnt holds the bucket's start here
+        }
+        return;
+    }
     // stable placement by one wave, 64 positions at a time: a lane's slot is its
     // bucket's cursor plus the number of lower lanes in the same bucket; the
     // highest such lane advances the cursor.  One wave's LDS reads and writes
     // complete in order.
     if (tid < 64) {
-        const int lane = tid;
         for (int base = 0; base < E; base += 64) {
             const int e = base + lane;
             int n = -1;
@@ -345,20 +358,14 @@ __global__ __launch_bounds__(256) void scatter_segment_sort_kernel(const int64_t
                 const int64_t nn = index[(int64_t)e * sE];
                 n = (nn >= 0 && nn < N) ? (int)nn : -1;
             }
-            // one vote per distinct bucket in the chunk: the lanes that share the
-            // first remaining lane's bucket, their ranks by popcount
             int rank = 0;
             bool higher = false;
-            unsigned long long remaining = __ballot(n >= 0);
-            while (remaining) {
-                const int leader = __ffsll((long long)remaining) - 1;
-                const int nl = __builtin_amdgcn_readlane(n, leader);
-                const unsigned long long same = __ballot(n == nl);
-                if (n == nl) {
-                    rank = __popcll(same & ((1ull << lane) - 1ull));
-                    higher = lane < 63 && (same >> (lane + 1)) != 0ull;
+            for (int j = 0; j < 64; j++) {
+                const int nj = __shfl(n, j, 64);
+                if (nj == n) {
+                    rank += j < lane;
+                    higher |= j > lane;
                 }
-                remaining &= ~same;
             }
             if (n >= 0) {
                 const int pos = cnt[n] + rank;
@@ -369,7 +376,6 @@ __global__ __launch_bounds__(256) void scatter_segment_sort_kernel(const int64_t
     }
 }
 
-// Per-element running state of one output element of the gather.
 template <typename T, int OP> struct SegAcc {
     using Cv = typename Val<T>::C;
     Cv acc;
