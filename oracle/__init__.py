@@ -26,4 +26,6 @@ geom  : slam_ext geometry siblings iproj, projmap, frame_distance,
         depth_filter (csrc/slam_ext/geom_kernels.cu:526-953).
 spatial_corr : corr_ext spatial correlation sampler forward/backward
         (csrc/corr_ext/correlation_cuda_kernel.cu).
+scatter : scatter_ext sum / mul / mean / min / max with arg_out
+        (csrc/scatter_ext/cuda/scatter_cuda.cu, scatter.cpp).
 """
