@@ -190,6 +190,27 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
               float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* workspace,
               void* stream);
 
+/* Robust kernels of the generic solver (vipe/slam/ba/kernel.py:21-162,
+ * build_robust_kernel :142-162), applied inside the fused BA: each flow
+ * residual component's weight is multiplied by kernel(r) at the
+ * linearisation point (terms.py:73-75).  The reference's fused BA rejects
+ * robust kernels (buffer.py:540-545); this entry point has no counterpart
+ * there and is what lets the fused path serve those configurations.
+ * threshold: Huber k, Tukey c, GNC-TLS c_bar; gnc_mu / gnc_mu_max: the GNC
+ * continuation parameter of this call (mu >= mu_max: the hard TLS indicator). */
+#define VH_ROBUST_NONE 0
+#define VH_ROBUST_HUBER 1
+#define VH_ROBUST_TUKEY 2
+#define VH_ROBUST_GNC_TLS 3
+int vh_ba_run_robust(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
+                     const float* targets, const float* weights, const float* eta, const float* depth_active,
+                     int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
+                     float intrinsics_scale, int flags, int robust_type, float threshold, float gnc_mu,
+                     float gnc_mu_max, float* dx, float* dz, float* energy, void* workspace, void* stream);
+/* RobustKernel.apply (kernel.py:41-131): w[i] = kernel(x[i]) for n FP32 values. */
+int vh_robust_weights(int robust_type, float threshold, float gnc_mu, float gnc_mu_max, const float* x, float* w,
+                      int64_t n, void* stream);
+
 /* Implicit adjoint of one Gauss-Newton iteration of vh_ba_run (no counterpart
  * in the reference, whose BA layer has no backward).  Call it after a
  * vh_ba_run with iterations = 1 on the same plan and workspace, with nothing

@@ -35,6 +35,8 @@ from dataclasses import dataclass
 
 import torch
 
+from . import robust as robust_mod
+
 from . import lie
 
 F64 = torch.float64
@@ -105,8 +107,11 @@ class EdgeTerms:
     z: torch.Tensor
 
 
-def edge_terms(poses, disps, intrinsics, targets, weights, ii, jj, intrinsics_scale=1.0) -> EdgeTerms:
-    """Per-pixel residuals and Jacobians of projective_transform_kernel (:298-435)."""
+def edge_terms(poses, disps, intrinsics, targets, weights, ii, jj, intrinsics_scale=1.0, robust=None) -> EdgeTerms:
+    """Per-pixel residuals and Jacobians of projective_transform_kernel (:298-435).
+    ``robust`` = (kind, threshold, mu, mu_max) multiplies each residual
+    component's weight by the robust kernel's weight of that residual
+    (vipe/slam/ba/terms.py:73-75; weights in oracle/robust.py)."""
     E = ii.shape[0]
     N, H, W = disps.shape
     HW = H * W
@@ -133,6 +138,10 @@ def edge_terms(poses, disps, intrinsics, targets, weights, ii, jj, intrinsics_sc
     proj_v = fy * dinv * y + cy
     ru = tgt[:, 0] - proj_u
     rv = tgt[:, 1] - proj_v
+    if robust is not None and robust[0] != robust_mod.NONE:
+        kind, c, mu, mu_max = robust
+        wu = wu * robust_mod.weight(ru, kind, c, mu, mu_max)
+        wv = wv * robust_mod.weight(rv, kind, c, mu, mu_max)
     o = torch.zeros_like(z)
     Ju = fx * torch.stack([h * dinv, o, -x * h * d2, -x * y * d2, 1 + x * x * d2, -y * dinv], -1)
     Jv = fy * torch.stack([o, h * dinv, -y * h * d2, -1 - y * y * d2, x * y * d2, x * dinv], -1)
@@ -154,7 +163,7 @@ def _sym(E):
 def ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active,
                 t0, t1, iterations, lm, ep, motion_only, alpha, optimize_intrinsics,
                 intrinsics_lm, intrinsics_ep, intrinsics_scale, compute_energy=False,
-                skip_quirk=True, solver="cholesky"):
+                skip_quirk=True, solver="cholesky", robust=None):
     """Restatement of ba_cuda_impl (geom_kernels.cu:1626-1820).
 
     Mutates ``poses``, ``disps`` and ``intrinsics`` in place like the reference
@@ -177,7 +186,7 @@ def ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii,
     dim = 6 * P + (1 if optimize_intrinsics else 0)
 
     for itr in range(iterations):
-        T = edge_terms(poses, disps, intrinsics, targets, weights, ii, jj, intrinsics_scale)
+        T = edge_terms(poses, disps, intrinsics, targets, weights, ii, jj, intrinsics_scale, robust)
         act_d = dact[ii][:, None]
         wdu, wdv = T.wu * act_d, T.wv * act_d
         Cii = wdu * T.Jzu ** 2 + wdv * T.Jzv ** 2
@@ -342,7 +351,7 @@ def ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, t0, 
 # ----------------------------------------------------------------------------
 
 def dense_gn_step(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active,
-                  t0, t1, lm, ep, alpha, return_system=False):
+                  t0, t1, lm, ep, alpha, return_system=False, robust=None):
     """One GN step over [poses in window (6P), all depth pixels of kx (K*HW)] with the explicit
     Jacobian J of every residual row, the same damping of the pose block and the same depth
     prior / eta terms, solved with torch.linalg.solve.  Returns (dx [P,6], dz [K,HW]).
@@ -357,7 +366,7 @@ def dense_gn_step(poses, disps, intrinsics, disps_sens, targets, weights, eta, i
     bk = bookkeeping(ii, jj, t0, t1)
     P, kx = bk.P, bk.kx
     K = kx.shape[0]
-    T = edge_terms(poses, disps, intrinsics, targets, weights, ii, jj)
+    T = edge_terms(poses, disps, intrinsics, targets, weights, ii, jj, robust=robust)
     At = T.Adj.transpose(1, 2)
     Jiu = -(At[:, None] @ T.Ju[..., None])[..., 0]
     Jiv = -(At[:, None] @ T.Jv[..., None])[..., 0]

@@ -142,3 +142,33 @@ def test_fixture_matches_oracle(tracks, limited, intr, motion):
                                    atol=2e-4, rtol=2e-4)
     if motion:
         assert torch.equal(buf.disps, d0)
+
+
+def test_robust_kernels_run_on_the_fused_path_when_enabled(monkeypatch):
+    """ba.fused_robust: Huber / Tukey run n_iters iterations with their weights;
+    GNC-TLS runs gnc_n_mu_steps x gnc_gn_iters_per_mu over the mu schedule
+    (buffer.py:686-695).  The fused calls are recorded, not run."""
+    from oracle import robust as R
+
+    calls = []
+
+    def fake(*a, robust=None, **kw):
+        calls.append((a[8], robust))
+        return torch.zeros(0)
+
+    monkeypatch.setattr(graph_ba, "fused_ba", fake)
+    buf = make_buffer("cpu", robust="tukey")
+    buf.ba_config.update(fused_robust=True, robust_kernel_threshold=2.5)
+    inp = make_inputs(buf)
+    _call(buf, inp, n_iters=3)
+    assert calls == [(3, (2, 2.5, 1.0, 1.0))]
+    calls.clear()
+    buf.ba_config.update(robust_kernel="gnc_tls", gnc_mu_init=1.0, gnc_mu_step=2.0, gnc_mu_max=5.0,
+                         gnc_n_mu_steps=4, gnc_gn_iters_per_mu=2)
+    _call(buf, inp, n_iters=9)
+    assert [c[0] for c in calls] == [2, 2, 2, 2]
+    assert [c[1][2] for c in calls] == R.gnc_schedule(1.0, 2.0, 5.0, 4)
+    assert all(c[1][:2] == (3, 2.5) and c[1][3] == 5.0 for c in calls)
+    with pytest.raises(ValueError, match="Unknown robust_kernel"):
+        buf.ba_config.update(robust_kernel="cauchy")
+        _call(buf, inp)

@@ -73,6 +73,9 @@ _SIGS = {
     "vh_ba_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, f32, i32, vp, vp, vp, vp, vp, vp]),
     "vh_ba_run": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, vp, vp, vp, vp,
                         vp]),
+    "vh_ba_run_robust": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, i32, f32,
+                               f32, f32, vp, vp, vp, vp, vp]),
+    "vh_robust_weights": (i32, [i32, f32, f32, f32, vp, vp, i64, vp]),
     # geometry siblings
     "vh_geom_iproj": (i32, [vp, vp, vp, i32, i32, i32, vp, vp]),
     "vh_geom_projmap": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp]),

@@ -101,6 +101,40 @@ struct Work {
     int* flag;         // [4]
 };
 
+// Robust kernels of the generic solver (vipe/slam/ba/kernel.py:21-162) as an
+// iteratively reweighted factor on each flow-residual component's weight
+// (terms.py:73-75: w <- w * kernel(r)), evaluated at the linearisation point:
+//   1 Huber   w = 1 if |r| <= c else c / |r|
+//   2 Tukey   w = (max(1 - (r/c)^2, 0))^2
+//   3 GNC-TLS at mu, c2 = c^2: 1 if r^2 < mu/(mu+1) c2, 0 if r^2 > (mu+1)/mu c2,
+//             else sqrt(c2 mu (mu+1) / r^2) - mu (clamped at 0); mu >= mu_max
+//             gives the indicator r^2 < c2.
+struct RobustK {
+    int type;
+    float c, mu, mu_max;
+};
+
+__host__ __device__ inline float robust_weight(float r, const RobustK& k) {
+    if (k.type == 1) {
+        const float ar = fabsf(r);
+        return ar <= k.c ? 1.f : k.c / fmaxf(ar, 1e-8f);
+    }
+    if (k.type == 2) {
+        const float u = (r / k.c) * (r / k.c);
+        const float v = fmaxf(1.f - u, 0.f);
+        return v * v;
+    }
+    if (k.type == 3) {
+        const float r2 = r * r, c2 = k.c * k.c;
+        if (k.mu >= k.mu_max) return r2 < c2 ? 1.f : 0.f;
+        const float lo = (k.mu / (k.mu + 1.f)) * c2, hi = ((k.mu + 1.f) / k.mu) * c2;
+        if (r2 < lo) return 1.f;
+        if (r2 > hi) return 0.f;
+        return fmaxf(sqrtf(c2 * k.mu * (k.mu + 1.f) / fmaxf(r2, 1e-12f)) - k.mu, 0.f);
+    }
+    return 1.f;
+}
+
 struct RunArgs {
     float* poses;
     float* disps;
@@ -116,6 +150,7 @@ struct RunArgs {
     float* energy;
     int itr;
     int compute_energy;
+    RobustK rk;   // robust kernel on the flow residuals (type 0: plain least squares)
 };
 
 __host__ __device__ inline int pk(int kind, int idx) { return (kind << 27) | idx; }
@@ -346,6 +381,10 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
             if (valid) {
                 PixTerms T;
                 pixel_terms<INTR, true>(Rm, t, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale, T);
+                if (a.rk.type) {
+                    T.wu *= robust_weight(T.ru, a.rk);
+                    T.wv *= robust_weight(T.rv, a.rk);
+                }
                 const float wdu = T.wu * act, wdv = T.wv * act;
                 const float pu = stereo ? 0.f : T.wu;
                 const float pv = stereo ? 0.f : T.wv;
@@ -1344,8 +1383,16 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
         const int e = p.fr_edges[e0 + s];
         const int64_t eb = (int64_t)e * 2 * HW + pix;
         PixTerms T;
-        pixel_terms<INTR, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
-                                 0.f, 0.f, a.iscale, T);
+        if (a.rk.type) {
+            // the frame kernel's robust weights: residuals at the same linearisation point
+            pixel_terms<INTR, true>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb],
+                                    a.weights[eb + HW], a.targets[eb], a.targets[eb + HW], a.iscale, T);
+            T.wu *= robust_weight(T.ru, a.rk);
+            T.wv *= robust_weight(T.rv, a.rk);
+        } else {
+            pixel_terms<INTR, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb],
+                                     a.weights[eb + HW], 0.f, 0.f, a.iscale, T);
+        }
         const float wdu = T.wu * act, wdv = T.wv * act;
         const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
         float s6 = 0.f;
@@ -1836,6 +1883,12 @@ size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 }  // namespace
 
+__global__ __launch_bounds__(256) void robust_weight_kernel(RobustK k, const float* __restrict__ x,
+                                                            float* __restrict__ w, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = robust_weight(x[i], k);
+}
+
 extern "C" {
 
 vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W, int t0,
@@ -2099,11 +2152,11 @@ size_t vh_ba_workspace_size(const vh_ba_plan* plan) { return plan ? plan->ws_byt
 
 int vh_ba_plan_pixel_tile(const vh_ba_plan* plan) { return plan ? plan->d.TP : -1; }
 
-int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
-              const float* targets, const float* weights, const float* eta, const float* depth_active,
-              int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
-              float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* workspace,
-              void* stream) {
+static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics,
+                       const float* disps_sens, const float* targets, const float* weights, const float* eta,
+                       const float* depth_active, int iterations, float lm, float ep, float alpha,
+                       float intrinsics_lm, float intrinsics_ep, float intrinsics_scale, int flags, const RobustK& rk,
+                       float* dx, float* dz, float* energy, void* workspace, void* stream) {
     if (!plan) return fail(VH_ERR_ARG, "ba_run: null plan");
     if (iterations < 0) return fail(VH_ERR_ARG, "ba_run: negative iterations");
     if (iterations == 0) return VH_OK;
@@ -2135,7 +2188,8 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
     w.Li = (double*)(ws + plan->o_Li);
     w.y = (double*)(ws + plan->o_y);
     w.flag = (int*)(ws + plan->o_flag);
-    RunArgs a;
+    RunArgs a{};
+    a.rk = rk;
     a.poses = poses; a.disps = disps; a.intrinsics = intrinsics; a.disps_sens = disps_sens;
     a.targets = targets; a.weights = weights; a.eta = eta; a.depth_active = depth_active;
     a.lm = lm; a.ep = ep; a.alpha = alpha; a.ilm = intrinsics_lm; a.iep = intrinsics_ep;
@@ -2188,6 +2242,47 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
                            intr ? 1 : 0, intrinsics_scale);
         VH_LAUNCH_CHECK("ba_retract_kernel");
     }
+    return VH_OK;
+}
+
+int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
+              const float* targets, const float* weights, const float* eta, const float* depth_active,
+              int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
+              float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* workspace,
+              void* stream) {
+    return ba_run_impl(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta, depth_active, iterations,
+                       lm, ep, alpha, intrinsics_lm, intrinsics_ep, intrinsics_scale, flags, RobustK{0, 0.f, 0.f, 0.f},
+                       dx, dz, energy, workspace, stream);
+}
+
+int vh_ba_run_robust(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
+                     const float* targets, const float* weights, const float* eta, const float* depth_active,
+                     int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
+                     float intrinsics_scale, int flags, int robust_type, float threshold, float gnc_mu,
+                     float gnc_mu_max, float* dx, float* dz, float* energy, void* workspace, void* stream) {
+    if (robust_type < VH_ROBUST_NONE || robust_type > VH_ROBUST_GNC_TLS)
+        return fail(VH_ERR_ARG, "ba_run_robust: unknown robust kernel %d", robust_type);
+    if (robust_type != VH_ROBUST_NONE && !(threshold > 0.f))
+        return fail(VH_ERR_ARG, "ba_run_robust: threshold must be positive");
+    if (robust_type == VH_ROBUST_GNC_TLS && !(gnc_mu > 0.f))
+        return fail(VH_ERR_ARG, "ba_run_robust: GNC mu must be positive");
+    return ba_run_impl(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta, depth_active, iterations,
+                       lm, ep, alpha, intrinsics_lm, intrinsics_ep, intrinsics_scale, flags,
+                       RobustK{robust_type, threshold, gnc_mu, gnc_mu_max}, dx, dz, energy, workspace, stream);
+}
+
+int vh_robust_weights(int robust_type, float threshold, float gnc_mu, float gnc_mu_max, const float* x, float* w,
+                      int64_t n, void* stream) {
+    if (robust_type < VH_ROBUST_NONE || robust_type > VH_ROBUST_GNC_TLS)
+        return fail(VH_ERR_ARG, "robust_weights: unknown robust kernel %d", robust_type);
+    if (n < 0) return fail(VH_ERR_ARG, "robust_weights: negative size");
+    if (n == 0) return VH_OK;
+    if (!x || !w) return fail(VH_ERR_ARG, "robust_weights: null pointer");
+    const RobustK k{robust_type, threshold, gnc_mu, gnc_mu_max};
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(robust_weight_kernel, dim3((unsigned)std::min<int64_t>(blocks, 65536)), dim3(256), 0,
+                       (hipStream_t)stream, k, x, w, n);
+    VH_LAUNCH_CHECK("robust_weight_kernel");
     return VH_OK;
 }
 

@@ -114,9 +114,23 @@ def ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii,
     return [dx, dz, energy]
 
 
+def ba_extended_robust(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active, t0, t1,
+                       iterations, lm, ep, motion_only, alpha, optimize_intrinsics, intrinsics_lm, intrinsics_ep,
+                       intrinsics_scale, compute_energy=False, robust=None):
+    """ba_extended with a robust kernel on the flow residuals: ``robust`` is
+    (type, threshold, mu, mu_max) as given by ``RobustKernel.fused_args()``
+    (vipe_amd.ba_kernel), or None for plain least squares.  This package's
+    extension: the reference's fused BA rejects robust kernels."""
+    dx, dz, energy, _, _ = _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj,
+                                        depth_active, t0, t1, iterations, lm, ep, motion_only, alpha,
+                                        optimize_intrinsics, intrinsics_lm, intrinsics_ep, intrinsics_scale,
+                                        compute_energy, robust)
+    return [dx, dz, energy]
+
+
 def _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, depth_active, t0, t1,
                  iterations, lm, ep, motion_only, alpha, optimize_intrinsics, intrinsics_lm, intrinsics_ep,
-                 intrinsics_scale, compute_energy=False):
+                 intrinsics_scale, compute_energy=False, robust=None):
     """ba_extended that also returns the plan and the workspace of the run."""
     _check(targets=targets, weights=weights, poses=poses, disps=disps, intrinsics=intrinsics,
            disps_sens=disps_sens, ii=ii, jj=jj, depth_active=depth_active)
@@ -146,13 +160,24 @@ def _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii
     ws = torch.empty((max(plan.ws_bytes, 1),), dtype=torch.uint8, device=dev)
     flags = (1 if motion_only else 0) | (2 if optimize_intrinsics else 0) | (4 if compute_energy else 0)
     lib = _lib.load()
+    rk = (0, 0.0, 0.0, 0.0) if robust is None else tuple(robust)
     with torch.cuda.device(dev):
-        rc = lib.vh_ba_run(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(disps_sens),
-                           _lib.ptr(targets), _lib.ptr(weights), None if motion_only else _lib.ptr(eta_c),
-                           _lib.ptr(depth_active), iterations, float(lm), float(ep), float(alpha),
-                           float(intrinsics_lm), float(intrinsics_ep), float(intrinsics_scale), flags, _lib.ptr(dx),
-                           _lib.ptr(dz), _lib.ptr(energy) if compute_energy else None, _lib.ptr(ws),
-                           _lib.stream_of(poses))
+        if rk[0] == 0:
+            rc = lib.vh_ba_run(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(disps_sens),
+                               _lib.ptr(targets), _lib.ptr(weights), None if motion_only else _lib.ptr(eta_c),
+                               _lib.ptr(depth_active), iterations, float(lm), float(ep), float(alpha),
+                               float(intrinsics_lm), float(intrinsics_ep), float(intrinsics_scale), flags,
+                               _lib.ptr(dx), _lib.ptr(dz), _lib.ptr(energy) if compute_energy else None,
+                               _lib.ptr(ws), _lib.stream_of(poses))
+        else:
+            rc = lib.vh_ba_run_robust(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics),
+                                      _lib.ptr(disps_sens), _lib.ptr(targets), _lib.ptr(weights),
+                                      None if motion_only else _lib.ptr(eta_c), _lib.ptr(depth_active), iterations,
+                                      float(lm), float(ep), float(alpha), float(intrinsics_lm), float(intrinsics_ep),
+                                      float(intrinsics_scale), flags, int(rk[0]), float(rk[1]), float(rk[2]),
+                                      float(rk[3]), _lib.ptr(dx), _lib.ptr(dz),
+                                      _lib.ptr(energy) if compute_energy else None, _lib.ptr(ws),
+                                      _lib.stream_of(poses))
     _lib.check(rc, "slam_ext.ba")
     return dx, dz, energy, plan, ws
 

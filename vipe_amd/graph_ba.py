@@ -6,6 +6,11 @@ Restates the argument preparation of vipe/slam/components/buffer.py:364-489
 * rejects layouts the fused kernels do not cover with the reference's
   messages (multi-view or non-pinhole buffers, rig rotation, empty window,
   mismatched edge counts, non-default flow weights, robust kernels);
+* with ``ba_config.fused_robust`` set, runs the configured robust kernel
+  (``robust_kernel``, ``robust_kernel_threshold``, ``gnc_*``) on the fused
+  path instead of rejecting it: Huber / Tukey weights inside the frame kernel
+  for ``n_iters`` iterations, GNC-TLS as ``gnc_n_mu_steps`` x
+  ``gnc_gn_iters_per_mu`` iterations over the mu schedule of buffer.py:686-695;
 * drops self edges ii == jj; merges sparse-track targets and weights into the
   dense ones (weighted mean where the combined weight is positive);
 * ``[E, H*W, 2] -> [E, 2, H, W]``; eta = 0.2 * damping[kx] + 1e-7 with
@@ -31,6 +36,7 @@ import logging
 import torch
 from einops import rearrange
 
+from .ba_kernel import build_robust_kernel
 from .ext import slam_ext
 
 logger = logging.getLogger(__name__)
@@ -53,9 +59,10 @@ def _fail(reason: str):
 
 def fused_ba(buffer, target, weight, disp_damping, ii, jj, t0, t1, n_iters, pose_damping, pose_ep, motion_only,
              limited_disp, optimize_intrinsics, optimize_rig_rotation, weight_dense_disp=0.001,
-             weight_tracks=0.001, verbose=False):
+             weight_tracks=0.001, verbose=False, robust=None):
     """GraphBuffer._fused_ba (buffer.py:364-489).  Returns the per-iteration energy
-    (empty unless ``verbose``)."""
+    (empty unless ``verbose``).  ``robust``: (type, threshold, mu, mu_max) of a
+    robust kernel on the flow residuals, or None."""
     camera = str(getattr(buffer.camera_type, "name", buffer.camera_type)).lower()
     if buffer.n_views != 1 or camera != "pinhole":
         _fail("it only supports single-view pinhole-camera graph buffers")
@@ -107,11 +114,11 @@ def fused_ba(buffer, target, weight, disp_damping, ii, jj, t0, t1, n_iters, pose
     disps = buffer.disps[:, 0]
     if not disps.is_contiguous():
         raise RuntimeError("disps must be contiguous")
-    _, _, energy = slam_ext.ba_extended(
+    _, _, energy = slam_ext.ba_extended_robust(
         buffer.poses, disps, intrinsics, buffer.disps_sens[:, 0].contiguous(), target, weight, eta,
         ii.contiguous(), jj.contiguous(), depth_active.contiguous(), t0, t1, n_iters, pose_damping, pose_ep,
         motion_only, float(_cfg(buffer.ba_config, "dense_disp_alpha")), bool(optimize_intrinsics),
-        1e-6 * damp_scale, 1e-6 * damp_scale, intrinsics_scale, verbose)
+        1e-6 * damp_scale, 1e-6 * damp_scale, intrinsics_scale, verbose, robust)
     if verbose:
         logger.info(f"BA iters = {n_iters}, energy: {energy[0].item()} -> {energy[-1].item()}")
     if optimize_intrinsics:
@@ -124,10 +131,31 @@ def bundle_adjustment(buffer, target, weight, disp_damping, ii, jj, t0, t1, n_it
     """Fused branch of GraphBuffer.bundle_adjustment (buffer.py:501-566)."""
     if t0 > t1:
         raise ValueError("t0 must not exceed t1")
-    if _cfg(buffer.ba_config, "robust_kernel", None) is not None:
+    cfg = buffer.ba_config
+    kernel = build_robust_kernel(
+        _cfg(cfg, "robust_kernel", None), float(_cfg(cfg, "robust_kernel_threshold", 1.0)),
+        gnc_mu_init=float(_cfg(cfg, "gnc_mu_init", 1.0)), gnc_mu_step=float(_cfg(cfg, "gnc_mu_step", 1.4)),
+        gnc_mu_max=float(_cfg(cfg, "gnc_mu_max", 1.0e6)))
+    args = (buffer, target, weight, disp_damping, ii, jj, t0, t1)
+    rest = (pose_damping, pose_ep, motion_only, limited_disp, optimize_intrinsics, optimize_rig_rotation, 0.001,
+            0.001, verbose)
+    if kernel is None:
+        energy = fused_ba(*args, n_iters, *rest)
+    elif not bool(_cfg(cfg, "fused_robust", False)):
         raise RuntimeError("Fused BA is enabled, but robust kernels are not supported. "
                            "Set ba.fused=false to use the generic BA solver.")
-    energy = fused_ba(buffer, target, weight, disp_damping, ii, jj, t0, t1, n_iters, pose_damping, pose_ep,
-                      motion_only, limited_disp, optimize_intrinsics, optimize_rig_rotation, 0.001, 0.001, verbose)
+    elif kernel.is_gnc():
+        # outer mu schedule, inner GN iterations at a frozen mu (buffer.py:686-695)
+        n_mu = max(1, int(_cfg(cfg, "gnc_n_mu_steps", 4)))
+        per_mu = max(1, int(_cfg(cfg, "gnc_gn_iters_per_mu", 6)))
+        mu = kernel.mu_init
+        parts = []
+        for _ in range(n_mu):
+            kernel.set_mu(mu)
+            parts.append(fused_ba(*args, per_mu, *rest, robust=kernel.fused_args()))
+            mu = kernel.update_mu(mu)
+        energy = torch.cat(parts)
+    else:
+        energy = fused_ba(*args, n_iters, *rest, robust=kernel.fused_args())
     buffer.disps.clamp_(min=0.001)
     return energy
