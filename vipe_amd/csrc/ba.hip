@@ -294,7 +294,7 @@ __device__ __forceinline__ void gram_rows(int b, int Rext, int& ra, int& rb) {
 // ---------------------------------------------------------------------------
 // 1. frame kernel
 // ---------------------------------------------------------------------------
-template <bool INTR, bool MOTION>
+template <bool INTR, bool MOTION, bool ROBUST = false>
 __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a, Work w) {
     extern __shared__ float dyn[];  // rows [Rext_max*6][TP+1]; reused for the Gram reduction
     __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
             if (valid) {
                 PixTerms T;
                 pixel_terms<INTR, true>(Rm, t, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale, T);
-                if (a.rk.type) {
+                if (ROBUST) {
                     T.wu *= robust_weight(T.ru, a.rk);
                     T.wv *= robust_weight(T.rv, a.rk);
                 }
@@ -2200,7 +2200,16 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
     for (int itr = 0; itr < iterations; itr++) {
         a.itr = itr;
         if (d.E > 0 && d.K > 0) {
-            if (motion) {
+            if (rk.type) {
+                // robust weights: a separate instantiation, so plain least squares pays nothing
+                if (motion) {
+                    if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, true, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                    else hipLaunchKernelGGL((ba_frame_kernel<false, true, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                } else {
+                    if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, false, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                    else hipLaunchKernelGGL((ba_frame_kernel<false, false, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                }
+            } else if (motion) {
                 if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
                 else hipLaunchKernelGGL((ba_frame_kernel<false, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
             } else {
