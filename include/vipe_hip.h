@@ -304,6 +304,27 @@ int vh_scatter(int reduce, int dtype, const void* src, const int64_t* index, int
                int64_t sB, int64_t sE, int64_t sK, int64_t N, void* out, int64_t* arg_out, int out_given,
                void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- factor-graph update glue (vipe/slam/components/factor_graph.py) -----
+ * Motion features of FactorGraph.update / update_batch (:247-260, :339-345),
+ * with the reprojection of GraphBuffer.reproject_dense_disp (buffer.py:705-726)
+ * for a single-view pinhole graph: poses [N,7], disps [N,H,W], intrinsics [4]
+ * at the feature resolution, ii / jj [E] int64, target [E,H,W,2] (the graph's
+ * current targets).  Writes coords1 [E,H,W,2] and motn [E,4,H,W] =
+ * clamp([coords1 - grid, target - coords1], -64, 64).  Points with z < 0.1
+ * project with z = 1 (cameras.py:175-176).  An out-of-range frame index gives
+ * NaN coordinates for that edge and reads nothing. */
+int vh_graph_motion(const float* poses, const float* disps, const float* intrinsics, const int64_t* ii,
+                    const int64_t* jj, const float* target, int E, int N, int H, int W, float* coords1, float* motn,
+                    void* stream);
+/* The update's epilogue (:268-277, :359-362): target = coords1 + delta,
+ * weight = weight_net with the pixels of mask[frame[e]] (uint8 [NM,H,W], may be
+ * NULL) zeroed; delta / weight_net [E,H,W,2] in dtype VH_F32 or VH_F16 (the
+ * network's autocast output).  target / weight [E,H,W,2] f32 (graph state);
+ * target_ba / weight_ba [E,2,H,W] f32 (the layout vh_ba_run reads) when not NULL. */
+int vh_graph_targets(int dtype, const float* coords1, const void* delta, const void* weight_net, const uint8_t* mask,
+                     const int64_t* frame, int NM, int E, int H, int W, float* target, float* weight,
+                     float* target_ba, float* weight_ba, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,4 +28,7 @@ spatial_corr : corr_ext spatial correlation sampler forward/backward
         (csrc/corr_ext/correlation_cuda_kernel.cu).
 scatter : scatter_ext sum / mul / mean / min / max with arg_out
         (csrc/scatter_ext/cuda/scatter_cuda.cu, scatter.cpp).
+robust : robust kernels of the BA solver (vipe/slam/ba/kernel.py).
+graph_step : factor-graph update glue: reprojection, motion features,
+        target / weight epilogue (vipe/slam/components/factor_graph.py).
 """

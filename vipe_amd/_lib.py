@@ -87,6 +87,9 @@ _SIGS = {
     "vh_spatial_corr_forward": (i32, [i32, vp, vp] + [i32] * 16 + [vp, vp, sz, vp]),
     "vh_spatial_corr_backward_workspace_size": (sz, [i32] * 17),
     "vh_spatial_corr_backward": (i32, [i32, vp, vp, vp] + [i32] * 16 + [vp, vp, vp, sz, vp]),
+    # factor-graph update glue
+    "vh_graph_motion": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp]),
+    "vh_graph_targets": (i32, [i32, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, vp]),
     # scatter_ext
     "vh_scatter_workspace_size": (sz, [i64] * 7),
     "vh_scatter": (i32, [i32, i32, vp, vp] + [i64] * 7 + [vp, vp, i32, vp, sz, vp]),

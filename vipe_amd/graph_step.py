@@ -1,0 +1,106 @@
+"""The update step of the factor graph on fused kernels
+(vipe/slam/components/factor_graph.py:229-392), single-view pinhole graphs.
+
+``motion_features`` replaces the reprojection + concatenation + clamp that
+precede the update operator (one pass: ``vh_graph_motion``), and
+``update_targets`` the epilogue after it (``vh_graph_targets``: target =
+coords1 + delta, masked weights, written both as graph state [E,H,W,2] and in
+the [E,2,H,W] layout the fused BA reads).  ``update`` strings them together
+with the caller's correlation lookup and update operator and the fused BA, as
+``FactorGraph.update`` does.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from . import graph_ba
+
+
+def _dev(name, t):
+    if t.device.type != "cuda":
+        raise RuntimeError(f"graph_step runs on the GPU; {name} is on {t.device}")
+    if not t.is_contiguous():
+        raise RuntimeError(f"{name} must be contiguous")
+
+
+def motion_features(poses, disps, intrinsics, ii, jj, target):
+    """(coords1 [E,H,W,2], motn [E,4,H,W]) for edges ii -> jj; ``intrinsics``
+    [4] at the resolution of ``disps`` [N,H,W]; ``target`` [E,H,W,2]."""
+    for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics), ("ii", ii), ("jj", jj),
+                    ("target", target)):
+        _dev(name, t)
+    N, H, W = disps.shape
+    E = ii.shape[0]
+    if target.shape != (E, H, W, 2):
+        raise RuntimeError(f"target must be [E,H,W,2] = {(E, H, W, 2)} (got {tuple(target.shape)})")
+    coords1 = torch.empty((E, H, W, 2), dtype=torch.float32, device=disps.device)
+    motn = torch.empty((E, 4, H, W), dtype=torch.float32, device=disps.device)
+    with torch.cuda.device(disps.device):
+        rc = _lib.load().vh_graph_motion(_lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(ii),
+                                         _lib.ptr(jj), _lib.ptr(target), E, N, H, W, _lib.ptr(coords1),
+                                         _lib.ptr(motn), _lib.stream_of(disps))
+    _lib.check(rc, "graph_step.motion_features")
+    return coords1, motn
+
+
+def update_targets(coords1, delta, weight_net, mask=None, frame=None, ba_layout=True):
+    """(target, weight [E,H,W,2], target_ba, weight_ba [E,2,H,W] or None)."""
+    E, H, W, _ = coords1.shape
+    for name, t in (("coords1", coords1), ("delta", delta), ("weight_net", weight_net)):
+        _dev(name, t)
+    if delta.dtype != weight_net.dtype or delta.dtype not in (torch.float32, torch.float16):
+        raise RuntimeError("delta and weight_net must both be float32 or float16")
+    if mask is not None:
+        _dev("mask", mask)
+        _dev("frame", frame)
+        mask = mask.to(torch.uint8)
+    dev = coords1.device
+    target = torch.empty((E, H, W, 2), dtype=torch.float32, device=dev)
+    weight = torch.empty((E, H, W, 2), dtype=torch.float32, device=dev)
+    tba = torch.empty((E, 2, H, W), dtype=torch.float32, device=dev) if ba_layout else None
+    wba = torch.empty((E, 2, H, W), dtype=torch.float32, device=dev) if ba_layout else None
+    with torch.cuda.device(dev):
+        rc = _lib.load().vh_graph_targets(_lib.dtype_code(delta), _lib.ptr(coords1), _lib.ptr(delta),
+                                          _lib.ptr(weight_net), _lib.ptr(mask), _lib.ptr(frame),
+                                          0 if mask is None else mask.shape[0], E, H, W, _lib.ptr(target),
+                                          _lib.ptr(weight), _lib.ptr(tba), _lib.ptr(wba), _lib.stream_of(coords1))
+    _lib.check(rc, "graph_step.update_targets")
+    return target, weight, tba, wba
+
+
+def update(graph, update_op, corr_fn, t0=None, t1=None, itrs=3, motion_only=False, fixed_motion=False,
+           limited_disp=False):
+    """FactorGraph.update (factor_graph.py:229-313) without inactive edges.
+
+    ``graph`` carries ``buffer`` (a GraphBuffer-like object as in
+    ``vipe_amd.graph_ba``, with ``masks [N,1,H,W]`` optional), ``ii``, ``jj``,
+    ``target`` [1,E,H,W,2], ``weight`` [1,E,H,W,2], ``damping`` [N,H,W],
+    ``f_net``, ``inp`` and ``age``.  ``corr_fn(coords1 [1,E,H,W,2])`` returns the
+    correlation features; ``update_op(f_net, inp, corr, motn [1,E,4,H,W], ix=dix)``
+    returns (f_net, delta, weight, damping, upmask) like DroidNet's update module.
+    """
+    buf = graph.buffer
+    ii, jj = graph.ii, graph.jj
+    if t0 is None:
+        t0 = int(max(1, ii.min().item() + 1))
+    if t1 is None:
+        t1 = int(max(ii.max().item(), jj.max().item()) + 1)
+    ht, wd = buf.height // 8, buf.width // 8
+    intr = (buf.intrinsics[0, :4] / 8.0).contiguous()
+    disps = buf.disps[:, 0]
+    coords1, motn = motion_features(buf.poses, disps, intr, ii.contiguous(), jj.contiguous(),
+                                    graph.target[0].contiguous())
+    corr = corr_fn(coords1[None])
+    di, dix = torch.unique(ii, return_inverse=True)
+    graph.f_net, delta, weight, damping, _ = update_op(graph.f_net, graph.inp, corr, motn[None], ix=dix)
+    masks = getattr(buf, "masks", None)
+    mask = None if masks is None else masks[:, 0].contiguous()
+    target, wgt, _, _ = update_targets(coords1, delta[0].contiguous(), weight[0].contiguous(), mask,
+                                       ii.contiguous() if mask is not None else None, ba_layout=False)
+    graph.target, graph.weight = target[None], wgt[None]
+    graph.damping[di] = damping[0] if damping.dim() == 4 else damping
+    graph_ba.bundle_adjustment(buf, target.reshape(-1, ht * wd, 2), wgt.reshape(-1, ht * wd, 2), graph.damping,
+                               ii, jj, t0, t1 if not fixed_motion else t0, itrs, 1e-3, 0.1, motion_only, limited_disp,
+                               False, False, False)
+    graph.age += 1
