@@ -221,15 +221,36 @@ __global__ __launch_bounds__(256) void spatial_corr_to_cl_kernel(const T* __rest
     }
 }
 
-// grid (ceil(oW/64), oH * patchH, N), 4 waves; wave v covers w0 = 64 bx + 16 v.
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b on XCD b % 8),
+// and each XCD has its own L2.  Renumbering so that XCD x runs the x-th
+// contiguous range of logical blocks keeps the rows a range shares (one image's
+// in1 / in2 rows at the local-attention shapes) in one L2.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
+    const int64_t xcd = b % kXcds, local = b / kXcds;
+    const int64_t per = (nb + kXcds - 1) / kXcds, r = nb % kXcds;
+    if (r == 0 || xcd < r) return xcd * per + local;
+    return r * per + (xcd - r) * (per - 1) + local;
+}
+
+// grid: ceil(oW/64) * oH * patchH * N workgroups in one dimension, 4 waves;
+// logical block (n, h, ph, wx), wx fastest; wave v covers w0 = 64 wx + 16 v.
 template <typename T, int NT>
 __global__ __launch_bounds__(256) void spatial_corr_mfma_kernel(const T* __restrict__ a_cl, const T* __restrict__ b_cl,
                                                                 T* __restrict__ out, SCArgs a, int Cp) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int w0 = blockIdx.x * 64 + wave * 16;
+    const int nwx = (a.oW + 63) / 64;
+    const int64_t nb = (int64_t)nwx * a.oH * a.patchH * a.N;
+    int64_t lb = xcd_logical(blockIdx.x, nb);
+    const int wx = (int)(lb % nwx);
+    lb /= nwx;
+    const int ph = (int)(lb % a.patchH);
+    lb /= a.patchH;
+    const int h = (int)(lb % a.oH);
+    const int n = (int)(lb / a.oH);
+    const int w0 = wx * 64 + wave * 16;
     if (w0 >= a.oW) return;   // no barriers below
-    const int h = blockIdx.y / a.patchH, ph = blockIdx.y % a.patchH;
-    const int n = blockIdx.z;
     const int radH = a.dpH * (a.patchH - 1) / 2, radW = a.dpW * (a.patchW - 1) / 2;
     const int y1 = h - a.padH, y2 = y1 + ph * a.dpH - radH;
     const int g = lane >> 4, c16 = lane & 15;
@@ -308,7 +329,7 @@ bool mfma_path(int dtype, const SCArgs& a) {
     const int nt = (16 + (a.patchW - 1) * a.dpW + 15) / 16;
     const size_t elem = dtype == VH_F32 ? 4 : 2;
     const int64_t copy_bytes = (int64_t)a.N * a.H * a.W * cl_channels(a.C, elem) * (int64_t)elem;
-    return nt <= kMaxNT && (int64_t)a.oH * a.patchH <= 65535 && a.N <= 65535 &&
+    return nt <= kMaxNT && (int64_t)((a.oW + 63) / 64) * a.oH * a.patchH * a.N < ((int64_t)1 << 31) &&
            copy_bytes + 64 < (int64_t)kScRange;
 }
 
@@ -331,7 +352,7 @@ int fwd_mfma_launch(const void* in1, const void* in2, void* out, const SCArgs& a
     VH_LAUNCH_CHECK("spatial_corr channels-last copy");
     hipLaunchKernelGGL(spatial_corr_to_cl_kernel<T>, tg, dim3(256), 0, s, (const T*)in2, b_cl, a.C, HW, Cp);
     VH_LAUNCH_CHECK("spatial_corr channels-last copy");
-    const dim3 grid((unsigned)((a.oW + 63) / 64), (unsigned)(a.oH * a.patchH), (unsigned)a.N);
+    const dim3 grid((unsigned)((int64_t)((a.oW + 63) / 64) * a.oH * a.patchH * a.N));
     const int nt = (16 + (a.patchW - 1) * a.dpW + 15) / 16;
     switch (nt) {
         case 1: hipLaunchKernelGGL((spatial_corr_mfma_kernel<T, 1>), grid, dim3(256), 0, s, a_cl, b_cl, (T*)out, a, Cp); break;
@@ -377,17 +398,27 @@ template <> __device__ __forceinline__ float load_g<__hip_bfloat16>(__amdgpu_buf
     return __uint_as_float(buf_u16(r, off) << 16);
 }
 
-// grid (ceil(ceil(W/16) / 4), H, N * ceil(Cp / (16 kBwdCT))), 4 waves
+// grid: ceil(ceil(W/16) / 4) * H * N * ncg workgroups in one dimension
+// (ncg = ceil(Cp / (16 kBwdCT))), 4 waves; logical block (n, cg, y, bx), bx
+// fastest, spread over the XCDs in contiguous ranges (xcd_logical) so that the
+// rows neighbouring y share stay in one L2.
 template <typename T, bool SECOND>
 __global__ __launch_bounds__(256) void spatial_corr_bwd_mfma_kernel(const T* __restrict__ gout,
                                                                     const float* __restrict__ other_cl,
                                                                     T* __restrict__ grad, SCArgs a, int Cp) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int x0 = (blockIdx.x * 4 + wave) * 16;
-    if (x0 >= a.W) return;   // no barriers below
-    const int y = blockIdx.y;
     const int ncg = (Cp + 16 * kBwdCT - 1) / (16 * kBwdCT);
-    const int n = blockIdx.z / ncg, cg = blockIdx.z % ncg;
+    const int nbx = ((a.W + 15) / 16 + 3) / 4;
+    const int64_t nb = (int64_t)nbx * a.H * a.N * ncg;
+    int64_t lb = xcd_logical(blockIdx.x, nb);
+    const int bx = (int)(lb % nbx);
+    lb /= nbx;
+    const int y = (int)(lb % a.H);
+    lb /= a.H;
+    const int cg = (int)(lb % ncg);
+    const int n = (int)(lb / ncg);
+    const int x0 = (bx * 4 + wave) * 16;
+    if (x0 >= a.W) return;   // no barriers below
     const int nct = min(kBwdCT, Cp / 16 - cg * kBwdCT);
     const int i16 = lane & 15, g = lane >> 4;
     const int rbH = (a.patchH - 1) / 2 * a.dpH, rbW = (a.patchW - 1) / 2 * a.dpW;
@@ -462,7 +493,7 @@ bool bwd_mfma_path(int dtype, const SCArgs& a) {
     const int64_t gout_bytes = (int64_t)a.patchH * a.patchW * a.oH * a.oW * (dtype == VH_F32 ? 4 : 2);
     const int64_t ncg = (Cp + 16 * kBwdCT - 1) / (16 * kBwdCT);
     return copy_bytes / a.N + 64 < (int64_t)kScRange && gout_bytes + 64 < (int64_t)kScRange &&
-           (int64_t)a.N * ncg <= 65535 && a.H <= 65535;
+           (int64_t)(((a.W + 15) / 16 + 3) / 4) * a.H * a.N * ncg < ((int64_t)1 << 31);
 }
 
 size_t bwd_mfma_workspace(int dtype, const SCArgs& a) {
@@ -485,7 +516,7 @@ int bwd_mfma_launch(const void* in1, const void* in2, const void* gout, void* g1
     hipLaunchKernelGGL((spatial_corr_to_cl_kernel<T, float>), tg, dim3(256), 0, s, (const T*)in2, in2_cl, a.C, HW, Cp);
     VH_LAUNCH_CHECK("spatial_corr channels-last copy");
     const int ncg = (Cp + 16 * kBwdCT - 1) / (16 * kBwdCT);
-    const dim3 grid((unsigned)(((a.W + 15) / 16 + 3) / 4), (unsigned)a.H, (unsigned)(a.N * ncg));
+    const dim3 grid((unsigned)((int64_t)(((a.W + 15) / 16 + 3) / 4) * a.H * a.N * ncg));
     hipLaunchKernelGGL((spatial_corr_bwd_mfma_kernel<T, false>), grid, dim3(256), 0, s, (const T*)gout,
                        (const float*)in2_cl, (T*)g1, a, Cp);
     VH_LAUNCH_CHECK("spatial_corr_backward input1 (matrix cores)");
