@@ -639,50 +639,35 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
 // ---------------------------------------------------------------------------
 // Signed contribution of one assembly-list item to entry (i, j) of its block,
 // branch-free so that the loads of several items can be in flight together.
-__device__ __forceinline__ double block_item(const Work& w, int item, int i, int j, int motion) {
-    const int kind = item >> 27, idx = item & ((1 << 27) - 1);
-    const bool gram = kind >= K_G;
-    const bool tr = (kind == K_HIJT) || (kind == K_GT);
-    const int ij = tr ? j * 6 + i : i * 6 + j;
-    const int base = kind == K_HII ? E_HII : (kind == K_HJJ ? E_HJJ : E_HIJ);
-    const double* src = gram ? w.gd + idx : w.ed + (int64_t)idx * kEDS + base;
-    // Schur terms exist only when the depth system is solved; in motion-only
-    // mode the Gram buffer is never written.
-    if (gram && motion) return 0.0;
-    const double v = src[ij];
-    return gram ? -v : v;
-}
-
-constexpr int kAsmBlocks = kThreads / 36;  // pose blocks per assembly workgroup (36 threads each)
-
-__global__ __launch_bounds__(kThreads) void ba_assemble_kernel(DevPlan p, RunArgs a, Work w, int motion) {
+__global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, Work w, int motion) {
     const int tid = threadIdx.x;
-    const int nbw = (p.nlb + kAsmBlocks - 1) / kAsmBlocks;  // workgroups of pose blocks
+    const int bid = blockIdx.x;
     const int ld = p.ld;
     const int n6 = 6 * p.P;
-    if ((int)blockIdx.x < nbw) {
-        // thread (g, o): entry o of pose block 7 * blockIdx + g; its list's item
-        // loads are issued four at a time
-        const int g = tid / 36, o = tid % 36;
-        const int blk = blockIdx.x * kAsmBlocks + g;
-        if (g >= kAsmBlocks || blk >= p.nlb) return;
-        const int ab = p.blk_ab[2 * blk], bb = p.blk_ab[2 * blk + 1];
-        const int i = o / 6, j = o % 6;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        const int it0 = p.blk_ptr[blk], it1 = p.blk_ptr[blk + 1];
-        int it = it0;
-        for (; it + 4 <= it1; it += 4) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[u] += block_item(w, p.blk_items[it + u], i, j, motion);
+    if (bid < p.nlb) {
+        if (tid >= 36) return;
+        const int ab = p.blk_ab[2 * bid], bb = p.blk_ab[2 * bid + 1];
+        const int i = tid / 6, j = tid % 6;
+        double s = 0.0;
+        for (int it = p.blk_ptr[bid]; it < p.blk_ptr[bid + 1]; it++) {
+            const int item = p.blk_items[it];
+            const int kind = item >> 27, idx = item & ((1 << 27) - 1);
+            switch (kind) {
+                case K_HII: s += w.ed[(int64_t)idx * kEDS + E_HII + i * 6 + j]; break;
+                case K_HJJ: s += w.ed[(int64_t)idx * kEDS + E_HJJ + i * 6 + j]; break;
+                case K_HIJ: s += w.ed[(int64_t)idx * kEDS + E_HIJ + i * 6 + j]; break;
+                case K_HIJT: s += w.ed[(int64_t)idx * kEDS + E_HIJ + j * 6 + i]; break;
+                // Schur terms exist only when the depth system is solved; in motion-only
+                // mode the Gram buffer is never written.
+                case K_G: if (!motion) s -= w.gd[idx + i * 6 + j]; break;
+                case K_GT: if (!motion) s -= w.gd[idx + j * 6 + i]; break;
+                default: break;
+            }
         }
-        for (; it < it1; it++) acc[0] += block_item(w, p.blk_items[it], i, j, motion);
-        double s = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         if (ab == bb && i == j) s += (double)a.ep + (double)a.lm * s;
         w.A[(int64_t)(6 * ab + i) * ld + 6 * bb + j] = s;
         return;
     }
-    const int bid = blockIdx.x - nbw + p.nlb;  // the remaining roles keep their former numbering
-    if (tid >= 64) return;
     const int r = bid - p.nlb;
     if (r < p.P) {
         if (tid >= 6) return;
@@ -2227,8 +2212,7 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
             hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + d.K * kGramSplit), dim3(kThreads), 0, s, d, a, w, 0);
             VH_LAUNCH_CHECK("ba_finalize_kernel");
         }
-        hipLaunchKernelGGL(ba_assemble_kernel, dim3((d.nlb + kAsmBlocks - 1) / kAsmBlocks + d.P + 1 + npad),
-                           dim3(kThreads), 0, s, d, a, w, motion ? 1 : 0);
+        hipLaunchKernelGGL(ba_assemble_kernel, dim3(d.nlb + d.P + 1 + npad), dim3(64), 0, s, d, a, w, motion ? 1 : 0);
         VH_LAUNCH_CHECK("ba_assemble_kernel");
         hipLaunchKernelGGL(chol_factor_kernel, dim3(1), dim3(256), 0, s, w.A, w.Li, d.ld, 0, w.flag);
         VH_LAUNCH_CHECK("chol_factor_kernel");
