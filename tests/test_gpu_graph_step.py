@@ -93,3 +93,57 @@ def test_update_step_matches_oracle_composition():
     assert (graph.target[0].cpu().double() - t).abs().max().item() < 1e-4
     torch.testing.assert_close(buf.poses[:5].cpu().double(), ref.poses[:5].double(), atol=2e-5, rtol=1e-5)
     torch.testing.assert_close(buf.disps[:5, 0].cpu().double(), disps[:5].double(), atol=2e-4, rtol=2e-4)
+
+
+def test_update_batch_matches_oracle_composition():
+    """update_batch with two source-frame batches (batch = 2) and two steps vs
+    the same rounds composed from the oracle pieces."""
+    import test_graph_ba as TG
+    from vipe_amd import graph_step
+
+    buf = TG.make_buffer("cuda")
+    ref = TG.make_buffer("cpu")
+    ii = torch.tensor([1, 2, 3, 4, 0, 3, 2, 0])
+    jj = torch.tensor([0, 0, 1, 2, 2, 4, 3, 4])
+    E = ii.shape[0]
+    intr8 = ref.intrinsics[0, :4] / 8.0
+    target0 = (OG.reproject(ref.poses, ref.disps[:, 0], intr8, ii, jj) - 0.2).float()
+    graph = SimpleNamespace(buffer=buf, ii=ii.cuda(), jj=jj.cuda(), target=target0[None].cuda().clone(),
+                            weight=torch.zeros(1, E, 4, 5, 2, device="cuda"),
+                            damping=torch.full((6, 4, 5), 0.1, device="cuda"), f_net=torch.zeros(1, E, 1, device="cuda"),
+                            inps=torch.zeros(6, 1, device="cuda"), n_frames=5)
+    seen = []
+
+    def alt_corr(c, a, b):
+        seen.append(tuple(a.tolist()))
+        return None
+
+    def op(f_net, inp, corr, motn, ix):
+        n, d, w, dmp, u = _fake_update(f_net, inp, corr, motn, ix)
+        return n, d, w, dmp, u
+
+    graph_step.update_batch(graph, op, alt_corr, itrs=2, steps=2, batch=2)
+    assert len(seen) == 2 * 3   # batches [0,2), [2,4), [4,6) in each of two steps
+    # oracle composition
+    rt = target0.double().clone()
+    damping = torch.full((6, 4, 5), 0.1)
+    for _ in range(2):
+        c1, motn = OG.motion_features(ref.poses, ref.disps[:, 0], intr8, ii, jj, rt)
+        w_all = torch.zeros(E, 4, 5, 2, dtype=torch.float64)
+        for i0 in range(0, 5, 2):
+            v = (ii >= i0) & (ii < i0 + 2)
+            dis, dix = torch.unique(ii[v], return_inverse=True)
+            _, delta, weight, _, _ = _fake_update(torch.zeros(1), None, None, motn[v][None].float(), dix)
+            t, w = OG.targets(c1[v], delta[0].double(), weight[0].double())
+            rt[v], w_all[v] = t, w
+            damping[dis] = 0.05
+        tb = rt.float().permute(0, 3, 1, 2).contiguous()
+        wb = w_all.float().permute(0, 3, 1, 2).contiguous()
+        kx = torch.unique(torch.cat([torch.arange(1, 5), ii]))
+        disps = ref.disps[:, 0]
+        OB.ba_extended(ref.poses, disps, intr8, ref.disps_sens[:, 0], tb, wb, 0.2 * damping[kx] + 1e-7, ii, jj,
+                       torch.ones(6), 1, 5, 2, 1e-5, 1e-2, False, 0.001, False, 1e-6, 1e-6, 0.125, False)
+        ref.disps.clamp_(min=0.001)
+    assert (graph.target[0].cpu().double() - rt).abs().max().item() < 2e-4
+    torch.testing.assert_close(buf.poses[:5].cpu().double(), ref.poses[:5].double(), atol=5e-5, rtol=1e-5)
+    torch.testing.assert_close(buf.disps[:5, 0].cpu().double(), ref.disps[:5, 0].double(), atol=5e-4, rtol=5e-4)

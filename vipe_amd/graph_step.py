@@ -104,3 +104,46 @@ def update(graph, update_op, corr_fn, t0=None, t1=None, itrs=3, motion_only=Fals
                                ii, jj, t0, t1 if not fixed_motion else t0, itrs, 1e-3, 0.1, motion_only, limited_disp,
                                False, False, False)
     graph.age += 1
+
+
+def update_batch(graph, update_op, alt_corr_fn, itrs, steps, optimize_intrinsics=False, batch=8):
+    """FactorGraph.update_batch (factor_graph.py:315-392): ``steps`` rounds of
+    motion features for all edges, the update operator over batches of
+    ``batch`` source frames, and a global BA (t0 = 1, t1 = n_frames,
+    pose damping 1e-5, ep 1e-2).
+
+    ``alt_corr_fn(coords1 [1,Eb,H,W,2], ii_b, jj_b)`` returns the batch's
+    correlation features (the reference's AltCorrBlock); ``update_op`` as in
+    ``update``, called with ``graph.f_net[:, mask]`` and ``graph.inps[ii_b]``
+    where ``graph.inps`` holds each frame's context features [N, ...].
+    ``graph.n_frames`` is the number of frames in the window."""
+    buf = graph.buffer
+    ii, jj = graph.ii, graph.jj
+    ht, wd = buf.height // 8, buf.width // 8
+    intr = (buf.intrinsics[0, :4] / 8.0).contiguous()
+    masks = getattr(buf, "masks", None)
+    mask = None if masks is None else masks[:, 0].contiguous()
+    if int(jj.max()) < int(ii.max()):
+        raise RuntimeError("update_batch expects max(jj) >= max(ii)")
+    for _ in range(steps):
+        coords1, motn = motion_features(buf.poses, buf.disps[:, 0], intr, ii.contiguous(), jj.contiguous(),
+                                        graph.target[0].contiguous())
+        for i in range(0, int(jj.max()) + 1, batch):
+            v = (ii >= i) & (ii < i + batch)
+            if not bool(v.any()):
+                continue
+            iis, jjs = ii[v], jj[v]
+            corr1 = alt_corr_fn(coords1[v][None], iis, jjs)
+            dis, dixs = torch.unique(iis, return_inverse=True)
+            net, delta, weight, damping, _ = update_op(graph.f_net[:, v], graph.inps[iis][None], corr1,
+                                                       motn[v][None], ix=dixs)
+            graph.f_net[:, v] = net
+            t, wgt, _, _ = update_targets(coords1[v].contiguous(), delta[0].contiguous(), weight[0].contiguous(),
+                                          mask, iis.contiguous() if mask is not None else None, ba_layout=False)
+            graph.target[0, v] = t
+            graph.weight[0, v] = wgt
+            graph.damping[dis] = damping[0] if damping.dim() == 4 else damping
+        graph_ba.bundle_adjustment(buf, graph.target[0].reshape(-1, ht * wd, 2),
+                                   graph.weight[0].reshape(-1, ht * wd, 2), graph.damping, ii, jj, 1,
+                                   int(graph.n_frames), itrs, 1e-5, 1e-2, False, False, bool(optimize_intrinsics),
+                                   False, False)
