@@ -115,3 +115,32 @@ def test_graph_caller_runs_the_gnc_schedule():
     rd = rd.clamp(min=0.001)
     torch.testing.assert_close(buf.poses[:5].cpu().double(), rp[:5].double(), atol=2e-5, rtol=1e-5)
     torch.testing.assert_close(buf.disps[:5, 0].cpu().double(), rd[:5].double(), atol=2e-4, rtol=2e-4)
+
+
+@pytest.mark.parametrize("intr,motion", [(True, False), (False, True), (True, True)])
+def test_robust_with_intrinsics_and_motion_only(intr, motion):
+    """The other robust frame-kernel instantiations: intrinsics optimisation and
+    motion-only, with Tukey weights and planted outliers."""
+    from vipe_amd.ext import slam_ext
+
+    g = _outlier_graph()
+    N = g.disps.shape[0]
+    da = torch.ones(N)
+    kind = (R.TUKEY, 2.0, 1.0, 1e6)
+    p, d, it = g.poses.clone().cuda(), g.disps.clone().cuda(), g.intrinsics.clone().cuda()
+    dx, dz, en = slam_ext.ba_extended_robust(p, d, it, g.disps_sens.cuda(), g.targets.cuda(), g.weights.cuda(),
+                                             g.eta.cuda(), g.ii.cuda(), g.jj.cuda(), da.cuda(), g.t0, g.t1, 2, 1e-4,
+                                             0.1, motion, 0.05, intr, 1e-6, 1e-6, 0.125, True, kind)
+    rp, rd, rit = g.poses.clone(), g.disps.clone(), g.intrinsics.clone()
+    rdx, rdz, ren = OB.ba_extended(rp, rd, rit, g.disps_sens, g.targets, g.weights, g.eta, g.ii, g.jj, da, g.t0,
+                                   g.t1, 2, 1e-4, 0.1, motion, 0.05, intr, 1e-6, 1e-6, 0.125, True, robust=kind)
+    torch.cuda.synchronize()
+    assert (dx.cpu().double() - rdx).abs().max().item() < 1e-5 * max(1.0, rdx.abs().max().item())
+    assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
+    if motion:
+        assert dz is None and torch.equal(d.cpu(), g.disps)
+    else:
+        assert (d.cpu().double() - rd.double()).abs().max().item() < 1e-4
+    if intr:
+        assert (it.cpu().double() - rit.double()).abs().max().item() < 1e-4
+    assert torch.allclose(en.cpu().double(), ren, rtol=1e-4, atol=1e-6)
