@@ -217,3 +217,27 @@ def test_autograd_matches_reference_formulas():
     expect = torch.zeros(6, 21, 3, dtype=torch.float64, device="cuda")
     expect.scatter_(1, arg, w)
     assert torch.equal(src.grad, expect[:, :20])
+
+
+@pytest.mark.parametrize("reduce", ["sum", "min", "max"])
+def test_segmented_path_with_given_out(reduce):
+    """A broadcast 1-D index (segmented path) into a given out that takes part
+    in the reduction, against the oracle."""
+    from vipe_amd.ext import scatter_ext
+
+    g = torch.Generator().manual_seed(12)
+    src = torch.randn(2, 50, 64, generator=g)
+    idx = torch.randint(0, 9, (50,), generator=g)
+    out = torch.randn(2, 11, 64, generator=g).cuda()
+    out0 = out.cpu().numpy()
+    fn = getattr(scatter_ext, f"scatter_{reduce}")
+    res = fn(src.cuda(), idx.cuda(), 1, out, None)
+    ref, arg = OSc.scatter(src.numpy(), idx.numpy(), 1, out0, None, reduce)
+    if reduce == "sum":
+        assert res.data_ptr() == out.data_ptr()
+        mag, _ = OSc.scatter(np.abs(src.numpy()), idx.numpy(), 1, np.abs(out0), None, "sum")
+        assert np.all(np.abs(out.cpu().numpy() - ref) <= 16 * 2 ** -24 * mag + 1e-30)
+    else:
+        assert res[0].data_ptr() == out.data_ptr()
+        assert np.array_equal(out.cpu().numpy(), ref.astype(np.float32))
+        assert np.array_equal(res[1].cpu().numpy(), arg)

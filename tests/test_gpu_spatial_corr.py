@@ -120,3 +120,23 @@ def test_argument_errors_on_device():
         corr_ext.forward(a, a, 9, 9, 3, 3, 0, 0, 1, 1, 1, 1, 1, 1)
     with pytest.raises(RuntimeError, match="grad_output"):
         corr_ext.backward(a, a, torch.zeros(1, 3, 3, 7, 7, device="cuda"), 1, 1, 3, 3, 0, 0, 1, 1, 1, 1, 1, 1)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float16])
+def test_matrix_core_path_wide_channels_odd_width(dtype):
+    """C = 300 (several channel steps and a padded tail), odd width, padding:
+    forward and backward of the kernel-1 path against the oracle."""
+    from vipe_amd.ext import corr_ext
+
+    case = (1, 1, 7, 9, 1, 2, 1, 1, 1, 2, 1, 1)
+    a, b = _inputs((3, 300, 11, 37), dtype, seed=77)
+    out = corr_ext.forward(a.cuda(), b.cuda(), *case)
+    an, bn = a.double().numpy(), b.double().numpy()
+    ref = OS.forward(an, bn, *case)
+    assert np.abs(out.cpu().double().numpy() - ref).max() <= TOL[dtype] * _scale(an, bn, case)
+    go = torch.randn(out.shape, generator=torch.Generator().manual_seed(3)).to(dtype)
+    g1, g2 = corr_ext.backward(a.cuda(), b.cuda(), go.cuda(), *case)
+    r1, r2 = OS.backward(an, bn, go.double().numpy(), *case)
+    s1, s2 = OS.backward(np.abs(an), np.abs(bn), np.abs(go.double().numpy()), *case)
+    assert np.abs(g1.cpu().double().numpy() - r1).max() <= TOL[dtype] * max(1.0, s1.max())
+    assert np.abs(g2.cpu().double().numpy() - r2).max() <= TOL[dtype] * max(1.0, s2.max())
