@@ -12,8 +12,8 @@
 //      accumulates (a) the edge's j-side pose terms Hjj, vj (+ intrinsics
 //      terms, energy) - the i-side follows from Ji = -Adj(Tij)^T Jj; (b) the
 //      depth terms C, w; (c) the E rows (self row + one per edge) in LDS, and
-//      from them the frame's Schur Gram sum_p Q E_a E_b^T, plus
-//      [w, Ef] as a pseudo row so the Schur right-hand side and the
+//      from them the frame's Schur Gram sum_p Q E_a E_b^T on the f32 matrix
+//      cores (16 x 16 tiles of E rows), plus [w, Ef] as a pseudo row so the Schur right-hand side and the
 //      intrinsics coupling come out of the same Gram.  Writes per-tile
 //      partials and Q, w per pixel.  E is never written to memory.
 //   2. ba_finalize_kernel  sums partials over tiles in FP64, forms
@@ -291,14 +291,110 @@ __device__ __forceinline__ void gram_rows(int b, int Rext, int& ra, int& rb) {
     rb = a + b;
 }
 
+// Schur Gram of a frame on v_mfma_f32_16x16x4f32.  The Rext*6 E rows are cut
+// into nT = ceil(Rext*6 / 16) row tiles; the upper tile pairs (ti <= tj),
+// np = nT (nT+1) / 2 of them, are 16 x 16 MFMA accumulators over the pixels.
+// A sub-tile's TP pixels are split into ks parts; the 4 / ks waves of one part
+// share its pairs (at most kGramSlots each).  ks minimises the per-wave work
+// (pairs x pixels), the smaller ks on a tie.
+constexpr int kGramSlots = 12;
+constexpr int kRowPad = 4;  // E-row stride TP + 4: each row starts 16-byte aligned
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline int gram_tiles(int Rext) { return (Rext * 6 + 15) / 16; }
+
+__host__ __device__ inline int gram_ksplit(int Rext) {
+    const int nT = gram_tiles(Rext);
+    const int np = nT * (nT + 1) / 2;
+    int best = 1, bestc = 1 << 30;
+    for (int ks = 1; ks <= 4; ks *= 2) {
+        const int wpk = 4 / ks;
+        const int slots = (np + wpk - 1) / wpk;
+        if (slots > kGramSlots) continue;
+        const int c = slots * wpk;  // pairs x (TP / ks) in units of TP / 4
+        if (c < bestc) {
+            bestc = c;
+            best = ks;
+        }
+    }
+    return best;
+}
+
+// floats of the cross-part reduction buffer [ks][np][256]
+__host__ __device__ inline int gram_red_floats(int Rext) {
+    const int nT = gram_tiles(Rext);
+    return gram_ksplit(Rext) * (nT * (nT + 1) / 2) * 256;
+}
+
+// dynamic LDS of the frame kernel: the E rows of a sub-tile, or the Gram
+// reduction buffer of the largest frame, whichever is larger
+inline size_t frame_lds_bytes(int Rext_max, int TP) {
+    size_t fl = (size_t)Rext_max * 6 * (TP + kRowPad);
+    for (int r = 1; r <= Rext_max; r++) fl = std::max(fl, (size_t)gram_red_floats(r));
+    return fl * 4;
+}
+
+// One sub-tile's Gram pass of a wave that owns S tile pairs (pair0 + pstride j):
+// G[16ti + r][16tj + c] += sum_q Q_q E[16ti + r][q] E[16tj + c][q] over pixels
+// [q0, q1), 16 pixels (four k-steps) per step.  Lane (gi, gk) feeds row gi of a
+// tile at pixels q + 4gk .. q + 4gk + 3.  A lane whose row is at or past nrows
+// reads the last row instead: row r of A reaches only row r of D and row c of B
+// only column c, so those padding rows and columns of D, never read back, are
+// the only entries it touches.  The loads of a group of up to four
+// pairs are issued before its MFMAs, which interleave over the group's
+// independent accumulators.
+template <int S>
+__device__ __forceinline__ void gram_pass(f32x4 (&acc)[kGramSlots], const float* dyn, const float* sQ, int q0, int q1,
+                                          int pair0, int pstride, int nT, int nrows, int ldr, int gi, int gk) {
+    int oA[S], oB[S];
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        int ti, tj;
+        gram_rows(pair0 + pstride * j, nT, ti, tj);
+        oA[j] = min(16 * ti + gi, nrows - 1) * ldr + 4 * gk;
+        oB[j] = min(16 * tj + gi, nrows - 1) * ldr + 4 * gk;
+    }
+    for (int q = q0; q < q1; q += 16) {
+        const float4 qv = *reinterpret_cast<const float4*>(&sQ[q + 4 * gk]);
+        // groups of four pairs bound the registers the loads hold
+#pragma unroll
+        for (int j0 = 0; j0 < S; j0 += 4) {
+            float4 av[4], bv[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (j0 + j < S) {
+                    const float4 x = *reinterpret_cast<const float4*>(dyn + oA[j0 + j] + q);
+                    const float4 y = *reinterpret_cast<const float4*>(dyn + oB[j0 + j] + q);
+                    av[j] = make_float4(x.x * qv.x, x.y * qv.y, x.z * qv.z, x.w * qv.w);
+                    bv[j] = y;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j0 + j < S) acc[j0 + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j].x, bv[j].x, acc[j0 + j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j0 + j < S) acc[j0 + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j].y, bv[j].y, acc[j0 + j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j0 + j < S) acc[j0 + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j].z, bv[j].z, acc[j0 + j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j0 + j < S) acc[j0 + j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j].w, bv[j].w, acc[j0 + j], 0, 0, 0);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 1. frame kernel
 // ---------------------------------------------------------------------------
 template <bool INTR, bool MOTION, bool ROBUST = false>
-__global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a, Work w) {
-    extern __shared__ float dyn[];  // rows [Rext_max*6][TP+1]; reused for the Gram reduction
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void ba_frame_kernel(DevPlan p, RunArgs a, Work w) {
+    extern __shared__ f32x4 dyn4[];  // rows [Rext_max*6][TP+kRowPad]; reused for the Gram reduction
+    float* dyn = reinterpret_cast<float*>(dyn4);
     __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo
-    __shared__ float sQ[256];
+    __shared__ __align__(16) float sQ[256];
     __shared__ float sAcc[4][kMaxDeg][kNV];  // per-wave edge partials (each wave adds into its own slot)
     __shared__ float sFen[4];
 
@@ -307,7 +403,7 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int TP = p.TP;
-    const int ldr = TP + 1;
+    const int ldr = TP + kRowPad;
     const int kf = p.kx[k];
     const int e0 = p.fr_ptr[k];
     const int deg = p.fr_ptr[k + 1] - e0;
@@ -315,7 +411,6 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
     const int R = has_self + deg;
     const int Rext = R + 1;
     const int NB = Rext * (Rext + 1) / 2;
-    const int NS = kThreads / NB;
     const int HW = p.HW;
     const float act = a.depth_active[kf];
     const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
@@ -332,14 +427,21 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
     for (int i = tid; i < 4 * kMaxDeg * kNV; i += kThreads) (&sAcc[0][0][0])[i] = 0.f;
     if (tid < 4) sFen[tid] = 0.f;
 
-    // Gram thread mapping (fixed for all sub-tiles)
-    int gra = 0, grb = 0;
-    const int gslice = (NB > 0) ? tid / NB : 0;
-    const bool gactive = !MOTION && (gslice < NS);
-    if (gactive) gram_rows(tid % NB, Rext, gra, grb);
-    float G[36];
+    // Gram wave mapping (fixed for all sub-tiles): this wave's pixel part and
+    // tile pairs; lane (gi, gk) feeds row gi of a tile at pixel offset 4 gk
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int gnrows = Rext * 6;
+    const int gnT = gram_tiles(Rext);
+    const int gnp = gnT * (gnT + 1) / 2;
+    const int gks = gram_ksplit(Rext);
+    const int gwpk = 4 / gks;
+    const int gpart = wv / gwpk;
+    const int gklen = TP / gks;
+    const int gnslot = (gnp - wv % gwpk + gwpk - 1) / gwpk;
+    const int gi = lane & 15, gk = lane >> 4;
+    f32x4 gacc[kGramSlots];
 #pragma unroll
-    for (int v = 0; v < 36; v++) G[v] = 0.f;
+    for (int j = 0; j < kGramSlots; j++) gacc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
     float fen_acc = 0.f;
     __syncthreads();
 
@@ -462,22 +564,19 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
                 sQ[tid] = Q;
             }
             __syncthreads();
-            if (gactive) {
-                const float* ra = dyn + gra * 6 * ldr;
-                const float* rb = dyn + grb * 6 * ldr;
-                for (int q = gslice; q < TP; q += NS) {
-                    const float qq = sQ[q];
-                    float av[6], bv[6];
-#pragma unroll
-                    for (int n = 0; n < 6; n++) {
-                        av[n] = ra[n * ldr + q] * qq;
-                        bv[n] = rb[n * ldr + q];
-                    }
-#pragma unroll
-                    for (int n = 0; n < 6; n++)
-#pragma unroll
-                        for (int m = 0; m < 6; m++) G[n * 6 + m] += av[n] * bv[m];
-                }
+            // G[16ti + r][16tj + c] += sum_q Q_q E[16ti + r][q] E[16tj + c][q]
+            // over this wave's pixel part, 16 pixels (four k-steps) per pass;
+            // rows at or past Rext*6 feed zeros
+            const int qb = gpart * gklen;
+            switch (gnslot) {
+#define VH_GRAM_CASE(S)                                                                                    \
+    case S:                                                                                                \
+        gram_pass<S>(gacc, dyn, sQ, qb, qb + gklen, wv % gwpk, gwpk, gnT, gnrows, ldr, gi, gk);            \
+        break;
+                VH_GRAM_CASE(1) VH_GRAM_CASE(2) VH_GRAM_CASE(3) VH_GRAM_CASE(4) VH_GRAM_CASE(5) VH_GRAM_CASE(6)
+                VH_GRAM_CASE(7) VH_GRAM_CASE(8) VH_GRAM_CASE(9) VH_GRAM_CASE(10) VH_GRAM_CASE(11) VH_GRAM_CASE(12)
+#undef VH_GRAM_CASE
+                default: break;
             }
             __syncthreads();
         }
@@ -494,20 +593,34 @@ __global__ __launch_bounds__(kThreads) void ba_frame_kernel(DevPlan p, RunArgs a
         // depth-prior energy
         const float f = wave_sum(fen_acc);
         if (lane == 0) sFen[wave] = f;
-        // cross-slice Gram reduction through LDS
-        float* red = dyn;  // [NS][NB][36]
-        if (gactive) {
-            const int b = tid % NB;
+        // Gram tiles of every pixel part into LDS [ks][np][lane][4] (the E rows are dead)
+        float* red = dyn;
 #pragma unroll
-            for (int v = 0; v < 36; v++) red[(gslice * NB + b) * 36 + v] = G[v];
-        }
+        for (int j = 0; j < kGramSlots; j++)
+            if (j < gnslot)
+                reinterpret_cast<f32x4*>(red)[(gpart * gnp + wv % gwpk + gwpk * j) * 64 + lane] = gacc[j];
         __syncthreads();
         if (tid == 0) w.fen_part[(int64_t)tile * p.K + k] = sFen[0] + sFen[1] + sFen[2] + sFen[3];
+        // 6 x 6 blocks (ra <= rb): element (n, m) is G[6ra + n][6rb + m], read as
+        // G[c][r] below the diagonal; D[r][c] of a tile sits at lane c + 16 (r / 4),
+        // component r % 4
         const int nvals = NB * 36;
         float* out = w.gram_part + (int64_t)tile * p.GT + p.gram_off[k];
         for (int i = tid; i < nvals; i += kThreads) {
+            const int b = i / 36, v = i - 36 * (i / 36);
+            int ra, rb;
+            gram_rows(b, Rext, ra, rb);
+            int r = 6 * ra + v / 6, c = 6 * rb + v % 6;
+            if (r > c) {
+                const int t = r;
+                r = c;
+                c = t;
+            }
+            const int ti = r >> 4, tj = c >> 4;
+            const int pair = ti * gnT - ti * (ti - 1) / 2 + (tj - ti);
+            const int off = pair * 256 + ((c & 15) + 16 * ((r & 15) >> 2)) * 4 + (r & 3);
             float s = 0.f;
-            for (int sl = 0; sl < NS; sl++) s += red[sl * nvals + i];
+            for (int kp = 0; kp < gks; kp++) s += red[kp * gnp * 256 + off];
             out[i] = s;
         }
     }
@@ -1966,12 +2079,12 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
     // tiles: TP pixels per sub-tile, sized so the E rows fit the LDS budget
     // unless the caller chose TP (then it must fit the workgroup's LDS)
     int TP = 256;
-    while (TP > 64 && (size_t)Rext_max * 6 * (TP + 1) * 4 > kLdsBudget) TP /= 2;
+    while (TP > 64 && frame_lds_bytes(Rext_max, TP) > (size_t)kLdsBudget) TP /= 2;
     if (pixel_tile != 0) {
-        if ((size_t)Rext_max * 6 * (pixel_tile + 1) * 4 > kLdsMax) {
+        if (frame_lds_bytes(Rext_max, pixel_tile) > (size_t)kLdsMax) {
             delete pl;
             fail(VH_ERR_UNSUPPORTED, "ba_plan_create: %d-pixel tiles need %zu B of LDS for %d E rows (limit %d)",
-                 pixel_tile, (size_t)Rext_max * 6 * (pixel_tile + 1) * 4, Rext_max * 6, kLdsMax);
+                 pixel_tile, frame_lds_bytes(Rext_max, pixel_tile), Rext_max * 6, kLdsMax);
             return nullptr;
         }
         TP = pixel_tile;
@@ -2179,7 +2292,7 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
     a.targets = targets; a.weights = weights; a.eta = eta; a.depth_active = depth_active;
     a.lm = lm; a.ep = ep; a.alpha = alpha; a.ilm = intrinsics_lm; a.iep = intrinsics_ep;
     a.iscale = intrinsics_scale; a.dx = dx; a.dz = dz; a.energy = energy; a.compute_energy = cen ? 1 : 0;
-    const size_t lds = std::max((size_t)d.Rext_max * 6 * (d.TP + 1), (size_t)kThreads * 36) * 4;
+    const size_t lds = frame_lds_bytes(d.Rext_max, d.TP);
     const dim3 fgrid(d.T, d.K);
     const int npad = d.ld - d.n;
     for (int itr = 0; itr < iterations; itr++) {
