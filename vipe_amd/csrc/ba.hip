@@ -249,35 +249,97 @@ __device__ __forceinline__ void neg_adjT(const float* R, const float* t, const f
     o[5] = -(R[2] * b0 + R[5] * b1 + R[8] * b2);
 }
 
-// Sum of 32 values over the 64 lanes of a wave with 32 exchanges: each
-// halving step S keeps half of the live values (the half selected by lane bit
-// S) and adds the partner lane's other half, so after S = 16..1 lane l holds the
-// partial of value (l & 31); a last step adds the other half-wave.  The step
-// width is a template constant so every array index is static.
-template <int S>
-__device__ __forceinline__ void halve_step(float (&v)[32], bool hi) {
+// Cross-lane exchanges without the LDS crossbar.  Every stage pairs each lane
+// with one whose stage bit differs; the partner need not be lane ^ S, only a
+// lane whose partial so far covers a disjoint set of lanes, which any such
+// pairing (xor, or a mirror that also flips the lower bits) gives.
+//   S = 1, 2: DPP quad_perm [1,0,3,2], [2,3,0,1]
+//   S = 4, 8: DPP row_half_mirror, row_mirror
+//   S = 16:   v_permlane16_swap (odd rows of one register <-> even rows of the other)
+//   S = 32:   v_permlane32_swap (upper half of one register <-> lower half of the other)
+template <int CTRL>
+__device__ __forceinline__ float dpp_partner(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+// x + y after the 16-lane-row swap: rows 0, 2 hold x of the row pair, rows
+// 1, 3 hold y of the row pair
+__device__ __forceinline__ float swap16_sum(float x, float y) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap32_sum(float x) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// The same exchanges on doubles, one 32-bit half at a time.
+template <int CTRL>
+__device__ __forceinline__ double dpp_partner_f64(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double swap16_sum_f64(double x, double y) {
+    const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)(bx & 0xffffffffll), (unsigned)(by & 0xffffffffll), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) + __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+__device__ __forceinline__ double swap32_sum_f64(double x, double y) {
+    const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)(bx & 0xffffffffll), (unsigned)(by & 0xffffffffll), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(bx >> 32), (unsigned)(by >> 32), false, false);
+    return __longlong_as_double(((long long)hi[0] << 32) | lo[0]) + __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+// sum over the 16 lanes of a row, in every lane of it
+__device__ __forceinline__ double row_sum16_f64(double v) {
+    v += dpp_partner_f64<0x140>(v);
+    v += dpp_partner_f64<0x141>(v);
+    v += dpp_partner_f64<0x4E>(v);
+    v += dpp_partner_f64<0xB1>(v);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v = row_sum16_f64(v);
+    v = swap16_sum_f64(v, v);
+    return swap32_sum_f64(v, v);
+}
+
+// Sum of 32 values over the 64 lanes of a wave: each halving step S keeps
+// half of the live values (the half selected by lane bit S) and adds the
+// partner lane's copy of the same half, so after S = 16..1 lane l holds the
+// partial of value (l & 31); a last step adds the other half-wave.
+template <int S, int CTRL>
+__device__ __forceinline__ void halve_step_dpp(float (&v)[32], bool hi) {
 #pragma unroll
     for (int i = 0; i < S; i++) {
-        const float send = hi ? v[i] : v[i + S];
-        const float keep = hi ? v[i + S] : v[i];
-        v[i] = keep + __shfl_xor(send, S, 64);
+        const float a = v[i] + dpp_partner<CTRL>(v[i]);
+        const float b = v[i + S] + dpp_partner<CTRL>(v[i + S]);
+        v[i] = hi ? b : a;
     }
 }
 
 __device__ __forceinline__ float wave_sum32(float (&v)[32]) {
     const int lane = threadIdx.x & 63;
-    halve_step<16>(v, (lane & 16) != 0);
-    halve_step<8>(v, (lane & 8) != 0);
-    halve_step<4>(v, (lane & 4) != 0);
-    halve_step<2>(v, (lane & 2) != 0);
-    halve_step<1>(v, (lane & 1) != 0);
-    return v[0] + __shfl_xor(v[0], 32, 64);
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = swap16_sum(v[i], v[i + 16]);
+    halve_step_dpp<8, kDppMirror>(v, (lane & 8) != 0);
+    halve_step_dpp<4, kDppHalfMirror>(v, (lane & 4) != 0);
+    halve_step_dpp<2, kDppXor2>(v, (lane & 2) != 0);
+    halve_step_dpp<1, kDppXor1>(v, (lane & 1) != 0);
+    return swap32_sum(v[0]);
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
-    return v;
+    v += dpp_partner<kDppXor1>(v);
+    v += dpp_partner<kDppXor2>(v);
+    v += dpp_partner<kDppHalfMirror>(v);
+    v += dpp_partner<kDppMirror>(v);
+    v = swap16_sum(v, v);
+    return swap32_sum(v);
 }
 
 // Decode Gram block b of a frame with Rext rows into (ra, rb), ra <= rb.
@@ -819,11 +881,9 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
                 en += w.fen[k];
             }
         }
-        for (int o = 32; o >= 1; o >>= 1) {
-            hff += __shfl_xor(hff, o, 64);
-            vf += __shfl_xor(vf, o, 64);
-            en += __shfl_xor(en, o, 64);
-        }
+        hff = wave_sum_f64(hff);
+        vf = wave_sum_f64(vf);
+        en = wave_sum_f64(en);
         if (tid == 0) {
             if (p.intr) {
                 w.A[(int64_t)n6 * ld + n6] = hff + (double)a.iep + (double)a.ilm * hff;
@@ -1317,20 +1377,16 @@ __device__ __forceinline__ void back_fetch(BackOps& o, const double* __restrict_
     }
 }
 
-// Sum of 4 per-lane values over the wave with 7 exchanges: the xor-32 step
-// keeps rows {0,1} or {2,3}, the xor-16 step one of those, and the last four
-// steps sum within groups of 16 lanes.  Lane l ends with the sum of row
-// back_row(l).
-__device__ __forceinline__ double back_reduce4(const double (&s)[4], int lane) {
-    const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
-    const double k0 = h32 ? s[2] : s[0], k1 = h32 ? s[3] : s[1];
-    const double g0 = h32 ? s[0] : s[2], g1 = h32 ? s[1] : s[3];
-    const double a0 = k0 + __shfl_xor(g0, 32, 64);
-    const double a1 = k1 + __shfl_xor(g1, 32, 64);
-    double v = (h16 ? a1 : a0) + __shfl_xor(h16 ? a0 : a1, 16, 64);
-#pragma unroll
-    for (int sh = 8; sh >= 1; sh >>= 1) v += __shfl_xor(v, sh, 64);
-    return v;
+// Sum of 4 per-lane values over the wave: a half-wave swap keeps rows {0,1}
+// or {2,3}, a 16-lane-row swap one of those, and four DPP steps sum within the
+// 16-lane row.  Lane l ends with the sum of row back_row(l).
+__device__ __forceinline__ double back_reduce4(const double (&s)[4]) {
+    // after the half-wave swap the lower half holds rows 0 / 1 and the upper
+    // half rows 2 / 3; after the row swap, rows 0, 2 of the wave hold the
+    // first of those and rows 1, 3 the second
+    const double a0 = swap32_sum_f64(s[0], s[2]);
+    const double a1 = swap32_sum_f64(s[1], s[3]);
+    return row_sum16_f64(swap16_sum_f64(a0, a1));
 }
 
 __global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __restrict__ A,
@@ -1385,7 +1441,7 @@ __global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __res
         }
         if (kb > 0) back_fetch(o, A, LinvT, y, ld, nb, kb - 1, tid);
         if (!last) {
-            const double r = back_reduce4(s, lane);
+            const double r = back_reduce4(s);
             if ((lane & 15) == 0) t[wv + kBW * back_row(lane)] = yc - r;
         }
         __syncthreads();
@@ -1762,9 +1818,7 @@ __global__ __launch_bounds__(1024) void chol_solve_full_kernel(const double* __r
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            double v = s[u];
-            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-            s[u] = v;
+            s[u] = wave_sum_f64(s[u]);
         }
         if (lane == 0) {
 #pragma unroll
