@@ -48,7 +48,7 @@ constexpr int kNV = 40;       // per-edge partial stride (floats)
 constexpr int kEDS = 136;     // per-edge finalised record (doubles)
 constexpr int kCB = 64;       // Cholesky block size
 constexpr int kMaxLd = 2048;  // leading dimension of the reduced system (6P+1 rounded up to kCB)
-constexpr int kGramSplit = 4; // finalize workgroups per depth frame
+constexpr int kGramSplit = 8; // finalize workgroups per depth frame
 constexpr int kLdsBudget = 64 * 1024;  // dynamic E-row tile; with ~15 KB static, two frame workgroups fit a CU
 constexpr int kLdsMax = 128 * 1024;    // largest dynamic E-row tile a caller may ask for (one workgroup per CU)
 
@@ -708,11 +708,12 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
         if (sl < kSl) {
             double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
             for (int t0 = sl; t0 < p.T; t0 += 8 * kSl) {
+                float x[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int t = t0 + u * kSl;
-                    if (t < p.T) acc[u] += (double)w.edge_part[((int64_t)t * p.E + e) * kNV + v];
-                }
+                for (int u = 0; u < 8; u++)
+                    x[u] = w.edge_part[((int64_t)min(t0 + u * kSl, p.T - 1) * p.E + e) * kNV + v];
+#pragma unroll
+                for (int u = 0; u < 8; u++) acc[u] += (t0 + u * kSl < p.T) ? (double)x[u] : 0.0;
             }
             tpart[sl][v] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         } else if (tid == kThreads - 1) {
@@ -793,18 +794,28 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
     const int per = (nv + kGramSplit - 1) / kGramSplit;
     const int i0 = part * per, i1 = min(nv, i0 + per);
     for (int i = i0 + tid; i < i1; i += kThreads) {
-        // 8 tiles' loads in flight per round, 8 partial sums
+        // the loads of 32 tiles are issued together (a tile past T reads the
+        // last tile and is selected away), 8 partial sums
+        const float* g = w.gram_part + off + i;
         double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int t0 = 0; t0 < p.T; t0 += 8) {
+        for (int t0 = 0; t0 < p.T; t0 += 32) {
+            float v[32];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (t0 + u < p.T) acc[u] += (double)w.gram_part[(int64_t)(t0 + u) * p.GT + off + i];
+            for (int u = 0; u < 32; u++) v[u] = g[(int64_t)min(t0 + u, p.T - 1) * p.GT];
+#pragma unroll
+            for (int u = 0; u < 32; u++) acc[u & 7] += (t0 + u < p.T) ? (double)v[u] : 0.0;
         }
         w.gd[off + i] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     }
     if (part == 0 && tid == 0) {
         double acc = 0.0;
-        for (int t = 0; t < p.T; t++) acc += (double)w.fen_part[(int64_t)t * p.K + k];
+        for (int t0 = 0; t0 < p.T; t0 += 32) {
+            float v[32];
+#pragma unroll
+            for (int u = 0; u < 32; u++) v[u] = w.fen_part[(int64_t)min(t0 + u, p.T - 1) * p.K + k];
+#pragma unroll
+            for (int u = 0; u < 32; u++) acc += (t0 + u < p.T) ? (double)v[u] : 0.0;
+        }
         w.fen[k] = acc;
     }
 }
