@@ -823,8 +823,21 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
 // ---------------------------------------------------------------------------
 // 3. assemble the dense lower triangle of H - S (+ intrinsics) and b - v
 // ---------------------------------------------------------------------------
-// Signed contribution of one assembly-list item to entry (i, j) of its block,
-// branch-free so that the loads of several items can be in flight together.
+// Address and sign of one assembly-list item's contribution to entry (i, j) of
+// its block (sign 0: no contribution; the address is then still a valid one),
+// computed without branches so that the loads of several items can be in
+// flight together.
+__device__ __forceinline__ const double* assembly_item(const Work& w, int item, int i, int j, int motion,
+                                                      double& sign) {
+    const int kind = item >> 27, idx = item & ((1 << 27) - 1);
+    const bool tr = (kind == K_HIJT) || (kind == K_GT);
+    const int ij = tr ? j * 6 + i : i * 6 + j;
+    const bool gram = (kind == K_G) || (kind == K_GT);
+    const int eoff = (kind == K_HII) ? E_HII : (kind == K_HJJ) ? E_HJJ : E_HIJ;
+    sign = gram ? (motion ? 0.0 : -1.0) : (kind <= K_HIJT ? 1.0 : 0.0);
+    return gram ? w.gd + idx + ij : w.ed + (int64_t)idx * kEDS + eoff + ij;
+}
+
 __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, Work w, int motion) {
     const int tid = threadIdx.x;
     const int bid = blockIdx.x;
@@ -834,21 +847,23 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
         if (tid >= 36) return;
         const int ab = p.blk_ab[2 * bid], bb = p.blk_ab[2 * bid + 1];
         const int i = tid / 6, j = tid % 6;
+        // items in list order, 8 per round: their indices, then their values,
+        // are loaded together.  Schur terms exist only when the depth system is
+        // solved; in motion-only mode the Gram buffer is never written.
         double s = 0.0;
-        for (int it = p.blk_ptr[bid]; it < p.blk_ptr[bid + 1]; it++) {
-            const int item = p.blk_items[it];
-            const int kind = item >> 27, idx = item & ((1 << 27) - 1);
-            switch (kind) {
-                case K_HII: s += w.ed[(int64_t)idx * kEDS + E_HII + i * 6 + j]; break;
-                case K_HJJ: s += w.ed[(int64_t)idx * kEDS + E_HJJ + i * 6 + j]; break;
-                case K_HIJ: s += w.ed[(int64_t)idx * kEDS + E_HIJ + i * 6 + j]; break;
-                case K_HIJT: s += w.ed[(int64_t)idx * kEDS + E_HIJ + j * 6 + i]; break;
-                // Schur terms exist only when the depth system is solved; in motion-only
-                // mode the Gram buffer is never written.
-                case K_G: if (!motion) s -= w.gd[idx + i * 6 + j]; break;
-                case K_GT: if (!motion) s -= w.gd[idx + j * 6 + i]; break;
-                default: break;
+        const int beg = p.blk_ptr[bid], end = p.blk_ptr[bid + 1];
+        for (int it0 = beg; it0 < end; it0 += 8) {
+            int item[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) item[u] = p.blk_items[min(it0 + u, end - 1)];
+            double val[8], sg[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                val[u] = *assembly_item(w, item[u], i, j, motion, sg[u]);
+                if (it0 + u >= end) sg[u] = 0.0;
             }
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += (sg[u] != 0.0) ? sg[u] * val[u] : 0.0;
         }
         if (ab == bb && i == j) s += (double)a.ep + (double)a.lm * s;
         w.A[(int64_t)(6 * ab + i) * ld + 6 * bb + j] = s;
@@ -858,18 +873,27 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
     if (r < p.P) {
         if (tid >= 6) return;
         double sb = 0.0, sf = 0.0;
-        for (int it = p.rhs_ptr[r]; it < p.rhs_ptr[r + 1]; it++) {
-            const int item = p.rhs_items[it];
-            const int kind = item >> 27, idx = item & ((1 << 27) - 1);
-            if (kind == K_VI) {
-                sb += w.ed[(int64_t)idx * kEDS + E_VI + tid];
-                sf += w.ed[(int64_t)idx * kEDS + E_FI + tid];
-            } else if (kind == K_VJ) {
-                sb += w.ed[(int64_t)idx * kEDS + E_VJ + tid];
-                sf += w.ed[(int64_t)idx * kEDS + E_FJ + tid];
-            } else if (kind == K_ROW && !motion) {
-                sb -= w.gd[idx + tid * 6 + 0];
-                sf -= w.gd[idx + tid * 6 + 1];
+        const int beg = p.rhs_ptr[r], end = p.rhs_ptr[r + 1];
+        for (int it0 = beg; it0 < end; it0 += 8) {
+            int item[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) item[u] = p.rhs_items[min(it0 + u, end - 1)];
+            double vb[8], vf[8], sg[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int kind = item[u] >> 27, idx = item[u] & ((1 << 27) - 1);
+                const bool row = (kind == K_ROW);
+                const double* pb = row ? w.gd + idx + tid * 6 : w.ed + (int64_t)idx * kEDS + (kind == K_VI ? E_VI : E_VJ) + tid;
+                const double* pf = row ? w.gd + idx + tid * 6 + 1 : w.ed + (int64_t)idx * kEDS + (kind == K_VI ? E_FI : E_FJ) + tid;
+                vb[u] = *pb;
+                vf[u] = *pf;
+                sg[u] = row ? (motion ? 0.0 : -1.0) : ((kind == K_VI || kind == K_VJ) ? 1.0 : 0.0);
+                if (it0 + u >= end) sg[u] = 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                sb += (sg[u] != 0.0) ? sg[u] * vb[u] : 0.0;
+                sf += (sg[u] != 0.0) ? sg[u] * vf[u] : 0.0;
             }
         }
         w.b[6 * r + tid] = sb;
