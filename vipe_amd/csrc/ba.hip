@@ -953,17 +953,12 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
 // Entries above the diagonal of A are scratch: the assembly never writes a
 // result there, and only the stored L_ik^T blocks are read back.
 
-// 1/sqrt(d): hardware estimate (about 2^-24) and two Newton steps (within
-// about 1 ulp for normal positive d; zero, negative and NaN pivots are flagged
-// by the caller).
+// 1/sqrt(d): hardware estimate and one Newton step, which squares its
+// relative error (zero, negative and NaN pivots are flagged by the caller).
 __device__ __forceinline__ double rsqrt_nr(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-#pragma unroll
-    for (int it = 0; it < 2; it++) {
-        const double e = fma(-d * y, y, 1.0);  // 1 - d y^2
-        y = fma(0.5 * y, e, y);
-    }
-    return y;
+    const double y = __builtin_amdgcn_rsq(d);
+    const double e = fma(-d * y, y, 1.0);  // 1 - d y^2
+    return fma(0.5 * y, e, y);
 }
 
 constexpr int kSB = 16;  // sub-block of the diagonal-block inverse
