@@ -1553,29 +1553,50 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
         }
         for (int n = 0; n < 6; n++) sU[tid][n] = (float)u[n];
     }
-    __syncthreads();
-    const double df = INTR ? w.x[6 * p.P] : 0.0;
+    // this pixel's own inputs and the first two edges' are requested before the
+    // barrier, so their latency overlaps the edge set-up above
     const int pix = blockIdx.x * kThreads + tid;
-    if (pix >= HW) return;
+    const bool valid = pix < HW;
+    const int64_t dpos = (int64_t)kf * HW + pix;
+    const float d = valid ? a.disps[dpos] : 0.f;
+    const float Q = valid ? w.Qb[(int64_t)k * HW + pix] : 0.f;
+    const float wt = valid ? w.Wb[(int64_t)k * HW + pix] : 0.f;
+    const bool rob = a.rk.type != 0;
+    // weights (u, v) and, with a robust kernel, targets (u, v) of edge slot s
+    auto edge_in = [&](int s, float4& o) {
+        if (valid && s < deg) {
+            const int64_t eb = (int64_t)p.fr_edges[e0 + s] * 2 * HW + pix;
+            o.x = a.weights[eb];
+            o.y = a.weights[eb + HW];
+            if (rob) {
+                o.z = a.targets[eb];
+                o.w = a.targets[eb + HW];
+            }
+        }
+    };
+    float4 in1 = make_float4(0.f, 0.f, 0.f, 0.f), in2 = in1;
+    edge_in(0, in1);
+    edge_in(1, in2);
+    __syncthreads();
+    if (!valid) return;
+    const double df = INTR ? w.x[6 * p.P] : 0.0;
     const int py = pix / p.W, px = pix - py * p.W;
     const float X0 = ((float)px - cx) / fx;
     const float Y0 = ((float)py - cy) / fy;
-    const int64_t dpos = (int64_t)kf * HW + pix;
-    const float d = a.disps[dpos];
     float dot = 0.f, Ef = 0.f;
     for (int s = 0; s < deg; s++) {
-        const int e = p.fr_edges[e0 + s];
-        const int64_t eb = (int64_t)e * 2 * HW + pix;
+        const float4 in = in1;
+        in1 = in2;
+        edge_in(s + 2, in2);
         PixTerms T;
-        if (a.rk.type) {
+        if (rob) {
             // the frame kernel's robust weights: residuals at the same linearisation point
-            pixel_terms<INTR, true>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb],
-                                    a.weights[eb + HW], a.targets[eb], a.targets[eb + HW], a.iscale, T);
+            pixel_terms<INTR, true>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale,
+                                    T);
             T.wu *= robust_weight(T.ru, a.rk);
             T.wv *= robust_weight(T.rv, a.rk);
         } else {
-            pixel_terms<INTR, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb],
-                                     a.weights[eb + HW], 0.f, 0.f, a.iscale, T);
+            pixel_terms<INTR, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, 0.f, 0.f, a.iscale, T);
         }
         const float wdu = T.wu * act, wdv = T.wv * act;
         const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
@@ -1585,8 +1606,6 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
         dot += s6;
         if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
     }
-    const float Q = w.Qb[(int64_t)k * HW + pix];
-    const float wt = w.Wb[(int64_t)k * HW + pix];
     float dz = Q * (wt - dot);
     if (INTR) dz = dz - Q * (Ef * act) * (float)df;
     a.dz[(int64_t)k * HW + pix] = dz;
