@@ -1222,6 +1222,23 @@ __device__ __forceinline__ void mfma_quad_64(const double (*P)[kCB + 1], const d
     }
 }
 
+// acc[cb] += P X over the 16-row strip w of the result and its four column
+// tiles cb, for an upper triangular X (X[m][c] = 0 for m > c): column tile cb
+// needs m < 16 (cb + 1) only, 40 of the 64 k-steps, the same for every strip.
+// Element r of acc[cb] in this lane is row 16 w + 4 r + lane / 16, column
+// 16 cb + lane % 16.
+__device__ __forceinline__ void mfma_strip_upper(const double (*P)[kCB + 1], const double (*X)[kCB + 1], int w,
+                                                 int lane, f64x4 (&acc)[4]) {
+    const int i = lane & 15, k = lane >> 4;
+#pragma unroll
+    for (int m0 = 0; m0 < kCB; m0 += 4) {
+        const double a = P[16 * w + i][m0 + k];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++)
+            if (m0 < 16 * (cb + 1)) acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, X[m0 + k][16 * cb + i], acc[cb], 0, 0, 0);
+    }
+}
+
 __global__ __launch_bounds__(256) void chol_factor_kernel(const double* __restrict__ A, double* __restrict__ LinvT,
                                                           int ld, int kb, int* flag) {
     __shared__ double S[kCB][kCB + 1];
@@ -1282,13 +1299,11 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
                     cur[ta][tb][r] = Ab[(int64_t)quad_row(qr, ta, r, lane) * ld + quad_col(qc, tb, lane)];
     }
     __syncthreads();
-    f64x4 li[2][2], lj[2][2];
+    f64x4 li[4], lj[4];
 #pragma unroll
-    for (int ta = 0; ta < 2; ta++)
-#pragma unroll
-        for (int tb = 0; tb < 2; tb++) li[ta][tb] = lj[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
-    mfma_quad_64<false>(R1, R0, qr, qc, lane, li);  // L_ik = A_ik X_k
-    if (!diag) mfma_quad_64<false>(R2, R0, qr, qc, lane, lj);
+    for (int cb = 0; cb < 4; cb++) li[cb] = lj[cb] = f64x4{0.0, 0.0, 0.0, 0.0};
+    mfma_strip_upper(R1, R0, wv, lane, li);  // L_ik = A_ik X_k
+    if (!diag) mfma_strip_upper(R2, R0, wv, lane, lj);
     if (diag && tid < kCB) {
         double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1297,15 +1312,13 @@ __global__ __launch_bounds__(256) void chol_step_kernel(double* __restrict__ A, 
     }
     __syncthreads();
 #pragma unroll
-    for (int ta = 0; ta < 2; ta++)
+    for (int cb = 0; cb < 4; cb++)
 #pragma unroll
-        for (int tb = 0; tb < 2; tb++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int rr = quad_row(qr, ta, r, lane), cc = quad_col(qc, tb, lane);
-                R1[rr][cc] = li[ta][tb][r];
-                if (!diag) R2[rr][cc] = lj[ta][tb][r];
-            }
+        for (int r = 0; r < 4; r++) {
+            const int rr = 16 * wv + 4 * r + (lane >> 4), cc = 16 * cb + (lane & 15);
+            R1[rr][cc] = li[cb][r];
+            if (!diag) R2[rr][cc] = lj[cb][r];
+        }
     if (next && tid < kCB) y[c0 + tid] = yk[tid];
     __syncthreads();
     if (!diag) {
