@@ -239,3 +239,74 @@ def test_ba_implicit_adjoint_against_fp64_central_differences(motion_only):
         errs.append(abs(fd - an) / max(abs(fd), 1e-6))
     print(f"[BA adjoint motion_only={motion_only}] max rel directional error over 64 directions: {max(errs):.3e}")
     assert max(errs) < 2e-3
+
+
+def _high_degree_graph(seed=21):
+    """24 frames of 16 x 24 pixels.  Frame 1 has 20 outgoing edges (22 E rows with
+    its self row and the pseudo row: 132 rows, 9 tiles of 16, 45 tile pairs of the
+    Schur Gram), frame 8 has 13 (15 rows of E: 21 pairs), frame 5 has 9 (11: 15
+    pairs), frames 10-23 one to three each."""
+    from vipe_amd.synth import FrameGraph, _quat_exp
+
+    gen = torch.Generator().manual_seed(seed)
+    N, H, W, t0 = 24, 16, 24, 1
+    ii, jj = [], []
+
+    def add(i, js):
+        ii.extend([i] * len(js))
+        jj.extend(js)
+
+    add(1, list(range(2, 22)))
+    add(5, [0, 1, 2, 3, 4, 6, 7, 9, 12])
+    add(8, [0, 1, 2, 3, 4, 5, 6, 7, 9, 10, 11, 12, 13])
+    for i in range(10, N):
+        add(i, [(i + 1) % N, (i + 2) % N, (i + N - 3) % N][: 1 + i % 3])
+    ii, jj = torch.tensor(ii), torch.tensor(jj)
+    E = ii.shape[0]
+    tau = torch.randn(N, 3, generator=gen, dtype=torch.float64) * 0.1
+    phi = torch.randn(N, 3, generator=gen, dtype=torch.float64) * 0.05
+    poses = torch.cat([tau, _quat_exp(phi)], -1)
+    disps = 0.2 + 0.8 * torch.rand(N, H, W, generator=gen, dtype=torch.float64)
+    intr = torch.tensor([20.0, 20.0, 12.0, 8.0], dtype=torch.float64)
+    coords, _ = reproject(poses, disps, intr, ii, jj)
+    targets = (coords + torch.randn(E, H, W, 2, generator=gen, dtype=torch.float64) * 0.5).permute(0, 3, 1, 2)
+    weights = torch.rand(E, 2, H, W, generator=gen, dtype=torch.float64)
+    kx = torch.unique(torch.cat([torch.arange(t0, N), ii]))
+    eta = 0.2 * (0.05 + 0.1 * torch.rand(kx.shape[0], H, W, generator=gen, dtype=torch.float64)) + 1e-7
+    return FrameGraph(poses=poses.float().contiguous(), disps=disps.float().contiguous(),
+                      disps_sens=torch.zeros(N, H, W), intrinsics=intr.float(), ii=ii, jj=jj,
+                      targets=targets.float().contiguous(), weights=weights.float().contiguous(),
+                      eta=eta.float().contiguous(), fmaps=torch.zeros(0), coords=coords.float().contiguous(), t0=t0,
+                      t1=N)
+
+
+@pytest.mark.parametrize("tile", [0, 64, 128])
+def test_ba_high_degree_frames_match_oracle(tile):
+    """Frames with 9, 13 and 20 outgoing edges: the Schur Gram's larger tile-pair
+    counts per wave (up to 12) and the padding rows past 132 E rows, at every
+    pixel tile the plan allows (256 needs more LDS than 22 E rows may take)."""
+    from vipe_amd.ext import slam_ext
+
+    g = _high_degree_graph()
+    N, H, W = g.disps.shape
+    da = torch.ones(N)
+    ds = g.disps.clone() * 1.01
+    ds[:, ::2] = 0.0
+    iters, lm, ep, alpha = 2, 1e-4, 0.1, 0.05
+    d = _cuda(g)
+    p, dd = d["poses"].clone(), d["disps"].clone()
+    plan = slam_ext.get_plan(d["ii"], d["jj"], N, H, W, g.t0, g.t1, tile)
+    assert plan.pixel_tile == (tile or 64)
+    ws = torch.empty((plan.ws_bytes,), dtype=torch.uint8, device="cuda")
+    dx = torch.empty((g.t1 - g.t0, 6), device="cuda")
+    dz = torch.empty((plan.K, H * W), device="cuda")
+    slam_ext.run_planned(plan, ws, p, dd, d["intrinsics"], ds.cuda(), d["targets"], d["weights"], d["eta"],
+                         da.cuda(), iters, lm, ep, alpha, dx, dz)
+    rp, rd = g.poses.clone(), g.disps.clone()
+    rdx, rdz, _ = OB.ba_extended(rp, rd, g.intrinsics.clone(), ds, g.targets, g.weights, g.eta, g.ii, g.jj, da,
+                                 g.t0, g.t1, iters, lm, ep, False, alpha, False, 1e-6, 1e-6, 1.0, False)
+    torch.cuda.synchronize()
+    assert (dx.cpu().double() - rdx).abs().max().item() < 1e-5 * max(1.0, rdx.abs().max().item())
+    assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
+    assert (dd.cpu().double() - rd.double()).abs().max().item() < 1e-4
+    assert _reproj_err(g, p.cpu(), dd.cpu(), rp, rd, g.intrinsics) < 1e-3

@@ -117,8 +117,8 @@ def test_tiled_plan_rejects_tile_over_lds():
     from vipe_amd.ext import slam_ext
 
     # frame 1 (inside the window [1, 24)) with 20 outgoing edges: 22 E rows
-    # (self, 20 edges, pseudo row); 256-pixel tiles would need 22 * 6 * 257 * 4 B
-    # = 135696 B of LDS, over the 131072 B a caller may ask for
+    # (self, 20 edges, pseudo row); 256-pixel tiles would need 22 * 6 * 260 * 4 B
+    # = 137280 B of LDS (rows of 256 + 4 floats), over the 131072 B a caller may ask for
     N = 24
     ii = torch.ones(20, dtype=torch.long)
     jj = torch.arange(2, 22)
