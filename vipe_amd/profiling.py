@@ -5,6 +5,11 @@
         ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE``, it gives the factor between
         the counters' KiB and real bytes on this GPU.
 
+    python3 -m vipe_amd.profiling stream [--mib 1024] [--iters 10]
+        What plain streaming reaches on this GPU: torch's read-only sum and its
+        device copy (read + write) of an FP32 buffer, timed with HIP events; the
+        reference points for the HBM figures of the kernels.
+
     python3 -m vipe_amd.profiling build [--contraction f32|f16|bf16|fp8]
         Two correlation-pyramid builds of the bench workload (for counters).
 
@@ -43,6 +48,26 @@ def _copy(mib: int) -> None:
         b.copy_(a)
     torch.cuda.synchronize()
     print(f"copy: {n * 4} bytes read and {n * 4} bytes written per launch", flush=True)
+
+
+def _stream(mib: int, iters: int) -> None:
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for name, fn, nbytes in (("read (sum)", lambda: a.sum(), n * 4), ("copy (read + write)", lambda: b.copy_(a),
+                                                                    2 * n * 4)):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(iters):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        s = e0.elapsed_time(e1) * 1e-3 / iters
+        print(f"{name}: {nbytes / s / 1e12:.2f} TB/s ({nbytes} bytes in {s * 1e6:.1f} us)", flush=True)
 
 
 def _build(contraction: str, workload: str) -> None:
@@ -189,6 +214,9 @@ def main(argv=None) -> None:
     sub = ap.add_subparsers(dest="cmd", required=True)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
+    st = sub.add_parser("stream")
+    st.add_argument("--mib", type=int, default=1024)
+    st.add_argument("--iters", type=int, default=10)
     bl = sub.add_parser("build")
     bl.add_argument("--contraction", choices=["f32", "f16", "bf16", "fp8"], default="f32")
     bl.add_argument("--workload", default="kf64_e384_64x80")
@@ -208,6 +236,8 @@ def main(argv=None) -> None:
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         _copy(a.mib)
+    elif a.cmd == "stream":
+        _stream(a.mib, a.iters)
     elif a.cmd == "build":
         _build(a.contraction, a.workload)
     elif a.cmd == "spatial_corr":
