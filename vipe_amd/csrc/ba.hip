@@ -552,17 +552,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                 const float wdu = T.wu * act, wdv = T.wv * act;
                 const float pu = stereo ? 0.f : T.wu;
                 const float pv = stereo ? 0.f : T.wv;
-                // pose j-side terms
+                // pose j-side terms; Ju[1] and Jv[0] are structurally zero, so
+                // the products they enter are left out (no fast-math folds them)
                 int l = 0;
 #pragma unroll
                 for (int n = 0; n < 6; n++)
 #pragma unroll
                     for (int m = 0; m <= n; m++) {
-                        v[l] = pu * T.Ju[n] * T.Ju[m] + pv * T.Jv[n] * T.Jv[m];
+                        const float hu = (n != 1 && m != 1) ? pu * T.Ju[n] * T.Ju[m] : 0.f;
+                        const float hv = (n != 0 && m != 0) ? pv * T.Jv[n] * T.Jv[m] : 0.f;
+                        v[l] = (n != 1 && m != 1) ? ((n != 0 && m != 0) ? hu + hv : hu) : hv;
                         l++;
                     }
 #pragma unroll
-                for (int n = 0; n < 6; n++) v[21 + n] = pu * T.ru * T.Ju[n] + pv * T.rv * T.Jv[n];
+                for (int n = 0; n < 6; n++)
+                    v[21 + n] = n == 1 ? pv * T.rv * T.Jv[n] : n == 0 ? pu * T.ru * T.Ju[n]
+                                                                      : pu * T.ru * T.Ju[n] + pv * T.rv * T.Jv[n];
                 v[27] = T.wu * T.ru * T.ru + T.wv * T.rv * T.rv;
                 if (INTR) {
                     v[28] = pu * T.Jfu * T.Jfu + pv * T.Jfv * T.Jfv;
@@ -576,7 +581,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                     if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
                     const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
 #pragma unroll
-                    for (int n = 0; n < 6; n++) er[n] = cu * T.Ju[n] + cv * T.Jv[n];
+                    for (int n = 0; n < 6; n++)
+                        er[n] = n == 1 ? cv * T.Jv[n] : n == 0 ? cu * T.Ju[n] : cu * T.Ju[n] + cv * T.Jv[n];
                     if (has_self) {
                         float o[6];
                         neg_adjT(Rm, t, er, o);
