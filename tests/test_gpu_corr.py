@@ -197,3 +197,30 @@ def test_corr_lookup_gradients_to_both_feature_maps():
         errs.append(abs(fd - an) / gv)
     print(f"[corr feature grads] max normalised directional error over 64 directions: {max(errs):.3e}")
     assert max(errs) < 1e-5
+
+
+@pytest.mark.parametrize("C", [128, 40])
+@pytest.mark.parametrize("dt", [torch.float32, torch.float16])
+def test_altcorr_index_vector_and_scalar_paths(C, dt):
+    """altcorr_index_forward with 32-channel chunks read as 16-byte vectors
+    (C = 128) and the per-element path (C = 40, not whole chunks), FP32 and FP16
+    features, against the FP64 oracle; some windows lie partly or wholly
+    outside the target image."""
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(5 + C)
+    Nf, M, H, W, r = 4, 5, 12, 14, 3
+    fmaps = (torch.randn(1, Nf, C, H, W, generator=g) / math.sqrt(C)).to(dt)
+    pyr = OC.altcorr_pyramid(fmaps.double(), 4)
+    pyr_t = [p.to(dt).contiguous() for p in pyr]
+    ii = torch.tensor([0, 1, 2, 3, 0])
+    jj = torch.tensor([1, 2, 3, 0, 2])
+    yy, xx = torch.meshgrid(torch.arange(H), torch.arange(W), indexing="ij")
+    cc = torch.stack([xx, yy], -1).float()[None, None].repeat(1, M, 1, 1, 1)
+    cc = cc + 1.5 * torch.randn(cc.shape, generator=g)
+    cc[0, 1, :3] += 30.0   # windows wholly outside
+    cc[0, 2, :, :2] -= 5.0  # windows across the left border
+    (out,) = D.altcorr_index_forward(pyr_t[0].cuda(), [p.cuda() for p in pyr_t], cc.cuda(), ii.cuda(), jj.cuda(), r)
+    ref = OC.altcorr_index_forward(pyr_t[0].double(), [p.double() for p in pyr_t], cc.double(), ii, jj, r)
+    assert out.dtype == torch.float32 and out.shape == ref.shape
+    assert (out.double().cpu() - ref).abs().max().item() < 2e-5 * max(1.0, ref.abs().max().item())

@@ -634,7 +634,99 @@ __device__ __forceinline__ void altcorr_pixel(const T* __restrict__ f1, const T*
         }
 }
 
+// Vector form of altcorr_pixel for C a multiple of kChunk, 16-byte aligned
+// rows and levels under 2 GiB.  f1 is read as 16-byte vectors; each tap's
+// chunk of f2 as 16-byte loads through a resource based at the level's first
+// byte (f2off: the byte offset of this lane's plane), and a tap outside the
+// plane gets an offset past the resource's range, which reads zeros without
+// a memory access.  FP32 features use FMA chains, FP16 v_dot2_f32_f16; both
+// accumulate in FP32.
+typedef unsigned int acu32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 acf16x2 __attribute__((ext_vector_type(2)));
+
+template <typename T> struct AltDot;
+template <> struct AltDot<float> {
+    static constexpr int NV = kChunk / 4;  // 16-byte vectors per chunk
+    static __device__ __forceinline__ float dot(const acu32x4 (&a)[NV], const acu32x4 (&b)[NV]) {
+        float p0 = 0.f, p1 = 0.f;
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            p0 = fmaf(__uint_as_float(a[q][0]), __uint_as_float(b[q][0]), p0);
+            p1 = fmaf(__uint_as_float(a[q][1]), __uint_as_float(b[q][1]), p1);
+            p0 = fmaf(__uint_as_float(a[q][2]), __uint_as_float(b[q][2]), p0);
+            p1 = fmaf(__uint_as_float(a[q][3]), __uint_as_float(b[q][3]), p1);
+        }
+        return p0 + p1;
+    }
+};
+template <> struct AltDot<__half> {
+    static constexpr int NV = kChunk / 8;
+    static __device__ __forceinline__ float dot(const acu32x4 (&a)[NV], const acu32x4 (&b)[NV]) {
+        float p0 = 0.f, p1 = 0.f;
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            // each element goes through a named scalar: a bit_cast of a vector
+            // subscript would read element 0 (clang takes the vector's address)
+#pragma unroll
+            for (int k = 0; k < 4; k += 2) {
+                const uint32_t a0 = a[q][k], b0 = b[q][k], a1 = a[q][k + 1], b1 = b[q][k + 1];
+                p0 = __builtin_amdgcn_fdot2(__builtin_bit_cast(acf16x2, a0), __builtin_bit_cast(acf16x2, b0), p0, false);
+                p1 = __builtin_amdgcn_fdot2(__builtin_bit_cast(acf16x2, a1), __builtin_bit_cast(acf16x2, b1), p1, false);
+            }
+        }
+        return p0 + p1;
+    }
+};
+
 template <typename T, int R>
+__device__ __forceinline__ void altcorr_pixel_vec(const T* __restrict__ f1, __amdgpu_buffer_rsrc_t f2r, uint32_t f2off,
+                                                  int H2, int W2, int C, float x, float y, float* __restrict__ o,
+                                                  int64_t ostride) {
+    constexpr int RD = 2 * R + 1;
+    constexpr int NT = RD + 1;
+    constexpr int NV = AltDot<T>::NV;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float dx = x - fx0, dy = y - fy0;
+    const int xb = (int)fx0 - R, yb = (int)fy0 - R;
+    const uint32_t rowb = (uint32_t)C * (uint32_t)sizeof(T);
+    float s[NT][NT];
+#pragma unroll
+    for (int a = 0; a < NT; a++)
+#pragma unroll
+        for (int b = 0; b < NT; b++) s[a][b] = 0.f;
+    for (int c0 = 0; c0 < C; c0 += kChunk) {
+        acu32x4 a1[NV];
+        const acu32x4* f1v = reinterpret_cast<const acu32x4*>(f1 + c0);
+#pragma unroll
+        for (int q = 0; q < NV; q++) a1[q] = f1v[q];
+        const uint32_t cb = f2off + (uint32_t)c0 * (uint32_t)sizeof(T);
+#pragma unroll
+        for (int iy = 0; iy < NT; iy++) {
+            const int h2 = yb + iy;
+            const bool yok = (unsigned)h2 < (unsigned)H2;
+#pragma unroll
+            for (int ix = 0; ix < NT; ix++) {
+                const int w2 = xb + ix;
+                const bool ok = yok && (unsigned)w2 < (unsigned)W2;
+                const uint32_t off = ok ? cb + (uint32_t)(h2 * W2 + w2) * rowb : kBufInvalid;
+                acu32x4 g[NV];
+#pragma unroll
+                for (int q = 0; q < NV; q++) g[q] = __builtin_amdgcn_raw_buffer_load_b128(f2r, off + 16u * q, 0, 0);
+                s[iy][ix] += AltDot<T>::dot(a1, g);
+            }
+        }
+    }
+#pragma unroll
+    for (int ix = 0; ix < RD; ix++)
+#pragma unroll
+        for (int iy = 0; iy < RD; iy++) {
+            const float v = s[iy][ix] * (1 - dy) * (1 - dx) + s[iy][ix + 1] * (1 - dy) * dx +
+                            s[iy + 1][ix] * dy * (1 - dx) + s[iy + 1][ix + 1] * dy * dx;
+            o[(ix * RD + iy) * ostride] = v;
+        }
+}
+
+template <typename T, int R, bool VEC>
 __global__ __launch_bounds__(128) void altcorr_fwd_kernel(const T* __restrict__ fmap1, const T* __restrict__ fmap2,
                                                           const float* __restrict__ coords, T* __restrict__ out,
                                                           int64_t B, int N, int H1, int W1, int H2, int W2, int C) {
@@ -648,14 +740,18 @@ __global__ __launch_bounds__(128) void altcorr_fwd_kernel(const T* __restrict__ 
     const float x = coords[gid * 2 + 0];
     const float y = coords[gid * 2 + 1];
     float tmp[RD * RD];
-    altcorr_pixel<T, R>(fmap1 + (b * HW1 + yx) * C, fmap2 + b * (int64_t)H2 * W2 * C, H2, W2, C, x, y, tmp, 1);
+    if constexpr (VEC)
+        altcorr_pixel_vec<T, R>(fmap1 + (b * HW1 + yx) * C, buf_rsrc(fmap2),
+                                (uint32_t)(b * (int64_t)H2 * W2 * C * (int64_t)sizeof(T)), H2, W2, C, x, y, tmp, 1);
+    else
+        altcorr_pixel<T, R>(fmap1 + (b * HW1 + yx) * C, fmap2 + b * (int64_t)H2 * W2 * C, H2, W2, C, x, y, tmp, 1);
     T* o = out + bn * RD * RD * HW1 + yx;
 #pragma unroll
     for (int c = 0; c < RD * RD; c++) o[c * HW1] = from_f<T>(tmp[c]);
 }
 
 // Indexed multi-level forward: one lane per (b, m, pixel, level).
-template <typename T, int R>
+template <typename T, int R, bool VEC>
 __global__ __launch_bounds__(128) void altcorr_index_kernel(const T* __restrict__ fmap1, const T* __restrict__ f2l,
                                                             const float* __restrict__ coords,
                                                             const int64_t* __restrict__ ii,
@@ -674,8 +770,14 @@ __global__ __launch_bounds__(128) void altcorr_index_kernel(const T* __restrict_
     const float scale = (float)(1 << level);
     const float x = coords[gid * 2 + 0] / scale;
     const float y = coords[gid * 2 + 1] / scale;
-    altcorr_pixel<T, R>(fmap1 + ((b * Nf + src) * HW + yx) * C, f2l + (b * Nf + dst) * (int64_t)H2 * W2 * C, H2, W2,
-                        C, x, y, out + (bm * L * RD * RD + level * RD * RD) * HW + yx, HW);
+    float* o = out + (bm * L * RD * RD + level * RD * RD) * HW + yx;
+    if constexpr (VEC)
+        altcorr_pixel_vec<T, R>(fmap1 + ((b * Nf + src) * HW + yx) * C, buf_rsrc(f2l),
+                                (uint32_t)((b * Nf + dst) * (int64_t)H2 * W2 * C * (int64_t)sizeof(T)), H2, W2, C, x, y,
+                                o, HW);
+    else
+        altcorr_pixel<T, R>(fmap1 + ((b * Nf + src) * HW + yx) * C, f2l + (b * Nf + dst) * (int64_t)H2 * W2 * C, H2,
+                            W2, C, x, y, o, HW);
 }
 
 // Backward (float only, as the reference): one lane per (b, n, pixel) and tap;
@@ -1366,6 +1468,22 @@ int vh_corr_build(int in_dtype, int mma_dtype, int out_dtype, int64_t E, int C, 
     return fail(VH_ERR_UNSUPPORTED, "corr_build: dtypes in=%d out=%d", in_dtype, out_dtype);
 }
 
+}  // extern "C"
+
+namespace vh {
+namespace {
+// Whether the vector altcorr path applies: channels in whole 32-element chunks,
+// 16-byte aligned feature rows, and a level whose bytes (all frames) stay under
+// the buffer resource's 31-bit offsets.
+bool altcorr_vec_ok(int C, int elem, const void* f1, const void* f2, int64_t level_bytes) {
+    return C % kChunk == 0 && (C * elem) % 16 == 0 && reinterpret_cast<uintptr_t>(f1) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(f2) % 16 == 0 && level_bytes < (int64_t)kBufRange - (1 << 20);
+}
+}  // namespace
+}  // namespace vh
+
+extern "C" {
+
 int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
                        const void* fmap2, const float* coords, int radius, void* corr, void* stream) {
     if (radius != 3) return fail(VH_ERR_UNSUPPORTED, "altcorr_forward: radius %d (kernels built for r=3)", radius);
@@ -1373,14 +1491,24 @@ int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int 
     if (n == 0) return VH_OK;
     hipStream_t s = (hipStream_t)stream;
     dim3 grid(ceil_div(n, 128));
+    const int elem = dtype == VH_F16 ? 2 : 4;
+    const bool vec = altcorr_vec_ok(C, elem, fmap1, fmap2, B * (int64_t)H2 * W2 * C * elem);
     switch (dtype) {
         case VH_F16:
-            hipLaunchKernelGGL((altcorr_fwd_kernel<__half, 3>), grid, dim3(128), 0, s, (const __half*)fmap1,
-                               (const __half*)fmap2, coords, (__half*)corr, B, N, H1, W1, H2, W2, C);
+            if (vec)
+                hipLaunchKernelGGL((altcorr_fwd_kernel<__half, 3, true>), grid, dim3(128), 0, s, (const __half*)fmap1,
+                                   (const __half*)fmap2, coords, (__half*)corr, B, N, H1, W1, H2, W2, C);
+            else
+                hipLaunchKernelGGL((altcorr_fwd_kernel<__half, 3, false>), grid, dim3(128), 0, s, (const __half*)fmap1,
+                                   (const __half*)fmap2, coords, (__half*)corr, B, N, H1, W1, H2, W2, C);
             break;
         case VH_F32:
-            hipLaunchKernelGGL((altcorr_fwd_kernel<float, 3>), grid, dim3(128), 0, s, (const float*)fmap1,
-                               (const float*)fmap2, coords, (float*)corr, B, N, H1, W1, H2, W2, C);
+            if (vec)
+                hipLaunchKernelGGL((altcorr_fwd_kernel<float, 3, true>), grid, dim3(128), 0, s, (const float*)fmap1,
+                                   (const float*)fmap2, coords, (float*)corr, B, N, H1, W1, H2, W2, C);
+            else
+                hipLaunchKernelGGL((altcorr_fwd_kernel<float, 3, false>), grid, dim3(128), 0, s, (const float*)fmap1,
+                                   (const float*)fmap2, coords, (float*)corr, B, N, H1, W1, H2, W2, C);
             break;
         default: return fail(VH_ERR_UNSUPPORTED, "altcorr_forward: dtype %d", dtype);
     }
@@ -1410,18 +1538,30 @@ int vh_altcorr_index_forward(int dtype, int64_t B, int Nf, int M, int H, int W, 
     const int64_t n = B * M * (int64_t)H * W;
     if (n == 0) return VH_OK;
     hipStream_t s = (hipStream_t)stream;
+    const int elem = dtype == VH_F16 ? 2 : 4;
     for (int l = 0; l < num_levels; l++) {
         dim3 grid(ceil_div(n, 128));
+        const bool vec = altcorr_vec_ok(C, elem, fmap1, levels[l], B * (int64_t)Nf * Hl[l] * Wl[l] * C * elem);
         switch (dtype) {
             case VH_F16:
-                hipLaunchKernelGGL((altcorr_index_kernel<__half, 3>), grid, dim3(128), 0, s, (const __half*)fmap1,
-                                   (const __half*)levels[l], coords, ii, jj, (float*)corr, B, Nf, M, H, W, C, Hl[l],
-                                   Wl[l], l, num_levels);
+                if (vec)
+                    hipLaunchKernelGGL((altcorr_index_kernel<__half, 3, true>), grid, dim3(128), 0, s,
+                                       (const __half*)fmap1, (const __half*)levels[l], coords, ii, jj, (float*)corr, B,
+                                       Nf, M, H, W, C, Hl[l], Wl[l], l, num_levels);
+                else
+                    hipLaunchKernelGGL((altcorr_index_kernel<__half, 3, false>), grid, dim3(128), 0, s,
+                                       (const __half*)fmap1, (const __half*)levels[l], coords, ii, jj, (float*)corr, B,
+                                       Nf, M, H, W, C, Hl[l], Wl[l], l, num_levels);
                 break;
             case VH_F32:
-                hipLaunchKernelGGL((altcorr_index_kernel<float, 3>), grid, dim3(128), 0, s, (const float*)fmap1,
-                                   (const float*)levels[l], coords, ii, jj, (float*)corr, B, Nf, M, H, W, C, Hl[l],
-                                   Wl[l], l, num_levels);
+                if (vec)
+                    hipLaunchKernelGGL((altcorr_index_kernel<float, 3, true>), grid, dim3(128), 0, s,
+                                       (const float*)fmap1, (const float*)levels[l], coords, ii, jj, (float*)corr, B,
+                                       Nf, M, H, W, C, Hl[l], Wl[l], l, num_levels);
+                else
+                    hipLaunchKernelGGL((altcorr_index_kernel<float, 3, false>), grid, dim3(128), 0, s,
+                                       (const float*)fmap1, (const float*)levels[l], coords, ii, jj, (float*)corr, B,
+                                       Nf, M, H, W, C, Hl[l], Wl[l], l, num_levels);
                 break;
             default: return fail(VH_ERR_UNSUPPORTED, "altcorr_index_forward: dtype %d", dtype);
         }

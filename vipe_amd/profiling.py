@@ -21,6 +21,11 @@
         scatter_sum and scatter_max of [1, 384, 128 * 48 * 64] sources into 64
         slots along dim 1 by a 1-D index (the per-frame aggregation shape).
 
+    python3 -m vipe_amd.profiling altcorr [--dtype f32|f16] [--iters 5] [--workload NAME]
+        AltCorrBlock lookups (on-the-fly correlation, 4 levels, r = 3) of the
+        bench workload's edges over its feature pyramid, one call per iteration
+        after a warm-up.
+
     python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
         --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
         Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
@@ -155,6 +160,28 @@ def _scatter(dtype: str, iters: int) -> None:
     print(f"scatter: {iters} x (sum, max), src [1, 384, 393216] {dtype} -> 64 slots", flush=True)
 
 
+def _altcorr(dtype: str, iters: int, workload: str) -> None:
+    import torch
+
+    from . import synth
+    from .corr import AltCorrBlock
+
+    g = synth.make_graph(workload, seed=0)
+    dt = getattr(torch, _DT[dtype])
+    fm = g.fmaps.cuda().to(dt)[None]  # [1, N, C, H, W]
+    ii, jj = g.ii.cuda(), g.jj.cuda()
+    coords = g.coords.cuda()[None]  # [1, E, H, W, 2]
+    blk = AltCorrBlock(fm, num_levels=4, radius=3)
+    out = blk(coords, ii, jj)
+    torch.cuda.synchronize()
+    for _ in range(iters):
+        out = blk(coords, ii, jj)
+    torch.cuda.synchronize()
+    E, H, W = coords.shape[1:4]
+    print(f"altcorr: {iters} x lookup of {E} edges at {H}x{W}, C={fm.shape[2]} {dtype} -> {tuple(out.shape)}",
+          flush=True)
+
+
 def _short(name: str, width: int = 90) -> str:
     return name if len(name) <= width else name[: width - 3] + "..."
 
@@ -224,6 +251,10 @@ def main(argv=None) -> None:
         w = sub.add_parser(name)
         w.add_argument("--dtype", choices=list(_DT), default="f32")
         w.add_argument("--iters", type=int, default=5)
+    ac = sub.add_parser("altcorr")
+    ac.add_argument("--dtype", choices=["f32", "f16"], default="f32")
+    ac.add_argument("--iters", type=int, default=5)
+    ac.add_argument("--workload", default="kf64_e384_64x80")
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch")
@@ -244,6 +275,8 @@ def main(argv=None) -> None:
         _spatial_corr(a.dtype, a.iters)
     elif a.cmd == "scatter":
         _scatter(a.dtype, a.iters)
+    elif a.cmd == "altcorr":
+        _altcorr(a.dtype, a.iters, a.workload)
     else:
         _summarize(a)
 
