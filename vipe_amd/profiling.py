@@ -26,6 +26,11 @@
         bench workload's edges over its feature pyramid, one call per iteration
         after a warm-up.
 
+    python3 -m vipe_amd.profiling altcorr_layer [--iters 3]
+        CorrLayer (altcorr forward and backward, one level, r = 3) on 16 pairs of
+        128-channel 64x80 FP32 feature maps, coordinates the identity plus
+        N(0, 2 px) noise.
+
     python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
         --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
         Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
@@ -182,6 +187,26 @@ def _altcorr(dtype: str, iters: int, workload: str) -> None:
           flush=True)
 
 
+def _altcorr_layer(iters: int) -> None:
+    import torch
+
+    from .corr import CorrLayer
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    B, H, W, C = 16, 64, 80, 128
+    f1 = (torch.randn(B, H, W, C, device="cuda", generator=g) / C ** 0.5).requires_grad_()
+    f2 = (torch.randn(B, H, W, C, device="cuda", generator=g) / C ** 0.5).requires_grad_()
+    yy, xx = torch.meshgrid(torch.arange(H, device="cuda"), torch.arange(W, device="cuda"), indexing="ij")
+    coords = torch.stack([xx, yy], -1).float()[None, None].repeat(B, 1, 1, 1, 1)
+    coords = coords + 2.0 * torch.randn(coords.shape, device="cuda", generator=g)
+    for _ in range(iters + 1):
+        out = CorrLayer.apply(f1, f2, coords, 3)
+        out.sum().backward()
+    torch.cuda.synchronize()
+    print(f"altcorr_layer: {iters + 1} x forward + backward, [{B}, {H}, {W}, {C}] f32 -> {tuple(out.shape)}",
+          flush=True)
+
+
 def _short(name: str, width: int = 90) -> str:
     return name if len(name) <= width else name[: width - 3] + "..."
 
@@ -255,6 +280,8 @@ def main(argv=None) -> None:
     ac.add_argument("--dtype", choices=["f32", "f16"], default="f32")
     ac.add_argument("--iters", type=int, default=5)
     ac.add_argument("--workload", default="kf64_e384_64x80")
+    al = sub.add_parser("altcorr_layer")
+    al.add_argument("--iters", type=int, default=3)
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch")
@@ -277,6 +304,8 @@ def main(argv=None) -> None:
         _scatter(a.dtype, a.iters)
     elif a.cmd == "altcorr":
         _altcorr(a.dtype, a.iters, a.workload)
+    elif a.cmd == "altcorr_layer":
+        _altcorr_layer(a.iters)
     else:
         _summarize(a)
 
