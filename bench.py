@@ -80,8 +80,8 @@ def profiled_traffic(kernel: str):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(the multi-GPU sharding of the path is a design note only).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(how the path would shard over several GPUs is described, not implemented).", file=sys.stderr)
         sys.exit(2)
     from vipe_amd import synth
     from vipe_amd.corr import CorrBlock
