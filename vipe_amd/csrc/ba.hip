@@ -257,6 +257,9 @@ __device__ __forceinline__ void neg_adjT(const float* R, const float* t, const f
 //   S = 4, 8: DPP row_half_mirror, row_mirror
 //   S = 16:   v_permlane16_swap (odd rows of one register <-> even rows of the other)
 //   S = 32:   v_permlane32_swap (upper half of one register <-> lower half of the other)
+// Every lane of the wave must be active at these calls (an inactive lane's
+// register would be read as a partner's value); the callers reach them with
+// the whole wave, masking their contributions by value instead.
 template <int CTRL>
 __device__ __forceinline__ float dpp_partner(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
