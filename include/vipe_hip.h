@@ -133,7 +133,9 @@ int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int 
                        const void* fmap2, const float* coords, int radius, void* corr, void* stream);
 /* altcorr_backward (droid.cpp:62, altcorr_kernel.cu:242-366, 426-455); F32 only
  * like the reference.  fmap1_grad [B,H1,W1,C], fmap2_grad [B,H2,W2,C] out, fully
- * written; coords_grad is not produced (the reference returns zeros). */
+ * written (fmap2_grad is cleared by the call, then accumulated with FP32 atomic
+ * adds, so its summation order varies from run to run); coords_grad is not
+ * produced (the reference returns zeros). */
 int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
                         const void* fmap2, const float* coords, const void* corr_grad, int radius, void* fmap1_grad,
                         void* fmap2_grad, void* stream);

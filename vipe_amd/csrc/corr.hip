@@ -780,66 +780,75 @@ __global__ __launch_bounds__(128) void altcorr_index_kernel(const T* __restrict_
                             W2, C, x, y, o, HW);
 }
 
-// Backward (float only, as the reference): one lane per (b, n, pixel) and tap;
-// fmap1_grad accumulates per lane then atomically, fmap2_grad atomically.
+// Backward with one wave per (batch b, source pixel): the window geometry and
+// tap weights of each of the N coordinate sets are the same for the whole wave
+// (scalar loads), and the lanes run along the channels, so a tap's f2 row and
+// its gradient row are contiguous accesses.  fmap1_grad of the pixel is summed
+// over the N sets and taps in registers and stored once; fmap2_grad takes FP32
+// atomic adds, the same scatter as the reference's kernel.
 template <int R>
-__global__ __launch_bounds__(128) void altcorr_bwd_kernel(const float* __restrict__ fmap1,
-                                                          const float* __restrict__ fmap2,
-                                                          const float* __restrict__ coords,
-                                                          const float* __restrict__ gout, float* __restrict__ g1,
-                                                          float* __restrict__ g2, int64_t B, int N, int H1, int W1,
-                                                          int H2, int W2, int C) {
+__global__ __launch_bounds__(256) void altcorr_bwd_wave_kernel(const float* __restrict__ fmap1,
+                                                               const float* __restrict__ fmap2,
+                                                               const float* __restrict__ coords,
+                                                               const float* __restrict__ gout, float* __restrict__ g1,
+                                                               float* __restrict__ g2, int64_t B, int N, int H1,
+                                                               int W1, int H2, int W2, int C) {
     constexpr int RD = 2 * R + 1;
     constexpr int NT = RD + 1;
     const int64_t HW1 = (int64_t)H1 * W1;
-    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (gid >= B * N * HW1) return;
-    const int64_t bn = gid / HW1;
-    const int64_t yx = gid - bn * HW1;
-    const int64_t b = bn / N;
-    const float x = coords[gid * 2 + 0];
-    const float y = coords[gid * 2 + 1];
-    const float fx0 = floorf(x), fy0 = floorf(y);
-    const float dx = x - fx0, dy = y - fy0;
-    const int xb = (int)fx0 - R, yb = (int)fy0 - R;
-    const float* go = gout + bn * RD * RD * HW1 + yx;
-    // tap weights g[iy][ix] = sum of output gradients routed to this tap
-    float gt[NT][NT];
-#pragma unroll
-    for (int a = 0; a < NT; a++)
-#pragma unroll
-        for (int c = 0; c < NT; c++) gt[a][c] = 0.f;
-#pragma unroll
-    for (int ix = 0; ix < RD; ix++)
-#pragma unroll
-        for (int iy = 0; iy < RD; iy++) {
-            const float g = go[(ix * RD + iy) * HW1];
-            gt[iy][ix] += g * (1 - dy) * (1 - dx);
-            gt[iy][ix + 1] += g * (1 - dy) * dx;
-            gt[iy + 1][ix] += g * dy * (1 - dx);
-            gt[iy + 1][ix + 1] += g * dy * dx;
-        }
+    const int64_t wid = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wid >= B * HW1) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t b = wid / HW1;
+    const int64_t yx = wid - b * HW1;
     const float* f1 = fmap1 + (b * HW1 + yx) * C;
-    float* gf1 = g1 + (b * HW1 + yx) * C;
     const float* f2b = fmap2 + b * (int64_t)H2 * W2 * C;
     float* g2b = g2 + b * (int64_t)H2 * W2 * C;
-    for (int c = 0; c < C; c++) {
-        const float a = f1[c];
+    for (int c0 = 0; c0 < C; c0 += 64) {
+        const int c = c0 + lane;
+        const bool cok = c < C;
+        const float a = cok ? f1[c] : 0.f;
         float acc = 0.f;
+        for (int n = 0; n < N; n++) {
+            const int64_t bn = b * N + n;
+            const int64_t gid = bn * HW1 + yx;
+            const float x = coords[gid * 2 + 0], y = coords[gid * 2 + 1];
+            const float fx0 = floorf(x), fy0 = floorf(y);
+            const float dx = x - fx0, dy = y - fy0;
+            const int xb = (int)fx0 - R, yb = (int)fy0 - R;
+            const float* go = gout + bn * RD * RD * HW1 + yx;
+            float gt[NT][NT];
 #pragma unroll
-        for (int iy = 0; iy < NT; iy++) {
-            const int h2 = yb + iy;
-            if (h2 < 0 || h2 >= H2) continue;
+            for (int i = 0; i < NT; i++)
 #pragma unroll
-            for (int ix = 0; ix < NT; ix++) {
-                const int w2 = xb + ix;
-                if (w2 < 0 || w2 >= W2) continue;
-                const int64_t off = ((int64_t)h2 * W2 + w2) * C + c;
-                acc += gt[iy][ix] * f2b[off];
-                atomicAdd(g2b + off, gt[iy][ix] * a);
+                for (int j = 0; j < NT; j++) gt[i][j] = 0.f;
+#pragma unroll
+            for (int ix = 0; ix < RD; ix++)
+#pragma unroll
+                for (int iy = 0; iy < RD; iy++) {
+                    const float gv = go[(ix * RD + iy) * HW1];
+                    gt[iy][ix] += gv * (1 - dy) * (1 - dx);
+                    gt[iy][ix + 1] += gv * (1 - dy) * dx;
+                    gt[iy + 1][ix] += gv * dy * (1 - dx);
+                    gt[iy + 1][ix + 1] += gv * dy * dx;
+                }
+#pragma unroll
+            for (int iy = 0; iy < NT; iy++) {
+                const int h2 = yb + iy;
+                if (h2 < 0 || h2 >= H2) continue;
+#pragma unroll
+                for (int ix = 0; ix < NT; ix++) {
+                    const int w2 = xb + ix;
+                    if (w2 < 0 || w2 >= W2) continue;
+                    if (cok) {
+                        const int64_t off = ((int64_t)h2 * W2 + w2) * C + c;
+                        acc += gt[iy][ix] * f2b[off];
+                        atomicAdd(g2b + off, gt[iy][ix] * a);
+                    }
+                }
             }
         }
-        atomicAdd(gf1 + c, acc);
+        if (cok) g1[(b * HW1 + yx) * C + c] = acc;
     }
 }
 
@@ -1523,9 +1532,17 @@ int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int
     if (radius != 3) return fail(VH_ERR_UNSUPPORTED, "altcorr_backward: radius %d (kernels built for r=3)", radius);
     const int64_t n = B * N * (int64_t)H1 * W1;
     if (n == 0) return VH_OK;
-    hipLaunchKernelGGL(altcorr_bwd_kernel<3>, dim3(ceil_div(n, 128)), dim3(128), 0, (hipStream_t)stream,
-                       (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad, (float*)fmap1_grad,
-                       (float*)fmap2_grad, B, N, H1, W1, H2, W2, C);
+    // fmap2_grad is cleared here (a fill kernel, so the call can be captured),
+    // then accumulated; fmap1_grad is stored by the wave that owns its pixel
+    const int64_t n2 = B * (int64_t)H2 * W2 * C;
+    if (n2 > 0) {
+        hipLaunchKernelGGL(zero_fill_kernel<float>, dim3((unsigned)std::min<int64_t>(ceil_div(n2, 256), 65535)),
+                           dim3(256), 0, (hipStream_t)stream, (float*)fmap2_grad, n2);
+        VH_LAUNCH_CHECK("altcorr_backward clear");
+    }
+    hipLaunchKernelGGL(altcorr_bwd_wave_kernel<3>, dim3(ceil_div(B * (int64_t)H1 * W1, 4)), dim3(256), 0,
+                       (hipStream_t)stream, (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad,
+                       (float*)fmap1_grad, (float*)fmap2_grad, B, N, H1, W1, H2, W2, C);
     VH_LAUNCH_CHECK("altcorr_backward");
     return VH_OK;
 }
