@@ -199,12 +199,13 @@ def test_corr_lookup_gradients_to_both_feature_maps():
     assert max(errs) < 1e-5
 
 
-@pytest.mark.parametrize("C", [128, 40])
+@pytest.mark.parametrize("C", [128, 96, 40])
 @pytest.mark.parametrize("dt", [torch.float32, torch.float16])
 def test_altcorr_index_vector_and_scalar_paths(C, dt):
-    """altcorr_index_forward with 32-channel chunks read as 16-byte vectors
-    (C = 128) and the per-element path (C = 40, not whole chunks), FP32 and FP16
-    features, against the FP64 oracle; some windows lie partly or wholly
+    """altcorr_index_forward on each of its paths against the FP64 oracle: one
+    wave per pixel (C = 128; C = 96 in FP32), one lane per pixel with 16-byte
+    vectors (C = 96 in FP16, not a whole 64-channel step) and per element
+    (C = 40, not whole 32-channel chunks); some windows lie partly or wholly
     outside the target image."""
     from vipe_amd.ext import droid_net_ext as D
 

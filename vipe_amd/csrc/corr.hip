@@ -750,6 +750,116 @@ __global__ __launch_bounds__(128) void altcorr_fwd_kernel(const T* __restrict__ 
     for (int c = 0; c < RD * RD; c++) o[c * HW1] = from_f<T>(tmp[c]);
 }
 
+// Indexed multi-level forward, one wave per (b, m, pixel) at one level.  Lane
+// (ix, cg) = (lane / 8, lane % 8) takes tap column ix and, in channel step k,
+// the 16 bytes at channel 32k + 4 cg (FP32) or 64k + 8 cg (FP16) of each tap
+// row: one load instruction reads 8 whole 128-byte lines, one per tap column.
+// The 8 channel groups are summed with three DPP steps; the 8 x 8 tap sums go
+// through LDS to 49 lanes that form the bilinear outputs.  The window
+// geometry is the same for the whole wave.
+template <typename T> struct AltStep;
+template <> struct AltStep<float> {
+    static constexpr int CH = 32;  // channels per step (8 lanes x 4)
+    static __device__ __forceinline__ float dot(acu32x4 a, acu32x4 b) {
+        float p = __uint_as_float(a[0]) * __uint_as_float(b[0]);
+        p = fmaf(__uint_as_float(a[1]), __uint_as_float(b[1]), p);
+        p = fmaf(__uint_as_float(a[2]), __uint_as_float(b[2]), p);
+        return fmaf(__uint_as_float(a[3]), __uint_as_float(b[3]), p);
+    }
+};
+template <> struct AltStep<__half> {
+    static constexpr int CH = 64;  // 8 lanes x 8
+    static __device__ __forceinline__ float dot(acu32x4 a, acu32x4 b) {
+        float p = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t x = a[k], y = b[k];  // through named scalars (see AltDot)
+            p = __builtin_amdgcn_fdot2(__builtin_bit_cast(acf16x2, x), __builtin_bit_cast(acf16x2, y), p, false);
+        }
+        return p;
+    }
+};
+
+template <int CTRL>
+__device__ __forceinline__ float alt_dpp(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void altcorr_index_wave_kernel(const T* __restrict__ fmap1, const T* __restrict__ f2l,
+                                                                 const float* __restrict__ coords,
+                                                                 const int64_t* __restrict__ ii,
+                                                                 const int64_t* __restrict__ jj, float* __restrict__ out,
+                                                                 int64_t B, int Nf, int M, int H, int W, int C, int H2,
+                                                                 int W2, int level, int L) {
+    constexpr int RD = 2 * R + 1;
+    constexpr int NT = RD + 1;
+    static_assert(NT == 8, "8 tap columns x 8 channel groups per wave");
+    constexpr int CH = AltStep<T>::CH;
+    __shared__ float S[4][NT][NT + 1];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t HW = (int64_t)H * W;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + wv;  // over B*M*HW
+    if (wid >= B * M * HW) return;
+    const int lane = threadIdx.x & 63;
+    const int ix = lane >> 3, cg = lane & 7;
+    const int64_t bm = wid / HW;
+    const int64_t yx = wid - bm * HW;
+    const int64_t b = bm / M;
+    const int m = (int)(bm - b * M);
+    const int64_t src = ii[m], dst = jj[m];
+    const float scale = (float)(1 << level);
+    const float x = coords[wid * 2 + 0] / scale;
+    const float y = coords[wid * 2 + 1] / scale;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float dx = x - fx0, dy = y - fy0;
+    const int xb = (int)fx0 - R, yb = (int)fy0 - R;
+    const int w2 = xb + ix;
+    const bool xok = (unsigned)w2 < (unsigned)W2;
+    const int lc = cg * (CH / 8);  // this lane's first channel of a step
+    const T* f1 = fmap1 + ((b * Nf + src) * HW + yx) * C + lc;
+    const T* f2 = f2l + ((b * Nf + dst) * (int64_t)H2 * W2 + (xok ? w2 : 0)) * C + lc;
+    float sv[NT];
+#pragma unroll
+    for (int iy = 0; iy < NT; iy++) sv[iy] = 0.f;
+    for (int c0 = 0; c0 < C; c0 += CH) {
+        const acu32x4 a = *reinterpret_cast<const acu32x4*>(f1 + c0);
+        acu32x4 g[NT];
+#pragma unroll
+        for (int iy = 0; iy < NT; iy++) {
+            // a tap outside the plane reads row 0 / column 0 of it and is zeroed
+            const int h2 = yb + iy;
+            const bool yok = (unsigned)h2 < (unsigned)H2;
+            const acu32x4 v = *reinterpret_cast<const acu32x4*>(f2 + (int64_t)(yok ? h2 : 0) * W2 * C + c0);
+            g[iy] = (xok && yok) ? v : acu32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int iy = 0; iy < NT; iy++) sv[iy] += AltStep<T>::dot(a, g[iy]);
+    }
+    // sum over the 8 channel groups (lanes differing in bits 0..2)
+#pragma unroll
+    for (int iy = 0; iy < NT; iy++) {
+        float v = sv[iy];
+        v += alt_dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+        v += alt_dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+        v += alt_dpp<0x141>(v);  // row_half_mirror
+        sv[iy] = v;
+    }
+    if (cg == 0) {
+#pragma unroll
+        for (int iy = 0; iy < NT; iy++) S[wv][iy][ix] = sv[iy];
+    }
+    // a wavefront's LDS accesses complete in order, so its own writes are
+    // visible to its reads; the wave barrier keeps the compiler from moving them
+    __builtin_amdgcn_wave_barrier();
+    if (lane < RD * RD) {
+        const int oix = lane / RD, oiy = lane - oix * RD;
+        const float v = S[wv][oiy][oix] * (1 - dy) * (1 - dx) + S[wv][oiy][oix + 1] * (1 - dy) * dx +
+                        S[wv][oiy + 1][oix] * dy * (1 - dx) + S[wv][oiy + 1][oix + 1] * dy * dx;
+        out[((bm * L + level) * RD * RD + lane) * HW + yx] = v;
+    }
+}
+
 // Indexed multi-level forward: one lane per (b, m, pixel, level).
 template <typename T, int R, bool VEC>
 __global__ __launch_bounds__(128) void altcorr_index_kernel(const T* __restrict__ fmap1, const T* __restrict__ f2l,
@@ -1559,6 +1669,22 @@ int vh_altcorr_index_forward(int dtype, int64_t B, int Nf, int M, int H, int W, 
     for (int l = 0; l < num_levels; l++) {
         dim3 grid(ceil_div(n, 128));
         const bool vec = altcorr_vec_ok(C, elem, fmap1, levels[l], B * (int64_t)Nf * Hl[l] * Wl[l] * C * elem);
+        // one wave per pixel when a channel step (8 lanes x 16 bytes) divides C
+        if (vec && C % (8 * 16 / elem) == 0) {
+            const dim3 wgrid((unsigned)ceil_div(n, 4));
+            if (dtype == VH_F16)
+                hipLaunchKernelGGL((altcorr_index_wave_kernel<__half, 3>), wgrid, dim3(256), 0, s, (const __half*)fmap1,
+                                   (const __half*)levels[l], coords, ii, jj, (float*)corr, B, Nf, M, H, W, C, Hl[l],
+                                   Wl[l], l, num_levels);
+            else if (dtype == VH_F32)
+                hipLaunchKernelGGL((altcorr_index_wave_kernel<float, 3>), wgrid, dim3(256), 0, s, (const float*)fmap1,
+                                   (const float*)levels[l], coords, ii, jj, (float*)corr, B, Nf, M, H, W, C, Hl[l],
+                                   Wl[l], l, num_levels);
+            else
+                return fail(VH_ERR_UNSUPPORTED, "altcorr_index_forward: dtype %d", dtype);
+            VH_LAUNCH_CHECK("altcorr_index_forward");
+            continue;
+        }
         switch (dtype) {
             case VH_F16:
                 if (vec)
