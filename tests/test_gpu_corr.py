@@ -225,3 +225,25 @@ def test_altcorr_index_vector_and_scalar_paths(C, dt):
     ref = OC.altcorr_index_forward(pyr_t[0].double(), [p.double() for p in pyr_t], cc.double(), ii, jj, r)
     assert out.dtype == torch.float32 and out.shape == ref.shape
     assert (out.double().cpu() - ref).abs().max().item() < 2e-5 * max(1.0, ref.abs().max().item())
+
+
+@pytest.mark.parametrize("C", [128, 96, 40])
+@pytest.mark.parametrize("dt", [torch.float32, torch.float16])
+def test_altcorr_forward_paths(C, dt):
+    """altcorr_forward (CorrLayer) on each of its paths against the FP64 oracle:
+    one wave per pixel (C = 128; C = 96 in FP32), one lane per pixel with 16-byte
+    vectors (C = 96 in FP16) and per element (C = 40).  FP16 outputs are compared
+    at one FP16 rounding of the largest output."""
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(17 + C)
+    B, N, H, W, r = 2, 2, 9, 13, 3
+    f1 = (torch.randn(B, H, W, C, generator=g) / math.sqrt(C)).to(dt)
+    f2 = (torch.randn(B, H, W, C, generator=g) / math.sqrt(C)).to(dt)
+    coords = _coords(B * N, H, W, H, W, g).view(B, N, 2, H, W).permute(0, 1, 3, 4, 2).contiguous()
+    coords[0, 1, :2] += 20.0  # windows wholly outside
+    (out,) = D.altcorr_forward(f1.cuda(), f2.cuda(), coords.cuda(), r)
+    ref = OC.altcorr_forward(f1.double(), f2.double(), coords.double(), r)
+    assert out.dtype == dt and out.shape == ref.shape
+    tol = 1e-5 if dt == torch.float32 else 2 ** -10 * max(1.0, ref.abs().max().item())
+    assert (out.double().cpu() - ref).abs().max().item() < tol

@@ -785,52 +785,39 @@ __device__ __forceinline__ float alt_dpp(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
 }
 
-template <typename T, int R>
-__global__ __launch_bounds__(256) void altcorr_index_wave_kernel(const T* __restrict__ fmap1, const T* __restrict__ f2l,
-                                                                 const float* __restrict__ coords,
-                                                                 const int64_t* __restrict__ ii,
-                                                                 const int64_t* __restrict__ jj, float* __restrict__ out,
-                                                                 int64_t B, int Nf, int M, int H, int W, int C, int H2,
-                                                                 int W2, int level, int L) {
+// One wave's (pixel, level): f1 is the pixel's feature row, f2 the target
+// plane [H2, W2, C]; writes the 49 bilinear outputs o[c * ostride] from lanes
+// c < 49.  S is this wave's [8][9] LDS slot.
+template <typename T, int R, typename TO>
+__device__ __forceinline__ void altcorr_wave_pixel(const T* __restrict__ f1, const T* __restrict__ f2, int H2, int W2,
+                                                   int C, float x, float y, float (*S)[2 * R + 3], TO* __restrict__ o,
+                                                   int64_t ostride) {
     constexpr int RD = 2 * R + 1;
     constexpr int NT = RD + 1;
     static_assert(NT == 8, "8 tap columns x 8 channel groups per wave");
     constexpr int CH = AltStep<T>::CH;
-    __shared__ float S[4][NT][NT + 1];
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t HW = (int64_t)H * W;
-    const int64_t wid = (int64_t)blockIdx.x * 4 + wv;  // over B*M*HW
-    if (wid >= B * M * HW) return;
     const int lane = threadIdx.x & 63;
     const int ix = lane >> 3, cg = lane & 7;
-    const int64_t bm = wid / HW;
-    const int64_t yx = wid - bm * HW;
-    const int64_t b = bm / M;
-    const int m = (int)(bm - b * M);
-    const int64_t src = ii[m], dst = jj[m];
-    const float scale = (float)(1 << level);
-    const float x = coords[wid * 2 + 0] / scale;
-    const float y = coords[wid * 2 + 1] / scale;
     const float fx0 = floorf(x), fy0 = floorf(y);
     const float dx = x - fx0, dy = y - fy0;
     const int xb = (int)fx0 - R, yb = (int)fy0 - R;
     const int w2 = xb + ix;
     const bool xok = (unsigned)w2 < (unsigned)W2;
     const int lc = cg * (CH / 8);  // this lane's first channel of a step
-    const T* f1 = fmap1 + ((b * Nf + src) * HW + yx) * C + lc;
-    const T* f2 = f2l + ((b * Nf + dst) * (int64_t)H2 * W2 + (xok ? w2 : 0)) * C + lc;
+    const T* f1l = f1 + lc;
+    const T* f2l = f2 + (int64_t)(xok ? w2 : 0) * C + lc;
     float sv[NT];
 #pragma unroll
     for (int iy = 0; iy < NT; iy++) sv[iy] = 0.f;
     for (int c0 = 0; c0 < C; c0 += CH) {
-        const acu32x4 a = *reinterpret_cast<const acu32x4*>(f1 + c0);
+        const acu32x4 a = *reinterpret_cast<const acu32x4*>(f1l + c0);
         acu32x4 g[NT];
 #pragma unroll
         for (int iy = 0; iy < NT; iy++) {
             // a tap outside the plane reads row 0 / column 0 of it and is zeroed
             const int h2 = yb + iy;
             const bool yok = (unsigned)h2 < (unsigned)H2;
-            const acu32x4 v = *reinterpret_cast<const acu32x4*>(f2 + (int64_t)(yok ? h2 : 0) * W2 * C + c0);
+            const acu32x4 v = *reinterpret_cast<const acu32x4*>(f2l + (int64_t)(yok ? h2 : 0) * W2 * C + c0);
             g[iy] = (xok && yok) ? v : acu32x4{0u, 0u, 0u, 0u};
         }
 #pragma unroll
@@ -847,17 +834,60 @@ __global__ __launch_bounds__(256) void altcorr_index_wave_kernel(const T* __rest
     }
     if (cg == 0) {
 #pragma unroll
-        for (int iy = 0; iy < NT; iy++) S[wv][iy][ix] = sv[iy];
+        for (int iy = 0; iy < NT; iy++) S[iy][ix] = sv[iy];
     }
     // a wavefront's LDS accesses complete in order, so its own writes are
     // visible to its reads; the wave barrier keeps the compiler from moving them
     __builtin_amdgcn_wave_barrier();
     if (lane < RD * RD) {
         const int oix = lane / RD, oiy = lane - oix * RD;
-        const float v = S[wv][oiy][oix] * (1 - dy) * (1 - dx) + S[wv][oiy][oix + 1] * (1 - dy) * dx +
-                        S[wv][oiy + 1][oix] * dy * (1 - dx) + S[wv][oiy + 1][oix + 1] * dy * dx;
-        out[((bm * L + level) * RD * RD + lane) * HW + yx] = v;
+        const float v = S[oiy][oix] * (1 - dy) * (1 - dx) + S[oiy][oix + 1] * (1 - dy) * dx +
+                        S[oiy + 1][oix] * dy * (1 - dx) + S[oiy + 1][oix + 1] * dy * dx;
+        o[lane * ostride] = from_f<TO>(v);
     }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void altcorr_index_wave_kernel(const T* __restrict__ fmap1, const T* __restrict__ f2l,
+                                                                 const float* __restrict__ coords,
+                                                                 const int64_t* __restrict__ ii,
+                                                                 const int64_t* __restrict__ jj, float* __restrict__ out,
+                                                                 int64_t B, int Nf, int M, int H, int W, int C, int H2,
+                                                                 int W2, int level, int L) {
+    constexpr int RD = 2 * R + 1;
+    __shared__ float S[4][RD + 1][RD + 2];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t HW = (int64_t)H * W;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + wv;  // over B*M*HW
+    if (wid >= B * M * HW) return;
+    const int64_t bm = wid / HW;
+    const int64_t yx = wid - bm * HW;
+    const int64_t b = bm / M;
+    const int m = (int)(bm - b * M);
+    const int64_t src = ii[m], dst = jj[m];
+    const float scale = (float)(1 << level);
+    altcorr_wave_pixel<T, R, float>(fmap1 + ((b * Nf + src) * HW + yx) * C, f2l + (b * Nf + dst) * (int64_t)H2 * W2 * C,
+                                    H2, W2, C, coords[wid * 2 + 0] / scale, coords[wid * 2 + 1] / scale, S[wv],
+                                    out + ((bm * L + level) * RD * RD) * HW + yx, HW);
+}
+
+// CorrLayer forward, one wave per (b, n, pixel): out [B, N, 49, H1, W1].
+template <typename T, int R>
+__global__ __launch_bounds__(256) void altcorr_fwd_wave_kernel(const T* __restrict__ fmap1, const T* __restrict__ fmap2,
+                                                               const float* __restrict__ coords, T* __restrict__ out,
+                                                               int64_t B, int N, int H1, int W1, int H2, int W2,
+                                                               int C) {
+    constexpr int RD = 2 * R + 1;
+    __shared__ float S[4][RD + 1][RD + 2];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t HW1 = (int64_t)H1 * W1;
+    const int64_t wid = (int64_t)blockIdx.x * 4 + wv;  // over B*N*HW1
+    if (wid >= B * N * HW1) return;
+    const int64_t bn = wid / HW1;
+    const int64_t yx = wid - bn * HW1;
+    const int64_t b = bn / N;
+    altcorr_wave_pixel<T, R, T>(fmap1 + (b * HW1 + yx) * C, fmap2 + b * (int64_t)H2 * W2 * C, H2, W2, C,
+                                coords[wid * 2 + 0], coords[wid * 2 + 1], S[wv], out + bn * RD * RD * HW1 + yx, HW1);
 }
 
 // Indexed multi-level forward: one lane per (b, m, pixel, level).
@@ -1612,6 +1642,20 @@ int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int 
     dim3 grid(ceil_div(n, 128));
     const int elem = dtype == VH_F16 ? 2 : 4;
     const bool vec = altcorr_vec_ok(C, elem, fmap1, fmap2, B * (int64_t)H2 * W2 * C * elem);
+    // one wave per pixel when a channel step (8 lanes x 16 bytes) divides C
+    if (vec && C % (8 * 16 / elem) == 0) {
+        const dim3 wgrid((unsigned)ceil_div(n, 4));
+        if (dtype == VH_F16)
+            hipLaunchKernelGGL((altcorr_fwd_wave_kernel<__half, 3>), wgrid, dim3(256), 0, s, (const __half*)fmap1,
+                               (const __half*)fmap2, coords, (__half*)corr, B, N, H1, W1, H2, W2, C);
+        else if (dtype == VH_F32)
+            hipLaunchKernelGGL((altcorr_fwd_wave_kernel<float, 3>), wgrid, dim3(256), 0, s, (const float*)fmap1,
+                               (const float*)fmap2, coords, (float*)corr, B, N, H1, W1, H2, W2, C);
+        else
+            return fail(VH_ERR_UNSUPPORTED, "altcorr_forward: dtype %d", dtype);
+        VH_LAUNCH_CHECK("altcorr_forward");
+        return VH_OK;
+    }
     switch (dtype) {
         case VH_F16:
             if (vec)
