@@ -31,6 +31,11 @@
         128-channel 64x80 FP32 feature maps, coordinates the identity plus
         N(0, 2 px) noise.
 
+    python3 -m vipe_amd.profiling corr_index [--dtype f32|f16] [--edges 96] [--iters 5]
+        CorrSampler's lookup (corr_index_forward, r = 3) into a plain level-0
+        volume [edges, 64, 80, 64, 80] (random values), coordinates the
+        identity plus N(0, 2 px) noise.
+
     python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
         --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
         Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
@@ -207,6 +212,24 @@ def _altcorr_layer(iters: int) -> None:
           flush=True)
 
 
+def _corr_index(dtype: str, edges: int, iters: int) -> None:
+    import torch
+
+    from .ext import droid_net_ext
+
+    dt = getattr(torch, _DT[dtype])
+    g = torch.Generator(device="cuda").manual_seed(0)
+    H, W = 64, 80
+    vol = torch.randn(edges, H, W, H, W, device="cuda", generator=g, dtype=dt)
+    yy, xx = torch.meshgrid(torch.arange(H, device="cuda"), torch.arange(W, device="cuda"), indexing="ij")
+    coords = torch.stack([xx, yy], 0).float()[None].repeat(edges, 1, 1, 1)
+    coords = (coords + 2.0 * torch.randn(coords.shape, device="cuda", generator=g)).contiguous()
+    for _ in range(iters + 1):
+        (out,) = droid_net_ext.corr_index_forward(vol, coords, 3)
+    torch.cuda.synchronize()
+    print(f"corr_index: {iters + 1} x lookup of [{edges}, {H}, {W}, {H}, {W}] {dtype} -> {tuple(out.shape)}", flush=True)
+
+
 def _short(name: str, width: int = 90) -> str:
     return name if len(name) <= width else name[: width - 3] + "..."
 
@@ -282,6 +305,10 @@ def main(argv=None) -> None:
     ac.add_argument("--workload", default="kf64_e384_64x80")
     al = sub.add_parser("altcorr_layer")
     al.add_argument("--iters", type=int, default=3)
+    ci = sub.add_parser("corr_index")
+    ci.add_argument("--dtype", choices=["f32", "f16"], default="f32")
+    ci.add_argument("--edges", type=int, default=96)
+    ci.add_argument("--iters", type=int, default=5)
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch")
@@ -306,6 +333,8 @@ def main(argv=None) -> None:
         _altcorr(a.dtype, a.iters, a.workload)
     elif a.cmd == "altcorr_layer":
         _altcorr_layer(a.iters)
+    elif a.cmd == "corr_index":
+        _corr_index(a.dtype, a.edges, a.iters)
     else:
         _summarize(a)
 
