@@ -161,12 +161,24 @@ __device__ __forceinline__ void unpack_vec<__half, 2>(const unsigned int& v, flo
     e[0] = f.x; e[1] = f.y;
 }
 
-template <typename T, int R, int VEC, bool TILED = false>
-__device__ __forceinline__ void load_tap_row_vec(const T* __restrict__ plane, int y1, int xb, int H2, int W2,
-                                                 float (&row)[2 * R + 2]) {
+// One tap row of a window in two steps: the raw vector loads (tap_row_load),
+// then the unpack and the shift to the window's first column (tap_row_finish).
+// A caller issues every row's loads before finishing any row: the shift's
+// empty asm statements keep the compiler from moving loads across them.
+template <typename T, int R, int VEC>
+struct TapRow {
+    static constexpr int NT = 2 * R + 2;
+    static constexpr int NL = (NT + VEC - 1 + VEC - 1) / VEC;  // vectors covering NT taps at any shift < VEC
     using V = typename VecOf<T, VEC>::type;
-    constexpr int NT = 2 * R + 2;
-    constexpr int NL = (NT + VEC - 1 + VEC - 1) / VEC;  // vectors covering NT taps at any shift < VEC
+    V v[NL];
+    bool ok[NL];
+};
+
+template <typename T, int R, int VEC, bool TILED = false>
+__device__ __forceinline__ void tap_row_load(const T* __restrict__ plane, int y1, int xb, int H2, int W2,
+                                             TapRow<T, R, VEC>& t) {
+    using V = typename TapRow<T, R, VEC>::V;
+    constexpr int NL = TapRow<T, R, VEC>::NL;
     constexpr int TW = kTileBytes / 4 / (int)sizeof(T);  // tile width (tiled layout)
     // tiled: H2, W2 are the padded sizes, whose extra rows and columns hold zeros
     const bool yok = (y1 >= 0) && (y1 < H2);
@@ -174,17 +186,27 @@ __device__ __forceinline__ void load_tap_row_vec(const T* __restrict__ plane, in
     const T* rp = TILED ? plane + (int64_t)(yy >> 2) * (W2 / TW) * (4 * TW) + (yy & 3) * TW
                         : plane + (int64_t)yy * W2;
     const int xa = xb & ~(VEC - 1);
-    float g[NL][VEC];
 #pragma unroll
     for (int q = 0; q < NL; q++) {
         // out-of-plane vectors read the (aligned, valid) plane start and are zeroed
         const int x = xa + q * VEC;
-        const bool ok = yok && x >= 0 && x < W2;
+        t.ok[q] = yok && x >= 0 && x < W2;
         const T* src = TILED ? rp + (x / TW) * (4 * TW) + (x % TW) : rp + x;
-        const V v = *reinterpret_cast<const V*>(ok ? src : plane);
-        unpack_vec<T, VEC>(v, g[q]);
+        t.v[q] = *reinterpret_cast<const V*>(t.ok[q] ? src : plane);
+    }
+}
+
+template <typename T, int R, int VEC>
+__device__ __forceinline__ void tap_row_finish(const TapRow<T, R, VEC>& t, int xb, float (&row)[2 * R + 2]) {
+    constexpr int NT = TapRow<T, R, VEC>::NT;
+    constexpr int NL = TapRow<T, R, VEC>::NL;
+    const int xa = xb & ~(VEC - 1);
+    float g[NL][VEC];
 #pragma unroll
-        for (int e = 0; e < VEC; e++) g[q][e] = ok ? g[q][e] : 0.f;
+    for (int q = 0; q < NL; q++) {
+        unpack_vec<T, VEC>(t.v[q], g[q]);
+#pragma unroll
+        for (int e = 0; e < VEC; e++) g[q][e] = t.ok[q] ? g[q][e] : 0.f;
     }
     // Shift by sh = xb - xa in two select stages.  The candidates pass through an
     // empty asm so that the compiler keeps per-lane selects instead of turning
@@ -218,16 +240,21 @@ __device__ __forceinline__ void lookup_level_vec(const T* __restrict__ plane, in
     const float dx = x0 - fx0, dy = y0 - fy0;
     const int xb = (int)fx0 - R, yb = (int)fy0 - R;
     const float w00 = (1.f - dx) * (1.f - dy), w10 = dx * (1.f - dy), w01 = (1.f - dx) * dy, w11 = dx * dy;
-    float r0[RD + 1], r1[RD + 1];
-    load_tap_row_vec<T, R, VEC, TILED>(plane, yb, xb, H2, W2, r0);
+    // every tap row is loaded before the first output is stored: stores count
+    // against the same vector-memory counter as loads, and the compiler may not
+    // move a load above a store it cannot prove disjoint, so interleaving them
+    // would serialise the rows' round trips
+    TapRow<T, R, VEC> raw[RD + 1];
+#pragma unroll
+    for (int j = 0; j <= RD; j++) tap_row_load<T, R, VEC, TILED>(plane, yb + j, xb, H2, W2, raw[j]);
+    float rows[RD + 1][RD + 1];
+#pragma unroll
+    for (int j = 0; j <= RD; j++) tap_row_finish<T, R, VEC>(raw[j], xb, rows[j]);
 #pragma unroll
     for (int j = 0; j < RD; j++) {
-        float (&top)[RD + 1] = (j & 1) ? r1 : r0;
-        float (&bot)[RD + 1] = (j & 1) ? r0 : r1;
-        load_tap_row_vec<T, R, VEC, TILED>(plane, yb + j + 1, xb, H2, W2, bot);
 #pragma unroll
         for (int i = 0; i < RD; i++) {
-            const float v = top[i] * w00 + top[i + 1] * w10 + bot[i] * w01 + bot[i + 1] * w11;
+            const float v = rows[j][i] * w00 + rows[j][i + 1] * w10 + rows[j + 1][i] * w01 + rows[j + 1][i + 1] * w11;
             o[(int64_t)(i * RD + j) * ostride] = store_from<T>(v);
         }
     }
