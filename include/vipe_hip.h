@@ -127,21 +127,23 @@ int vh_corr_build(int in_dtype, int mma_dtype, int out_dtype, int64_t E, int C, 
                   const void* fmap2, int num_levels, void* const* levels, int layout, void* workspace, void* stream);
 
 /* altcorr_forward (droid.cpp:41, altcorr_kernel.cu:26-138, 368-392).
- * fmap1 [B,H1,W1,C], fmap2 [B,H2,W2,C] (F16/F32/F64), coords [B,N,H1,W1,2] f32,
- * corr out [B,N,(2r+1)^2,H1,W1] in the fmap dtype, fully written. */
+ * fmap1 [B,H1,W1,C], fmap2 [B,H2,W2,C] (F16/F32/F64; sums in FP64 for F64),
+ * coords [B,N,H1,W1,2] f32, corr out [B,N,(2r+1)^2,H1,W1] in the fmap dtype,
+ * fully written. */
 int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
                        const void* fmap2, const float* coords, int radius, void* corr, void* stream);
 /* altcorr_backward (droid.cpp:62, altcorr_kernel.cu:242-366, 426-455); F32 only
  * like the reference.  fmap1_grad [B,H1,W1,C], fmap2_grad [B,H2,W2,C] out, fully
- * written (fmap2_grad is cleared by the call, then accumulated with FP32 atomic
- * adds, so its summation order varies from run to run); coords_grad is not
- * produced (the reference returns zeros). */
+ * written whatever N is, zeros when N == 0 (fmap2_grad is cleared by the call,
+ * then accumulated with FP32 atomic adds, so its summation order varies from
+ * run to run); coords_grad is not produced (the reference returns zeros). */
 int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
                         const void* fmap2, const float* coords, const void* corr_grad, int radius, void* fmap1_grad,
                         void* fmap2_grad, void* stream);
 /* altcorr_index_forward (droid.cpp:50, altcorr_kernel.cu:140-240, 394-424).
  * fmap1 [B,Nf,H,W,C]; levels[l] [B,Nf,Hl[l],Wl[l],C]; coords [B,M,H,W,2] f32;
- * ii, jj [M] int64 (device); corr out [B,M,L*(2r+1)^2,H,W] f32, fully written. */
+ * ii, jj [M] int64 (device); features F16/F32/F64 (sums in FP64 for F64);
+ * corr out [B,M,L*(2r+1)^2,H,W] f32, fully written. */
 int vh_altcorr_index_forward(int dtype, int64_t B, int Nf, int M, int H, int W, int C, const void* fmap1,
                              int num_levels, const void* const* levels, const int* Hl, const int* Wl,
                              const float* coords, const int64_t* ii, const int64_t* jj, int radius, void* corr,

@@ -247,3 +247,71 @@ def test_altcorr_forward_paths(C, dt):
     assert out.dtype == dt and out.shape == ref.shape
     tol = 1e-5 if dt == torch.float32 else 2 ** -10 * max(1.0, ref.abs().max().item())
     assert (out.double().cpu() - ref).abs().max().item() < tol
+
+
+@pytest.mark.parametrize("C", [128, 96])
+@pytest.mark.parametrize("down", [2, 8])
+def test_altcorr_backward_production_shapes(C, down):
+    """altcorr_backward as AltCorrBlock's pooled levels use it: C = 128 (two
+    passes of the 64-lane channel loop) and C = 96 (a partial second pass),
+    fmap2 pooled to H/2 or H/8, N = 2 coordinate sets, some windows partly or
+    wholly outside the target; both gradients against the FP64 oracle."""
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(40 + C + down)
+    B, N, H, W, r = 2, 2, 16, 24, 3
+    H2, W2 = H // down, W // down
+    f1 = torch.randn(B, H, W, C, generator=g) / math.sqrt(C)
+    f2 = torch.randn(B, H2, W2, C, generator=g) / math.sqrt(C)
+    coords = _coords(B * N, H, W, H2, W2, g, spread=1.5).view(B, N, 2, H, W).permute(0, 1, 3, 4, 2).contiguous()
+    coords[0, 1, :3] += 40.0   # wholly outside
+    coords[1, 0, :, :4] -= 3.0  # across the left border
+    cg = torch.randn(B, N, 49, H, W, generator=g)
+    g1, g2, gc = D.altcorr_backward(f1.cuda(), f2.cuda(), coords.cuda(), cg.cuda(), r)
+    r1, r2, _ = OC.altcorr_backward(f1.double(), f2.double(), coords.double(), cg.double(), r)
+    for got, ref in ((g1, r1), (g2, r2)):
+        err = (got.double().cpu() - ref).abs().max().item()
+        assert err <= 1e-5 * max(1.0, ref.abs().max().item()), err
+    assert torch.count_nonzero(gc) == 0
+
+
+def test_altcorr_backward_with_no_coordinate_sets_writes_zeros():
+    from vipe_amd.ext import droid_net_ext as D
+
+    B, H, W, C = 1, 6, 8, 64
+    f1 = torch.randn(B, H, W, C).cuda()
+    f2 = torch.randn(B, H, W, C).cuda()
+    coords = torch.zeros(B, 0, H, W, 2).cuda()
+    cg = torch.zeros(B, 0, 49, H, W).cuda()
+    g1, g2, _ = D.altcorr_backward(f1, f2, coords, cg, 3)
+    assert torch.count_nonzero(g1) == 0 and torch.count_nonzero(g2) == 0
+
+
+@pytest.mark.parametrize("C", [128, 40])
+def test_altcorr_float64_forwards(C):
+    """float64 features (the reference dispatches floating types and half):
+    altcorr_forward returns float64, altcorr_index_forward float32."""
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(60 + C)
+    B, N, H, W, r = 1, 2, 9, 13, 3
+    f1 = torch.randn(B, H, W, C, generator=g, dtype=torch.float64) / math.sqrt(C)
+    f2 = torch.randn(B, H, W, C, generator=g, dtype=torch.float64) / math.sqrt(C)
+    coords = _coords(B * N, H, W, H, W, g).view(B, N, 2, H, W).permute(0, 1, 3, 4, 2).contiguous()
+    (out,) = D.altcorr_forward(f1.cuda(), f2.cuda(), coords.cuda(), r)
+    ref = OC.altcorr_forward(f1, f2, coords.double(), r)
+    assert out.dtype == torch.float64
+    # bilinear weights are formed from float coordinates, as in the reference
+    assert (out.cpu() - ref).abs().max().item() < 1e-7
+
+    Nf, M = 3, 2
+    fmaps = torch.randn(1, Nf, C, H, W, generator=g, dtype=torch.float64) / math.sqrt(C)
+    pyr = [p.contiguous() for p in OC.altcorr_pyramid(fmaps, 3)]
+    ii, jj = torch.tensor([0, 2]), torch.tensor([1, 0])
+    yy, xx = torch.meshgrid(torch.arange(H), torch.arange(W), indexing="ij")
+    cc = (torch.stack([xx, yy], -1).float()[None, None].repeat(1, M, 1, 1, 1)
+          + torch.randn(1, M, H, W, 2, generator=g)).contiguous()
+    (idx,) = D.altcorr_index_forward(pyr[0].cuda(), [p.cuda() for p in pyr], cc.cuda(), ii.cuda(), jj.cuda(), r)
+    iref = OC.altcorr_index_forward(pyr[0], pyr, cc.double(), ii, jj, r)
+    assert idx.dtype == torch.float32
+    assert (idx.double().cpu() - iref).abs().max().item() < 1e-6 * max(1.0, iref.abs().max().item())

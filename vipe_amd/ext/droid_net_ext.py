@@ -83,7 +83,7 @@ def altcorr_forward(fmap1, fmap2, coords, radius):
     rd = 2 * int(radius) + 1
     corr = torch.empty((B, N, rd * rd, H1, W1), dtype=fmap1.dtype, device=fmap1.device)
     with torch.cuda.device(fmap1.device):
-        rc = _lib.load().vh_altcorr_forward(_code(fmap1, (torch.float16, torch.float32)), B, N, H1, W1, H2, W2,
+        rc = _lib.load().vh_altcorr_forward(_code(fmap1, _VOL_DT), B, N, H1, W1, H2, W2,
                                             fmap1.shape[3], _lib.ptr(fmap1), _lib.ptr(fmap2), _lib.ptr(coords),
                                             int(radius), _lib.ptr(corr), _lib.stream_of(fmap1))
     _lib.check(rc, "altcorr_forward")
@@ -95,8 +95,9 @@ def altcorr_backward(fmap1, fmap2, coords, corr_grad, radius):
     _gpu(fmap1, fmap2, coords, corr_grad)
     B, N, H1, W1, _ = coords.shape
     _, H2, W2, Cc = fmap2.shape
-    fmap1_grad = torch.zeros((B, H1, W1, fmap1.shape[3]), dtype=fmap1.dtype, device=fmap1.device)
-    fmap2_grad = torch.zeros((B, H2, W2, Cc), dtype=fmap1.dtype, device=fmap1.device)
+    # both are fully written by the call (fmap2_grad is cleared on the stream first)
+    fmap1_grad = torch.empty((B, H1, W1, fmap1.shape[3]), dtype=fmap1.dtype, device=fmap1.device)
+    fmap2_grad = torch.empty((B, H2, W2, Cc), dtype=fmap1.dtype, device=fmap1.device)
     coords_grad = torch.zeros((B, N, H1, W1, 2), dtype=fmap1.dtype, device=fmap1.device)
     with torch.cuda.device(fmap1.device):
         rc = _lib.load().vh_altcorr_backward(_code(fmap1, (torch.float32,)), B, N, H1, W1, H2, W2, fmap1.shape[3],
@@ -123,7 +124,7 @@ def altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius):
     iil = ii if ii.dtype == torch.int64 else ii.long().contiguous()
     jjl = jj if jj.dtype == torch.int64 else jj.long().contiguous()
     with torch.cuda.device(fmap1.device):
-        rc = _lib.load().vh_altcorr_index_forward(_code(fmap1, (torch.float16, torch.float32)), B, Nf, M, H, W, Cc,
+        rc = _lib.load().vh_altcorr_index_forward(_code(fmap1, _VOL_DT), B, Nf, M, H, W, Cc,
                                                   _lib.ptr(fmap1), L, lv, Hl, Wl, _lib.ptr(coords), _lib.ptr(iil),
                                                   _lib.ptr(jjl), int(radius), _lib.ptr(corr), _lib.stream_of(fmap1))
     _lib.check(rc, "altcorr_index_forward")
