@@ -194,6 +194,24 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
               float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* workspace,
               void* stream);
 
+/* Allocation-free vh_ba_run.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as vh_ba_run without `workspace`: the workspace is a file-scope
+ * device array of VH_BA_STATIC_SCRATCH_BYTES that each GPU the library runs on
+ * holds until the process ends (its address is looked up on the first call on
+ * each GPU).  A plan whose vh_ba_workspace_size exceeds that limit is rejected
+ * with VH_ERR_UNSUPPORTED.  The plan itself is created beforehand
+ * (vh_ba_plan_create allocates).  All calls on one GPU share its array, so
+ * calls that overlap on one GPU (other streams or threads) race: serialise
+ * them.  vh_ba_adjoint cannot follow this entry point (its workspace is not
+ * the caller's). */
+#define VH_BA_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int vh_ba_run_static(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
+                     const float* targets, const float* weights, const float* eta, const float* depth_active,
+                     int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
+                     float intrinsics_scale, int flags, float* dx, float* dz, float* energy, void* stream);
+
 /* Robust kernels of the generic solver (vipe/slam/ba/kernel.py:21-162,
  * build_robust_kernel :142-162), applied inside the fused BA: each flow
  * residual component's weight is multiplied by kernel(r) at the

@@ -73,6 +73,8 @@ _SIGS = {
     "vh_ba_adjoint": (i32, [vp, vp, vp, vp, vp, vp, vp, f32, i32, vp, vp, vp, vp, vp, vp]),
     "vh_ba_run": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, vp, vp, vp, vp,
                         vp]),
+    "vh_ba_run_static": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, vp, vp,
+                               vp, vp]),
     "vh_ba_run_robust": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, i32, f32,
                                f32, f32, vp, vp, vp, vp, vp]),
     "vh_robust_weights": (i32, [i32, f32, f32, f32, vp, vp, i64, vp]),

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -2104,6 +2105,41 @@ namespace {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Workspace of vh_ba_run_static: one array per GPU, held until the process
+// ends.  Host code never uses its name as an address: the device address of
+// each GPU's copy comes from hipGetSymbolAddress on that GPU's first call.
+constexpr size_t kStaticScratchBytes = VH_BA_STATIC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) char scratch[kStaticScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current GPU's scratch array (nullptr and the error
+// recorded on failure).
+void* static_scratch() {
+    int dev = -1;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) {
+        hip_fail(err, "ba_run_static hipGetDevice");
+        return nullptr;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        fail(VH_ERR_UNSUPPORTED, "ba_run_static: device %d (at most %d devices)", dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr) {
+            hip_fail(err != hipSuccess ? err : hipErrorInvalidSymbol, "ba_run_static hipGetSymbolAddress");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(256) void robust_weight_kernel(RobustK k, const float* __restrict__ x,
@@ -2484,6 +2520,24 @@ int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrins
     return ba_run_impl(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta, depth_active, iterations,
                        lm, ep, alpha, intrinsics_lm, intrinsics_ep, intrinsics_scale, flags, RobustK{0, 0.f, 0.f, 0.f},
                        dx, dz, energy, workspace, stream);
+}
+
+int vh_ba_run_static(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics,
+                     const float* disps_sens, const float* targets, const float* weights, const float* eta,
+                     const float* depth_active, int iterations, float lm, float ep, float alpha, float intrinsics_lm,
+                     float intrinsics_ep, float intrinsics_scale, int flags, float* dx, float* dz, float* energy,
+                     void* stream) {
+    if (!plan) return fail(VH_ERR_ARG, "ba_run_static: null plan");
+    if (plan->ws_bytes > kStaticScratchBytes)
+        return fail(VH_ERR_UNSUPPORTED, "ba_run_static: the plan needs %zu B of workspace (static limit %zu B)",
+                    plan->ws_bytes, kStaticScratchBytes);
+    // no kernel reads workspace memory it has not written in the same call,
+    // so the array needs no clearing between calls
+    void* ws = static_scratch();
+    if (!ws) return VH_ERR_HIP;
+    return ba_run_impl(plan, poses, disps, intrinsics, disps_sens, targets, weights, eta, depth_active, iterations,
+                       lm, ep, alpha, intrinsics_lm, intrinsics_ep, intrinsics_scale, flags, RobustK{0, 0.f, 0.f, 0.f},
+                       dx, dz, energy, ws, stream);
 }
 
 int vh_ba_run_robust(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
