@@ -26,16 +26,22 @@ def _small(name="kf8_e24_48x64", H=6, W=8):
     return d
 
 
-def test_jacobians_match_finite_differences():
-    d = _small()
+SE3_SIM3 = ["kf8_e24_48x64", "kf8_e24_48x64_sim3"]
+
+
+@pytest.mark.parametrize("name", SE3_SIM3)
+def test_jacobians_match_finite_differences(name):
+    d = _small(name)
     poses = d["poses"].double()
+    D = OB.pose_dof(poses)
+    gid = 4 if D == 7 else 3
     T0 = OB.edge_terms(poses, d["disps"].double(), d["intrinsics"].double(), d["targets"], d["weights"], d["ii"],
                        d["jj"], intrinsics_scale=0.125)
     eps = 1e-6
     E = d["ii"].shape[0]
     sel = ~(d["ii"] == d["jj"])
-    for k in range(6):
-        xi = torch.zeros(6, dtype=F64)
+    for k in range(D):
+        xi = torch.zeros(D, dtype=F64)
         xi[k] = eps
         def proj_with(which, sgn):
             p = poses.clone()
@@ -45,7 +51,7 @@ def test_jacobians_match_finite_differences():
                     continue
             # perturb every frame on one side by exp(+-xi) (edges touch each frame once per side)
             for f in torch.unique(idx).tolist():
-                p[f] = L.mul(3, L.expm(3, (sgn * xi)[None]), p[f][None])[0]
+                p[f] = L.mul(gid, L.expm(gid, (sgn * xi)[None]), p[f][None])[0]
             return p
         for which, Jname in (("j", "J"), ("i", "Ji")):
             pp = proj_with(which, 1.0)
@@ -91,8 +97,9 @@ def _args(d):
     return (d["disps_sens"], d["targets"], d["weights"], d["eta"], d["ii"], d["jj"])
 
 
-def test_schur_path_equals_dense_gauss_newton():
-    d = _small()
+@pytest.mark.parametrize("name", SE3_SIM3)
+def test_schur_path_equals_dense_gauss_newton(name):
+    d = _small(name)
     N = d["poses"].shape[0]
     da = torch.ones(N)
     p, dd, it = d["poses"].clone(), d["disps"].clone(), d["intrinsics"].clone()
@@ -104,10 +111,12 @@ def test_schur_path_equals_dense_gauss_newton():
     assert (dz - rdz).abs().max() < 1e-10
 
 
-def test_back_substitution_skips_pose_t0_like_the_reference():
+@pytest.mark.parametrize("name", SE3_SIM3)
+def test_back_substitution_skips_pose_t0_like_the_reference(name):
     """dz with the reference's EvT quirk equals the dense back-substitution with the
     columns of local pose 0 dropped (geom_kernels.cu:1186)."""
-    d = _small()
+    d = _small(name)
+    D = OB.pose_dof(d["poses"])
     N = d["poses"].shape[0]
     da = torch.ones(N)
     p, dd, it = d["poses"].clone(), d["disps"].clone(), d["intrinsics"].clone()
@@ -116,9 +125,9 @@ def test_back_substitution_skips_pose_t0_like_the_reference():
     rdx, rdz, Hm, gvec = OB.dense_gn_step(d["poses"], d["disps"], d["intrinsics"], *_args(d)[:4], d["ii"], d["jj"],
                                           da, 1, N, 0.0, 0.1, 0.05, return_system=True)
     P = N - 1
-    Hzz = torch.diagonal(Hm[6 * P:, 6 * P:])
-    Hzp = Hm[6 * P:, 6:6 * P]  # drop local pose 0
-    dz_q = (gvec[6 * P:] - Hzp @ rdx[1:].reshape(-1)) / Hzz
+    Hzz = torch.diagonal(Hm[D * P:, D * P:])
+    Hzp = Hm[D * P:, D:D * P]  # drop local pose 0
+    dz_q = (gvec[D * P:] - Hzp @ rdx[1:].reshape(-1)) / Hzz
     assert (dz.reshape(-1) - dz_q).abs().max() < 1e-10
     assert (dz.reshape(-1) - rdz.reshape(-1)).abs().max() > 1e-6  # the quirk is visible
 
@@ -165,3 +174,32 @@ def test_failed_factorisation_returns_zero_update():
                               -1.0, True, 0.05, False, 1e-6, 1e-6, 1.0, False)
     assert torch.count_nonzero(dx) == 0
     assert torch.equal(p, d["poses"])
+
+
+def test_sim3_step_retracts_by_the_group_product_and_observes_scale():
+    """Sim3 poses: the update is T <- Exp(dx) T as 4x4 matrices (sim3.h Exp and
+    product), the log-scale unknowns receive a non-zero update, and a frame's
+    dx has 7 entries."""
+    d = _small("kf8_e24_48x64_sim3")
+    N = d["poses"].shape[0]
+    p = d["poses"].clone()
+    dx, dz, _ = OB.ba_extended(p, d["disps"].clone(), d["intrinsics"].clone(), *_args(d)[:4], d["ii"], d["jj"],
+                               torch.ones(N), 1, N, 1, 1e-4, 0.1, False, 0.05, False, 1e-6, 1e-6, 1.0, False)
+    assert dx.shape == (N - 1, 7) and dx[:, 6].abs().max() > 1e-6
+    want = L.Sim3.matrix4(L.Sim3.exp(dx)) @ L.Sim3.matrix4(d["poses"][1:].double())
+    got = L.Sim3.matrix4(p[1:].double())
+    assert (got - want).abs().max() < 1e-6  # poses are stored in float32
+    assert torch.equal(p[0], d["poses"][0])
+
+
+def test_sim3_poses_with_unit_scale_keep_the_se3_residuals():
+    """A Sim3 pose with s = 1 is the SE3 pose: identical residuals, j-side
+    Jacobians equal in the first six columns and zero in the scale column."""
+    d = _small()
+    p7 = d["poses"].double()
+    p8 = torch.cat([p7, torch.ones(p7.shape[0], 1, dtype=F64)], -1)
+    a = OB.edge_terms(p7, d["disps"], d["intrinsics"], d["targets"], d["weights"], d["ii"], d["jj"])
+    b = OB.edge_terms(p8, d["disps"], d["intrinsics"], d["targets"], d["weights"], d["ii"], d["jj"])
+    assert torch.equal(a.ru, b.ru) and torch.equal(a.rv, b.rv)
+    assert torch.allclose(b.Ju[..., :6], a.Ju) and torch.count_nonzero(b.Ju[..., 6]) == 0
+    assert torch.allclose(b.Adj[:, :6, :6], a.Adj)

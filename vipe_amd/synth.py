@@ -51,6 +51,9 @@ CONFIGS = {
     # 64 keyframes, 384 edges, 64x80 (1/8 of 512x640)
     "kf64_e384_64x80": dict(n_frames=64, n_edges=384, H=64, W=80, intr=(80.0, 80.0, 40.0, 32.0),
                             window=3, n_random=128, sim3=False),
+    # Sim3 poses at the 8-keyframe size
+    "kf8_e24_48x64_sim3": dict(n_frames=8, n_edges=24, H=48, W=64, intr=(60.0, 60.0, 32.0, 24.0),
+                               window=2, n_random=4, sim3=True),
     # Sim3 poses at the 64-keyframe size
     "kf64_e384_64x80_sim3": dict(n_frames=64, n_edges=384, H=64, W=80, intr=(80.0, 80.0, 40.0, 32.0),
                                  window=3, n_random=128, sim3=True),
