@@ -179,6 +179,13 @@ size_t vh_ba_workspace_size(const vh_ba_plan* plan);
 #define VH_BA_MOTION_ONLY 1
 #define VH_BA_OPTIMIZE_INTRINSICS 2
 #define VH_BA_COMPUTE_ENERGY 4
+/* Sim3 poses: poses [N,8] (t, q xyzw, s), dx [P,7] (tau, phi, sigma) with
+ * 7-dof pose blocks and the retraction T <- Exp(dx) T of lietorch's Sim3
+ * (sim3.h; vipe/slam/maths/geom.py:147-182 gives the Jacobian of the point
+ * map).  The reference's fused BA takes SE3 only; this flag extends it.  At
+ * most 18 edges may leave one depth frame, and the window holds at most 292
+ * poses. */
+#define VH_BA_SIM3 8
 
 /* ba / ba_extended (slam.cpp:41-51).  All pointers device, f32 unless noted:
  * poses [N,7] (in place), disps [N,H,W] (in place), intrinsics [4]

@@ -310,3 +310,82 @@ def test_ba_high_degree_frames_match_oracle(tile):
     assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
     assert (dd.cpu().double() - rd.double()).abs().max().item() < 1e-4
     assert _reproj_err(g, p.cpu(), dd.cpu(), rp, rd, g.intrinsics) < 1e-3
+
+
+# ---- Sim3 poses ([N,8], 7-dof pose blocks) --------------------------------
+
+@pytest.mark.parametrize("iters", [1, 2])
+@pytest.mark.parametrize("intr", [False, True])
+def test_sim3_ba_matches_oracle_kf8(iters, intr):
+    g, mine, ref = _run_pair("kf8_e24_48x64_sim3", iters, intr=intr)
+    p, d, it, dx, dz, en = mine
+    rp, rd, rit, rdx, rdz, ren = ref
+    assert dx.shape == (g.t1 - g.t0, 7) and rdx.shape == dx.shape
+    assert (dx.double() - rdx).abs().max().item() < 1e-5 * max(1.0, rdx.abs().max().item())
+    assert (dz.double() - rdz).abs().max().item() < 1e-4
+    assert (p.double() - rp.double()).abs().max().item() < 1e-5
+    assert (d.double() - rd.double()).abs().max().item() < 1e-4
+    assert (it.double() - rit.double()).abs().max().item() < 1e-4
+    assert _reproj_err(g, p, d, rp, rd, it) < 1e-3
+    assert torch.allclose(en.double(), ren, rtol=1e-4, atol=1e-6)
+    assert (p[g.t0:, 7] - g.poses[g.t0:, 7]).abs().max().item() > 0.0  # the scales move
+
+
+def test_sim3_ba_motion_only_leaves_disps_bit_exact():
+    g, mine, ref = _run_pair("kf8_e24_48x64_sim3", 2, motion_only=True)
+    p, d, it, dx, dz, en = mine
+    assert dz is None and torch.equal(d, g.disps)
+    assert (p.double() - ref[0].double()).abs().max().item() < 1e-5
+
+
+@pytest.mark.parametrize("name", ["kf64_e384_64x80", "kf64_e384_64x80_sim3"])
+def test_ba_full_size_8_iterations_meets_accuracy_targets(name):
+    """64 KF / 384 edges / 64x80, the 8 GN iterations of the measured step, SE3
+    and Sim3 poses: FP32 device BA vs the FP64 oracle within 1e-4 px
+    reprojection and 1e-5 relative pose error."""
+    g, mine, ref = _run_pair(name, 8)
+    p, d, it, dx, dz, en = mine
+    rp, rd, rit, rdx, rdz, ren = ref
+    px = _reproj_err(g, p, d, rp, rd, g.intrinsics)
+    rel_pose = ((p.double() - rp.double()).norm(dim=1) / rp.double().norm(dim=1)).max().item()
+    print(f"[full-size BA {name}, 8 iterations] reprojection err {px:.3e} px, relative pose err {rel_pose:.3e}")
+    assert px < 1e-4
+    assert rel_pose < 1e-5
+    assert torch.allclose(en.double(), ren, rtol=1e-5)
+
+
+def test_sim3_ba_high_degree_frames_and_limit():
+    """Sim3 frames with up to 18 outgoing edges (20 E-row blocks of 7 rows) match
+    the oracle; a frame with 20 edges is refused with a clear error."""
+    from vipe_amd.ext import slam_ext
+
+    g = _high_degree_graph()
+    N, H, W = g.disps.shape
+    s = torch.exp(torch.randn(N, 1, generator=torch.Generator().manual_seed(3)) * 0.02)
+    g.poses = torch.cat([g.poses, s], -1).contiguous()
+    d = _cuda(g)
+    da = torch.ones(N, device="cuda")
+    with pytest.raises(RuntimeError, match="Sim3 limit"):
+        slam_ext.ba_extended(d["poses"].clone(), d["disps"].clone(), d["intrinsics"].clone(), d["disps_sens"],
+                             d["targets"], d["weights"], d["eta"], d["ii"], d["jj"], da, g.t0, g.t1, 1, 1e-4, 0.1,
+                             False, 0.05, False, 1e-6, 1e-6, 1.0)
+    # keep 18 of frame 1's 20 edges
+    keep = torch.ones(g.ii.shape[0], dtype=torch.bool)
+    keep[torch.nonzero(g.ii == 1).flatten()[18:]] = False
+    g.ii, g.jj = g.ii[keep], g.jj[keep]
+    g.targets, g.weights = g.targets[keep].contiguous(), g.weights[keep].contiguous()
+    kx = torch.unique(torch.cat([torch.arange(g.t0, g.t1), g.ii]))
+    assert kx.shape[0] == g.eta.shape[0]
+    d = _cuda(g)
+    iters = 2
+    p, dd = d["poses"].clone(), d["disps"].clone()
+    dx, dz, _ = slam_ext.ba_extended(p, dd, d["intrinsics"].clone(), d["disps_sens"], d["targets"], d["weights"],
+                                     d["eta"], d["ii"], d["jj"], da, g.t0, g.t1, iters, 1e-4, 0.1, False, 0.05, False,
+                                     1e-6, 1e-6, 1.0)
+    rp, rd = g.poses.clone(), g.disps.clone()
+    rdx, _, _ = OB.ba_extended(rp, rd, g.intrinsics.clone(), g.disps_sens, g.targets, g.weights, g.eta, g.ii, g.jj,
+                               torch.ones(N), g.t0, g.t1, iters, 1e-4, 0.1, False, 0.05, False, 1e-6, 1e-6, 1.0, False)
+    torch.cuda.synchronize()
+    assert (dx.cpu().double() - rdx).abs().max().item() < 1e-5 * max(1.0, rdx.abs().max().item())
+    assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
+    assert (dd.cpu().double() - rd.double()).abs().max().item() < 1e-4

@@ -35,6 +35,7 @@
 #include <new>
 #include <vector>
 
+#include "lie_math.hpp"
 #include "vh_common.hpp"
 
 #define MIN_DEPTH 0.25f
@@ -45,44 +46,56 @@ namespace ba {
 constexpr int kThreads = 256;
 constexpr int kMaxDeg = 20;   // edges leaving one depth frame
 constexpr int kMaxRext = 22;  // self row + edges + pseudo row
+constexpr int kMaxDegSim3 = 18;  // Sim3: 7-row blocks, the Gram's per-wave slots hold 20 row blocks
 constexpr int kNV = 40;       // per-edge partial stride (floats)
-constexpr int kEDS = 136;     // per-edge finalised record (doubles)
 constexpr int kCB = 64;       // Cholesky block size
 constexpr int kMaxLd = 2048;  // leading dimension of the reduced system (6P+1 rounded up to kCB)
 constexpr int kGramSplit = 8; // finalize workgroups per depth frame
 constexpr int kLdsBudget = 64 * 1024;  // dynamic E-row tile; with ~15 KB static, two frame workgroups fit a CU
 constexpr int kLdsMax = 128 * 1024;    // largest dynamic E-row tile a caller may ask for (one workgroup per CU)
 
+// Poses: SE3 [t, q] with 6-dof blocks (PD = 6), or Sim3 [t, q, s] with 7-dof
+// blocks (PD = 7, tangent [tau, phi, sigma]).  The projection of a point does
+// not change when the target frame's scale does (a Sim3 scale moves X, Y, Z
+// together), so every j-side Jacobian has a zero sigma column: the per-pixel
+// partials below are the same 6-dof sums for both groups, and sigma enters
+// through the i side, Ji = -Adj(Tij)^T Jj.
+//
 // per-edge partial layout (floats)
 //   [0,21)  Hjj upper triangle, index(n,m) n>=m: n(n+1)/2 + m
 //   [21,27) vj   27 energy  28 Hff  29 vf  [32,38) Fj
-// finalised edge record (doubles)
-//   [0,36) Hii  [36,72) Hij  [72,108) Hjj  [108,114) vi  [114,120) vj
-//   [120,126) Fi [126,132) Fj  132 Hff  133 vf  134 energy
-enum { E_HII = 0, E_HIJ = 36, E_HJJ = 72, E_VI = 108, E_VJ = 114, E_FI = 120, E_FJ = 126, E_HFF = 132, E_VF = 133,
-       E_EN = 134 };
+// finalised edge record (doubles), PD x PD blocks:
+//   Hii, Hij, Hjj, vi, vj, Fi, Fj, Hff, vf, energy
+template <int PD>
+struct Rec {
+    static constexpr int HII = 0, HIJ = PD * PD, HJJ = 2 * PD * PD, VI = 3 * PD * PD, VJ = VI + PD, FI = VJ + PD,
+                         FJ = FI + PD, HFF = FJ + PD, VF = HFF + 1, EN = HFF + 2;
+    static constexpr int SIZE = (EN + 1 + 7) / 8 * 8;  // 136 (SE3), 184 (Sim3)
+};
+constexpr int kEDSMax = Rec<7>::SIZE;
 
 // contribution item kinds for the assembly lists
 enum { K_HII = 0, K_HJJ = 1, K_HIJ = 2, K_HIJT = 3, K_G = 4, K_GT = 5, K_VI = 6, K_VJ = 7, K_ROW = 8 };
 
 struct DevPlan {
     int E, N, H, W, HW, t0, t1, P, K;
-    int TP, SUB, T, Rext_max, GT;
-    int n, ld, nb, nlb;  // system size, leading dim, blocks, lower pose blocks
+    int TP, SUB, T, Rext_max, GT;  // GT: Gram floats per tile (blocks x PD^2, set per run)
+    int n, ld, nb, nlb;  // system size, leading dim, blocks, lower pose blocks (set per run)
     int intr;            // system includes the intrinsics scalar (set per run)
+    int pd;              // pose dof: 6 (SE3) or 7 (Sim3), set per run
     const int* kx;       // [K] frame id of slot k
     const int* fr_ptr;   // [K+1] CSR into fr_edges
     const int* fr_edges; // [E]
     const int* fr_self;  // [K]
-    const int* gram_off; // [K+1] float offsets of each frame's Gram blocks
+    const int* gram_off; // [K+1] first Gram block of each frame (blocks of PD x PD)
     const int* ii;       // [E]
     const int* jj;       // [E]
     const int* blk_ab;   // [nlb*2]
     const int* blk_ptr;  // [nlb+1]
-    const int* blk_items;// packed (kind << 27 | index)
+    const int* blk_items;// packed (kind << 27 | index): edge index, or Gram block index
     const int* rhs_ptr;  // [P+1]
     const int* rhs_items;
-    const int* pseudo_off; // [K] gram float offset of the (pseudo, pseudo) block
+    const int* pseudo_off; // [K] Gram block index of the (pseudo, pseudo) block
 };
 
 struct Work {
@@ -91,7 +104,7 @@ struct Work {
     float* fen_part;   // [T][K]
     float* Qb;         // [K][HW]
     float* Wb;         // [K][HW]
-    double* ed;        // [E][kEDS]
+    double* ed;        // [E][Rec<PD>::SIZE]
     double* gd;        // [GT]
     double* fen;       // [K]
     double* A;         // [ld][ld]
@@ -156,16 +169,22 @@ struct RunArgs {
 
 __host__ __device__ inline int pk(int kind, int idx) { return (kind << 27) | idx; }
 
-// Relative pose Tij = Tj * Ti^-1 as rotation matrix + translation (relSE3,
-// geom_kernels.cu:101-114); stereo edges use (t = (-0.1,0,0), R = I) (:231-241).
-__device__ __forceinline__ void edge_transform(const float* __restrict__ poses, int i, int j, float* R, float* t) {
+// Relative pose Tij = Tj * Ti^-1 as rotation matrix, translation and scale
+// (relSE3, geom_kernels.cu:101-114; for Sim3 the group product of sim3.h:
+// s = s_j / s_i, t = t_j - s R t_i); stereo edges use (t = (-0.1,0,0), R = I,
+// s = 1) (:231-241).
+template <int PD>
+__device__ __forceinline__ void edge_transform(const float* __restrict__ poses, int i, int j, float* R, float* t,
+                                               float& sc) {
+    sc = 1.f;
     if (i == j) {
         R[0] = 1; R[1] = 0; R[2] = 0; R[3] = 0; R[4] = 1; R[5] = 0; R[6] = 0; R[7] = 0; R[8] = 1;
         t[0] = -0.1f; t[1] = 0.f; t[2] = 0.f;
         return;
     }
-    const float* Pi = poses + 7 * i;
-    const float* Pj = poses + 7 * j;
+    const float* Pi = poses + (PD + 1) * i;
+    const float* Pj = poses + (PD + 1) * j;
+    if (PD == 7) sc = Pj[7] / Pi[7];
     const float qi[4] = {Pi[3], Pi[4], Pi[5], Pi[6]};
     const float qj[4] = {Pj[3], Pj[4], Pj[5], Pj[6]};
     float q[4];
@@ -178,9 +197,9 @@ __device__ __forceinline__ void edge_transform(const float* __restrict__ poses, 
     R[3] = 2 * (x * y + w * z);     R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - w * x);
     R[6] = 2 * (x * z - w * y);     R[7] = 2 * (y * z + w * x);     R[8] = 1 - 2 * (x * x + y * y);
     const float ti[3] = {Pi[0], Pi[1], Pi[2]};
-    t[0] = Pj[0] - (R[0] * ti[0] + R[1] * ti[1] + R[2] * ti[2]);
-    t[1] = Pj[1] - (R[3] * ti[0] + R[4] * ti[1] + R[5] * ti[2]);
-    t[2] = Pj[2] - (R[6] * ti[0] + R[7] * ti[1] + R[8] * ti[2]);
+    t[0] = Pj[0] - sc * (R[0] * ti[0] + R[1] * ti[1] + R[2] * ti[2]);
+    t[1] = Pj[1] - sc * (R[3] * ti[0] + R[4] * ti[1] + R[5] * ti[2]);
+    t[2] = Pj[2] - sc * (R[6] * ti[0] + R[7] * ti[1] + R[8] * ti[2]);
 }
 
 // Per-pixel terms of one edge (projective_transform_kernel :298-435).
@@ -190,13 +209,23 @@ struct PixTerms {
     bool ok;  // depth in front of MIN_DEPTH: the residual carries weight
 };
 
-template <bool INTR, bool RES>
-__device__ __forceinline__ void pixel_terms(const float* R, const float* t, float X0, float Y0, float d,
+// The homogeneous point [X0, Y0, 1, d] maps to [s R X + t d, d]; sc = s is 1
+// for SE3 (PD = 6), where the products by it are left out.
+template <bool INTR, bool RES, int PD = 6>
+__device__ __forceinline__ void pixel_terms(const float* R, const float* t, float sc, float X0, float Y0, float d,
                                             float fx, float fy, float cx, float cy, float wgt_u, float wgt_v,
                                             float tgt_u, float tgt_v, float iscale, PixTerms& o) {
-    const float Xj0 = R[0] * X0 + R[1] * Y0 + R[2] + t[0] * d;
-    const float Xj1 = R[3] * X0 + R[4] * Y0 + R[5] + t[1] * d;
-    const float Xj2 = R[6] * X0 + R[7] * Y0 + R[8] + t[2] * d;
+    float Xj0 = R[0] * X0 + R[1] * Y0 + R[2];
+    float Xj1 = R[3] * X0 + R[4] * Y0 + R[5];
+    float Xj2 = R[6] * X0 + R[7] * Y0 + R[8];
+    if (PD == 7) {
+        Xj0 *= sc;
+        Xj1 *= sc;
+        Xj2 *= sc;
+    }
+    Xj0 += t[0] * d;
+    Xj1 += t[1] * d;
+    Xj2 += t[2] * d;
     const bool ok = !(Xj2 < MIN_DEPTH);
     o.ok = ok;
     const float dinv = ok ? 1.0f / Xj2 : 0.0f;
@@ -223,11 +252,16 @@ __device__ __forceinline__ void pixel_terms(const float* R, const float* t, floa
     o.Jzu = fx * (t[0] * dinv - t[2] * (x * d2));
     o.Jzv = fy * (t[1] * dinv - t[2] * (y * d2));
     if (INTR) {
-        // dXi/df = (-X0/fx, -Y0/fy, 0, 0) mapped by R only (:325-331)
+        // dXi/df = (-X0/fx, -Y0/fy, 0, 0) mapped by s R only (:325-331)
         const float a0 = -X0 / fx, a1 = -Y0 / fy;
-        const float e0 = R[0] * a0 + R[1] * a1;
-        const float e1 = R[3] * a0 + R[4] * a1;
-        const float e2 = R[6] * a0 + R[7] * a1;
+        float e0 = R[0] * a0 + R[1] * a1;
+        float e1 = R[3] * a0 + R[4] * a1;
+        float e2 = R[6] * a0 + R[7] * a1;
+        if (PD == 7) {
+            e0 *= sc;
+            e1 *= sc;
+            e2 *= sc;
+        }
         o.Jfu = iscale * (fx * dinv * e0 - fx * x * d2 * e2 + x * dinv);
         o.Jfv = iscale * (fy * dinv * e1 - fy * y * d2 * e2 + y * dinv);
     } else {
@@ -235,9 +269,13 @@ __device__ __forceinline__ void pixel_terms(const float* R, const float* t, floa
     }
 }
 
-// -Adj(T)^T e for Adj = [[R, [t]x R], [0, R]]:  top = -R^T e_top,
-// bottom = -(R^T (e_top x t) + R^T e_bot)   (adjSE3, geom_kernels.cu:81-99).
-__device__ __forceinline__ void neg_adjT(const float* R, const float* t, const float* e, float* o) {
+// -Adj(T)^T e.  SE3: Adj = [[R, [t]x R], [0, R]], so top = -R^T e_top and
+// bottom = -(R^T (e_top x t) + R^T e_bot) (adjSE3, geom_kernels.cu:81-99).
+// Sim3: Adj = [[sR, [t]x R, -t], [0, R, 0], [0, 0, 1]] (sim3.h Adj), so the top
+// is scaled by s and o[6] = t . e_top - e[6].  e has PD entries (e[6] read only
+// for Sim3).
+template <int PD = 6>
+__device__ __forceinline__ void neg_adjT(const float* R, const float* t, float sc, const float* e, float* o) {
     const float c0 = e[1] * t[2] - e[2] * t[1];
     const float c1 = e[2] * t[0] - e[0] * t[2];
     const float c2 = e[0] * t[1] - e[1] * t[0];
@@ -248,6 +286,12 @@ __device__ __forceinline__ void neg_adjT(const float* R, const float* t, const f
     o[3] = -(R[0] * b0 + R[3] * b1 + R[6] * b2);
     o[4] = -(R[1] * b0 + R[4] * b1 + R[7] * b2);
     o[5] = -(R[2] * b0 + R[5] * b1 + R[8] * b2);
+    if (PD == 7) {
+        o[0] *= sc;
+        o[1] *= sc;
+        o[2] *= sc;
+        o[PD - 1] = (t[0] * e[0] + t[1] * e[1] + t[2] * e[2]) - e[PD - 1];
+    }
 }
 
 // Cross-lane exchanges without the LDS crossbar.  Every stage pairs each lane
@@ -368,10 +412,10 @@ constexpr int kRowPad = 4;  // E-row stride TP + 4: each row starts 16-byte alig
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__host__ __device__ inline int gram_tiles(int Rext) { return (Rext * 6 + 15) / 16; }
+__host__ __device__ inline int gram_tiles(int Rext, int pd) { return (Rext * pd + 15) / 16; }
 
-__host__ __device__ inline int gram_ksplit(int Rext) {
-    const int nT = gram_tiles(Rext);
+__host__ __device__ inline int gram_ksplit(int Rext, int pd) {
+    const int nT = gram_tiles(Rext, pd);
     const int np = nT * (nT + 1) / 2;
     int best = 1, bestc = 1 << 30;
     for (int ks = 1; ks <= 4; ks *= 2) {
@@ -388,16 +432,24 @@ __host__ __device__ inline int gram_ksplit(int Rext) {
 }
 
 // floats of the cross-part reduction buffer [ks][np][256]
-__host__ __device__ inline int gram_red_floats(int Rext) {
-    const int nT = gram_tiles(Rext);
-    return gram_ksplit(Rext) * (nT * (nT + 1) / 2) * 256;
+__host__ __device__ inline int gram_red_floats(int Rext, int pd) {
+    const int nT = gram_tiles(Rext, pd);
+    return gram_ksplit(Rext, pd) * (nT * (nT + 1) / 2) * 256;
+}
+
+// whether a frame of Rext E-row blocks fits the Gram's per-wave slots
+__host__ __device__ inline bool gram_fits(int Rext, int pd) {
+    const int nT = gram_tiles(Rext, pd);
+    const int np = nT * (nT + 1) / 2;
+    const int wpk = 4 / gram_ksplit(Rext, pd);
+    return (np + wpk - 1) / wpk <= kGramSlots;
 }
 
 // dynamic LDS of the frame kernel: the E rows of a sub-tile, or the Gram
 // reduction buffer of the largest frame, whichever is larger
-inline size_t frame_lds_bytes(int Rext_max, int TP) {
-    size_t fl = (size_t)Rext_max * 6 * (TP + kRowPad);
-    for (int r = 1; r <= Rext_max; r++) fl = std::max(fl, (size_t)gram_red_floats(r));
+inline size_t frame_lds_bytes(int Rext_max, int TP, int pd) {
+    size_t fl = (size_t)Rext_max * pd * (TP + kRowPad);
+    for (int r = 1; r <= Rext_max; r++) fl = std::max(fl, (size_t)gram_red_floats(r, pd));
     return fl * 4;
 }
 
@@ -455,11 +507,11 @@ __device__ __forceinline__ void gram_pass(f32x4 (&acc)[kGramSlots], const float*
 // ---------------------------------------------------------------------------
 // 1. frame kernel
 // ---------------------------------------------------------------------------
-template <bool INTR, bool MOTION, bool ROBUST = false>
+template <bool INTR, bool MOTION, bool ROBUST = false, int PD = 6>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void ba_frame_kernel(DevPlan p, RunArgs a, Work w) {
-    extern __shared__ f32x4 dyn4[];  // rows [Rext_max*6][TP+kRowPad]; reused for the Gram reduction
+    extern __shared__ f32x4 dyn4[];  // rows [Rext_max*PD][TP+kRowPad]; reused for the Gram reduction
     float* dyn = reinterpret_cast<float*>(dyn4);
-    __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo
+    __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo(12) edge(13) scale(14)
     __shared__ __align__(16) float sQ[256];
     __shared__ float sAcc[4][kMaxDeg][kNV];  // per-wave edge partials (each wave adds into its own slot)
     __shared__ float sFen[4];
@@ -483,12 +535,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 
     if (tid < deg) {
         const int e = p.fr_edges[e0 + tid];
-        float Rm[9], t[3];
-        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        float Rm[9], t[3], sc;
+        edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
         for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
         for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
         sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
         sEP[tid][13] = __int_as_float(e);
+        sEP[tid][14] = sc;
     }
     for (int i = tid; i < 4 * kMaxDeg * kNV; i += kThreads) (&sAcc[0][0][0])[i] = 0.f;
     if (tid < 4) sFen[tid] = 0.f;
@@ -496,10 +549,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     // Gram wave mapping (fixed for all sub-tiles): this wave's pixel part and
     // tile pairs; lane (gi, gk) feeds row gi of a tile at pixel offset 4 gk
     const int wv = __builtin_amdgcn_readfirstlane(wave);
-    const int gnrows = Rext * 6;
-    const int gnT = gram_tiles(Rext);
+    const int gnrows = Rext * PD;
+    const int gnT = gram_tiles(Rext, PD);
     const int gnp = gnT * (gnT + 1) / 2;
-    const int gks = gram_ksplit(Rext);
+    const int gks = gram_ksplit(Rext, PD);
     const int gwpk = 4 / gks;
     const int gpart = wv / gwpk;
     const int gklen = TP / gks;
@@ -533,10 +586,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         edge_in(0, pix, valid, in1);
         edge_in(1, pix, valid, in2);
         float Csum = 0.f, wsum = 0.f, Ef = 0.f;
-        float es[6] = {0, 0, 0, 0, 0, 0};
+        float es[PD];
+#pragma unroll
+        for (int n = 0; n < PD; n++) es[n] = 0.f;
         for (int s = 0; s < deg; s++) {
             const float* Rm = sEP[s];
             const float* t = sEP[s] + 9;
+            const float sc = sEP[s][14];
             const bool stereo = sEP[s][12] != 0.f;
             const float4 in = in1;
             in1 = in2;
@@ -545,10 +601,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
             for (int i = 0; i < 32; i++) v[i] = 0.f;
             float fj[6] = {0, 0, 0, 0, 0, 0};
-            float er[6] = {0, 0, 0, 0, 0, 0};
+            float er[PD];  // j-side E row; its sigma entry (Sim3) is zero
+#pragma unroll
+            for (int n = 0; n < PD; n++) er[n] = 0.f;
             if (valid) {
                 PixTerms T;
-                pixel_terms<INTR, true>(Rm, t, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale, T);
+                pixel_terms<INTR, true, PD>(Rm, t, sc, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale,
+                                            T);
                 if (ROBUST) {
                     T.wu *= robust_weight(T.ru, a.rk);
                     T.wv *= robust_weight(T.rv, a.rk);
@@ -588,16 +647,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                     for (int n = 0; n < 6; n++)
                         er[n] = n == 1 ? cv * T.Jv[n] : n == 0 ? cu * T.Ju[n] : cu * T.Ju[n] + cv * T.Jv[n];
                     if (has_self) {
-                        float o[6];
-                        neg_adjT(Rm, t, er, o);
+                        float o[PD];
+                        neg_adjT<PD>(Rm, t, sc, er, o);
 #pragma unroll
-                        for (int n = 0; n < 6; n++) es[n] += o[n];
+                        for (int n = 0; n < PD; n++) es[n] += o[n];
                     }
                 }
             }
             if (!MOTION && tid < TP) {
 #pragma unroll
-                for (int n = 0; n < 6; n++) dyn[((has_self + s) * 6 + n) * ldr + tid] = er[n];
+                for (int n = 0; n < PD; n++) dyn[((has_self + s) * PD + n) * ldr + tid] = er[n];
             }
             // this wave's sums of the edge's partials; its LDS slot is its own, and
             // a wavefront's LDS accesses complete in order, so no barrier is needed
@@ -627,12 +686,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             if (tid < TP) {
                 if (has_self) {
 #pragma unroll
-                    for (int n = 0; n < 6; n++) dyn[n * ldr + tid] = es[n];
+                    for (int n = 0; n < PD; n++) dyn[n * ldr + tid] = es[n];
                 }
-                dyn[(R * 6 + 0) * ldr + tid] = wt;
-                dyn[(R * 6 + 1) * ldr + tid] = Eft;
+                dyn[(R * PD + 0) * ldr + tid] = wt;
+                dyn[(R * PD + 1) * ldr + tid] = Eft;
 #pragma unroll
-                for (int n = 2; n < 6; n++) dyn[(R * 6 + n) * ldr + tid] = 0.f;
+                for (int n = 2; n < PD; n++) dyn[(R * PD + n) * ldr + tid] = 0.f;
                 sQ[tid] = Q;
             }
             __syncthreads();
@@ -673,16 +732,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                 reinterpret_cast<f32x4*>(red)[(gpart * gnp + wv % gwpk + gwpk * j) * 64 + lane] = gacc[j];
         __syncthreads();
         if (tid == 0) w.fen_part[(int64_t)tile * p.K + k] = sFen[0] + sFen[1] + sFen[2] + sFen[3];
-        // 6 x 6 blocks (ra <= rb): element (n, m) is G[6ra + n][6rb + m], read as
-        // G[c][r] below the diagonal; D[r][c] of a tile sits at lane c + 16 (r / 4),
+        // PD x PD blocks (ra <= rb): element (n, m) is G[PD ra + n][PD rb + m], read
+        // as G[c][r] below the diagonal; D[r][c] of a tile sits at lane c + 16 (r / 4),
         // component r % 4
-        const int nvals = NB * 36;
-        float* out = w.gram_part + (int64_t)tile * p.GT + p.gram_off[k];
+        constexpr int PD2 = PD * PD;
+        const int nvals = NB * PD2;
+        float* out = w.gram_part + (int64_t)tile * p.GT + (int64_t)p.gram_off[k] * PD2;
         for (int i = tid; i < nvals; i += kThreads) {
-            const int b = i / 36, v = i - 36 * (i / 36);
+            const int b = i / PD2, v = i - PD2 * (i / PD2);
             int ra, rb;
             gram_rows(b, Rext, ra, rb);
-            int r = 6 * ra + v / 6, c = 6 * rb + v % 6;
+            int r = PD * ra + v / PD, c = PD * rb + v % PD;
             if (r > c) {
                 const int t = r;
                 r = c;
@@ -701,11 +761,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 // ---------------------------------------------------------------------------
 // 2. finalize: tile sums in FP64, i-side edge blocks
 // ---------------------------------------------------------------------------
+template <int PD>
 __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArgs a, Work w, int motion) {
+    using RC = Rec<PD>;
+    constexpr int PD2 = PD * PD;
     __shared__ double s[64];
-    __shared__ double Ad[36];
-    __shared__ double RT[12];  // R (row-major) and t of the edge's relative pose
-    __shared__ double Tm[36];  // Adj^T Hjj
+    __shared__ double Ad[PD2];
+    __shared__ double RT[13];  // R (row-major), t and s of the edge's relative pose
+    __shared__ double Tm[PD2];  // Adj^T Hjj
     __shared__ double tpart[kThreads / kNV][kNV];  // per-slice tile sums of an edge
     const int tid = threadIdx.x;
     const int bid = blockIdx.x;
@@ -728,10 +791,11 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
             tpart[sl][v] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         } else if (tid == kThreads - 1) {
             // the relative pose, while the other threads sum the tiles
-            float Rm[9], t[3];
-            edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+            float Rm[9], t[3], sc;
+            edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
             for (int i = 0; i < 9; i++) RT[i] = Rm[i];
             for (int i = 0; i < 3; i++) RT[9 + i] = t[i];
+            RT[12] = sc;
         }
         __syncthreads();
         if (tid < kNV) {
@@ -740,67 +804,74 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
             for (int u = 0; u < kSl; u++) a2 += tpart[u][tid];
             s[tid] = a2;
         }
-        // Adj = [[R, [t]x R], [0, R]], one entry per thread
-        if (tid < 36) {
-            const int r = tid / 6, c = tid % 6;
+        // Adj = [[R, [t]x R], [0, R]] (SE3) or [[sR, [t]x R, -t], [0, R, 0], [0, 0, 1]]
+        // (Sim3), one entry per thread
+        if (tid < PD2) {
+            const int r = tid / PD, c = tid % PD;
             double v = 0.0;
             if (r < 3 && c < 3) {
-                v = RT[r * 3 + c];
-            } else if (r >= 3 && c >= 3) {
+                v = RT[r * 3 + c] * (PD == 7 ? RT[12] : 1.0);
+            } else if (r >= 3 && r < 6 && c >= 3 && c < 6) {
                 v = RT[(r - 3) * 3 + (c - 3)];
-            } else if (r < 3) {  // ([t]x R)[r][c-3]
+            } else if (r < 3 && c < 6) {  // ([t]x R)[r][c-3]
                 const double* td = RT + 9;
                 const double tx[3] = {r == 0 ? 0.0 : (r == 1 ? td[2] : -td[1]),
                                       r == 0 ? -td[2] : (r == 1 ? 0.0 : td[0]),
                                       r == 0 ? td[1] : (r == 1 ? -td[0] : 0.0)};
                 for (int q = 0; q < 3; q++) v += tx[q] * RT[q * 3 + (c - 3)];
+            } else if (r < 3 && c == 6) {
+                v = -RT[9 + r];
+            } else if (r == 6 && c == 6) {
+                v = 1.0;
             }
             Ad[tid] = v;
         }
         __syncthreads();
-        double* rec = w.ed + (int64_t)e * kEDS;
-        // T = Adj^T Hjj (the i-j block is -T), then Hii = T Adj: two 6-term steps
-        if (tid < 36) {
-            const int i = tid / 6, j = tid % 6;
-            const int n = i > j ? i : j, m = i > j ? j : i;
-            rec[E_HJJ + tid] = s[n * (n + 1) / 2 + m];
+        double* rec = w.ed + (int64_t)e * RC::SIZE;
+        // Hjj from the 6-dof partials (its sigma row and column are zero);
+        // T = Adj^T Hjj (the i-j block is -T), then Hii = T Adj
+        auto hjj = [&](int n, int m) -> double {
+            if (n >= 6 || m >= 6) return 0.0;
+            const int hi = n > m ? n : m, lo = n > m ? m : n;
+            return s[hi * (hi + 1) / 2 + lo];
+        };
+        if (tid < PD2) {
+            const int i = tid / PD, j = tid % PD;
+            rec[RC::HJJ + tid] = hjj(i, j);
             double aTh = 0.0;
-            for (int q = 0; q < 6; q++) {
-                const int nq = q > j ? q : j, mq = q > j ? j : q;
-                aTh += Ad[q * 6 + i] * s[nq * (nq + 1) / 2 + mq];
-            }
+            for (int q = 0; q < PD; q++) aTh += Ad[q * PD + i] * hjj(q, j);
             Tm[tid] = aTh;
-            rec[E_HIJ + tid] = -aTh;
+            rec[RC::HIJ + tid] = -aTh;
         }
         __syncthreads();
-        if (tid < 36) {
-            const int i = tid / 6, j = tid % 6;
+        if (tid < PD2) {
+            const int i = tid / PD, j = tid % PD;
             double aTha = 0.0;
-            for (int q = 0; q < 6; q++) aTha += Tm[i * 6 + q] * Ad[q * 6 + j];
-            rec[E_HII + tid] = aTha;
-        } else if (tid >= 64 && tid < 70) {
+            for (int q = 0; q < PD; q++) aTha += Tm[i * PD + q] * Ad[q * PD + j];
+            rec[RC::HII + tid] = aTha;
+        } else if (tid >= 64 && tid < 64 + PD) {
             const int i = tid - 64;
             double vi = 0.0, fi = 0.0;
             for (int q = 0; q < 6; q++) {
-                vi += Ad[q * 6 + i] * s[21 + q];
-                fi += Ad[q * 6 + i] * s[32 + q];
+                vi += Ad[q * PD + i] * s[21 + q];
+                fi += Ad[q * PD + i] * s[32 + q];
             }
-            rec[E_VI + i] = -vi;
-            rec[E_VJ + i] = s[21 + i];
-            rec[E_FI + i] = -fi;
-            rec[E_FJ + i] = s[32 + i];
-        } else if (tid == 70) {
-            rec[E_HFF] = s[28];
-            rec[E_VF] = s[29];
-            rec[E_EN] = s[27];
+            rec[RC::VI + i] = -vi;
+            rec[RC::VJ + i] = i < 6 ? s[21 + i] : 0.0;
+            rec[RC::FI + i] = -fi;
+            rec[RC::FJ + i] = i < 6 ? s[32 + i] : 0.0;
+        } else if (tid == 64 + PD) {
+            rec[RC::HFF] = s[28];
+            rec[RC::VF] = s[29];
+            rec[RC::EN] = s[27];
         }
         return;
     }
     if (motion) return;
     // kGramSplit workgroups per depth frame, each a contiguous share of its Gram
     const int k = (bid - p.E) / kGramSplit, part = (bid - p.E) % kGramSplit;
-    const int off = p.gram_off[k];
-    const int nv = p.gram_off[k + 1] - off;
+    const int64_t off = (int64_t)p.gram_off[k] * PD2;
+    const int nv = (p.gram_off[k + 1] - p.gram_off[k]) * PD2;
     const int per = (nv + kGramSplit - 1) / kGramSplit;
     const int i0 = part * per, i1 = min(nv, i0 + per);
     for (int i = i0 + tid; i < i1; i += kThreads) {
@@ -837,26 +908,31 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
 // its block (sign 0: no contribution; the address is then still a valid one),
 // computed without branches so that the loads of several items can be in
 // flight together.
+template <int PD>
 __device__ __forceinline__ const double* assembly_item(const Work& w, int item, int i, int j, int motion,
                                                       double& sign) {
+    using RC = Rec<PD>;
     const int kind = item >> 27, idx = item & ((1 << 27) - 1);
     const bool tr = (kind == K_HIJT) || (kind == K_GT);
-    const int ij = tr ? j * 6 + i : i * 6 + j;
+    const int ij = tr ? j * PD + i : i * PD + j;
     const bool gram = (kind == K_G) || (kind == K_GT);
-    const int eoff = (kind == K_HII) ? E_HII : (kind == K_HJJ) ? E_HJJ : E_HIJ;
+    const int eoff = (kind == K_HII) ? RC::HII : (kind == K_HJJ) ? RC::HJJ : RC::HIJ;
     sign = gram ? (motion ? 0.0 : -1.0) : (kind <= K_HIJT ? 1.0 : 0.0);
-    return gram ? w.gd + idx + ij : w.ed + (int64_t)idx * kEDS + eoff + ij;
+    return gram ? w.gd + (int64_t)idx * (PD * PD) + ij : w.ed + (int64_t)idx * RC::SIZE + eoff + ij;
 }
 
+template <int PD>
 __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, Work w, int motion) {
+    using RC = Rec<PD>;
+    constexpr int PD2 = PD * PD;
     const int tid = threadIdx.x;
     const int bid = blockIdx.x;
     const int ld = p.ld;
-    const int n6 = 6 * p.P;
+    const int n6 = PD * p.P;  // first row after the pose unknowns
     if (bid < p.nlb) {
-        if (tid >= 36) return;
+        if (tid >= PD2) return;
         const int ab = p.blk_ab[2 * bid], bb = p.blk_ab[2 * bid + 1];
-        const int i = tid / 6, j = tid % 6;
+        const int i = tid / PD, j = tid % PD;
         // items in list order, 8 per round: their indices, then their values,
         // are loaded together.  Schur terms exist only when the depth system is
         // solved; in motion-only mode the Gram buffer is never written.
@@ -869,19 +945,19 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
             double val[8], sg[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                val[u] = *assembly_item(w, item[u], i, j, motion, sg[u]);
+                val[u] = *assembly_item<PD>(w, item[u], i, j, motion, sg[u]);
                 if (it0 + u >= end) sg[u] = 0.0;
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) s += (sg[u] != 0.0) ? sg[u] * val[u] : 0.0;
         }
         if (ab == bb && i == j) s += (double)a.ep + (double)a.lm * s;
-        w.A[(int64_t)(6 * ab + i) * ld + 6 * bb + j] = s;
+        w.A[(int64_t)(PD * ab + i) * ld + PD * bb + j] = s;
         return;
     }
     const int r = bid - p.nlb;
     if (r < p.P) {
-        if (tid >= 6) return;
+        if (tid >= PD) return;
         double sb = 0.0, sf = 0.0;
         const int beg = p.rhs_ptr[r], end = p.rhs_ptr[r + 1];
         for (int it0 = beg; it0 < end; it0 += 8) {
@@ -893,8 +969,11 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
             for (int u = 0; u < 8; u++) {
                 const int kind = item[u] >> 27, idx = item[u] & ((1 << 27) - 1);
                 const bool row = (kind == K_ROW);
-                const double* pb = row ? w.gd + idx + tid * 6 : w.ed + (int64_t)idx * kEDS + (kind == K_VI ? E_VI : E_VJ) + tid;
-                const double* pf = row ? w.gd + idx + tid * 6 + 1 : w.ed + (int64_t)idx * kEDS + (kind == K_VI ? E_FI : E_FJ) + tid;
+                // Gram block (row r, pseudo row): element (tid, 0) is sum Q E w, (tid, 1) sum Q E Ef
+                const double* pb = row ? w.gd + (int64_t)idx * PD2 + tid * PD
+                                       : w.ed + (int64_t)idx * RC::SIZE + (kind == K_VI ? RC::VI : RC::VJ) + tid;
+                const double* pf = row ? w.gd + (int64_t)idx * PD2 + tid * PD + 1
+                                       : w.ed + (int64_t)idx * RC::SIZE + (kind == K_VI ? RC::FI : RC::FJ) + tid;
                 vb[u] = *pb;
                 vf[u] = *pf;
                 sg[u] = row ? (motion ? 0.0 : -1.0) : ((kind == K_VI || kind == K_VJ) ? 1.0 : 0.0);
@@ -906,23 +985,23 @@ __global__ __launch_bounds__(64) void ba_assemble_kernel(DevPlan p, RunArgs a, W
                 sf += (sg[u] != 0.0) ? sg[u] * vf[u] : 0.0;
             }
         }
-        w.b[6 * r + tid] = sb;
-        if (p.intr) w.A[(int64_t)n6 * ld + 6 * r + tid] = sf;
+        w.b[PD * r + tid] = sb;
+        if (p.intr) w.A[(int64_t)n6 * ld + PD * r + tid] = sf;
         return;
     }
     if (r == p.P) {
         // intrinsics scalar, energy, failure flag: sums split over the wave
         double hff = 0.0, vf = 0.0, en = 0.0;
         for (int e = tid; e < p.E; e += 64) {
-            hff += w.ed[(int64_t)e * kEDS + E_HFF];
-            vf += w.ed[(int64_t)e * kEDS + E_VF];
-            en += w.ed[(int64_t)e * kEDS + E_EN];
+            hff += w.ed[(int64_t)e * RC::SIZE + RC::HFF];
+            vf += w.ed[(int64_t)e * RC::SIZE + RC::VF];
+            en += w.ed[(int64_t)e * RC::SIZE + RC::EN];
         }
         if (!motion) {
             for (int k = tid; k < p.K; k += 64) {
-                const int po = p.pseudo_off[k];
-                hff -= w.gd[po + 1 * 6 + 1];
-                vf -= w.gd[po + 0 * 6 + 1];
+                const int64_t po = (int64_t)p.pseudo_off[k] * PD2;
+                hff -= w.gd[po + 1 * PD + 1];
+                vf -= w.gd[po + 0 * PD + 1];
                 en += w.fen[k];
             }
         }
@@ -1446,7 +1525,7 @@ __global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __res
                                                              const double* __restrict__ LinvT, int ld, int nb,
                                                              const double* __restrict__ b,
                                                              const double* __restrict__ y, double* __restrict__ x,
-                                                             const int* flag, int P, float* dx) {
+                                                             const int* flag, int npose, float* dx) {
     static_assert(kBR == 4, "back_reduce4 sums 4 rows per wave");
     static_assert(kBT * 2 * (kBR / 2) == kCB * kCB, "X_k pairs cover the block");
     __shared__ double xs[kMaxLd];
@@ -1457,7 +1536,7 @@ __global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __res
     const int lane = tid & 63, wv = tid >> 6;
     if (flag[0] != 0) {
         for (int i = tid; i < ld; i += kBT) x[i] = 0.0;
-        for (int i = tid; i < 6 * P; i += kBT) dx[i] = 0.f;
+        for (int i = tid; i < npose; i += kBT) dx[i] = 0.f;
         return;
     }
     // The operands of block kb-1 are requested as soon as those of block kb
@@ -1528,16 +1607,16 @@ __global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __res
         __syncthreads();
     }
     for (int i = tid; i < ld; i += kBT) x[i] = xs[i];
-    for (int i = tid; i < 6 * P; i += kBT) dx[i] = (float)xs[i];
+    for (int i = tid; i < npose; i += kBT) dx[i] = (float)xs[i];
 }
 
 // ---------------------------------------------------------------------------
 // 5. back-substitution: dz = Q (w - sum_r E_r . dx_r) - Q Ef df
 // ---------------------------------------------------------------------------
-template <bool INTR>
+template <bool INTR, int PD = 6>
 __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs a, Work w) {
     __shared__ float sEP[kMaxDeg][16];
-    __shared__ float sU[kMaxDeg][6];
+    __shared__ float sU[kMaxDeg][6];  // the sigma entry of u meets a zero E entry
     const int k = blockIdx.y;
     const int tid = threadIdx.x;
     const int kf = p.kx[k];
@@ -1549,27 +1628,31 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
     const int HW = p.HW;
     if (tid < deg) {
         const int e = p.fr_edges[e0 + tid];
-        float Rm[9], t[3];
-        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        float Rm[9], t[3], sc;
+        edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
         for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
         for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+        sEP[tid][14] = sc;
         // u = dx[jj] (if 1 <= jj-t0 < P) - Adj(Tij) dx[k] (self row, if 1 <= k-t0 < P)
         double u[6] = {0, 0, 0, 0, 0, 0};
         const int lj = p.jj[e] - p.t0;
         if (lj >= 1 && lj < p.P)
-            for (int n = 0; n < 6; n++) u[n] += w.x[6 * lj + n];
+            for (int n = 0; n < 6; n++) u[n] += w.x[PD * lj + n];
         const int lk = kf - p.t0;
         if (has_self && lk >= 1 && lk < p.P) {
-            const double* v = w.x + 6 * lk;
+            const double* v = w.x + PD * lk;
             double rv[3], rw[3];
             for (int r = 0; r < 3; r++) {
                 rv[r] = Rm[3 * r] * v[0] + Rm[3 * r + 1] * v[1] + Rm[3 * r + 2] * v[2];
                 rw[r] = Rm[3 * r] * v[3] + Rm[3 * r + 1] * v[4] + Rm[3 * r + 2] * v[5];
             }
-            // top = R v_tau + t x (R v_phi), bottom = R v_phi
-            u[0] -= rv[0] + (t[1] * rw[2] - t[2] * rw[1]);
-            u[1] -= rv[1] + (t[2] * rw[0] - t[0] * rw[2]);
-            u[2] -= rv[2] + (t[0] * rw[1] - t[1] * rw[0]);
+            // top = s R v_tau + t x (R v_phi) - t v_sigma, bottom = R v_phi
+            // (the sigma entry of Adj v is v_sigma, dotted with a zero E entry)
+            const double sv = PD == 7 ? (double)sc : 1.0;
+            const double vs = PD == 7 ? v[PD - 1] : 0.0;
+            u[0] -= sv * rv[0] + (t[1] * rw[2] - t[2] * rw[1]) - t[0] * vs;
+            u[1] -= sv * rv[1] + (t[2] * rw[0] - t[0] * rw[2]) - t[1] * vs;
+            u[2] -= sv * rv[2] + (t[0] * rw[1] - t[1] * rw[0]) - t[2] * vs;
             u[3] -= rw[0];
             u[4] -= rw[1];
             u[5] -= rw[2];
@@ -1602,7 +1685,7 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
     edge_in(1, in2);
     __syncthreads();
     if (!valid) return;
-    const double df = INTR ? w.x[6 * p.P] : 0.0;
+    const double df = INTR ? w.x[PD * p.P] : 0.0;
     const int py = pix / p.W, px = pix - py * p.W;
     const float X0 = ((float)px - cx) / fx;
     const float Y0 = ((float)py - cy) / fy;
@@ -1614,12 +1697,13 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
         PixTerms T;
         if (rob) {
             // the frame kernel's robust weights: residuals at the same linearisation point
-            pixel_terms<INTR, true>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale,
-                                    T);
+            pixel_terms<INTR, true, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z,
+                                        in.w, a.iscale, T);
             T.wu *= robust_weight(T.ru, a.rk);
             T.wv *= robust_weight(T.rv, a.rk);
         } else {
-            pixel_terms<INTR, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, 0.f, 0.f, a.iscale, T);
+            pixel_terms<INTR, false, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y, 0.f,
+                                         0.f, a.iscale, T);
         }
         const float wdu = T.wu * act, wdv = T.wv * act;
         const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
@@ -1691,6 +1775,29 @@ __global__ __launch_bounds__(64) void ba_retract_kernel(float* poses, float* int
     for (int r = 0; r < 4; r++) T[3 + r] = (float)q1[r];
 }
 
+// Sim3 retraction T <- Exp(dx) T in FP64 with lietorch's Exp and product
+// (sim3.h Exp and operator*, rxso3.h calcW; quaternions normalised as the
+// group's constructors do), i.e. Sim3.retr of groups.py:157-160.
+__global__ __launch_bounds__(64) void ba_retract_sim3_kernel(float* poses, float* intrinsics, const double* x, int t0,
+                                                             int P, int intr, float iscale) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (intr && i == 0) {
+        const double df = x[7 * P] * (double)iscale;
+        intrinsics[0] = (float)((double)intrinsics[0] + df);
+        intrinsics[1] = (float)((double)intrinsics[1] + df);
+    }
+    if (i >= P) return;
+    float* T = poses + 8 * (t0 + i);
+    double Td[8];
+    for (int r = 0; r < 8; r++) Td[r] = T[r];
+    lie::Elem<double> X, Ex, Y;
+    lie::Sim3T<double>::load(Td, X);
+    lie::Sim3T<double>::exp(x + 7 * i, Ex);
+    lie::elem_mul(Ex, X, Y);
+    lie::Sim3T<double>::store(Y, Td);
+    for (int r = 0; r < 8; r++) T[r] = (float)Td[r];
+}
+
 // ---------------------------------------------------------------------------
 // 7. implicit adjoint of one Gauss-Newton iteration (no counterpart in the
 //    reference, whose BA has no backward)
@@ -1752,8 +1859,8 @@ __global__ __launch_bounds__(kThreads) void ba_adj_rhs_kernel(DevPlan p, RunArgs
     const int HW = p.HW;
     if (tid < deg) {
         const int e = p.fr_edges[e0 + tid];
-        float Rm[9], t[3];
-        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        float Rm[9], t[3], sc;
+        edge_transform<6>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
         for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
         for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
     }
@@ -1770,7 +1877,7 @@ __global__ __launch_bounds__(kThreads) void ba_adj_rhs_kernel(DevPlan p, RunArgs
         if (valid) {
             const int64_t eb = (int64_t)e * 2 * HW + pix;
             PixTerms T;
-            pixel_terms<false, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
+            pixel_terms<false, false>(sEP[s], sEP[s] + 9, 1.f, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
                                       0.f, 0.f, 1.f, T);
             float er[6];
             e_row(T, act * act, er);
@@ -1807,11 +1914,11 @@ __global__ __launch_bounds__(256) void ba_adj_pose_kernel(DevPlan p, RunArgs a, 
             if (lj == l) acc -= g.esum[(int64_t)e * 6 + n];
             if (li == l) {
                 // kf side: E_kf = -Adj^T er, so -E~ t adds Adj^T esum
-                float Rm[9], t[3];
-                edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+                float Rm[9], t[3], sc;
+                edge_transform<6>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
                 float es[6], o[6];
                 for (int q = 0; q < 6; q++) es[q] = (float)g.esum[(int64_t)e * 6 + q];
-                neg_adjT(Rm, t, es, o);
+                neg_adjT(Rm, t, 1.f, es, o);
                 acc -= (double)o[n];
             }
         }
@@ -1940,8 +2047,8 @@ __global__ __launch_bounds__(kThreads) void ba_adj_grad_kernel(DevPlan p, RunArg
     const double* mu = g.mu;
     if (tid < deg) {
         const int e = p.fr_edges[e0 + tid];
-        float Rm[9], t[3];
-        edge_transform(a.poses, p.ii[e], p.jj[e], Rm, t);
+        float Rm[9], t[3], sc;
+        edge_transform<6>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
         for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
         for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
         const bool stereo = p.ii[e] == p.jj[e];
@@ -2010,12 +2117,12 @@ __global__ __launch_bounds__(kThreads) void ba_adj_grad_kernel(DevPlan p, RunArg
             const int e = p.fr_edges[e0 + s];
             const int64_t eb = (int64_t)e * 2 * HW + pix;
             PixTerms T;
-            pixel_terms<false, false>(sEP[s], sEP[s] + 9, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
+            pixel_terms<false, false>(sEP[s], sEP[s] + 9, 1.f, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW],
                                       0.f, 0.f, 1.f, T);
             float er[6];
             e_row(T, a2, er);
             float o[6];
-            neg_adjT(sEP[s], sEP[s] + 9, er, o);
+            neg_adjT(sEP[s], sEP[s] + 9, 1.f, er, o);
             const bool stereo = sEP[s][12] != 0.f;
 #pragma unroll
             for (int n = 0; n < 6; n++) {
@@ -2039,7 +2146,7 @@ __global__ __launch_bounds__(kThreads) void ba_adj_grad_kernel(DevPlan p, RunArg
         const float* tt = sEP[s] + 9;
         const bool stereo = sEP[s][12] != 0.f;
         PixTerms T;
-        pixel_terms<false, true>(Rm, tt, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW], a.targets[eb],
+        pixel_terms<false, true>(Rm, tt, 1.f, X0, Y0, d, fx, fy, cx, cy, a.weights[eb], a.weights[eb + HW], a.targets[eb],
                                  a.targets[eb + HW], 1.f, T);
         float er[6];
         e_row(T, a2, er);
@@ -2050,7 +2157,7 @@ __global__ __launch_bounds__(kThreads) void ba_adj_grad_kernel(DevPlan p, RunArg
             const float wr = c == 0 ? T.wu : T.wv;
             const float r = c == 0 ? T.ru : T.rv;
             float Ji[6];
-            neg_adjT(Rm, tt, J, Ji);
+            neg_adjT(Rm, tt, 1.f, J, Ji);
             float Jm = 0.f, Jy = 0.f, Jq = 0.f, lmp = 0.f, lmr = 0.f;
 #pragma unroll
             for (int n = 0; n < 6; n++) {
@@ -2094,6 +2201,7 @@ struct vh_ba_plan {
     // workspace offsets
     size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_y, o_Li, o_flag;
     size_t o_esum, o_ybar, o_mu;  // adjoint
+    int gram_blocks = 0;          // Gram blocks over all depth frames
     // plan variants: the system size depends on whether intrinsics are optimised
     int P;
     // every non-stereo edge of a depth frame reaches a different pose (the
@@ -2222,16 +2330,16 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         fail(VH_ERR_UNSUPPORTED, "ba_plan_create: a depth frame has %d outgoing edges (limit %d)", deg_max, kMaxDeg);
         return nullptr;
     }
-    // Gram storage offsets: frame k holds Rext_k (Rext_k+1)/2 blocks of 36
+    // Gram storage, in blocks of PD x PD: frame k holds Rext_k (Rext_k+1)/2 blocks
     std::vector<int> gram_off(K + 1, 0), pseudo_off(K);
     for (int k = 0; k < K; k++) {
         const int Rext = fr_self[k] + (fr_ptr[k + 1] - fr_ptr[k]) + 1;
-        gram_off[k + 1] = gram_off[k] + Rext * (Rext + 1) / 2 * 36;
+        gram_off[k + 1] = gram_off[k] + Rext * (Rext + 1) / 2;
     }
     auto gblock = [&](int k, int ra, int rb) {  // ra <= rb
         const int Rext = fr_self[k] + (fr_ptr[k + 1] - fr_ptr[k]) + 1;
         const int b = ra * Rext - ra * (ra - 1) / 2 + (rb - ra);
-        return gram_off[k] + b * 36;
+        return gram_off[k] + b;
     };
     for (int k = 0; k < K; k++) {
         const int Rext = fr_self[k] + (fr_ptr[k + 1] - fr_ptr[k]) + 1;
@@ -2239,13 +2347,14 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
     }
     // tiles: TP pixels per sub-tile, sized so the E rows fit the LDS budget
     // unless the caller chose TP (then it must fit the workgroup's LDS)
+    // (sized for SE3 blocks; a Sim3 run checks its 7-row blocks against kLdsMax)
     int TP = 256;
-    while (TP > 64 && frame_lds_bytes(Rext_max, TP) > (size_t)kLdsBudget) TP /= 2;
+    while (TP > 64 && frame_lds_bytes(Rext_max, TP, 6) > (size_t)kLdsBudget) TP /= 2;
     if (pixel_tile != 0) {
-        if (frame_lds_bytes(Rext_max, pixel_tile) > (size_t)kLdsMax) {
+        if (frame_lds_bytes(Rext_max, pixel_tile, 6) > (size_t)kLdsMax) {
             delete pl;
             fail(VH_ERR_UNSUPPORTED, "ba_plan_create: %d-pixel tiles need %zu B of LDS for %d E rows (limit %d)",
-                 pixel_tile, frame_lds_bytes(Rext_max, pixel_tile), Rext_max * 6, kLdsMax);
+                 pixel_tile, frame_lds_bytes(Rext_max, pixel_tile, 6), Rext_max * 6, kLdsMax);
             return nullptr;
         }
         TP = pixel_tile;
@@ -2362,33 +2471,38 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
 
     DevPlan& d = pl->d;
     d.E = E; d.N = N; d.H = H; d.W = W; d.HW = HW; d.t0 = t0; d.t1 = t1; d.P = P; d.K = K;
-    d.TP = TP; d.SUB = SUB; d.T = T; d.Rext_max = Rext_max; d.GT = gram_off[K];
+    d.TP = TP; d.SUB = SUB; d.T = T; d.Rext_max = Rext_max;
     d.nlb = nlb;
-    const int nmax = 6 * P + 1;
-    d.ld = (nmax + kCB - 1) / kCB * kCB;
+    pl->gram_blocks = gram_off[K];
+    // SE3 sizes; ba_run_impl sets them for the run's pose group
+    d.pd = 6;
+    d.GT = pl->gram_blocks * 36;
+    d.ld = (6 * P + 1 + kCB - 1) / kCB * kCB;
     d.nb = d.ld / kCB;
     d.n = 6 * P;
     d.intr = 0;
     pl->deg_max = deg_max;
-    // workspace layout
+    // workspace layout, sized for the larger (Sim3) blocks so one workspace serves both groups
+    const int GTmax = std::max(pl->gram_blocks * 49, 1);
+    const int ldmax = (7 * P + 1 + kCB - 1) / kCB * kCB;
     size_t o = 0;
     pl->o_edge = o; o += align256((size_t)T * E * kNV * 4);
-    pl->o_gram = o; o += align256((size_t)T * d.GT * 4);
+    pl->o_gram = o; o += align256((size_t)T * GTmax * 4);
     pl->o_fen = o; o += align256((size_t)T * K * 4);
     pl->o_Q = o; o += align256((size_t)K * HW * 4);
     pl->o_W = o; o += align256((size_t)K * HW * 4);
-    pl->o_ed = o; o += align256((size_t)std::max(E, 1) * kEDS * 8);
-    pl->o_gd = o; o += align256((size_t)std::max(d.GT, 1) * 8);
+    pl->o_ed = o; o += align256((size_t)std::max(E, 1) * kEDSMax * 8);
+    pl->o_gd = o; o += align256((size_t)GTmax * 8);
     pl->o_fenk = o; o += align256((size_t)K * 8);
-    pl->o_A = o; o += align256((size_t)d.ld * d.ld * 8);
-    pl->o_b = o; o += align256((size_t)d.ld * 8);
-    pl->o_x = o; o += align256((size_t)d.ld * 8);
-    pl->o_Li = o; o += align256((size_t)d.ld * kCB * 8);
-    pl->o_y = o; o += align256((size_t)d.ld * 8);
+    pl->o_A = o; o += align256((size_t)ldmax * ldmax * 8);
+    pl->o_b = o; o += align256((size_t)ldmax * 8);
+    pl->o_x = o; o += align256((size_t)ldmax * 8);
+    pl->o_Li = o; o += align256((size_t)ldmax * kCB * 8);
+    pl->o_y = o; o += align256((size_t)ldmax * 8);
     pl->o_flag = o; o += 256;
     pl->o_esum = o; o += align256((size_t)std::max(E, 1) * 6 * 8);
-    pl->o_ybar = o; o += align256((size_t)d.ld * 8);
-    pl->o_mu = o; o += align256((size_t)d.ld * 8);
+    pl->o_ybar = o; o += align256((size_t)ldmax * 8);
+    pl->o_mu = o; o += align256((size_t)ldmax * 8);
     pl->ws_bytes = o;
     return pl;
 }
@@ -2411,6 +2525,79 @@ size_t vh_ba_workspace_size(const vh_ba_plan* plan) { return plan ? plan->ws_byt
 
 int vh_ba_plan_pixel_tile(const vh_ba_plan* plan) { return plan ? plan->d.TP : -1; }
 
+}  // extern "C"
+
+// The launches of `iterations` Gauss-Newton iterations for pose blocks of PD.
+template <int PD>
+static int ba_iterations(const DevPlan& d, RunArgs a, const Work& w, int iterations, bool motion, bool intr,
+                         const RobustK& rk, float* poses, float* intrinsics, float* dx, float intrinsics_scale,
+                         hipStream_t s) {
+    const size_t lds = frame_lds_bytes(d.Rext_max, d.TP, PD);
+    const dim3 fgrid(d.T, d.K);
+    const int npad = d.ld - d.n;
+    for (int itr = 0; itr < iterations; itr++) {
+        a.itr = itr;
+        if (d.E > 0 && d.K > 0) {
+            if (rk.type) {
+                // robust weights: a separate instantiation, so plain least squares pays nothing
+                if (motion) {
+                    if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, true, true, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                    else hipLaunchKernelGGL((ba_frame_kernel<false, true, true, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                } else {
+                    if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, false, true, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                    else hipLaunchKernelGGL((ba_frame_kernel<false, false, true, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                }
+            } else if (motion) {
+                if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, true, false, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                else hipLaunchKernelGGL((ba_frame_kernel<false, true, false, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+            } else {
+                if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, false, false, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+                else hipLaunchKernelGGL((ba_frame_kernel<false, false, false, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+            }
+            VH_LAUNCH_CHECK("ba_frame_kernel");
+            hipLaunchKernelGGL(ba_finalize_kernel<PD>, dim3(d.E + (motion ? 0 : d.K * kGramSplit)), dim3(kThreads), 0,
+                               s, d, a, w, motion ? 1 : 0);
+            VH_LAUNCH_CHECK("ba_finalize_kernel");
+        } else if (!motion && d.K > 0) {
+            // no edges: every depth frame still has its prior / damping terms
+            hipLaunchKernelGGL((ba_frame_kernel<false, false, false, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
+            VH_LAUNCH_CHECK("ba_frame_kernel");
+            hipLaunchKernelGGL(ba_finalize_kernel<PD>, dim3(d.E + d.K * kGramSplit), dim3(kThreads), 0, s, d, a, w, 0);
+            VH_LAUNCH_CHECK("ba_finalize_kernel");
+        }
+        hipLaunchKernelGGL(ba_assemble_kernel<PD>, dim3(d.nlb + d.P + 1 + npad), dim3(64), 0, s, d, a, w,
+                           motion ? 1 : 0);
+        VH_LAUNCH_CHECK("ba_assemble_kernel");
+        hipLaunchKernelGGL(chol_factor_kernel, dim3(1), dim3(256), 0, s, w.A, w.Li, d.ld, 0, w.flag);
+        VH_LAUNCH_CHECK("chol_factor_kernel");
+        for (int kb = 0; kb + 1 < d.nb; kb++) {
+            const int rem = d.nb - kb - 1;
+            hipLaunchKernelGGL(chol_step_kernel, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, w.A, w.Li, w.b, w.y, d.ld,
+                               kb, w.flag);
+            VH_LAUNCH_CHECK("chol_step_kernel");
+        }
+        hipLaunchKernelGGL(chol_backsolve_kernel, dim3(1), dim3(kBT), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.y, w.x,
+                           w.flag, PD * d.P, dx);
+        VH_LAUNCH_CHECK("chol_backsolve_kernel");
+        if (!motion && d.K > 0) {
+            const dim3 bgrid((d.HW + kThreads - 1) / kThreads, d.K);
+            if (intr) hipLaunchKernelGGL((ba_backsub_kernel<true, PD>), bgrid, dim3(kThreads), 0, s, d, a, w);
+            else hipLaunchKernelGGL((ba_backsub_kernel<false, PD>), bgrid, dim3(kThreads), 0, s, d, a, w);
+            VH_LAUNCH_CHECK("ba_backsub_kernel");
+        }
+        if (PD == 7)
+            hipLaunchKernelGGL(ba_retract_sim3_kernel, dim3((d.P + 63) / 64), dim3(64), 0, s, poses, intrinsics, w.x,
+                               d.t0, d.P, intr ? 1 : 0, intrinsics_scale);
+        else
+            hipLaunchKernelGGL(ba_retract_kernel, dim3((d.P + 63) / 64), dim3(64), 0, s, poses, intrinsics, w.x,
+                               d.t0, d.P, intr ? 1 : 0, intrinsics_scale);
+        VH_LAUNCH_CHECK("ba_retract_kernel");
+    }
+    return VH_OK;
+}
+
+extern "C" {
+
 static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics,
                        const float* disps_sens, const float* targets, const float* weights, const float* eta,
                        const float* depth_active, int iterations, float lm, float ep, float alpha,
@@ -2425,12 +2612,26 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
     const bool motion = (flags & VH_BA_MOTION_ONLY) != 0;
     const bool intr = (flags & VH_BA_OPTIMIZE_INTRINSICS) != 0;
     const bool cen = (flags & VH_BA_COMPUTE_ENERGY) != 0;
+    const bool sim3 = (flags & VH_BA_SIM3) != 0;
     if (!motion && (!eta || !dz)) return fail(VH_ERR_ARG, "ba_run: eta and dz are required unless motion only");
     if (cen && !energy) return fail(VH_ERR_ARG, "ba_run: energy output required");
     hipStream_t s = (hipStream_t)stream;
     DevPlan d = plan->d;
+    const int pd = sim3 ? 7 : 6;
+    if (sim3) {
+        // the plan was sized for SE3 blocks; 7-row blocks must still fit
+        if (pd * d.P + 1 > kMaxLd)
+            return fail(VH_ERR_UNSUPPORTED, "ba_run: %d Sim3 poses in the window (limit %d)", d.P, (kMaxLd - 1) / 7);
+        if (!gram_fits(d.Rext_max, pd) || frame_lds_bytes(d.Rext_max, d.TP, pd) > (size_t)kLdsMax)
+            return fail(VH_ERR_UNSUPPORTED, "ba_run: a depth frame has %d outgoing edges (Sim3 limit %d)",
+                        plan->deg_max, kMaxDegSim3);
+    }
+    d.pd = pd;
+    d.GT = plan->gram_blocks * pd * pd;
     d.intr = intr ? 1 : 0;
-    d.n = 6 * d.P + (intr ? 1 : 0);
+    d.n = pd * d.P + (intr ? 1 : 0);
+    d.ld = (pd * d.P + 1 + kCB - 1) / kCB * kCB;
+    d.nb = d.ld / kCB;
     char* ws = (char*)workspace;
     Work w;
     w.edge_part = (float*)(ws + plan->o_edge);
@@ -2453,63 +2654,8 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
     a.targets = targets; a.weights = weights; a.eta = eta; a.depth_active = depth_active;
     a.lm = lm; a.ep = ep; a.alpha = alpha; a.ilm = intrinsics_lm; a.iep = intrinsics_ep;
     a.iscale = intrinsics_scale; a.dx = dx; a.dz = dz; a.energy = energy; a.compute_energy = cen ? 1 : 0;
-    const size_t lds = frame_lds_bytes(d.Rext_max, d.TP);
-    const dim3 fgrid(d.T, d.K);
-    const int npad = d.ld - d.n;
-    for (int itr = 0; itr < iterations; itr++) {
-        a.itr = itr;
-        if (d.E > 0 && d.K > 0) {
-            if (rk.type) {
-                // robust weights: a separate instantiation, so plain least squares pays nothing
-                if (motion) {
-                    if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, true, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
-                    else hipLaunchKernelGGL((ba_frame_kernel<false, true, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
-                } else {
-                    if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, false, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
-                    else hipLaunchKernelGGL((ba_frame_kernel<false, false, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
-                }
-            } else if (motion) {
-                if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
-                else hipLaunchKernelGGL((ba_frame_kernel<false, true>), fgrid, dim3(kThreads), lds, s, d, a, w);
-            } else {
-                if (intr) hipLaunchKernelGGL((ba_frame_kernel<true, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
-                else hipLaunchKernelGGL((ba_frame_kernel<false, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
-            }
-            VH_LAUNCH_CHECK("ba_frame_kernel");
-            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + (motion ? 0 : d.K * kGramSplit)), dim3(kThreads), 0, s,
-                               d, a, w, motion ? 1 : 0);
-            VH_LAUNCH_CHECK("ba_finalize_kernel");
-        } else if (!motion && d.K > 0) {
-            // no edges: every depth frame still has its prior / damping terms
-            hipLaunchKernelGGL((ba_frame_kernel<false, false>), fgrid, dim3(kThreads), lds, s, d, a, w);
-            VH_LAUNCH_CHECK("ba_frame_kernel");
-            hipLaunchKernelGGL(ba_finalize_kernel, dim3(d.E + d.K * kGramSplit), dim3(kThreads), 0, s, d, a, w, 0);
-            VH_LAUNCH_CHECK("ba_finalize_kernel");
-        }
-        hipLaunchKernelGGL(ba_assemble_kernel, dim3(d.nlb + d.P + 1 + npad), dim3(64), 0, s, d, a, w, motion ? 1 : 0);
-        VH_LAUNCH_CHECK("ba_assemble_kernel");
-        hipLaunchKernelGGL(chol_factor_kernel, dim3(1), dim3(256), 0, s, w.A, w.Li, d.ld, 0, w.flag);
-        VH_LAUNCH_CHECK("chol_factor_kernel");
-        for (int kb = 0; kb + 1 < d.nb; kb++) {
-            const int rem = d.nb - kb - 1;
-            hipLaunchKernelGGL(chol_step_kernel, dim3(rem * (rem + 1) / 2), dim3(256), 0, s, w.A, w.Li, w.b, w.y, d.ld,
-                               kb, w.flag);
-            VH_LAUNCH_CHECK("chol_step_kernel");
-        }
-        hipLaunchKernelGGL(chol_backsolve_kernel, dim3(1), dim3(kBT), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.y, w.x,
-                           w.flag, d.P, dx);
-        VH_LAUNCH_CHECK("chol_backsolve_kernel");
-        if (!motion && d.K > 0) {
-            const dim3 bgrid((d.HW + kThreads - 1) / kThreads, d.K);
-            if (intr) hipLaunchKernelGGL(ba_backsub_kernel<true>, bgrid, dim3(kThreads), 0, s, d, a, w);
-            else hipLaunchKernelGGL(ba_backsub_kernel<false>, bgrid, dim3(kThreads), 0, s, d, a, w);
-            VH_LAUNCH_CHECK("ba_backsub_kernel");
-        }
-        hipLaunchKernelGGL(ba_retract_kernel, dim3((d.P + 63) / 64), dim3(64), 0, s, poses, intrinsics, w.x, d.t0, d.P,
-                           intr ? 1 : 0, intrinsics_scale);
-        VH_LAUNCH_CHECK("ba_retract_kernel");
-    }
-    return VH_OK;
+    if (sim3) return ba_iterations<7>(d, a, w, iterations, motion, intr, rk, poses, intrinsics, dx, intrinsics_scale, s);
+    return ba_iterations<6>(d, a, w, iterations, motion, intr, rk, poses, intrinsics, dx, intrinsics_scale, s);
 }
 
 int vh_ba_run(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics, const float* disps_sens,
@@ -2579,6 +2725,7 @@ int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps
     if (!plan) return fail(VH_ERR_ARG, "ba_adjoint: null plan");
     const bool motion = (flags & VH_BA_MOTION_ONLY) != 0;
     if (flags & VH_BA_OPTIMIZE_INTRINSICS) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: intrinsics optimisation");
+    if (flags & VH_BA_SIM3) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: Sim3 poses (SE3 only)");
     if (!poses || !disps || !intrinsics || !targets || !weights || !depth_active || !grad_dx || !workspace ||
         (!motion && !grad_dz) || (plan->d.E > 0 && (!grad_targets || !grad_weights)))
         return fail(VH_ERR_ARG, "ba_adjoint: null tensor pointer");

@@ -12,7 +12,9 @@
 [|kx|, H*W] from the last iteration (``None`` when motion only or when no
 iteration ran, as the reference returns undefined tensors), ``energy`` has
 ``iterations`` entries when requested and is empty otherwise
-(geom_kernels.cu:1626-1847).  Non-contiguous inputs raise
+(geom_kernels.cu:1626-1847).  Beyond the reference, ``poses`` may also be
+Sim3 [N,8] (t, q, s): the same step with 7-dof pose blocks, ``dx`` [t1-t0, 7]
+and the retraction of lietorch's Sim3.  Non-contiguous inputs raise
 ``RuntimeError("<name> must be contiguous")`` (geom_kernels.cu:30-31,
 1634-1642).
 
@@ -142,8 +144,9 @@ def _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii
     N, H, W = disps.shape
     E = ii.shape[0]
     t0, t1, iterations = int(t0), int(t1), int(iterations)
-    if poses.shape != (N, 7):
-        raise RuntimeError(f"poses must be [N,7] SE3 (got {tuple(poses.shape)})")
+    if poses.shape not in ((N, 7), (N, 8)):
+        raise RuntimeError(f"poses must be [N,7] SE3 or [N,8] Sim3 (got {tuple(poses.shape)})")
+    sim3 = poses.shape[1] == 8
     if targets.shape != (E, 2, H, W) or weights.shape != (E, 2, H, W):
         raise RuntimeError("targets/weights must be [E,2,H,W]")
     plan = get_plan(ii, jj, N, H, W, t0, t1)
@@ -155,10 +158,11 @@ def _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii
     energy = torch.empty((iterations if compute_energy else 0,), dtype=torch.float32, device=dev)
     if iterations <= 0:
         return None, None, energy, plan, None
-    dx = torch.empty((P, 6), dtype=torch.float32, device=dev)
+    dx = torch.empty((P, 7 if sim3 else 6), dtype=torch.float32, device=dev)
     dz = None if motion_only else torch.empty((plan.K, H * W), dtype=torch.float32, device=dev)
     ws = torch.empty((max(plan.ws_bytes, 1),), dtype=torch.uint8, device=dev)
-    flags = (1 if motion_only else 0) | (2 if optimize_intrinsics else 0) | (4 if compute_energy else 0)
+    flags = (1 if motion_only else 0) | (2 if optimize_intrinsics else 0) | (4 if compute_energy else 0) | \
+        (8 if sim3 else 0)
     lib = _lib.load()
     rk = (0, 0.0, 0.0, 0.0) if robust is None else tuple(robust)
     with torch.cuda.device(dev):
@@ -185,11 +189,12 @@ def _ba_extended(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii
 def run_planned(plan, ws, poses, disps, intrinsics, disps_sens, targets, weights, eta, depth_active, iterations,
                 lm, ep, alpha, dx, dz, motion_only=False):
     """``iterations`` GN iterations of ``ba_extended`` on a prepared plan with
-    caller-allocated workspace and outputs (dx [P,6], dz [|kx|,H*W] or None).
+    caller-allocated workspace and outputs (dx [P,6] or [P,7] for Sim3 poses
+    [N,8], dz [|kx|,H*W] or None).
     It allocates nothing, copies nothing to the host and launches on torch's
     current stream only, so it may be captured into a graph.  The inputs must be
     contiguous float32 device tensors of the plan's shapes (not re-checked)."""
-    flags = 1 if motion_only else 0
+    flags = (1 if motion_only else 0) | (8 if poses.shape[-1] == 8 else 0)
     rc = _lib.load().vh_ba_run(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics), _lib.ptr(disps_sens),
                                _lib.ptr(targets), _lib.ptr(weights), None if motion_only else _lib.ptr(eta),
                                _lib.ptr(depth_active), int(iterations), float(lm), float(ep), float(alpha), 1e-6,
@@ -204,6 +209,8 @@ def ba_adjoint(plan, ws, poses, disps, intrinsics, targets, weights, depth_activ
     returns [dL/dtargets, dL/dweights] ([E,2,H,W] float32) from dL/d(dx) [P,6]
     and dL/d(dz) [|kx|,H*W].  poses / disps / intrinsics are the values before
     that iteration.  This package's extension: the reference BA has no backward."""
+    if poses.shape[-1] != 7:
+        raise RuntimeError("slam_ext.ba_adjoint: SE3 poses [N,7] only")
     E = targets.shape[0]
     gt = torch.empty_like(targets)
     gw = torch.empty_like(weights)
