@@ -1,18 +1,34 @@
 #!/usr/bin/env python3
 """Frame-graph step throughput of the SLAM inner loop on one MI355X.
 
-Workload: 64 keyframes, 384 edges, 64x80 maps (1/8 of 512x640), 128-channel
-float32 features, fx=fy=80, cx=40, cy=32, seed 0 (vipe_amd.synth
-"kf64_e384_64x80").  One step = one correlation lookup (4-level pyramid,
-radius 3, all 384 edges -> [384, 196, 64, 80]) + 8 Gauss-Newton iterations of
-dense BA (``slam_ext.ba_extended``, FP64 reduced-camera Cholesky).  The
-pyramid is built once before timing (its own throughput is reported under
-``corr_build``).  Inputs are resident in HBM when timing starts.
+Workloads (vipe_amd.synth, seed 0): 64 keyframes, 384 edges, 64x80 maps (1/8
+of 512x640), 128-channel float32 features, fx=fy=80, cx=40, cy=32.
+``kf64_e384_64x80`` has SE3 poses, ``kf64_e384_64x80_sim3`` the same graph
+with Sim3 poses (log-scale N(0, 0.02)).  One step = one correlation lookup
+(4-level pyramid, radius 3, all 384 edges -> [384, 196, 64, 80]) + 8
+Gauss-Newton iterations of dense BA (``slam_ext.ba_extended``, FP64
+reduced-camera Cholesky).  The pyramid is built once before timing (its time
+is reported as ``corr_build_ms``).  Inputs are resident in HBM when timing
+starts.
+
+The line carries the whole metric:
+- ``value``: GN iterations/s of the plain FP32 step on the SE3 workload;
+- ``mixed``: the step on the Sim3 workload with an FP16 volume built on FP16
+  matrix cores (FP32 accumulation), FP32 Jacobians and Hessian, FP64 Cholesky
+  and retraction; its speed-up over the FP32 step, with and without the
+  pyramid build, and over an FP32 step on the same Sim3 workload
+  (``fp32_sim3``);
+- each run's error against the FP64 oracle: BA reprojection (px) and
+  relative pose error after the 8 iterations, and the lookup's largest error
+  on three sampled edges (first, middle, last);
+- ``roofline`` (the lookup kernel, the step's dominant kernel),
+  ``roofline_ba`` (one GN iteration) and ``roofline_adjoint`` (one GN
+  iteration plus its implicit adjoint);
+- ``cpu_baseline``: the CPU restatement on the box's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fp32|mixed]
 
-Prints one JSON line.  ``value`` = BA GN iterations/s of the combined step;
-``corr_lookups_per_s`` = lookups/s of the same run.
+``--mode mixed`` makes the mixed run the headline ``value``.
 """
 from __future__ import annotations
 
@@ -27,9 +43,9 @@ import torch
 
 METRIC = "BA GN iters/s + corr lookups/s at 64 KF; mixed vs FP32 GPU at equal px error"
 HBM_PEAK = 8.0e12
-BF16_PEAK = 2.5e15
-FP32_PEAK = 157.3e12
 GN_ITERS = 8
+SE3_WORKLOAD = "kf64_e384_64x80"
+SIM3_WORKLOAD = "kf64_e384_64x80_sim3"
 
 
 def parse():
@@ -37,10 +53,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["fp32", "mixed"], default="fp32")
-    ap.add_argument("--workload", default="kf64_e384_64x80")
+    ap.add_argument("--mode", choices=["fp32", "mixed"], default="fp32",
+                    help="which run is the headline value (both always run)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-edges", type=int, default=32, help="edges of the bounded CPU lookup sample")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 oracle comparisons")
+    ap.add_argument("--only", choices=["fp32", "mixed"], default=None,
+                    help="run one mode only (profiling); the line then lacks the comparison")
     ap.add_argument("--build-variants", action="store_true",
                     help="also time the volume build with f16 / bf16 / fp8 contractions and report their error")
     return ap.parse_args()
@@ -60,6 +78,12 @@ def ba_bytes(E, N, H, W):
     return E * 4 * H * W * 4 + N * H * W * 4 * 4
 
 
+def adjoint_bytes(E, N, K, H, W):
+    """Compulsory bytes of the implicit adjoint of one iteration: targets +
+    weights read, their two gradients written, disps and dL/d(dz) read."""
+    return 2 * E * 4 * H * W * 4 + N * H * W * 4 + K * H * W * 4
+
+
 def profiled_traffic(kernel: str):
     """HBM bytes per dispatch of ``kernel`` from the committed rocprofv3 summaries
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration); None when no
@@ -77,137 +101,299 @@ def profiled_traffic(kernel: str):
     return None
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
-              "(how the path would shard over several GPUs is described, not implemented).", file=sys.stderr)
-        sys.exit(2)
+def _events():
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _ba(d, depth_active, g, iters):
+    from vipe_amd.ext import slam_ext
+
+    return slam_ext.ba_extended(d["poses"], d["disps"], d["intrinsics"], d["disps_sens"], d["targets"],
+                                d["weights"], d["eta"], d["ii"], d["jj"], depth_active, g.t0, g.t1, iters, 1e-4,
+                                0.1, False, 0.05, False, 1e-6, 1e-6, 1.0, False)
+
+
+def gpu_run(workload, mode, steps, warmup, build_variants=False):
+    """Times the step of one (workload, mode) and returns its figures plus what
+    the accuracy check needs (the lookup output of the sampled edges and the
+    state after one step from the initial poses)."""
     from vipe_amd import synth
     from vipe_amd.corr import CorrBlock
-    from vipe_amd.ext import droid_net_ext, slam_ext
 
     dev = torch.device("cuda", 0)
-    g = synth.make_graph(a.workload, seed=0)
+    s = torch.cuda.current_stream()
+    g = synth.make_graph(workload, seed=0)
     N, H, W = g.disps.shape
     E = g.ii.shape[0]
     d = {k: (v.to(dev) if torch.is_tensor(v) else v) for k, v in g.__dict__.items()}
     depth_active = torch.ones(N, device=dev)
-    vol_dtype = torch.float32 if a.mode == "fp32" else torch.float16
+    vol_dtype = torch.float32 if mode == "fp32" else torch.float16
     fm = d["fmaps"]
     f1 = fm[d["ii"]].contiguous()[None]
     f2 = fm[d["jj"]].contiguous()[None]
-    if a.mode == "mixed":
+    if mode == "mixed":
         f1, f2 = f1.half(), f2.half()
 
-    # ---- pyramid build (timed separately, after a warm build so that the
-    # allocator's cached blocks serve the pyramid and workspace) ----
-    s = torch.cuda.current_stream()
+    # pyramid build, timed after a warm build so that the allocator's cached
+    # blocks serve the pyramid and its workspace
     corr = CorrBlock(f1, f2, num_levels=4, radius=3, volume_dtype=vol_dtype)
     del corr
     torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0, e1 = _events()
     e0.record(s)
     corr = CorrBlock(f1, f2, num_levels=4, radius=3, volume_dtype=vol_dtype)
     e1.record(s)
     torch.cuda.synchronize()
     build_ms = e0.elapsed_time(e1)
-    build_variants = build_variant_report(f1, f2, vol_dtype) if a.build_variants else None
+    variants = build_variant_report(f1, f2, vol_dtype) if build_variants else None
     del f1, f2
     coords = d["coords"][None]  # [1, E, H, W, 2]
-
-    poses0 = d["poses"].clone()
-    disps0 = d["disps"].clone()
+    poses0, disps0 = d["poses"].clone(), d["disps"].clone()
     lk_events = []
 
     def step(record=False):
         if record:
-            ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ea, eb = _events()
             ea.record(s)
         out = corr(coords)
         if record:
             eb.record(s)
             lk_events.append((ea, eb))
-        slam_ext.ba_extended(d["poses"], d["disps"], d["intrinsics"], d["disps_sens"], d["targets"], d["weights"],
-                             d["eta"], d["ii"], d["jj"], depth_active, g.t0, g.t1, GN_ITERS, 1e-4, 0.1, False, 0.05,
-                             False, 1e-6, 1e-6, 1.0, False)
+        _ba(d, depth_active, g, GN_ITERS)
         return out
 
-    for _ in range(a.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
     d["poses"].copy_(poses0)
     d["disps"].copy_(disps0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0, ev1 = _events()
     ev0.record(s)
-    for _ in range(a.steps):
+    for _ in range(steps):
         step(record=True)
     ev1.record(s)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
-    ms_per_step = gpu_ms / a.steps
     lk_ms = sum(x.elapsed_time(y) for x, y in lk_events) / len(lk_events)
     elem = 4 if vol_dtype == torch.float32 else 2
-    lk_alg = lookup_bytes(E, H, W, 4, elem)
-    lk_gbs = lk_alg / (lk_ms * 1e-3)
-    lk_kernel = (f"corr_lookup_tiled_kernel<{'float' if elem == 4 else '__half'}, 3>" if corr.layout == "tiled"
-                 else f"corr_lookup_pyramid_kernel<{'float' if elem == 4 else '__half'}, 3, false>")
 
-    # BA-only timing (same inputs, separate region)
+    # BA alone (same inputs, separate region)
     d["poses"].copy_(poses0)
     d["disps"].copy_(disps0)
     torch.cuda.synchronize()
-    eb0, eb1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    eb0, eb1 = _events()
     eb0.record(s)
-    nba = max(3, a.steps // 2)
+    nba = max(3, steps // 2)
     for _ in range(nba):
-        slam_ext.ba_extended(d["poses"], d["disps"], d["intrinsics"], d["disps_sens"], d["targets"], d["weights"],
-                             d["eta"], d["ii"], d["jj"], depth_active, g.t0, g.t1, GN_ITERS, 1e-4, 0.1, False, 0.05,
-                             False, 1e-6, 1e-6, 1.0, False)
+        _ba(d, depth_active, g, GN_ITERS)
     eb1.record(s)
     torch.cuda.synchronize()
     ba_ms_per_iter = eb0.elapsed_time(eb1) / (nba * GN_ITERS)
 
-    value = GN_ITERS * a.steps / (gpu_ms * 1e-3)
+    # the state after one step from the initial poses, and the lookup of the
+    # sampled edges (for the accuracy check, outside the timed regions)
+    d["poses"].copy_(poses0)
+    d["disps"].copy_(disps0)
+    sample = torch.tensor([0, E // 2, E - 1])
+    out = corr(coords)
+    lk_sample = out[0, sample.to(dev)].float().cpu()
+    del out
+    _ba(d, depth_active, g, GN_ITERS)
+    torch.cuda.synchronize()
+    state = (d["poses"].cpu(), d["disps"].cpu())
+    lk_kernel = (f"corr_lookup_tiled_kernel<{'float' if elem == 4 else '__half'}, 3>" if corr.layout == "tiled"
+                 else f"corr_lookup_pyramid_kernel<{'float' if elem == 4 else '__half'}, 3, false>")
+    del corr
+    torch.cuda.empty_cache()
+    return {
+        "workload": workload, "mode": mode, "graph": g, "ms_per_step": gpu_ms / steps, "gpu_ms": gpu_ms,
+        "wall": wall, "lookup_ms": lk_ms, "ba_ms_per_iter": ba_ms_per_iter, "build_ms": build_ms,
+        "lookup_bytes": lookup_bytes(E, H, W, 4, elem), "lookup_kernel": lk_kernel, "vol_dtype": vol_dtype,
+        "lk_sample_edges": sample, "lk_sample": lk_sample, "state": state, "build_variants": variants,
+    }
+
+
+def adjoint_run(workload, reps=10):
+    """Time of one GN iteration (forward) followed by its implicit adjoint
+    (vh_ba_adjoint), averaged over ``reps`` pairs on the same inputs."""
+    from vipe_amd import synth
+    from vipe_amd.ext import slam_ext
+
+    dev = torch.device("cuda", 0)
+    s = torch.cuda.current_stream()
+    g = synth.make_graph(workload, seed=0, with_fmaps=False)
+    N, H, W = g.disps.shape
+    E = g.ii.shape[0]
+    d = {k: (v.to(dev) if torch.is_tensor(v) else v) for k, v in g.__dict__.items()}
+    da = torch.ones(N, device=dev)
+    p0, d0 = d["poses"].clone(), d["disps"].clone()
+    gen = torch.Generator().manual_seed(1)
+    gdx = torch.randn(g.t1 - g.t0, 6, generator=gen).to(dev)
+
+    def pair(adjoint=True):
+        p, dd = p0.clone(), d0.clone()
+        dx, dz, _, plan, ws = slam_ext._ba_extended(p, dd, d["intrinsics"].clone(), d["disps_sens"], d["targets"],
+                                                    d["weights"], d["eta"], d["ii"], d["jj"], da, g.t0, g.t1, 1,
+                                                    1e-4, 0.1, False, 0.05, False, 1e-6, 1e-6, 1.0, False)
+        if adjoint:
+            slam_ext.ba_adjoint(plan, ws, p0, d0, d["intrinsics"], d["targets"], d["weights"], da, 1e-4, False, gdx,
+                                torch.ones_like(dz))
+        return plan
+
+    plan = pair()
+    torch.cuda.synchronize()
+    times = {}
+    for name, adj in (("forward", False), ("forward_adjoint", True)):
+        ea, eb = _events()
+        ea.record(s)
+        for _ in range(reps):
+            pair(adj)
+        eb.record(s)
+        torch.cuda.synchronize()
+        times[name] = ea.elapsed_time(eb) / reps
+    alg = ba_bytes(E, N, H, W) + adjoint_bytes(E, N, plan.K, H, W)
+    return times, alg
+
+
+def accuracy(run, oracle_cache):
+    """Errors of one run against the FP64 oracle: BA after one step (8
+    iterations) from the initial state, and the lookup on the sampled edges."""
+    from oracle import ba as OB
+    from oracle import corr as OC
+    from vipe_amd.synth import reproject
+
+    g = run["graph"]
+    key = run["workload"]
+    if key not in oracle_cache:
+        N = g.disps.shape[0]
+        rp, rd = g.poses.clone(), g.disps.clone()
+        OB.ba_extended(rp, rd, g.intrinsics.clone(), g.disps_sens, g.targets, g.weights, g.eta, g.ii, g.jj,
+                       torch.ones(N), g.t0, g.t1, GN_ITERS, 1e-4, 0.1, False, 0.05, False, 1e-6, 1e-6, 1.0, False)
+        sel = run["lk_sample_edges"]
+        f1 = g.fmaps[g.ii[sel]][None].double()
+        f2 = g.fmaps[g.jj[sel]][None].double()
+        ref_lk = OC.corr_block_lookup(OC.corr_pyramid(OC.corr_volume(f1, f2), 4), g.coords[sel][None].double(), 3)[0]
+        oracle_cache[key] = (rp, rd, ref_lk)
+    rp, rd, ref_lk = oracle_cache[key]
+    p, dd = run["state"]
+    ca, _ = reproject(p, dd, g.intrinsics, g.ii, g.jj)
+    cb, _ = reproject(rp, rd, g.intrinsics, g.ii, g.jj)
+    px = (ca - cb).abs().max().item()
+    rel_pose = ((p.double() - rp.double()).norm(dim=1) / rp.double().norm(dim=1)).max().item()
+    lk_err = (run["lk_sample"].double() - ref_lk).abs().max().item()
+    return {"ba_px_err_vs_fp64": float(f"{px:.3e}"), "ba_rel_pose_err_vs_fp64": float(f"{rel_pose:.3e}"),
+            "lookup_max_err_vs_fp64": float(f"{lk_err:.3e}"),
+            "lookup_max_abs": float(f"{ref_lk.abs().max().item():.3e}")}
+
+
+def summary(run):
+    return {
+        "workload": run["workload"],
+        "ms_per_step": round(run["ms_per_step"], 4),
+        "gn_iters_per_s": round(GN_ITERS / (run["ms_per_step"] * 1e-3), 2),
+        "corr_lookups_per_s": round(1e3 / run["ms_per_step"], 2),
+        "lookup_ms": round(run["lookup_ms"], 4),
+        "ba_ms_per_gn_iter": round(run["ba_ms_per_iter"], 4),
+        "corr_build_ms": round(run["build_ms"], 3),
+        "volume_dtype": str(run["vol_dtype"]).replace("torch.", ""),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(how the path would shard over several GPUs is described, not implemented).", file=sys.stderr)
+        sys.exit(2)
+    runs = {}
+    if a.only in (None, "fp32"):
+        runs["fp32"] = gpu_run(SE3_WORKLOAD, "fp32", a.steps, a.warmup, a.build_variants)
+    if a.only in (None, "mixed"):
+        runs["mixed"] = gpu_run(SIM3_WORKLOAD, "mixed", a.steps, a.warmup)
+    if a.only is None:
+        runs["fp32_sim3"] = gpu_run(SIM3_WORKLOAD, "fp32", a.steps, a.warmup)
+    adj_times, adj_alg = adjoint_run(SE3_WORKLOAD)
+
+    head = runs[a.mode] if a.mode in runs else next(iter(runs.values()))
+    g = head["graph"]
+    N, H, W = g.disps.shape
+    E = g.ii.shape[0]
+    lk_gbs = head["lookup_bytes"] / (head["lookup_ms"] * 1e-3)
+    ba_alg = ba_bytes(E, N, H, W)
+    ba_gbs = ba_alg / (head["ba_ms_per_iter"] * 1e-3)
+    adj_gbs = adj_alg / (adj_times["forward_adjoint"] * 1e-3)
     result = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": round(GN_ITERS / (head["ms_per_step"] * 1e-3), 2),
         "unit": "GN iters/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if a.mode == "fp32" else "f16-volume/f32/f64",
+        "dtype": "f32" if head["mode"] == "fp32" else "f16-volume/f32/f64",
         "data": "synthetic (seeded frame graph, random features)",
-        "config": {"workload": f"{a.workload}: 64 KF, 384 edges, 64x80, 128-ch, 4-level r=3 lookup + "
-                               f"{GN_ITERS} GN iters (FP64 Cholesky)", "mode": a.mode,
-                   "volume_dtype": str(vol_dtype).replace("torch.", "")},
-        "corr_lookups_per_s": round(a.steps / (gpu_ms * 1e-3), 2),
-        "lookup_ms": round(lk_ms, 4),
-        "ba_ms_per_gn_iter": round(ba_ms_per_iter, 4),
-        "ba_gn_iters_per_s_alone": round(1e3 / ba_ms_per_iter, 1),
-        "corr_build_ms": round(build_ms, 3),
-        "corr_build_tflops": round(2.0 * E * (H * W) ** 2 * 128 / (build_ms * 1e-3) / 1e12, 2),
-        "corr_build_variants": build_variants,
-        "wall_s": round(wall, 3),
-        "roofline": {
-            "kernel": lk_kernel,
-            "bound": "hbm",
-            "achieved": round(lk_gbs / 1e9, 1),
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": round(lk_gbs / HBM_PEAK, 4),
-            "traffic": profiled_traffic(lk_kernel),
-        },
+        "config": {"workload": f"{head['workload']}: 64 KF, 384 edges, 64x80, 128-ch, 4-level r=3 lookup + "
+                               f"{GN_ITERS} GN iters (FP64 Cholesky)", "mode": head["mode"]},
+    }
+    result.update({k: v for k, v in summary(head).items() if k != "workload"})
+    if "fp32" in runs and "mixed" in runs:
+        f, m, fs = runs["fp32"], runs["mixed"], runs["fp32_sim3"]
+        result["mixed"] = dict(summary(m), **{
+            "dtype": "f16 volume (f16 MFMA build, f32 accumulation), f32 Jacobians/Hessian, f64 Cholesky + retraction",
+            "speedup_vs_fp32": round(f["ms_per_step"] / m["ms_per_step"], 3),
+            "speedup_vs_fp32_same_poses": round(fs["ms_per_step"] / m["ms_per_step"], 3),
+            "speedup_vs_fp32_with_build": round((f["ms_per_step"] + f["build_ms"]) / (m["ms_per_step"] + m["build_ms"]),
+                                                3),
+        })
+        result["fp32_sim3"] = summary(fs)
+    if not a.no_accuracy:
+        cache = {}
+        for name, run in runs.items():
+            acc = accuracy(run, cache)
+            if run is head:
+                result.update(acc)
+            elif name in result:
+                result[name].update(acc)
+    if head["build_variants"] is not None:
+        result["corr_build_variants"] = head["build_variants"]
+    result["wall_s"] = round(head["wall"], 3)
+    result["roofline"] = {
+        "kernel": head["lookup_kernel"],
+        "bound": "hbm",
+        "achieved": round(lk_gbs / 1e9, 1),
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": round(lk_gbs / HBM_PEAK, 4),
+        "traffic": profiled_traffic(head["lookup_kernel"]),
+    }
+    result["roofline_ba"] = {
+        "kernel": "one BA GN iteration (all its kernels)",
+        "bound": "hbm",
+        "achieved": round(ba_gbs / 1e9, 1),
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": round(ba_gbs / HBM_PEAK, 4),
+        "algorithmic_bytes": ba_alg,
+    }
+    result["roofline_adjoint"] = {
+        "kernel": "one BA GN iteration + its implicit adjoint (vh_ba_adjoint), SE3 workload",
+        "bound": "hbm",
+        "ms": round(adj_times["forward_adjoint"], 4),
+        "forward_ms": round(adj_times["forward"], 4),
+        "achieved": round(adj_gbs / 1e9, 1),
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": round(adj_gbs / HBM_PEAK, 4),
+        "algorithmic_bytes": adj_alg,
     }
     if not a.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(g, a.cpu_edges)
+        result["cpu_baseline"] = cpu_baseline(runs.get("fp32", head)["graph"])
     print(json.dumps(result), flush=True)
 
 
@@ -230,7 +416,7 @@ def build_variant_report(f1, f2, vol_dtype, sample_edges=2):
         lv = droid_net_ext.corr_build(x1, x2, 4, out_dtype=vol_dtype, contraction_dtype=mma)
         del lv
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0, e1 = _events()
         e0.record(s)
         lv = droid_net_ext.corr_build(x1, x2, 4, out_dtype=vol_dtype, contraction_dtype=mma)
         e1.record(s)
@@ -244,7 +430,7 @@ def build_variant_report(f1, f2, vol_dtype, sample_edges=2):
     return out
 
 
-def cpu_baseline(g, n_edges):
+def cpu_baseline(g, n_edges=32):
     """Oracle restatement (float64 torch on the host CPU) on a bounded sample of
     the same workload: 1 GN iteration of the full 384-edge graph and one
     4-level lookup (pyramid built outside the timer, as on the GPU) over
