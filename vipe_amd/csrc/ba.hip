@@ -445,12 +445,36 @@ __host__ __device__ inline bool gram_fits(int Rext, int pd) {
     return (np + wpk - 1) / wpk <= kGramSlots;
 }
 
-// dynamic LDS of the frame kernel: the E rows of a sub-tile, or the Gram
+// E rows a frame stores in LDS (see RowMap): at most 6 Rext + pd - 9 (a frame
+// with its self block) or 6 Rext - 3 (without)
+__host__ __device__ inline int frame_phys_rows(int Rext, int pd) {
+    return max(6 * Rext + pd - 9, 6 * Rext - 3);
+}
+
+// dynamic LDS of the frame kernel: the stored E rows of a sub-tile, or the Gram
 // reduction buffer of the largest frame, whichever is larger
 inline size_t frame_lds_bytes(int Rext_max, int TP, int pd) {
-    size_t fl = (size_t)Rext_max * pd * (TP + kRowPad);
+    size_t fl = (size_t)frame_phys_rows(Rext_max, pd) * (TP + kRowPad);
     for (int r = 1; r <= Rext_max; r++) fl = std::max(fl, (size_t)gram_red_floats(r, pd));
     return fl * 4;
+}
+
+// The Gram works on Rext blocks of PD logical E rows, but LDS stores only the
+// rows that can be non-zero: the self block's PD rows, the first 6 rows of each
+// edge block (an edge's Sim3 sigma entry is zero) and the pseudo block's two
+// rows [w, Ef]; every other logical row reads one shared zero row.  Layout:
+// self rows [0, hs PD), edge s at hs PD + 6 s, pseudo at pbase = hs PD + 6 deg,
+// the zero row at pbase + 2.
+struct RowMap {
+    int hs, R, pbase, zrow;
+};
+
+template <int PD>
+__device__ __forceinline__ int phys_row(const RowMap& m, int r) {
+    const int blk = r / PD, n = r - blk * PD;
+    if (m.hs && blk == 0) return n;
+    if (blk < m.R) return n < 6 ? m.hs * PD + (blk - m.hs) * 6 + n : m.zrow;
+    return n < 2 ? m.pbase + n : m.zrow;
 }
 
 // One sub-tile's Gram pass of a wave that owns S tile pairs (pair0 + pstride j):
@@ -462,16 +486,17 @@ inline size_t frame_lds_bytes(int Rext_max, int TP, int pd) {
 // the only entries it touches.  The loads of a group of up to four
 // pairs are issued before its MFMAs, which interleave over the group's
 // independent accumulators.
-template <int S>
+template <int S, int PD>
 __device__ __forceinline__ void gram_pass(f32x4 (&acc)[kGramSlots], const float* dyn, const float* sQ, int q0, int q1,
-                                          int pair0, int pstride, int nT, int nrows, int ldr, int gi, int gk) {
+                                          int pair0, int pstride, int nT, int nrows, const RowMap& rm, int ldr, int gi,
+                                          int gk) {
     int oA[S], oB[S];
 #pragma unroll
     for (int j = 0; j < S; j++) {
         int ti, tj;
         gram_rows(pair0 + pstride * j, nT, ti, tj);
-        oA[j] = min(16 * ti + gi, nrows - 1) * ldr + 4 * gk;
-        oB[j] = min(16 * tj + gi, nrows - 1) * ldr + 4 * gk;
+        oA[j] = phys_row<PD>(rm, min(16 * ti + gi, nrows - 1)) * ldr + 4 * gk;
+        oB[j] = phys_row<PD>(rm, min(16 * tj + gi, nrows - 1)) * ldr + 4 * gk;
     }
     for (int q = q0; q < q1; q += 16) {
         const float4 qv = *reinterpret_cast<const float4*>(&sQ[q + 4 * gk]);
@@ -509,7 +534,7 @@ __device__ __forceinline__ void gram_pass(f32x4 (&acc)[kGramSlots], const float*
 // ---------------------------------------------------------------------------
 template <bool INTR, bool MOTION, bool ROBUST = false, int PD = 6>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void ba_frame_kernel(DevPlan p, RunArgs a, Work w) {
-    extern __shared__ f32x4 dyn4[];  // rows [Rext_max*PD][TP+kRowPad]; reused for the Gram reduction
+    extern __shared__ f32x4 dyn4[];  // stored E rows (RowMap) [.][TP+kRowPad]; reused for the Gram reduction
     float* dyn = reinterpret_cast<float*>(dyn4);
     __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo(12) edge(13) scale(14)
     __shared__ __align__(16) float sQ[256];
@@ -558,6 +583,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int gklen = TP / gks;
     const int gnslot = (gnp - wv % gwpk + gwpk - 1) / gwpk;
     const int gi = lane & 15, gk = lane >> 4;
+    const RowMap rmap{has_self, R, has_self * PD + 6 * deg, has_self * PD + 6 * deg + 2};
     f32x4 gacc[kGramSlots];
 #pragma unroll
     for (int j = 0; j < kGramSlots; j++) gacc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -656,7 +682,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             }
             if (!MOTION && tid < TP) {
 #pragma unroll
-                for (int n = 0; n < PD; n++) dyn[((has_self + s) * PD + n) * ldr + tid] = er[n];
+                for (int n = 0; n < 6; n++) dyn[(has_self * PD + 6 * s + n) * ldr + tid] = er[n];
             }
             // this wave's sums of the edge's partials; its LDS slot is its own, and
             // a wavefront's LDS accesses complete in order, so no barrier is needed
@@ -688,10 +714,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
                     for (int n = 0; n < PD; n++) dyn[n * ldr + tid] = es[n];
                 }
-                dyn[(R * PD + 0) * ldr + tid] = wt;
-                dyn[(R * PD + 1) * ldr + tid] = Eft;
-#pragma unroll
-                for (int n = 2; n < PD; n++) dyn[(R * PD + n) * ldr + tid] = 0.f;
+                dyn[(rmap.pbase + 0) * ldr + tid] = wt;
+                dyn[(rmap.pbase + 1) * ldr + tid] = Eft;
+                dyn[rmap.zrow * ldr + tid] = 0.f;
                 sQ[tid] = Q;
             }
             __syncthreads();
@@ -702,7 +727,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             switch (gnslot) {
 #define VH_GRAM_CASE(S)                                                                                    \
     case S:                                                                                                \
-        gram_pass<S>(gacc, dyn, sQ, qb, qb + gklen, wv % gwpk, gwpk, gnT, gnrows, ldr, gi, gk);            \
+        gram_pass<S, PD>(gacc, dyn, sQ, qb, qb + gklen, wv % gwpk, gwpk, gnT, gnrows, rmap, ldr, gi, gk);  \
         break;
                 VH_GRAM_CASE(1) VH_GRAM_CASE(2) VH_GRAM_CASE(3) VH_GRAM_CASE(4) VH_GRAM_CASE(5) VH_GRAM_CASE(6)
                 VH_GRAM_CASE(7) VH_GRAM_CASE(8) VH_GRAM_CASE(9) VH_GRAM_CASE(10) VH_GRAM_CASE(11) VH_GRAM_CASE(12)
@@ -1898,32 +1923,44 @@ __global__ __launch_bounds__(kThreads) void ba_adj_rhs_kernel(DevPlan p, RunArgs
 }
 
 // Pass 2: ybar = gy - E~ t on the pose entries (quirk mask: local index >= 1),
-// zeros on the padding.  One thread per entry of the system.
-__global__ __launch_bounds__(256) void ba_adj_pose_kernel(DevPlan p, RunArgs a, AdjArgs g, int motion) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.ld) return;
-    if (i >= 6 * p.P) {
-        g.ybar[i] = 0.0;
+// zeros on the padding.  One wave per pose walks the pose's right-hand-side
+// list (the edges whose ii or jj is that pose): an edge reaching it as jj adds
+// its esum, as ii its kf-side row -Adj^T esum.  Blocks past P clear the padding.
+__global__ __launch_bounds__(64) void ba_adj_pose_kernel(DevPlan p, RunArgs a, AdjArgs g, int motion) {
+    const int l = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (l >= p.P) {
+        const int i = 6 * p.P + (l - p.P) * 64 + lane;
+        if (i < p.ld) g.ybar[i] = 0.0;
         return;
     }
-    const int l = i / 6, n = i % 6;
-    double acc = g.gdx[i];
+    double c[6] = {0, 0, 0, 0, 0, 0};
     if (!motion && l >= 1) {
-        for (int e = 0; e < p.E; e++) {
-            const int lj = p.jj[e] - p.t0, li = p.ii[e] - p.t0;
-            if (lj == l) acc -= g.esum[(int64_t)e * 6 + n];
-            if (li == l) {
+        const int beg = p.rhs_ptr[l], end = p.rhs_ptr[l + 1];
+        for (int it = beg + lane; it < end; it += 64) {
+            const int item = p.rhs_items[it];
+            const int kind = item >> 27, e = item & ((1 << 27) - 1);
+            if (kind != K_VI && kind != K_VJ) continue;
+            const double* es = g.esum + (int64_t)e * 6;
+            if (kind == K_VJ) {
+                for (int n = 0; n < 6; n++) c[n] += es[n];
+            } else {
                 // kf side: E_kf = -Adj^T er, so -E~ t adds Adj^T esum
                 float Rm[9], t[3], sc;
                 edge_transform<6>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
-                float es[6], o[6];
-                for (int q = 0; q < 6; q++) es[q] = (float)g.esum[(int64_t)e * 6 + q];
-                neg_adjT(Rm, t, 1.f, es, o);
-                acc -= (double)o[n];
+                float ef[6], o[6];
+                for (int q = 0; q < 6; q++) ef[q] = (float)es[q];
+                neg_adjT(Rm, t, 1.f, ef, o);
+                for (int n = 0; n < 6; n++) c[n] += (double)o[n];
             }
         }
     }
-    g.ybar[i] = acc;
+#pragma unroll
+    for (int n = 0; n < 6; n++) c[n] = wave_sum_f64(c[n]);
+    double mine = c[0];
+#pragma unroll
+    for (int n = 1; n < 6; n++) mine = lane == n ? c[n] : mine;
+    if (lane < 6) g.ybar[6 * l + lane] = (double)g.gdx[6 * l + lane] - mine;
 }
 
 // x = (L L^T)^-1 rhs with the stored factor (X_k = L_kk^-T in LinvT, L_ik^T in
@@ -2202,6 +2239,7 @@ struct vh_ba_plan {
     size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_y, o_Li, o_flag;
     size_t o_esum, o_ybar, o_mu;  // adjoint
     int gram_blocks = 0;          // Gram blocks over all depth frames
+    int TP7 = 64;                 // frame kernel pixel tile for Sim3 runs
     // plan variants: the system size depends on whether intrinsics are optimised
     int P;
     // every non-stereo edge of a depth frame reaches a different pose (the
@@ -2359,6 +2397,11 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         }
         TP = pixel_tile;
     }
+    // Sim3 frames store one more self row: their pixel tile is chosen on its own
+    int TP7 = 256;
+    while (TP7 > 64 && frame_lds_bytes(Rext_max, TP7, 7) > (size_t)kLdsBudget) TP7 /= 2;
+    if (pixel_tile != 0) TP7 = pixel_tile;
+    pl->TP7 = TP7;
     const int SUB = std::max(1, 256 / TP);  // 256-pixel tiles
     const int T = (HW + TP * SUB - 1) / (TP * SUB);
     // lower pose blocks and their contribution lists (fixed order)
@@ -2622,9 +2665,14 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
         // the plan was sized for SE3 blocks; 7-row blocks must still fit
         if (pd * d.P + 1 > kMaxLd)
             return fail(VH_ERR_UNSUPPORTED, "ba_run: %d Sim3 poses in the window (limit %d)", d.P, (kMaxLd - 1) / 7);
-        if (!gram_fits(d.Rext_max, pd) || frame_lds_bytes(d.Rext_max, d.TP, pd) > (size_t)kLdsMax)
+        if (!gram_fits(d.Rext_max, pd) || frame_lds_bytes(d.Rext_max, plan->TP7, pd) > (size_t)kLdsMax)
             return fail(VH_ERR_UNSUPPORTED, "ba_run: a depth frame has %d outgoing edges (Sim3 limit %d)",
                         plan->deg_max, kMaxDegSim3);
+    }
+    if (sim3) {
+        // 256-pixel tiles either way, so the tile count T (and every buffer) is unchanged
+        d.TP = plan->TP7;
+        d.SUB = std::max(1, 256 / d.TP);
     }
     d.pd = pd;
     d.GT = plan->gram_blocks * pd * pd;
@@ -2766,7 +2814,7 @@ int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps
         hipLaunchKernelGGL(ba_adj_rhs_kernel, pgrid, dim3(kThreads), 0, s, d, a, w, g);
         VH_LAUNCH_CHECK("ba_adj_rhs_kernel");
     }
-    hipLaunchKernelGGL(ba_adj_pose_kernel, dim3((d.ld + 255) / 256), dim3(256), 0, s, d, a, g,
+    hipLaunchKernelGGL(ba_adj_pose_kernel, dim3(d.P + (d.ld - 6 * d.P + 63) / 64), dim3(64), 0, s, d, a, g,
                        (motion || d.E == 0) ? 1 : 0);
     VH_LAUNCH_CHECK("ba_adj_pose_kernel");
     hipLaunchKernelGGL(chol_solve_full_kernel, dim3(1), dim3(1024), 0, s, w.A, w.Li, d.ld, d.nb, g.ybar, g.mu,
