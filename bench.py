@@ -24,7 +24,8 @@ The line carries the whole metric:
 - ``roofline`` (the lookup kernel, the step's dominant kernel),
   ``roofline_ba`` (one GN iteration) and ``roofline_adjoint`` (one GN
   iteration plus its implicit adjoint);
-- ``cpu_baseline``: the CPU restatement on the box's host cores.
+- ``cpu_baseline``: the reference's generic CPU path, restated in float32
+  torch (oracle/generic_ba.py), on the box's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fp32|mixed]
 
@@ -430,34 +431,79 @@ def build_variant_report(f1, f2, vol_dtype, sample_edges=2):
     return out
 
 
-def cpu_baseline(g, n_edges=32):
-    """Oracle restatement (float64 torch on the host CPU) on a bounded sample of
-    the same workload: 1 GN iteration of the full 384-edge graph and one
-    4-level lookup (pyramid built outside the timer, as on the GPU) over
-    ``n_edges`` edges, scaled to 384."""
-    from oracle import ba as OB
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def _generic_inputs(g):
+    """The graph in the generic path's layout: targets / weights [E, HW, 2] and
+    the per-frame damping whose 0.2 x + 1e-7 is the fused path's eta."""
+    N, H, W = g.disps.shape
+    E = g.ii.shape[0]
+    tgt = g.targets.permute(0, 2, 3, 1).reshape(E, H * W, 2).contiguous()
+    wt = g.weights.permute(0, 2, 3, 1).reshape(E, H * W, 2).contiguous()
+    damp = torch.zeros(N, H, W)
+    kx = torch.unique(torch.cat([torch.arange(g.t0, g.t1), g.ii]))
+    damp[kx] = (g.eta - 1e-7) / 0.2
+    return tgt, wt, damp
+
+
+def cpu_baseline(g, sample_edges=8):
+    """The reference's CPU path restated in float32 torch on the host cores
+    (oracle/generic_ba.py: the generic Solver with scipy's sparse LU; the
+    CorrBlock volume as a matmul, avg_pool2d pyramid and bilinear lookup).
+    ``value``: GN iterations/s of the 64-KF step (one 4-level lookup over all
+    384 edges + 8 GN iterations, pyramid built outside the step as on the
+    GPU), from a bounded sample: one full GN iteration and the lookup of
+    ``sample_edges`` edges scaled to 384.  ``kf8_e24_48x64``: the 8-KF step
+    (lookup + 2 GN iterations, and its pyramid build) timed in full."""
     from oracle import corr as OC
+    from oracle import generic_ba as G
+    from vipe_amd import synth
 
     threads = os.cpu_count() or 1
     torch.set_num_threads(threads)
-    N = g.disps.shape[0]
-    t0 = time.perf_counter()
-    OB.ba_extended(g.poses.clone(), g.disps.clone(), g.intrinsics.clone(), g.disps_sens, g.targets, g.weights,
-                   g.eta, g.ii, g.jj, torch.ones(N), g.t0, g.t1, 1, 1e-4, 0.1, False, 0.05, False, 1e-6, 1e-6, 1.0,
-                   False)
-    t_iter = time.perf_counter() - t0
-    sel = torch.arange(n_edges)
-    f1 = g.fmaps[g.ii[sel]][None]
-    f2 = g.fmaps[g.jj[sel]][None]
-    pyr = OC.corr_pyramid(OC.corr_volume(f1, f2), 4)
-    t1 = time.perf_counter()
-    OC.corr_block_lookup(pyr, g.coords[sel][None].double(), 3)
-    t_lk = (time.perf_counter() - t1) * g.ii.shape[0] / n_edges
-    step = GN_ITERS * t_iter + t_lk
+    f32 = torch.float32
+
+    def gn(graph, iters):
+        tgt, wt, damp = _generic_inputs(graph)
+        t0 = time.perf_counter()
+        G.bundle_adjustment(graph.poses.clone(), graph.disps.clone(), graph.intrinsics.clone(), graph.disps_sens, tgt,
+                            wt, damp, graph.ii, graph.jj, graph.t0, graph.t1, iters, 1e-4, 0.1, dtype=f32)
+        return time.perf_counter() - t0
+
+    def lookup(graph, sel):
+        f1 = graph.fmaps[graph.ii[sel]][None]
+        f2 = graph.fmaps[graph.jj[sel]][None]
+        t0 = time.perf_counter()
+        pyr = OC.corr_pyramid(OC.corr_volume(f1, f2, f32), 4)
+        t1 = time.perf_counter()
+        OC.corr_block_lookup(pyr, graph.coords[sel][None], 3, f32)
+        return t1 - t0, time.perf_counter() - t1
+
+    E = g.ii.shape[0]
+    t_gn = gn(g, 1)
+    _, t_lk = lookup(g, torch.arange(sample_edges))
+    step = GN_ITERS * t_gn + t_lk * E / sample_edges
+    g0 = synth.make_graph("kf8_e24_48x64", seed=0)
+    b0, l0 = lookup(g0, torch.arange(g0.ii.shape[0]))
+    t0_gn = gn(g0, 2)
     return {"value": round(GN_ITERS / step, 4), "unit": "GN iters/s", "cores": threads, "kind": "port",
-            "sample": f"1 GN iteration on the full graph ({t_iter:.2f} s) + lookup of {n_edges} of "
-                      f"{g.ii.shape[0]} edges scaled ({t_lk:.2f} s per lookup); float64 oracle; "
-                      f"cpu={platform.processor() or platform.machine()}"}
+            "sample": f"generic-path restatement, float32 torch + scipy spsolve on {_cpu_model()}: 1 GN iteration "
+                      f"of the 64-KF graph ({t_gn:.2f} s) x {GN_ITERS} + lookup of {sample_edges} of {E} edges "
+                      f"({t_lk:.2f} s) scaled to {E}",
+            "kf8_e24_48x64": {"ms_per_step": round(1e3 * (l0 + t0_gn), 1), "gn_iters_per_s": round(2 / (l0 + t0_gn), 3),
+                              "lookup_ms": round(1e3 * l0, 1), "gn_ms_per_iter": round(1e3 * t0_gn / 2, 1),
+                              "corr_build_ms": round(1e3 * b0, 1),
+                              "what": "8 KF, 24 edges, 48x64: 4-level r=3 lookup over all edges + 2 GN iterations, "
+                                      "timed in full"}}
 
 
 if __name__ == "__main__":

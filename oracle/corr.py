@@ -25,11 +25,12 @@ import torch.nn.functional as F
 F64 = torch.float64
 
 
-def corr_volume(fmap1: torch.Tensor, fmap2: torch.Tensor) -> torch.Tensor:
-    """fmap [B, N, C, H, W] -> [B, N, H, W, H, W] (droid_net.py:94-102)."""
+def corr_volume(fmap1: torch.Tensor, fmap2: torch.Tensor, dtype=F64) -> torch.Tensor:
+    """fmap [B, N, C, H, W] -> [B, N, H, W, H, W] (droid_net.py:94-102).
+    ``dtype`` float32 restates the reference's CPU arithmetic (the CPU baseline)."""
     B, N, C, H, W = fmap1.shape
-    f1 = fmap1.reshape(B * N, C, H * W).to(F64) / 4.0
-    f2 = fmap2.reshape(B * N, C, H * W).to(F64) / 4.0
+    f1 = fmap1.reshape(B * N, C, H * W).to(dtype) / 4.0
+    f2 = fmap2.reshape(B * N, C, H * W).to(dtype) / 4.0
     corr = torch.bmm(f1.transpose(1, 2).contiguous(), f2)
     return corr.view(B, N, H, W, H, W)
 
@@ -55,19 +56,19 @@ def _window(coord_x, coord_y, r):
     return x0f.long(), y0f.long(), dx, dy
 
 
-def corr_index_forward(volume: torch.Tensor, coords: torch.Tensor, r: int) -> torch.Tensor:
+def corr_index_forward(volume: torch.Tensor, coords: torch.Tensor, r: int, dtype=F64) -> torch.Tensor:
     """volume [B,H1,W1,H2,W2], coords [B,2,H1,W1] -> [B, 2r+1, 2r+1, H1, W1]
     with out[b,i,j] = bilinear(volume plane, x0-r+i, y0-r+j), zero padding
     (correlation_kernels.cu:22-66)."""
     B, H1, W1, H2, W2 = volume.shape
     rd = 2 * r + 1
-    v = volume.to(F64).reshape(B, H1 * W1, H2 * W2)
-    xi, yi, dx, dy = _window(coords[:, 0].to(F64), coords[:, 1].to(F64), r)
+    v = volume.to(dtype).reshape(B, H1 * W1, H2 * W2)
+    xi, yi, dx, dy = _window(coords[:, 0].to(dtype), coords[:, 1].to(dtype), r)
     xi = xi.reshape(B, H1 * W1)
     yi = yi.reshape(B, H1 * W1)
     dx = dx.reshape(B, H1 * W1)
     dy = dy.reshape(B, H1 * W1)
-    out = torch.zeros(B, rd, rd, H1 * W1, dtype=F64)
+    out = torch.zeros(B, rd, rd, H1 * W1, dtype=dtype)
     for i in range(rd + 1):
         for j in range(rd + 1):
             x1 = xi - r + i
@@ -116,13 +117,13 @@ def corr_index_backward(volume_shape, coords: torch.Tensor, corr_grad: torch.Ten
     return vg.view(B, H1, W1, H2, W2)
 
 
-def corr_block_lookup(pyramid, coords: torch.Tensor, r: int = 3) -> torch.Tensor:
+def corr_block_lookup(pyramid, coords: torch.Tensor, r: int = 3, dtype=F64) -> torch.Tensor:
     """CorrBlock.__call__: coords [B, N, H, W, 2] -> [B, N, L*(2r+1)^2, H, W] (droid_net.py:71-84)."""
     B, N, H, W, _ = coords.shape
     c = coords.permute(0, 1, 4, 2, 3).reshape(B * N, 2, H, W)
     outs = []
     for i, vol in enumerate(pyramid):
-        o = corr_index_forward(vol, c / 2**i, r)
+        o = corr_index_forward(vol, c / 2**i, r, dtype)
         outs.append(o.view(B, N, -1, H, W))
     return torch.cat(outs, 2)
 

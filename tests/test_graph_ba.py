@@ -172,3 +172,36 @@ def test_robust_kernels_run_on_the_fused_path_when_enabled(monkeypatch):
     with pytest.raises(ValueError, match="Unknown robust_kernel"):
         buf.ba_config.update(robust_kernel="cauchy")
         _call(buf, inp)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tracks,limited,intr,motion", [(True, True, True, False), (False, False, False, True),
+                                                        (False, False, False, False)])
+def test_fused_gpu_matches_generic_path(tracks, limited, intr, motion):
+    """tests/test_graph_buffer_fused_ba.py:203-249: the fused path (here the HIP
+    kernels) and the generic path (the float64 restatement of core/solver.py's
+    Solver) agree within 2e-4 after one iteration on the same graph."""
+    from oracle import generic_ba as G
+
+    buf = make_buffer("cuda", tracks=tracks)
+    inp = make_inputs(buf)
+    _call(buf, inp, limited_disp=limited, optimize_intrinsics=intr, motion_only=motion)
+    ref = make_buffer("cpu", tracks=tracks)
+    target, weight, damping, ii, jj = [t.cpu() for t in inp]
+    if tracks:
+        st, sw = ref.sparse_tracks.compute_dense_disp_target_weight(ii * 0, ref.tstamp[ii], jj * 0, ref.tstamp[jj],
+                                                                    (32, 40), (4, 5))
+        st, sw = st.flatten(1, 2), sw.flatten(1, 2)
+        cw = weight + sw
+        target = torch.where(cw > 0, (weight * target + sw * st) / cw.clamp_min(1e-12), target)
+        weight = cw
+    f64 = torch.float64
+    poses = ref.poses.double().clone()
+    disps = ref.disps[:, 0].double().clone()
+    focal = ref.intrinsics[0, :4].double().clone()
+    G.bundle_adjustment(poses, disps, focal, ref.disps_sens[:, 0].double(), target.double(), weight.double(),
+                        damping.double(), ii, jj, 1, 5, 1, 1e-3, 0.1, motion_only=motion, limited_disp=limited,
+                        optimize_intrinsics=intr, intrinsics_factor=8.0, dtype=f64)
+    torch.testing.assert_close(buf.poses[:5].cpu().double(), poses[:5], atol=2e-4, rtol=2e-4)
+    torch.testing.assert_close(buf.disps[:5, 0].cpu().double(), disps[:5], atol=2e-4, rtol=2e-4)
+    torch.testing.assert_close(buf.intrinsics[0, :4].cpu().double(), focal, atol=2e-4, rtol=2e-4)
