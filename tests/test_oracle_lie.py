@@ -128,6 +128,25 @@ def test_sim3_left_jacobian_series_quirk():
     assert X5.abs().max() > 1e-4  # the dropped term is visible at this size
 
 
+def test_sim3_left_jacobian_inverse_series():
+    """Sim3 inverse left Jacobian = Bernoulli series X/(e^X - I) truncated after
+    the Xi^4 term (sim3.h:171-178): at |a| ~ 0.5 it differs from the inverse of
+    the exact Jacobian by the first dropped term, Xi^6/30240, to within the next
+    one, while a missing or mis-scaled Xi^2 / Xi^4 term would be far outside."""
+    a, _ = _rand(4, 16, 10, scale=0.5)
+    Xi = L.Sim3.adj(a)
+    P = [torch.linalg.matrix_power(Xi, k) for k in range(9)]
+    true_inv = torch.linalg.inv(_left_jac_fd(4, a))
+    diff = true_inv - L.Sim3.left_jac_inv(a)
+    X6 = P[6] / 30240.0
+    X8 = P[8] / 1209600.0
+    bound = 3 * X8.abs().max() + 1e-6
+    assert (diff - X6).abs().max() < bound
+    assert X6.abs().max() > 4 * bound  # the truncation point itself is resolved
+    assert (P[4] / 720.0).abs().max() > 50 * bound
+    assert (P[2] / 12.0).abs().max() > 50 * bound
+
+
 @pytest.mark.parametrize("gid", GIDS)
 def test_projector_is_data_jacobian(gid):
     a, _ = _rand(gid, 8, 9)
