@@ -158,7 +158,9 @@ typedef struct vh_ba_plan vh_ba_plan;
  * kx = unique(cat(arange(t0,t1), ii)) and its inverse, the per-depth-frame
  * edge lists, the Schur row pairs and the block-assembly lists
  * (geom_kernels.cu:1648-1655, 1437-1468).  ii/jj are HOST arrays.  Allocates
- * and uploads device index tables (synchronously).  Returns NULL on error. */
+ * and uploads device index tables (synchronously).  The window [t0, t1) holds
+ * at most 1023 poses (a whole 1024-keyframe buffer; the reduced system is
+ * dense, (6 or 7) x P square).  Returns NULL on error. */
 vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W, int t0,
                               int t1);
 /* vh_ba_plan_create with the frame kernel's pixel tile chosen by the caller
@@ -183,8 +185,7 @@ size_t vh_ba_workspace_size(const vh_ba_plan* plan);
  * 7-dof pose blocks and the retraction T <- Exp(dx) T of lietorch's Sim3
  * (sim3.h; vipe/slam/maths/geom.py:147-182 gives the Jacobian of the point
  * map).  The reference's fused BA takes SE3 only; this flag extends it.  At
- * most 18 edges may leave one depth frame, and the window holds at most 292
- * poses. */
+ * most 18 edges may leave one depth frame. */
 #define VH_BA_SIM3 8
 
 /* ba / ba_extended (slam.cpp:41-51).  All pointers device, f32 unless noted:

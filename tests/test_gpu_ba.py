@@ -241,33 +241,25 @@ def test_ba_implicit_adjoint_against_fp64_central_differences(motion_only):
     assert max(errs) < 2e-3
 
 
-def _high_degree_graph(seed=21):
-    """24 frames of 16 x 24 pixels.  Frame 1 has 20 outgoing edges (22 E rows with
-    its self row and the pseudo row: 132 rows, 9 tiles of 16, 45 tile pairs of the
-    Schur Gram), frame 8 has 13 (15 rows of E: 21 pairs), frame 5 has 9 (11: 15
-    pairs), frames 10-23 one to three each."""
+def _graph_with_edges(N, H, W, spec, seed, sim3=False, t0=1):
+    """N frames of H x W pixels with the outgoing edges spec[i] of frame i, the
+    synthetic targets / weights / damping of vipe_amd.synth."""
     from vipe_amd.synth import FrameGraph, _quat_exp
 
     gen = torch.Generator().manual_seed(seed)
-    N, H, W, t0 = 24, 16, 24, 1
     ii, jj = [], []
-
-    def add(i, js):
+    for i, js in spec.items():
         ii.extend([i] * len(js))
         jj.extend(js)
-
-    add(1, list(range(2, 22)))
-    add(5, [0, 1, 2, 3, 4, 6, 7, 9, 12])
-    add(8, [0, 1, 2, 3, 4, 5, 6, 7, 9, 10, 11, 12, 13])
-    for i in range(10, N):
-        add(i, [(i + 1) % N, (i + 2) % N, (i + N - 3) % N][: 1 + i % 3])
     ii, jj = torch.tensor(ii), torch.tensor(jj)
     E = ii.shape[0]
     tau = torch.randn(N, 3, generator=gen, dtype=torch.float64) * 0.1
     phi = torch.randn(N, 3, generator=gen, dtype=torch.float64) * 0.05
     poses = torch.cat([tau, _quat_exp(phi)], -1)
+    if sim3:
+        poses = torch.cat([poses, torch.exp(torch.randn(N, 1, generator=gen, dtype=torch.float64) * 0.02)], -1)
     disps = 0.2 + 0.8 * torch.rand(N, H, W, generator=gen, dtype=torch.float64)
-    intr = torch.tensor([20.0, 20.0, 12.0, 8.0], dtype=torch.float64)
+    intr = torch.tensor([20.0, 20.0, W / 2.0, H / 2.0], dtype=torch.float64)
     coords, _ = reproject(poses, disps, intr, ii, jj)
     targets = (coords + torch.randn(E, H, W, 2, generator=gen, dtype=torch.float64) * 0.5).permute(0, 3, 1, 2)
     weights = torch.rand(E, 2, H, W, generator=gen, dtype=torch.float64)
@@ -280,11 +272,37 @@ def _high_degree_graph(seed=21):
                       t1=N)
 
 
-@pytest.mark.parametrize("tile", [0, 64, 128])
+def _high_degree_graph(seed=21):
+    """24 frames of 16 x 24 pixels.  Frame 1 has 20 outgoing edges (more than the
+    18 whose E rows stay in LDS: the wide-frame path), frame 8 has 13 (15 rows
+    of E: 21 tile pairs of the Schur Gram), frame 5 has 9 (11: 15 pairs),
+    frames 10-23 one to three each."""
+    N = 24
+    spec = {1: list(range(2, 22)), 5: [0, 1, 2, 3, 4, 6, 7, 9, 12], 8: [0, 1, 2, 3, 4, 5, 6, 7, 9, 10, 11, 12, 13]}
+    for i in range(10, N):
+        spec[i] = [(i + 1) % N, (i + 2) % N, (i + N - 3) % N][: 1 + i % 3]
+    return _graph_with_edges(N, 16, 24, spec, seed)
+
+
+def _wide_graph(sim3=False, seed=41):
+    """48 frames of 12 x 20 pixels: frame 1 with 40 outgoing edges (42 E-row
+    blocks, 252 / 294 rows, 136 / 190 tile pairs: three and four Gram chunks),
+    frame 30 with 33 including a stereo edge and edges to the fixed frame 0,
+    frame 0 (outside the window) with 34, frame 8 with 13 and the others one
+    to three."""
+    N = 48
+    spec = {1: list(range(2, 42)), 30: [30, 0] + list(range(31, 48)) + list(range(2, 16)),
+            0: list(range(1, 35)), 8: [0, 1, 2, 3, 4, 5, 6, 7, 9, 10, 11, 12, 13]}
+    for i in range(9, N):
+        if i not in spec:
+            spec[i] = [(i + 1) % N, (i + 2) % N, (i + N - 3) % N][: 1 + i % 3]
+    return _graph_with_edges(N, 12, 20, spec, seed, sim3=sim3)
+
+
+@pytest.mark.parametrize("tile", [0, 64, 128, 256])
 def test_ba_high_degree_frames_match_oracle(tile):
     """Frames with 9, 13 and 20 outgoing edges: the Schur Gram's larger tile-pair
-    counts per wave (up to 12) and the padding rows past 132 E rows, at every
-    pixel tile the plan allows (256 needs more LDS than 22 E rows may take)."""
+    counts per wave and its padding rows, and a wide frame, at every pixel tile."""
     from vipe_amd.ext import slam_ext
 
     g = _high_degree_graph()
@@ -296,7 +314,8 @@ def test_ba_high_degree_frames_match_oracle(tile):
     d = _cuda(g)
     p, dd = d["poses"].clone(), d["disps"].clone()
     plan = slam_ext.get_plan(d["ii"], d["jj"], N, H, W, g.t0, g.t1, tile)
-    assert plan.pixel_tile == (tile or 64)
+    if tile:
+        assert plan.pixel_tile == tile
     ws = torch.empty((plan.ws_bytes,), dtype=torch.uint8, device="cuda")
     dx = torch.empty((g.t1 - g.t0, 6), device="cuda")
     dz = torch.empty((plan.K, H * W), device="cuda")
@@ -310,6 +329,61 @@ def test_ba_high_degree_frames_match_oracle(tile):
     assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
     assert (dd.cpu().double() - rd.double()).abs().max().item() < 1e-4
     assert _reproj_err(g, p.cpu(), dd.cpu(), rp, rd, g.intrinsics) < 1e-3
+
+
+@pytest.mark.parametrize("sim3", [False, True])
+@pytest.mark.parametrize("variant", ["plain", "intrinsics", "motion", "huber"])
+def test_wide_frames_match_oracle(sim3, variant):
+    """Frames with 40, 34 and 33 outgoing edges (one of them fixed, one with a
+    stereo edge) through the wide-frame kernel, in each frame-kernel variant,
+    against the FP64 oracle after 2 iterations."""
+    from oracle import robust as R
+    from vipe_amd.ext import slam_ext
+
+    g = _wide_graph(sim3)
+    N, H, W = g.disps.shape
+    da = torch.ones(N)
+    ds = g.disps.clone() * 1.01
+    ds[:, ::2] = 0.0
+    iters, lm, ep, alpha = 2, 1e-4, 0.1, 0.05
+    intr, motion = variant == "intrinsics", variant == "motion"
+    robust = (R.HUBER, 0.5, 1.0, 1e6) if variant == "huber" else None
+    d = _cuda(g)
+    p, dd, it = d["poses"].clone(), d["disps"].clone(), d["intrinsics"].clone()
+    args = (ds.cuda(), d["targets"], d["weights"], d["eta"], d["ii"], d["jj"], da.cuda(),
+            g.t0, g.t1, iters, lm, ep, motion, alpha, intr, 1e-6, 1e-6, 0.125, True)
+    if robust:
+        dx, dz, en = slam_ext.ba_extended_robust(p, dd, it, *args, robust)
+    else:
+        dx, dz, en = slam_ext.ba_extended(p, dd, it, *args)
+    rp, rd, rit = g.poses.clone(), g.disps.clone(), g.intrinsics.clone()
+    rdx, rdz, ren = OB.ba_extended(rp, rd, rit, ds, g.targets, g.weights, g.eta, g.ii, g.jj, da, g.t0, g.t1, iters,
+                                   lm, ep, motion, alpha, intr, 1e-6, 1e-6, 0.125, True, robust=robust)
+    torch.cuda.synchronize()
+    print(f"[wide frames sim3={sim3} {variant}] dx err {(dx.cpu().double() - rdx).abs().max().item():.3e}")
+    assert (dx.cpu().double() - rdx).abs().max().item() < 1e-5 * max(1.0, rdx.abs().max().item())
+    assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
+    if motion:
+        assert torch.equal(dd, d["disps"])
+    else:
+        assert (dz.cpu().double() - rdz).abs().max().item() < 1e-4
+        assert (dd.cpu().double() - rd.double()).abs().max().item() < 1e-4
+    if intr:
+        assert (it.cpu().double() - rit.double()).abs().max().item() < 1e-4
+    assert torch.allclose(en.cpu().double(), ren, rtol=1e-4, atol=1e-6)
+
+
+def test_adjoint_refuses_wide_frames():
+    from vipe_amd.ba_layer import ba_iteration
+
+    g = _wide_graph()
+    N = g.disps.shape[0]
+    d = _cuda(g)
+    tg = d["targets"].clone().requires_grad_(True)
+    dx, dz = ba_iteration(tg, d["weights"], d["poses"], d["disps"], d["intrinsics"], d["disps_sens"], d["eta"],
+                          d["ii"], d["jj"], torch.ones(N, device="cuda"), g.t0, g.t1, 1e-4, 0.1, False, 0.05)
+    with pytest.raises(RuntimeError, match="outgoing edges"):
+        (dx.sum() + dz.sum()).backward()
 
 
 # ---- Sim3 poses ([N,8], 7-dof pose blocks) --------------------------------
@@ -354,9 +428,9 @@ def test_ba_full_size_8_iterations_meets_accuracy_targets(name):
     assert torch.allclose(en.double(), ren, rtol=1e-5)
 
 
-def test_sim3_ba_high_degree_frames_and_limit():
-    """Sim3 frames with up to 18 outgoing edges (20 E-row blocks of 7 rows) match
-    the oracle; a frame with 20 edges is refused with a clear error."""
+def test_sim3_ba_high_degree_frames():
+    """Sim3 frames with 9, 13 and 20 outgoing edges (the 20-edge frame on the
+    wide-frame path) match the oracle."""
     from vipe_amd.ext import slam_ext
 
     g = _high_degree_graph()
@@ -365,18 +439,6 @@ def test_sim3_ba_high_degree_frames_and_limit():
     g.poses = torch.cat([g.poses, s], -1).contiguous()
     d = _cuda(g)
     da = torch.ones(N, device="cuda")
-    with pytest.raises(RuntimeError, match="Sim3 limit"):
-        slam_ext.ba_extended(d["poses"].clone(), d["disps"].clone(), d["intrinsics"].clone(), d["disps_sens"],
-                             d["targets"], d["weights"], d["eta"], d["ii"], d["jj"], da, g.t0, g.t1, 1, 1e-4, 0.1,
-                             False, 0.05, False, 1e-6, 1e-6, 1.0)
-    # keep 18 of frame 1's 20 edges
-    keep = torch.ones(g.ii.shape[0], dtype=torch.bool)
-    keep[torch.nonzero(g.ii == 1).flatten()[18:]] = False
-    g.ii, g.jj = g.ii[keep], g.jj[keep]
-    g.targets, g.weights = g.targets[keep].contiguous(), g.weights[keep].contiguous()
-    kx = torch.unique(torch.cat([torch.arange(g.t0, g.t1), g.ii]))
-    assert kx.shape[0] == g.eta.shape[0]
-    d = _cuda(g)
     iters = 2
     p, dd = d["poses"].clone(), d["disps"].clone()
     dx, dz, _ = slam_ext.ba_extended(p, dd, d["intrinsics"].clone(), d["disps_sens"], d["targets"], d["weights"],
@@ -389,3 +451,40 @@ def test_sim3_ba_high_degree_frames_and_limit():
     assert (dx.cpu().double() - rdx).abs().max().item() < 1e-5 * max(1.0, rdx.abs().max().item())
     assert (p.cpu().double() - rp.double()).abs().max().item() < 1e-5
     assert (dd.cpu().double() - rd.double()).abs().max().item() < 1e-4
+
+
+LONG_WINDOW = dict(n_frames=640, n_edges=2554 + 256, H=8, W=12, intr=(10.0, 10.0, 6.0, 4.0), window=2,
+                   n_random=256, sim3=False)
+
+
+@pytest.mark.parametrize("sim3", [False, True])
+def test_long_window_matches_oracle(sim3):
+    """640 keyframes optimised over [1, 640) (639 poses: a 3834 / 4473-wide
+    reduced system, 60 / 70 Cholesky blocks), 2810 edges, 2 GN iterations,
+    against the FP64 oracle; a window of 1024 poses is refused."""
+    from vipe_amd.ext import slam_ext
+    from vipe_amd.synth import make_graph_from
+
+    g = make_graph_from(dict(LONG_WINDOW, sim3=sim3), seed=5, with_fmaps=False)
+    N = g.disps.shape[0]
+    d = _cuda(g)
+    da = torch.ones(N)
+    iters = 2
+    p, dd = d["poses"].clone(), d["disps"].clone()
+    dx, dz, en = slam_ext.ba_extended(p, dd, d["intrinsics"].clone(), d["disps_sens"], d["targets"], d["weights"],
+                                      d["eta"], d["ii"], d["jj"], da.cuda(), g.t0, g.t1, iters, 1e-4, 0.1, False, 0.05,
+                                      False, 1e-6, 1e-6, 1.0, True)
+    rp, rd = g.poses.clone(), g.disps.clone()
+    rdx, rdz, ren = OB.ba_extended(rp, rd, g.intrinsics.clone(), g.disps_sens, g.targets, g.weights, g.eta, g.ii,
+                                   g.jj, da, g.t0, g.t1, iters, 1e-4, 0.1, False, 0.05, False, 1e-6, 1e-6, 1.0, True)
+    torch.cuda.synchronize()
+    assert dx.shape == (N - 1, 7 if sim3 else 6)
+    px = _reproj_err(g, p.cpu(), dd.cpu(), rp, rd, g.intrinsics)
+    rel_pose = ((p.cpu().double() - rp.double()).norm(dim=1) / rp.double().norm(dim=1)).max().item()
+    print(f"[long window, sim3={sim3}] dx err {(dx.cpu().double() - rdx).abs().max().item():.3e}, "
+          f"px err {px:.3e}, relative pose err {rel_pose:.3e}")
+    assert (dx.cpu().double() - rdx).abs().max().item() < 1e-5 * max(1.0, rdx.abs().max().item())
+    assert px < 1e-4 and rel_pose < 1e-5
+    assert torch.allclose(en.cpu().double(), ren, rtol=1e-5)
+    with pytest.raises(RuntimeError, match="1024 poses in the window"):
+        slam_ext.get_plan(torch.tensor([0, 1]), torch.tensor([1, 0]), 1100, 4, 4, 1, 1025)

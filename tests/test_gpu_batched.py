@@ -1,7 +1,7 @@
 """Batched uneven clips (vipe_amd.batched): the batch, eager and replayed from
 one captured graph, equals one ba_extended call per clip; the tile autotuner
-picks a feasible tile and caches it; the tiled plan rejects tiles whose E rows
-exceed the LDS."""
+picks a feasible tile and caches it; the tiled plan takes every tile when the
+only high-degree frame is a wide one."""
 import pytest
 import torch
 
@@ -113,15 +113,16 @@ def test_tuned_tile_matches_default_tile_closely():
         assert (d - outs[0][1]).abs().max().item() < 1e-4
 
 
-def test_tiled_plan_rejects_tile_over_lds():
+def test_tiled_plan_takes_every_tile_with_a_wide_frame():
     from vipe_amd.ext import slam_ext
 
-    # frame 1 (inside the window [1, 24)) with 20 outgoing edges: 22 E rows
-    # (self, 20 edges, pseudo row); 256-pixel tiles would need 22 * 6 * 260 * 4 B
-    # = 137280 B of LDS (rows of 256 + 4 floats), over the 131072 B a caller may ask for
+    # frame 1 (inside the window [1, 24)) with 20 outgoing edges: more than the
+    # 18 whose E rows stay in LDS, so it takes the wide-frame kernel and the
+    # frame kernel's tile is chosen for the other frames alone
     N = 24
     ii = torch.ones(20, dtype=torch.long)
     jj = torch.arange(2, 22)
-    with pytest.raises(RuntimeError, match="LDS"):
-        slam_ext.get_plan(ii.cuda(), jj.cuda(), N, 16, 32, 1, N, 256)
-    assert slam_ext.get_plan(ii.cuda(), jj.cuda(), N, 16, 32, 1, N, 128).pixel_tile == 128
+    for tile in (64, 128, 256):
+        assert slam_ext.get_plan(ii.cuda(), jj.cuda(), N, 16, 32, 1, N, tile).pixel_tile == tile
+    with pytest.raises(RuntimeError, match="pixel_tile"):
+        slam_ext.get_plan(ii.cuda(), jj.cuda(), N, 16, 32, 1, N, 96)

@@ -44,12 +44,16 @@ namespace vh {
 namespace ba {
 
 constexpr int kThreads = 256;
-constexpr int kMaxDeg = 20;   // edges leaving one depth frame
-constexpr int kMaxRext = 22;  // self row + edges + pseudo row
-constexpr int kMaxDegSim3 = 18;  // Sim3: 7-row blocks, the Gram's per-wave slots hold 20 row blocks
+// A depth frame with at most kMaxDeg outgoing edges keeps its E rows in LDS
+// (the frame kernel); one with more takes the wide-frame kernel, which stages
+// them in the workspace, so the edge count of a frame is not limited.
+constexpr int kMaxDeg = 18;   // edges leaving one depth frame on the LDS path
+constexpr int kMaxRext = 20;  // self row + edges + pseudo row
+constexpr int kWideGroup = 16;  // edges of a wide frame set up and reduced together
 constexpr int kNV = 40;       // per-edge partial stride (floats)
 constexpr int kCB = 64;       // Cholesky block size
-constexpr int kMaxLd = 2048;  // leading dimension of the reduced system (6P+1 rounded up to kCB)
+constexpr int kMaxPoses = 1023;  // poses in the window: a whole 1024-keyframe buffer optimised over [1, 1024)
+constexpr int kMaxLd = (7 * kMaxPoses + 1 + kCB - 1) / kCB * kCB;  // leading dimension of the reduced system
 constexpr int kGramSplit = 8; // finalize workgroups per depth frame
 constexpr int kLdsBudget = 64 * 1024;  // dynamic E-row tile; with ~15 KB static, two frame workgroups fit a CU
 constexpr int kLdsMax = 128 * 1024;    // largest dynamic E-row tile a caller may ask for (one workgroup per CU)
@@ -96,6 +100,9 @@ struct DevPlan {
     const int* rhs_ptr;  // [P+1]
     const int* rhs_items;
     const int* pseudo_off; // [K] Gram block index of the (pseudo, pseudo) block
+    const int* wide;     // [n_wide] slots of the depth frames with more than kMaxDeg outgoing edges
+    int n_wide;          // number of such frames
+    int wide_rows;       // stored E rows per pixel of the largest of them (zero row included)
 };
 
 struct Work {
@@ -113,6 +120,7 @@ struct Work {
     double* Li;        // [nb][64][64] L_kk^-T of each diagonal block
     double* y;         // [ld] forward-substitution result
     int* flag;         // [4]
+    float* wide_e;     // [n_wide][T][wide_rows][256] E rows of the wide frames
 };
 
 // Robust kernels of the generic solver (vipe/slam/ba/kernel.py:21-162) as an
@@ -529,6 +537,75 @@ __device__ __forceinline__ void gram_pass(f32x4 (&acc)[kGramSlots], const float*
     }
 }
 
+// One edge at this lane's pixel: the j-side pose partials v[0, 28) (Hjj upper
+// triangle, vj, energy; v[28], v[29] Hff, vf with INTR; the rest zero), the
+// focal partials fj, the j-side E row er (its Sim3 sigma entry is zero), and the
+// frame's depth sums Csum, wsum, Ef and self E row es, which it adds to.  The
+// edge is (R, t, sc) with targets / weights `in` = (wu, wv, tu, tv); an invalid
+// pixel gives zeros.
+template <bool INTR, bool MOTION, bool ROBUST, int PD>
+__device__ __forceinline__ void edge_pixel(const float* Rm, const float* t, float sc, bool stereo, const float4& in,
+                                           bool valid, float X0, float Y0, float d, float act, int has_self,
+                                           const RunArgs& a, float fx, float fy, float cx, float cy, float (&v)[32],
+                                           float (&fj)[6], float (&er)[PD], float& Csum, float& wsum, float& Ef,
+                                           float (&es)[PD]) {
+#pragma unroll
+    for (int i = 0; i < 32; i++) v[i] = 0.f;
+#pragma unroll
+    for (int n = 0; n < 6; n++) fj[n] = 0.f;
+#pragma unroll
+    for (int n = 0; n < PD; n++) er[n] = 0.f;
+    if (valid) {
+        PixTerms T;
+        pixel_terms<INTR, true, PD>(Rm, t, sc, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale, T);
+        if (ROBUST) {
+            T.wu *= robust_weight(T.ru, a.rk);
+            T.wv *= robust_weight(T.rv, a.rk);
+        }
+        const float wdu = T.wu * act, wdv = T.wv * act;
+        const float pu = stereo ? 0.f : T.wu;
+        const float pv = stereo ? 0.f : T.wv;
+        // pose j-side terms; Ju[1] and Jv[0] are structurally zero, so
+        // the products they enter are left out (no fast-math folds them)
+        int l = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++)
+#pragma unroll
+            for (int m = 0; m <= n; m++) {
+                const float hu = (n != 1 && m != 1) ? pu * T.Ju[n] * T.Ju[m] : 0.f;
+                const float hv = (n != 0 && m != 0) ? pv * T.Jv[n] * T.Jv[m] : 0.f;
+                v[l] = (n != 1 && m != 1) ? ((n != 0 && m != 0) ? hu + hv : hu) : hv;
+                l++;
+            }
+#pragma unroll
+        for (int n = 0; n < 6; n++)
+            v[21 + n] = n == 1 ? pv * T.rv * T.Jv[n] : n == 0 ? pu * T.ru * T.Ju[n]
+                                                              : pu * T.ru * T.Ju[n] + pv * T.rv * T.Jv[n];
+        v[27] = T.wu * T.ru * T.ru + T.wv * T.rv * T.rv;
+        if (INTR) {
+            v[28] = pu * T.Jfu * T.Jfu + pv * T.Jfv * T.Jfv;
+            v[29] = pu * T.ru * T.Jfu + pv * T.rv * T.Jfv;
+#pragma unroll
+            for (int n = 0; n < 6; n++) fj[n] = pu * T.Ju[n] * T.Jfu + pv * T.Jv[n] * T.Jfv;
+        }
+        if (!MOTION) {
+            Csum += wdu * T.Jzu * T.Jzu + wdv * T.Jzv * T.Jzv;
+            wsum += wdu * T.ru * T.Jzu + wdv * T.rv * T.Jzv;
+            if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
+            const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
+#pragma unroll
+            for (int n = 0; n < 6; n++)
+                er[n] = n == 1 ? cv * T.Jv[n] : n == 0 ? cu * T.Ju[n] : cu * T.Ju[n] + cv * T.Jv[n];
+            if (has_self) {
+                float o[PD];
+                neg_adjT<PD>(Rm, t, sc, er, o);
+#pragma unroll
+                for (int n = 0; n < PD; n++) es[n] += o[n];
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 1. frame kernel
 // ---------------------------------------------------------------------------
@@ -550,6 +627,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int kf = p.kx[k];
     const int e0 = p.fr_ptr[k];
     const int deg = p.fr_ptr[k + 1] - e0;
+    if (deg > kMaxDeg) return;  // a wide frame: ba_frame_wide_kernel
     const int has_self = p.fr_self[k];
     const int R = has_self + deg;
     const int Rext = R + 1;
@@ -623,63 +701,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             const float4 in = in1;
             in1 = in2;
             edge_in(s + 2, pix, valid, in2);
-            float v[32];
-#pragma unroll
-            for (int i = 0; i < 32; i++) v[i] = 0.f;
-            float fj[6] = {0, 0, 0, 0, 0, 0};
-            float er[PD];  // j-side E row; its sigma entry (Sim3) is zero
-#pragma unroll
-            for (int n = 0; n < PD; n++) er[n] = 0.f;
-            if (valid) {
-                PixTerms T;
-                pixel_terms<INTR, true, PD>(Rm, t, sc, X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z, in.w, a.iscale,
-                                            T);
-                if (ROBUST) {
-                    T.wu *= robust_weight(T.ru, a.rk);
-                    T.wv *= robust_weight(T.rv, a.rk);
-                }
-                const float wdu = T.wu * act, wdv = T.wv * act;
-                const float pu = stereo ? 0.f : T.wu;
-                const float pv = stereo ? 0.f : T.wv;
-                // pose j-side terms; Ju[1] and Jv[0] are structurally zero, so
-                // the products they enter are left out (no fast-math folds them)
-                int l = 0;
-#pragma unroll
-                for (int n = 0; n < 6; n++)
-#pragma unroll
-                    for (int m = 0; m <= n; m++) {
-                        const float hu = (n != 1 && m != 1) ? pu * T.Ju[n] * T.Ju[m] : 0.f;
-                        const float hv = (n != 0 && m != 0) ? pv * T.Jv[n] * T.Jv[m] : 0.f;
-                        v[l] = (n != 1 && m != 1) ? ((n != 0 && m != 0) ? hu + hv : hu) : hv;
-                        l++;
-                    }
-#pragma unroll
-                for (int n = 0; n < 6; n++)
-                    v[21 + n] = n == 1 ? pv * T.rv * T.Jv[n] : n == 0 ? pu * T.ru * T.Ju[n]
-                                                                      : pu * T.ru * T.Ju[n] + pv * T.rv * T.Jv[n];
-                v[27] = T.wu * T.ru * T.ru + T.wv * T.rv * T.rv;
-                if (INTR) {
-                    v[28] = pu * T.Jfu * T.Jfu + pv * T.Jfv * T.Jfv;
-                    v[29] = pu * T.ru * T.Jfu + pv * T.rv * T.Jfv;
-#pragma unroll
-                    for (int n = 0; n < 6; n++) fj[n] = pu * T.Ju[n] * T.Jfu + pv * T.Jv[n] * T.Jfv;
-                }
-                if (!MOTION) {
-                    Csum += wdu * T.Jzu * T.Jzu + wdv * T.Jzv * T.Jzv;
-                    wsum += wdu * T.ru * T.Jzu + wdv * T.rv * T.Jzv;
-                    if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
-                    const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
-#pragma unroll
-                    for (int n = 0; n < 6; n++)
-                        er[n] = n == 1 ? cv * T.Jv[n] : n == 0 ? cu * T.Ju[n] : cu * T.Ju[n] + cv * T.Jv[n];
-                    if (has_self) {
-                        float o[PD];
-                        neg_adjT<PD>(Rm, t, sc, er, o);
-#pragma unroll
-                        for (int n = 0; n < PD; n++) es[n] += o[n];
-                    }
-                }
-            }
+            float v[32], fj[6], er[PD];
+            edge_pixel<INTR, MOTION, ROBUST, PD>(Rm, t, sc, stereo, in, valid, X0, Y0, d, act, has_self, a, fx, fy,
+                                                 cx, cy, v, fj, er, Csum, wsum, Ef, es);
             if (!MOTION && tid < TP) {
 #pragma unroll
                 for (int n = 0; n < 6; n++) dyn[(has_self * PD + 6 * s + n) * ldr + tid] = er[n];
@@ -779,6 +803,177 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             float s = 0.f;
             for (int kp = 0; kp < gks; kp++) s += red[kp * gnp * 256 + off];
             out[i] = s;
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// 1b. wide-frame kernel: depth frames with more than kMaxDeg outgoing edges
+// ---------------------------------------------------------------------------
+// The same sums as the frame kernel for one 256-pixel tile of one wide frame,
+// one pixel per thread.  Its edges are set up and reduced kWideGroup at a time
+// (their per-edge partials are complete after one pass over the tile), and its
+// stored E rows (the RowMap layout) go to the workspace instead of LDS.  The
+// Schur Gram then runs over the tile's 256 pixels in chunks of 4 x kGramSlots
+// tile pairs (each wave owns its pairs, so no cross-wave reduction), and each
+// chunk's elements go straight to their PD x PD blocks.
+template <bool INTR, bool MOTION, bool ROBUST, int PD>
+__global__ __launch_bounds__(kThreads) void ba_frame_wide_kernel(DevPlan p, RunArgs a, Work w) {
+    __shared__ float sEP[kWideGroup][16];  // R(9) t(3) stereo(12) edge(13) scale(14)
+    __shared__ __align__(16) float sQ[kThreads];
+    __shared__ float sAcc[4][kWideGroup][kNV];
+    __shared__ float sFen[4];
+    const int slot = blockIdx.y;
+    const int k = p.wide[slot];
+    const int tile = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int kf = p.kx[k];
+    const int e0 = p.fr_ptr[k];
+    const int deg = p.fr_ptr[k + 1] - e0;
+    const int has_self = p.fr_self[k];
+    const int R = has_self + deg;
+    const int Rext = R + 1;
+    const int HW = p.HW;
+    const float act = a.depth_active[kf];
+    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+    const int pix = tile * kThreads + tid;  // 256-pixel tiles, as the frame kernel's TP x SUB
+    const bool valid = pix < HW;
+    const int py = valid ? pix / p.W : 0;
+    const int px = valid ? pix - py * p.W : 0;
+    const float X0 = ((float)px - cx) / fx;
+    const float Y0 = ((float)py - cy) / fy;
+    const float d = valid ? a.disps[(int64_t)kf * HW + pix] : 0.f;
+    const RowMap rmap{has_self, R, has_self * PD + 6 * deg, has_self * PD + 6 * deg + 2};
+    float* Eg = w.wide_e + ((int64_t)slot * p.T + tile) * (int64_t)p.wide_rows * kThreads;
+    float Csum = 0.f, wsum = 0.f, Ef = 0.f;
+    float es[PD];
+#pragma unroll
+    for (int n = 0; n < PD; n++) es[n] = 0.f;
+
+    auto edge_in = [&](int s, float4& o) {
+        if (valid && s < deg) {
+            const int64_t eb = (int64_t)p.fr_edges[e0 + s] * 2 * HW + pix;
+            o = make_float4(a.weights[eb], a.weights[eb + HW], a.targets[eb], a.targets[eb + HW]);
+        }
+    };
+    for (int g0 = 0; g0 < deg; g0 += kWideGroup) {
+        const int gn = min(kWideGroup, deg - g0);
+        if (tid < gn) {
+            const int e = p.fr_edges[e0 + g0 + tid];
+            float Rm[9], t[3], sc;
+            edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
+            for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+            for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+            sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
+            sEP[tid][13] = __int_as_float(e);
+            sEP[tid][14] = sc;
+        }
+        for (int i = tid; i < 4 * kWideGroup * kNV; i += kThreads) (&sAcc[0][0][0])[i] = 0.f;
+        float4 in1 = make_float4(0.f, 0.f, 0.f, 0.f), in2 = in1;
+        edge_in(g0, in1);
+        edge_in(g0 + 1, in2);
+        __syncthreads();
+        for (int s = 0; s < gn; s++) {
+            const float4 in = in1;
+            in1 = in2;
+            edge_in(g0 + s + 2 < g0 + gn ? g0 + s + 2 : deg, in2);
+            float v[32], fj[6], er[PD];
+            edge_pixel<INTR, MOTION, ROBUST, PD>(sEP[s], sEP[s] + 9, sEP[s][14], sEP[s][12] != 0.f, in, valid, X0, Y0,
+                                                 d, act, has_self, a, fx, fy, cx, cy, v, fj, er, Csum, wsum, Ef, es);
+            if (!MOTION) {
+#pragma unroll
+                for (int n = 0; n < 6; n++) Eg[(int64_t)(has_self * PD + 6 * (g0 + s) + n) * kThreads + tid] = er[n];
+            }
+            const float tot = wave_sum32(v);
+            if (lane < 28) sAcc[wave][s][lane] += tot;
+            if (INTR) {
+                if (lane >= 28 && lane < 30) sAcc[wave][s][lane] += tot;
+#pragma unroll
+                for (int n = 0; n < 6; n++) {
+                    const float f = wave_sum(fj[n]);
+                    if (lane == 0) sAcc[wave][s][32 + n] += f;
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < gn * kNV; i += kThreads) {
+            const int sl = i / kNV, v = i % kNV;
+            const int e = __float_as_int(sEP[sl][13]);
+            w.edge_part[((int64_t)tile * p.E + e) * kNV + v] =
+                (sAcc[0][sl][v] + sAcc[1][sl][v]) + (sAcc[2][sl][v] + sAcc[3][sl][v]);
+        }
+        __syncthreads();
+    }
+    if (MOTION) return;
+
+    float Q = 0.f, wt = 0.f, Eft = 0.f, fen = 0.f;
+    if (valid) {
+        const float ds = a.disps_sens[(int64_t)kf * HW + pix];
+        const float et = a.eta[(int64_t)k * HW + pix];
+        const float m = (ds > 0.f ? 1.f : 0.f) * act;
+        const float C = act * (Csum + et + m * a.alpha) + (1.f - act);
+        wt = act * (wsum - m * a.alpha * (d - ds));
+        Q = 1.f / C;
+        Eft = Ef * act;
+        fen = m * a.alpha * (d - ds) * (d - ds);
+        w.Qb[(int64_t)k * HW + pix] = Q;
+        w.Wb[(int64_t)k * HW + pix] = wt;
+    }
+    if (has_self) {
+#pragma unroll
+        for (int n = 0; n < PD; n++) Eg[(int64_t)n * kThreads + tid] = es[n];
+    }
+    Eg[(int64_t)rmap.pbase * kThreads + tid] = wt;
+    Eg[(int64_t)(rmap.pbase + 1) * kThreads + tid] = Eft;
+    Eg[(int64_t)rmap.zrow * kThreads + tid] = 0.f;
+    sQ[tid] = Q;
+    const float fsum = wave_sum(fen);
+    if (lane == 0) sFen[wave] = fsum;
+    __syncthreads();
+    if (tid == 0) w.fen_part[(int64_t)tile * p.K + k] = sFen[0] + sFen[1] + sFen[2] + sFen[3];
+
+    // Gram chunks: pair c0 + wave + 4 j of the upper tile pairs
+    constexpr int PD2 = PD * PD;
+    const int nrows = Rext * PD;
+    const int nT = gram_tiles(Rext, PD);
+    const int np = nT * (nT + 1) / 2;
+    const int gi = lane & 15, gk = lane >> 4;
+    float* out = w.gram_part + (int64_t)tile * p.GT + (int64_t)p.gram_off[k] * PD2;
+    for (int c0 = 0; c0 < np; c0 += 4 * kGramSlots) {
+        const int nslot = (min(np - c0, 4 * kGramSlots) - wave + 3) / 4;
+        f32x4 acc[kGramSlots];
+#pragma unroll
+        for (int j = 0; j < kGramSlots; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        switch (nslot) {
+#define VH_GRAM_CASE(S)                                                                                      \
+    case S:                                                                                                  \
+        gram_pass<S, PD>(acc, Eg, sQ, 0, kThreads, c0 + wave, 4, nT, nrows, rmap, kThreads, gi, gk);        \
+        break;
+            VH_GRAM_CASE(1) VH_GRAM_CASE(2) VH_GRAM_CASE(3) VH_GRAM_CASE(4) VH_GRAM_CASE(5) VH_GRAM_CASE(6)
+            VH_GRAM_CASE(7) VH_GRAM_CASE(8) VH_GRAM_CASE(9) VH_GRAM_CASE(10) VH_GRAM_CASE(11) VH_GRAM_CASE(12)
+#undef VH_GRAM_CASE
+            default: break;
+        }
+        // D[r][c] of a tile sits at lane c + 16 (r / 4), component r % 4; the
+        // upper elements (r <= c) land in block (r / PD, c / PD), and a
+        // diagonal block also takes their mirror
+#pragma unroll
+        for (int j = 0; j < kGramSlots; j++) {
+            if (j >= nslot) break;
+            int ti, tj;
+            gram_rows(c0 + wave + 4 * j, nT, ti, tj);
+            const int c = 16 * tj + (lane & 15);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int r = 16 * ti + 4 * (lane >> 4) + m;
+                if (r > c || c >= nrows) continue;
+                const int ra = r / PD, rb = c / PD, n = r - PD * ra, q = c - PD * rb;
+                const int blk = ra * Rext - ra * (ra - 1) / 2 + (rb - ra);
+                out[blk * PD2 + n * PD + q] = acc[j][m];
+                if (ra == rb && n != q) out[blk * PD2 + q * PD + n] = acc[j][m];
+            }
         }
     }
 }
@@ -1078,7 +1273,6 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 constexpr int kSB = 16;  // sub-block of the diagonal-block inverse
 
 struct FactorLds {
-    double col[2][kCB];
     double invd[kCB];
     double T[3][kSB][kSB + 1];
 };
@@ -1110,29 +1304,42 @@ __device__ __forceinline__ f64x4 mfma_tile16(const double* p, int ps, const doub
 }
 
 // Wave 0: factor the 16-column panel at column o (rows o..63, row r in lane r).
-// The pivot chain never leaves registers: the next pivot is lane (o+j+1)'s own
-// entry after the step's update, read with readlane; the other columns of the
-// step are broadcast through LDS, which a single wavefront reads back in
-// program order without a barrier.  Writes L's panel to S and 1/L_jj to invd.
+// Every value a step needs from another row is read from that row's lane with
+// readlane into scalar registers: the pivot, and the pivot column's entries
+// that update the columns to its right.  Nothing goes through LDS, so the
+// pivot chain is VALU latency only (about 4000 cycles per panel against 5500
+// with an LDS column broadcast).  Lanes above the panel's diagonal compute
+// values that are never read or stored.  Writes L's panel to S and 1/L_jj to
+// invd.
 __device__ __forceinline__ void factor_panel(double (*S)[kCB + 1], FactorLds& F, int o, int lane, bool& bad) {
-    double v[kSB];
+    double v[kSB], invs[kSB];
 #pragma unroll
-    for (int m = 0; m < kSB; m++) v[m] = (lane >= o + m) ? S[lane][o + m] : 0.0;
+    for (int m = 0; m < kSB; m++) v[m] = S[lane][o + m];
     double dn = readlane_f64(v[0], o);
 #pragma unroll
     for (int j = 0; j < kSB; j++) {
-        double* cb = F.col[j & 1];
-        cb[lane] = v[j];  // column o+j below the pivot (rows above it are ignored)
         const double d = dn;
         bad |= (d <= 0.0);  // a NaN pivot propagates, like the reference's LLT
         const double inv = rsqrt_nr(d);
         const double sc = v[j] * (inv * inv);  // A'[r][o+j] / d
-        // next pivot: row o+j+1 updates its own diagonal with its own entry
-        if (j + 1 < kSB) dn = readlane_f64(v[j + 1] - sc * v[j], o + j + 1);
+        if (j + 1 < kSB) {
+            // the next pivot's column first: its diagonal entry is the next pivot
+            v[j + 1] -= sc * readlane_f64(v[j], o + j + 1);
+            dn = readlane_f64(v[j + 1], o + j + 1);
+        }
 #pragma unroll
-        for (int c = j + 1; c < kSB; c++) v[c] -= sc * cb[o + c];
+        for (int c = j + 2; c < kSB; c++) {
+            v[c] -= sc * readlane_f64(v[j], o + c);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         v[j] *= inv;
-        if (lane == 0) F.invd[o + j] = inv;
+        invs[j] = inv;
+    }
+    if (lane < kSB) {
+        double iv = invs[0];
+#pragma unroll
+        for (int m = 1; m < kSB; m++) iv = (lane == m) ? invs[m] : iv;
+        F.invd[o + lane] = iv;
     }
 #pragma unroll
     for (int m = 0; m < kSB; m++)
@@ -1553,7 +1760,7 @@ __global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __res
                                                              const int* flag, int npose, float* dx) {
     static_assert(kBR == 4, "back_reduce4 sums 4 rows per wave");
     static_assert(kBT * 2 * (kBR / 2) == kCB * kCB, "X_k pairs cover the block");
-    __shared__ double xs[kMaxLd];
+    extern __shared__ double xs[];  // [ld]
     __shared__ double Xk[kCB][kCB + 1];
     __shared__ double t[kCB];
     __shared__ double part[kBW][kCB];
@@ -1651,14 +1858,16 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
     const float act = a.depth_active[kf];
     const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
     const int HW = p.HW;
-    if (tid < deg) {
-        const int e = p.fr_edges[e0 + tid];
+    // edge slot tid of the group starting at edge g0: its relative pose and
+    // u = dx[jj] (if 1 <= jj-t0 < P) - Adj(Tij) dx[k] (self row, if 1 <= k-t0 < P)
+    auto setup = [&](int g0, int gn) {
+        if (tid >= gn) return;
+        const int e = p.fr_edges[e0 + g0 + tid];
         float Rm[9], t[3], sc;
         edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
         for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
         for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
         sEP[tid][14] = sc;
-        // u = dx[jj] (if 1 <= jj-t0 < P) - Adj(Tij) dx[k] (self row, if 1 <= k-t0 < P)
         double u[6] = {0, 0, 0, 0, 0, 0};
         const int lj = p.jj[e] - p.t0;
         if (lj >= 1 && lj < p.P)
@@ -1683,7 +1892,8 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
             u[5] -= rw[2];
         }
         for (int n = 0; n < 6; n++) sU[tid][n] = (float)u[n];
-    }
+    };
+    setup(0, min(deg, kMaxDeg));
     // this pixel's own inputs and the first two edges' are requested before the
     // barrier, so their latency overlaps the edge set-up above
     const int pix = blockIdx.x * kThreads + tid;
@@ -1693,9 +1903,9 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
     const float Q = valid ? w.Qb[(int64_t)k * HW + pix] : 0.f;
     const float wt = valid ? w.Wb[(int64_t)k * HW + pix] : 0.f;
     const bool rob = a.rk.type != 0;
-    // weights (u, v) and, with a robust kernel, targets (u, v) of edge slot s
-    auto edge_in = [&](int s, float4& o) {
-        if (valid && s < deg) {
+    // weights (u, v) and, with a robust kernel, targets (u, v) of edge s (below `end`)
+    auto edge_in = [&](int s, int end, float4& o) {
+        if (valid && s < end) {
             const int64_t eb = (int64_t)p.fr_edges[e0 + s] * 2 * HW + pix;
             o.x = a.weights[eb];
             o.y = a.weights[eb + HW];
@@ -1705,39 +1915,48 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
             }
         }
     };
-    float4 in1 = make_float4(0.f, 0.f, 0.f, 0.f), in2 = in1;
-    edge_in(0, in1);
-    edge_in(1, in2);
-    __syncthreads();
-    if (!valid) return;
     const double df = INTR ? w.x[PD * p.P] : 0.0;
-    const int py = pix / p.W, px = pix - py * p.W;
+    const int py = valid ? pix / p.W : 0, px = valid ? pix - py * p.W : 0;
     const float X0 = ((float)px - cx) / fx;
     const float Y0 = ((float)py - cy) / fy;
     float dot = 0.f, Ef = 0.f;
-    for (int s = 0; s < deg; s++) {
-        const float4 in = in1;
-        in1 = in2;
-        edge_in(s + 2, in2);
-        PixTerms T;
-        if (rob) {
-            // the frame kernel's robust weights: residuals at the same linearisation point
-            pixel_terms<INTR, true, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y, in.z,
-                                        in.w, a.iscale, T);
-            T.wu *= robust_weight(T.ru, a.rk);
-            T.wv *= robust_weight(T.rv, a.rk);
-        } else {
-            pixel_terms<INTR, false, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y, 0.f,
-                                         0.f, a.iscale, T);
+    // edges in groups of kMaxDeg (one group unless the frame is wide)
+    for (int g0 = 0; g0 < deg; g0 += kMaxDeg) {
+        const int gn = min(kMaxDeg, deg - g0);
+        if (g0 > 0) {
+            __syncthreads();  // the previous group's set-up is read
+            setup(g0, gn);
         }
-        const float wdu = T.wu * act, wdv = T.wv * act;
-        const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
-        float s6 = 0.f;
+        float4 in1 = make_float4(0.f, 0.f, 0.f, 0.f), in2 = in1;
+        edge_in(g0, g0 + gn, in1);
+        edge_in(g0 + 1, g0 + gn, in2);
+        __syncthreads();
+        if (!valid) continue;
+        for (int s = 0; s < gn; s++) {
+            const float4 in = in1;
+            in1 = in2;
+            edge_in(g0 + s + 2, g0 + gn, in2);
+            PixTerms T;
+            if (rob) {
+                // the frame kernel's robust weights: residuals at the same linearisation point
+                pixel_terms<INTR, true, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y,
+                                            in.z, in.w, a.iscale, T);
+                T.wu *= robust_weight(T.ru, a.rk);
+                T.wv *= robust_weight(T.rv, a.rk);
+            } else {
+                pixel_terms<INTR, false, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y,
+                                             0.f, 0.f, a.iscale, T);
+            }
+            const float wdu = T.wu * act, wdv = T.wv * act;
+            const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
+            float s6 = 0.f;
 #pragma unroll
-        for (int n = 0; n < 6; n++) s6 += (cu * T.Ju[n] + cv * T.Jv[n]) * sU[s][n];
-        dot += s6;
-        if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
+            for (int n = 0; n < 6; n++) s6 += (cu * T.Ju[n] + cv * T.Jv[n]) * sU[s][n];
+            dot += s6;
+            if (INTR) Ef += wdu * T.Jzu * T.Jfu + wdv * T.Jzv * T.Jfv;
+        }
     }
+    if (!valid) return;
     float dz = Q * (wt - dot);
     if (INTR) dz = dz - Q * (Ef * act) * (float)df;
     a.dz[(int64_t)k * HW + pix] = dz;
@@ -1971,8 +2190,11 @@ __global__ __launch_bounds__(1024) void chol_solve_full_kernel(const double* __r
                                                                const double* __restrict__ LinvT, int ld, int nb,
                                                                const double* __restrict__ rhs,
                                                                double* __restrict__ out, const int* flag) {
-    __shared__ double zs[kMaxLd];
-    __shared__ double xs[kMaxLd];
+    // z of the forward sweep, overwritten block by block with x by the backward
+    // sweep (block kb reads its z before it stores its x, and reads x only of
+    // the blocks after it)
+    extern __shared__ double zs[];  // [ld]
+    double* xs = zs;
     __shared__ double Xk[kCB][kCB + 1];
     __shared__ double t[kCB];
     __shared__ double part[16][kCB];
@@ -2238,6 +2460,7 @@ struct vh_ba_plan {
     // workspace offsets
     size_t o_edge, o_gram, o_fen, o_Q, o_W, o_ed, o_gd, o_fenk, o_A, o_b, o_x, o_y, o_Li, o_flag;
     size_t o_esum, o_ybar, o_mu;  // adjoint
+    size_t o_wide;                // E rows of the wide frames
     int gram_blocks = 0;          // Gram blocks over all depth frames
     int TP7 = 64;                 // frame kernel pixel tile for Sim3 runs
     // plan variants: the system size depends on whether intrinsics are optimised
@@ -2326,9 +2549,9 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
     const int P = t1 - t0;
     const int HW = H * W;
     pl->P = P;
-    if (6 * P + 1 > kMaxLd) {
+    if (P > kMaxPoses) {
         delete pl;
-        fail(VH_ERR_UNSUPPORTED, "ba_plan_create: %d poses in the window (limit %d)", P, (kMaxLd - 1) / 6);
+        fail(VH_ERR_UNSUPPORTED, "ba_plan_create: %d poses in the window (limit %d)", P, kMaxPoses);
         return nullptr;
     }
     // kx = unique(cat(arange(t0,t1), ii)), sorted (geom_kernels.cu:1648-1652)
@@ -2349,7 +2572,8 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         std::vector<int> fill(fr_ptr.begin(), fr_ptr.end() - 1);
         for (int e = 0; e < E; e++) fr_edges[fill[slot[ii_host[e]]]++] = e;
     }
-    int deg_max = 0, Rext_max = 1;
+    int deg_max = 0, Rext_max = 1, wide_rows = 0;
+    std::vector<int> wide;
     for (int k = 0; k < K; k++) {
         std::vector<int64_t> js;
         for (int q = fr_ptr[k]; q < fr_ptr[k + 1]; q++) {
@@ -2361,12 +2585,13 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         fr_self[k] = (cand[k] >= t0 && cand[k] < t1) ? 1 : 0;
         const int deg = fr_ptr[k + 1] - fr_ptr[k];
         deg_max = std::max(deg_max, deg);
-        Rext_max = std::max(Rext_max, fr_self[k] + deg + 1);
-    }
-    if (deg_max > kMaxDeg || Rext_max > kMaxRext) {
-        delete pl;
-        fail(VH_ERR_UNSUPPORTED, "ba_plan_create: a depth frame has %d outgoing edges (limit %d)", deg_max, kMaxDeg);
-        return nullptr;
+        if (deg > kMaxDeg) {
+            // stored rows: self block (sized for 7-dof poses), 6 per edge, the pseudo pair and the zero row
+            wide.push_back(k);
+            wide_rows = std::max(wide_rows, fr_self[k] * 7 + 6 * deg + 3);
+        } else {
+            Rext_max = std::max(Rext_max, fr_self[k] + deg + 1);
+        }
     }
     // Gram storage, in blocks of PD x PD: frame k holds Rext_k (Rext_k+1)/2 blocks
     std::vector<int> gram_off(K + 1, 0), pseudo_off(K);
@@ -2487,7 +2712,7 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         {&kx32, &pl->d.kx}, {&fr_ptr, &pl->d.fr_ptr}, {&fr_edges, &pl->d.fr_edges}, {&fr_self, &pl->d.fr_self},
         {&gram_off, &pl->d.gram_off}, {&ii32, &pl->d.ii}, {&jj32, &pl->d.jj}, {&blk_ab, &pl->d.blk_ab},
         {&blk_ptr, &pl->d.blk_ptr}, {&blk_items, &pl->d.blk_items}, {&rhs_ptr, &pl->d.rhs_ptr},
-        {&rhs_items, &pl->d.rhs_items}, {&pseudo_off, &pl->d.pseudo_off}};
+        {&rhs_items, &pl->d.rhs_items}, {&pseudo_off, &pl->d.pseudo_off}, {&wide, &pl->d.wide}};
     size_t total = 0;
     std::vector<size_t> offs;
     for (auto& t : tabs) {
@@ -2515,6 +2740,8 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
     DevPlan& d = pl->d;
     d.E = E; d.N = N; d.H = H; d.W = W; d.HW = HW; d.t0 = t0; d.t1 = t1; d.P = P; d.K = K;
     d.TP = TP; d.SUB = SUB; d.T = T; d.Rext_max = Rext_max;
+    d.n_wide = (int)wide.size();
+    d.wide_rows = wide_rows;
     d.nlb = nlb;
     pl->gram_blocks = gram_off[K];
     // SE3 sizes; ba_run_impl sets them for the run's pose group
@@ -2546,6 +2773,7 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
     pl->o_esum = o; o += align256((size_t)std::max(E, 1) * 6 * 8);
     pl->o_ybar = o; o += align256((size_t)ldmax * 8);
     pl->o_mu = o; o += align256((size_t)ldmax * 8);
+    pl->o_wide = o; o += align256((size_t)wide.size() * T * wide_rows * kThreads * 4);
     pl->ws_bytes = o;
     return pl;
 }
@@ -2569,6 +2797,30 @@ size_t vh_ba_workspace_size(const vh_ba_plan* plan) { return plan ? plan->ws_byt
 int vh_ba_plan_pixel_tile(const vh_ba_plan* plan) { return plan ? plan->d.TP : -1; }
 
 }  // extern "C"
+
+// The wide-frame kernel over the plan's wide frames, in the frame kernel's variant.
+template <int PD>
+static void launch_wide(const DevPlan& d, const RunArgs& a, const Work& w, bool motion, bool intr, bool robust,
+                        hipStream_t s) {
+    const dim3 g(d.T, d.n_wide), b(kThreads);
+#define VH_WIDE(I, M, R) hipLaunchKernelGGL((ba_frame_wide_kernel<I, M, R, PD>), g, b, 0, s, d, a, w)
+    if (robust) {
+        if (motion) {
+            if (intr) VH_WIDE(true, true, true);
+            else VH_WIDE(false, true, true);
+        } else {
+            if (intr) VH_WIDE(true, false, true);
+            else VH_WIDE(false, false, true);
+        }
+    } else if (motion) {
+        if (intr) VH_WIDE(true, true, false);
+        else VH_WIDE(false, true, false);
+    } else {
+        if (intr) VH_WIDE(true, false, false);
+        else VH_WIDE(false, false, false);
+    }
+#undef VH_WIDE
+}
 
 // The launches of `iterations` Gauss-Newton iterations for pose blocks of PD.
 template <int PD>
@@ -2598,6 +2850,10 @@ static int ba_iterations(const DevPlan& d, RunArgs a, const Work& w, int iterati
                 else hipLaunchKernelGGL((ba_frame_kernel<false, false, false, PD>), fgrid, dim3(kThreads), lds, s, d, a, w);
             }
             VH_LAUNCH_CHECK("ba_frame_kernel");
+            if (d.n_wide > 0) {
+                launch_wide<PD>(d, a, w, motion, intr, rk.type != 0, s);
+                VH_LAUNCH_CHECK("ba_frame_wide_kernel");
+            }
             hipLaunchKernelGGL(ba_finalize_kernel<PD>, dim3(d.E + (motion ? 0 : d.K * kGramSplit)), dim3(kThreads), 0,
                                s, d, a, w, motion ? 1 : 0);
             VH_LAUNCH_CHECK("ba_finalize_kernel");
@@ -2619,8 +2875,8 @@ static int ba_iterations(const DevPlan& d, RunArgs a, const Work& w, int iterati
                                kb, w.flag);
             VH_LAUNCH_CHECK("chol_step_kernel");
         }
-        hipLaunchKernelGGL(chol_backsolve_kernel, dim3(1), dim3(kBT), 0, s, w.A, w.Li, d.ld, d.nb, w.b, w.y, w.x,
-                           w.flag, PD * d.P, dx);
+        hipLaunchKernelGGL(chol_backsolve_kernel, dim3(1), dim3(kBT), (size_t)d.ld * 8, s, w.A, w.Li, d.ld, d.nb, w.b,
+                           w.y, w.x, w.flag, PD * d.P, dx);
         VH_LAUNCH_CHECK("chol_backsolve_kernel");
         if (!motion && d.K > 0) {
             const dim3 bgrid((d.HW + kThreads - 1) / kThreads, d.K);
@@ -2666,8 +2922,8 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
         if (pd * d.P + 1 > kMaxLd)
             return fail(VH_ERR_UNSUPPORTED, "ba_run: %d Sim3 poses in the window (limit %d)", d.P, (kMaxLd - 1) / 7);
         if (!gram_fits(d.Rext_max, pd) || frame_lds_bytes(d.Rext_max, plan->TP7, pd) > (size_t)kLdsMax)
-            return fail(VH_ERR_UNSUPPORTED, "ba_run: a depth frame has %d outgoing edges (Sim3 limit %d)",
-                        plan->deg_max, kMaxDegSim3);
+            return fail(VH_ERR_UNSUPPORTED, "ba_run: the frame kernel's 7-row blocks do not fit (%d E-row blocks)",
+                        d.Rext_max);
     }
     if (sim3) {
         // 256-pixel tiles either way, so the tile count T (and every buffer) is unchanged
@@ -2696,6 +2952,7 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
     w.Li = (double*)(ws + plan->o_Li);
     w.y = (double*)(ws + plan->o_y);
     w.flag = (int*)(ws + plan->o_flag);
+    w.wide_e = (float*)(ws + plan->o_wide);
     RunArgs a{};
     a.rk = rk;
     a.poses = poses; a.disps = disps; a.intrinsics = intrinsics; a.disps_sens = disps_sens;
@@ -2774,6 +3031,9 @@ int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps
     const bool motion = (flags & VH_BA_MOTION_ONLY) != 0;
     if (flags & VH_BA_OPTIMIZE_INTRINSICS) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: intrinsics optimisation");
     if (flags & VH_BA_SIM3) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: Sim3 poses (SE3 only)");
+    if (plan->d.n_wide > 0)
+        return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: a depth frame has %d outgoing edges (limit %d)", plan->deg_max,
+                    kMaxDeg);
     if (!poses || !disps || !intrinsics || !targets || !weights || !depth_active || !grad_dx || !workspace ||
         (!motion && !grad_dz) || (plan->d.E > 0 && (!grad_targets || !grad_weights)))
         return fail(VH_ERR_ARG, "ba_adjoint: null tensor pointer");
@@ -2791,6 +3051,7 @@ int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps
     w.x = (double*)(ws + plan->o_x);
     w.Li = (double*)(ws + plan->o_Li);
     w.flag = (int*)(ws + plan->o_flag);
+    w.wide_e = (float*)(ws + plan->o_wide);
     RunArgs a{};
     a.poses = const_cast<float*>(poses);
     a.disps = const_cast<float*>(disps);
@@ -2817,8 +3078,8 @@ int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps
     hipLaunchKernelGGL(ba_adj_pose_kernel, dim3(d.P + (d.ld - 6 * d.P + 63) / 64), dim3(64), 0, s, d, a, g,
                        (motion || d.E == 0) ? 1 : 0);
     VH_LAUNCH_CHECK("ba_adj_pose_kernel");
-    hipLaunchKernelGGL(chol_solve_full_kernel, dim3(1), dim3(1024), 0, s, w.A, w.Li, d.ld, d.nb, g.ybar, g.mu,
-                       w.flag);
+    hipLaunchKernelGGL(chol_solve_full_kernel, dim3(1), dim3(1024), (size_t)d.ld * 8, s, w.A, w.Li, d.ld, d.nb, g.ybar,
+                       g.mu, w.flag);
     VH_LAUNCH_CHECK("chol_solve_full_kernel");
     if (d.E > 0 && d.K > 0) {
         if (motion) hipLaunchKernelGGL(ba_adj_grad_kernel<true>, pgrid, dim3(kThreads), 0, s, d, a, w, g);
