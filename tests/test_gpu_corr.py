@@ -315,3 +315,35 @@ def test_altcorr_float64_forwards(C):
     iref = OC.altcorr_index_forward(pyr[0], pyr, cc.double(), ii, jj, r)
     assert idx.dtype == torch.float32
     assert (idx.double().cpu() - iref).abs().max().item() < 1e-6 * max(1.0, iref.abs().max().item())
+
+
+@pytest.mark.parametrize("C", [64, 128, 256])
+def test_altcorr_index_matrix_core_path(C):
+    """FP16 altcorr_index_forward on the matrix cores (32-pixel groups, their
+    union of window rows as 32 x 64 MFMA blocks) against the FP64 oracle:
+    smooth flow (the MFMA path), a row whose coordinates diverge (the per-pixel
+    fallback), windows past every border, rows of 80 pixels (a partial last
+    group), all 4 levels."""
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(9 + C)
+    Nf, M, H, W, r = 3, 4, 10, 80, 3
+    fmaps = (torch.randn(1, Nf, C, H, W, generator=g) / math.sqrt(C)).half()
+    pyr = OC.altcorr_pyramid(fmaps.double(), 4)
+    pyr_t = [p.half().contiguous() for p in pyr]
+    ii = torch.tensor([0, 1, 2, 0])
+    jj = torch.tensor([1, 2, 0, 2])
+    yy, xx = torch.meshgrid(torch.arange(H, dtype=torch.float32), torch.arange(W, dtype=torch.float32), indexing="ij")
+    flow = torch.stack([3.0 * torch.sin(0.05 * xx + 0.3 * yy), 2.0 * torch.cos(0.04 * xx - 0.2 * yy)], -1)
+    cc = (torch.stack([xx, yy], -1) + flow)[None, None].repeat(1, M, 1, 1, 1)
+    cc = cc + 0.3 * torch.randn(cc.shape, generator=g)
+    cc[0, 1, 4] = torch.rand(W, 2, generator=g) * torch.tensor([160.0, 40.0]) - 40.0  # diverging row
+    cc[0, 2, :2] -= torch.tensor([0.0, 12.0])   # above the top
+    cc[0, 3, :, :20] -= torch.tensor([30.0, 0.0])  # left of the left border
+    cc[0, 3, :, 60:] += torch.tensor([25.0, 0.0])  # right of the right border
+    (out,) = D.altcorr_index_forward(pyr_t[0].cuda(), [p.cuda() for p in pyr_t], cc.cuda(), ii.cuda(), jj.cuda(), r)
+    ref = OC.altcorr_index_forward(pyr_t[0].double(), [p.double() for p in pyr_t], cc.double(), ii, jj, r)
+    assert out.dtype == torch.float32 and out.shape == ref.shape
+    err = (out.double().cpu() - ref).abs().max().item()
+    print(f"[altcorr mma C={C}] max err {err:.3e} (max |ref| {ref.abs().max().item():.3e})")
+    assert err < 2e-5 * max(1.0, ref.abs().max().item())

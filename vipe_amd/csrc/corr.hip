@@ -1681,6 +1681,181 @@ bool altcorr_vec_ok(int C, int elem, const void* f1, const void* f2, int64_t lev
     return C % kChunk == 0 && (C * elem) % 16 == 0 && reinterpret_cast<uintptr_t>(f1) % 16 == 0 &&
            reinterpret_cast<uintptr_t>(f2) % 16 == 0 && level_bytes < (int64_t)kBufRange - (1 << 20);
 }
+
+// ---------------------------------------------------------------------------
+// altcorr indexed forward on the matrix cores (FP16 features).  One wave per
+// group of 32 consecutive source pixels of one (b, m, source row) at one
+// level; the four waves of a workgroup take four groups and never wait for
+// each other.  The wave walks the 32 windows' union of target rows, one
+// 32-column tile at a time: D = F1 F2^T (32 pixels x 32 target columns) on
+// v_mfma_f32_32x32x16_f16 with FP32 accumulation, both operands loaded
+// straight into the MFMA lane layout (the pixels' features once per group,
+// the next tile's target columns while a tile is computed).  Accumulator
+// elements that are taps of their pixel's window go to that pixel's 8 x 8
+// taps in the wave's LDS slot, and at the end every pixel mixes its taps
+// bilinearly into its 49 outputs.  Work per union row is 32 x 32 x C MACs per
+// tile, of which 32 x 8 x C are taps.  A group whose union spans more than 64
+// columns or kAltUY rows (coordinates that diverge across the 32 pixels) takes
+// the per-pixel wave path instead.
+// ---------------------------------------------------------------------------
+constexpr int kAltPix = 32;   // source pixels per group
+constexpr int kAltUX = 64;    // union columns handled on the matrix cores (two tiles)
+constexpr int kAltUY = 40;    // union rows handled on the matrix cores
+
+template <int R, int C>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void altcorr_index_mma_kernel(const __half* __restrict__ fmap1,
+                                                                const __half* __restrict__ f2l,
+                                                                const float* __restrict__ coords,
+                                                                const int64_t* __restrict__ ii,
+                                                                const int64_t* __restrict__ jj, float* __restrict__ out,
+                                                                int64_t B, int Nf, int M, int H, int W, int H2, int W2,
+                                                                int level, int L) {
+    constexpr int RD = 2 * R + 1, NT = RD + 1;
+    static_assert(NT == 8, "8 x 8 taps per window");
+    constexpr int KS = C / 16;  // MFMA steps over the channels
+    __shared__ float Sxs[4][kAltPix * NT * NT];  // taps, one slot per wave
+    __shared__ int sxbs[4][kAltPix], sybs[4][kAltPix];
+    __shared__ float sdxs[4][kAltPix], sdys[4][kAltPix];
+    __shared__ float Sw[4][NT][NT + 1];  // per-wave slots of the fallback path
+
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* Sx = Sxs[wv];
+    int* sxb = sxbs[wv];
+    int* syb = sybs[wv];
+    // group decomposition in 32 bits (the launch checks that the groups fit)
+    const uint32_t nxs = (uint32_t)(W + kAltPix - 1) / kAltPix;
+    const uint32_t ngroups = (uint32_t)(B * M) * (uint32_t)H * nxs;
+    const uint32_t gid = blockIdx.x * 4u + (uint32_t)wv;  // over (b * M + m, row, 32-pixel segment)
+    if (gid >= ngroups) return;
+    const uint32_t bm32 = gid / ((uint32_t)H * nxs);
+    const uint32_t rem = gid - bm32 * (uint32_t)H * nxs;
+    const int y = (int)(rem / nxs);
+    const int x0 = (int)(rem - (uint32_t)y * nxs) * kAltPix;
+    const int64_t bm = bm32;
+    const int64_t b = bm32 / (uint32_t)M;
+    const int m = (int)(bm32 - (uint32_t)b * (uint32_t)M);
+    const int64_t HW = (int64_t)H * W;
+    const int64_t src = ii[m], dst = jj[m];
+    const float scale = (float)(1 << level);
+    const __half* f1row = fmap1 + ((b * Nf + src) * HW + (int64_t)y * W) * C;
+    const __half* f2 = f2l + (b * Nf + dst) * (int64_t)H2 * W2 * C;
+    float* obase = out + ((bm * L + level) * RD * RD) * HW + (int64_t)y * W + x0;
+    const int np = min(kAltPix, W - x0);
+    const int pl = lane & 31;
+
+    // this lane's pixel (both half-waves hold the same 32) and the union box
+    int xb = 1 << 29, yb = 1 << 29;
+    float dx = 0.f, dy = 0.f;
+    if (pl < np) {
+        const int64_t ci = ((bm * H + y) * W + x0 + pl) * 2;
+        const float xs = coords[ci] / scale, ys = coords[ci + 1] / scale;
+        const float fx0 = floorf(xs), fy0 = floorf(ys);
+        dx = xs - fx0;
+        dy = ys - fy0;
+        xb = (int)fx0 - R;
+        yb = (int)fy0 - R;
+    }
+    int xmn = pl < np ? xb : INT_MAX, xmx = pl < np ? xb : INT_MIN;
+    int ymn = pl < np ? yb : INT_MAX, ymx = pl < np ? yb : INT_MIN;
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) {
+        xmn = min(xmn, __shfl_xor(xmn, o, 32));
+        xmx = max(xmx, __shfl_xor(xmx, o, 32));
+        ymn = min(ymn, __shfl_xor(ymn, o, 32));
+        ymx = max(ymx, __shfl_xor(ymx, o, 32));
+    }
+    xmn = __builtin_amdgcn_readfirstlane(xmn);
+    xmx = __builtin_amdgcn_readfirstlane(xmx);
+    ymn = __builtin_amdgcn_readfirstlane(ymn);
+    ymx = __builtin_amdgcn_readfirstlane(ymx);
+    if (xmx - xmn + NT > kAltUX || ymx - ymn + NT > kAltUY) {
+        // diverging coordinates: pixel by pixel, as altcorr_index_wave_kernel
+        for (int p = 0; p < np; p++) {
+            const int64_t ci = ((bm * H + y) * W + x0 + p) * 2;
+            altcorr_wave_pixel<__half, R, float>(f1row + (int64_t)(x0 + p) * C, f2, H2, W2, C, coords[ci] / scale,
+                                                 coords[ci + 1] / scale, Sw[wv], obase + p, HW);
+        }
+        return;
+    }
+    if (lane < kAltPix) {
+        sxb[lane] = xb;
+        syb[lane] = yb;
+        sdxs[wv][lane] = dx;
+        sdys[wv][lane] = dy;
+    }
+    for (int i = lane; i < kAltPix * NT * NT; i += 64) Sx[i] = 0.f;
+    const int X0 = max(xmn, 0), X1 = min(xmx + NT - 1, W2 - 1);
+    const int Y0 = max(ymn, 0), Y1 = min(ymx + NT - 1, H2 - 1);
+    const int ncol = X1 - X0 + 1;
+    const int tsh = ncol > 32 ? 1 : 0;  // log2 of the 32-column tiles per row
+    const int nsteps = (Y1 >= Y0 && ncol > 0) ? (Y1 - Y0 + 1) << tsh : 0;
+    const int kh = 8 * (lane >> 5);  // this lane's 8 channels of each 16-channel MFMA step
+    // A: pixel lane % 32, channels 16 k + kh
+    f16x8 av[KS];
+    {
+        // lanes past the group read its last pixel: their rows of D are never used
+        const __half* fp = f1row + (int64_t)(x0 + min(pl, np - 1)) * C + kh;
+#pragma unroll
+        for (int k = 0; k < KS; k++) av[k] = *reinterpret_cast<const f16x8*>(fp + 16 * k);
+    }
+    // B of step st: union row Y0 + st >> tsh, tile st & tsh; column 32 t + lane % 32.
+    // Loads are unconditional (a column past the union reads column X0; its
+    // column of D is never used), so that no branch waits for them.
+    auto loadB = [&](f16x8 (&bv)[KS], int st) {
+        const int yy = Y0 + (st >> tsh), col = 32 * (st & tsh) + pl;
+        const __half* fp = f2 + ((int64_t)yy * W2 + X0 + (col < ncol ? col : 0)) * C + kh;
+#pragma unroll
+        for (int k = 0; k < KS; k++) bv[k] = *reinterpret_cast<const f16x8*>(fp + 16 * k);
+    };
+    // dot products of step st; elements that are taps of their pixel go to Sx
+    auto stepTaps = [&](const f16x8 (&bv)[KS], int st) {
+        f32x16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[v] = 0.f;
+#pragma unroll
+        for (int k = 0; k < KS; k++) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av[k], bv[k], acc, 0, 0, 0);
+        // lane holds union column 32 t + lane % 32 of pixels 8 (v / 4) + 4 (lane / 32) + v % 4
+        const int yy = Y0 + (st >> tsh), col = 32 * (st & tsh) + pl;
+        const int xc = X0 + col;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int pm = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            const int c = xc - sxb[pm], r = yy - syb[pm];
+            if (col < ncol && (unsigned)c < (unsigned)NT && (unsigned)r < (unsigned)NT)
+                Sx[(pm * NT + r) * NT + c] = acc[v];
+        }
+    };
+    // a wavefront's LDS accesses complete in order: the slot writes above are
+    // seen by the reads below; the wave barrier keeps the compiler in order
+    __builtin_amdgcn_wave_barrier();
+    // the next step's loads are issued on every path (the last step reloads
+    // itself), so the wait before a step's MFMAs covers its own loads only
+    if (nsteps > 0) {
+        f16x8 b0[KS], b1[KS];
+        loadB(b0, 0);
+        for (int st = 0; st < nsteps; st += 2) {
+            loadB(b1, min(st + 1, nsteps - 1));
+            stepTaps(b0, st);
+            if (st + 1 >= nsteps) break;
+            loadB(b0, min(st + 2, nsteps - 1));
+            stepTaps(b1, st + 1);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // 49 outputs per pixel: lane (p, h) forms channels h, h + 2, ...
+    if (pl < np) {
+        const float* S = Sx + pl * NT * NT;
+        const float w00 = (1.f - dy) * (1.f - dx), w01 = (1.f - dy) * dx, w10 = dy * (1.f - dx), w11 = dy * dx;
+        for (int ch = lane >> 5; ch < RD * RD; ch += 2) {
+            const int ix = ch / RD, iy = ch - ix * RD;
+            const float v = S[iy * NT + ix] * w00 + S[iy * NT + ix + 1] * w01 + S[(iy + 1) * NT + ix] * w10 +
+                            S[(iy + 1) * NT + ix + 1] * w11;
+            obase[ch * HW + pl] = v;
+        }
+    }
+}
+
 }  // namespace
 }  // namespace vh
 
@@ -1779,6 +1954,21 @@ int vh_altcorr_index_forward(int dtype, int64_t B, int Nf, int M, int H, int W, 
             continue;
         }
         const bool vec = altcorr_vec_ok(C, elem, fmap1, levels[l], B * (int64_t)Nf * Hl[l] * Wl[l] * C * elem);
+        // FP16 on the matrix cores: 32-pixel groups; C = 64 or 128 (operands
+        // held in registers, whole channel steps of the per-pixel fallback)
+        if (dtype == VH_F16 && vec && (C == 64 || C == 128) &&
+            B * M * H * ceil_div(W, kAltPix) < ((int64_t)1 << 31)) {
+            const dim3 mgrid((unsigned)ceil_div(B * M * H * ceil_div(W, kAltPix), 4));
+#define VH_ALT_MMA(CC)                                                                                          \
+    hipLaunchKernelGGL((altcorr_index_mma_kernel<3, CC>), mgrid, dim3(256), 0, s, (const __half*)fmap1,          \
+                       (const __half*)levels[l], coords, ii, jj, (float*)corr, B, Nf, M, H, W, Hl[l], Wl[l], l, \
+                       num_levels)
+            if (C == 64) VH_ALT_MMA(64);
+            else VH_ALT_MMA(128);
+#undef VH_ALT_MMA
+            VH_LAUNCH_CHECK("altcorr_index_forward");
+            continue;
+        }
         // one wave per pixel when a channel step (8 lanes x 16 bytes) divides C
         if (vec && C % (8 * 16 / elem) == 0) {
             const dim3 wgrid((unsigned)ceil_div(n, 4));
