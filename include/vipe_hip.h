@@ -134,12 +134,17 @@ int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int 
                        const void* fmap2, const float* coords, int radius, void* corr, void* stream);
 /* altcorr_backward (droid.cpp:62, altcorr_kernel.cu:242-366, 426-455); F32 only
  * like the reference.  fmap1_grad [B,H1,W1,C], fmap2_grad [B,H2,W2,C] out, fully
- * written whatever N is, zeros when N == 0 (fmap2_grad is cleared by the call,
- * then accumulated with FP32 atomic adds, so its summation order varies from
- * run to run); coords_grad is not produced (the reference returns zeros). */
+ * written whatever N is, zeros when N == 0; coords_grad is not produced (the
+ * reference returns zeros).  With a workspace of
+ * vh_altcorr_backward_workspace_size bytes (device) the f2 gradient is gathered
+ * per target row with no atomics, the same on every run (C a multiple of 4,
+ * 16-byte aligned fmap1 and fmap2_grad, a row of W2 x C floats within LDS);
+ * otherwise (workspace NULL) it is cleared and accumulated with FP32 atomic adds
+ * like the reference, whose summation order varies from run to run. */
+size_t vh_altcorr_backward_workspace_size(int64_t B, int N, int H1, int W1);
 int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
                         const void* fmap2, const float* coords, const void* corr_grad, int radius, void* fmap1_grad,
-                        void* fmap2_grad, void* stream);
+                        void* fmap2_grad, void* workspace, void* stream);
 /* altcorr_index_forward (droid.cpp:50, altcorr_kernel.cu:140-240, 394-424).
  * fmap1 [B,Nf,H,W,C]; levels[l] [B,Nf,Hl[l],Wl[l],C]; coords [B,M,H,W,2] f32;
  * ii, jj [M] int64 (device); features F16/F32/F64 (sums in FP64 for F64);

@@ -59,7 +59,8 @@ _SIGS = {
     "vh_corr_build_workspace": (sz, [i32, i64, i32, i32, i32]),
     "vh_corr_build": (i32, [i32, i32, i32, i64, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp]),
     "vh_altcorr_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp]),
-    "vh_altcorr_backward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp]),
+    "vh_altcorr_backward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp]),
+    "vh_altcorr_backward_workspace_size": (C.c_size_t, [i64, i32, i32, i32]),
     "vh_altcorr_index_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, i32,
                                        vp, vp]),
     # bundle adjustment

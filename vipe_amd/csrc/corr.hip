@@ -979,13 +979,19 @@ __global__ __launch_bounds__(128) void altcorr_index_kernel(const T* __restrict_
 // its gradient row are contiguous accesses.  fmap1_grad of the pixel is summed
 // over the N sets and taps in registers and stored once; fmap2_grad takes FP32
 // atomic adds, the same scatter as the reference's kernel.
-template <int R>
+// GATHER: no f2 atomics; instead tap gradient t of (b, n, source pixel) goes to
+// taps[((b N + n) H1 W1 + pixel) 64 + t] (t = 8 row + column of the window) and
+// the window origin (xb, yb) to org[(b N + n) H1 W1 + pixel], for
+// altcorr_bwd_gather_kernel.
+template <int R, bool GATHER = false>
 __global__ __launch_bounds__(256) void altcorr_bwd_wave_kernel(const float* __restrict__ fmap1,
                                                                const float* __restrict__ fmap2,
                                                                const float* __restrict__ coords,
                                                                const float* __restrict__ gout, float* __restrict__ g1,
                                                                float* __restrict__ g2, int64_t B, int N, int H1,
-                                                               int W1, int H2, int W2, int C) {
+                                                               int W1, int H2, int W2, int C,
+                                                               float* __restrict__ taps = nullptr,
+                                                               int2* __restrict__ org = nullptr) {
     constexpr int RD = 2 * R + 1;
     constexpr int NT = RD + 1;
     const int64_t HW1 = (int64_t)H1 * W1;
@@ -1036,13 +1042,130 @@ __global__ __launch_bounds__(256) void altcorr_bwd_wave_kernel(const float* __re
                     if (cok) {
                         const int64_t off = ((int64_t)h2 * W2 + w2) * C + c;
                         acc += gt[iy][ix] * f2b[off];
-                        atomicAdd(g2b + off, gt[iy][ix] * a);
+                        if (!GATHER) atomicAdd(g2b + off, gt[iy][ix] * a);
                     }
                 }
             }
         }
         if (cok) g1[(b * HW1 + yx) * C + c] = acc;
     }
+    if (GATHER) {
+        // lane t: tap (row t / 8, column t % 8) from the (up to) four outputs it feeds
+        const int ty = lane >> 3, tx = lane & 7;
+        for (int n = 0; n < N; n++) {
+            const int64_t bn = b * N + n;
+            const int64_t gid = bn * HW1 + yx;
+            const float x = coords[gid * 2 + 0], y = coords[gid * 2 + 1];
+            const float fx0 = floorf(x), fy0 = floorf(y);
+            const float dx = x - fx0, dy = y - fy0;
+            const float* go = gout + bn * RD * RD * HW1 + yx;
+            float gv = 0.f;
+            if (tx < RD && ty < RD) gv += go[(tx * RD + ty) * HW1] * (1 - dy) * (1 - dx);
+            if (tx >= 1 && ty < RD) gv += go[((tx - 1) * RD + ty) * HW1] * (1 - dy) * dx;
+            if (tx < RD && ty >= 1) gv += go[(tx * RD + ty - 1) * HW1] * dy * (1 - dx);
+            if (tx >= 1 && ty >= 1) gv += go[((tx - 1) * RD + ty - 1) * HW1] * dy * dx;
+            taps[gid * NT * NT + lane] = gv;
+            if (lane == 0) org[gid] = make_int2((int)fx0 - R, (int)fy0 - R);
+        }
+    }
+}
+
+// fmap2 gradient without atomics: one workgroup per target row (b, y2).  The
+// windows (n, source pixel) of batch b are scanned in ascending order, those
+// that cover row y2 are compacted (order kept), and kAltWin at a time their
+// tap gradients of this row and their source features are staged in LDS.
+// Element (x2, channel group cg) of the row is owned by the thread of slot
+// (x2 mod 8) * C/4 + cg, which adds each window's term in window order: the
+// sums need no atomics and come out the same on every run.  The row
+// accumulates in LDS and is stored once (zeros where no window reaches).
+constexpr int kAltWin = 64;  // windows staged per pass
+
+template <int R>
+__global__ __launch_bounds__(256) void altcorr_bwd_gather_kernel(const float* __restrict__ fmap1,
+                                                                 const float* __restrict__ taps,
+                                                                 const int2* __restrict__ org, float* __restrict__ g2,
+                                                                 int64_t B, int N, int H1, int W1, int H2, int W2,
+                                                                 int C, int nwin) {
+    constexpr int NT = 2 * R + 2;
+    static_assert((NT & (NT - 1)) == 0, "window width a power of two (column phases)");
+    extern __shared__ __align__(16) float gdyn[];
+    const int C4 = C / 4;
+    float4* acc = reinterpret_cast<float4*>(gdyn);              // [W2][C4] the row's gradient
+    float4* sF = acc + (int64_t)W2 * C4;                         // [nwin][C4] source features
+    float* sG = reinterpret_cast<float*>(sF + (int64_t)nwin * C4);  // [nwin][8] tap gradients of the row
+    int* sX = reinterpret_cast<int*>(sG + nwin * NT);            // [nwin] window column origin
+    int* sP = sX + nwin;                                         // [nwin] source pixel
+    __shared__ int wl[256];
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t b = blockIdx.x / H2;
+    const int y2 = (int)(blockIdx.x - b * H2);
+    const int64_t HW1 = (int64_t)H1 * W1;
+    const int64_t nw = (int64_t)N * HW1;  // windows of batch b
+    for (int i = tid; i < W2 * C4; i += 256) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t s0 = 0; s0 < nw; s0 += 256) {
+        // compact the covering windows of this scan chunk, in order
+        const int64_t w = s0 + tid;
+        bool cov = false;
+        if (w < nw) {
+            const int yb = org[b * nw + w].y;
+            cov = y2 >= yb && y2 < yb + NT;
+        }
+        const uint64_t bal = __ballot(cov);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wv] = __popcll(bal);
+        __syncthreads();
+        int base = 0, total = 0;
+        for (int q = 0; q < 4; q++) {
+            base += q < wv ? wsum[q] : 0;
+            total += wsum[q];
+        }
+        if (cov) wl[base + before] = tid;
+        __syncthreads();
+        for (int sub = 0; sub < total; sub += nwin) {
+            const int cnt = min(nwin, total - sub);
+            for (int j = tid; j < cnt; j += 256) {
+                const int64_t id = s0 + wl[sub + j];  // n * HW1 + pixel
+                const int2 o = org[b * nw + id];
+                sX[j] = o.x;
+                sP[j] = (int)(id % HW1);
+                const float* tp = taps + (b * nw + id) * NT * NT + (y2 - o.y) * NT;
+#pragma unroll
+                for (int t = 0; t < NT; t++) sG[j * NT + t] = tp[t];
+            }
+            for (int i = tid; i < cnt * C4; i += 256) {
+                const int j = i / C4, cg = i - j * C4;
+                const int64_t id = s0 + wl[sub + j];
+                sF[i] = *reinterpret_cast<const float4*>(fmap1 + (b * HW1 + id % HW1) * C + 4 * cg);
+            }
+            __syncthreads();
+            for (int sl = tid; sl < NT * C4; sl += 256) {
+                const int ph = sl / C4, cg = sl - ph * C4;
+                for (int j = 0; j < cnt; j++) {
+                    const int xb = sX[j];
+                    const int tx = (ph - xb) & (NT - 1);  // the window column on this slot's phase
+                    const int x2 = xb + tx;
+                    if (x2 < 0 || x2 >= W2) continue;
+                    const float gv = sG[j * NT + tx];
+                    const float4 f = sF[j * C4 + cg];
+                    float4 a = acc[x2 * C4 + cg];
+                    a.x += gv * f.x;
+                    a.y += gv * f.y;
+                    a.z += gv * f.z;
+                    a.w += gv * f.w;
+                    acc[x2 * C4 + cg] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    float4* out = reinterpret_cast<float4*>(g2 + ((b * H2 + y2) * (int64_t)W2) * C);
+    for (int i = tid; i < W2 * C4; i += 256) out[i] = acc[i];
+}
+
+// LDS bytes of altcorr_bwd_gather_kernel with nwin windows per pass
+inline size_t altcorr_gather_lds(int W2, int C, int nwin) {
+    return (size_t)W2 * C * 4 + (size_t)nwin * C * 4 + (size_t)nwin * (8 + 2) * 4;
 }
 
 // ---------------------------------------------------------------------------
@@ -1913,24 +2036,56 @@ int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int 
     return VH_OK;
 }
 
+size_t vh_altcorr_backward_workspace_size(int64_t B, int N, int H1, int W1) {
+    const int64_t nwin = B * (int64_t)N * H1 * W1;
+    return (size_t)(nwin > 0 ? nwin : 0) * (64 * sizeof(float) + sizeof(int2)) + 256;
+}
+
 int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
                         const void* fmap2, const float* coords, const void* corr_grad, int radius, void* fmap1_grad,
-                        void* fmap2_grad, void* stream) {
+                        void* fmap2_grad, void* workspace, void* stream) {
     if (dtype != VH_F32) return fail(VH_ERR_UNSUPPORTED, "altcorr_backward: float32 only");
     if (radius != 3) return fail(VH_ERR_UNSUPPORTED, "altcorr_backward: radius %d (kernels built for r=3)", radius);
-    // Both gradients are always fully written.  fmap2_grad is cleared here (a
-    // fill kernel, so the call can be captured), then accumulated; fmap1_grad
-    // is stored by the wave that owns its pixel (zeros when N == 0).
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nwin = B * (int64_t)N * H1 * W1;
+    // the gather path: C in whole float4 groups and a row of fmap2_grad in LDS
+    int gw = 0;
+    if (workspace && C % 4 == 0 && (reinterpret_cast<uintptr_t>(fmap1) & 15) == 0 &&
+        (reinterpret_cast<uintptr_t>(fmap2_grad) & 15) == 0)
+        for (int w = kAltWin; w >= 8 && !gw; w /= 2)
+            if (altcorr_gather_lds(W2, C, w) <= 150 * 1024) gw = w;
+    if (gw) {
+        // fmap1_grad and the tap gradients per window, then fmap2_grad by target
+        // rows: both gradients fully written, no atomics
+        float* taps = (float*)workspace;
+        int2* org = (int2*)((char*)workspace + ((size_t)std::max<int64_t>(nwin, 0) * 64 * sizeof(float) + 255) / 256 * 256);
+        if (B * (int64_t)H1 * W1 * C > 0) {
+            hipLaunchKernelGGL((altcorr_bwd_wave_kernel<3, true>), dim3(ceil_div(B * (int64_t)H1 * W1, 4)), dim3(256),
+                               0, s, (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad,
+                               (float*)fmap1_grad, (float*)fmap2_grad, B, N, H1, W1, H2, W2, C, taps, org);
+            VH_LAUNCH_CHECK("altcorr_backward");
+        }
+        if (B * (int64_t)H2 * W2 * C > 0) {
+            hipLaunchKernelGGL(altcorr_bwd_gather_kernel<3>, dim3((unsigned)(B * H2)), dim3(256),
+                               altcorr_gather_lds(W2, C, gw), s, (const float*)fmap1, taps, org, (float*)fmap2_grad,
+                               B, N, H1, W1, H2, W2, C, gw);
+            VH_LAUNCH_CHECK("altcorr_backward gather");
+        }
+        return VH_OK;
+    }
+    // without a workspace (or a row too wide for LDS): fmap2_grad is cleared here
+    // (a fill kernel, so the call can be captured), then accumulated with FP32
+    // atomic adds; fmap1_grad is stored by the wave that owns its pixel
     const int64_t n2 = B * (int64_t)H2 * W2 * C;
     if (n2 > 0) {
         hipLaunchKernelGGL(zero_fill_kernel<float>, dim3((unsigned)std::min<int64_t>(ceil_div(n2, 256), 65535)),
-                           dim3(256), 0, (hipStream_t)stream, (float*)fmap2_grad, n2);
+                           dim3(256), 0, s, (float*)fmap2_grad, n2);
         VH_LAUNCH_CHECK("altcorr_backward clear");
     }
     if (B * (int64_t)H1 * W1 * C == 0) return VH_OK;
-    hipLaunchKernelGGL(altcorr_bwd_wave_kernel<3>, dim3(ceil_div(B * (int64_t)H1 * W1, 4)), dim3(256), 0,
-                       (hipStream_t)stream, (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad,
-                       (float*)fmap1_grad, (float*)fmap2_grad, B, N, H1, W1, H2, W2, C);
+    hipLaunchKernelGGL(altcorr_bwd_wave_kernel<3>, dim3(ceil_div(B * (int64_t)H1 * W1, 4)), dim3(256), 0, s,
+                       (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad, (float*)fmap1_grad,
+                       (float*)fmap2_grad, B, N, H1, W1, H2, W2, C);
     VH_LAUNCH_CHECK("altcorr_backward");
     return VH_OK;
 }

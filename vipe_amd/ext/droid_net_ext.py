@@ -95,15 +95,19 @@ def altcorr_backward(fmap1, fmap2, coords, corr_grad, radius):
     _gpu(fmap1, fmap2, coords, corr_grad)
     B, N, H1, W1, _ = coords.shape
     _, H2, W2, Cc = fmap2.shape
-    # both are fully written by the call (fmap2_grad is cleared on the stream first)
+    # both are fully written by the call; the workspace holds the per-window tap
+    # gradients from which fmap2_grad is gathered without atomics
     fmap1_grad = torch.empty((B, H1, W1, fmap1.shape[3]), dtype=fmap1.dtype, device=fmap1.device)
     fmap2_grad = torch.empty((B, H2, W2, Cc), dtype=fmap1.dtype, device=fmap1.device)
     coords_grad = torch.zeros((B, N, H1, W1, 2), dtype=fmap1.dtype, device=fmap1.device)
+    lib = _lib.load()
+    ws = torch.empty((int(lib.vh_altcorr_backward_workspace_size(B, N, H1, W1)),), dtype=torch.uint8,
+                     device=fmap1.device)
     with torch.cuda.device(fmap1.device):
-        rc = _lib.load().vh_altcorr_backward(_code(fmap1, (torch.float32,)), B, N, H1, W1, H2, W2, fmap1.shape[3],
-                                             _lib.ptr(fmap1), _lib.ptr(fmap2), _lib.ptr(coords), _lib.ptr(corr_grad),
-                                             int(radius), _lib.ptr(fmap1_grad), _lib.ptr(fmap2_grad),
-                                             _lib.stream_of(fmap1))
+        rc = lib.vh_altcorr_backward(_code(fmap1, (torch.float32,)), B, N, H1, W1, H2, W2, fmap1.shape[3],
+                                     _lib.ptr(fmap1), _lib.ptr(fmap2), _lib.ptr(coords), _lib.ptr(corr_grad),
+                                     int(radius), _lib.ptr(fmap1_grad), _lib.ptr(fmap2_grad), _lib.ptr(ws),
+                                     _lib.stream_of(fmap1))
     _lib.check(rc, "altcorr_backward")
     return [fmap1_grad, fmap2_grad, coords_grad]
 
