@@ -1387,6 +1387,27 @@ __device__ __forceinline__ void store_run<__half, 2>(__half* dst, const float (&
     }
 }
 
+// A whole FP16 tile row (8 elements, 16 bytes; the tile's padding columns
+// hold zeros in LDS as well).  Other output types never take this path.
+template <typename Tout>
+__device__ __forceinline__ void store_run8(Tout* dst, const float (&v)[8]) {
+    if constexpr (sizeof(Tout) == 2) {
+        uint4 u;
+        unsigned int w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const Tout a = from_f<Tout>(v[2 * k]), b = from_f<Tout>(v[2 * k + 1]);
+            w[k] = (unsigned int)__builtin_bit_cast(unsigned short, a) |
+                   ((unsigned int)__builtin_bit_cast(unsigned short, b) << 16);
+        }
+        u.x = w[0];
+        u.y = w[1];
+        u.z = w[2];
+        u.w = w[3];
+        *reinterpret_cast<uint4*>(dst) = u;
+    }
+}
+
 // Workgroup barrier that orders LDS only.  The build kernel never reads back
 // what it writes to global memory, so its stores need not drain at each
 // barrier (__syncthreads would wait for them).
@@ -1397,7 +1418,7 @@ struct LevelGeoms {
 };
 
 template <typename Mma, typename Tout, bool FP8>
-__global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* __restrict__ A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void corr_build_kernel(const typename Mma::T* __restrict__ A,
                                                          const typename Mma::T* __restrict__ B,
                                                          const float* __restrict__ s1, const float* __restrict__ s2,
                                                          Tout* __restrict__ o0, Tout* __restrict__ o1,
@@ -1522,7 +1543,20 @@ __global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* 
             }
         lds_barrier();
         const bool vec0 = lg.g[0].tiled || (W % 4) == 0;
-        {
+        if (sizeof(Tout) == 2 && lg.g[0].tiled) {
+            // level 0, FP16 tiles: 8 target columns (one 16-byte tile row) per store
+#pragma unroll 4
+            for (int q = 0; q < kBM * kBN / 8 / 256; q++) {
+                const int idx = tid + q * 256;
+                const int px = (idx & 1) * 8, py = (idx >> 1) & (kPY - 1), ml = idx >> 4;
+                const int m = m0 + ml, y = y0 + py, x = x0 + px;
+                if (m >= HW || y >= H || x >= W) continue;
+                float vv[8];
+                load4(Os + ml * kOS + py * kPX + px, vv);
+                load4(Os + ml * kOS + py * kPX + px + 4, vv + 4);
+                store_run8(o0 + lg.g[0].at(ob + m, y, x), vv);
+            }
+        } else {
             // level 0: 4 target columns per store
 #pragma unroll 4
             for (int q = 0; q < kBM * kBN / 4 / 256; q++) {
@@ -1535,7 +1569,30 @@ __global__ __launch_bounds__(256) void corr_build_kernel(const typename Mma::T* 
                 store_run<Tout, 4>(o0 + lg.g[0].at(ob + m, y, x), vv, min(4, W - x), vec0);
             }
         }
-        if (L >= 2) {
+        if (L >= 2 && sizeof(Tout) == 2 && lg.g[1].tiled) {
+            // level 1 of the patch (4 x 8), FP16 tiles: one whole tile row (8
+            // columns, 16 bytes) per store, from rounded level 0
+            const int Hl = H / 2, Wl = W / 2;
+#pragma unroll
+            for (int q = 0; q < kBM * 4 / 256; q++) {
+                const int idx = tid + q * 256;
+                const int py = idx & 3, ml = idx >> 2;
+                float r0[16], r1[16];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    load4(Os + ml * kOS + (2 * py) * kPX + 4 * k, r0 + 4 * k);
+                    load4(Os + ml * kOS + (2 * py + 1) * kPX + 4 * k, r1 + 4 * k);
+                }
+                float vv[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
+                    P1[ml][py * 8 + u] = from_f<Tout>(vv[u]);
+                }
+                const int m = m0 + ml, y = y0 / 2 + py, x = x0 / 2;
+                if (m < HW && y < Hl && x < Wl) store_run8(o1 + lg.g[1].at(ob + m, y, x), vv);
+            }
+        } else if (L >= 2) {
             // level 1 of the patch (4 x 8), once each, from rounded level 0
             const int Hl = H / 2, Wl = W / 2;
             const bool vec = lg.g[1].tiled || (Wl % 4) == 0;
