@@ -360,6 +360,82 @@ int vh_graph_targets(int dtype, const float* coords1, const void* delta, const v
                      const int64_t* frame, int NM, int E, int H, int W, float* target, float* weight,
                      float* target_ba, float* weight_ba, void* stream);
 
+/* ---- DroidNet update operator (vipe/slam/networks/droid_net.py:393-519) ---
+ * The layers of UpdateModule.forward (:487-519), ConvGRU.forward (:406-420)
+ * and GraphAgg.forward (:434-449), FP16 activations and weights with FP32
+ * accumulation (the reference runs them under autocast, factor_graph.py:228).
+ * Activations are channels-last f16: [image][pixel][channel], pixel = y*W + x.
+ *
+ * vh_uo_conv: a 'same'-padded KHxKW convolution (odd sizes, stride 1) of the
+ * channel concatenation of up to 4 input segments (torch.cat(dim=1), :407-408,
+ * :417), each seg_c channels (a multiple of 32) at channel seg_off of a buffer
+ * with seg_stride channels per pixel (multiples of 8; 16-byte aligned
+ * pointers).  weight: f16 [KH*KW][Cout_pad][sum seg_c] (Cout_pad a multiple of
+ * 128, rows past Cout zero); bias f32 [Cout].  glo (may be NULL): f32
+ * [nimg][glo_stride], added at channel n before the activation (the
+ * convz_glo / convr_glo / convq_glo terms, :415-417).  Output element
+ * (pixel p, channel n) goes to out[p*out_stride + out_off + n] (f32 when
+ * out_f32, else f16), or with VH_UO_NCHW to
+ * out[((img*out_stride) + out_off + n)*H*W + pixel].  Epilogues:
+ *   VH_UO_NONE, VH_UO_RELU, VH_UO_SIGMOID   the activation
+ *   VH_UO_SIGMUL  sigmoid(x) * aux0[p][n]                  (:413)
+ *   VH_UO_GATES   Cout 256 = [convz; convr]: out[p][n] = z = sigmoid(x) for
+ *                 n < 128, out2[p*128 + n-128] = sigmoid(x) * aux0[p][n-128]
+ *                 (r * net) for the rest                   (:415-417)
+ *   VH_UO_GRU     Cout 128: (1 - aux0) * aux1 + aux0 * tanh(x), with aux0 = z
+ *                 and aux1 = net                           (:417-419)
+ * aux buffers are f16 with aux_stride channels per pixel. */
+#define VH_UO_NONE 0
+#define VH_UO_RELU 1
+#define VH_UO_SIGMOID 2
+#define VH_UO_ETA 3 /* 0.01 * softplus(x): GraphAgg's damping head, :430, :449 */
+#define VH_UO_SIGMUL 4
+#define VH_UO_GATES 5
+#define VH_UO_GRU 6
+#define VH_UO_NCHW 16 /* flag or-ed into the epilogue */
+typedef struct {
+    const void* seg[4];
+    int seg_c[4];
+    int seg_stride[4];
+    int seg_off[4];
+    int nseg;
+    const void* weight;
+    const float* bias;
+    int nimg, H, W, Cout, Cout_pad, KH, KW;
+    int epilogue;
+    void* out;
+    int out_f32, out_stride, out_off;
+    const float* glo;
+    int glo_stride;
+    const void* aux0;
+    const void* aux1;
+    int aux_stride;
+    void* out2;
+} vh_uo_conv_args;
+/* sizeof(vh_uo_conv_args), for bindings that lay the struct out themselves. */
+size_t vh_uo_conv_args_size(void);
+int vh_uo_conv(const vh_uo_conv_args* args, void* stream);
+/* The 1- and 2-channel heads (delta.2, weight.2 :475-485, eta :430): in f16
+ * channels [in_off, in_off + cin) of [P][in_stride]; weight f32
+ * [KH*KW][cin][Cout] (KH*KW*cin*Cout <= 4608), bias f32 [Cout]; out f32
+ * [P][out_stride]; epilogue VH_UO_NONE / RELU / SIGMOID / ETA. */
+int vh_uo_conv_small(const void* in, int in_stride, int in_off, int cin, const float* weight, const float* bias,
+                     int nimg, int H, int W, int KH, int KW, int Cout, int epilogue, float* out, int out_stride,
+                     void* stream);
+/* Layout changes at the module boundary (the reference's [B*N, C, H, W]
+ * views, :499-502): in [nimg][C][HW] (dtype VH_F32 / VH_F16) -> out f16
+ * [nimg][HW][Cpad], channels past C zero; and f16 [nimg][HW][stride]
+ * channels [off, off + C) -> out [nimg][C][HW] in dtype. */
+int vh_uo_to_nhwc(int dtype, const void* in, int nimg, int C, int HW, int Cpad, void* out, void* stream);
+int vh_uo_to_nchw(int dtype, const void* in, int nimg, int C, int HW, int stride, int off, void* out, void* stream);
+/* ConvGRU's global context (:413-414): out[img][c] = mean over the HW pixels
+ * of in f32 [nimg][HW][C], summed in a fixed order; and the 1x1 convolutions
+ * of that 1x1 map (:415-417): out[img][k] = sum_c weight[k][c] in[img][c] +
+ * bias[k], f32. */
+int vh_uo_mean(const float* in, int nimg, int HW, int C, float* out, void* stream);
+int vh_uo_fc(const float* in, int nimg, int Cin, const float* weight, const float* bias, int Cout, float* out,
+             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,4 +31,6 @@ scatter : scatter_ext sum / mul / mean / min / max with arg_out
 robust : robust kernels of the BA solver (vipe/slam/ba/kernel.py).
 graph_step : factor-graph update glue: reprojection, motion features,
         target / weight epilogue (vipe/slam/components/factor_graph.py).
+update_op : the DroidNet update operator, ConvGRU + GraphAgg + heads
+        (vipe/slam/networks/droid_net.py:393-519), in float64.
 """

@@ -47,3 +47,20 @@ def test_errors_are_reported_without_a_device():
     assert "dtype" in _lib.last_error()
     rc = lib.vh_lie_expm(9, _lib.VH_F32, 1, None, None, _lib.VH_HOST, None)
     assert rc == _lib.VH_ERR_UNDEFINED
+
+
+def test_update_op_argument_struct_matches_the_header():
+    from vipe_amd import _lib
+
+    lib = _lib.load()
+    assert lib.vh_uo_conv_args_size() == ctypes.sizeof(_lib.UoConvArgs)
+    a = _lib.UoConvArgs()
+    a.nseg = 0
+    assert lib.vh_uo_conv(ctypes.byref(a), None) == _lib.VH_ERR_ARG
+    assert "segments" in _lib.last_error()
+    a.nseg, a.seg_c[0], a.seg_stride[0], a.KH, a.KW = 1, 48, 64, 3, 3
+    a.seg[0] = 16
+    a.Cout, a.Cout_pad = 128, 128
+    assert lib.vh_uo_conv(ctypes.byref(a), None) == _lib.VH_ERR_ARG
+    assert "segment 0" in _lib.last_error()
+    assert lib.vh_uo_conv_small(None, 128, 0, 128, None, None, 1, 4, 4, 3, 3, 3, 0, None, 3, None) == _lib.VH_ERR_ARG

@@ -93,10 +93,32 @@ _SIGS = {
     # factor-graph update glue
     "vh_graph_motion": (i32, [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp]),
     "vh_graph_targets": (i32, [i32, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp, vp, vp]),
+    # DroidNet update operator
+    "vh_uo_conv_args_size": (sz, []),
+    "vh_uo_conv": (i32, [vp, vp]),
+    "vh_uo_conv_small": (i32, [vp, i32, i32, i32, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp, i32, vp]),
+    "vh_uo_to_nhwc": (i32, [i32, vp, i32, i32, i32, i32, vp, vp]),
+    "vh_uo_to_nchw": (i32, [i32, vp, i32, i32, i32, i32, i32, vp, vp]),
+    "vh_uo_mean": (i32, [vp, i32, i32, i32, vp, vp]),
+    "vh_uo_fc": (i32, [vp, i32, i32, vp, vp, i32, vp, vp]),
     # scatter_ext
     "vh_scatter_workspace_size": (sz, [i64] * 7),
     "vh_scatter": (i32, [i32, i32, vp, vp] + [i64] * 7 + [vp, vp, i32, vp, sz, vp]),
 }
+
+
+
+class UoConvArgs(C.Structure):
+    """vh_uo_conv_args (include/vipe_hip.h)."""
+    _fields_ = [("seg", vp * 4), ("seg_c", i32 * 4), ("seg_stride", i32 * 4), ("seg_off", i32 * 4), ("nseg", i32),
+                ("weight", vp), ("bias", vp), ("nimg", i32), ("H", i32), ("W", i32), ("Cout", i32),
+                ("Cout_pad", i32), ("KH", i32), ("KW", i32), ("epilogue", i32), ("out", vp), ("out_f32", i32),
+                ("out_stride", i32), ("out_off", i32), ("glo", vp), ("glo_stride", i32), ("aux0", vp), ("aux1", vp),
+                ("aux_stride", i32), ("out2", vp)]
+
+
+VH_UO_NONE, VH_UO_RELU, VH_UO_SIGMOID, VH_UO_ETA, VH_UO_SIGMUL, VH_UO_GATES, VH_UO_GRU = 0, 1, 2, 3, 4, 5, 6
+VH_UO_NCHW = 16
 
 _lib = None
 _load_error = None

@@ -1,0 +1,512 @@
+// The DroidNet update operator (vipe/slam/networks/droid_net.py:393-519:
+// ConvGRU, GraphAgg, UpdateModule) on gfx950, FP16 activations and weights with
+// FP32 accumulation, as the reference runs it under autocast.
+//
+// Activations live channels-last (NHWC: [image][pixel][channel], f16), so the
+// channels of a pixel are one contiguous row and a convolution is an implicit
+// GEMM: M = pixels, N = output channels, K = taps x input channels.
+//
+//   uo_conv_kernel        the implicit GEMM on v_mfma_f32_32x32x16_f16.  One
+//                         workgroup per 128 pixels x 128 output channels; each
+//                         of its 4 waves owns 64 x 64 as 2 x 2 tiles of 32 x 32.
+//                         K advances one 32-channel chunk of one tap of one input
+//                         segment at a time: the chunk of the 128 pixels (shifted
+//                         by the tap, zero outside the image) and of the 128
+//                         weight rows go to LDS, the next chunk's loads are in
+//                         flight meanwhile.  Up to four input segments stand in
+//                         for torch.cat along channels.  The epilogue adds the
+//                         bias (and a per-image addend) and applies the layer's
+//                         activation, or the ConvGRU gates / state update.
+//   uo_conv_small_kernel  convolutions with 1 or 2 output channels (the delta,
+//                         weight and eta heads): one lane per pixel on the VALU.
+//   uo_to_nhwc / uo_to_nchw_kernel  layout changes at the module boundary.
+//   uo_mean_kernel / uo_fc_kernel   ConvGRU's global context: the per-image mean
+//                         of sigmoid(w(net)) * net and the three 1x1 convolutions
+//                         applied to it.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+
+#include "vh_common.hpp"
+
+namespace vh {
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kUoBM = 128;  // pixels per workgroup
+constexpr int kUoBN = 128;  // output channels per workgroup
+constexpr int kUoKC = 32;   // channels per K chunk
+constexpr int kUoRS = kUoKC + 8;  // LDS row stride (halves): 80 bytes, 16-byte aligned rows
+
+struct ConvArgs {
+    const _Float16* seg[4];  // NHWC inputs
+    int seg_c[4];            // channels taken from each (a multiple of 32)
+    int seg_stride[4];       // channels per pixel of each buffer
+    int seg_off[4];          // first channel taken
+    int nseg, cin;           // segments, total input channels (the weights' row length)
+    const _Float16* w;       // [KH * KW][Cout_pad][cin]
+    const float* bias;       // [Cout]
+    int nimg, H, W, Cout, Cout_pad, KH, KW;
+    int epi;
+    void* out;
+    int out_f32, out_stride, out_off;
+    const float* glo;        // [nimg][glo_stride] addend at channel n, or null
+    int glo_stride;
+    const _Float16* aux0;    // gates: net; gru: z; sigmul: net   ([P][aux_stride])
+    const _Float16* aux1;    // gru: net
+    int aux_stride;
+    _Float16* out2;          // gates: r * net
+};
+
+__device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + __expf(-x)); }
+__device__ __forceinline__ float softplusf_(float x) { return x > 20.f ? x : log1pf(__expf(x)); }
+
+template <int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void uo_conv_kernel(ConvArgs a) {
+    __shared__ __align__(16) _Float16 As[kUoBM * kUoRS];
+    __shared__ __align__(16) _Float16 Bs[kUoBN * kUoRS];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+    const int HW = a.H * a.W;
+    const int P = a.nimg * HW;  // < 2^31, checked by the host
+    const int m0 = blockIdx.x * kUoBM;
+    const int n0 = blockIdx.y * kUoBN;
+    const int ph = a.KH / 2, pw = a.KW / 2;
+    // this thread's two staged chunks (pixel or weight row r, 8 channels at 8 q)
+    int r_[2], q_[2], img_[2], py_[2], px_[2];
+    bool pok_[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int i = tid + 256 * u;
+        r_[u] = i >> 2;
+        q_[u] = i & 3;
+        const int p = m0 + r_[u];
+        pok_[u] = p < P;
+        const int pc = pok_[u] ? p : 0;
+        img_[u] = pc / HW;
+        const int rem = pc - img_[u] * HW;
+        py_[u] = rem / a.W;
+        px_[u] = rem - py_[u] * a.W;
+    }
+    // K walks (tap, segment, 32-channel chunk) in that nesting; the cursor
+    // (uniform) holds the chunk whose loads are issued next.  The segment is
+    // picked by integer sums over the segment table, not by selecting among
+    // its entries: selects of kernel-argument loads become a pointer table in
+    // scratch memory.
+    const uint64_t sb0 = (uint64_t)a.seg[0];
+    const uint64_t sd1 = (uint64_t)a.seg[1] - sb0, sd2 = (uint64_t)a.seg[2] - (uint64_t)a.seg[1],
+                   sd3 = (uint64_t)a.seg[3] - (uint64_t)a.seg[2];
+    const int sc0 = a.seg_c[0], sc1 = a.seg_c[1], sc2 = a.seg_c[2], sc3 = a.seg_c[3];
+    const int ss0 = a.seg_stride[0], ss1 = a.seg_stride[1] - ss0, ss2 = a.seg_stride[2] - a.seg_stride[1],
+              ss3 = a.seg_stride[3] - a.seg_stride[2];
+    const int so0 = a.seg_off[0], so1 = a.seg_off[1] - so0, so2 = a.seg_off[2] - a.seg_off[1],
+              so3 = a.seg_off[3] - a.seg_off[2];
+    const int nseg = a.nseg, KW = a.KW, H = a.H, W = a.W, Cout_pad = a.Cout_pad, cin = a.cin;
+    const _Float16* wp = a.w;
+    const int per_tap = (sc0 + (nseg > 1 ? sc1 : 0) + (nseg > 2 ? sc2 : 0) + (nseg > 3 ? sc3 : 0)) / kUoKC;
+    const int nchunks = a.KH * KW * per_tap;
+    int cur_tap = 0, cur_sg = 0, cur_c0 = 0, cur_wc0 = 0;
+    uint64_t cur_sp = sb0;
+    int cur_sc = sc0, cur_ss = ss0, cur_so = so0;
+    uint4 va0, va1, vb0, vb1;
+#define UO_LOAD(u, va, vb)                                                                                          \
+    {                                                                                                               \
+        const int yy = py_[u] + dy, xx = px_[u] + dx;                                                               \
+        const bool ok = pok_[u] && yy >= 0 && yy < H && xx >= 0 && xx < W;                                         \
+        const int pix = img_[u] * HW + (ok ? yy * W + xx : 0);                                                      \
+        const uint4 v = *reinterpret_cast<const uint4*>((const _Float16*)cur_sp + (int64_t)pix * cur_ss + cur_so +  \
+                                                        cur_c0 + 8 * q_[u]);                                        \
+        va = ok ? v : make_uint4(0, 0, 0, 0);                                                                       \
+        vb = *reinterpret_cast<const uint4*>(wp + ((int64_t)cur_tap * Cout_pad + n0 + r_[u]) * cin + cur_wc0 +     \
+                                             8 * q_[u]);                                                            \
+    }
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
+    {
+        const int dy = -ph, dx = -pw;
+        UO_LOAD(0, va0, vb0)
+        UO_LOAD(1, va1, vb1)
+    }
+    for (int kc = 0; kc < nchunks; kc++) {
+        __syncthreads();  // the previous chunk's LDS reads are done
+        *reinterpret_cast<uint4*>(As + r_[0] * kUoRS + 8 * q_[0]) = va0;
+        *reinterpret_cast<uint4*>(Bs + r_[0] * kUoRS + 8 * q_[0]) = vb0;
+        *reinterpret_cast<uint4*>(As + r_[1] * kUoRS + 8 * q_[1]) = va1;
+        *reinterpret_cast<uint4*>(Bs + r_[1] * kUoRS + 8 * q_[1]) = vb1;
+        // the next chunk's loads (the last chunk reloads itself, so every path
+        // issues them and the MFMAs below wait for nothing)
+        if (kc + 1 < nchunks) {
+            cur_c0 += kUoKC;
+            cur_wc0 += kUoKC;
+            if (cur_c0 == cur_sc) {
+                cur_c0 = 0;
+                if (++cur_sg == nseg) {
+                    cur_sg = 0;
+                    cur_wc0 = 0;
+                    cur_tap++;
+                }
+                const int sg = cur_sg;
+                cur_sp = sb0 + (sg >= 1 ? sd1 : 0) + (sg >= 2 ? sd2 : 0) + (sg >= 3 ? sd3 : 0);
+                cur_sc = sg == 0 ? sc0 : sg == 1 ? sc1 : sg == 2 ? sc2 : sc3;
+                cur_ss = ss0 + (sg >= 1 ? ss1 : 0) + (sg >= 2 ? ss2 : 0) + (sg >= 3 ? ss3 : 0);
+                cur_so = so0 + (sg >= 1 ? so1 : 0) + (sg >= 2 ? so2 : 0) + (sg >= 3 ? so3 : 0);
+            }
+        }
+        {
+            const int dy = cur_tap / KW - ph, dx = cur_tap % KW - pw;
+            UO_LOAD(0, va0, vb0)
+            UO_LOAD(1, va1, vb1)
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kUoKC; kk += 16) {
+            const int k = kk + 8 * (lane >> 5);
+            const f16x8 a0 = *reinterpret_cast<const f16x8*>(As + (wm + (lane & 31)) * kUoRS + k);
+            const f16x8 a1 = *reinterpret_cast<const f16x8*>(As + (wm + 32 + (lane & 31)) * kUoRS + k);
+            const f16x8 b0 = *reinterpret_cast<const f16x8*>(Bs + (wn + (lane & 31)) * kUoRS + k);
+            const f16x8 b1 = *reinterpret_cast<const f16x8*>(Bs + (wn + 32 + (lane & 31)) * kUoRS + k);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, acc[1][1], 0, 0, 0);
+        }
+    }
+#undef UO_LOAD
+    // epilogue: lane holds output channel n = wn + 32 j + lane % 32 of pixels
+    // m = wm + 32 i + 8 (v / 4) + 4 (lane / 32) + v % 4
+    const bool nchw = (a.epi & VH_UO_NCHW) != 0;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int n = n0 + wn + 32 * j + (lane & 31);
+        if (n >= a.Cout) continue;
+        const float bn = a.bias[n];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int p = m0 + wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (p >= P) continue;
+                float x = acc[i][j][v] + bn;
+                int img = 0;
+                if (a.glo || nchw) img = (unsigned)p / (unsigned)HW;
+                if (a.glo) x += a.glo[img * a.glo_stride + n];
+                if constexpr (EPI == VH_UO_GATES) {
+                    // ConvGRU: channels [0, 128) are z, [128, 256) are r (r * net goes to out2)
+                    const float g = sigmoidf_(x);
+                    if (n < 128) {
+                        ((_Float16*)a.out)[(int64_t)p * a.out_stride + a.out_off + n] = (_Float16)g;
+                    } else {
+                        const float h = (float)a.aux0[(int64_t)p * a.aux_stride + (n - 128)];
+                        a.out2[(int64_t)p * 128 + (n - 128)] = (_Float16)(g * h);
+                    }
+                    continue;
+                }
+                if constexpr (EPI == VH_UO_GRU) {
+                    const float q = tanhf(x);
+                    const float z = (float)a.aux0[(int64_t)p * a.aux_stride + n];
+                    const float h = (float)a.aux1[(int64_t)p * a.aux_stride + n];
+                    x = (1.f - z) * h + z * q;
+                } else if constexpr (EPI == VH_UO_RELU) {
+                    x = fmaxf(x, 0.f);
+                } else if constexpr (EPI == VH_UO_SIGMOID) {
+                    x = sigmoidf_(x);
+                } else if constexpr (EPI == VH_UO_SIGMUL) {
+                    x = sigmoidf_(x) * (float)a.aux0[(int64_t)p * a.aux_stride + n];
+                }
+                const int64_t o = nchw ? ((int64_t)img * a.out_stride + a.out_off + n) * HW + (p - img * HW)
+                                       : (int64_t)p * a.out_stride + a.out_off + n;
+                if (a.out_f32) ((float*)a.out)[o] = x;
+                else ((_Float16*)a.out)[o] = (_Float16)x;
+            }
+    }
+}
+
+// Convolution with Cout <= 2 on the VALU: one lane per pixel, the weights in LDS.
+constexpr int kUoSmallMaxK = 9 * 256 * 2;  // taps x input channels x outputs held in LDS
+
+__global__ __launch_bounds__(256) void uo_conv_small_kernel(const _Float16* __restrict__ in, int in_stride, int in_off,
+                                                            int cin, const float* __restrict__ w,
+                                                            const float* __restrict__ bias, int nimg, int H, int W,
+                                                            int KH, int KW, int Cout, int epi, float* __restrict__ out,
+                                                            int out_stride) {
+    __shared__ float sw[kUoSmallMaxK];  // [tap][c][o]
+    const int nw = KH * KW * cin * Cout;
+    for (int i = threadIdx.x; i < nw; i += 256) sw[i] = w[i];
+    __syncthreads();
+    const int HW = H * W;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= (int64_t)nimg * HW) return;
+    const int img = (int)(p / HW);
+    const int rem = (int)(p - (int64_t)img * HW);
+    const int py = rem / W, px = rem - py * W;
+    float s0 = 0.f, s1 = 0.f;
+    for (int t = 0; t < KH * KW; t++) {
+        const int yy = py + t / KW - KH / 2, xx = px + t % KW - KW / 2;
+        if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
+        const _Float16* ip = in + ((int64_t)img * HW + yy * W + xx) * in_stride + in_off;
+        const float* wt = sw + t * cin * Cout;
+        for (int c = 0; c < cin; c += 8) {
+            const f16x8 v = *reinterpret_cast<const f16x8*>(ip + c);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float xv = (float)v[k];
+                s0 += xv * wt[(c + k) * Cout];
+                if (Cout > 1) s1 += xv * wt[(c + k) * Cout + 1];
+            }
+        }
+    }
+    float r[2] = {s0 + bias[0], Cout > 1 ? s1 + bias[1] : 0.f};
+    for (int o = 0; o < Cout; o++) {
+        float x = r[o];
+        if (epi == VH_UO_SIGMOID) x = sigmoidf_(x);
+        else if (epi == VH_UO_ETA) x = 0.01f * softplusf_(x);
+        else if (epi == VH_UO_RELU) x = fmaxf(x, 0.f);
+        out[p * out_stride + o] = x;
+    }
+}
+
+// [nimg][C][HW] (f32 or f16) -> [nimg][HW][Cpad] f16, channels past C zero.
+// One workgroup per (image, 64 pixels); the tile goes through LDS so both the
+// reads (along pixels) and the writes (along channels) are contiguous.
+template <typename T>
+__global__ __launch_bounds__(256) void uo_to_nhwc_kernel(const T* __restrict__ in, int nimg, int C, int HW, int Cpad,
+                                                         _Float16* __restrict__ out) {
+    __shared__ _Float16 tile[64][257];
+    const int npb = (HW + 63) / 64;
+    const int img = blockIdx.x / npb;
+    const int p0 = (blockIdx.x - img * npb) * 64;
+    const int np = min(64, HW - p0);
+    for (int c0 = 0; c0 < Cpad; c0 += 256) {
+        const int cn = min(256, Cpad - c0);
+        for (int i = threadIdx.x; i < cn * 64; i += 256) {
+            const int c = i / 64, pp = i % 64;
+            float v = 0.f;
+            if (c0 + c < C && pp < np) v = (float)in[((int64_t)img * C + c0 + c) * HW + p0 + pp];
+            tile[pp][c] = (_Float16)v;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < cn * 64; i += 256) {
+            const int pp = i / cn, c = i % cn;
+            if (pp < np) out[((int64_t)img * HW + p0 + pp) * Cpad + c0 + c] = tile[pp][c];
+        }
+        __syncthreads();
+    }
+}
+
+// [nimg][HW][stride] f16 (channels [off, off + C)) -> [nimg][C][HW] (f32 or f16).
+template <typename T>
+__global__ __launch_bounds__(256) void uo_to_nchw_kernel(const _Float16* __restrict__ in, int nimg, int C, int HW,
+                                                         int stride, int off, T* __restrict__ out) {
+    __shared__ _Float16 tile[64][257];
+    const int npb = (HW + 63) / 64;
+    const int img = blockIdx.x / npb;
+    const int p0 = (blockIdx.x - img * npb) * 64;
+    const int np = min(64, HW - p0);
+    for (int c0 = 0; c0 < C; c0 += 256) {
+        const int cn = min(256, C - c0);
+        for (int i = threadIdx.x; i < cn * 64; i += 256) {
+            const int pp = i / cn, c = i % cn;
+            if (pp < np) tile[pp][c] = in[((int64_t)img * HW + p0 + pp) * stride + off + c0 + c];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < cn * 64; i += 256) {
+            const int c = i / 64, pp = i % 64;
+            if (pp < np) out[((int64_t)img * C + c0 + c) * HW + p0 + pp] = (T)(float)tile[pp][c];
+        }
+        __syncthreads();
+    }
+}
+
+// Per-image mean over the pixels of [nimg][HW][C] f32: one workgroup per
+// (image, 64 channels); each thread sums a fixed strided set of pixels, then a
+// fixed-order tree, so the result is the same on every run.
+__global__ __launch_bounds__(256) void uo_mean_kernel(const float* __restrict__ in, int nimg, int HW, int C,
+                                                      float* __restrict__ out) {
+    __shared__ float part[4][64];
+    const int ncb = (C + 63) / 64;
+    const int img = blockIdx.x / ncb;
+    const int c = (blockIdx.x - img * ncb) * 64 + (threadIdx.x & 63);
+    const int g = threadIdx.x >> 6;
+    float s = 0.f;
+    if (c < C)
+        for (int p = g; p < HW; p += 4) s += in[((int64_t)img * HW + p) * C + c];
+    part[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0 && c < C) out[(int64_t)img * C + c] = ((part[0][threadIdx.x] + part[1][threadIdx.x]) +
+                                                      (part[2][threadIdx.x] + part[3][threadIdx.x])) / (float)HW;
+}
+
+// out[img][k] = sum_c w[k][c] in[img][c] + b[k] (1x1 convolutions of a 1x1 map).
+__global__ __launch_bounds__(256) void uo_fc_kernel(const float* __restrict__ in, int nimg, int Cin,
+                                                    const float* __restrict__ w, const float* __restrict__ b, int Cout,
+                                                    float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)nimg * Cout) return;
+    const int img = (int)(i / Cout), k = (int)(i - (int64_t)img * Cout);
+    float s = 0.f;
+    for (int c = 0; c < Cin; c++) s += w[(int64_t)k * Cin + c] * in[(int64_t)img * Cin + c];
+    out[i] = s + b[k];
+}
+
+}  // namespace
+}  // namespace vh
+
+using namespace vh;
+
+extern "C" {
+
+size_t vh_uo_conv_args_size(void) { return sizeof(vh_uo_conv_args); }
+
+int vh_uo_conv(const vh_uo_conv_args* args, void* stream) {
+    if (!args) return fail(VH_ERR_ARG, "uo_conv: null arguments");
+    const vh_uo_conv_args& g = *args;
+    if (g.nseg < 1 || g.nseg > 4) return fail(VH_ERR_ARG, "uo_conv: %d input segments (1 to 4)", g.nseg);
+    if (g.KH < 1 || g.KW < 1 || (g.KH & 1) == 0 || (g.KW & 1) == 0)
+        return fail(VH_ERR_ARG, "uo_conv: kernel %dx%d (odd sizes, 'same' padding)", g.KH, g.KW);
+    if (g.Cout < 1 || g.Cout_pad < g.Cout || g.Cout_pad % kUoBN != 0)
+        return fail(VH_ERR_ARG, "uo_conv: Cout %d, Cout_pad %d (a multiple of %d)", g.Cout, g.Cout_pad, kUoBN);
+    ConvArgs a{};
+    int cin = 0;
+    for (int s = 0; s < g.nseg; s++) {
+        if (!g.seg[s] || ((uintptr_t)g.seg[s] & 15) || g.seg_c[s] <= 0 || g.seg_c[s] % kUoKC != 0 ||
+            g.seg_off[s] % 8 != 0 || g.seg_stride[s] % 8 != 0 || g.seg_off[s] < 0 ||
+            g.seg_off[s] + g.seg_c[s] > g.seg_stride[s])
+            return fail(VH_ERR_ARG, "uo_conv: segment %d (channels %d at %d of %d; multiples of 32 / 8)", s,
+                        g.seg_c[s], g.seg_off[s], g.seg_stride[s]);
+        a.seg[s] = (const _Float16*)g.seg[s];
+        a.seg_c[s] = g.seg_c[s];
+        a.seg_stride[s] = g.seg_stride[s];
+        a.seg_off[s] = g.seg_off[s];
+        cin += g.seg_c[s];
+    }
+    const int epi = g.epilogue & 15;
+    if (epi > VH_UO_GRU || epi == VH_UO_ETA || (g.epilogue & ~(15 | VH_UO_NCHW)))
+        return fail(VH_ERR_ARG, "uo_conv: epilogue %d", g.epilogue);
+    if ((epi == VH_UO_GATES || epi == VH_UO_GRU || epi == VH_UO_SIGMUL) && !g.aux0)
+        return fail(VH_ERR_ARG, "uo_conv: the gate epilogues need aux0");
+    if (epi == VH_UO_GATES && (g.Cout != 256 || !g.out2 || g.out_f32))
+        return fail(VH_ERR_ARG, "uo_conv: the gates epilogue takes 256 f16 outputs (z, r) and out2");
+    if (epi == VH_UO_GRU && (!g.aux1 || g.Cout != 128))
+        return fail(VH_ERR_ARG, "uo_conv: the GRU epilogue takes 128 outputs and aux1");
+    if (!g.weight || ((uintptr_t)g.weight & 15) || !g.bias || !g.out)
+        return fail(VH_ERR_ARG, "uo_conv: weight (16-byte aligned), bias and out are required");
+    const int nout = epi == VH_UO_GATES ? 128 : g.Cout;  // channels written per pixel to out
+    const bool nchw = (g.epilogue & VH_UO_NCHW) != 0;
+    if (g.out_off < 0 || g.out_off + nout > g.out_stride || (nchw && epi == VH_UO_GATES))
+        return fail(VH_ERR_ARG, "uo_conv: output channels [%d, %d) of %d per pixel", g.out_off, g.out_off + nout,
+                    g.out_stride);
+    if ((epi == VH_UO_GATES || epi == VH_UO_GRU || epi == VH_UO_SIGMUL) && g.aux_stride < nout)
+        return fail(VH_ERR_ARG, "uo_conv: aux_stride %d < %d", g.aux_stride, nout);
+    if (g.glo && g.glo_stride < g.Cout) return fail(VH_ERR_ARG, "uo_conv: glo_stride %d < Cout", g.glo_stride);
+    if (g.nimg < 0 || g.H < 0 || g.W < 0) return fail(VH_ERR_ARG, "uo_conv: image size");
+    a.nseg = g.nseg;
+    a.cin = cin;
+    a.w = (const _Float16*)g.weight;
+    a.bias = g.bias;
+    a.nimg = g.nimg; a.H = g.H; a.W = g.W; a.Cout = g.Cout; a.Cout_pad = g.Cout_pad; a.KH = g.KH; a.KW = g.KW;
+    a.epi = g.epilogue;
+    a.out = g.out; a.out_f32 = g.out_f32; a.out_stride = g.out_stride; a.out_off = g.out_off;
+    a.glo = g.glo; a.glo_stride = g.glo_stride;
+    a.aux0 = (const _Float16*)g.aux0; a.aux1 = (const _Float16*)g.aux1; a.aux_stride = g.aux_stride;
+    a.out2 = (_Float16*)g.out2;
+    const int64_t P = (int64_t)g.nimg * g.H * g.W;
+    if (P == 0) return VH_OK;
+    if (P + kUoBM >= (int64_t)1 << 31) return fail(VH_ERR_UNSUPPORTED, "uo_conv: %lld pixels", (long long)P);
+    const dim3 grid(ceil_div(P, kUoBM), g.Cout_pad / kUoBN);
+    const hipStream_t st = (hipStream_t)stream;
+    switch (epi) {
+        case VH_UO_NONE: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_NONE>, grid, dim3(256), 0, st, a); break;
+        case VH_UO_RELU: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_RELU>, grid, dim3(256), 0, st, a); break;
+        case VH_UO_SIGMOID: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_SIGMOID>, grid, dim3(256), 0, st, a); break;
+        case VH_UO_SIGMUL: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_SIGMUL>, grid, dim3(256), 0, st, a); break;
+        case VH_UO_GATES: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_GATES>, grid, dim3(256), 0, st, a); break;
+        case VH_UO_GRU: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_GRU>, grid, dim3(256), 0, st, a); break;
+        default: return fail(VH_ERR_ARG, "uo_conv: epilogue %d", g.epilogue);
+    }
+    VH_LAUNCH_CHECK("uo_conv");
+    return VH_OK;
+}
+
+int vh_uo_conv_small(const void* in, int in_stride, int in_off, int cin, const float* weight, const float* bias,
+                     int nimg, int H, int W, int KH, int KW, int Cout, int epilogue, float* out, int out_stride,
+                     void* stream) {
+    if (Cout < 1 || Cout > 2) return fail(VH_ERR_ARG, "uo_conv_small: Cout %d (1 or 2)", Cout);
+    if (cin <= 0 || cin % 8 != 0 || in_off < 0 || in_off % 8 != 0 || in_stride % 8 != 0 || in_off + cin > in_stride)
+        return fail(VH_ERR_ARG, "uo_conv_small: channels %d at %d of %d (multiples of 8)", cin, in_off, in_stride);
+    if (KH < 1 || KW < 1 || (KH & 1) == 0 || (KW & 1) == 0)
+        return fail(VH_ERR_ARG, "uo_conv_small: kernel %dx%d (odd sizes)", KH, KW);
+    if (epilogue != VH_UO_NONE && epilogue != VH_UO_RELU && epilogue != VH_UO_SIGMOID && epilogue != VH_UO_ETA)
+        return fail(VH_ERR_ARG, "uo_conv_small: epilogue %d", epilogue);
+    if (!in || ((uintptr_t)in & 15) || !weight || !bias || !out || out_stride < Cout || nimg < 0 || H < 0 || W < 0)
+        return fail(VH_ERR_ARG, "uo_conv_small: null or misaligned buffer, or out_stride %d < %d", out_stride, Cout);
+    if (KH * KW * cin * Cout > kUoSmallMaxK)
+        return fail(VH_ERR_UNSUPPORTED, "uo_conv_small: %d weights (limit %d)", KH * KW * cin * Cout, kUoSmallMaxK);
+    const int64_t P = (int64_t)nimg * H * W;
+    if (P == 0) return VH_OK;
+    hipLaunchKernelGGL(uo_conv_small_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const _Float16*)in, in_stride, in_off, cin, weight, bias, nimg, H, W, KH, KW, Cout, epilogue,
+                       out, out_stride);
+    VH_LAUNCH_CHECK("uo_conv_small");
+    return VH_OK;
+}
+
+int vh_uo_to_nhwc(int dtype, const void* in, int nimg, int C, int HW, int Cpad, void* out, void* stream) {
+    if (Cpad < C) return fail(VH_ERR_ARG, "uo_to_nhwc: Cpad %d < C %d", Cpad, C);
+    if ((int64_t)nimg * HW == 0) return VH_OK;
+    const dim3 grid((unsigned)((int64_t)nimg * ((HW + 63) / 64)));
+    if (dtype == VH_F32)
+        hipLaunchKernelGGL(uo_to_nhwc_kernel<float>, grid, dim3(256), 0, (hipStream_t)stream, (const float*)in, nimg,
+                           C, HW, Cpad, (_Float16*)out);
+    else if (dtype == VH_F16)
+        hipLaunchKernelGGL(uo_to_nhwc_kernel<_Float16>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const _Float16*)in, nimg, C, HW, Cpad, (_Float16*)out);
+    else
+        return fail(VH_ERR_UNSUPPORTED, "uo_to_nhwc: dtype %d", dtype);
+    VH_LAUNCH_CHECK("uo_to_nhwc");
+    return VH_OK;
+}
+
+int vh_uo_to_nchw(int dtype, const void* in, int nimg, int C, int HW, int stride, int off, void* out, void* stream) {
+    if (off + C > stride) return fail(VH_ERR_ARG, "uo_to_nchw: channels [%d, %d) of %d", off, off + C, stride);
+    if ((int64_t)nimg * HW == 0) return VH_OK;
+    const dim3 grid((unsigned)((int64_t)nimg * ((HW + 63) / 64)));
+    if (dtype == VH_F32)
+        hipLaunchKernelGGL(uo_to_nchw_kernel<float>, grid, dim3(256), 0, (hipStream_t)stream, (const _Float16*)in,
+                           nimg, C, HW, stride, off, (float*)out);
+    else if (dtype == VH_F16)
+        hipLaunchKernelGGL(uo_to_nchw_kernel<_Float16>, grid, dim3(256), 0, (hipStream_t)stream,
+                           (const _Float16*)in, nimg, C, HW, stride, off, (_Float16*)out);
+    else
+        return fail(VH_ERR_UNSUPPORTED, "uo_to_nchw: dtype %d", dtype);
+    VH_LAUNCH_CHECK("uo_to_nchw");
+    return VH_OK;
+}
+
+int vh_uo_mean(const float* in, int nimg, int HW, int C, float* out, void* stream) {
+    if ((int64_t)nimg * C == 0) return VH_OK;
+    hipLaunchKernelGGL(uo_mean_kernel, dim3((unsigned)((int64_t)nimg * ((C + 63) / 64))), dim3(256), 0,
+                       (hipStream_t)stream, in, nimg, HW, C, out);
+    VH_LAUNCH_CHECK("uo_mean");
+    return VH_OK;
+}
+
+int vh_uo_fc(const float* in, int nimg, int Cin, const float* weight, const float* bias, int Cout, float* out,
+             void* stream) {
+    const int64_t n = (int64_t)nimg * Cout;
+    if (n == 0) return VH_OK;
+    hipLaunchKernelGGL(uo_fc_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, in, nimg, Cin, weight,
+                       bias, Cout, out);
+    VH_LAUNCH_CHECK("uo_fc");
+    return VH_OK;
+}
+
+}  // extern "C"
