@@ -417,8 +417,8 @@ size_t vh_uo_conv_args_size(void);
 int vh_uo_conv(const vh_uo_conv_args* args, void* stream);
 /* The 1- and 2-channel heads (delta.2, weight.2 :475-485, eta :430): in f16
  * channels [in_off, in_off + cin) of [P][in_stride]; weight f32
- * [KH*KW][cin][Cout] (KH*KW*cin*Cout <= 4608), bias f32 [Cout]; out f32
- * [P][out_stride]; epilogue VH_UO_NONE / RELU / SIGMOID / ETA. */
+ * [KH*KW][cin][Cout] (KH*KW <= 9, cin <= 256, Cout 1 or 2), bias f32 [Cout];
+ * out f32 [P][out_stride]; epilogue VH_UO_NONE / RELU / SIGMOID / ETA. */
 int vh_uo_conv_small(const void* in, int in_stride, int in_off, int cin, const float* weight, const float* bias,
                      int nimg, int H, int W, int KH, int KW, int Cout, int epilogue, float* out, int out_stride,
                      void* stream);

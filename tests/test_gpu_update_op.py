@@ -178,6 +178,56 @@ def test_gate_epilogues_match_fp64():
     assert float((nchw(out) - ref).abs().max()) < 2e-3
 
 
+SMALL_CASES = [
+    # (kernel, cin, in_off, in_stride, Cout, epilogue, nimg, H, W)
+    (3, 128, 0, 256, 2, "none", 3, 7, 9),
+    (3, 128, 128, 256, 2, "sigmoid", 2, 12, 20),
+    (3, 128, 0, 128, 1, "eta", 5, 3, 5),
+    (1, 64, 8, 80, 2, "relu", 1, 11, 13),
+    (3, 200, 0, 200, 1, "none", 2, 9, 10),
+    (3, 256, 0, 256, 2, "none", 1, 64, 80),
+]
+
+
+@pytest.mark.parametrize("case", SMALL_CASES, ids=lambda c: f"k{c[0]}_c{c[1]}@{c[2]}_o{c[4]}_{c[5]}_{c[7]}x{c[8]}")
+def test_small_conv_heads_match_fp64(case):
+    """The 1- and 2-channel heads against an FP64 convolution of the same f16
+    inputs: only the FP32 accumulation order differs, so |err| <= 1e-5 |ref|max
+    + 1e-5 (sigmoid / eta outputs compared after the activation)."""
+    L = _lib()
+    from vipe_amd.update_op import _vp
+
+    k, cin, off, stride, cout, epi, nimg, H, W = case
+    print(f"small conv case {case}", flush=True)
+    g = torch.Generator().manual_seed(cin + k + cout + H)
+    P = nimg * H * W
+    buf = _h(torch.randn(P, stride, generator=g))
+    w = torch.randn(cout, cin, k, k, generator=g) / np.sqrt(cin * k * k)
+    b = torch.randn(cout, generator=g) * 0.1
+    wp = w.permute(2, 3, 1, 0).reshape(k * k, cin, cout).contiguous()
+    out = torch.full((P, cout + 1), 9.0, device=DEV)
+    code = {"none": L.VH_UO_NONE, "relu": L.VH_UO_RELU, "sigmoid": L.VH_UO_SIGMOID, "eta": L.VH_UO_ETA}[epi]
+    r = _runner()
+    bd, wd, bb = buf.to(DEV), wp.to(DEV), b.to(DEV)
+    L.check(r.lib.vh_uo_conv_small(_vp(bd), stride, off, cin, _vp(wd), _vp(bb), nimg, H, W, k, k, cout, code,
+                                   _vp(out), cout + 1, r.stream), "uo_conv_small")
+    torch.cuda.synchronize()
+    x = buf[:, off:off + cin].double().view(nimg, H * W, cin).permute(0, 2, 1).reshape(nimg, cin, H, W)
+    ref = F.conv2d(x, w.double(), b.double(), padding=k // 2)
+    if epi == "relu":
+        ref = torch.relu(ref)
+    elif epi == "sigmoid":
+        ref = torch.sigmoid(ref)
+    elif epi == "eta":
+        ref = 0.01 * F.softplus(ref)
+    got = out.double().cpu()
+    assert bool((got[:, cout] == 9.0).all())
+    got = got[:, :cout].view(nimg, H * W, cout).permute(0, 2, 1).reshape(nimg, cout, H, W)
+    err = float((got - ref).abs().max())
+    print(f"  max err {err:.3e} (ref max {float(ref.abs().max()):.3f})", flush=True)
+    assert err <= 1e-5 * float(ref.abs().max()) + 1e-5
+
+
 @pytest.mark.parametrize("C,Cpad,HW", [(196, 224, 95), (4, 32, 64), (128, 128, 130), (300, 320, 7)])
 def test_layout_round_trip(C, Cpad, HW):
     L = _lib()

@@ -9,7 +9,7 @@
 //   uo_conv_kernel        the implicit GEMM on v_mfma_f32_32x32x16_f16.  One
 //                         workgroup per 128 pixels x 128 output channels; each
 //                         of its 4 waves owns 64 x 64 as 2 x 2 tiles of 32 x 32.
-//                         K advances one 32-channel chunk of one tap of one input
+//                         K advances one tap of one 32-channel chunk of one input
 //                         segment at a time: the chunk of the 128 pixels (shifted
 //                         by the tap, zero outside the image) and of the 128
 //                         weight rows go to LDS, the next chunk's loads are in
@@ -18,7 +18,8 @@
 //                         bias (and a per-image addend) and applies the layer's
 //                         activation, or the ConvGRU gates / state update.
 //   uo_conv_small_kernel  convolutions with 1 or 2 output channels (the delta,
-//                         weight and eta heads): one lane per pixel on the VALU.
+//                         weight and eta heads) on the VALU: lanes along the
+//                         input channels, a cross-lane sum per batch of pixels.
 //   uo_to_nhwc / uo_to_nchw_kernel  layout changes at the module boundary.
 //   uo_mean_kernel / uo_fc_kernel   ConvGRU's global context: the per-image mean
 //                         of sigmoid(w(net)) * net and the three 1x1 convolutions
@@ -92,8 +93,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         py_[u] = rem / a.W;
         px_[u] = rem - py_[u] * a.W;
     }
-    // K walks (tap, segment, 32-channel chunk) in that nesting; the cursor
-    // (uniform) holds the chunk whose loads are issued next.  The segment is
+    // K walks (segment, 32-channel chunk, tap) in that nesting: the taps of one
+    // chunk read the same input lines (shifted by at most a row and a column),
+    // so they come from the caches instead of HBM.  The cursor (uniform) holds
+    // the chunk whose loads are issued next.  The segment is
     // picked by integer sums over the segment table, not by selecting among
     // its entries: selects of kernel-argument loads become a pointer table in
     // scratch memory.
@@ -108,7 +111,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const int nseg = a.nseg, KW = a.KW, H = a.H, W = a.W, Cout_pad = a.Cout_pad, cin = a.cin;
     const _Float16* wp = a.w;
     const int per_tap = (sc0 + (nseg > 1 ? sc1 : 0) + (nseg > 2 ? sc2 : 0) + (nseg > 3 ? sc3 : 0)) / kUoKC;
-    const int nchunks = a.KH * KW * per_tap;
+    const int ntap = a.KH * KW;
+    const int nchunks = ntap * per_tap;
     int cur_tap = 0, cur_sg = 0, cur_c0 = 0, cur_wc0 = 0;
     uint64_t cur_sp = sb0;
     int cur_sc = sc0, cur_ss = ss0, cur_so = so0;
@@ -144,16 +148,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         *reinterpret_cast<uint4*>(Bs + r_[1] * kUoRS + 8 * q_[1]) = vb1;
         // the next chunk's loads (the last chunk reloads itself, so every path
         // issues them and the MFMAs below wait for nothing)
-        if (kc + 1 < nchunks) {
+        if (kc + 1 < nchunks && ++cur_tap == ntap) {
+            cur_tap = 0;
             cur_c0 += kUoKC;
             cur_wc0 += kUoKC;
             if (cur_c0 == cur_sc) {
                 cur_c0 = 0;
-                if (++cur_sg == nseg) {
-                    cur_sg = 0;
-                    cur_wc0 = 0;
-                    cur_tap++;
-                }
+                ++cur_sg;  // below nseg: the chunk after the last tap of the last segment's last chunk is never loaded
                 const int sg = cur_sg;
                 cur_sp = sb0 + (sg >= 1 ? sd1 : 0) + (sg >= 2 ? sd2 : 0) + (sg >= 3 ? sd3 : 0);
                 cur_sc = sg == 0 ? sc0 : sg == 1 ? sc1 : sg == 2 ? sc2 : sc3;
@@ -230,47 +231,122 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     }
 }
 
-// Convolution with Cout <= 2 on the VALU: one lane per pixel, the weights in LDS.
-constexpr int kUoSmallMaxK = 9 * 256 * 2;  // taps x input channels x outputs held in LDS
+// Convolution with Cout <= 2 (the delta, weight and eta heads) on the VALU.
+// Lanes run along the input channels: lane l holds channels 2l, 2l + 1 of each
+// 128-channel group, so one load instruction reads a pixel's whole channel row
+// (256 contiguous bytes) and the weights of the lane's channels stay in
+// registers.  A wave takes 64 consecutive pixels, kSmallBatch / Cout at a time:
+// every tap of every pixel of the batch is loaded before any is used, each
+// lane accumulates its channels' share of the kSmallBatch (pixel, output)
+// values, and one cross-lane sum of the kSmallBatch values leaves the total of
+// value (lane & 31) in lane l.
+constexpr int kUoSmallMaxCin = 256;  // two 128-channel groups
+constexpr int kUoSmallMaxTaps = 9;   // 3 x 3 kernels
+constexpr int kSmallBatch = 32;      // (pixel, output) values reduced together
 
+template <int CTRL>
+__device__ __forceinline__ float uo_dpp(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+
+// Sum over the 64 lanes of each of 32 values: each halving step keeps the half
+// selected by one lane bit and adds the partner lane's copy (the partner of a
+// DPP mirror or swap step differs in that bit); after the last step lane l
+// holds the total of value (l & 31).  Every lane of the wave must be active.
+__device__ __forceinline__ float uo_wave_sum32(float (&v)[32]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[i]), __float_as_uint(v[i + 16]), false,
+                                                        false);
+        v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+#define UO_HALVE(S, CTRL, BIT)                                          \
+    _Pragma("unroll") for (int i = 0; i < S; i++) {                     \
+        const float a = v[i] + uo_dpp<CTRL>(v[i]);                      \
+        const float b = v[i + S] + uo_dpp<CTRL>(v[i + S]);              \
+        v[i] = (lane & BIT) ? b : a;                                    \
+    }
+    UO_HALVE(8, 0x140, 8)
+    UO_HALVE(4, 0x141, 4)
+    UO_HALVE(2, 0x4E, 2)
+    UO_HALVE(1, 0xB1, 1)
+#undef UO_HALVE
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[0]), __float_as_uint(v[0]), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+template <int COUT>
 __global__ __launch_bounds__(256) void uo_conv_small_kernel(const _Float16* __restrict__ in, int in_stride, int in_off,
                                                             int cin, const float* __restrict__ w,
                                                             const float* __restrict__ bias, int nimg, int H, int W,
-                                                            int KH, int KW, int Cout, int epi, float* __restrict__ out,
+                                                            int KH, int KW, int epi, float* __restrict__ out,
                                                             int out_stride) {
-    __shared__ float sw[kUoSmallMaxK];  // [tap][c][o]
-    const int nw = KH * KW * cin * Cout;
-    for (int i = threadIdx.x; i < nw; i += 256) sw[i] = w[i];
-    __syncthreads();
+    constexpr int NPB = kSmallBatch / COUT;  // pixels per batch
+    const int lane = threadIdx.x & 63;
     const int HW = H * W;
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= (int64_t)nimg * HW) return;
-    const int img = (int)(p / HW);
-    const int rem = (int)(p - (int64_t)img * HW);
-    const int py = rem / W, px = rem - py * W;
-    float s0 = 0.f, s1 = 0.f;
-    for (int t = 0; t < KH * KW; t++) {
-        const int yy = py + t / KW - KH / 2, xx = px + t % KW - KW / 2;
-        if (yy < 0 || yy >= H || xx < 0 || xx >= W) continue;
-        const _Float16* ip = in + ((int64_t)img * HW + yy * W + xx) * in_stride + in_off;
-        const float* wt = sw + t * cin * Cout;
-        for (int c = 0; c < cin; c += 8) {
-            const f16x8 v = *reinterpret_cast<const f16x8*>(ip + c);
+    const int P = nimg * HW;  // < 2^31, checked by the host
+    const int pw0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;  // this wave's first pixel
+    const int ntap = KH * KW;
+    const int ngrp = (cin + 127) / 128;
+    for (int b0 = 0; b0 < 64; b0 += NPB) {
+        // the batch's pixels (wave-uniform): image base, row and column; a
+        // pixel past P gets a row that fails every bounds test
+        int pb[NPB], py[NPB], px[NPB];
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const float xv = (float)v[k];
-                s0 += xv * wt[(c + k) * Cout];
-                if (Cout > 1) s1 += xv * wt[(c + k) * Cout + 1];
+        for (int i = 0; i < NPB; i++) {
+            const int p = pw0 + b0 + i;
+            const int img = p / HW;
+            const int rem = p - img * HW;
+            pb[i] = img * HW;
+            py[i] = p < P ? rem / W : -(1 << 20);
+            px[i] = rem - (rem / W) * W;
+        }
+        float acc[32];
+#pragma unroll
+        for (int i = 0; i < 32; i++) acc[i] = 0.f;
+#pragma unroll
+        for (int g = 0; g < ngrp; g++) {
+            const int c = 128 * g + 2 * lane;
+            const bool cok = c < cin;
+            for (int t = 0; t < ntap; t++) {
+                const int dy = t / KW - KH / 2, dx = t % KW - KW / 2;
+                // this lane's two channels' weights for the tap (the same for
+                // every wave: served by the caches)
+                float w0[COUT], w1[COUT];
+#pragma unroll
+                for (int o = 0; o < COUT; o++) {
+                    w0[o] = cok ? w[((int64_t)t * cin + c) * COUT + o] : 0.f;
+                    w1[o] = cok ? w[((int64_t)t * cin + c + 1) * COUT + o] : 0.f;
+                }
+                __half2 x[NPB];
+#pragma unroll
+                for (int i = 0; i < NPB; i++) {
+                    const int yy = py[i] + dy, xx = px[i] + dx;
+                    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W && cok;
+                    const int64_t q = ok ? (int64_t)(pb[i] + yy * W + xx) : 0;
+                    const __half2 v = *reinterpret_cast<const __half2*>(in + q * in_stride + in_off + (cok ? c : 0));
+                    x[i] = ok ? v : __half2(__float2half(0.f), __float2half(0.f));
+                }
+#pragma unroll
+                for (int i = 0; i < NPB; i++) {
+                    const float x0 = __low2float(x[i]), x1 = __high2float(x[i]);
+#pragma unroll
+                    for (int o = 0; o < COUT; o++) acc[i * COUT + o] += x0 * w0[o] + x1 * w1[o];
+                }
             }
         }
-    }
-    float r[2] = {s0 + bias[0], Cout > 1 ? s1 + bias[1] : 0.f};
-    for (int o = 0; o < Cout; o++) {
-        float x = r[o];
-        if (epi == VH_UO_SIGMOID) x = sigmoidf_(x);
-        else if (epi == VH_UO_ETA) x = 0.01f * softplusf_(x);
-        else if (epi == VH_UO_RELU) x = fmaxf(x, 0.f);
-        out[p * out_stride + o] = x;
+        const float tot = uo_wave_sum32(acc);
+        // lane l < 32 holds value l: pixel b0 + l / COUT, output l % COUT
+        const int p = pw0 + b0 + (lane & 31) / COUT;
+        const int o = (lane & 31) % COUT;
+        if (lane < 32 && p < P) {
+            float x = tot + bias[o];
+            if (epi == VH_UO_SIGMOID) x = sigmoidf_(x);
+            else if (epi == VH_UO_ETA) x = 0.01f * softplusf_(x);
+            else if (epi == VH_UO_RELU) x = fmaxf(x, 0.f);
+            out[(int64_t)p * out_stride + o] = x;
+        }
     }
 }
 
@@ -448,13 +524,20 @@ int vh_uo_conv_small(const void* in, int in_stride, int in_off, int cin, const f
         return fail(VH_ERR_ARG, "uo_conv_small: epilogue %d", epilogue);
     if (!in || ((uintptr_t)in & 15) || !weight || !bias || !out || out_stride < Cout || nimg < 0 || H < 0 || W < 0)
         return fail(VH_ERR_ARG, "uo_conv_small: null or misaligned buffer, or out_stride %d < %d", out_stride, Cout);
-    if (KH * KW * cin * Cout > kUoSmallMaxK)
-        return fail(VH_ERR_UNSUPPORTED, "uo_conv_small: %d weights (limit %d)", KH * KW * cin * Cout, kUoSmallMaxK);
+    if (KH * KW > kUoSmallMaxTaps || cin > kUoSmallMaxCin)
+        return fail(VH_ERR_UNSUPPORTED, "uo_conv_small: %dx%d kernel with %d input channels (limits %d taps, %d)", KH,
+                    KW, cin, kUoSmallMaxTaps, kUoSmallMaxCin);
     const int64_t P = (int64_t)nimg * H * W;
     if (P == 0) return VH_OK;
-    hipLaunchKernelGGL(uo_conv_small_kernel, dim3(ceil_div(P, 256)), dim3(256), 0, (hipStream_t)stream,
-                       (const _Float16*)in, in_stride, in_off, cin, weight, bias, nimg, H, W, KH, KW, Cout, epilogue,
-                       out, out_stride);
+    if (P + 256 >= (int64_t)1 << 31) return fail(VH_ERR_UNSUPPORTED, "uo_conv_small: %lld pixels", (long long)P);
+    if (Cout == 2)
+        hipLaunchKernelGGL(uo_conv_small_kernel<2>, dim3(ceil_div(P, 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const _Float16*)in, in_stride, in_off, cin, weight, bias, nimg, H, W, KH, KW, epilogue,
+                           out, out_stride);
+    else
+        hipLaunchKernelGGL(uo_conv_small_kernel<1>, dim3(ceil_div(P, 256)), dim3(256), 0, (hipStream_t)stream,
+                           (const _Float16*)in, in_stride, in_off, cin, weight, bias, nimg, H, W, KH, KW, epilogue,
+                           out, out_stride);
     VH_LAUNCH_CHECK("uo_conv_small");
     return VH_OK;
 }

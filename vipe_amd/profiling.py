@@ -36,6 +36,12 @@
         volume [edges, 64, 80, 64, 80] (random values), coordinates the
         identity plus N(0, 2 px) noise.
 
+    python3 -m vipe_amd.profiling update_op [--edges 384] [--per-frame 6] [--iters 3]
+        The DroidNet update operator (UpdateModule.forward with GraphAgg) on
+        the bench shape: 384 edges at 64 x 80, six edges per frame (64 frames),
+        random FP16 state and context, random parameters; one warm-up call,
+        then `iters` calls.
+
     python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
         --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
         Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
@@ -230,6 +236,27 @@ def _corr_index(dtype: str, edges: int, iters: int) -> None:
     print(f"corr_index: {iters + 1} x lookup of [{edges}, {H}, {W}, {H}, {W}] {dtype} -> {tuple(out.shape)}", flush=True)
 
 
+def _update_op(edges: int, per_frame: int, iters: int) -> None:
+    import torch
+
+    from .update_op import UpdateModule
+
+    torch.manual_seed(0)
+    m = UpdateModule().cuda()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    H, W = 64, 80
+    net = torch.tanh(torch.randn(1, edges, 128, H, W, device="cuda", generator=g)).half()
+    inp = torch.relu(torch.randn(1, edges, 128, H, W, device="cuda", generator=g)).half()
+    corr = torch.randn(1, edges, 196, H, W, device="cuda", generator=g) * 0.5
+    flow = (torch.randn(1, edges, 4, H, W, device="cuda", generator=g) * 4).clamp(-64, 64)
+    ix = torch.arange(edges, device="cuda") // per_frame
+    for _ in range(iters + 1):
+        out = m(net, inp, corr, flow, ix)
+    torch.cuda.synchronize()
+    print(f"update_op: {iters + 1} x forward, {edges} edges ({per_frame} per frame) at {H}x{W} -> "
+          f"net {tuple(out[0].shape)}, upmask {tuple(out[4].shape)}", flush=True)
+
+
 def _short(name: str, width: int = 90) -> str:
     return name if len(name) <= width else name[: width - 3] + "..."
 
@@ -309,6 +336,10 @@ def main(argv=None) -> None:
     ci.add_argument("--dtype", choices=["f32", "f16"], default="f32")
     ci.add_argument("--edges", type=int, default=96)
     ci.add_argument("--iters", type=int, default=5)
+    uo = sub.add_parser("update_op")
+    uo.add_argument("--edges", type=int, default=384)
+    uo.add_argument("--per-frame", type=int, default=6)
+    uo.add_argument("--iters", type=int, default=3)
     s = sub.add_parser("summarize")
     s.add_argument("--stats", required=True)
     s.add_argument("--fetch")
@@ -335,6 +366,8 @@ def main(argv=None) -> None:
         _altcorr_layer(a.iters)
     elif a.cmd == "corr_index":
         _corr_index(a.dtype, a.edges, a.iters)
+    elif a.cmd == "update_op":
+        _update_op(a.edges, a.per_frame, a.iters)
     else:
         _summarize(a)
 
