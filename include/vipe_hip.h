@@ -242,6 +242,37 @@ int vh_ba_run_robust(const vh_ba_plan* plan, float* poses, float* disps, float* 
                      int iterations, float lm, float ep, float alpha, float intrinsics_lm, float intrinsics_ep,
                      float intrinsics_scale, int flags, int robust_type, float threshold, float gnc_mu,
                      float gnc_mu_max, float* dx, float* dz, float* energy, void* workspace, void* stream);
+/* Fused BA on multi-view rig buffers and on the Mei camera: the layouts the
+ * reference's fused path rejects and sends to its generic solver
+ * (buffer.py:384-410; the generic model is DenseDepthFlowTerm, terms.py:98-262,
+ * with geom.iproj_i_proj_j_disp, geom.py:187-298, and the pinhole / Mei
+ * models of vipe/utils/cameras.py:123-354).  No counterpart in the reference's
+ * fused kernel; it keeps that kernel's conventions (MIN_DEPTH 0.25, weight
+ * scale 0.001, the back-substitution skip of local pose index <= 0, damping,
+ * zeros on a failed factorisation).
+ *
+ * The plan takes HOST arrays: edge e runs from view qi[e] of pose pi[e] to
+ * view qj[e] of pose pj[e] (pi == pj with qi != qj: an edge within one frame,
+ * which constrains the depth only).  Depth frames are pi V + qi over the
+ * flattened disparities [N V, H, W]; kx = every view of the window's frames
+ * and the edges' source depth frames, sorted.
+ *
+ * The run: poses [N,7] SE3 (in place), disps [N V, H, W] (in place),
+ * intrinsics [V][4] (pinhole fx fy cx cy) or [V][5] (Mei: fx fy cx cy k1) at
+ * the maps' resolution, fixed; rig [V][7] camera-to-rig SE3 (t, q xyzw) of
+ * each view; disps_sens [N V, H, W]; eta [|kx|, H, W]; depth_active [N V];
+ * flags VH_BA_MOTION_ONLY / VH_BA_COMPUTE_ENERGY; robust_* as
+ * vh_ba_run_robust (VH_ROBUST_NONE for least squares).  Outputs as vh_ba_run. */
+#define VH_CAM_PINHOLE 0
+#define VH_CAM_MEI 1
+vh_ba_plan* vh_ba_plan_create_rig(const int64_t* pi_host, const int64_t* pj_host, const int64_t* qi_host,
+                                  const int64_t* qj_host, int E, int N, int V, int H, int W, int t0, int t1,
+                                  int pixel_tile);
+int vh_ba_run_rig(const vh_ba_plan* plan, float* poses, float* disps, const float* intrinsics, int camera,
+                  const float* rig, const float* disps_sens, const float* targets, const float* weights,
+                  const float* eta, const float* depth_active, int iterations, float lm, float ep, float alpha,
+                  int flags, int robust_type, float threshold, float gnc_mu, float gnc_mu_max, float* dx, float* dz,
+                  float* energy, void* workspace, void* stream);
 /* RobustKernel.apply (kernel.py:41-131): w[i] = kernel(x[i]) for n FP32 values. */
 int vh_robust_weights(int robust_type, float threshold, float gnc_mu, float gnc_mu_max, const float* x, float* w,
                       int64_t n, void* stream);

@@ -79,6 +79,9 @@ _SIGS = {
     "vh_ba_run_robust": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, i32, i32, f32,
                                f32, f32, vp, vp, vp, vp, vp]),
     "vh_robust_weights": (i32, [i32, f32, f32, f32, vp, vp, i64, vp]),
+    "vh_ba_plan_create_rig": (vp, [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, i32]),
+    "vh_ba_run_rig": (i32, [vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, i32, f32, f32, f32, i32, i32, f32, f32,
+                            f32, vp, vp, vp, vp, vp]),
     # geometry siblings
     "vh_geom_iproj": (i32, [vp, vp, vp, i32, i32, i32, vp, vp]),
     "vh_geom_projmap": (i32, [vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, vp, vp]),

@@ -103,6 +103,12 @@ struct DevPlan {
     const int* wide;     // [n_wide] slots of the depth frames with more than kMaxDeg outgoing edges
     int n_wide;          // number of such frames
     int wide_rows;       // stored E rows per pixel of the largest of them (zero row included)
+    // rig plans (V > 0): depth frame ids are p V + q over the flattened
+    // [N V, H, W] disparities; ii / jj hold the edges' poses
+    int V;               // views (0: a single-view plan, depth frame id = pose id)
+    const int* qi;       // [E] source view of each edge
+    const int* qj;       // [E] target view
+    const int* kpose;    // [K] pose of depth frame k
 };
 
 struct Work {
@@ -173,6 +179,8 @@ struct RunArgs {
     int itr;
     int compute_energy;
     RobustK rk;   // robust kernel on the flow residuals (type 0: plain least squares)
+    const float* rig;  // rig runs: [V][7] camera-to-rig SE3 (t, q xyzw) of each view
+    int kstride;       // rig runs: intrinsics per view, 4 (pinhole) or 5 (Mei: fx fy cx cy k1)
 };
 
 __host__ __device__ inline int pk(int kind, int idx) { return (kind << 27) | idx; }
@@ -300,6 +308,147 @@ __device__ __forceinline__ void neg_adjT(const float* R, const float* t, float s
         o[2] *= sc;
         o[PD - 1] = (t[0] * e[0] + t[1] * e[1] + t[2] * e[2]) - e[PD - 1];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-view rigs and the Mei camera (rig plans; CAM 1 pinhole, 2 Mei)
+// ---------------------------------------------------------------------------
+// Edge e of a rig plan runs from view qi of pose pi to view qj of pose pj.
+// With G = Pj Pi^-1 (the identity when pi == pj) and rig[q] the camera-to-rig
+// SE3 of view q (the generic path's model, vipe/slam/maths/geom.py:187-298):
+//   camera transform    T = rig[qj]^-1 G rig[qi]   (maps the point)
+//   self-row transform  M = rig[qj]^-1 G           (Ji = -Jc Adj(M))
+//   j-side transform    A = Adj(rig[qj]^-1)        (Jj = Jc A)
+// with Jc the camera-frame Jacobian of a left perturbation of T.  The frame
+// kernels sum camera-frame partials and E rows; finalize applies A to the
+// edge blocks and ba_gram_rig_kernel to the Gram blocks of the edge rows, in
+// FP64; the back-substitution dots the camera-frame E rows with
+// A (dx_j - Adj(G) dx_i).  An edge within one frame (pi == pj) has Ji = -Jj,
+// so its pose terms cancel: it carries none and no Schur rows, and constrains
+// the depth only.  Intrinsics are per view and fixed.
+constexpr int kRigEP = 36;  // per-edge LDS record: T (R 0-8, t 9-11), flags 12-14, M (R 16-24, t 25-27), Kj 28-32
+
+__device__ __forceinline__ void quat_rot(const float* q, float* R) {
+    const float x = q[0], y = q[1], z = q[2], w = q[3];
+    R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - w * z);     R[2] = 2 * (x * z + w * y);
+    R[3] = 2 * (x * y + w * z);     R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - w * x);
+    R[6] = 2 * (x * z - w * y);     R[7] = 2 * (y * z + w * x);     R[8] = 1 - 2 * (x * x + y * y);
+}
+
+// G of rig-plan edge e (R, t); the identity for an edge within one frame
+__device__ __forceinline__ void rig_pose_transform(const DevPlan& p, const RunArgs& a, int e, float* RG, float* tG) {
+    float sc;
+    if (p.ii[e] == p.jj[e]) {
+        for (int i = 0; i < 9; i++) RG[i] = (i % 4 == 0) ? 1.f : 0.f;
+        tG[0] = tG[1] = tG[2] = 0.f;
+    } else {
+        edge_transform<6>(a.poses, p.ii[e], p.jj[e], RG, tG, sc);
+    }
+}
+
+// M (Rm, tm) and T (Rc, tc) of rig-plan edge e
+__device__ __forceinline__ void rig_edge_transforms(const DevPlan& p, const RunArgs& a, int e, float* Rc, float* tc,
+                                                    float* Rm, float* tm) {
+    float RG[9], tG[3], Rj[9], Ri[9];
+    rig_pose_transform(p, a, e, RG, tG);
+    const float* rj = a.rig + 7 * p.qj[e];
+    const float* ri = a.rig + 7 * p.qi[e];
+    quat_rot(rj + 3, Rj);
+    quat_rot(ri + 3, Ri);
+    const float dt[3] = {tG[0] - rj[0], tG[1] - rj[1], tG[2] - rj[2]};
+    for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++) Rm[3 * r + c] = Rj[r] * RG[c] + Rj[3 + r] * RG[3 + c] + Rj[6 + r] * RG[6 + c];
+        tm[r] = Rj[r] * dt[0] + Rj[3 + r] * dt[1] + Rj[6 + r] * dt[2];
+    }
+    for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++)
+            Rc[3 * r + c] = Rm[3 * r] * Ri[c] + Rm[3 * r + 1] * Ri[3 + c] + Rm[3 * r + 2] * Ri[6 + c];
+        tc[r] = Rm[3 * r] * ri[0] + Rm[3 * r + 1] * ri[1] + Rm[3 * r + 2] * ri[2] + tm[r];
+    }
+}
+
+// Adj(rig[q]^-1) = [[R^T, hat(-R^T t) R^T], [0, R^T]] in FP64, row-major
+__device__ __forceinline__ void rig_adj_inv(const float* rig, int q, double* A) {
+    float Rf[9];
+    const float* r = rig + 7 * q;
+    quat_rot(r + 3, Rf);
+    double Rt[9], u[3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Rt[3 * i + j] = Rf[3 * j + i];
+    for (int i = 0; i < 3; i++) u[i] = -(Rt[3 * i] * r[0] + Rt[3 * i + 1] * r[1] + Rt[3 * i + 2] * r[2]);
+    const double hu[9] = {0.0, -u[2], u[1], u[2], 0.0, -u[0], -u[1], u[0], 0.0};
+    for (int i = 0; i < 36; i++) A[i] = 0.0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            A[6 * i + j] = Rt[3 * i + j];
+            A[6 * (i + 3) + j + 3] = Rt[3 * i + j];
+            A[6 * i + j + 3] = hu[3 * i] * Rt[j] + hu[3 * i + 1] * Rt[3 + j] + hu[3 * i + 2] * Rt[6 + j];
+        }
+}
+
+// The unit-depth ray [X0, Y0, 1] of pixel (u, v) in a view with intrinsics K
+// (vipe/utils/cameras.py:131-159 pinhole, :228-251 Mei)
+template <int CAM>
+__device__ __forceinline__ void rig_iproj(const float* K, float u, float v, float& X0, float& Y0) {
+    const float ub = (u - K[2]) / K[0], vb = (v - K[3]) / K[1];
+    if (CAM == 2) {
+        const float k1 = K[4];
+        const float r2 = ub * ub + vb * vb;
+        const float q = sqrtf(1.f + (1.f - k1 * k1) * r2);
+        const float f = (k1 + q) / (1.f + r2);
+        const float s = f / (f - k1);
+        X0 = ub * s;
+        Y0 = vb * s;
+    } else {
+        X0 = ub;
+        Y0 = vb;
+    }
+}
+
+// Per-pixel terms of one edge under the Mei model (cameras.py:283-337): the
+// point X = R [X0, Y0, 1] + t d projects to fx X / (Z + k1 |X|) + cx; g = d(u,
+// v)/dX gives Jc = [d g, X x g] and Jz = g . t.  Below MIN_DEPTH the projection
+// and every Jacobian are zero, as in the fused kernel.
+template <bool RES>
+__device__ __forceinline__ void pixel_terms_mei(const float* R, const float* t, float X0, float Y0, float d,
+                                                const float* K, float wgt_u, float wgt_v, float tgt_u, float tgt_v,
+                                                PixTerms& o) {
+    const float X = R[0] * X0 + R[1] * Y0 + R[2] + t[0] * d;
+    const float Y = R[3] * X0 + R[4] * Y0 + R[5] + t[1] * d;
+    const float Z = R[6] * X0 + R[7] * Y0 + R[8] + t[2] * d;
+    const bool ok = !(Z < MIN_DEPTH);
+    o.ok = ok;
+    const float fx = K[0], fy = K[1], cx = K[2], cy = K[3], k1 = K[4];
+    const float r = sqrtf(X * X + Y * Y + Z * Z);
+    const float rb = Z + k1 * r;
+    const float inv = ok ? 1.0f / rb : 0.0f;
+    const float den = ok ? inv * inv / r : 0.0f;
+    o.wu = ok ? 0.001f * wgt_u : 0.0f;
+    o.wv = ok ? 0.001f * wgt_v : 0.0f;
+    if (RES) {
+        o.ru = tgt_u - (fx * X * inv + cx);
+        o.rv = tgt_v - (fy * Y * inv + cy);
+    }
+    const float fu = fx * den, fv = fy * den;
+    const float gu[3] = {fu * (rb * r - k1 * X * X), -fu * k1 * X * Y, -fu * X * (k1 * Z + r)};
+    const float gv[3] = {-fv * k1 * X * Y, fv * (rb * r - k1 * Y * Y), -fv * Y * (k1 * Z + r)};
+    o.Ju[0] = d * gu[0]; o.Ju[1] = d * gu[1]; o.Ju[2] = d * gu[2];
+    o.Ju[3] = Y * gu[2] - Z * gu[1]; o.Ju[4] = Z * gu[0] - X * gu[2]; o.Ju[5] = X * gu[1] - Y * gu[0];
+    o.Jv[0] = d * gv[0]; o.Jv[1] = d * gv[1]; o.Jv[2] = d * gv[2];
+    o.Jv[3] = Y * gv[2] - Z * gv[1]; o.Jv[4] = Z * gv[0] - X * gv[2]; o.Jv[5] = X * gv[1] - Y * gv[0];
+    o.Jzu = gu[0] * t[0] + gu[1] * t[1] + gu[2] * t[2];
+    o.Jzv = gv[0] * t[0] + gv[1] * t[1] + gv[2] * t[2];
+    o.Jfu = o.Jfv = 0.f;
+}
+
+// The pixel terms of a rig-plan edge from its LDS record ep (kRigEP layout)
+template <int CAM, bool RES>
+__device__ __forceinline__ void rig_pixel_terms(const float* ep, float X0, float Y0, float d, float wgt_u, float wgt_v,
+                                                float tgt_u, float tgt_v, PixTerms& o) {
+    const float* K = ep + 28;
+    if (CAM == 2) pixel_terms_mei<RES>(ep, ep + 9, X0, Y0, d, K, wgt_u, wgt_v, tgt_u, tgt_v, o);
+    else pixel_terms<false, RES, 6>(ep, ep + 9, 1.f, X0, Y0, d, K[0], K[1], K[2], K[3], wgt_u, wgt_v, tgt_u, tgt_v,
+                                    1.f, o);
 }
 
 // Cross-lane exchanges without the LDS crossbar.  Every stage pairs each lane
@@ -606,14 +755,95 @@ __device__ __forceinline__ void edge_pixel(const float* Rm, const float* t, floa
     }
 }
 
+// One rig-plan edge at this lane's pixel: as edge_pixel, with camera-frame
+// pose partials v and E row er, the self row taking -Adj(M)^T er, and an edge
+// within one frame contributing to C and w only.  Under the pinhole model
+// Ju[1] and Jv[0] are structurally zero and their products are left out, as in
+// edge_pixel; the Mei model has no such zeros.
+template <bool MOTION, bool ROBUST, int CAM>
+__device__ __forceinline__ void edge_pixel_rig(const float* ep, const float4& in, bool valid, float X0, float Y0,
+                                               float d, float act, int has_self, const RunArgs& a, float (&v)[32],
+                                               float (&er)[6], float& Csum, float& wsum, float (&es)[6]) {
+    constexpr bool SP = CAM == 1;
+#pragma unroll
+    for (int i = 0; i < 32; i++) v[i] = 0.f;
+#pragma unroll
+    for (int n = 0; n < 6; n++) er[n] = 0.f;
+    if (valid) {
+        PixTerms T;
+        rig_pixel_terms<CAM, true>(ep, X0, Y0, d, in.x, in.y, in.z, in.w, T);
+        if (ROBUST) {
+            T.wu *= robust_weight(T.ru, a.rk);
+            T.wv *= robust_weight(T.rv, a.rk);
+        }
+        const bool inframe = ep[12] != 0.f;
+        const float wdu = T.wu * act, wdv = T.wv * act;
+        const float pu = inframe ? 0.f : T.wu;
+        const float pv = inframe ? 0.f : T.wv;
+        int l = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++)
+#pragma unroll
+            for (int m = 0; m <= n; m++) {
+                if (SP) {
+                    const float hu = (n != 1 && m != 1) ? pu * T.Ju[n] * T.Ju[m] : 0.f;
+                    const float hv = (n != 0 && m != 0) ? pv * T.Jv[n] * T.Jv[m] : 0.f;
+                    v[l] = (n != 1 && m != 1) ? ((n != 0 && m != 0) ? hu + hv : hu) : hv;
+                } else {
+                    v[l] = pu * T.Ju[n] * T.Ju[m] + pv * T.Jv[n] * T.Jv[m];
+                }
+                l++;
+            }
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            if (SP) v[21 + n] = n == 1 ? pv * T.rv * T.Jv[n] : n == 0 ? pu * T.ru * T.Ju[n]
+                                                                      : pu * T.ru * T.Ju[n] + pv * T.rv * T.Jv[n];
+            else v[21 + n] = pu * T.ru * T.Ju[n] + pv * T.rv * T.Jv[n];
+        }
+        v[27] = T.wu * T.ru * T.ru + T.wv * T.rv * T.rv;
+        if (!MOTION) {
+            Csum += wdu * T.Jzu * T.Jzu + wdv * T.Jzv * T.Jzv;
+            wsum += wdu * T.ru * T.Jzu + wdv * T.rv * T.Jzv;
+            if (!inframe) {
+                const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
+#pragma unroll
+                for (int n = 0; n < 6; n++) {
+                    if (SP) er[n] = n == 1 ? cv * T.Jv[n] : n == 0 ? cu * T.Ju[n] : cu * T.Ju[n] + cv * T.Jv[n];
+                    else er[n] = cu * T.Ju[n] + cv * T.Jv[n];
+                }
+                if (has_self) {
+                    float o[6];
+                    neg_adjT<6>(ep + 16, ep + 25, 1.f, er, o);
+#pragma unroll
+                    for (int n = 0; n < 6; n++) es[n] += o[n];
+                }
+            }
+        }
+    }
+}
+
+// The LDS record of rig-plan edge e (kRigEP layout)
+__device__ __forceinline__ void rig_edge_record(const DevPlan& p, const RunArgs& a, int e, float* ep) {
+    rig_edge_transforms(p, a, e, ep, ep + 9, ep + 16, ep + 25);
+    ep[12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
+    ep[13] = __int_as_float(e);
+    ep[14] = 1.f;
+    const float* Kj = a.intrinsics + p.qj[e] * a.kstride;
+    for (int i = 0; i < 4; i++) ep[28 + i] = Kj[i];
+    ep[32] = a.kstride > 4 ? Kj[4] : 0.f;
+}
+
 // ---------------------------------------------------------------------------
 // 1. frame kernel
 // ---------------------------------------------------------------------------
-template <bool INTR, bool MOTION, bool ROBUST = false, int PD = 6>
+// CAM 0: single-view plans (DROID's pinhole model); CAM 1 / 2: rig plans with
+// the pinhole / Mei model (SE3, fixed intrinsics).
+template <bool INTR, bool MOTION, bool ROBUST = false, int PD = 6, int CAM = 0>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void ba_frame_kernel(DevPlan p, RunArgs a, Work w) {
+    static_assert(CAM == 0 || (PD == 6 && !INTR), "rig plans: SE3 poses, fixed intrinsics");
     extern __shared__ f32x4 dyn4[];  // stored E rows (RowMap) [.][TP+kRowPad]; reused for the Gram reduction
     float* dyn = reinterpret_cast<float*>(dyn4);
-    __shared__ float sEP[kMaxDeg][16];  // R(9) t(3) stereo(12) edge(13) scale(14)
+    __shared__ float sEP[kMaxDeg][CAM ? kRigEP : 16];  // R(9) t(3) stereo(12) edge(13) scale(14); rig: kRigEP
     __shared__ __align__(16) float sQ[256];
     __shared__ float sAcc[4][kMaxDeg][kNV];  // per-wave edge partials (each wave adds into its own slot)
     __shared__ float sFen[4];
@@ -634,17 +864,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const int NB = Rext * (Rext + 1) / 2;
     const int HW = p.HW;
     const float act = a.depth_active[kf];
-    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+    // intrinsics of the frame's view (rig plans) or of the single camera
+    const float* Kf = CAM ? a.intrinsics + (kf % max(p.V, 1)) * a.kstride : a.intrinsics;
+    const float fx = Kf[0], fy = Kf[1], cx = Kf[2], cy = Kf[3];
 
     if (tid < deg) {
         const int e = p.fr_edges[e0 + tid];
-        float Rm[9], t[3], sc;
-        edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
-        for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
-        for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
-        sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
-        sEP[tid][13] = __int_as_float(e);
-        sEP[tid][14] = sc;
+        if (CAM) {
+            rig_edge_record(p, a, e, sEP[tid]);
+        } else {
+            float Rm[9], t[3], sc;
+            edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
+            for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+            for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+            sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
+            sEP[tid][13] = __int_as_float(e);
+            sEP[tid][14] = sc;
+        }
     }
     for (int i = tid; i < 4 * kMaxDeg * kNV; i += kThreads) (&sAcc[0][0][0])[i] = 0.f;
     if (tid < 4) sFen[tid] = 0.f;
@@ -680,8 +916,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const bool valid = (tid < TP) && (pix < HW);
         const int py = valid ? pix / p.W : 0;
         const int px = valid ? pix - py * p.W : 0;
-        const float X0 = ((float)px - cx) / fx;
-        const float Y0 = ((float)py - cy) / fy;
+        float X0 = ((float)px - cx) / fx;
+        float Y0 = ((float)py - cy) / fy;
+        if (CAM) rig_iproj<CAM>(Kf, (float)px, (float)py, X0, Y0);
         const float d = valid ? a.disps[(int64_t)kf * HW + pix] : 0.f;
         const float ds = (!MOTION && valid) ? a.disps_sens[(int64_t)kf * HW + pix] : 0.f;
         const float et = (!MOTION && valid) ? a.eta[(int64_t)k * HW + pix] : 0.f;
@@ -702,8 +939,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             in1 = in2;
             edge_in(s + 2, pix, valid, in2);
             float v[32], fj[6], er[PD];
-            edge_pixel<INTR, MOTION, ROBUST, PD>(Rm, t, sc, stereo, in, valid, X0, Y0, d, act, has_self, a, fx, fy,
-                                                 cx, cy, v, fj, er, Csum, wsum, Ef, es);
+            if constexpr (CAM != 0) {
+#pragma unroll
+                for (int n = 0; n < 6; n++) fj[n] = 0.f;
+                edge_pixel_rig<MOTION, ROBUST, CAM>(sEP[s], in, valid, X0, Y0, d, act, has_self, a, v, er, Csum,
+                                                    wsum, es);
+            } else {
+                edge_pixel<INTR, MOTION, ROBUST, PD>(Rm, t, sc, stereo, in, valid, X0, Y0, d, act, has_self, a, fx,
+                                                     fy, cx, cy, v, fj, er, Csum, wsum, Ef, es);
+            }
             if (!MOTION && tid < TP) {
 #pragma unroll
                 for (int n = 0; n < 6; n++) dyn[(has_self * PD + 6 * s + n) * ldr + tid] = er[n];
@@ -818,9 +1062,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 // Schur Gram then runs over the tile's 256 pixels in chunks of 4 x kGramSlots
 // tile pairs (each wave owns its pairs, so no cross-wave reduction), and each
 // chunk's elements go straight to their PD x PD blocks.
-template <bool INTR, bool MOTION, bool ROBUST, int PD>
+template <bool INTR, bool MOTION, bool ROBUST, int PD, int CAM = 0>
 __global__ __launch_bounds__(kThreads) void ba_frame_wide_kernel(DevPlan p, RunArgs a, Work w) {
-    __shared__ float sEP[kWideGroup][16];  // R(9) t(3) stereo(12) edge(13) scale(14)
+    static_assert(CAM == 0 || (PD == 6 && !INTR), "rig plans: SE3 poses, fixed intrinsics");
+    __shared__ float sEP[kWideGroup][CAM ? kRigEP : 16];  // R(9) t(3) stereo(12) edge(13) scale(14); rig: kRigEP
     __shared__ __align__(16) float sQ[kThreads];
     __shared__ float sAcc[4][kWideGroup][kNV];
     __shared__ float sFen[4];
@@ -837,13 +1082,15 @@ __global__ __launch_bounds__(kThreads) void ba_frame_wide_kernel(DevPlan p, RunA
     const int Rext = R + 1;
     const int HW = p.HW;
     const float act = a.depth_active[kf];
-    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+    const float* Kf = CAM ? a.intrinsics + (kf % max(p.V, 1)) * a.kstride : a.intrinsics;
+    const float fx = Kf[0], fy = Kf[1], cx = Kf[2], cy = Kf[3];
     const int pix = tile * kThreads + tid;  // 256-pixel tiles, as the frame kernel's TP x SUB
     const bool valid = pix < HW;
     const int py = valid ? pix / p.W : 0;
     const int px = valid ? pix - py * p.W : 0;
-    const float X0 = ((float)px - cx) / fx;
-    const float Y0 = ((float)py - cy) / fy;
+    float X0 = ((float)px - cx) / fx;
+    float Y0 = ((float)py - cy) / fy;
+    if (CAM) rig_iproj<CAM>(Kf, (float)px, (float)py, X0, Y0);
     const float d = valid ? a.disps[(int64_t)kf * HW + pix] : 0.f;
     const RowMap rmap{has_self, R, has_self * PD + 6 * deg, has_self * PD + 6 * deg + 2};
     float* Eg = w.wide_e + ((int64_t)slot * p.T + tile) * (int64_t)p.wide_rows * kThreads;
@@ -862,13 +1109,17 @@ __global__ __launch_bounds__(kThreads) void ba_frame_wide_kernel(DevPlan p, RunA
         const int gn = min(kWideGroup, deg - g0);
         if (tid < gn) {
             const int e = p.fr_edges[e0 + g0 + tid];
-            float Rm[9], t[3], sc;
-            edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
-            for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
-            for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
-            sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
-            sEP[tid][13] = __int_as_float(e);
-            sEP[tid][14] = sc;
+            if (CAM) {
+                rig_edge_record(p, a, e, sEP[tid]);
+            } else {
+                float Rm[9], t[3], sc;
+                edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
+                for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+                for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+                sEP[tid][12] = (p.ii[e] == p.jj[e]) ? 1.f : 0.f;
+                sEP[tid][13] = __int_as_float(e);
+                sEP[tid][14] = sc;
+            }
         }
         for (int i = tid; i < 4 * kWideGroup * kNV; i += kThreads) (&sAcc[0][0][0])[i] = 0.f;
         float4 in1 = make_float4(0.f, 0.f, 0.f, 0.f), in2 = in1;
@@ -880,8 +1131,16 @@ __global__ __launch_bounds__(kThreads) void ba_frame_wide_kernel(DevPlan p, RunA
             in1 = in2;
             edge_in(g0 + s + 2 < g0 + gn ? g0 + s + 2 : deg, in2);
             float v[32], fj[6], er[PD];
-            edge_pixel<INTR, MOTION, ROBUST, PD>(sEP[s], sEP[s] + 9, sEP[s][14], sEP[s][12] != 0.f, in, valid, X0, Y0,
-                                                 d, act, has_self, a, fx, fy, cx, cy, v, fj, er, Csum, wsum, Ef, es);
+            if constexpr (CAM != 0) {
+#pragma unroll
+                for (int n = 0; n < 6; n++) fj[n] = 0.f;
+                edge_pixel_rig<MOTION, ROBUST, CAM>(sEP[s], in, valid, X0, Y0, d, act, has_self, a, v, er, Csum,
+                                                    wsum, es);
+            } else {
+                edge_pixel<INTR, MOTION, ROBUST, PD>(sEP[s], sEP[s] + 9, sEP[s][14], sEP[s][12] != 0.f, in, valid,
+                                                     X0, Y0, d, act, has_self, a, fx, fy, cx, cy, v, fj, er, Csum,
+                                                     wsum, Ef, es);
+            }
             if (!MOTION) {
 #pragma unroll
                 for (int n = 0; n < 6; n++) Eg[(int64_t)(has_self * PD + 6 * (g0 + s) + n) * kThreads + tid] = er[n];
@@ -981,12 +1240,13 @@ __global__ __launch_bounds__(kThreads) void ba_frame_wide_kernel(DevPlan p, RunA
 // ---------------------------------------------------------------------------
 // 2. finalize: tile sums in FP64, i-side edge blocks
 // ---------------------------------------------------------------------------
-template <int PD>
+template <int PD, int CAM = 0>
 __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArgs a, Work w, int motion) {
     using RC = Rec<PD>;
     constexpr int PD2 = PD * PD;
     __shared__ double s[64];
     __shared__ double Ad[PD2];
+    __shared__ double Arig[CAM ? 36 : 1], Trig[CAM ? 36 : 1], Hrig[CAM ? 36 : 1], vrig[CAM ? 6 : 1];
     __shared__ double RT[13];  // R (row-major), t and s of the edge's relative pose
     __shared__ double Tm[PD2];  // Adj^T Hjj
     __shared__ double tpart[kThreads / kNV][kNV];  // per-slice tile sums of an edge
@@ -1011,11 +1271,14 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
             tpart[sl][v] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         } else if (tid == kThreads - 1) {
             // the relative pose, while the other threads sum the tiles
-            float Rm[9], t[3], sc;
-            edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
+            float Rm[9], t[3], sc = 1.f;
+            if (CAM) rig_pose_transform(p, a, e, Rm, t);
+            else edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
             for (int i = 0; i < 9; i++) RT[i] = Rm[i];
             for (int i = 0; i < 3; i++) RT[9 + i] = t[i];
             RT[12] = sc;
+        } else if (CAM && tid == kThreads - 2) {
+            rig_adj_inv(a.rig, p.qj[e], Arig);
         }
         __syncthreads();
         if (tid < kNV) {
@@ -1048,13 +1311,39 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
         }
         __syncthreads();
         double* rec = w.ed + (int64_t)e * RC::SIZE;
+        auto hsum = [&](int n, int m) -> double {
+            const int hi = n > m ? n : m, lo = n > m ? m : n;
+            return s[hi * (hi + 1) / 2 + lo];
+        };
+        if (CAM) {
+            // rig plans: the partials are camera-frame; Hjj = A^T Hc A, vj = A^T vc
+            if (tid < 36) {
+                const int i = tid / 6, j = tid % 6;
+                double t = 0.0;
+                for (int q = 0; q < 6; q++) t += Arig[q * 6 + i] * hsum(q, j);
+                Trig[tid] = t;
+            } else if (tid >= 64 && tid < 70) {
+                const int i = tid - 64;
+                double t = 0.0;
+                for (int q = 0; q < 6; q++) t += Arig[q * 6 + i] * s[21 + q];
+                vrig[i] = t;
+            }
+            __syncthreads();
+            if (tid < 36) {
+                const int i = tid / 6, j = tid % 6;
+                double h = 0.0;
+                for (int q = 0; q < 6; q++) h += Trig[i * 6 + q] * Arig[q * 6 + j];
+                Hrig[tid] = h;
+            }
+            __syncthreads();
+        }
         // Hjj from the 6-dof partials (its sigma row and column are zero);
         // T = Adj^T Hjj (the i-j block is -T), then Hii = T Adj
         auto hjj = [&](int n, int m) -> double {
             if (n >= 6 || m >= 6) return 0.0;
-            const int hi = n > m ? n : m, lo = n > m ? m : n;
-            return s[hi * (hi + 1) / 2 + lo];
+            return CAM ? Hrig[n * 6 + m] : hsum(n, m);
         };
+        auto vjv = [&](int q) -> double { return CAM ? vrig[q] : s[21 + q]; };
         if (tid < PD2) {
             const int i = tid / PD, j = tid % PD;
             rec[RC::HJJ + tid] = hjj(i, j);
@@ -1073,11 +1362,11 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
             const int i = tid - 64;
             double vi = 0.0, fi = 0.0;
             for (int q = 0; q < 6; q++) {
-                vi += Ad[q * PD + i] * s[21 + q];
+                vi += Ad[q * PD + i] * vjv(q);
                 fi += Ad[q * PD + i] * s[32 + q];
             }
             rec[RC::VI + i] = -vi;
-            rec[RC::VJ + i] = i < 6 ? s[21 + i] : 0.0;
+            rec[RC::VJ + i] = i < 6 ? vjv(i) : 0.0;
             rec[RC::FI + i] = -fi;
             rec[RC::FJ + i] = i < 6 ? s[32 + i] : 0.0;
         } else if (tid == 64 + PD) {
@@ -1118,6 +1407,51 @@ __global__ __launch_bounds__(kThreads) void ba_finalize_kernel(DevPlan p, RunArg
             for (int u = 0; u < 32; u++) acc += (t0 + u < p.T) ? (double)v[u] : 0.0;
         }
         w.fen[k] = acc;
+    }
+}
+
+// Rig plans: a depth frame's Gram blocks hold camera-frame E rows for its
+// edge rows.  Block (ra, rb), ra <= rb, becomes A_a^T B A_b, where A is the
+// edge's Adj(rig[qj]^-1) for an edge row and the identity for the self row and
+// the pseudo row [w, Ef].  One workgroup per depth frame, one block per thread
+// at a time, a row (then a column) of the block in registers.
+__global__ __launch_bounds__(64) void ba_gram_rig_kernel(DevPlan p, RunArgs a, Work w) {
+    const int k = blockIdx.x;
+    const int has_self = p.fr_self[k];
+    const int e0 = p.fr_ptr[k];
+    const int R = has_self + p.fr_ptr[k + 1] - e0;
+    const int Rext = R + 1;
+    const int NB = Rext * (Rext + 1) / 2;
+    for (int b = threadIdx.x; b < NB; b += 64) {
+        int ra, rb;
+        gram_rows(b, Rext, ra, rb);
+        const bool ea = ra >= has_self && ra < R, eb = rb >= has_self && rb < R;
+        double* B = w.gd + ((int64_t)p.gram_off[k] + b) * 36;
+        double A[36];
+        if (eb) {
+            rig_adj_inv(a.rig, p.qj[p.fr_edges[e0 + rb - has_self]], A);
+            for (int i = 0; i < 6; i++) {
+                double row[6];
+                for (int q = 0; q < 6; q++) row[q] = B[6 * i + q];
+                for (int j = 0; j < 6; j++) {
+                    double o = 0.0;
+                    for (int q = 0; q < 6; q++) o += row[q] * A[6 * q + j];
+                    B[6 * i + j] = o;
+                }
+            }
+        }
+        if (ea) {
+            rig_adj_inv(a.rig, p.qj[p.fr_edges[e0 + ra - has_self]], A);
+            for (int j = 0; j < 6; j++) {
+                double col[6];
+                for (int q = 0; q < 6; q++) col[q] = B[6 * q + j];
+                for (int i = 0; i < 6; i++) {
+                    double o = 0.0;
+                    for (int q = 0; q < 6; q++) o += A[6 * q + i] * col[q];
+                    B[6 * i + j] = o;
+                }
+            }
+        }
     }
 }
 
@@ -1845,9 +2179,10 @@ __global__ __launch_bounds__(kBT) void chol_backsolve_kernel(const double* __res
 // ---------------------------------------------------------------------------
 // 5. back-substitution: dz = Q (w - sum_r E_r . dx_r) - Q Ef df
 // ---------------------------------------------------------------------------
-template <bool INTR, int PD = 6>
+template <bool INTR, int PD = 6, int CAM = 0>
 __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs a, Work w) {
-    __shared__ float sEP[kMaxDeg][16];
+    static_assert(CAM == 0 || (PD == 6 && !INTR), "rig plans: SE3 poses, fixed intrinsics");
+    __shared__ float sEP[kMaxDeg][CAM ? kRigEP : 16];
     __shared__ float sU[kMaxDeg][6];  // the sigma entry of u meets a zero E entry
     const int k = blockIdx.y;
     const int tid = threadIdx.x;
@@ -1856,23 +2191,31 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
     const int deg = p.fr_ptr[k + 1] - e0;
     const int has_self = p.fr_self[k];
     const float act = a.depth_active[kf];
-    const float fx = a.intrinsics[0], fy = a.intrinsics[1], cx = a.intrinsics[2], cy = a.intrinsics[3];
+    const float* Kf = CAM ? a.intrinsics + (kf % max(p.V, 1)) * a.kstride : a.intrinsics;
+    const float fx = Kf[0], fy = Kf[1], cx = Kf[2], cy = Kf[3];
     const int HW = p.HW;
     // edge slot tid of the group starting at edge g0: its relative pose and
-    // u = dx[jj] (if 1 <= jj-t0 < P) - Adj(Tij) dx[k] (self row, if 1 <= k-t0 < P)
+    // u = dx[jj] (if 1 <= jj-t0 < P) - Adj(Tij) dx[k] (self row, if 1 <= k-t0 < P);
+    // rig plans: u = A (dx[pj] - Adj(G) dx[pi]) with the same conditions, and
+    // zero for an edge within one frame (its E rows are zero)
     auto setup = [&](int g0, int gn) {
         if (tid >= gn) return;
         const int e = p.fr_edges[e0 + g0 + tid];
-        float Rm[9], t[3], sc;
-        edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
-        for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
-        for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
-        sEP[tid][14] = sc;
+        float Rm[9], t[3], sc = 1.f;
+        if (CAM) {
+            rig_edge_record(p, a, e, sEP[tid]);
+            rig_pose_transform(p, a, e, Rm, t);
+        } else {
+            edge_transform<PD>(a.poses, p.ii[e], p.jj[e], Rm, t, sc);
+            for (int i = 0; i < 9; i++) sEP[tid][i] = Rm[i];
+            for (int i = 0; i < 3; i++) sEP[tid][9 + i] = t[i];
+            sEP[tid][14] = sc;
+        }
         double u[6] = {0, 0, 0, 0, 0, 0};
         const int lj = p.jj[e] - p.t0;
         if (lj >= 1 && lj < p.P)
             for (int n = 0; n < 6; n++) u[n] += w.x[PD * lj + n];
-        const int lk = kf - p.t0;
+        const int lk = (CAM ? p.kpose[k] : kf) - p.t0;
         if (has_self && lk >= 1 && lk < p.P) {
             const double* v = w.x + PD * lk;
             double rv[3], rw[3];
@@ -1890,6 +2233,17 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
             u[3] -= rw[0];
             u[4] -= rw[1];
             u[5] -= rw[2];
+        }
+        if (CAM) {
+            double A[36], ua[6];
+            rig_adj_inv(a.rig, p.qj[e], A);
+            const bool inframe = p.ii[e] == p.jj[e];
+            for (int i = 0; i < 6; i++) {
+                double o = 0.0;
+                for (int q = 0; q < 6; q++) o += A[6 * i + q] * u[q];
+                ua[i] = inframe ? 0.0 : o;
+            }
+            for (int n = 0; n < 6; n++) u[n] = ua[n];
         }
         for (int n = 0; n < 6; n++) sU[tid][n] = (float)u[n];
     };
@@ -1917,8 +2271,9 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
     };
     const double df = INTR ? w.x[PD * p.P] : 0.0;
     const int py = valid ? pix / p.W : 0, px = valid ? pix - py * p.W : 0;
-    const float X0 = ((float)px - cx) / fx;
-    const float Y0 = ((float)py - cy) / fy;
+    float X0 = ((float)px - cx) / fx;
+    float Y0 = ((float)py - cy) / fy;
+    if (CAM) rig_iproj<CAM>(Kf, (float)px, (float)py, X0, Y0);
     float dot = 0.f, Ef = 0.f;
     // edges in groups of kMaxDeg (one group unless the frame is wide)
     for (int g0 = 0; g0 < deg; g0 += kMaxDeg) {
@@ -1939,13 +2294,15 @@ __global__ __launch_bounds__(kThreads) void ba_backsub_kernel(DevPlan p, RunArgs
             PixTerms T;
             if (rob) {
                 // the frame kernel's robust weights: residuals at the same linearisation point
-                pixel_terms<INTR, true, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y,
-                                            in.z, in.w, a.iscale, T);
+                if (CAM) rig_pixel_terms<CAM, true>(sEP[s], X0, Y0, d, in.x, in.y, in.z, in.w, T);
+                else pixel_terms<INTR, true, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x,
+                                                 in.y, in.z, in.w, a.iscale, T);
                 T.wu *= robust_weight(T.ru, a.rk);
                 T.wv *= robust_weight(T.rv, a.rk);
             } else {
-                pixel_terms<INTR, false, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x, in.y,
-                                             0.f, 0.f, a.iscale, T);
+                if (CAM) rig_pixel_terms<CAM, false>(sEP[s], X0, Y0, d, in.x, in.y, 0.f, 0.f, T);
+                else pixel_terms<INTR, false, PD>(sEP[s], sEP[s] + 9, sEP[s][14], X0, Y0, d, fx, fy, cx, cy, in.x,
+                                                  in.y, 0.f, 0.f, a.iscale, T);
             }
             const float wdu = T.wu * act, wdv = T.wv * act;
             const float cu = act * wdu * T.Jzu, cv = act * wdv * T.Jzv;
@@ -2517,6 +2874,10 @@ __global__ __launch_bounds__(256) void robust_weight_kernel(RobustK k, const flo
         w[i] = robust_weight(x[i], k);
 }
 
+static vh_ba_plan* plan_build(const int64_t* ii_host, const int64_t* jj_host, const int64_t* qi_host,
+                              const int64_t* qj_host, int V, int E, int N, int H, int W, int t0, int t1,
+                              int pixel_tile);
+
 extern "C" {
 
 vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W, int t0,
@@ -2526,12 +2887,37 @@ vh_ba_plan* vh_ba_plan_create(const int64_t* ii_host, const int64_t* jj_host, in
 
 vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_host, int E, int N, int H, int W,
                                     int t0, int t1, int pixel_tile) {
+    return plan_build(ii_host, jj_host, nullptr, nullptr, 0, E, N, H, W, t0, t1, pixel_tile);
+}
+
+vh_ba_plan* vh_ba_plan_create_rig(const int64_t* pi_host, const int64_t* pj_host, const int64_t* qi_host,
+                                  const int64_t* qj_host, int E, int N, int V, int H, int W, int t0, int t1,
+                                  int pixel_tile) {
+    if (V < 1) {
+        fail(VH_ERR_ARG, "ba_plan_create_rig: %d views", V);
+        return nullptr;
+    }
+    if (E > 0 && (!pi_host || !pj_host || !qi_host || !qj_host)) {
+        fail(VH_ERR_ARG, "ba_plan_create_rig: null index array");
+        return nullptr;
+    }
+    return plan_build(pi_host, pj_host, qi_host, qj_host, V, E, N, H, W, t0, t1, pixel_tile);
+}
+
+}  // extern "C"
+
+// Plan of a single-view graph (V = 0: depth frame id = pose id = ii) or of a
+// rig graph (V >= 1: edge e from view qi of pose ii to view qj of pose jj;
+// depth frame id ii V + qi over the flattened [N V, H, W] disparities).
+static vh_ba_plan* plan_build(const int64_t* ii_host, const int64_t* jj_host, const int64_t* qi_host,
+                              const int64_t* qj_host, int V, int E, int N, int H, int W, int t0, int t1,
+                              int pixel_tile) {
     if (pixel_tile != 0 && pixel_tile != 64 && pixel_tile != 128 && pixel_tile != 256) {
         fail(VH_ERR_ARG, "ba_plan_create: pixel_tile %d is not 0, 64, 128 or 256", pixel_tile);
         return nullptr;
     }
-    if (E < 0 || N <= 0 || H <= 0 || W <= 0 || t0 < 0 || t1 <= t0 || t1 > N) {
-        fail(VH_ERR_ARG, "ba_plan_create: bad sizes (E=%d N=%d H=%d W=%d t0=%d t1=%d)", E, N, H, W, t0, t1);
+    if (E < 0 || N <= 0 || H <= 0 || W <= 0 || t0 < 0 || t1 <= t0 || t1 > N || (int64_t)N * std::max(V, 1) >= (1 << 27)) {
+        fail(VH_ERR_ARG, "ba_plan_create: bad sizes (E=%d N=%d V=%d H=%d W=%d t0=%d t1=%d)", E, N, V, H, W, t0, t1);
         return nullptr;
     }
     for (int e = 0; e < E; e++) {
@@ -2539,6 +2925,18 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
             fail(VH_ERR_ARG, "ba_plan_create: edge %d (%lld -> %lld) outside [0, %d)", e, (long long)ii_host[e],
                  (long long)jj_host[e], N);
             return nullptr;
+        }
+        if (V > 0) {
+            if (qi_host[e] < 0 || qi_host[e] >= V || qj_host[e] < 0 || qj_host[e] >= V) {
+                fail(VH_ERR_ARG, "ba_plan_create_rig: edge %d views (%lld -> %lld) outside [0, %d)", e,
+                     (long long)qi_host[e], (long long)qj_host[e], V);
+                return nullptr;
+            }
+            if (ii_host[e] == jj_host[e] && qi_host[e] == qj_host[e]) {
+                fail(VH_ERR_ARG, "ba_plan_create_rig: edge %d maps view %lld of frame %lld onto itself", e,
+                     (long long)qi_host[e], (long long)ii_host[e]);
+                return nullptr;
+            }
         }
     }
     vh_ba_plan* pl = new (std::nothrow) vh_ba_plan();
@@ -2554,23 +2952,29 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         fail(VH_ERR_UNSUPPORTED, "ba_plan_create: %d poses in the window (limit %d)", P, kMaxPoses);
         return nullptr;
     }
-    // kx = unique(cat(arange(t0,t1), ii)), sorted (geom_kernels.cu:1648-1652)
+    // kx = unique(cat(arange(t0,t1), ii)), sorted (geom_kernels.cu:1648-1652);
+    // rig plans: every view of the window's frames and the edges' source
+    // depth frames ii V + qi
+    const int VV = std::max(V, 1);
+    std::vector<int64_t> dsrc(E);
+    for (int e = 0; e < E; e++) dsrc[e] = V > 0 ? ii_host[e] * V + qi_host[e] : ii_host[e];
     std::vector<int64_t> cand;
-    for (int t = t0; t < t1; t++) cand.push_back(t);
-    for (int e = 0; e < E; e++) cand.push_back(ii_host[e]);
+    for (int64_t t = (int64_t)t0 * VV; t < (int64_t)t1 * VV; t++) cand.push_back(t);
+    for (int e = 0; e < E; e++) cand.push_back(dsrc[e]);
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     pl->kx = cand;
     const int K = (int)cand.size();
-    std::vector<int> slot(N, -1);
+    std::vector<int> slot((size_t)N * VV, -1);
     for (int k = 0; k < K; k++) slot[cand[k]] = k;
+    auto kpose_of = [&](int k) { return (int)(cand[k] / VV); };
     // per-frame edge lists in edge order
     std::vector<int> fr_ptr(K + 1, 0), fr_edges(E), fr_self(K);
-    for (int e = 0; e < E; e++) fr_ptr[slot[ii_host[e]] + 1]++;
+    for (int e = 0; e < E; e++) fr_ptr[slot[dsrc[e]] + 1]++;
     for (int k = 0; k < K; k++) fr_ptr[k + 1] += fr_ptr[k];
     {
         std::vector<int> fill(fr_ptr.begin(), fr_ptr.end() - 1);
-        for (int e = 0; e < E; e++) fr_edges[fill[slot[ii_host[e]]]++] = e;
+        for (int e = 0; e < E; e++) fr_edges[fill[slot[dsrc[e]]]++] = e;
     }
     int deg_max = 0, Rext_max = 1, wide_rows = 0;
     std::vector<int> wide;
@@ -2582,7 +2986,7 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         }
         std::sort(js.begin(), js.end());
         if (std::adjacent_find(js.begin(), js.end()) != js.end()) pl->adj_distinct = false;
-        fr_self[k] = (cand[k] >= t0 && cand[k] < t1) ? 1 : 0;
+        fr_self[k] = (kpose_of(k) >= t0 && kpose_of(k) < t1) ? 1 : 0;
         const int deg = fr_ptr[k + 1] - fr_ptr[k];
         deg_max = std::max(deg_max, deg);
         if (deg > kMaxDeg) {
@@ -2649,7 +3053,7 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         const int R = fr_self[k] + deg;
         std::vector<int> rpose(R);
         for (int r = 0; r < R; r++) {
-            if (fr_self[k] && r == 0) rpose[r] = (int)cand[k] - t0;
+            if (fr_self[k] && r == 0) rpose[r] = kpose_of(k) - t0;
             else rpose[r] = (int)jj_host[fr_edges[fr_ptr[k] + r - fr_self[k]]] - t0;
         }
         for (int ra = 0; ra < R; ra++) {
@@ -2679,7 +3083,7 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         const int R = fr_self[k] + deg;
         for (int r = 0; r < R; r++) {
             int pr;
-            if (fr_self[k] && r == 0) pr = (int)cand[k] - t0;
+            if (fr_self[k] && r == 0) pr = kpose_of(k) - t0;
             else pr = (int)jj_host[fr_edges[fr_ptr[k] + r - fr_self[k]]] - t0;
             if (pr < 0 || pr >= P) continue;
             ritems[pr].push_back(pk(K_ROW, gblock(k, r, R)));
@@ -2701,18 +3105,26 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
         rhs_ptr[a + 1] = rhs_ptr[a] + (int)ritems[a].size();
         rhs_items.insert(rhs_items.end(), ritems[a].begin(), ritems[a].end());
     }
-    std::vector<int> kx32(K), ii32(E), jj32(E);
-    for (int k = 0; k < K; k++) kx32[k] = (int)cand[k];
+    std::vector<int> kx32(K), ii32(E), jj32(E), qi32(E, 0), qj32(E, 0), kpose32(K);
+    for (int k = 0; k < K; k++) {
+        kx32[k] = (int)cand[k];
+        kpose32[k] = kpose_of(k);
+    }
     for (int e = 0; e < E; e++) {
         ii32[e] = (int)ii_host[e];
         jj32[e] = (int)jj_host[e];
+        if (V > 0) {
+            qi32[e] = (int)qi_host[e];
+            qj32[e] = (int)qj_host[e];
+        }
     }
     // upload all tables in one allocation
     std::vector<std::pair<const std::vector<int>*, const int**>> tabs = {
         {&kx32, &pl->d.kx}, {&fr_ptr, &pl->d.fr_ptr}, {&fr_edges, &pl->d.fr_edges}, {&fr_self, &pl->d.fr_self},
         {&gram_off, &pl->d.gram_off}, {&ii32, &pl->d.ii}, {&jj32, &pl->d.jj}, {&blk_ab, &pl->d.blk_ab},
         {&blk_ptr, &pl->d.blk_ptr}, {&blk_items, &pl->d.blk_items}, {&rhs_ptr, &pl->d.rhs_ptr},
-        {&rhs_items, &pl->d.rhs_items}, {&pseudo_off, &pl->d.pseudo_off}, {&wide, &pl->d.wide}};
+        {&rhs_items, &pl->d.rhs_items}, {&pseudo_off, &pl->d.pseudo_off}, {&wide, &pl->d.wide},
+        {&qi32, &pl->d.qi}, {&qj32, &pl->d.qj}, {&kpose32, &pl->d.kpose}};
     size_t total = 0;
     std::vector<size_t> offs;
     for (auto& t : tabs) {
@@ -2742,6 +3154,7 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
     d.TP = TP; d.SUB = SUB; d.T = T; d.Rext_max = Rext_max;
     d.n_wide = (int)wide.size();
     d.wide_rows = wide_rows;
+    d.V = V;
     d.nlb = nlb;
     pl->gram_blocks = gram_off[K];
     // SE3 sizes; ba_run_impl sets them for the run's pose group
@@ -2778,6 +3191,8 @@ vh_ba_plan* vh_ba_plan_create_tiled(const int64_t* ii_host, const int64_t* jj_ho
     return pl;
 }
 
+extern "C" {
+
 void vh_ba_plan_destroy(vh_ba_plan* plan) {
     if (!plan) return;
     if (plan->dev) (void)hipFree(plan->dev);
@@ -2799,10 +3214,21 @@ int vh_ba_plan_pixel_tile(const vh_ba_plan* plan) { return plan ? plan->d.TP : -
 }  // extern "C"
 
 // The wide-frame kernel over the plan's wide frames, in the frame kernel's variant.
-template <int PD>
+template <int PD, int CAM = 0>
 static void launch_wide(const DevPlan& d, const RunArgs& a, const Work& w, bool motion, bool intr, bool robust,
                         hipStream_t s) {
     const dim3 g(d.T, d.n_wide), b(kThreads);
+    if constexpr (CAM != 0) {
+        // rig plans: fixed intrinsics
+        if (robust) {
+            if (motion) hipLaunchKernelGGL((ba_frame_wide_kernel<false, true, true, 6, CAM>), g, b, 0, s, d, a, w);
+            else hipLaunchKernelGGL((ba_frame_wide_kernel<false, false, true, 6, CAM>), g, b, 0, s, d, a, w);
+        } else {
+            if (motion) hipLaunchKernelGGL((ba_frame_wide_kernel<false, true, false, 6, CAM>), g, b, 0, s, d, a, w);
+            else hipLaunchKernelGGL((ba_frame_wide_kernel<false, false, false, 6, CAM>), g, b, 0, s, d, a, w);
+        }
+        return;
+    }
 #define VH_WIDE(I, M, R) hipLaunchKernelGGL((ba_frame_wide_kernel<I, M, R, PD>), g, b, 0, s, d, a, w)
     if (robust) {
         if (motion) {
@@ -2822,8 +3248,23 @@ static void launch_wide(const DevPlan& d, const RunArgs& a, const Work& w, bool 
 #undef VH_WIDE
 }
 
-// The launches of `iterations` Gauss-Newton iterations for pose blocks of PD.
-template <int PD>
+// The frame kernel of a rig plan (fixed intrinsics) in its variant.
+template <int CAM>
+static void launch_frame_rig(const DevPlan& d, const RunArgs& a, const Work& w, bool motion, bool robust, size_t lds,
+                             hipStream_t s) {
+    const dim3 g(d.T, d.K), b(kThreads);
+    if (robust) {
+        if (motion) hipLaunchKernelGGL((ba_frame_kernel<false, true, true, 6, CAM>), g, b, lds, s, d, a, w);
+        else hipLaunchKernelGGL((ba_frame_kernel<false, false, true, 6, CAM>), g, b, lds, s, d, a, w);
+    } else {
+        if (motion) hipLaunchKernelGGL((ba_frame_kernel<false, true, false, 6, CAM>), g, b, lds, s, d, a, w);
+        else hipLaunchKernelGGL((ba_frame_kernel<false, false, false, 6, CAM>), g, b, lds, s, d, a, w);
+    }
+}
+
+// The launches of `iterations` Gauss-Newton iterations for pose blocks of PD
+// (CAM 0: a single-view plan; 1 / 2: a rig plan with the pinhole / Mei model).
+template <int PD, int CAM = 0>
 static int ba_iterations(const DevPlan& d, RunArgs a, const Work& w, int iterations, bool motion, bool intr,
                          const RobustK& rk, float* poses, float* intrinsics, float* dx, float intrinsics_scale,
                          hipStream_t s) {
@@ -2832,7 +3273,23 @@ static int ba_iterations(const DevPlan& d, RunArgs a, const Work& w, int iterati
     const int npad = d.ld - d.n;
     for (int itr = 0; itr < iterations; itr++) {
         a.itr = itr;
-        if (d.E > 0 && d.K > 0) {
+        if constexpr (CAM != 0) {
+          if (d.K > 0 && (d.E > 0 || !motion)) {
+            launch_frame_rig<CAM>(d, a, w, motion, rk.type != 0, lds, s);
+            VH_LAUNCH_CHECK("ba_frame_kernel");
+            if (d.n_wide > 0) {
+                launch_wide<PD, CAM>(d, a, w, motion, false, rk.type != 0, s);
+                VH_LAUNCH_CHECK("ba_frame_wide_kernel");
+            }
+            hipLaunchKernelGGL((ba_finalize_kernel<PD, CAM>), dim3(d.E + (motion ? 0 : d.K * kGramSplit)),
+                               dim3(kThreads), 0, s, d, a, w, motion ? 1 : 0);
+            VH_LAUNCH_CHECK("ba_finalize_kernel");
+            if (!motion) {
+                hipLaunchKernelGGL(ba_gram_rig_kernel, dim3(d.K), dim3(64), 0, s, d, a, w);
+                VH_LAUNCH_CHECK("ba_gram_rig_kernel");
+            }
+          }  // motion only without edges: nothing to accumulate
+        } else if (d.E > 0 && d.K > 0) {
             if (rk.type) {
                 // robust weights: a separate instantiation, so plain least squares pays nothing
                 if (motion) {
@@ -2880,8 +3337,12 @@ static int ba_iterations(const DevPlan& d, RunArgs a, const Work& w, int iterati
         VH_LAUNCH_CHECK("chol_backsolve_kernel");
         if (!motion && d.K > 0) {
             const dim3 bgrid((d.HW + kThreads - 1) / kThreads, d.K);
-            if (intr) hipLaunchKernelGGL((ba_backsub_kernel<true, PD>), bgrid, dim3(kThreads), 0, s, d, a, w);
-            else hipLaunchKernelGGL((ba_backsub_kernel<false, PD>), bgrid, dim3(kThreads), 0, s, d, a, w);
+            if constexpr (CAM != 0) {
+                hipLaunchKernelGGL((ba_backsub_kernel<false, 6, CAM>), bgrid, dim3(kThreads), 0, s, d, a, w);
+            } else {
+                if (intr) hipLaunchKernelGGL((ba_backsub_kernel<true, PD>), bgrid, dim3(kThreads), 0, s, d, a, w);
+                else hipLaunchKernelGGL((ba_backsub_kernel<false, PD>), bgrid, dim3(kThreads), 0, s, d, a, w);
+            }
             VH_LAUNCH_CHECK("ba_backsub_kernel");
         }
         if (PD == 7)
@@ -2897,12 +3358,21 @@ static int ba_iterations(const DevPlan& d, RunArgs a, const Work& w, int iterati
 
 extern "C" {
 
+// camera < 0: a single-view plan; VH_CAM_PINHOLE / VH_CAM_MEI: a rig plan
+// with per-view intrinsics [V][4 or 5] and rig [V][7]
 static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float* intrinsics,
                        const float* disps_sens, const float* targets, const float* weights, const float* eta,
                        const float* depth_active, int iterations, float lm, float ep, float alpha,
                        float intrinsics_lm, float intrinsics_ep, float intrinsics_scale, int flags, const RobustK& rk,
-                       float* dx, float* dz, float* energy, void* workspace, void* stream) {
+                       float* dx, float* dz, float* energy, void* workspace, void* stream, int camera = -1,
+                       const float* rig = nullptr) {
     if (!plan) return fail(VH_ERR_ARG, "ba_run: null plan");
+    if (plan->d.V > 0 && camera < 0)
+        return fail(VH_ERR_ARG, "ba_run: a rig plan (vh_ba_plan_create_rig) runs through vh_ba_run_rig");
+    if (plan->d.V == 0 && camera >= 0)
+        return fail(VH_ERR_ARG, "ba_run_rig: the plan is a single-view plan (make it with vh_ba_plan_create_rig)");
+    if (camera >= 0 && (flags & (VH_BA_SIM3 | VH_BA_OPTIMIZE_INTRINSICS)))
+        return fail(VH_ERR_UNSUPPORTED, "ba_run_rig: SE3 poses with fixed intrinsics only");
     if (iterations < 0) return fail(VH_ERR_ARG, "ba_run: negative iterations");
     if (iterations == 0) return VH_OK;
     if (!poses || !disps || !intrinsics || !disps_sens || !targets || !weights || !depth_active || !dx ||
@@ -2959,6 +3429,16 @@ static int ba_run_impl(const vh_ba_plan* plan, float* poses, float* disps, float
     a.targets = targets; a.weights = weights; a.eta = eta; a.depth_active = depth_active;
     a.lm = lm; a.ep = ep; a.alpha = alpha; a.ilm = intrinsics_lm; a.iep = intrinsics_ep;
     a.iscale = intrinsics_scale; a.dx = dx; a.dz = dz; a.energy = energy; a.compute_energy = cen ? 1 : 0;
+    if (camera >= 0) {
+        if (camera != VH_CAM_PINHOLE && camera != VH_CAM_MEI)
+            return fail(VH_ERR_ARG, "ba_run_rig: unknown camera model %d", camera);
+        if (!rig) return fail(VH_ERR_ARG, "ba_run_rig: null rig");
+        a.rig = rig;
+        a.kstride = camera == VH_CAM_MEI ? 5 : 4;
+        if (camera == VH_CAM_MEI)
+            return ba_iterations<6, 2>(d, a, w, iterations, motion, false, rk, poses, intrinsics, dx, 1.f, s);
+        return ba_iterations<6, 1>(d, a, w, iterations, motion, false, rk, poses, intrinsics, dx, 1.f, s);
+    }
     if (sim3) return ba_iterations<7>(d, a, w, iterations, motion, intr, rk, poses, intrinsics, dx, intrinsics_scale, s);
     return ba_iterations<6>(d, a, w, iterations, motion, intr, rk, poses, intrinsics, dx, intrinsics_scale, s);
 }
@@ -3007,6 +3487,24 @@ int vh_ba_run_robust(const vh_ba_plan* plan, float* poses, float* disps, float* 
                        RobustK{robust_type, threshold, gnc_mu, gnc_mu_max}, dx, dz, energy, workspace, stream);
 }
 
+int vh_ba_run_rig(const vh_ba_plan* plan, float* poses, float* disps, const float* intrinsics, int camera,
+                  const float* rig, const float* disps_sens, const float* targets, const float* weights,
+                  const float* eta, const float* depth_active, int iterations, float lm, float ep, float alpha,
+                  int flags, int robust_type, float threshold, float gnc_mu, float gnc_mu_max, float* dx, float* dz,
+                  float* energy, void* workspace, void* stream) {
+    if (camera < 0) return fail(VH_ERR_ARG, "ba_run_rig: unknown camera model %d", camera);
+    if (robust_type < VH_ROBUST_NONE || robust_type > VH_ROBUST_GNC_TLS)
+        return fail(VH_ERR_ARG, "ba_run_rig: unknown robust kernel %d", robust_type);
+    if (robust_type != VH_ROBUST_NONE && !(threshold > 0.f))
+        return fail(VH_ERR_ARG, "ba_run_rig: threshold must be positive");
+    if (robust_type == VH_ROBUST_GNC_TLS && !(gnc_mu > 0.f))
+        return fail(VH_ERR_ARG, "ba_run_rig: GNC mu must be positive");
+    return ba_run_impl(plan, poses, disps, const_cast<float*>(intrinsics), disps_sens, targets, weights, eta,
+                       depth_active, iterations, lm, ep, alpha, 0.f, 0.f, 1.f, flags,
+                       RobustK{robust_type, threshold, gnc_mu, gnc_mu_max}, dx, dz, energy, workspace, stream, camera,
+                       rig);
+}
+
 int vh_robust_weights(int robust_type, float threshold, float gnc_mu, float gnc_mu_max, const float* x, float* w,
                       int64_t n, void* stream) {
     if (robust_type < VH_ROBUST_NONE || robust_type > VH_ROBUST_GNC_TLS)
@@ -3028,6 +3526,7 @@ int vh_ba_adjoint(const vh_ba_plan* plan, const float* poses, const float* disps
                   const float* grad_dx, const float* grad_dz, float* grad_targets, float* grad_weights,
                   void* workspace, void* stream) {
     if (!plan) return fail(VH_ERR_ARG, "ba_adjoint: null plan");
+    if (plan->d.V > 0) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: rig plans (single-view plans only)");
     const bool motion = (flags & VH_BA_MOTION_ONLY) != 0;
     if (flags & VH_BA_OPTIMIZE_INTRINSICS) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: intrinsics optimisation");
     if (flags & VH_BA_SIM3) return fail(VH_ERR_UNSUPPORTED, "ba_adjoint: Sim3 poses (SE3 only)");

@@ -226,6 +226,109 @@ def ba_adjoint(plan, ws, poses, disps, intrinsics, targets, weights, depth_activ
     return [gt, gw]
 
 
+class _RigPlan(_Plan):
+    def __init__(self, pi, pj, qi, qj, N, V, H, W, t0, t1, pixel_tile=0):
+        lib = _lib.load()
+        idx = [x.to(torch.int64).contiguous() for x in (pi, pj, qi, qj)]
+        h = lib.vh_ba_plan_create_rig(*[C.c_void_p(x.data_ptr()) for x in idx], int(idx[0].numel()), int(N), int(V),
+                                      int(H), int(W), int(t0), int(t1), int(pixel_tile))
+        if not h:
+            raise RuntimeError(f"slam_ext.ba_rig: {_lib.last_error()}")
+        self.h = C.c_void_p(h)
+        self.K = lib.vh_ba_plan_num_depth_frames(self.h)
+        self.ws_bytes = int(lib.vh_ba_workspace_size(self.h))
+        self.pixel_tile = int(lib.vh_ba_plan_pixel_tile(self.h))
+        kx = torch.empty(self.K, dtype=torch.int64)
+        lib.vh_ba_plan_depth_frames(self.h, C.c_void_p(kx.data_ptr()))
+        self.kx = kx
+
+
+def get_rig_plan(pi, pj, qi, qj, N, V, H, W, t0, t1, pixel_tile=0) -> _RigPlan:
+    """The (cached) plan of a rig edge list: edge e from view qi[e] of pose
+    pi[e] to view qj[e] of pose pj[e]; ``plan.kx`` lists the depth frames
+    (p V + q) whose maps ``eta`` must hold, in that order."""
+    idx = [x.detach().to("cpu", torch.int64) for x in (pi, pj, qi, qj)]
+    key = ("rig",) + tuple(x.numpy().tobytes() for x in idx) + (int(N), int(V), int(H), int(W), int(t0), int(t1),
+                                                              int(pixel_tile))
+    plan = _PLANS.get(key)
+    if plan is None:
+        plan = _RigPlan(*idx, N, V, H, W, t0, t1, pixel_tile)
+        _PLANS[key] = plan
+        while len(_PLANS) > _PLAN_CACHE_SIZE:
+            _PLANS.popitem(last=False)
+    else:
+        _PLANS.move_to_end(key)
+    return plan
+
+
+CAMERAS = {"pinhole": 0, "mei": 1}
+
+
+def ba_rig(poses, disps, intrinsics, camera, rig, disps_sens, targets, weights, eta, pi, pj, qi, qj, depth_active,
+           t0, t1, iterations, lm, ep, motion_only, alpha, compute_energy=False, robust=None):
+    """Fused BA on a multi-view rig buffer or a Mei camera: the layouts the
+    reference's fused path rejects (buffer.py:384-410).  This package's
+    extension, with the fused kernel's conventions.
+
+    poses [N,7] SE3 and disps [N*V,H,W] (the flattened [N,V,H,W] buffer) are
+    updated in place; intrinsics [V,4] (pinhole) or [V,5] (Mei, fx fy cx cy
+    k1) at the maps' resolution, fixed; ``camera`` "pinhole" or "mei"; rig
+    [V,7] camera-to-rig SE3 per view; disps_sens [N*V,H,W]; targets / weights
+    [E,2,H,W]; eta [|kx|,H,W] with kx = ``get_rig_plan(...).kx``;
+    depth_active [N*V]; edge e runs from view qi[e] of pose pi[e] to view
+    qj[e] of pose pj[e].  Returns [dx [t1-t0,6], dz [|kx|,H*W] or None,
+    energy]."""
+    if camera not in CAMERAS:
+        raise ValueError(f"slam_ext.ba_rig: camera {camera!r} (pinhole or mei)")
+    _check(targets=targets, weights=weights, poses=poses, disps=disps, intrinsics=intrinsics, rig=rig,
+           disps_sens=disps_sens, pi=pi, pj=pj, qi=qi, qj=qj, depth_active=depth_active)
+    for name, t in (("poses", poses), ("disps", disps), ("intrinsics", intrinsics), ("rig", rig),
+                    ("disps_sens", disps_sens), ("targets", targets), ("weights", weights),
+                    ("depth_active", depth_active)):
+        _f32(name, t)
+        if t.device.type != "cuda":
+            raise RuntimeError(f"slam_ext.ba_rig runs on the GPU; {name} is on {t.device}")
+    V = rig.shape[0]
+    NV, H, W = disps.shape
+    if rig.shape != (V, 7) or NV % V != 0:
+        raise RuntimeError(f"rig must be [V,7] and disps [N*V,H,W] (got {tuple(rig.shape)}, {tuple(disps.shape)})")
+    N = NV // V
+    kd = 5 if camera == "mei" else 4
+    if poses.shape != (N, 7):
+        raise RuntimeError(f"poses must be [N,7] SE3 (got {tuple(poses.shape)})")
+    if intrinsics.shape != (V, kd):
+        raise RuntimeError(f"intrinsics must be [V,{kd}] for the {camera} model (got {tuple(intrinsics.shape)})")
+    E = pi.shape[0]
+    if targets.shape != (E, 2, H, W) or weights.shape != (E, 2, H, W):
+        raise RuntimeError("targets/weights must be [E,2,H,W]")
+    if depth_active.shape != (NV,) or disps_sens.shape != disps.shape:
+        raise RuntimeError("depth_active must be [N*V] and disps_sens [N*V,H,W]")
+    t0, t1, iterations = int(t0), int(t1), int(iterations)
+    plan = get_rig_plan(pi, pj, qi, qj, N, V, H, W, t0, t1)
+    dev = poses.device
+    if not motion_only and eta.numel() != plan.K * H * W:
+        raise RuntimeError(f"eta must hold |kx| = {plan.K} maps of {H}x{W}")
+    eta_c = eta.contiguous().to(torch.float32)
+    energy = torch.empty((iterations if compute_energy else 0,), dtype=torch.float32, device=dev)
+    if iterations <= 0:
+        return [None, None, energy]
+    dx = torch.empty((t1 - t0, 6), dtype=torch.float32, device=dev)
+    dz = None if motion_only else torch.empty((plan.K, H * W), dtype=torch.float32, device=dev)
+    ws = torch.empty((max(plan.ws_bytes, 1),), dtype=torch.uint8, device=dev)
+    flags = (1 if motion_only else 0) | (4 if compute_energy else 0)
+    rk = (0, 0.0, 0.0, 0.0) if robust is None else tuple(robust)
+    with torch.cuda.device(dev):
+        rc = _lib.load().vh_ba_run_rig(plan.h, _lib.ptr(poses), _lib.ptr(disps), _lib.ptr(intrinsics),
+                                       CAMERAS[camera], _lib.ptr(rig), _lib.ptr(disps_sens), _lib.ptr(targets),
+                                       _lib.ptr(weights), None if motion_only else _lib.ptr(eta_c),
+                                       _lib.ptr(depth_active), iterations, float(lm), float(ep), float(alpha), flags,
+                                       int(rk[0]), float(rk[1]), float(rk[2]), float(rk[3]), _lib.ptr(dx),
+                                       _lib.ptr(dz), _lib.ptr(energy) if compute_energy else None, _lib.ptr(ws),
+                                       _lib.stream_of(poses))
+    _lib.check(rc, "slam_ext.ba_rig")
+    return [dx, dz, energy]
+
+
 def ba(poses, disps, intrinsics, disps_sens, targets, weights, eta, ii, jj, t0, t1, iterations, lm, ep,
        motion_only, alpha=0.05):
     """Legacy entry: depth_active = 1, no intrinsics, no energy (geom_kernels.cu:1822-1830)."""
