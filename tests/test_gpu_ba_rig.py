@@ -201,3 +201,18 @@ def test_rig_entry_point_errors():
     assert rc != 0 and "SE3" in _lib.last_error()
     with pytest.raises(ValueError, match="camera"):
         slam_ext.ba_rig(*[None] * 3, "fisheye", *[None] * 19)
+
+
+@pytest.mark.parametrize("camera", ["pinhole", "mei"])
+def test_synthetic_rig_graph_kf8_matches_oracle(camera):
+    """The rig version of the 8-keyframe graph (two views, 62 edges at 48 x 64)."""
+    from vipe_amd.synth import make_rig_graph
+
+    g = make_rig_graph("kf8_e24_48x64", camera=camera)
+    N, V = g.poses.shape[0], 2
+    ds = g.disps * 1.01
+    ds[::2] = 0.0
+    d = dict(poses=g.poses, disps=g.disps, intr=g.intrinsics, rig=g.rig, pi=g.pi, pj=g.pj, qi=g.qi, qj=g.qj,
+             targets=g.targets, weights=g.weights, disps_sens=ds, eta=g.eta, t0=g.t0, t1=g.t1, N=N, V=V)
+    mine, ref = _run(d, OR.MEI if camera == "mei" else OR.PINHOLE, 2)
+    _compare(mine, ref)

@@ -205,3 +205,112 @@ def test_fused_gpu_matches_generic_path(tracks, limited, intr, motion):
     torch.testing.assert_close(buf.poses[:5].cpu().double(), poses[:5], atol=2e-4, rtol=2e-4)
     torch.testing.assert_close(buf.disps[:5, 0].cpu().double(), disps[:5], atol=2e-4, rtol=2e-4)
     torch.testing.assert_close(buf.intrinsics[0, :4].cpu().double(), focal, atol=2e-4, rtol=2e-4)
+
+
+def make_rig_buffer(device, camera="mei", views=2, seed=7):
+    """A two-view buffer (Mei or pinhole, the second view 0.15 to the right),
+    6 frames of 4 x 5 maps, with ``ba.fused_rig`` enabled."""
+    g = torch.Generator().manual_seed(seed)
+    buf = make_buffer("cpu")
+    n, h8, w8 = 6, 4, 5
+    from oracle import lie as L
+
+    xi = torch.zeros(n, 6, dtype=torch.float64)
+    xi[:, :3] = torch.randn(n, 3, generator=g, dtype=torch.float64) * 0.03
+    xi[:, 3:] = torch.randn(n, 3, generator=g, dtype=torch.float64) * 0.02
+    buf.poses = L.expm(3, xi).float()
+    rig = [[0.0, 0, 0, 0, 0, 0, 1], [0.15, 0, 0, 0, 0.0499792, 0, 0.99875026]][:views]
+    buf.rig = torch.tensor(rig)
+    buf.n_views = views
+    buf.camera_type = camera
+    k = [38.0, 38.0, 19.5, 15.5] + ([0.6] if camera == "mei" else [])
+    buf.intrinsics = torch.tensor([k, [37.0, 38.5, 20.0, 15.0] + ([0.4] if camera == "mei" else [])][:views])
+    buf.disps = (0.5 + 0.5 * torch.rand(n, views, h8, w8, generator=g)).contiguous()
+    buf.disps_sens = buf.disps * 1.01
+    buf.disps_sens[::2] = 0.0
+    buf.cross_view_idx = None
+    buf.ba_config["fused_rig"] = True
+    for k_ in ("poses", "disps", "disps_sens", "intrinsics", "rig", "tstamp"):
+        setattr(buf, k_, getattr(buf, k_).to(device))
+    return buf
+
+
+def rig_inputs(buf, seed=8):
+    """Edges 1->0, 2->1, 3->2, 4->3, 0->2 and the stereo edges 2->2, 3->3,
+    expanded per view; targets from the FP64 projection + noise."""
+    from oracle import ba_rig as OR
+
+    g = torch.Generator().manual_seed(seed)
+    ii = torch.tensor([1, 2, 3, 4, 0, 2, 3])
+    jj = torch.tensor([0, 1, 2, 3, 2, 2, 3])
+    cpu = SimpleNamespace(n_views=buf.n_views, cross_view_idx=None)
+    pi, qi, di, pj, qj, dj = graph_ba.expand_edge_multiview(cpu, ii, jj)
+    cam = OR.MEI if buf.camera_type == "mei" else OR.PINHOLE
+    intr = buf.intrinsics.cpu().clone()
+    intr[:, :4] /= 8.0
+    V, (n, _, h8, w8) = buf.n_views, buf.disps.shape
+    M = pi.shape[0]
+    T = OR.edge_terms(buf.poses.cpu(), buf.disps.cpu().reshape(n * V, h8, w8), intr, cam, buf.rig.cpu(),
+                      torch.zeros(M, 2, h8, w8), torch.ones(M, 2, h8, w8), pi, pj, qi, qj, V)
+    target = (torch.stack([T["u"], T["v"]], -1) + 0.2 * torch.randn(M, h8 * w8, 2, generator=g,
+                                                                   dtype=torch.float64)).float()
+    weight = torch.rand(M, h8 * w8, 2, generator=g)
+    damping = 0.05 + 0.01 * torch.rand(n * V, h8, w8, generator=g)
+    dev = buf.poses.device
+    return target.to(dev), weight.to(dev), damping.to(dev), ii.to(dev), jj.to(dev)
+
+
+def test_multiview_expansion_and_rig_errors():
+    buf = SimpleNamespace(n_views=3, cross_view_idx=None)
+    pi, qi, di, pj, qj, dj = graph_ba.expand_edge_multiview(buf, torch.tensor([2, 4]), torch.tensor([1, 4]))
+    assert pi.tolist() == [2, 2, 2, 4, 4, 4] and qi.tolist() == [0, 1, 2, 0, 1, 2]
+    assert pj.tolist() == [1, 1, 1, 4, 4, 4] and qj.tolist() == [0, 1, 2, 1, 2, 0]
+    assert di.tolist() == [6, 7, 8, 12, 13, 14] and dj.tolist() == [3, 4, 5, 13, 14, 12]
+    rb = make_rig_buffer("cpu")
+    inp = rig_inputs(rb)
+    with pytest.raises(RuntimeError, match="does not optimize intrinsics"):
+        _call(rb, inp, optimize_intrinsics=True)
+    rb.camera_type = "panorama"
+    with pytest.raises(RuntimeError, match="no projection model"):
+        _call(rb, inp)
+    rb.camera_type = "mei"
+    rb.ba_config["fused_rig"] = False
+    with pytest.raises(RuntimeError, match="single-view pinhole-camera"):
+        _call(rb, inp)
+    with pytest.raises(RuntimeError, match="expanded graph edges"):
+        rb.ba_config["fused_rig"] = True
+        t, w, d, ii, jj = inp
+        _call(rb, (t[:3], w[:3], d, ii, jj))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("camera,limited", [("mei", False), ("pinhole", True)])
+def test_rig_buffer_on_the_fused_path_matches_oracle(camera, limited):
+    """ba.fused_rig: a two-view buffer through bundle_adjustment against the
+    FP64 oracle given an independent preparation of the same step."""
+    from oracle import ba_rig as OR
+
+    buf = make_rig_buffer("cuda", camera)
+    inp = rig_inputs(buf)
+    ref = make_rig_buffer("cpu", camera)
+    _call(buf, inp, n_iters=2, limited_disp=limited)
+    target, weight, damping, ii, jj = [t.cpu() for t in inp]
+    pi, qi, _, pj, qj, _ = graph_ba.expand_edge_multiview(SimpleNamespace(n_views=2, cross_view_idx=None), ii, jj)
+    V, (n, _, h8, w8) = 2, ref.disps.shape
+    tg = target.view(-1, h8, w8, 2).permute(0, 3, 1, 2).contiguous()
+    wt = weight.view(-1, h8, w8, 2).permute(0, 3, 1, 2).contiguous()
+    kx, _ = OR.bookkeeping(pi, qi, V, 1, 5)
+    eta = 0.2 * damping[kx] + 1e-7
+    da = torch.ones(n * V)
+    if limited:
+        da.zero_()
+        da[V:5 * V] = 1.0
+    intr = ref.intrinsics.clone()
+    intr[:, :4] /= 8.0
+    disps = ref.disps.reshape(n * V, h8, w8).clone()
+    OR.ba_rig(ref.poses, disps, intr, OR.MEI if camera == "mei" else OR.PINHOLE, ref.rig, ref.disps_sens.reshape(
+        n * V, h8, w8), tg, wt, eta, pi, pj, qi, qj, da, 1, 5, 2, 1e-3, 0.1, False, 0.001)
+    disps = disps.clamp(min=0.001).reshape(n, V, h8, w8)
+    torch.testing.assert_close(buf.poses.cpu().double(), ref.poses.double(), atol=2e-5, rtol=1e-5)
+    torch.testing.assert_close(buf.disps.cpu().double(), disps.double(), atol=2e-4, rtol=2e-4)
+    assert (buf.poses.cpu() - make_rig_buffer("cpu", camera).poses).abs().max() > 1e-5

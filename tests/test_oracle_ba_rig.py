@@ -201,3 +201,25 @@ def test_back_substitution_skip_and_motion_only():
     dx3, dz3, en = OR.ba_rig(p3, d3, *args, True, 0.05, compute_energy=True)
     assert dz3 is None and torch.equal(d3, d["disps"]) and float(en[0]) > 0
     assert not torch.equal(p3, d["poses"])
+
+
+@pytest.mark.parametrize("camera", ["pinhole", "mei"])
+def test_synthetic_rig_graph_targets_are_the_projections(camera):
+    """vipe_amd.synth.make_rig_graph with zero noise: its targets are the
+    oracle's projections of the same points (its own float64 chain of rig,
+    pose and camera maps)."""
+    from vipe_amd.synth import make_rig_graph
+
+    g = make_rig_graph("kf8_e24_48x64", camera=camera, views=2, noise=0.0)
+    cam = OR.MEI if camera == "mei" else OR.PINHOLE
+    M = g.pi.shape[0]
+    H, W = g.disps.shape[1:]
+    T = OR.edge_terms(g.poses, g.disps, g.intrinsics, cam, g.rig, torch.zeros(M, 2, H, W), torch.ones(M, 2, H, W),
+                      g.pi, g.pj, g.qi, g.qj, 2)
+    ok = T["z"] > 0.3
+    assert float(ok.double().mean()) > 0.9
+    proj = torch.stack([T["u"], T["v"]], 1)
+    err = (g.targets.double().reshape(M, 2, H * W) - proj).abs().permute(0, 2, 1)[ok].max()
+    assert float(err) < 1e-3, float(err)
+    # every view of the window's frames, and the cross-view edges q -> q + 1
+    assert int(((g.pi == g.pj) & (g.qj == (g.qi + 1) % 2)).sum()) == 2 * (g.t1 - g.t0)

@@ -42,6 +42,12 @@
         random FP16 state and context, random parameters; one warm-up call,
         then `iters` calls.
 
+    python3 -m vipe_amd.profiling ba_rig [--camera mei|pinhole] [--views 2] [--iters 8]
+        The fused BA on the two-view rig version of the bench graph
+        (synth.make_rig_graph: 64 frames x 2 views, 894 edges at 64 x 80):
+        one warm-up call, then one call of `iters` Gauss-Newton iterations,
+        timed with HIP events.
+
     python3 -m vipe_amd.profiling summarize --stats DIR [--fetch DIR --write DIR
         --cfetch DIR --cwrite DIR --copy-mib 1024] [--kernel SUBSTR ...]
         Reads the CSV output of ``rocprofv3 --output-format csv`` runs and prints
@@ -257,6 +263,34 @@ def _update_op(edges: int, per_frame: int, iters: int) -> None:
           f"net {tuple(out[0].shape)}, upmask {tuple(out[4].shape)}", flush=True)
 
 
+def _ba_rig(camera: str, views: int, iters: int) -> None:
+    import torch
+
+    from . import synth
+    from .ext import slam_ext
+
+    g = synth.make_rig_graph("kf64_e384_64x80", camera=camera, views=views)
+    d = {k: (v.cuda() if torch.is_tensor(v) else v) for k, v in g.__dict__.items()}
+    act = torch.ones(d["disps"].shape[0], device="cuda")
+
+    def run(n):
+        p, dd = d["poses"].clone(), d["disps"].clone()
+        return slam_ext.ba_rig(p, dd, d["intrinsics"], camera, d["rig"], d["disps_sens"], d["targets"],
+                               d["weights"], d["eta"], d["pi"], d["pj"], d["qi"], d["qj"], act, g.t0, g.t1, n, 1e-4,
+                               0.1, False, 0.05, True)
+
+    run(1)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    _, _, en = run(iters)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e)
+    print(f"ba_rig {camera}: {views} views, {g.pi.shape[0]} edges, {iters} GN iterations in {ms:.3f} ms "
+          f"({ms / iters:.4f} ms per iteration); energy {en[0].item():.4g} -> {en[-1].item():.4g}", flush=True)
+
+
 def _short(name: str, width: int = 90) -> str:
     return name if len(name) <= width else name[: width - 3] + "..."
 
@@ -336,6 +370,10 @@ def main(argv=None) -> None:
     ci.add_argument("--dtype", choices=["f32", "f16"], default="f32")
     ci.add_argument("--edges", type=int, default=96)
     ci.add_argument("--iters", type=int, default=5)
+    br = sub.add_parser("ba_rig")
+    br.add_argument("--camera", choices=["mei", "pinhole"], default="mei")
+    br.add_argument("--views", type=int, default=2)
+    br.add_argument("--iters", type=int, default=8)
     uo = sub.add_parser("update_op")
     uo.add_argument("--edges", type=int, default=384)
     uo.add_argument("--per-frame", type=int, default=6)
@@ -366,6 +404,8 @@ def main(argv=None) -> None:
         _altcorr_layer(a.iters)
     elif a.cmd == "corr_index":
         _corr_index(a.dtype, a.edges, a.iters)
+    elif a.cmd == "ba_rig":
+        _ba_rig(a.camera, a.views, a.iters)
     elif a.cmd == "update_op":
         _update_op(a.edges, a.per_frame, a.iters)
     else:

@@ -225,3 +225,110 @@ def make_graph_from(cfg: dict, seed: int = 0, C: int = 128, with_fmaps: bool = T
         targets=targets.float().contiguous(), weights=weights.float().contiguous(),
         eta=eta.float().contiguous(), fmaps=fmaps, coords=coords.float().contiguous(), t0=t0, t1=t1,
     )
+
+
+@dataclass
+class RigGraph:
+    poses: torch.Tensor        # [N, 7] SE3
+    disps: torch.Tensor        # [N V, H, W] (the flattened [N, V, H, W] buffer)
+    disps_sens: torch.Tensor   # [N V, H, W]
+    intrinsics: torch.Tensor   # [V, 4] pinhole or [V, 5] Mei (fx fy cx cy k1)
+    rig: torch.Tensor          # [V, 7] camera-to-rig SE3 of each view
+    camera: str                # "pinhole" or "mei"
+    pi: torch.Tensor           # [M] edge source pose
+    pj: torch.Tensor           # [M] edge target pose
+    qi: torch.Tensor           # [M] edge source view
+    qj: torch.Tensor           # [M] edge target view
+    targets: torch.Tensor      # [M, 2, H, W]
+    weights: torch.Tensor      # [M, 2, H, W]
+    eta: torch.Tensor          # [|kx|, H, W]
+    t0: int
+    t1: int
+
+
+def _rig_project(cam, K, Xc):
+    """Pixel coordinates of camera-frame points Xc [..., 3] (float64): pinhole
+    fx X / Z + cx, or Mei fx X / (Z + k1 |X|) + cx."""
+    X, Y, Z = Xc.unbind(-1)
+    den = Z + (K[..., 4] * Xc.norm(dim=-1) if cam == "mei" else 0.0)
+    den = torch.where(den.abs() > 1e-9, den, torch.full_like(den, 1e-9))
+    return torch.stack([K[..., 0] * X / den + K[..., 2], K[..., 1] * Y / den + K[..., 3]], -1)
+
+
+def _rig_ray(cam, K, u, v):
+    """The unit-depth ray [X0, Y0, 1] of pixel (u, v) (float64)."""
+    ub, vb = (u - K[..., 2]) / K[..., 0], (v - K[..., 3]) / K[..., 1]
+    if cam == "mei":
+        k1 = K[..., 4]
+        r2 = ub * ub + vb * vb
+        q = torch.sqrt(1 + (1 - k1 * k1) * r2)
+        f = (k1 + q) / (1 + r2)
+        ub, vb = ub * f / (f - k1), vb * f / (f - k1)
+    return torch.stack([ub, vb, torch.ones_like(ub)], -1)
+
+
+def make_rig_graph(name: str = "kf64_e384_64x80", camera: str = "mei", views: int = 2, seed: int = 0,
+                   t0: int = 1, noise: float = 0.5) -> RigGraph:
+    """A rig version of the frame graph ``name``: the same poses and edges, seen
+    by ``views`` cameras (view q a 0.1 baseline step to the right of view q-1,
+    rotated 0.05 rad about y), every edge once per view (q -> q) and, for each
+    frame of the window, its cross-view edges q -> q + 1 mod V.  Mei
+    intrinsics keep the pinhole focal length at the centre (fx (1 + k1), k1
+    0.6 and 0.5).  Targets are the projections + N(0, noise px), weights U[0, 1]."""
+    cfg = CONFIGS[name]
+    g = make_graph_from(cfg, seed, with_fmaps=False, t0=t0)
+    gen = torch.Generator().manual_seed(seed + 1)
+    N, H, W = g.disps.shape
+    V = views
+    fx, fy, cx, cy = cfg["intr"]
+    Ks = []
+    for q in range(V):
+        k1 = 0.6 - 0.1 * (q % 2)
+        if camera == "mei":
+            Ks.append([fx * (1 + k1), fy * (1 + k1), cx, cy, k1])
+        else:
+            Ks.append([fx, fy, cx, cy])
+    K = torch.tensor(Ks, dtype=torch.float64)
+    rig = torch.zeros(V, 7, dtype=torch.float64)
+    for q in range(V):
+        rig[q, 0] = 0.1 * q
+        rig[q, 3:7] = _quat_exp(torch.tensor([0.0, 0.05 * q, 0.0], dtype=torch.float64))
+    keep = g.ii != g.jj
+    ii, jj = g.ii[keep], g.jj[keep]
+    pi = ii.repeat_interleave(V)
+    pj = jj.repeat_interleave(V)
+    qi = torch.arange(V).repeat(ii.shape[0])
+    qj = qi.clone()
+    if V > 1:
+        fr = torch.arange(t0, N)
+        pi = torch.cat([pi, fr.repeat_interleave(V)])
+        pj = torch.cat([pj, fr.repeat_interleave(V)])
+        qi = torch.cat([qi, torch.arange(V).repeat(fr.shape[0])])
+        qj = torch.cat([qj, (torch.arange(V) + 1).remainder(V).repeat(fr.shape[0])])
+    M = pi.shape[0]
+    disps = (0.2 + 0.8 * torch.rand(N * V, H, W, generator=gen, dtype=torch.float64))
+    P = g.poses.double()
+    vv, uu = torch.meshgrid(torch.arange(H, dtype=torch.float64), torch.arange(W, dtype=torch.float64), indexing="ij")
+    targets = torch.empty(M, 2, H, W, dtype=torch.float64)
+    for e0 in range(0, M, 256):
+        sl = slice(e0, min(M, e0 + 256))
+        a, b, c, d = pi[sl], pj[sl], qi[sl], qj[sl]
+        ray = _rig_ray(camera, K[c][:, None, None, :], uu, vv)                    # [m,H,W,3]
+        X = _qrot(rig[c, 3:7][:, None, None, :], ray) + rig[c, :3][:, None, None, :] * disps[a * V + c][..., None]
+        # rig -> world -> rig of frame b -> camera d (poses are world-to-rig)
+        qa_inv = torch.cat([-P[a, 3:6], P[a, 6:7]], -1)[:, None, None, :]
+        X = _qrot(qa_inv, X - P[a, :3][:, None, None, :] * disps[a * V + c][..., None])
+        X = _qrot(P[b, 3:7][:, None, None, :], X) + P[b, :3][:, None, None, :] * disps[a * V + c][..., None]
+        qd_inv = torch.cat([-rig[d, 3:6], rig[d, 6:7]], -1)[:, None, None, :]
+        X = _qrot(qd_inv, X - rig[d, :3][:, None, None, :] * disps[a * V + c][..., None])
+        uv = _rig_project(camera, K[d][:, None, None, :], X)
+        targets[sl] = uv.permute(0, 3, 1, 2)
+    targets += torch.randn(M, 2, H, W, generator=gen, dtype=torch.float64) * noise
+    weights = torch.rand(M, 2, H, W, generator=gen)
+    kx = torch.unique(torch.cat([torch.arange(t0 * V, N * V), pi * V + qi]))
+    eta = 0.2 * (0.05 + 0.1 * torch.rand(kx.shape[0], H, W, generator=gen, dtype=torch.float64)) + 1e-7
+    return RigGraph(poses=g.poses.contiguous(), disps=disps.float().contiguous(),
+                    disps_sens=torch.zeros(N * V, H, W), intrinsics=K.float().contiguous(),
+                    rig=rig.float().contiguous(), camera=camera, pi=pi, pj=pj, qi=qi, qj=qj,
+                    targets=targets.float().contiguous(), weights=weights.contiguous(), eta=eta.float().contiguous(),
+                    t0=t0, t1=N)
