@@ -221,19 +221,6 @@ __global__ __launch_bounds__(256) void spatial_corr_to_cl_kernel(const T* __rest
     }
 }
 
-// Workgroups are dispatched to the 8 XCDs in turn (workgroup b on XCD b % 8),
-// and each XCD has its own L2.  Renumbering so that XCD x runs the x-th
-// contiguous range of logical blocks keeps the rows a range shares (one image's
-// in1 / in2 rows at the local-attention shapes) in one L2.
-constexpr int kXcds = 8;
-
-__device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
-    const int64_t xcd = b % kXcds, local = b / kXcds;
-    const int64_t per = (nb + kXcds - 1) / kXcds, r = nb % kXcds;
-    if (r == 0 || xcd < r) return xcd * per + local;
-    return r * per + (xcd - r) * (per - 1) + local;
-}
-
 // grid: ceil(oW/64) * oH * patchH * N workgroups in one dimension, 4 waves;
 // logical block (n, h, ph, wx), wx fastest; wave v covers w0 = 64 wx + 16 v.
 template <typename T, int NT>

@@ -7,16 +7,18 @@
 // GEMM: M = pixels, N = output channels, K = taps x input channels.
 //
 //   uo_conv_kernel        the implicit GEMM on v_mfma_f32_32x32x16_f16.  One
-//                         workgroup per 128 pixels x 128 output channels; each
-//                         of its 4 waves owns 64 x 64 as 2 x 2 tiles of 32 x 32.
-//                         K advances one tap of one 32-channel chunk of one input
-//                         segment at a time: the chunk of the 128 pixels (shifted
-//                         by the tap, zero outside the image) and of the 128
-//                         weight rows go to LDS, the next chunk's loads are in
-//                         flight meanwhile.  Up to four input segments stand in
-//                         for torch.cat along channels.  The epilogue adds the
-//                         bias (and a per-image addend) and applies the layer's
-//                         activation, or the ConvGRU gates / state update.
+//                         workgroup of 8 waves per 256 pixels x 128 output
+//                         channels (128 x 256 where the layer has 256); each
+//                         wave owns 64 x 64 as 2 x 2 tiles of 32 x 32.  K
+//                         advances one tap of one 32-channel chunk of one input
+//                         segment per step: the step's pixel rows (shifted by the
+//                         tap, zero outside the image) and weight rows are copied
+//                         from memory straight into a three-stage LDS ring, two
+//                         steps ahead of the matrix cores.  Up to four input
+//                         segments stand in for torch.cat along channels.  The
+//                         epilogue adds the bias (and a per-image addend) and
+//                         applies the layer's activation, or the ConvGRU gates /
+//                         state update.
 //   uo_conv_small_kernel  convolutions with 1 or 2 output channels (the delta,
 //                         weight and eta heads) on the VALU: lanes along the
 //                         input channels, a cross-lane sum per batch of pixels.
@@ -38,10 +40,8 @@ namespace {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kUoBM = 128;  // pixels per workgroup
-constexpr int kUoBN = 128;  // output channels per workgroup
-constexpr int kUoKC = 32;   // channels per K chunk
-constexpr int kUoRS = kUoKC + 8;  // LDS row stride (halves): 80 bytes, 16-byte aligned rows
+constexpr int kUoKC = 32;         // channels per K step
+constexpr int kUoCoutAlign = 128;  // the weights' output channels are padded to a multiple of this
 
 struct ConvArgs {
     const _Float16* seg[4];  // NHWC inputs
@@ -66,142 +66,48 @@ struct ConvArgs {
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + __expf(-x)); }
 __device__ __forceinline__ float softplusf_(float x) { return x > 20.f ? x : log1pf(__expf(x)); }
 
+// LDS ring of the implicit GEMM.  A stage holds one K step: the tile's BM
+// pixel rows and BN weight rows of 64 bytes (32 channels), unpadded; piece q
+// (16 bytes) of row r sits in slot q ^ ((r >> 2) & 3), which keeps the MFMA
+// fragment reads (32 consecutive rows, one piece) on distinct banks.  The
+// stages are filled by global_load_lds_dwordx4 (lane l's 16 bytes land at the
+// instruction's LDS base + 16 l), so no registers stage the operands and the
+// loads of the next kUoStages - 1 steps are in flight while a step computes.
+constexpr int kUoStages = 3;
+__device__ __align__(16) uint4 uo_zero16[1];  // zero-initialised source of taps outside the image
+
+__device__ __forceinline__ void lds_load16(const void* g, unsigned lds_byte_addr) {
+    __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)(uintptr_t)lds_byte_addr, 16, 0, 0);
+}
+
+// s_waitcnt immediate that waits until at most n vector memory operations are
+// outstanding (vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left at their
+// maxima, i.e. not waited for)
+constexpr int uo_vmcnt(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+
+// Bias, per-image addend, activation or ConvGRU update, and the store of one
+// wave's 64 x 64 block at pixel m0 and channel n0.  Lane l holds output channel
+// n0 + 32 j + l % 32 of pixels m0 + 32 i + 8 (v / 4) + 4 (l / 32) + v % 4.
 template <int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void uo_conv_kernel(ConvArgs a) {
-    __shared__ __align__(16) _Float16 As[kUoBM * kUoRS];
-    __shared__ __align__(16) _Float16 Bs[kUoBN * kUoRS];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
-    const int HW = a.H * a.W;
-    const int P = a.nimg * HW;  // < 2^31, checked by the host
-    const int m0 = blockIdx.x * kUoBM;
-    const int n0 = blockIdx.y * kUoBN;
-    const int ph = a.KH / 2, pw = a.KW / 2;
-    // this thread's two staged chunks (pixel or weight row r, 8 channels at 8 q)
-    int r_[2], q_[2], img_[2], py_[2], px_[2];
-    bool pok_[2];
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-        const int i = tid + 256 * u;
-        r_[u] = i >> 2;
-        q_[u] = i & 3;
-        const int p = m0 + r_[u];
-        pok_[u] = p < P;
-        const int pc = pok_[u] ? p : 0;
-        img_[u] = pc / HW;
-        const int rem = pc - img_[u] * HW;
-        py_[u] = rem / a.W;
-        px_[u] = rem - py_[u] * a.W;
-    }
-    // K walks (segment, 32-channel chunk, tap) in that nesting: the taps of one
-    // chunk read the same input lines (shifted by at most a row and a column),
-    // so they come from the caches instead of HBM.  The cursor (uniform) holds
-    // the chunk whose loads are issued next.  The segment is
-    // picked by integer sums over the segment table, not by selecting among
-    // its entries: selects of kernel-argument loads become a pointer table in
-    // scratch memory.
-    const uint64_t sb0 = (uint64_t)a.seg[0];
-    const uint64_t sd1 = (uint64_t)a.seg[1] - sb0, sd2 = (uint64_t)a.seg[2] - (uint64_t)a.seg[1],
-                   sd3 = (uint64_t)a.seg[3] - (uint64_t)a.seg[2];
-    const int sc0 = a.seg_c[0], sc1 = a.seg_c[1], sc2 = a.seg_c[2], sc3 = a.seg_c[3];
-    const int ss0 = a.seg_stride[0], ss1 = a.seg_stride[1] - ss0, ss2 = a.seg_stride[2] - a.seg_stride[1],
-              ss3 = a.seg_stride[3] - a.seg_stride[2];
-    const int so0 = a.seg_off[0], so1 = a.seg_off[1] - so0, so2 = a.seg_off[2] - a.seg_off[1],
-              so3 = a.seg_off[3] - a.seg_off[2];
-    const int nseg = a.nseg, KW = a.KW, H = a.H, W = a.W, Cout_pad = a.Cout_pad, cin = a.cin;
-    const _Float16* wp = a.w;
-    const int per_tap = (sc0 + (nseg > 1 ? sc1 : 0) + (nseg > 2 ? sc2 : 0) + (nseg > 3 ? sc3 : 0)) / kUoKC;
-    const int ntap = a.KH * KW;
-    const int nchunks = ntap * per_tap;
-    int cur_tap = 0, cur_sg = 0, cur_c0 = 0, cur_wc0 = 0;
-    uint64_t cur_sp = sb0;
-    int cur_sc = sc0, cur_ss = ss0, cur_so = so0;
-    uint4 va0, va1, vb0, vb1;
-#define UO_LOAD(u, va, vb)                                                                                          \
-    {                                                                                                               \
-        const int yy = py_[u] + dy, xx = px_[u] + dx;                                                               \
-        const bool ok = pok_[u] && yy >= 0 && yy < H && xx >= 0 && xx < W;                                         \
-        const int pix = img_[u] * HW + (ok ? yy * W + xx : 0);                                                      \
-        const uint4 v = *reinterpret_cast<const uint4*>((const _Float16*)cur_sp + (int64_t)pix * cur_ss + cur_so +  \
-                                                        cur_c0 + 8 * q_[u]);                                        \
-        va = ok ? v : make_uint4(0, 0, 0, 0);                                                                       \
-        vb = *reinterpret_cast<const uint4*>(wp + ((int64_t)cur_tap * Cout_pad + n0 + r_[u]) * cin + cur_wc0 +     \
-                                             8 * q_[u]);                                                            \
-    }
-    f32x16 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
-    {
-        const int dy = -ph, dx = -pw;
-        UO_LOAD(0, va0, vb0)
-        UO_LOAD(1, va1, vb1)
-    }
-    for (int kc = 0; kc < nchunks; kc++) {
-        __syncthreads();  // the previous chunk's LDS reads are done
-        *reinterpret_cast<uint4*>(As + r_[0] * kUoRS + 8 * q_[0]) = va0;
-        *reinterpret_cast<uint4*>(Bs + r_[0] * kUoRS + 8 * q_[0]) = vb0;
-        *reinterpret_cast<uint4*>(As + r_[1] * kUoRS + 8 * q_[1]) = va1;
-        *reinterpret_cast<uint4*>(Bs + r_[1] * kUoRS + 8 * q_[1]) = vb1;
-        // the next chunk's loads (the last chunk reloads itself, so every path
-        // issues them and the MFMAs below wait for nothing)
-        if (kc + 1 < nchunks && ++cur_tap == ntap) {
-            cur_tap = 0;
-            cur_c0 += kUoKC;
-            cur_wc0 += kUoKC;
-            if (cur_c0 == cur_sc) {
-                cur_c0 = 0;
-                ++cur_sg;  // below nseg: the chunk after the last tap of the last segment's last chunk is never loaded
-                const int sg = cur_sg;
-                cur_sp = sb0 + (sg >= 1 ? sd1 : 0) + (sg >= 2 ? sd2 : 0) + (sg >= 3 ? sd3 : 0);
-                cur_sc = sg == 0 ? sc0 : sg == 1 ? sc1 : sg == 2 ? sc2 : sc3;
-                cur_ss = ss0 + (sg >= 1 ? ss1 : 0) + (sg >= 2 ? ss2 : 0) + (sg >= 3 ? ss3 : 0);
-                cur_so = so0 + (sg >= 1 ? so1 : 0) + (sg >= 2 ? so2 : 0) + (sg >= 3 ? so3 : 0);
-            }
-        }
-        {
-            const int dy = cur_tap / KW - ph, dx = cur_tap % KW - pw;
-            UO_LOAD(0, va0, vb0)
-            UO_LOAD(1, va1, vb1)
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kUoKC; kk += 16) {
-            const int k = kk + 8 * (lane >> 5);
-            const f16x8 a0 = *reinterpret_cast<const f16x8*>(As + (wm + (lane & 31)) * kUoRS + k);
-            const f16x8 a1 = *reinterpret_cast<const f16x8*>(As + (wm + 32 + (lane & 31)) * kUoRS + k);
-            const f16x8 b0 = *reinterpret_cast<const f16x8*>(Bs + (wn + (lane & 31)) * kUoRS + k);
-            const f16x8 b1 = *reinterpret_cast<const f16x8*>(Bs + (wn + 32 + (lane & 31)) * kUoRS + k);
-            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, acc[1][1], 0, 0, 0);
-        }
-    }
-#undef UO_LOAD
-    // epilogue: lane holds output channel n = wn + 32 j + lane % 32 of pixels
-    // m = wm + 32 i + 8 (v / 4) + 4 (lane / 32) + v % 4
+__device__ __forceinline__ void uo_epilogue(const ConvArgs& a, const f32x16 (&acc)[2][2], int m0, int n0, int lane,
+                                            int P, int HW) {
     const bool nchw = (a.epi & VH_UO_NCHW) != 0;
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const int n = n0 + wn + 32 * j + (lane & 31);
+        const int n = n0 + 32 * j + (lane & 31);
         if (n >= a.Cout) continue;
         const float bn = a.bias[n];
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int v = 0; v < 16; v++) {
-                const int p = m0 + wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int p = m0 + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 if (p >= P) continue;
                 float x = acc[i][j][v] + bn;
                 int img = 0;
                 if (a.glo || nchw) img = (unsigned)p / (unsigned)HW;
                 if (a.glo) x += a.glo[img * a.glo_stride + n];
                 if constexpr (EPI == VH_UO_GATES) {
-                    // ConvGRU: channels [0, 128) are z, [128, 256) are r (r * net goes to out2)
                     const float g = sigmoidf_(x);
                     if (n < 128) {
                         ((_Float16*)a.out)[(int64_t)p * a.out_stride + a.out_off + n] = (_Float16)g;
@@ -212,10 +118,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     continue;
                 }
                 if constexpr (EPI == VH_UO_GRU) {
-                    const float q = tanhf(x);
+                    const float qv = tanhf(x);
                     const float z = (float)a.aux0[(int64_t)p * a.aux_stride + n];
                     const float h = (float)a.aux1[(int64_t)p * a.aux_stride + n];
-                    x = (1.f - z) * h + z * q;
+                    x = (1.f - z) * h + z * qv;
                 } else if constexpr (EPI == VH_UO_RELU) {
                     x = fmaxf(x, 0.f);
                 } else if constexpr (EPI == VH_UO_SIGMOID) {
@@ -229,6 +135,171 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 else ((_Float16*)a.out)[o] = (_Float16)x;
             }
     }
+}
+
+// One K step's loads into the LDS stage at base: the wave's LA pixel-row
+// pieces (the tap's shifted pixels of channels sp[0, 32) at channel stride ss,
+// the zero page outside the image) and its LB weight-row pieces of wt.
+template <int LA, int LB, int BM>
+__device__ __forceinline__ void uo_issue_step(unsigned base, int wv, const _Float16* sp, int ss, int dy, int dx, int H,
+                                              int W, int HW, const int (&qa)[LA], const int (&img)[LA],
+                                              const int (&py)[LA], const int (&px)[LA], const bool (&pok)[LA],
+                                              const _Float16* zp, const _Float16* wt, int cin, const int (&rb)[LB],
+                                              const int (&qb)[LB]) {
+#pragma unroll
+    for (int u = 0; u < LA; u++) {
+        const int yy = py[u] + dy, xx = px[u] + dx;
+        const bool ok = pok[u] && yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const _Float16* g = sp + (int64_t)(img[u] * HW + yy * W + xx) * ss + 8 * qa[u];
+        lds_load16(ok ? g : zp, base + (wv * LA + u) * 1024);
+    }
+#pragma unroll
+    for (int u = 0; u < LB; u++) lds_load16(wt + (int64_t)rb[u] * cin + 8 * qb[u], base + BM * 64 + (wv * LB + u) * 1024);
+}
+
+// The implicit GEMM.  A workgroup of WM x WN waves computes BM = 64 WM pixels
+// x BN = 64 WN output channels; each wave owns 64 x 64 as 2 x 2 tiles of
+// 32 x 32.  Per K step each wave issues LA pixel-row and LB weight-row loads
+// of 16 rows each.
+template <int EPI, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(WM * WN == 8 ? 4 : 3))) void
+uo_conv_kernel(ConvArgs a) {
+    constexpr int NW = WM * WN, BM = 64 * WM, BN = 64 * WN;
+    constexpr int LA = BM / (16 * NW), LB = BN / (16 * NW);
+    constexpr int kStage = (BM + BN) * 64;
+    static_assert(LA >= 1 && LB >= 1 && LA * 16 * NW == BM && LB * 16 * NW == BN, "tile");
+    static_assert(kUoStages == 3, "the waits below count one younger step");
+    extern __shared__ __align__(16) unsigned char lds_[];
+    const unsigned lds0 = (unsigned)(uintptr_t)lds_;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = (wv / WN) * 64, wn = (wv % WN) * 64;
+    const int HW = a.H * a.W;
+    const int P = a.nimg * HW;  // < 2^31, checked by the host
+    // tiles in contiguous ranges per XCD, pixel tiles fastest
+    const int lb = (int)xcd_logical((int64_t)blockIdx.y * gridDim.x + blockIdx.x, (int64_t)gridDim.x * gridDim.y);
+    const int by = lb / (int)gridDim.x, bx = lb - by * (int)gridDim.x;
+    const int m0 = bx * BM;
+    const int n0 = by * BN;
+    const int ph = a.KH / 2, pw = a.KW / 2;
+    // this lane's LDS slots: load u of the wave fills slots s = (wv L + u) 64 + lane,
+    // row s / 4, slot s % 4, with piece (s % 4) ^ ((row >> 2) & 3) of the row
+    int qa[LA], img[LA], py[LA], px[LA], rb[LB], qb[LB];
+    bool pok[LA];
+#pragma unroll
+    for (int u = 0; u < LA; u++) {
+        const int s = (wv * LA + u) * 64 + lane, r = s >> 2;
+        qa[u] = (s & 3) ^ ((r >> 2) & 3);
+        pok[u] = m0 + r < P;
+        const int pc = pok[u] ? m0 + r : 0;
+        img[u] = pc / HW;
+        py[u] = (pc - img[u] * HW) / a.W;
+        px[u] = pc - img[u] * HW - py[u] * a.W;
+    }
+#pragma unroll
+    for (int u = 0; u < LB; u++) {
+        const int s = (wv * LB + u) * 64 + lane;
+        rb[u] = s >> 2;
+        qb[u] = (s & 3) ^ ((rb[u] >> 2) & 3);
+    }
+    // K walks (segment, 32-channel chunk, tap) in that nesting: the taps of one
+    // chunk read the same input lines (shifted by at most a row and a column),
+    // so they come from the caches instead of HBM.  The segment is picked by
+    // integer sums over the segment table, not by selecting among its entries:
+    // selects of kernel-argument loads become a pointer table in scratch memory.
+    const uint64_t sb0 = (uint64_t)a.seg[0];
+    const uint64_t sd1 = (uint64_t)a.seg[1] - sb0, sd2 = (uint64_t)a.seg[2] - (uint64_t)a.seg[1],
+                   sd3 = (uint64_t)a.seg[3] - (uint64_t)a.seg[2];
+    const int sc0 = a.seg_c[0], sc1 = a.seg_c[1], sc2 = a.seg_c[2], sc3 = a.seg_c[3];
+    const int ss0 = a.seg_stride[0], ss1 = a.seg_stride[1] - ss0, ss2 = a.seg_stride[2] - a.seg_stride[1],
+              ss3 = a.seg_stride[3] - a.seg_stride[2];
+    const int so0 = a.seg_off[0], so1 = a.seg_off[1] - so0, so2 = a.seg_off[2] - a.seg_off[1],
+              so3 = a.seg_off[3] - a.seg_off[2];
+    const int nseg = a.nseg, H = a.H, W = a.W, Cout_pad = a.Cout_pad, cin = a.cin;
+    const _Float16* wp = a.w;
+    const int per_tap = (sc0 + (nseg > 1 ? sc1 : 0) + (nseg > 2 ? sc2 : 0) + (nseg > 3 ? sc3 : 0)) / kUoKC;
+    const int ntap = a.KH * a.KW;
+    const int nchunks = ntap * per_tap;
+    const _Float16* zp = reinterpret_cast<const _Float16*>(uo_zero16);
+    // the issue cursor (uniform): segment, chunk and tap of the next step to load,
+    // the tap also as its offsets (dy, dx)
+    int cur_tap = 0, cur_dy = -ph, cur_dx = -pw, cur_sg = 0, cur_c0 = 0, cur_wc0 = 0;
+    uint64_t cur_sp = sb0;
+    int cur_ss = ss0, cur_so = so0, cur_sc = sc0;
+    // the cursor's step into stage ST, then the cursor's advance
+#define UO_ISSUE(ST)                                                                                             \
+    {                                                                                                            \
+        const unsigned base = lds0 + (ST) * kStage;                                                              \
+        uo_issue_step<LA, LB, BM>(base, wv, (const _Float16*)cur_sp + cur_so + cur_c0, cur_ss, cur_dy, cur_dx, H, \
+                                  W, HW, qa, img, py, px, pok, zp,                                               \
+                                  wp + ((int64_t)cur_tap * Cout_pad + n0) * cin + cur_wc0, cin, rb, qb);          \
+        ++cur_tap;                                                                                               \
+        if (++cur_dx > pw) {                                                                                     \
+            cur_dx = -pw;                                                                                        \
+            ++cur_dy;                                                                                            \
+        }                                                                                                        \
+        if (cur_tap == ntap) {                                                                                   \
+            cur_tap = 0;                                                                                         \
+            cur_dy = -ph;                                                                                        \
+            cur_c0 += kUoKC;                                                                                     \
+            cur_wc0 += kUoKC;                                                                                    \
+            if (cur_c0 == cur_sc && cur_sg + 1 < nseg) {                                                         \
+                cur_c0 = 0;                                                                                      \
+                const int sg = ++cur_sg;                                                                         \
+                cur_sp = sb0 + (sg >= 1 ? sd1 : 0) + (sg >= 2 ? sd2 : 0) + (sg >= 3 ? sd3 : 0);                  \
+                cur_sc = sg == 1 ? sc1 : sg == 2 ? sc2 : sc3;                                                    \
+                cur_ss = ss0 + (sg >= 1 ? ss1 : 0) + (sg >= 2 ? ss2 : 0) + (sg >= 3 ? ss3 : 0);                  \
+                cur_so = so0 + (sg >= 1 ? so1 : 0) + (sg >= 2 ? so2 : 0) + (sg >= 3 ? so3 : 0);                  \
+            }                                                                                                    \
+        }                                                                                                        \
+    }
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
+    // prologue: steps 0 .. kUoStages - 2 in flight
+#pragma unroll
+    for (int s = 0; s < kUoStages - 1; s++)
+        if (s < nchunks) UO_ISSUE(s)
+    const int ra0 = wm + (lane & 31), rb0 = wn + (lane & 31);  // fragment rows (+ 32 i)
+    for (int kc = 0; kc < nchunks; kc++) {
+        // step kc's loads are older than the LA + LB of step kc + 1, if issued
+        if (kc + 1 < nchunks) __builtin_amdgcn_s_waitcnt(uo_vmcnt(LA + LB));
+        else __builtin_amdgcn_s_waitcnt(uo_vmcnt(0));
+        // every wave's part of step kc is in LDS, and stage (kc - 1) % kUoStages is
+        // free (its fragments were consumed by the MFMAs).  A bare barrier, with
+        // compiler-only fences: __syncthreads would also wait for the younger
+        // step's loads.
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (kc + kUoStages - 1 < nchunks) UO_ISSUE((kc + kUoStages - 1) % kUoStages)
+        const unsigned char* st = lds_ + (kc % kUoStages) * kStage;
+#pragma unroll
+        for (int kk = 0; kk < kUoKC; kk += 16) {
+            const int q = (kk >> 3) + (lane >> 5);
+            f16x8 af[2], bf[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int r = ra0 + 32 * i;
+                af[i] = *reinterpret_cast<const f16x8*>(st + r * 64 + ((q ^ ((r >> 2) & 3)) << 4));
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int r = rb0 + 32 * j;
+                bf[j] = *reinterpret_cast<const f16x8*>(st + BM * 64 + r * 64 + ((q ^ ((r >> 2) & 3)) << 4));
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#undef UO_ISSUE
+    uo_epilogue<EPI>(a, acc, m0 + wm, n0 + wn, lane, P, HW);
 }
 
 // Convolution with Cout <= 2 (the delta, weight and eta heads) on the VALU.
@@ -433,6 +504,32 @@ __global__ __launch_bounds__(256) void uo_fc_kernel(const float* __restrict__ in
     out[i] = s + b[k];
 }
 
+// Launches the implicit GEMM with WM x WN waves per workgroup (a tile of
+// 64 WM pixels x 64 WN output channels; Cout_pad is a multiple of 64 WN).
+template <int WM, int WN>
+int uo_launch(const ConvArgs& a, int epi, int64_t P, hipStream_t st) {
+    const dim3 grid(ceil_div(P, 64 * WM), a.Cout_pad / (64 * WN)), block(64 * WM * WN);
+    const size_t lds = (size_t)kUoStages * (64 * WM + 64 * WN) * 64;
+    switch (epi) {
+#define UO_CASE(E)                                                                                         \
+    case E: {                                                                                              \
+        const void* fn = reinterpret_cast<const void*>(&uo_conv_kernel<E, WM, WN>);                        \
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)   \
+            return fail(VH_ERR_HIP, "uo_conv: %zu bytes of LDS", lds);                                     \
+        hipLaunchKernelGGL((uo_conv_kernel<E, WM, WN>), grid, block, lds, st, a);                          \
+        return VH_OK;                                                                                      \
+    }
+        UO_CASE(VH_UO_NONE)
+        UO_CASE(VH_UO_RELU)
+        UO_CASE(VH_UO_SIGMOID)
+        UO_CASE(VH_UO_SIGMUL)
+        UO_CASE(VH_UO_GATES)
+        UO_CASE(VH_UO_GRU)
+#undef UO_CASE
+        default: return fail(VH_ERR_ARG, "uo_conv: epilogue %d", a.epi);
+    }
+}
+
 }  // namespace
 }  // namespace vh
 
@@ -448,8 +545,9 @@ int vh_uo_conv(const vh_uo_conv_args* args, void* stream) {
     if (g.nseg < 1 || g.nseg > 4) return fail(VH_ERR_ARG, "uo_conv: %d input segments (1 to 4)", g.nseg);
     if (g.KH < 1 || g.KW < 1 || (g.KH & 1) == 0 || (g.KW & 1) == 0)
         return fail(VH_ERR_ARG, "uo_conv: kernel %dx%d (odd sizes, 'same' padding)", g.KH, g.KW);
-    if (g.Cout < 1 || g.Cout_pad < g.Cout || g.Cout_pad % kUoBN != 0)
-        return fail(VH_ERR_ARG, "uo_conv: Cout %d, Cout_pad %d (a multiple of %d)", g.Cout, g.Cout_pad, kUoBN);
+    if (g.Cout < 1 || g.Cout_pad < g.Cout || g.Cout_pad % kUoCoutAlign != 0)
+        return fail(VH_ERR_ARG, "uo_conv: Cout %d, Cout_pad %d (a multiple of %d)", g.Cout, g.Cout_pad,
+                    kUoCoutAlign);
     ConvArgs a{};
     int cin = 0;
     for (int s = 0; s < g.nseg; s++) {
@@ -496,18 +594,12 @@ int vh_uo_conv(const vh_uo_conv_args* args, void* stream) {
     a.out2 = (_Float16*)g.out2;
     const int64_t P = (int64_t)g.nimg * g.H * g.W;
     if (P == 0) return VH_OK;
-    if (P + kUoBM >= (int64_t)1 << 31) return fail(VH_ERR_UNSUPPORTED, "uo_conv: %lld pixels", (long long)P);
-    const dim3 grid(ceil_div(P, kUoBM), g.Cout_pad / kUoBN);
+    if (P + 256 >= (int64_t)1 << 31) return fail(VH_ERR_UNSUPPORTED, "uo_conv: %lld pixels", (long long)P);
     const hipStream_t st = (hipStream_t)stream;
-    switch (epi) {
-        case VH_UO_NONE: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_NONE>, grid, dim3(256), 0, st, a); break;
-        case VH_UO_RELU: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_RELU>, grid, dim3(256), 0, st, a); break;
-        case VH_UO_SIGMOID: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_SIGMOID>, grid, dim3(256), 0, st, a); break;
-        case VH_UO_SIGMUL: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_SIGMUL>, grid, dim3(256), 0, st, a); break;
-        case VH_UO_GATES: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_GATES>, grid, dim3(256), 0, st, a); break;
-        case VH_UO_GRU: hipLaunchKernelGGL(uo_conv_kernel<VH_UO_GRU>, grid, dim3(256), 0, st, a); break;
-        default: return fail(VH_ERR_ARG, "uo_conv: epilogue %d", g.epilogue);
-    }
+    // 128 pixels x 256 channels where the output has a multiple of 256 channels
+    // (each pixel row is staged once for all of them), else 256 pixels x 128
+    const int rc = g.Cout_pad % 256 == 0 ? uo_launch<2, 4>(a, epi, P, st) : uo_launch<4, 2>(a, epi, P, st);
+    if (rc != VH_OK) return rc;
     VH_LAUNCH_CHECK("uo_conv");
     return VH_OK;
 }

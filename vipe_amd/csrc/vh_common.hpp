@@ -44,6 +44,19 @@ inline int hip_fail(hipError_t e, const char* where) {
 
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b on XCD b % 8),
+// and each XCD has its own L2.  xcd_logical renumbers workgroup b of nb so
+// that XCD x runs the x-th contiguous range of logical blocks: neighbouring
+// blocks, which share input rows, then share one L2.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
+    const int64_t xcd = b % kXcds, local = b / kXcds;
+    const int64_t per = (nb + kXcds - 1) / kXcds, r = nb % kXcds;
+    if (r == 0 || xcd < r) return xcd * per + local;
+    return r * per + (xcd - r) * (per - 1) + local;
+}
+
 template <typename T>
 struct dtype_of;
 template <>
