@@ -49,7 +49,7 @@ BUILD_CASES = [
 
 
 @pytest.mark.parametrize("in_dt,mma_dt,out_dt,tol", BUILD_CASES)
-@pytest.mark.parametrize("E,C,H,W", [(3, 128, 12, 16), (2, 40, 13, 18), (1, 128, 24, 40)])
+@pytest.mark.parametrize("E,C,H,W", [(3, 128, 12, 16), (2, 40, 13, 18), (1, 128, 24, 40), (1, 32, 11, 21)])
 def test_corr_build_and_pyramid(in_dt, mma_dt, out_dt, tol, E, C, H, W):
     from vipe_amd.ext import droid_net_ext as D
 
@@ -110,7 +110,7 @@ def test_altcorr_forward_backward_and_indexed():
 
 @pytest.mark.parametrize("layout", ["planar", "tiled"])
 @pytest.mark.parametrize("vol_dt,tol", [(torch.float32, 1e-5), (torch.float16, 2e-3)])
-@pytest.mark.parametrize("H,W", [(12, 16), (13, 18)])
+@pytest.mark.parametrize("H,W", [(12, 16), (13, 18), (11, 21)])
 def test_fused_pyramid_lookup_matches_per_level_and_oracle(layout, vol_dt, tol, H, W):
     from vipe_amd.corr import CorrBlock
 
@@ -140,7 +140,7 @@ def test_fused_pyramid_lookup_matches_per_level_and_oracle(layout, vol_dt, tol, 
 
 
 @pytest.mark.parametrize("out_dt", [torch.float32, torch.float16])
-@pytest.mark.parametrize("E,C,H,W", [(2, 40, 13, 18), (1, 128, 24, 40)])
+@pytest.mark.parametrize("E,C,H,W", [(2, 40, 13, 18), (1, 128, 24, 40), (2, 32, 11, 21), (1, 160, 9, 35)])
 def test_tiled_build_equals_planar_build(out_dt, E, C, H, W):
     from vipe_amd.ext import droid_net_ext as D
 

@@ -1191,10 +1191,13 @@ inline size_t altcorr_gather_lds(int W2, int C, int nwin) {
 //                              of 32 x 32 on MFMA (v_mfma_f32_32x32x16_{f16,
 //                              bf16,fp8_fp8} or 32x32x2f32) with operands staged
 //                              in LDS 128 channels at a time (A once per band
-//                              when C <= 128).  The epilogue keeps the level-0
-//                              tile in LDS and writes level 0 and the pooled
-//                              levels 1-3 of the patch directly: the pyramid
-//                              costs no second pass over level 0.
+//                              when C <= 128).  Target pixels are the MFMA rows,
+//                              ordered so that a lane's four accumulator rows
+//                              are four adjacent columns (one vector LDS write).
+//                              The epilogue keeps the level-0 tile in LDS and
+//                              writes level 0 and the pooled levels 1-3 of the
+//                              patch directly: the pyramid costs no second pass
+//                              over level 0.
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
@@ -1413,9 +1416,69 @@ __device__ __forceinline__ void store_run8(Tout* dst, const float (&v)[8]) {
 // barrier (__syncthreads would wait for them).
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Target row r (0..127) of a patch is pixel (patch_y(r), patch_x(r)) of its
+// 8 x 16 block: r = 64 X + 32 Y + 8 a + c covers rows 4 Y + a and columns
+// 8 X + c, so that a lane's four consecutive accumulator rows of an MFMA
+// sub-tile are four adjacent columns of one row.
+__device__ __forceinline__ int patch_y(int r) { return 4 * ((r >> 5) & 1) + ((r >> 3) & 3); }
+__device__ __forceinline__ int patch_x(int r) { return 8 * (r >> 6) + (r & 7); }
+
+// Four consecutive values into LDS in the output type (one 8- or 16-byte write).
+__device__ __forceinline__ void lds_store4(float* p, const float (&v)[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void lds_store4(__half* p, const float (&v)[4]) {
+    const __half2 a = __floats2half2_rn(v[0], v[1]), b = __floats2half2_rn(v[2], v[3]);
+    uint2 u;
+    u.x = *reinterpret_cast<const unsigned int*>(&a);
+    u.y = *reinterpret_cast<const unsigned int*>(&b);
+    *reinterpret_cast<uint2*>(p) = u;
+}
+
+// Eight consecutive LDS values (16-byte aligned) as float, and back.
+__device__ __forceinline__ void lds_load8(const float* p, float (&v)[8]) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ __forceinline__ void lds_load8(const __half* p, float (&v)[8]) {
+    const uint4 u = *reinterpret_cast<const uint4*>(p);
+    const unsigned int w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float2 f = __half22float2(*reinterpret_cast<const __half2*>(&w[k]));
+        v[2 * k] = f.x;
+        v[2 * k + 1] = f.y;
+    }
+}
+__device__ __forceinline__ void lds_store8(float* p, const float (&v)[8]) {
+    reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+    reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+__device__ __forceinline__ void lds_store8(__half* p, const float (&v)[8]) {
+    unsigned int w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const __half2 h = __floats2half2_rn(v[2 * k], v[2 * k + 1]);
+        w[k] = *reinterpret_cast<const unsigned int*>(&h);
+    }
+    *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 struct LevelGeoms {
     LevelGeom g[4];
 };
+
+// Element offset of (y, x) inside a plane of level geometry g: LevelGeom::at
+// with the tile width TW (elements of a 16-byte tile row of the output type)
+// known at compile time, so the tile column and offset are shifts and masks.
+template <int TW>
+__device__ __forceinline__ int plane_off(const LevelGeom& g, int y, int x) {
+    constexpr int LT = TW == 2 ? 1 : TW == 4 ? 2 : 3;
+    static_assert(TW == (1 << LT), "tile width");
+    if (!g.tiled) return y * g.Wl + x;
+    return ((y >> 2) * (g.Wp >> LT) + (x >> LT)) * (4 * TW) + (y & 3) * TW + (x & (TW - 1));
+}
 
 template <typename Mma, typename Tout, bool FP8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void corr_build_kernel(const typename Mma::T* __restrict__ A,
@@ -1425,6 +1488,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                                                          Tout* __restrict__ o2, Tout* __restrict__ o3, int L, int Cp,
                                                          int H, int W, LevelGeoms lg) {
     using T = typename Mma::T;
+    constexpr int kTW = kTileBytes / 4 / (int)sizeof(Tout);  // tile width of the tiled layout
     constexpr int RS = kKC + 16 / (int)sizeof(T);  // operand row stride (elements): 16 B of padding
     constexpr int AB = kBM * RS * (int)sizeof(T);  // bytes of one staged operand
     constexpr int OB = kBM * kOS * (int)sizeof(Tout);
@@ -1470,7 +1534,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 if (isA) {
                     if (m0 + r < HW) v[u] = *reinterpret_cast<const uint4*>(Ae + (int64_t)(m0 + r) * Cp + c0 + q * CH);
                 } else {
-                    const int y = y0 + r / kPX, x = x0 + r % kPX;
+                    const int y = y0 + patch_y(r), x = x0 + patch_x(r);
                     if (y < H && x < W)
                         v[u] = *reinterpret_cast<const uint4*>(Be + (int64_t)(y * W + x) * Cp + c0 + q * CH);
                 }
@@ -1512,33 +1576,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
             }
             store_rows(Bs, breg);  // single pass: loaded during the previous patch
             lds_barrier();
+            // acc[i][j]: target rows wn + 32 i + (0..31) (MFMA rows) x source rows wm + 32 j + (0..31)
             for (int kk = 0; kk < kc; kk += Mma::KI) {
                 const int k = kk + PER * (lane >> 5);
-                const T* a0 = As + (wm + (lane & 31)) * RS + k;
-                const T* a1 = a0 + 32 * RS;
                 const T* b0 = Bs + (wn + (lane & 31)) * RS + k;
                 const T* b1 = b0 + 32 * RS;
-                acc[0][0] = Mma::mma(a0, b0, acc[0][0]);
-                acc[0][1] = Mma::mma(a0, b1, acc[0][1]);
-                acc[1][0] = Mma::mma(a1, b0, acc[1][0]);
-                acc[1][1] = Mma::mma(a1, b1, acc[1][1]);
+                const T* a0 = As + (wm + (lane & 31)) * RS + k;
+                const T* a1 = a0 + 32 * RS;
+                acc[0][0] = Mma::mma(b0, a0, acc[0][0]);
+                acc[0][1] = Mma::mma(b0, a1, acc[0][1]);
+                acc[1][0] = Mma::mma(b1, a0, acc[1][0]);
+                acc[1][1] = Mma::mma(b1, a1, acc[1][1]);
             }
             lds_barrier();
         }
         if (single_pass && tx + 1 < npx) load_rows(breg, false, x0 + kPX, 0, Cp / CH);
-        // level-0 tile, rounded to the output type, over Bs: lane holds column
-        // (lane % 32) and rows 8 (v / 4) + 4 (lane / 32) + (v % 4) of each sub-tile
+        // level-0 tile, rounded to the output type, over Bs: for source sub-tile j
+        // the lane holds source row wm + 32 j + lane % 32 and, in acc[i][j][4 a + c],
+        // target row 4 i + a, column wn / 8 + 4 (lane / 32) + c of the patch, so
+        // each four values are one vector write
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
             for (int j = 0; j < 2; j++) {
-                const int n = wn + 32 * j + (lane & 31);
+                const int m = wm + 32 * j + (lane & 31);
 #pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    const int m = wm + 32 * i + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                    float val = acc[i][j][v] * 0.0625f;
-                    if (FP8) val *= ss1[m] * ss2[n];
-                    Os[m * kOS + n] = from_f<Tout>(val);
+                for (int a4 = 0; a4 < 4; a4++) {
+                    const int n = (4 * i + a4) * kPX + (wn >> 3) + 4 * (lane >> 5);
+                    float v4[4];
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        v4[c] = acc[i][j][4 * a4 + c] * 0.0625f;
+                        if (FP8) v4[c] *= ss1[m] * ss2[n + c];
+                    }
+                    lds_store4(Os + m * kOS + n, v4);
                 }
             }
         lds_barrier();
@@ -1551,10 +1622,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 const int px = (idx & 1) * 8, py = (idx >> 1) & (kPY - 1), ml = idx >> 4;
                 const int m = m0 + ml, y = y0 + py, x = x0 + px;
                 if (m >= HW || y >= H || x >= W) continue;
-                float vv[8];
-                load4(Os + ml * kOS + py * kPX + px, vv);
-                load4(Os + ml * kOS + py * kPX + px + 4, vv + 4);
-                store_run8(o0 + lg.g[0].at(ob + m, y, x), vv);
+                // the rounded FP16 values are copied as they are (two 8-byte LDS reads)
+                const uint2 a = *reinterpret_cast<const uint2*>(Os + ml * kOS + py * kPX + px);
+                const uint2 b = *reinterpret_cast<const uint2*>(Os + ml * kOS + py * kPX + px + 4);
+                *reinterpret_cast<uint4*>(o0 + (ob + m) * lg.g[0].plane + plane_off<kTW>(lg.g[0], y, x)) =
+                    make_uint4(a.x, a.y, b.x, b.y);
             }
         } else {
             // level 0: 4 target columns per store
@@ -1566,7 +1638,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 if (m >= HW || y >= H || x >= W) continue;
                 float vv[4];
                 load4(Os + ml * kOS + py * kPX + px, vv);
-                store_run<Tout, 4>(o0 + lg.g[0].at(ob + m, y, x), vv, min(4, W - x), vec0);
+                store_run<Tout, 4>(o0 + (ob + m) * lg.g[0].plane + plane_off<kTW>(lg.g[0], y, x), vv, min(4, W - x), vec0);
             }
         }
         if (L >= 2 && sizeof(Tout) == 2 && lg.g[1].tiled) {
@@ -1585,12 +1657,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 }
                 float vv[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
+                for (int u = 0; u < 8; u++)
                     vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
-                    P1[ml][py * 8 + u] = from_f<Tout>(vv[u]);
-                }
+                lds_store8(&P1[ml][py * 8], vv);
                 const int m = m0 + ml, y = y0 / 2 + py, x = x0 / 2;
-                if (m < HW && y < Hl && x < Wl) store_run8(o1 + lg.g[1].at(ob + m, y, x), vv);
+                // columns past Wl are the tile's zero padding (with W odd, column
+                // Wl would pool the last level-0 column)
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (x + u >= Wl) vv[u] = 0.f;
+                if (m < HW && y < Hl && x < Wl) store_run8(o1 + (ob + m) * lg.g[1].plane + plane_off<kTW>(lg.g[1], y, x), vv);
             }
         } else if (L >= 2) {
             // level 1 of the patch (4 x 8), once each, from rounded level 0
@@ -1607,12 +1683,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 load4(Os + ml * kOS + (2 * py + 1) * kPX + 8 * hx + 4, r1 + 4);
                 float vv[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < 4; u++)
                     vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
-                    P1[ml][py * 8 + 4 * hx + u] = from_f<Tout>(vv[u]);
-                }
+                lds_store4(&P1[ml][py * 8 + 4 * hx], vv);
                 const int m = m0 + ml, y = y0 / 2 + py, x = x0 / 2 + 4 * hx;
-                if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o1 + lg.g[1].at(ob + m, y, x), vv, min(4, Wl - x), vec);
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o1 + (ob + m) * lg.g[1].plane + plane_off<kTW>(lg.g[1], y, x), vv, min(4, Wl - x), vec);
             }
         }
         if (L >= 3) {
@@ -1622,19 +1697,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
             const bool vec = lg.g[2].tiled || (Wl % 4) == 0;
             const int py = tid & 1, ml = tid >> 1;
             float r0[8], r1[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                r0[u] = to_f(P1[ml][(2 * py) * 8 + u]);
-                r1[u] = to_f(P1[ml][(2 * py + 1) * 8 + u]);
-            }
+            lds_load8(&P1[ml][(2 * py) * 8], r0);
+            lds_load8(&P1[ml][(2 * py + 1) * 8], r1);
             float vv[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < 4; u++)
                 vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
-                P2[ml][py * 4 + u] = from_f<Tout>(vv[u]);
-            }
+            lds_store4(&P2[ml][py * 4], vv);
             const int m = m0 + ml, y = y0 / 4 + py, x = x0 / 4;
-            if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o2 + lg.g[2].at(ob + m, y, x), vv, min(4, Wl - x), vec);
+            if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o2 + (ob + m) * lg.g[2].plane + plane_off<kTW>(lg.g[2], y, x), vv, min(4, Wl - x), vec);
         }
         if (L >= 4) {
             lds_barrier();
@@ -1642,13 +1713,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
             if (tid < kBM) {
                 const int Hl = H / 8, Wl = W / 8;
                 const bool vec = lg.g[3].tiled || (Wl % 2) == 0;
-                float vv[2];
+                float p2[8], vv[2];
+                lds_load8(&P2[tid][0], p2);
 #pragma unroll
                 for (int u = 0; u < 2; u++)
-                    vv[u] = round_to<Tout>((((to_f(P2[tid][2 * u]) + to_f(P2[tid][2 * u + 1])) + to_f(P2[tid][4 + 2 * u])) +
-                                            to_f(P2[tid][4 + 2 * u + 1])) * 0.25f);
+                    vv[u] = round_to<Tout>((((p2[2 * u] + p2[2 * u + 1]) + p2[4 + 2 * u]) + p2[4 + 2 * u + 1]) * 0.25f);
                 const int m = m0 + tid, y = y0 / 8, x = x0 / 8;
-                if (m < HW && y < Hl && x < Wl) store_run<Tout, 2>(o3 + lg.g[3].at(ob + m, y, x), vv, min(2, Wl - x), vec);
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 2>(o3 + (ob + m) * lg.g[3].plane + plane_off<kTW>(lg.g[3], y, x), vv, min(2, Wl - x), vec);
             }
         }
         lds_barrier();  // the next patch stages over Os
