@@ -40,8 +40,10 @@ extern "C" {
 
 /* volume plane layouts (vh_corr_build / vh_corr_lookup_pyramid) */
 #define VH_LAYOUT_PLANAR 0 /* [Hl][Wl] row-major: the reference's */
-#define VH_LAYOUT_TILED 1  /* padded to [ceil(Hl/4)*4][ceil(Wl/tw)*tw], stored as 4 x tw tiles of 64 bytes
-                            * (tw = 4 for F32, 8 for F16), tiles row-major, padding zero */
+#define VH_LAYOUT_TILED 1  /* each plane padded to [ceil(Hl/2)*2][ceil(Wl/tw)*tw] (tw = 4 for F32, 8 for
+                            * F16; padding zero); planes grouped by 4 (p / 4), a group stored as row-major
+                            * 128-byte tiles of rows 2a, 2a+1 x columns tw b .. tw b + tw-1 of its planes,
+                            * 16-byte piece (row % 2) * 4 + p % 4; needs H*W per edge a multiple of 4 */
 
 #define VH_DEVICE 0 /* pointers are device memory, kernels on `stream` */
 #define VH_HOST 1   /* pointers are host memory, computed on the CPU   */

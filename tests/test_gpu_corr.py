@@ -110,7 +110,7 @@ def test_altcorr_forward_backward_and_indexed():
 
 @pytest.mark.parametrize("layout", ["planar", "tiled"])
 @pytest.mark.parametrize("vol_dt,tol", [(torch.float32, 1e-5), (torch.float16, 2e-3)])
-@pytest.mark.parametrize("H,W", [(12, 16), (13, 18), (11, 21)])
+@pytest.mark.parametrize("H,W", [(12, 16), (13, 20), (12, 21)])
 def test_fused_pyramid_lookup_matches_per_level_and_oracle(layout, vol_dt, tol, H, W):
     from vipe_amd.corr import CorrBlock
 
@@ -140,7 +140,7 @@ def test_fused_pyramid_lookup_matches_per_level_and_oracle(layout, vol_dt, tol, 
 
 
 @pytest.mark.parametrize("out_dt", [torch.float32, torch.float16])
-@pytest.mark.parametrize("E,C,H,W", [(2, 40, 13, 18), (1, 128, 24, 40), (2, 32, 11, 21), (1, 160, 9, 35)])
+@pytest.mark.parametrize("E,C,H,W", [(2, 40, 13, 20), (1, 128, 24, 40), (2, 32, 12, 21), (1, 160, 12, 35)])
 def test_tiled_build_equals_planar_build(out_dt, E, C, H, W):
     from vipe_amd.ext import droid_net_ext as D
 
@@ -152,6 +152,25 @@ def test_tiled_build_equals_planar_build(out_dt, E, C, H, W):
     for lvl, (p, t) in enumerate(zip(planar, tiled)):
         assert torch.equal(D.untile(t, H >> lvl, W >> lvl), p)
         assert torch.equal(t, D.tile(p))  # padding is zero
+
+
+def test_tiled_layout_needs_whole_plane_groups():
+    """Tiled levels group neighbouring source planes by 4, so an edge's H*W must be a
+    multiple of 4: the build refuses other counts and CorrBlock keeps such volumes
+    planar."""
+    from vipe_amd.corr import CorrBlock
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(4)
+    f1 = torch.randn(1, 2, 32, 11, 21, generator=g)
+    f2 = torch.randn(1, 2, 32, 11, 21, generator=g)
+    with pytest.raises(RuntimeError, match="group"):
+        D.corr_build(f1[0].cuda(), f2[0].cuda(), 4, layout="tiled")
+    blk = CorrBlock(f1.cuda(), f2.cuda(), num_levels=4, radius=3)
+    assert blk.layout == "planar"
+    coords = torch.rand(1, 2, 11, 21, 2, generator=g) * torch.tensor([21.0, 11.0])
+    ref = OC.corr_block_lookup(OC.corr_pyramid(OC.corr_volume(f1.double(), f2.double()), 4), coords.double(), 3)
+    assert (blk(coords.cuda()).double().cpu() - ref).abs().max().item() < 1e-5
 
 
 def test_corr_lookup_gradients_to_both_feature_maps():

@@ -120,9 +120,11 @@ class CorrBlock:
     feature dtype.  ``contraction_dtype`` selects the MFMA operand type of the
     volume build (default: the feature dtype; float16, bfloat16 or
     float8_e4m3fn for the mixed-precision forms, all accumulating in FP32).
-    ``layout`` "tiled" (default) keeps each level's planes as 64-byte tiles,
-    which the build writes and the lookup reads in whole sectors; "planar"
-    keeps the reference's [B*N, H, W, h, w] levels.  Callers only build,
+    ``layout`` "tiled" (default) keeps each level's planes as 128-byte tiles
+    shared by four neighbouring source pixels, which the build writes and the
+    lookup reads in whole cache lines (more than 4 levels, or H*W not a multiple of 4, use
+    "planar");
+    "planar" keeps the reference's [B*N, H, W, h, w] levels.  Callers only build,
     look up, ``cat`` and index along the edge dimension, which both support.
     """
 
@@ -133,7 +135,7 @@ class CorrBlock:
         B, N, C, H, W = fmap1.shape
         f1 = fmap1.reshape(B * N, C, H, W).contiguous()
         f2 = fmap2.reshape(B * N, C, H, W).contiguous()
-        if layout == "tiled" and num_levels > 4:
+        if layout == "tiled" and (num_levels > 4 or (H * W) % droid_net_ext.TILE_PLANES):
             layout = "planar"
         levels = droid_net_ext.corr_build(f1.detach(), f2.detach(), num_levels, out_dtype=volume_dtype,
                                           contraction_dtype=contraction_dtype, layout=layout)

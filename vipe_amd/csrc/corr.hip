@@ -97,30 +97,49 @@ __device__ __forceinline__ void lookup_level(const T* __restrict__ plane, int H2
 }
 
 // Volume plane layouts.  Planar: [H2][W2] row-major (the reference's).
-// Tiled: the plane padded to Hp = ceil(H2/4)*4 rows and Wp = ceil(W2/tw)*tw
-// columns and stored as [Hp/4][Wp/tw] tiles of 4 rows x tw columns, one
-// 64-byte tile each (tw = 4 for f32, 8 for f16); the padding holds zeros.  A
-// lookup window then touches whole 64-byte sectors, and the build writes
-// whole tiles.
-constexpr int kTileBytes = 64;  // tile size in bytes (measured against 128: equal for f32, faster for f16)
+// Tiled: each plane is padded to Hp = ceil(H2/2)*2 rows and Wp = ceil(W2/tw)*tw
+// columns (tw = 4 for f32, 8 for f16: 16 bytes), the padding holding zeros.
+// Planes are grouped by four, pm / 4 (source pixels side by side in a row), and
+// a group is stored as [Hp/2][Wp/tw] tiles of 128 bytes, one cache line each; a
+// tile holds rows 2a, 2a + 1 and columns tw b .. tw b + tw - 1 of the four
+// planes as eight 16-byte pieces, piece (y % 2) * 4 + pm % 4.  The neighbouring
+// source pixels of a group look up nearly the same windows (the flow is
+// smooth), so the lines a window touches serve all four; per source and level
+// the 8 x 8 windows of the bench's graph touch 3.4 (f32) / 2.2 (f16) GB of
+// lines over 384 edges against 3.8 / 2.8 GB with a 4-row tile per plane, the
+// best of the 128-byte line contents compared.  The build writes whole pieces.
+// The number of planes of an edge (H1 * W1) must be a multiple of 4.
+constexpr int kPieceBytes = 16;                 // one tile row of one plane: tw columns
+constexpr int kTilePlanes = 4;                  // source planes sharing a tile
+constexpr int kTileRows = 2;                    // rows of a plane in a tile
+constexpr int kTilePieces = kTileRows * kTilePlanes;
+constexpr int kTileBytes = kTilePieces * kPieceBytes;
 
 struct LevelGeom {
     int Hl, Wl, Hp, Wp, tw;
     int tiled;
     int64_t plane;  // elements per source pixel
-    __host__ __device__ int64_t at(int64_t pm, int y, int x) const {
-        if (!tiled) return pm * plane + (int64_t)y * Wl + x;
-        return pm * plane + ((int64_t)(y >> 2) * (Wp / tw) + x / tw) * (4 * tw) + (y & 3) * tw + (x % tw);
-    }
 };
+
+// Element offset of the origin of plane pm of a tiled level (plane elements
+// per source): its pair's start plus its 16-byte piece within a tile row.
+__host__ __device__ __forceinline__ int64_t tiled_src(int64_t pm, int64_t plane, int tw) {
+    return (pm / kTilePlanes) * kTilePlanes * plane + (pm % kTilePlanes) * tw;
+}
+
+// Element offset of (y, x) from a plane's origin in a tiled level whose padded
+// row has ntx tiles of tw columns.
+__host__ __device__ __forceinline__ int tiled_yx(int y, int x, int ntx, int tw) {
+    return ((y / kTileRows) * ntx + x / tw) * (kTilePieces * tw) + (y % kTileRows) * (kTilePlanes * tw) + x % tw;
+}
 
 inline LevelGeom level_geom(int Hl, int Wl, int elem, bool tiled) {
     LevelGeom g;
     g.Hl = Hl;
     g.Wl = Wl;
     g.tiled = tiled ? 1 : 0;
-    g.tw = kTileBytes / 4 / elem;
-    g.Hp = tiled ? (Hl + 3) / 4 * 4 : Hl;
+    g.tw = kPieceBytes / elem;
+    g.Hp = tiled ? (Hl + kTileRows - 1) / kTileRows * kTileRows : Hl;
     g.Wp = tiled ? (Wl + g.tw - 1) / g.tw * g.tw : Wl;
     g.plane = (int64_t)g.Hp * g.Wp;
     return g;
@@ -179,19 +198,19 @@ __device__ __forceinline__ void tap_row_load(const T* __restrict__ plane, int y1
                                              TapRow<T, R, VEC>& t) {
     using V = typename TapRow<T, R, VEC>::V;
     constexpr int NL = TapRow<T, R, VEC>::NL;
-    constexpr int TW = kTileBytes / 4 / (int)sizeof(T);  // tile width (tiled layout)
-    // tiled: H2, W2 are the padded sizes, whose extra rows and columns hold zeros
+    constexpr int TW = kPieceBytes / (int)sizeof(T);  // tile width (tiled layout)
+    // tiled: H2, W2 are the padded sizes, whose extra rows and columns hold
+    // zeros, and plane is the source's origin (tiled_src)
     const bool yok = (y1 >= 0) && (y1 < H2);
     const int yy = yok ? y1 : 0;
-    const T* rp = TILED ? plane + (int64_t)(yy >> 2) * (W2 / TW) * (4 * TW) + (yy & 3) * TW
-                        : plane + (int64_t)yy * W2;
+    const T* rp = TILED ? plane + tiled_yx(yy, 0, W2 / TW, TW) : plane + (int64_t)yy * W2;
     const int xa = xb & ~(VEC - 1);
 #pragma unroll
     for (int q = 0; q < NL; q++) {
         // out-of-plane vectors read the (aligned, valid) plane start and are zeroed
         const int x = xa + q * VEC;
         t.ok[q] = yok && x >= 0 && x < W2;
-        const T* src = TILED ? rp + (x / TW) * (4 * TW) + (x % TW) : rp + x;
+        const T* src = TILED ? rp + (x / TW) * (kTilePieces * TW) + (x % TW) : rp + x;
         t.v[q] = *reinterpret_cast<const V*>(t.ok[q] ? src : plane);
     }
 }
@@ -453,7 +472,8 @@ __global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, 
     for (int l = 0; l < L; l++) {
         const float sc = 1.0f / (float)(1 << l);
         const int H2 = lv.h[l], W2 = lv.w[l];
-        const T* plane = (const T*)lv.p[l] + pix * (int64_t)H2 * W2;
+        const T* plane = (const T*)lv.p[l] + (TILED ? tiled_src(pix, (int64_t)H2 * W2, kPieceBytes / (int)sizeof(T))
+                                                     : pix * (int64_t)H2 * W2);
         if constexpr (TILED)
             lookup_level_vec<T, R, 4, true>(plane, H2, W2, cxf * sc, cyf * sc, o + (int64_t)l * RD * RD * HW1, HW1);
         else
@@ -463,11 +483,12 @@ __global__ __launch_bounds__(256) void corr_lookup_pyramid_kernel(LevelPtrs lv, 
 
 // Tiled pyramid lookup through buffer resources.  One lane per source pixel.
 // Per level a lane issues every load of its (2r+2)-row window before it uses
-// any of them, so the four rows that share a 64-byte tile are fetched back to
-// back; read row by row, a tile could leave L2 between its rows and be fetched
-// again.  The load resource starts at the wave's first plane and a lane's
-// plane is lane * plane_bytes into it; a vector outside the plane gets an
-// offset past the resource's range, which reads zeros without touching memory.
+// any of them, so the rows and the neighbouring planes that share a tile are
+// fetched back to back; read row by row, a tile could leave L2 between its
+// rows and be fetched again.  The load resource starts at the wave's first
+// plane (a multiple of 4) and a lane's plane origin is tiled_src(lane) into it;
+// a vector outside the plane gets an offset past the resource's range, which
+// reads zeros without touching memory.
 // The outputs go through a second resource: one 32-bit lane offset, and the
 // channel offset c * H1 * W1 in a scalar register.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -504,7 +525,7 @@ template <typename T, int R>
 __global__ __launch_bounds__(256) void corr_lookup_tiled_kernel(LevelPtrs lv, int L, const float* __restrict__ coords,
                                                                 T* __restrict__ out, int64_t B, int H1, int W1) {
     constexpr int RD = 2 * R + 1, NT = 2 * R + 2;
-    constexpr int TW = kTileBytes / 4 / (int)sizeof(T);  // elements of one 16-byte tile row
+    constexpr int TW = kPieceBytes / (int)sizeof(T);  // elements of one 16-byte tile row
     constexpr int LOG_TW = TW == 4 ? 2 : 3;
     constexpr int NL = (NT + TW - 1 + TW - 1) / TW;      // 16-byte vectors per tap row at any shift
     constexpr int NE = NL * TW;                          // elements loaded per tap row
@@ -525,7 +546,7 @@ __global__ __launch_bounds__(256) void corr_lookup_tiled_kernel(LevelPtrs lv, in
         const int Hp = lv.h[l], Wp = lv.w[l], ntx = Wp >> LOG_TW;
         const int64_t plane = (int64_t)Hp * Wp;
         const __amdgpu_buffer_rsrc_t rl = buf_rsrc((const T*)lv.p[l] + pix0 * plane);
-        const uint32_t lvo = (uint32_t)(lane * plane * (int64_t)sizeof(T));
+        const uint32_t lvo = (uint32_t)(tiled_src(lane, plane, TW) * (int64_t)sizeof(T));
         const float x0 = cx * sc, y0 = cy * sc;
         const float fx0 = floorf(x0), fy0 = floorf(y0);
         const float dx = x0 - fx0, dy = y0 - fy0;
@@ -536,7 +557,7 @@ __global__ __launch_bounds__(256) void corr_lookup_tiled_kernel(LevelPtrs lv, in
         for (int r = 0; r < NT; r++) {
             const int y = yb + r;
             const bool yok = (unsigned)y < (unsigned)Hp;
-            const uint32_t rowoff = lvo + (uint32_t)(((y >> 2) * ntx) * kTileBytes + (y & 3) * 16);
+            const uint32_t rowoff = lvo + (uint32_t)(tiled_yx(y, 0, ntx, TW) * (int)sizeof(T));
 #pragma unroll
             for (int q = 0; q < NL; q++) {
                 const bool ok = yok && (unsigned)(xt + q) < (unsigned)ntx;
@@ -1469,15 +1490,15 @@ struct LevelGeoms {
     LevelGeom g[4];
 };
 
-// Element offset of (y, x) inside a plane of level geometry g: LevelGeom::at
-// with the tile width TW (elements of a 16-byte tile row of the output type)
-// known at compile time, so the tile column and offset are shifts and masks.
+// Element offset of (y, x) of source plane pm in a level of geometry g, with
+// the tile width TW (elements of a 16-byte tile row of the output type) known
+// at compile time, so the tile column and offsets are shifts and masks.
 template <int TW>
-__device__ __forceinline__ int plane_off(const LevelGeom& g, int y, int x) {
+__device__ __forceinline__ int64_t level_off(const LevelGeom& g, int64_t pm, int y, int x) {
     constexpr int LT = TW == 2 ? 1 : TW == 4 ? 2 : 3;
     static_assert(TW == (1 << LT), "tile width");
-    if (!g.tiled) return y * g.Wl + x;
-    return ((y >> 2) * (g.Wp >> LT) + (x >> LT)) * (4 * TW) + (y & 3) * TW + (x & (TW - 1));
+    if (!g.tiled) return pm * g.plane + y * g.Wl + x;
+    return tiled_src(pm, g.plane, TW) + tiled_yx(y, 0, g.Wp >> LT, TW) + ((x >> LT) * (kTilePieces * TW) + (x & (TW - 1)));
 }
 
 template <typename Mma, typename Tout, bool FP8>
@@ -1488,7 +1509,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                                                          Tout* __restrict__ o2, Tout* __restrict__ o3, int L, int Cp,
                                                          int H, int W, LevelGeoms lg) {
     using T = typename Mma::T;
-    constexpr int kTW = kTileBytes / 4 / (int)sizeof(Tout);  // tile width of the tiled layout
+    constexpr int kTW = kPieceBytes / (int)sizeof(Tout);  // tile width of the tiled layout
     constexpr int RS = kKC + 16 / (int)sizeof(T);  // operand row stride (elements): 16 B of padding
     constexpr int AB = kBM * RS * (int)sizeof(T);  // bytes of one staged operand
     constexpr int OB = kBM * kOS * (int)sizeof(Tout);
@@ -1625,7 +1646,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 // the rounded FP16 values are copied as they are (two 8-byte LDS reads)
                 const uint2 a = *reinterpret_cast<const uint2*>(Os + ml * kOS + py * kPX + px);
                 const uint2 b = *reinterpret_cast<const uint2*>(Os + ml * kOS + py * kPX + px + 4);
-                *reinterpret_cast<uint4*>(o0 + (ob + m) * lg.g[0].plane + plane_off<kTW>(lg.g[0], y, x)) =
+                *reinterpret_cast<uint4*>(o0 + level_off<kTW>(lg.g[0], ob + m, y, x)) =
                     make_uint4(a.x, a.y, b.x, b.y);
             }
         } else {
@@ -1638,7 +1659,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 if (m >= HW || y >= H || x >= W) continue;
                 float vv[4];
                 load4(Os + ml * kOS + py * kPX + px, vv);
-                store_run<Tout, 4>(o0 + (ob + m) * lg.g[0].plane + plane_off<kTW>(lg.g[0], y, x), vv, min(4, W - x), vec0);
+                store_run<Tout, 4>(o0 + level_off<kTW>(lg.g[0], ob + m, y, x), vv, min(4, W - x), vec0);
             }
         }
         if (L >= 2 && sizeof(Tout) == 2 && lg.g[1].tiled) {
@@ -1666,7 +1687,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
 #pragma unroll
                 for (int u = 0; u < 8; u++)
                     if (x + u >= Wl) vv[u] = 0.f;
-                if (m < HW && y < Hl && x < Wl) store_run8(o1 + (ob + m) * lg.g[1].plane + plane_off<kTW>(lg.g[1], y, x), vv);
+                if (m < HW && y < Hl && x < Wl) store_run8(o1 + level_off<kTW>(lg.g[1], ob + m, y, x), vv);
             }
         } else if (L >= 2) {
             // level 1 of the patch (4 x 8), once each, from rounded level 0
@@ -1687,7 +1708,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                     vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
                 lds_store4(&P1[ml][py * 8 + 4 * hx], vv);
                 const int m = m0 + ml, y = y0 / 2 + py, x = x0 / 2 + 4 * hx;
-                if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o1 + (ob + m) * lg.g[1].plane + plane_off<kTW>(lg.g[1], y, x), vv, min(4, Wl - x), vec);
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o1 + level_off<kTW>(lg.g[1], ob + m, y, x), vv, min(4, Wl - x), vec);
             }
         }
         if (L >= 3) {
@@ -1705,7 +1726,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 vv[u] = round_to<Tout>((((r0[2 * u] + r0[2 * u + 1]) + r1[2 * u]) + r1[2 * u + 1]) * 0.25f);
             lds_store4(&P2[ml][py * 4], vv);
             const int m = m0 + ml, y = y0 / 4 + py, x = x0 / 4;
-            if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o2 + (ob + m) * lg.g[2].plane + plane_off<kTW>(lg.g[2], y, x), vv, min(4, Wl - x), vec);
+            if (m < HW && y < Hl && x < Wl) store_run<Tout, 4>(o2 + level_off<kTW>(lg.g[2], ob + m, y, x), vv, min(4, Wl - x), vec);
         }
         if (L >= 4) {
             lds_barrier();
@@ -1719,7 +1740,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
                 for (int u = 0; u < 2; u++)
                     vv[u] = round_to<Tout>((((p2[2 * u] + p2[2 * u + 1]) + p2[4 + 2 * u]) + p2[4 + 2 * u + 1]) * 0.25f);
                 const int m = m0 + tid, y = y0 / 8, x = x0 / 8;
-                if (m < HW && y < Hl && x < Wl) store_run<Tout, 2>(o3 + (ob + m) * lg.g[3].plane + plane_off<kTW>(lg.g[3], y, x), vv, min(2, Wl - x), vec);
+                if (m < HW && y < Hl && x < Wl) store_run<Tout, 2>(o3 + level_off<kTW>(lg.g[3], ob + m, y, x), vv, min(2, Wl - x), vec);
             }
         }
         lds_barrier();  // the next patch stages over Os
@@ -1885,6 +1906,10 @@ int vh_corr_lookup_pyramid(int dtype, int64_t B, int H1, int W1, int num_levels,
     const bool tiled = layout == VH_LAYOUT_TILED;
     if (layout != VH_LAYOUT_PLANAR && !tiled) return fail(VH_ERR_ARG, "corr_lookup_pyramid: layout %d", layout);
     if (tiled && dtype == VH_F64) return fail(VH_ERR_UNSUPPORTED, "corr_lookup_pyramid: tiled f64 volume");
+    if (tiled && ((int64_t)H1 * W1) % kTilePlanes != 0)
+        return fail(VH_ERR_UNSUPPORTED,
+                    "corr_lookup_pyramid: tiled levels group the planes of an edge by %d (H1*W1 = %lld)", kTilePlanes,
+                    (long long)H1 * W1);
     hipStream_t s = (hipStream_t)stream;
     switch (dtype) {
         case VH_F16: return corr_lookup_pyramid_launch<__half>(B, H1, W1, num_levels, levels, Hl, Wl, tiled, coords, out, s);
@@ -1908,6 +1933,9 @@ int vh_corr_build(int in_dtype, int mma_dtype, int out_dtype, int64_t E, int C, 
     if (E == 0) return VH_OK;
     if (!workspace) return fail(VH_ERR_ARG, "corr_build: null workspace");
     if (layout != VH_LAYOUT_PLANAR && layout != VH_LAYOUT_TILED) return fail(VH_ERR_ARG, "corr_build: layout %d", layout);
+    if (layout == VH_LAYOUT_TILED && ((int64_t)H * W) % kTilePlanes != 0)
+        return fail(VH_ERR_UNSUPPORTED, "corr_build: tiled levels group the planes of an edge by %d (H*W = %lld)",
+                    kTilePlanes, (long long)H * W);
     hipStream_t s = (hipStream_t)stream;
     if (out_dtype == VH_F32) {
         if (in_dtype == VH_F32) return corr_build_dispatch<float, float>(mma_dtype, E, C, H, W, fmap1, fmap2, num_levels, levels, layout == VH_LAYOUT_TILED, workspace, s);

@@ -135,31 +135,46 @@ def altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius):
     return [corr]
 
 
-TILE_BYTES = 64  # matches kTileBytes in corr.hip
+PIECE_BYTES = 16  # one tile row of one plane (kPieceBytes in corr.hip)
+TILE_PLANES = 4  # source planes sharing a tile (kTilePlanes)
+TILE_ROWS = 2  # rows of a plane in a tile (kTileRows)
+TILE_BYTES = PIECE_BYTES * TILE_PLANES * TILE_ROWS
 
 
 def tiled_plane(Hl, Wl, dtype):
-    """(Hp, Wp, tw) of a tiled volume plane: 4-row x tw-column tiles of 64 bytes."""
-    tw = TILE_BYTES // 4 // torch.empty((), dtype=dtype).element_size()
-    return (Hl + 3) // 4 * 4, (Wl + tw - 1) // tw * tw, tw
+    """(Hp, Wp, tw) of a tiled volume plane: Hl, Wl padded to TILE_ROWS x tw
+    blocks (tw columns = 16 bytes); the block of TILE_PLANES consecutive planes
+    is one TILE_BYTES tile."""
+    tw = PIECE_BYTES // torch.empty((), dtype=dtype).element_size()
+    return (Hl + TILE_ROWS - 1) // TILE_ROWS * TILE_ROWS, (Wl + tw - 1) // tw * tw, tw
+
+
+def _check_groups(n, what):
+    if n % TILE_PLANES:
+        raise RuntimeError(f"{what}: tiled levels group the planes of an edge by {TILE_PLANES}; "
+                           f"H*W must be a multiple of it (got {n})")
 
 
 def untile(level, Hl, Wl):
-    """Tiled level [B,H1,W1,Hp*Wp] -> planar [B,H1,W1,Hl,Wl] (copy)."""
+    """Tiled level [B,H1,W1,Hp*Wp] -> planar [B,H1,W1,Hl,Wl] (copy).  Planes p of a
+    group of TILE_PLANES share tiles of TILE_ROWS rows x tw columns, 16-byte piece
+    (row % TILE_ROWS) * TILE_PLANES + p % TILE_PLANES."""
     B, H1, W1, _ = level.shape
+    _check_groups(H1 * W1, "untile")
     Hp, Wp, tw = tiled_plane(Hl, Wl, level.dtype)
-    v = level.view(B, H1, W1, Hp // 4, Wp // tw, 4, tw).permute(0, 1, 2, 3, 5, 4, 6)
-    return v.reshape(B, H1, W1, Hp, Wp)[..., :Hl, :Wl].contiguous()
+    v = level.view(B * H1 * W1 // TILE_PLANES, Hp // TILE_ROWS, Wp // tw, TILE_ROWS, TILE_PLANES, tw)
+    return v.permute(0, 4, 1, 3, 2, 5).reshape(B, H1, W1, Hp, Wp)[..., :Hl, :Wl].contiguous()
 
 
 def tile(volume):
     """Planar [B,H1,W1,Hl,Wl] -> tiled [B,H1,W1,Hp*Wp] with zero padding (copy)."""
     B, H1, W1, Hl, Wl = volume.shape
+    _check_groups(H1 * W1, "tile")
     Hp, Wp, tw = tiled_plane(Hl, Wl, volume.dtype)
     pad = torch.zeros((B, H1, W1, Hp, Wp), dtype=volume.dtype, device=volume.device)
     pad[..., :Hl, :Wl] = volume
-    v = pad.view(B, H1, W1, Hp // 4, 4, Wp // tw, tw).permute(0, 1, 2, 3, 5, 4, 6)
-    return v.reshape(B, H1, W1, Hp * Wp).contiguous()
+    v = pad.view(B * H1 * W1 // TILE_PLANES, TILE_PLANES, Hp // TILE_ROWS, TILE_ROWS, Wp // tw, tw)
+    return v.permute(0, 2, 4, 3, 1, 5).reshape(B, H1, W1, Hp * Wp).contiguous()
 
 
 def corr_lookup_pyramid(levels, coords, radius, level_dims=None, layout="planar"):
@@ -203,7 +218,8 @@ def corr_build(fmap1, fmap2, num_levels=4, out_dtype=None, contraction_dtype=Non
     selects the MFMA operand type (default: the input dtype); float8_e4m3fn
     balances each channel between the two maps and scales each pixel row into
     e4m3, accumulating in FP32.  layout "tiled" stores each level's planes as
-    64-byte tiles, [E,H,W,Hp*Wp] (see tiled_plane / untile)."""
+    TILE_BYTES tiles shared by groups of TILE_PLANES planes, [E,H,W,Hp*Wp] (H*W a
+    multiple of TILE_PLANES; see tiled_plane / untile)."""
     _contig(fmap1=fmap1, fmap2=fmap2)
     _gpu(fmap1, fmap2)
     if fmap2.shape != fmap1.shape or fmap2.dtype != fmap1.dtype:
