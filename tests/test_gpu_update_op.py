@@ -228,22 +228,55 @@ def test_small_conv_heads_match_fp64(case):
     assert err <= 1e-5 * float(ref.abs().max()) + 1e-5
 
 
-@pytest.mark.parametrize("C,Cpad,HW", [(196, 224, 95), (4, 32, 64), (128, 128, 130), (300, 320, 7)])
-def test_layout_round_trip(C, Cpad, HW):
+# the 8 x 8 register-transpose form (HW a multiple of 8, 8-channel groups) and
+# the per-element form (other shapes), for f32 and f16 tensors on both sides
+@pytest.mark.parametrize("dt", [torch.float32, torch.float16])
+@pytest.mark.parametrize("C,Cpad,HW", [(196, 224, 95), (4, 32, 64), (128, 128, 130), (300, 320, 7), (128, 128, 320),
+                                       (196, 224, 64), (64, 72, 40), (16, 16, 8)])
+def test_layout_round_trip(C, Cpad, HW, dt):
     L = _lib()
     from vipe_amd.update_op import _vp
 
     g = torch.Generator().manual_seed(C)
     nimg = 3
-    x = torch.randn(nimg, C, HW, generator=g).to(DEV)
+    x = torch.randn(nimg, C, HW, generator=g).to(dt).to(DEV)
     r = _runner()
     h = r.to_nhwc(x, nimg, C, HW, Cpad)
-    back = torch.empty(nimg, C, HW, device=DEV)
-    L.check(r.lib.vh_uo_to_nchw(L.VH_F32, _vp(h), nimg, C, HW, Cpad, 0, _vp(back), r.stream), "to_nchw")
+    back = torch.empty(nimg, C, HW, device=DEV, dtype=dt)
+    code = L.VH_F32 if dt == torch.float32 else L.VH_F16
+    L.check(r.lib.vh_uo_to_nchw(code, _vp(h), nimg, C, HW, Cpad, 0, _vp(back), r.stream), "to_nchw")
+    # a channel window [8, 8 + k) of the padded rows
+    k = (C - 8) // 8 * 8
+    if k > 0:
+        win = torch.empty(nimg, k, HW, device=DEV, dtype=dt)
+        L.check(r.lib.vh_uo_to_nchw(code, _vp(h), nimg, k, HW, Cpad, 8, _vp(win), r.stream), "to_nchw window")
     torch.cuda.synchronize()
     assert torch.equal(h.view(nimg, HW, Cpad)[:, :, C:].float().cpu(), torch.zeros(nimg, HW, Cpad - C))
-    assert torch.equal(back.cpu(), x.half().float().cpu())
+    assert torch.equal(back.cpu(), x.half().to(dt).cpu())
     assert torch.equal(h.view(nimg, HW, Cpad)[:, :, :C].cpu(), x.half().permute(0, 2, 1).cpu())
+    if k > 0:
+        assert torch.equal(win.cpu(), x[:, 8:8 + k].half().to(dt).cpu())
+
+
+@pytest.mark.parametrize("C,HW", [(128, 5120), (128, 77), (64, 16), (70, 33)])
+def test_mean_over_pixels(C, HW):
+    """vh_uo_mean: per-image channel means of [nimg][HW][C] f32 (16-byte and
+    per-channel forms) against FP64, and the same bits on a second run."""
+    L = _lib()
+    from vipe_amd.update_op import _vp
+
+    g = torch.Generator().manual_seed(HW)
+    nimg = 5
+    x = torch.randn(nimg, HW, C, generator=g).to(DEV)
+    r = _runner()
+    out = torch.empty(nimg, C, device=DEV)
+    out2 = torch.empty(nimg, C, device=DEV)
+    L.check(r.lib.vh_uo_mean(_vp(x), nimg, HW, C, _vp(out), r.stream), "uo_mean")
+    L.check(r.lib.vh_uo_mean(_vp(x), nimg, HW, C, _vp(out2), r.stream), "uo_mean")
+    torch.cuda.synchronize()
+    ref = x.double().mean(1).cpu()
+    assert (out.double().cpu() - ref).abs().max().item() < 1e-6
+    assert torch.equal(out, out2)
 
 
 def _module(seed):

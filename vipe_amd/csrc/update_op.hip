@@ -421,75 +421,180 @@ __global__ __launch_bounds__(256) void uo_conv_small_kernel(const _Float16* __re
     }
 }
 
-// [nimg][C][HW] (f32 or f16) -> [nimg][HW][Cpad] f16, channels past C zero.
-// One workgroup per (image, 64 pixels); the tile goes through LDS so both the
-// reads (along pixels) and the writes (along channels) are contiguous.
+// Layout changes at the module boundary, as register transposes of 8 x 8
+// blocks.  Lane l of a wave takes channel group l % 8 (8 channels) and pixel
+// group l / 8 (8 pixels), so a wave covers 64 channels x 64 pixels: each of its
+// 8 loads reads one channel row of 64 pixels (whole lines) and each of its 8
+// stores one pixel row of 64 channels.  The vector form needs 8-pixel groups
+// inside an image and 16-byte aligned rows (checked by the host); other shapes
+// take one lane per output element.
+
+// Two 16-bit values as one dword: element e (0 or 1) of a and of b.
+__device__ __forceinline__ unsigned pick16(unsigned a, unsigned b, int e) {
+    return e ? ((a >> 16) | (b & 0xffff0000u)) : ((a & 0xffffu) | (b << 16));
+}
+
+__device__ __forceinline__ unsigned dw(const uint4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+// [nimg][C][HW] (f32 or f16) -> f16 [nimg][HW][Cpad], channels past C zero.
 template <typename T>
 __global__ __launch_bounds__(256) void uo_to_nhwc_kernel(const T* __restrict__ in, int nimg, int C, int HW, int Cpad,
                                                          _Float16* __restrict__ out) {
-    __shared__ _Float16 tile[64][257];
-    const int npb = (HW + 63) / 64;
-    const int img = blockIdx.x / npb;
-    const int p0 = (blockIdx.x - img * npb) * 64;
-    const int np = min(64, HW - p0);
-    for (int c0 = 0; c0 < Cpad; c0 += 256) {
-        const int cn = min(256, Cpad - c0);
-        for (int i = threadIdx.x; i < cn * 64; i += 256) {
-            const int c = i / 64, pp = i % 64;
-            float v = 0.f;
-            if (c0 + c < C && pp < np) v = (float)in[((int64_t)img * C + c0 + c) * HW + p0 + pp];
-            tile[pp][c] = (_Float16)v;
+    const int64_t P = (int64_t)nimg * HW;
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63) + 8 * (lane >> 3);
+    const int c0 = blockIdx.y * 64 + 8 * (lane & 7);
+    if (p >= P || c0 >= Cpad) return;
+    const int64_t img = p / HW, q = p - img * HW;
+    // v[i]: channel c0 + i at pixels p .. p + 7, as 8 halves (4 dwords)
+    uint4 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int c = c0 + i;
+        v[i] = make_uint4(0, 0, 0, 0);
+        if (c < C) {
+            const T* src = in + ((int64_t)img * C + c) * HW + q;
+            if constexpr (sizeof(T) == 2) {
+                v[i] = *reinterpret_cast<const uint4*>(src);
+            } else {
+                const float4 a = reinterpret_cast<const float4*>(src)[0], b = reinterpret_cast<const float4*>(src)[1];
+                const __half2 h0 = __floats2half2_rn(a.x, a.y), h1 = __floats2half2_rn(a.z, a.w);
+                const __half2 h2 = __floats2half2_rn(b.x, b.y), h3 = __floats2half2_rn(b.z, b.w);
+                v[i] = make_uint4(*reinterpret_cast<const unsigned*>(&h0), *reinterpret_cast<const unsigned*>(&h1),
+                                  *reinterpret_cast<const unsigned*>(&h2), *reinterpret_cast<const unsigned*>(&h3));
+            }
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < cn * 64; i += 256) {
-            const int pp = i / cn, c = i % cn;
-            if (pp < np) out[((int64_t)img * HW + p0 + pp) * Cpad + c0 + c] = tile[pp][c];
-        }
-        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        // pixel p + j: channels c0 .. c0 + 7
+        const uint4 o = make_uint4(pick16(dw(v[0], j >> 1), dw(v[1], j >> 1), j & 1),
+                                   pick16(dw(v[2], j >> 1), dw(v[3], j >> 1), j & 1),
+                                   pick16(dw(v[4], j >> 1), dw(v[5], j >> 1), j & 1),
+                                   pick16(dw(v[6], j >> 1), dw(v[7], j >> 1), j & 1));
+        *reinterpret_cast<uint4*>(out + (p + j) * Cpad + c0) = o;
     }
 }
 
-// [nimg][HW][stride] f16 (channels [off, off + C)) -> [nimg][C][HW] (f32 or f16).
+template <typename T>
+__global__ __launch_bounds__(256) void uo_to_nhwc_elem_kernel(const T* __restrict__ in, int nimg, int C, int HW,
+                                                              int Cpad, _Float16* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)nimg * HW * Cpad) return;
+    const int c = (int)(i % Cpad);
+    const int64_t p = i / Cpad, img = p / HW, q = p - img * HW;
+    out[i] = c < C ? (_Float16)(float)in[((int64_t)img * C + c) * HW + q] : (_Float16)0.f;
+}
+
+// f16 [nimg][HW][stride] (channels [off, off + C)) -> [nimg][C][HW] (f32 or f16).
 template <typename T>
 __global__ __launch_bounds__(256) void uo_to_nchw_kernel(const _Float16* __restrict__ in, int nimg, int C, int HW,
                                                          int stride, int off, T* __restrict__ out) {
-    __shared__ _Float16 tile[64][257];
-    const int npb = (HW + 63) / 64;
-    const int img = blockIdx.x / npb;
-    const int p0 = (blockIdx.x - img * npb) * 64;
-    const int np = min(64, HW - p0);
-    for (int c0 = 0; c0 < C; c0 += 256) {
-        const int cn = min(256, C - c0);
-        for (int i = threadIdx.x; i < cn * 64; i += 256) {
-            const int pp = i / cn, c = i % cn;
-            if (pp < np) tile[pp][c] = in[((int64_t)img * HW + p0 + pp) * stride + off + c0 + c];
+    const int64_t P = (int64_t)nimg * HW;
+    const int lane = threadIdx.x & 63;
+    const int64_t p = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63) + 8 * (lane >> 3);
+    const int c0 = blockIdx.y * 64 + 8 * (lane & 7);
+    if (p >= P || c0 >= C) return;
+    const int64_t img = p / HW, q = p - img * HW;
+    // w[j]: pixel p + j at channels c0 .. c0 + 7
+    uint4 w[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) w[j] = *reinterpret_cast<const uint4*>(in + (p + j) * stride + off + c0);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        // channel c0 + i: pixels p .. p + 7
+        const uint4 o = make_uint4(pick16(dw(w[0], i >> 1), dw(w[1], i >> 1), i & 1),
+                                   pick16(dw(w[2], i >> 1), dw(w[3], i >> 1), i & 1),
+                                   pick16(dw(w[4], i >> 1), dw(w[5], i >> 1), i & 1),
+                                   pick16(dw(w[6], i >> 1), dw(w[7], i >> 1), i & 1));
+        T* dst = out + ((int64_t)img * C + c0 + i) * HW + q;
+        if constexpr (sizeof(T) == 2) {
+            *reinterpret_cast<uint4*>(dst) = o;
+        } else {
+            const unsigned d[4] = {o.x, o.y, o.z, o.w};
+            float f[8];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float2 t = __half22float2(*reinterpret_cast<const __half2*>(&d[k]));
+                f[2 * k] = t.x;
+                f[2 * k + 1] = t.y;
+            }
+            reinterpret_cast<float4*>(dst)[0] = make_float4(f[0], f[1], f[2], f[3]);
+            reinterpret_cast<float4*>(dst)[1] = make_float4(f[4], f[5], f[6], f[7]);
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < cn * 64; i += 256) {
-            const int c = i / 64, pp = i % 64;
-            if (pp < np) out[((int64_t)img * C + c0 + c) * HW + p0 + pp] = (T)(float)tile[pp][c];
-        }
-        __syncthreads();
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void uo_to_nchw_elem_kernel(const _Float16* __restrict__ in, int nimg, int C, int HW,
+                                                              int stride, int off, T* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)nimg * C * HW) return;
+    const int64_t q = i % HW, ic = i / HW, img = ic / C;
+    const int c = (int)(ic - img * C);
+    out[i] = (T)(float)in[((int64_t)img * HW + q) * stride + off + c];
+}
+
 // Per-image mean over the pixels of [nimg][HW][C] f32: one workgroup per
-// (image, 64 channels); each thread sums a fixed strided set of pixels, then a
-// fixed-order tree, so the result is the same on every run.
+// (image, 64 channels).  With V4 a lane reads 4 channels as one 16-byte load, 16
+// lanes cover the 64 channels of a pixel and the workgroup's 16 lane groups take
+// pixels g, g + 16, ...; each lane keeps 4 running sums over its pixels (taken
+// in turn), and fixed-order trees combine them, so the result is the same on
+// every run.  Without V4 (C not a multiple of 4) a lane takes one channel.
+template <bool V4>
 __global__ __launch_bounds__(256) void uo_mean_kernel(const float* __restrict__ in, int nimg, int HW, int C,
                                                       float* __restrict__ out) {
-    __shared__ float part[4][64];
+    constexpr int NG = V4 ? 16 : 4;  // pixel groups of the workgroup
+    __shared__ float part[NG][64];
     const int ncb = (C + 63) / 64;
     const int img = blockIdx.x / ncb;
-    const int c = (blockIdx.x - img * ncb) * 64 + (threadIdx.x & 63);
-    const int g = threadIdx.x >> 6;
-    float s = 0.f;
-    if (c < C)
-        for (int p = g; p < HW; p += 4) s += in[((int64_t)img * HW + p) * C + c];
-    part[g][threadIdx.x & 63] = s;
+    const int cb = (blockIdx.x - img * ncb) * 64;
+    const float* base = in + (int64_t)img * HW * C;
+    if constexpr (V4) {
+        const int g = threadIdx.x >> 4, c = cb + 4 * (threadIdx.x & 15);
+        float4 acc[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) acc[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c < C) {
+            int p = g;
+            for (; p + 3 * NG < HW; p += 4 * NG) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float4 v = *reinterpret_cast<const float4*>(base + (int64_t)(p + u * NG) * C + c);
+                    acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 3; u++) {
+                const int pu = p + u * NG;
+                if (pu < HW) {
+                    const float4 v = *reinterpret_cast<const float4*>(base + (int64_t)pu * C + c);
+                    acc[u].x += v.x; acc[u].y += v.y; acc[u].z += v.z; acc[u].w += v.w;
+                }
+            }
+        }
+        const float4 t = make_float4((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x), (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y),
+                                     (acc[0].z + acc[1].z) + (acc[2].z + acc[3].z), (acc[0].w + acc[1].w) + (acc[2].w + acc[3].w));
+        const int cl = 4 * (threadIdx.x & 15);
+        part[g][cl] = t.x; part[g][cl + 1] = t.y; part[g][cl + 2] = t.z; part[g][cl + 3] = t.w;
+    } else {
+        const int g = threadIdx.x >> 6, c = cb + (threadIdx.x & 63);
+        float sum = 0.f;
+        if (c < C)
+            for (int p = g; p < HW; p += NG) sum += base[(int64_t)p * C + c];
+        part[g][threadIdx.x & 63] = sum;
+    }
     __syncthreads();
-    if (g == 0 && c < C) out[(int64_t)img * C + c] = ((part[0][threadIdx.x] + part[1][threadIdx.x]) +
-                                                      (part[2][threadIdx.x] + part[3][threadIdx.x])) / (float)HW;
+    if (threadIdx.x < 64 && cb + (int)threadIdx.x < C) {
+        float r[NG];
+#pragma unroll
+        for (int k = 0; k < NG; k++) r[k] = part[k][threadIdx.x];
+#pragma unroll
+        for (int w = NG / 2; w >= 1; w /= 2)
+#pragma unroll
+            for (int k = 0; k < w; k++) r[k] = r[k] + r[k + w];
+        out[(int64_t)img * C + cb + threadIdx.x] = r[0] / (float)HW;
+    }
 }
 
 // out[img][k] = sum_c w[k][c] in[img][c] + b[k] (1x1 convolutions of a 1x1 map).
@@ -635,41 +740,71 @@ int vh_uo_conv_small(const void* in, int in_stride, int in_off, int cin, const f
 }
 
 int vh_uo_to_nhwc(int dtype, const void* in, int nimg, int C, int HW, int Cpad, void* out, void* stream) {
-    if (Cpad < C) return fail(VH_ERR_ARG, "uo_to_nhwc: Cpad %d < C %d", Cpad, C);
-    if ((int64_t)nimg * HW == 0) return VH_OK;
-    const dim3 grid((unsigned)((int64_t)nimg * ((HW + 63) / 64)));
-    if (dtype == VH_F32)
-        hipLaunchKernelGGL(uo_to_nhwc_kernel<float>, grid, dim3(256), 0, (hipStream_t)stream, (const float*)in, nimg,
-                           C, HW, Cpad, (_Float16*)out);
-    else if (dtype == VH_F16)
-        hipLaunchKernelGGL(uo_to_nhwc_kernel<_Float16>, grid, dim3(256), 0, (hipStream_t)stream,
-                           (const _Float16*)in, nimg, C, HW, Cpad, (_Float16*)out);
-    else
-        return fail(VH_ERR_UNSUPPORTED, "uo_to_nhwc: dtype %d", dtype);
+    if (Cpad < C || C < 0 || nimg < 0 || HW < 0) return fail(VH_ERR_ARG, "uo_to_nhwc: Cpad %d < C %d", Cpad, C);
+    if (dtype != VH_F32 && dtype != VH_F16) return fail(VH_ERR_UNSUPPORTED, "uo_to_nhwc: dtype %d", dtype);
+    const int64_t P = (int64_t)nimg * HW;
+    if (P * Cpad == 0) return VH_OK;
+    const hipStream_t st = (hipStream_t)stream;
+    // the 8 x 8 transpose: 8-pixel groups inside an image, 16-byte rows
+    const bool vec = HW % 8 == 0 && Cpad % 8 == 0 && ((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0;
+    if (vec) {
+        const dim3 grid(ceil_div(P, 256), ceil_div(Cpad, 64));
+        if (dtype == VH_F32)
+            hipLaunchKernelGGL(uo_to_nhwc_kernel<float>, grid, dim3(256), 0, st, (const float*)in, nimg, C, HW, Cpad,
+                               (_Float16*)out);
+        else
+            hipLaunchKernelGGL(uo_to_nhwc_kernel<_Float16>, grid, dim3(256), 0, st, (const _Float16*)in, nimg, C, HW,
+                               Cpad, (_Float16*)out);
+    } else {
+        const dim3 grid(ceil_div(P * Cpad, 256));
+        if (dtype == VH_F32)
+            hipLaunchKernelGGL(uo_to_nhwc_elem_kernel<float>, grid, dim3(256), 0, st, (const float*)in, nimg, C, HW,
+                               Cpad, (_Float16*)out);
+        else
+            hipLaunchKernelGGL(uo_to_nhwc_elem_kernel<_Float16>, grid, dim3(256), 0, st, (const _Float16*)in, nimg, C,
+                               HW, Cpad, (_Float16*)out);
+    }
     VH_LAUNCH_CHECK("uo_to_nhwc");
     return VH_OK;
 }
 
 int vh_uo_to_nchw(int dtype, const void* in, int nimg, int C, int HW, int stride, int off, void* out, void* stream) {
-    if (off + C > stride) return fail(VH_ERR_ARG, "uo_to_nchw: channels [%d, %d) of %d", off, off + C, stride);
-    if ((int64_t)nimg * HW == 0) return VH_OK;
-    const dim3 grid((unsigned)((int64_t)nimg * ((HW + 63) / 64)));
-    if (dtype == VH_F32)
-        hipLaunchKernelGGL(uo_to_nchw_kernel<float>, grid, dim3(256), 0, (hipStream_t)stream, (const _Float16*)in,
-                           nimg, C, HW, stride, off, (float*)out);
-    else if (dtype == VH_F16)
-        hipLaunchKernelGGL(uo_to_nchw_kernel<_Float16>, grid, dim3(256), 0, (hipStream_t)stream,
-                           (const _Float16*)in, nimg, C, HW, stride, off, (_Float16*)out);
-    else
-        return fail(VH_ERR_UNSUPPORTED, "uo_to_nchw: dtype %d", dtype);
+    if (off < 0 || C < 0 || off + C > stride || nimg < 0 || HW < 0)
+        return fail(VH_ERR_ARG, "uo_to_nchw: channels [%d, %d) of %d", off, off + C, stride);
+    if (dtype != VH_F32 && dtype != VH_F16) return fail(VH_ERR_UNSUPPORTED, "uo_to_nchw: dtype %d", dtype);
+    const int64_t P = (int64_t)nimg * HW;
+    if (P * C == 0) return VH_OK;
+    const hipStream_t st = (hipStream_t)stream;
+    const bool vec = HW % 8 == 0 && C % 8 == 0 && stride % 8 == 0 && off % 8 == 0 && ((uintptr_t)in & 15) == 0 &&
+                     ((uintptr_t)out & 15) == 0;
+    if (vec) {
+        const dim3 grid(ceil_div(P, 256), ceil_div(C, 64));
+        if (dtype == VH_F32)
+            hipLaunchKernelGGL(uo_to_nchw_kernel<float>, grid, dim3(256), 0, st, (const _Float16*)in, nimg, C, HW,
+                               stride, off, (float*)out);
+        else
+            hipLaunchKernelGGL(uo_to_nchw_kernel<_Float16>, grid, dim3(256), 0, st, (const _Float16*)in, nimg, C, HW,
+                               stride, off, (_Float16*)out);
+    } else {
+        const dim3 grid(ceil_div(P * C, 256));
+        if (dtype == VH_F32)
+            hipLaunchKernelGGL(uo_to_nchw_elem_kernel<float>, grid, dim3(256), 0, st, (const _Float16*)in, nimg, C, HW,
+                               stride, off, (float*)out);
+        else
+            hipLaunchKernelGGL(uo_to_nchw_elem_kernel<_Float16>, grid, dim3(256), 0, st, (const _Float16*)in, nimg, C,
+                               HW, stride, off, (_Float16*)out);
+    }
     VH_LAUNCH_CHECK("uo_to_nchw");
     return VH_OK;
 }
 
 int vh_uo_mean(const float* in, int nimg, int HW, int C, float* out, void* stream) {
     if ((int64_t)nimg * C == 0) return VH_OK;
-    hipLaunchKernelGGL(uo_mean_kernel, dim3((unsigned)((int64_t)nimg * ((C + 63) / 64))), dim3(256), 0,
-                       (hipStream_t)stream, in, nimg, HW, C, out);
+    const dim3 grid((unsigned)((int64_t)nimg * ((C + 63) / 64)));
+    if (C % 4 == 0 && ((uintptr_t)in & 15) == 0)
+        hipLaunchKernelGGL(uo_mean_kernel<true>, grid, dim3(256), 0, (hipStream_t)stream, in, nimg, HW, C, out);
+    else
+        hipLaunchKernelGGL(uo_mean_kernel<false>, grid, dim3(256), 0, (hipStream_t)stream, in, nimg, HW, C, out);
     VH_LAUNCH_CHECK("uo_mean");
     return VH_OK;
 }
