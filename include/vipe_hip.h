@@ -147,6 +147,26 @@ size_t vh_altcorr_backward_workspace_size(int64_t B, int N, int H1, int W1);
 int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int W2, int C, const void* fmap1,
                         const void* fmap2, const float* coords, const void* corr_grad, int radius, void* fmap1_grad,
                         void* fmap2_grad, void* workspace, void* stream);
+/* Feature maps for the on-the-fly correlation and the feature gradients of the
+ * volume lookup.  vh_feat_to_nhwc: in [E][C][HW] (F32, F64, F16 or BF16) times
+ * scale -> out [E][HW][C] in out_dtype (F32 or dtype; AltCorrBlock's fmaps / 4
+ * in channels-last, droid_net.py:146-160).  vh_pool2x2_nhwc: [E][H][W][C] ->
+ * [E][H/2][W/2][C] (F32, F64, F16 or BF16), each output the mean of its 2x2
+ * block taken in FP32 (FP64 for F64) (avg_pool2d(2, stride 2) on channels-last
+ * maps, droid_net.py:157).  vh_corr_feature_grad: the gradients
+ * of the two feature maps of CorrBlock's lookup (the adjoint of the volume,
+ * its pooled pyramid and the lookup, droid_net.py:56-102, which the reference
+ * gets from torch autograd), from the per-level altcorr backward outputs:
+ * df1[e][c][y][x] = scale * sum_l g1[l][e][y][x][c] and
+ * df2[e][c][y][x] = scale * sum_l 4^-l g2[l][e][y >> l][x >> l][c] (levels whose
+ * (y >> l, x >> l) lie inside H >> l, W >> l), stored NCHW in dtype (F32, F16 or
+ * BF16); g1[l] f32 [E][H][W][C], g2[l] f32 [E][H >> l][W >> l][C], 1 to 4 levels.
+ * E is at most 65535. */
+int vh_feat_to_nhwc(int dtype, const void* in, int64_t E, int C, int HW, float scale, int out_dtype, void* out,
+                    void* stream);
+int vh_pool2x2_nhwc(int dtype, const void* in, int64_t E, int H, int W, int C, void* out, void* stream);
+int vh_corr_feature_grad(int dtype, int64_t E, int C, int H, int W, int num_levels, const float* const* g1,
+                         const float* const* g2, float scale, void* df1, void* df2, void* stream);
 /* altcorr_index_forward (droid.cpp:50, altcorr_kernel.cu:140-240, 394-424).
  * fmap1 [B,Nf,H,W,C]; levels[l] [B,Nf,Hl[l],Wl[l],C]; coords [B,M,H,W,2] f32;
  * ii, jj [M] int64 (device); features F16/F32/F64 (sums in FP64 for F64);

@@ -404,3 +404,66 @@ def test_altcorr_backward_gather_is_deterministic(C, W, conc):
     torch.cuda.synchronize()
     assert (k1 - g1).abs().max().item() <= 1e-6 * max(1.0, g1.abs().max().item())
     assert (k2 - g2).abs().max().item() <= 1e-5 * max(1.0, g2.abs().max().item())
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float16, torch.bfloat16, torch.float64])
+def test_feature_maps_channels_last_and_pooled(dt):
+    """feat_to_nhwc / pool2x2_nhwc against torch: the scaled channels-last copy is
+    exact; the 2x2 means match avg_pool2d to the map type's rounding."""
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(11)
+    f = torch.randn(3, 40, 13, 19, generator=g).to(dt)
+    for out_dt in (torch.float32, dt):
+        x = D.feat_to_nhwc(f.cuda(), 0.25, out_dtype=out_dt).cpu()
+        assert x.dtype == out_dt
+        assert torch.equal(x, (f.to(out_dt) * 0.25).permute(0, 2, 3, 1))
+    lvl = D.feat_to_nhwc(f.cuda(), 0.25, out_dtype=dt)
+    ref = f.double() * 0.25
+    tol = {torch.float32: 1e-6, torch.float64: 1e-12, torch.float16: 2e-3, torch.bfloat16: 1.6e-2}[dt]
+    for _ in range(3):
+        lvl = D.pool2x2_nhwc(lvl)
+        ref = torch.nn.functional.avg_pool2d(ref, 2, stride=2)
+        assert lvl.dtype == dt and lvl.shape == ref.permute(0, 2, 3, 1).shape
+        assert (lvl.cpu().double() - ref.permute(0, 2, 3, 1)).abs().max().item() <= tol
+
+
+@pytest.mark.parametrize("L,H,W", [(4, 12, 16), (3, 13, 19), (1, 8, 8)])
+def test_feature_grad_combination(L, H, W):
+    """corr_feature_grad against its torch restatement: df1 = s * sum_l g1_l and
+    df2 = s * sum_l 4^-l g2_l spread over the 2^l x 2^l blocks (the level's
+    floor-dropped rows and columns get nothing), both NCHW."""
+    from vipe_amd.ext import droid_net_ext as D
+
+    g = torch.Generator().manual_seed(L * H)
+    E, C = 2, 36
+    g1 = [torch.randn(E, H, W, C, generator=g) for _ in range(L)]
+    g2 = [torch.randn(E, H >> l, W >> l, C, generator=g) for l in range(L)]
+    for dt in (torch.float32, torch.float16):
+        df1, df2 = D.corr_feature_grad([t.cuda() for t in g1], [t.cuda() for t in g2], 0.25, dt)
+        r1 = sum(g1).permute(0, 3, 1, 2) * 0.25
+        r2 = torch.zeros(E, C, H, W)
+        for lv in range(L):
+            s = 2 ** lv
+            up = (g2[lv].permute(0, 3, 1, 2) / 4.0 ** lv).repeat_interleave(s, 2).repeat_interleave(s, 3)
+            r2[:, :, :up.shape[2], :up.shape[3]] += up
+        r2 *= 0.25
+        tol = 1e-6 if dt == torch.float32 else 4e-3
+        assert df1.dtype == dt and (df1.cpu().float() - r1).abs().max().item() <= tol * max(1.0, r1.abs().max().item())
+        assert (df2.cpu().float() - r2).abs().max().item() <= tol * max(1.0, r2.abs().max().item())
+
+
+def test_altcorr_block_pyramid_keeps_the_feature_dtype():
+    from vipe_amd.corr import AltCorrBlock
+
+    g = torch.Generator().manual_seed(12)
+    f = torch.randn(1, 3, 32, 12, 16, generator=g)
+    for dt in (torch.float32, torch.float16):
+        blk = AltCorrBlock(f.to(dt).cuda(), num_levels=4, radius=3)
+        ref = f.reshape(3, 32, 12, 16).to(dt).double() / 4.0
+        for i, lvl in enumerate(blk.pyramid):
+            assert lvl.dtype == dt and lvl.shape == (1, 3, 12 >> i, 16 >> i, 32)
+            if i > 0:
+                ref = torch.nn.functional.avg_pool2d(ref, 2, stride=2)
+            tol = 1e-6 if dt == torch.float32 else 2e-3
+            assert (lvl.cpu().double().reshape(3, 12 >> i, 16 >> i, 32) - ref.permute(0, 2, 3, 1)).abs().max() <= tol

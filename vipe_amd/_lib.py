@@ -60,6 +60,9 @@ _SIGS = {
     "vh_corr_build": (i32, [i32, i32, i32, i64, i32, i32, i32, vp, vp, i32, vp, i32, vp, vp]),
     "vh_altcorr_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, i32, vp, vp]),
     "vh_altcorr_backward": (i32, [i32, i64, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, vp, vp]),
+    "vh_feat_to_nhwc": (i32, [i32, vp, i64, i32, i32, f32, i32, vp, vp]),
+    "vh_pool2x2_nhwc": (i32, [i32, vp, i64, i32, i32, i32, vp, vp]),
+    "vh_corr_feature_grad": (i32, [i32, i64, i32, i32, i32, i32, vp, vp, f32, vp, vp, vp]),
     "vh_altcorr_backward_workspace_size": (C.c_size_t, [i64, i32, i32, i32]),
     "vh_altcorr_index_forward": (i32, [i32, i64, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, i32,
                                        vp, vp]),

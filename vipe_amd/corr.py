@@ -91,25 +91,21 @@ class CorrLookupFeatures(torch.autograd.Function):
         r, L = ctx.radius, ctx.num_levels
         rd2 = (2 * r + 1) ** 2
         g = grad_output.float()
-        F1 = (f1.float() / 4.0).permute(0, 2, 3, 1).contiguous()
-        df1 = torch.zeros((E, H, W, Cc), dtype=torch.float32, device=f1.device)
-        df2 = torch.zeros((E, Cc, H, W), dtype=torch.float32, device=f1.device)
-        f2l = f2.float()
+        F1 = droid_net_ext.feat_to_nhwc(f1, 0.25)
+        F2 = droid_net_ext.feat_to_nhwc(f2, 0.25)
+        g1s, g2s = [], []
         for lvl in range(L):
             if lvl > 0:
-                f2l = F.avg_pool2d(f2l, 2, stride=2)
-            F2 = (f2l / 4.0).permute(0, 2, 3, 1).contiguous()
+                F2 = droid_net_ext.pool2x2_nhwc(F2)
             gl = g[:, lvl * rd2:(lvl + 1) * rd2].reshape(E, 1, rd2, H, W).contiguous()
             cl = (coords / 2 ** lvl).reshape(E, 1, H, W, 2).contiguous()
             g1, g2, _ = droid_net_ext.altcorr_backward(F1, F2, cl, gl, r)
-            df1 += g1
-            s = 2 ** lvl
-            up = (g2.permute(0, 3, 1, 2) / 4.0 ** lvl).repeat_interleave(s, 2).repeat_interleave(s, 3)
-            df2[:, :, :up.shape[2], :up.shape[3]] += up
-        df1 = (df1 / 4.0).permute(0, 3, 1, 2).to(f1.dtype)
-        df2 = (df2 / 4.0).to(f2.dtype)
-        return (None, None, None, None, df1 if ctx.needs_input_grad[4] else None,
-                df2 if ctx.needs_input_grad[5] else None, *([None] * L))
+            g1s.append(g1)
+            g2s.append(g2)
+        dt = f1.dtype if f1.dtype in (torch.float32, torch.float16, torch.bfloat16) else torch.float32
+        df1, df2 = droid_net_ext.corr_feature_grad(g1s, g2s, 0.25, dt)
+        return (None, None, None, None, df1.to(f1.dtype) if ctx.needs_input_grad[4] else None,
+                df2.to(f2.dtype) if ctx.needs_input_grad[5] else None, *([None] * L))
 
 
 class CorrBlock:
@@ -215,13 +211,14 @@ class AltCorrBlock:
         self.num_levels = num_levels
         self.radius = radius
         B, N, C, H, W = fmaps.shape
-        fmaps = fmaps.view(B * N, C, H, W) / 4.0
+        # fmaps / 4 channels-last, then 2x2 means per level (droid_net.py:146-160)
+        f = fmaps.reshape(B * N, C, H, W).contiguous()
+        lvl = droid_net_ext.feat_to_nhwc(f, 0.25, out_dtype=f.dtype)
         self.pyramid = []
         for i in range(self.num_levels):
-            sz = (B, N, H // 2**i, W // 2**i, C)
-            fmap_lvl = fmaps.permute(0, 2, 3, 1).contiguous()
-            self.pyramid.append(fmap_lvl.view(*sz))
-            fmaps = F.avg_pool2d(fmaps, 2, stride=2)
+            if i > 0:
+                lvl = droid_net_ext.pool2x2_nhwc(lvl)
+            self.pyramid.append(lvl.view(B, N, H // 2**i, W // 2**i, C))
 
     def corr_fn_reference(self, coords, ii, jj):
         B, N, H, W, S, _ = coords.shape

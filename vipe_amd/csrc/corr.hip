@@ -1764,6 +1764,104 @@ __global__ __launch_bounds__(256) void pool2x2_kernel(const T* __restrict__ in, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Feature maps for the on-the-fly correlation and for the feature gradients of
+// the volume lookup (AltCorrBlock's pyramid, droid_net.py:146-160; the adjoint
+// of CorrBlock's pyramid, droid_net.py:56-102).
+//   feat_to_nhwc_kernel     [E][C][HW] (f32 / f16 / bf16) * scale -> [E][HW][C]
+//                           (f32 or the input type)
+//   pool2x2_nhwc_kernel     [E][H][W][C] -> [E][H/2][W/2][C], 2x2 means in FP32,
+//                           rounded to the map type
+//   corr_feature_grad_kernel  the level gradients of fmap1 (NHWC, full size)
+//                           summed, and of fmap2 (NHWC, level size) spread over
+//                           their 2^l x 2^l blocks with weight 4^-l, both times
+//                           scale and stored NCHW.
+// The transposes go through 32-channel x 64-pixel LDS tiles, so the NCHW side is
+// read or written along pixels and the NHWC side along channels.
+// ---------------------------------------------------------------------------
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void feat_to_nhwc_kernel(const T* __restrict__ in, int C, int HW, float scale,
+                                                           TO* __restrict__ out) {
+    using A = typename Acc<TO>::type;
+    __shared__ A tile[32][65];
+    const int p0 = blockIdx.x * 64, c0 = blockIdx.y * 32;
+    const int64_t e = blockIdx.z;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int c = ty + 4 * k, p = p0 + tx;
+        tile[c][tx] = (c0 + c < C && p < HW) ? load_as<A>(in + (e * C + c0 + c) * HW + p) * (A)scale : (A)0;
+    }
+    __syncthreads();
+    const int tc = threadIdx.x & 31, tp = threadIdx.x >> 5;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int p = tp + 8 * k;
+        if (c0 + tc < C && p0 + p < HW) out[(e * HW + p0 + p) * C + c0 + tc] = store_from<TO>(tile[tc][p]);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void pool2x2_nhwc_kernel(const T* __restrict__ in, int64_t E, int H, int W, int C,
+                                                           T* __restrict__ out) {
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n = E * Ho * Wo * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const int64_t q = i / C;
+        const int x = (int)(q % Wo);
+        const int64_t r = q / Wo;
+        const int y = (int)(r % Ho);
+        const int64_t e = r / Ho;
+        using A = typename Acc<T>::type;
+        const T* s = in + ((e * H + 2 * y) * W + 2 * x) * C + c;
+        out[i] = store_from<T>((((load_as<A>(s) + load_as<A>(s + C)) + load_as<A>(s + (int64_t)W * C)) +
+                                load_as<A>(s + (int64_t)W * C + C)) * (A)0.25);
+    }
+}
+
+struct FeatGradLevels {
+    const float* g1[4];  // [E][H][W][C]
+    const float* g2[4];  // [E][H >> l][W >> l][C]
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void corr_feature_grad_kernel(FeatGradLevels gl, int L, int C, int H, int W,
+                                                                float scale, T* __restrict__ df1, T* __restrict__ df2) {
+    __shared__ float t1[32][65], t2[32][65];
+    const int HW = H * W;
+    const int p0 = blockIdx.x * 64, c0 = blockIdx.y * 32;
+    const int64_t e = blockIdx.z;
+    const int tc = threadIdx.x & 31, tp = threadIdx.x >> 5;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int p = p0 + tp + 8 * k, c = c0 + tc;
+        float s1 = 0.f, s2 = 0.f;
+        if (p < HW && c < C) {
+            const int y = p / W, x = p - y * W;
+            float w = 1.f;
+            for (int l = 0; l < L; l++, w *= 0.25f) {
+                s1 += gl.g1[l][(e * HW + p) * C + c];
+                const int Hl = H >> l, Wl = W >> l, yl = y >> l, xl = x >> l;
+                if (yl < Hl && xl < Wl) s2 += w * gl.g2[l][((e * Hl + yl) * Wl + xl) * C + c];
+            }
+        }
+        t1[tc][tp + 8 * k] = s1 * scale;
+        t2[tc][tp + 8 * k] = s2 * scale;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int c = ty + 4 * k;
+        if (c0 + c < C && p0 + tx < HW) {
+            const int64_t o = (e * C + c0 + c) * HW + p0 + tx;
+            df1[o] = from_f<T>(t1[c][tx]);
+            df2[o] = from_f<T>(t2[c][tx]);
+        }
+    }
+}
+
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 inline int chan_pad(int C) { return (C + 15) / 16 * 16; }
 
@@ -2189,6 +2287,102 @@ int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int 
         default: return fail(VH_ERR_UNSUPPORTED, "altcorr_forward: dtype %d", dtype);
     }
     VH_LAUNCH_CHECK("altcorr_forward");
+    return VH_OK;
+}
+
+int vh_feat_to_nhwc(int dtype, const void* in, int64_t E, int C, int HW, float scale, int out_dtype, void* out,
+                    void* stream) {
+    if (E < 0 || C < 0 || HW < 0) return fail(VH_ERR_ARG, "feat_to_nhwc: negative size");
+    if (out_dtype != VH_F32 && out_dtype != dtype)
+        return fail(VH_ERR_UNSUPPORTED, "feat_to_nhwc: output dtype %d (F32 or the input's)", out_dtype);
+    if (E * C * HW == 0) return VH_OK;
+    if (E > 65535) return fail(VH_ERR_UNSUPPORTED, "feat_to_nhwc: %lld maps (at most 65535)", (long long)E);
+    const dim3 grid(ceil_div(HW, 64), ceil_div(C, 32), (unsigned)E);
+    hipStream_t s = (hipStream_t)stream;
+    const bool f32 = out_dtype == VH_F32;
+#define FEAT_LAUNCH(T, TO) \
+    hipLaunchKernelGGL((feat_to_nhwc_kernel<T, TO>), grid, dim3(256), 0, s, (const T*)in, C, HW, scale, (TO*)out)
+    switch (dtype) {
+        case VH_F32: FEAT_LAUNCH(float, float); break;
+        case VH_F64:
+            if (f32) FEAT_LAUNCH(double, float);
+            else FEAT_LAUNCH(double, double);
+            break;
+        case VH_F16:
+            if (f32) FEAT_LAUNCH(__half, float);
+            else FEAT_LAUNCH(__half, __half);
+            break;
+        case VH_BF16:
+            if (f32) FEAT_LAUNCH(__hip_bfloat16, float);
+            else FEAT_LAUNCH(__hip_bfloat16, __hip_bfloat16);
+            break;
+        default: return fail(VH_ERR_UNSUPPORTED, "feat_to_nhwc: dtype %d", dtype);
+    }
+#undef FEAT_LAUNCH
+    VH_LAUNCH_CHECK("feat_to_nhwc");
+    return VH_OK;
+}
+
+int vh_pool2x2_nhwc(int dtype, const void* in, int64_t E, int H, int W, int C, void* out, void* stream) {
+    if (E < 0 || H < 0 || W < 0 || C < 0) return fail(VH_ERR_ARG, "pool2x2_nhwc: negative size");
+    const int64_t n = E * (H / 2) * (int64_t)(W / 2) * C;
+    if (n == 0) return VH_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 1 << 20) blocks = 1 << 20;
+    hipStream_t s = (hipStream_t)stream;
+    switch (dtype) {
+        case VH_F32:
+            hipLaunchKernelGGL(pool2x2_nhwc_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, (const float*)in, E,
+                               H, W, C, (float*)out);
+            break;
+        case VH_F16:
+            hipLaunchKernelGGL(pool2x2_nhwc_kernel<__half>, dim3((unsigned)blocks), dim3(256), 0, s, (const __half*)in,
+                               E, H, W, C, (__half*)out);
+            break;
+        case VH_F64:
+            hipLaunchKernelGGL(pool2x2_nhwc_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, (const double*)in,
+                               E, H, W, C, (double*)out);
+            break;
+        case VH_BF16:
+            hipLaunchKernelGGL(pool2x2_nhwc_kernel<__hip_bfloat16>, dim3((unsigned)blocks), dim3(256), 0, s,
+                               (const __hip_bfloat16*)in, E, H, W, C, (__hip_bfloat16*)out);
+            break;
+        default: return fail(VH_ERR_UNSUPPORTED, "pool2x2_nhwc: dtype %d", dtype);
+    }
+    VH_LAUNCH_CHECK("pool2x2_nhwc");
+    return VH_OK;
+}
+
+int vh_corr_feature_grad(int dtype, int64_t E, int C, int H, int W, int num_levels, const float* const* g1,
+                         const float* const* g2, float scale, void* df1, void* df2, void* stream) {
+    if (E < 0 || C < 0 || H < 0 || W < 0) return fail(VH_ERR_ARG, "corr_feature_grad: negative size");
+    if (num_levels < 1 || num_levels > 4) return fail(VH_ERR_ARG, "corr_feature_grad: 1..4 levels");
+    if (E * C * H * W == 0) return VH_OK;
+    if (E > 65535) return fail(VH_ERR_UNSUPPORTED, "corr_feature_grad: %lld maps (at most 65535)", (long long)E);
+    FeatGradLevels gl{};
+    for (int l = 0; l < num_levels; l++) {
+        if (!g1[l] || (!g2[l] && (H >> l) * (W >> l) > 0)) return fail(VH_ERR_ARG, "corr_feature_grad: level %d", l);
+        gl.g1[l] = g1[l];
+        gl.g2[l] = g2[l];
+    }
+    const dim3 grid(ceil_div((int64_t)H * W, 64), ceil_div(C, 32), (unsigned)E);
+    hipStream_t s = (hipStream_t)stream;
+    switch (dtype) {
+        case VH_F32:
+            hipLaunchKernelGGL(corr_feature_grad_kernel<float>, grid, dim3(256), 0, s, gl, num_levels, C, H, W, scale,
+                               (float*)df1, (float*)df2);
+            break;
+        case VH_F16:
+            hipLaunchKernelGGL(corr_feature_grad_kernel<__half>, grid, dim3(256), 0, s, gl, num_levels, C, H, W, scale,
+                               (__half*)df1, (__half*)df2);
+            break;
+        case VH_BF16:
+            hipLaunchKernelGGL(corr_feature_grad_kernel<__hip_bfloat16>, grid, dim3(256), 0, s, gl, num_levels, C, H, W,
+                               scale, (__hip_bfloat16*)df1, (__hip_bfloat16*)df2);
+            break;
+        default: return fail(VH_ERR_UNSUPPORTED, "corr_feature_grad: dtype %d", dtype);
+    }
+    VH_LAUNCH_CHECK("corr_feature_grad");
     return VH_OK;
 }
 

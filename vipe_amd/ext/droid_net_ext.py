@@ -112,6 +112,59 @@ def altcorr_backward(fmap1, fmap2, coords, corr_grad, radius):
     return [fmap1_grad, fmap2_grad, coords_grad]
 
 
+_FEAT_DT = (torch.float32, torch.float16, torch.bfloat16, torch.float64)
+
+
+def feat_to_nhwc(fmap, scale=1.0, out_dtype=torch.float32):
+    """[E, C, H, W] (f32 / f64 / f16 / bf16) -> [E, H, W, C] times scale, in
+    out_dtype (float32 or the map's dtype)."""
+    _contig(fmap=fmap)
+    _gpu(fmap)
+    E, Cc, H, W = fmap.shape
+    out = torch.empty((E, H, W, Cc), dtype=out_dtype, device=fmap.device)
+    with torch.cuda.device(fmap.device):
+        rc = _lib.load().vh_feat_to_nhwc(_code(fmap, _FEAT_DT), _lib.ptr(fmap), E, Cc, H * W, float(scale),
+                                         _code(out, _FEAT_DT), _lib.ptr(out), _lib.stream_of(fmap))
+    _lib.check(rc, "feat_to_nhwc")
+    return out
+
+
+def pool2x2_nhwc(fmap):
+    """[E, H, W, C] -> [E, H/2, W/2, C]: means of 2x2 blocks (floor of odd sizes),
+    taken in FP32 (FP64 for float64 maps) and rounded to the map's dtype."""
+    _contig(fmap=fmap)
+    _gpu(fmap)
+    E, H, W, Cc = fmap.shape
+    out = torch.empty((E, H // 2, W // 2, Cc), dtype=fmap.dtype, device=fmap.device)
+    with torch.cuda.device(fmap.device):
+        rc = _lib.load().vh_pool2x2_nhwc(_code(fmap, _FEAT_DT), _lib.ptr(fmap), E, H, W, Cc, _lib.ptr(out),
+                                         _lib.stream_of(fmap))
+    _lib.check(rc, "pool2x2_nhwc")
+    return out
+
+
+def corr_feature_grad(g1_levels, g2_levels, scale, dtype):
+    """Gradients of CorrBlock's two feature maps from the per-level altcorr
+    backward outputs: g1_levels[l] f32 [E, H, W, C], g2_levels[l] f32
+    [E, H >> l, W >> l, C] -> (df1, df2) [E, C, H, W] in dtype, with
+    df1 = scale * sum_l g1_l and df2 = scale * sum_l 4^-l g2_l spread over the
+    2^l x 2^l blocks."""
+    L = len(g1_levels)
+    for t in (*g1_levels, *g2_levels):
+        _contig(grad=t)
+    _gpu(*g1_levels, *g2_levels)
+    E, H, W, Cc = g1_levels[0].shape
+    df1 = torch.empty((E, Cc, H, W), dtype=dtype, device=g1_levels[0].device)
+    df2 = torch.empty((E, Cc, H, W), dtype=dtype, device=g1_levels[0].device)
+    p1 = (C.c_void_p * L)(*[t.data_ptr() for t in g1_levels])
+    p2 = (C.c_void_p * L)(*[t.data_ptr() for t in g2_levels])
+    with torch.cuda.device(df1.device):
+        rc = _lib.load().vh_corr_feature_grad(_code(df1, _FEAT_DT), E, Cc, H, W, L, p1, p2, float(scale),
+                                              _lib.ptr(df1), _lib.ptr(df2), _lib.stream_of(df1))
+    _lib.check(rc, "corr_feature_grad")
+    return df1, df2
+
+
 def altcorr_index_forward(fmap1, fmap2_pyramid, coords, ii, jj, radius):
     _contig(fmap1=fmap1, coords=coords, ii=ii, jj=jj)
     for f in fmap2_pyramid:

@@ -263,6 +263,48 @@ def _update_op(edges: int, per_frame: int, iters: int) -> None:
           f"net {tuple(out[0].shape)}, upmask {tuple(out[4].shape)}", flush=True)
 
 
+def _feature_grad(workload: str, iters: int) -> None:
+    """CorrBlock built from features that need gradients: the fused lookup and its
+    backward to both feature maps (altcorr backward per level, NHWC pyramid and
+    gradient combination kernels), timed with HIP events over ``iters`` calls."""
+    import torch
+
+    from . import synth
+    from .corr import CorrBlock
+
+    g = synth.make_graph(workload, seed=0)
+    dev = torch.device("cuda", 0)
+    f1 = g.fmaps[g.ii][None].to(dev).requires_grad_(True)
+    f2 = g.fmaps[g.jj][None].to(dev).requires_grad_(True)
+    coords = g.coords[None].to(dev)
+    gen = torch.Generator(device=dev).manual_seed(0)
+    go = None
+
+    def step():
+        nonlocal go
+        blk = CorrBlock(f1, f2, num_levels=4, radius=3)
+        out = blk(coords)
+        if go is None:
+            go = torch.randn(out.shape, device=dev, generator=gen)
+        t0 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        gf1, gf2 = torch.autograd.grad(out, (f1, f2), go)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t1.record()
+        return t0, t1, gf1, gf2
+
+    step()
+    torch.cuda.synchronize()
+    tb = 0.0
+    for _ in range(iters):
+        t0, t1, _, _ = step()
+        torch.cuda.synchronize()
+        tb += t0.elapsed_time(t1)
+    E, C, H, W = g.fmaps[g.ii].shape
+    print(f"feature_grad: {workload}: {E} edges, C={C}, {H}x{W}: backward to both feature maps "
+          f"{tb / iters:.3f} ms per call ({iters} calls)", flush=True)
+
+
 def _ba_rig(camera: str, views: int, iters: int) -> None:
     import torch
 
@@ -374,6 +416,9 @@ def main(argv=None) -> None:
     br.add_argument("--camera", choices=["mei", "pinhole"], default="mei")
     br.add_argument("--views", type=int, default=2)
     br.add_argument("--iters", type=int, default=8)
+    fg = sub.add_parser("feature_grad")
+    fg.add_argument("--workload", default="kf32_e192_48x64")
+    fg.add_argument("--iters", type=int, default=5)
     uo = sub.add_parser("update_op")
     uo.add_argument("--edges", type=int, default=384)
     uo.add_argument("--per-frame", type=int, default=6)
@@ -406,6 +451,8 @@ def main(argv=None) -> None:
         _corr_index(a.dtype, a.edges, a.iters)
     elif a.cmd == "ba_rig":
         _ba_rig(a.camera, a.views, a.iters)
+    elif a.cmd == "feature_grad":
+        _feature_grad(a.workload, a.iters)
     elif a.cmd == "update_op":
         _update_op(a.edges, a.per_frame, a.iters)
     else:
