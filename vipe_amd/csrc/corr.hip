@@ -1010,7 +1010,7 @@ __global__ __launch_bounds__(256) void altcorr_bwd_wave_kernel(const float* __re
                                                                const float* __restrict__ coords,
                                                                const float* __restrict__ gout, float* __restrict__ g1,
                                                                float* __restrict__ g2, int64_t B, int N, int H1,
-                                                               int W1, int H2, int W2, int C,
+                                                               int W1, int H2, int W2, int C, bool vec2,
                                                                float* __restrict__ taps = nullptr,
                                                                int2* __restrict__ org = nullptr) {
     constexpr int RD = 2 * R + 1;
@@ -1024,7 +1024,61 @@ __global__ __launch_bounds__(256) void altcorr_bwd_wave_kernel(const float* __re
     const float* f1 = fmap1 + (b * HW1 + yx) * C;
     const float* f2b = fmap2 + b * (int64_t)H2 * W2 * C;
     float* g2b = g2 + b * (int64_t)H2 * W2 * C;
-    for (int c0 = 0; c0 < C; c0 += 64) {
+    // even C: lanes take channel pairs (one 8-byte load per tap row), 128 channels a pass
+    const int cstep = vec2 ? 128 : 64;
+    for (int c0 = 0; c0 < C; c0 += cstep) {
+        if (vec2) {
+            const int c = c0 + 2 * lane;
+            const bool cok = c < C;
+            const float2 a = cok ? *reinterpret_cast<const float2*>(f1 + c) : make_float2(0.f, 0.f);
+            float2 acc = make_float2(0.f, 0.f);
+            for (int n = 0; n < N; n++) {
+                const int64_t bn = b * N + n;
+                const int64_t gid = bn * HW1 + yx;
+                const float x = coords[gid * 2 + 0], y = coords[gid * 2 + 1];
+                const float fx0 = floorf(x), fy0 = floorf(y);
+                const float dx = x - fx0, dy = y - fy0;
+                const int xb = (int)fx0 - R, yb = (int)fy0 - R;
+                const float* go = gout + bn * RD * RD * HW1 + yx;
+                float gt[NT][NT];
+#pragma unroll
+                for (int i = 0; i < NT; i++)
+#pragma unroll
+                    for (int j = 0; j < NT; j++) gt[i][j] = 0.f;
+#pragma unroll
+                for (int ix = 0; ix < RD; ix++)
+#pragma unroll
+                    for (int iy = 0; iy < RD; iy++) {
+                        const float gv = go[(ix * RD + iy) * HW1];
+                        gt[iy][ix] += gv * (1 - dy) * (1 - dx);
+                        gt[iy][ix + 1] += gv * (1 - dy) * dx;
+                        gt[iy + 1][ix] += gv * dy * (1 - dx);
+                        gt[iy + 1][ix + 1] += gv * dy * dx;
+                    }
+#pragma unroll
+                for (int iy = 0; iy < NT; iy++) {
+                    const int h2 = yb + iy;
+                    if (h2 < 0 || h2 >= H2) continue;
+#pragma unroll
+                    for (int ix = 0; ix < NT; ix++) {
+                        const int w2 = xb + ix;
+                        if (w2 < 0 || w2 >= W2) continue;
+                        if (cok) {
+                            const int64_t off = ((int64_t)h2 * W2 + w2) * C + c;
+                            const float2 v = *reinterpret_cast<const float2*>(f2b + off);
+                            acc.x += gt[iy][ix] * v.x;
+                            acc.y += gt[iy][ix] * v.y;
+                            if (!GATHER) {
+                                atomicAdd(g2b + off, gt[iy][ix] * a.x);
+                                atomicAdd(g2b + off + 1, gt[iy][ix] * a.y);
+                            }
+                        }
+                    }
+                }
+            }
+            if (cok) *reinterpret_cast<float2*>(g1 + (b * HW1 + yx) * C + c) = acc;
+            continue;
+        }
         const int c = c0 + lane;
         const bool cok = c < C;
         const float a = cok ? f1[c] : 0.f;
@@ -2398,6 +2452,10 @@ int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int
     if (radius != 3) return fail(VH_ERR_UNSUPPORTED, "altcorr_backward: radius %d (kernels built for r=3)", radius);
     hipStream_t s = (hipStream_t)stream;
     const int64_t nwin = B * (int64_t)N * H1 * W1;
+    // channel pairs per lane where C is even and the maps are 8-byte aligned
+    const bool vec2 = C % 2 == 0 && (reinterpret_cast<uintptr_t>(fmap1) & 7) == 0 &&
+                      (reinterpret_cast<uintptr_t>(fmap2) & 7) == 0 && (reinterpret_cast<uintptr_t>(fmap1_grad) & 7) == 0 &&
+                      (reinterpret_cast<uintptr_t>(fmap2_grad) & 7) == 0;
     // the gather path: C in whole float4 groups and a row of fmap2_grad in LDS
     int gw = 0;
     if (workspace && C % 4 == 0 && (reinterpret_cast<uintptr_t>(fmap1) & 15) == 0 &&
@@ -2412,7 +2470,7 @@ int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int
         if (B * (int64_t)H1 * W1 * C > 0) {
             hipLaunchKernelGGL((altcorr_bwd_wave_kernel<3, true>), dim3(ceil_div(B * (int64_t)H1 * W1, 4)), dim3(256),
                                0, s, (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad,
-                               (float*)fmap1_grad, (float*)fmap2_grad, B, N, H1, W1, H2, W2, C, taps, org);
+                               (float*)fmap1_grad, (float*)fmap2_grad, B, N, H1, W1, H2, W2, C, vec2, taps, org);
             VH_LAUNCH_CHECK("altcorr_backward");
         }
         if (B * (int64_t)H2 * W2 * C > 0) {
@@ -2435,7 +2493,7 @@ int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int
     if (B * (int64_t)H1 * W1 * C == 0) return VH_OK;
     hipLaunchKernelGGL(altcorr_bwd_wave_kernel<3>, dim3(ceil_div(B * (int64_t)H1 * W1, 4)), dim3(256), 0, s,
                        (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad, (float*)fmap1_grad,
-                       (float*)fmap2_grad, B, N, H1, W1, H2, W2, C);
+                       (float*)fmap2_grad, B, N, H1, W1, H2, W2, C, vec2);
     VH_LAUNCH_CHECK("altcorr_backward");
     return VH_OK;
 }
