@@ -368,21 +368,25 @@ def test_altcorr_index_matrix_core_path(C):
     assert err < 2e-5 * max(1.0, ref.abs().max().item())
 
 
-@pytest.mark.parametrize("C,W,conc", [(128, 80, False), (96, 24, True), (36, 20, False)])
-def test_altcorr_backward_gather_is_deterministic(C, W, conc):
+@pytest.mark.parametrize("C,W,conc,spread", [(128, 80, False, 2.0), (96, 24, True, 2.0), (36, 20, False, 2.0),
+                                             (64, 40, False, 0.2), (28, 21, False, 0.2)])
+def test_altcorr_backward_gather_is_deterministic(C, W, conc, spread):
     """fmap2_grad gathered by target rows (no atomics): bit-identical on repeated
     calls, equal to the FP64 oracle, and close to the atomic path (workspace
     NULL) of the same entry point; with every window crowded onto a few target
-    rows (conc) and with channel counts of 128, 96 and 36."""
+    rows (conc), with channel counts of 128, 96, 64, 36 and 28, with smooth
+    coordinates (four neighbouring windows share one bounding box) and with
+    scattered ones (each window walked alone), and with H * W not a multiple
+    of four (one pixel per wave)."""
     from vipe_amd import _lib
     from vipe_amd.ext import droid_net_ext as D
 
     g = torch.Generator().manual_seed(70 + C)
-    B, N, H, r = 2, 3, 12, 3
+    B, N, H, r = 2, 3, 12 if W % 2 == 0 else 13, 3
     H2, W2 = H, W
     f1 = torch.randn(B, H, W, C, generator=g) / math.sqrt(C)
     f2 = torch.randn(B, H2, W2, C, generator=g) / math.sqrt(C)
-    coords = _coords(B * N, H, W, H2, W2, g, spread=2.0).view(B, N, 2, H, W).permute(0, 1, 3, 4, 2).contiguous()
+    coords = _coords(B * N, H, W, H2, W2, g, spread=spread).view(B, N, 2, H, W).permute(0, 1, 3, 4, 2).contiguous()
     if conc:
         coords[..., 1] = 4.0 + 0.5 * torch.rand(coords[..., 1].shape, generator=g)
     cg = torch.randn(B, N, 49, H, W, generator=g)

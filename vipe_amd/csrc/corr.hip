@@ -1145,6 +1145,122 @@ __global__ __launch_bounds__(256) void altcorr_bwd_wave_kernel(const float* __re
     }
 }
 
+// The fmap1 gradient and the per-window tap gradients of the atomic-free
+// backward, four neighbouring source pixels per wave (H1 * W1 a multiple of 4,
+// C even).  Lanes take channel pairs, 128 channels a pass.  Lane t first forms
+// tap t's gradient of each of the four windows (kept in a register, stored to
+// the workspace for the gather); the wave then walks the bounding box of the
+// four windows and loads each target pixel's features once for all the windows
+// that contain it, broadcasting their tap weights with readlane.  Groups whose
+// windows lie far apart (bounding box above kAltGroupArea pixels) walk each
+// window separately.
+constexpr int kAltGroupArea = 144;
+
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void altcorr_bwd_group_kernel(const float* __restrict__ fmap1,
+                                                                const float* __restrict__ fmap2,
+                                                                const float* __restrict__ coords,
+                                                                const float* __restrict__ gout,
+                                                                float* __restrict__ g1, int64_t B, int N, int H1,
+                                                                int W1, int H2, int W2, int C,
+                                                                float* __restrict__ taps, int2* __restrict__ org) {
+    constexpr int RD = 2 * R + 1;
+    constexpr int NT = RD + 1;
+    static_assert(NT * NT == 64, "one tap per lane");
+    constexpr int PG = 4;  // source pixels per wave
+    const int64_t HW1 = (int64_t)H1 * W1;
+    const int64_t grp = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (grp * PG >= B * HW1) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t b = grp * PG / HW1;
+    const int64_t yx0 = grp * PG - b * HW1;
+    const float* f2b = fmap2 + b * (int64_t)H2 * W2 * C;
+    const int ty = lane >> 3, tx = lane & 7;
+    for (int cb = 0; cb < C; cb += 128) {
+        const int c = cb + 2 * lane;
+        const bool cok = c < C;
+        float2 acc[PG];
+#pragma unroll
+        for (int p = 0; p < PG; p++) acc[p] = make_float2(0.f, 0.f);
+        for (int n = 0; n < N; n++) {
+            const int64_t bn = b * N + n;
+            int xb[PG], yb[PG];
+            float gtv[PG];  // tap (ty, tx) gradient of window p
+#pragma unroll
+            for (int p = 0; p < PG; p++) {
+                const int64_t gid = bn * HW1 + yx0 + p;
+                const float x = coords[gid * 2 + 0], y = coords[gid * 2 + 1];
+                const float fx0 = floorf(x), fy0 = floorf(y);
+                const float dx = x - fx0, dy = y - fy0;
+                xb[p] = (int)fx0 - R;
+                yb[p] = (int)fy0 - R;
+                const float* go = gout + bn * RD * RD * HW1 + yx0 + p;
+                float gv = 0.f;
+                if (tx < RD && ty < RD) gv += go[(tx * RD + ty) * HW1] * (1 - dy) * (1 - dx);
+                if (tx >= 1 && ty < RD) gv += go[((tx - 1) * RD + ty) * HW1] * (1 - dy) * dx;
+                if (tx < RD && ty >= 1) gv += go[(tx * RD + ty - 1) * HW1] * dy * (1 - dx);
+                if (tx >= 1 && ty >= 1) gv += go[((tx - 1) * RD + ty - 1) * HW1] * dy * dx;
+                gtv[p] = gv;
+                if (cb == 0) {
+                    taps[gid * NT * NT + lane] = gv;
+                    if (lane == 0) org[gid] = make_int2(xb[p], yb[p]);
+                }
+            }
+            int ymin = yb[0], ymax = yb[0], xmin = xb[0], xmax = xb[0];
+#pragma unroll
+            for (int p = 1; p < PG; p++) {
+                ymin = min(ymin, yb[p]);
+                ymax = max(ymax, yb[p]);
+                xmin = min(xmin, xb[p]);
+                xmax = max(xmax, xb[p]);
+            }
+            ymax += NT - 1;
+            xmax += NT - 1;
+            if ((int64_t)(ymax - ymin + 1) * (xmax - xmin + 1) <= kAltGroupArea) {
+                const int y0 = max(ymin, 0), y1 = min(ymax, H2 - 1), x0 = max(xmin, 0), x1 = min(xmax, W2 - 1);
+                for (int y = y0; y <= y1; y++)
+                    for (int x = x0; x <= x1; x++) {
+                        const float2 v = cok ? *reinterpret_cast<const float2*>(f2b + ((int64_t)y * W2 + x) * C + c)
+                                             : make_float2(0.f, 0.f);
+#pragma unroll
+                        for (int p = 0; p < PG; p++) {
+                            const int ry = y - yb[p], rx = x - xb[p];
+                            if ((unsigned)ry < (unsigned)NT && (unsigned)rx < (unsigned)NT) {
+                                const float w = readlane_f32(gtv[p], ry * NT + rx);
+                                acc[p].x += w * v.x;
+                                acc[p].y += w * v.y;
+                            }
+                        }
+                    }
+            } else {
+#pragma unroll
+                for (int p = 0; p < PG; p++)
+                    for (int ry = 0; ry < NT; ry++) {
+                        const int y = yb[p] + ry;
+                        if (y < 0 || y >= H2) continue;
+                        for (int rx = 0; rx < NT; rx++) {
+                            const int x = xb[p] + rx;
+                            if (x < 0 || x >= W2) continue;
+                            const float2 v = cok ? *reinterpret_cast<const float2*>(f2b + ((int64_t)y * W2 + x) * C + c)
+                                                 : make_float2(0.f, 0.f);
+                            const float w = readlane_f32(gtv[p], ry * NT + rx);
+                            acc[p].x += w * v.x;
+                            acc[p].y += w * v.y;
+                        }
+                    }
+            }
+        }
+        if (cok) {
+#pragma unroll
+            for (int p = 0; p < PG; p++) *reinterpret_cast<float2*>(g1 + (b * HW1 + yx0 + p) * C + c) = acc[p];
+        }
+    }
+}
+
 // fmap2 gradient without atomics: one workgroup per target row (b, y2).  The
 // windows (n, source pixel) of batch b are scanned in ascending order, those
 // that cover row y2 are compacted (order kept), and kAltWin at a time their
@@ -2468,9 +2584,15 @@ int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int
         float* taps = (float*)workspace;
         int2* org = (int2*)((char*)workspace + ((size_t)std::max<int64_t>(nwin, 0) * 64 * sizeof(float) + 255) / 256 * 256);
         if (B * (int64_t)H1 * W1 * C > 0) {
-            hipLaunchKernelGGL((altcorr_bwd_wave_kernel<3, true>), dim3(ceil_div(B * (int64_t)H1 * W1, 4)), dim3(256),
-                               0, s, (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad,
-                               (float*)fmap1_grad, (float*)fmap2_grad, B, N, H1, W1, H2, W2, C, vec2, taps, org);
+            if (vec2 && ((int64_t)H1 * W1) % 4 == 0)
+                hipLaunchKernelGGL(altcorr_bwd_group_kernel<3>, dim3(ceil_div(B * (int64_t)H1 * W1 / 4, 4)), dim3(256),
+                                   0, s, (const float*)fmap1, (const float*)fmap2, coords, (const float*)corr_grad,
+                                   (float*)fmap1_grad, B, N, H1, W1, H2, W2, C, taps, org);
+            else
+                hipLaunchKernelGGL((altcorr_bwd_wave_kernel<3, true>), dim3(ceil_div(B * (int64_t)H1 * W1, 4)),
+                                   dim3(256), 0, s, (const float*)fmap1, (const float*)fmap2, coords,
+                                   (const float*)corr_grad, (float*)fmap1_grad, (float*)fmap2_grad, B, N, H1, W1, H2,
+                                   W2, C, vec2, taps, org);
             VH_LAUNCH_CHECK("altcorr_backward");
         }
         if (B * (int64_t)H2 * W2 * C > 0) {
