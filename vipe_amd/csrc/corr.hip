@@ -560,7 +560,9 @@ __global__ __launch_bounds__(256) void corr_lookup_tiled_kernel(LevelPtrs lv, in
             const uint32_t rowoff = lvo + (uint32_t)(tiled_yx(y, 0, ntx, TW) * (int)sizeof(T));
 #pragma unroll
             for (int q = 0; q < NL; q++) {
-                const bool ok = yok && (unsigned)(xt + q) < (unsigned)ntx;
+                // a window on a piece boundary (sh == 0) needs one piece fewer:
+                // the last one reads nothing and stays zero
+                const bool ok = yok && (unsigned)(xt + q) < (unsigned)ntx && (q + 1 < NL || sh != 0);
                 g[r][q] = __builtin_amdgcn_raw_buffer_load_b128(rl, ok ? rowoff + (uint32_t)(xt + q) * kTileBytes
                                                                        : kBufInvalid, 0, 0);
             }
