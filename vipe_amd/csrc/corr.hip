@@ -2575,11 +2575,17 @@ int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int
                       (reinterpret_cast<uintptr_t>(fmap2) & 7) == 0 && (reinterpret_cast<uintptr_t>(fmap1_grad) & 7) == 0 &&
                       (reinterpret_cast<uintptr_t>(fmap2_grad) & 7) == 0;
     // the gather path: C in whole float4 groups and a row of fmap2_grad in LDS
+    // (windows per pass: 64 or 32 if that leaves room for three workgroups per
+    // CU, else the most that fit one; fewer than 32 cost more in passes than
+    // the occupancy gains)
     int gw = 0;
     if (workspace && C % 4 == 0 && (reinterpret_cast<uintptr_t>(fmap1) & 15) == 0 &&
-        (reinterpret_cast<uintptr_t>(fmap2_grad) & 15) == 0)
+        (reinterpret_cast<uintptr_t>(fmap2_grad) & 15) == 0) {
+        for (int w = kAltWin; w >= 32 && !gw; w /= 2)
+            if (altcorr_gather_lds(W2, C, w) <= 53 * 1024) gw = w;
         for (int w = kAltWin; w >= 8 && !gw; w /= 2)
             if (altcorr_gather_lds(W2, C, w) <= 150 * 1024) gw = w;
+    }
     if (gw) {
         // fmap1_grad and the tap gradients per window, then fmap2_grad by target
         // rows: both gradients fully written, no atomics
