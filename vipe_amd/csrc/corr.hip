@@ -934,7 +934,8 @@ __global__ __launch_bounds__(256) void altcorr_index_wave_kernel(const T* __rest
     __shared__ float S[4][RD + 1][RD + 2];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t HW = (int64_t)H * W;
-    const int64_t wid = (int64_t)blockIdx.x * 4 + wv;  // over B*M*HW
+    // XCD-contiguous: neighbouring pixels read overlapping target rows
+    const int64_t wid = xcd_logical(blockIdx.x, gridDim.x) * 4 + wv;  // over B*M*HW
     if (wid >= B * M * HW) return;
     const int64_t bm = wid / HW;
     const int64_t yx = wid - bm * HW;
@@ -957,7 +958,8 @@ __global__ __launch_bounds__(256) void altcorr_fwd_wave_kernel(const T* __restri
     __shared__ float S[4][RD + 1][RD + 2];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t HW1 = (int64_t)H1 * W1;
-    const int64_t wid = (int64_t)blockIdx.x * 4 + wv;  // over B*N*HW1
+    // XCD-contiguous: neighbouring pixels read overlapping target rows
+    const int64_t wid = xcd_logical(blockIdx.x, gridDim.x) * 4 + wv;  // over B*N*HW1
     if (wid >= B * N * HW1) return;
     const int64_t bn = wid / HW1;
     const int64_t yx = wid - bn * HW1;
@@ -1018,7 +1020,8 @@ __global__ __launch_bounds__(256) void altcorr_bwd_wave_kernel(const float* __re
     constexpr int RD = 2 * R + 1;
     constexpr int NT = RD + 1;
     const int64_t HW1 = (int64_t)H1 * W1;
-    const int64_t wid = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // XCD-contiguous: neighbouring pixels read overlapping target rows
+    const int64_t wid = xcd_logical(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wid >= B * HW1) return;
     const int lane = threadIdx.x & 63;
     const int64_t b = wid / HW1;
@@ -1175,7 +1178,9 @@ __global__ __launch_bounds__(256) void altcorr_bwd_group_kernel(const float* __r
     static_assert(NT * NT == 64, "one tap per lane");
     constexpr int PG = 4;  // source pixels per wave
     const int64_t HW1 = (int64_t)H1 * W1;
-    const int64_t grp = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // each XCD takes a contiguous range of groups: neighbouring groups read
+    // overlapping target rows, which then come from that XCD's L2
+    const int64_t grp = xcd_logical(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (grp * PG >= B * HW1) return;
     const int lane = threadIdx.x & 63;
     const int64_t b = grp * PG / HW1;
@@ -1291,8 +1296,11 @@ __global__ __launch_bounds__(256) void altcorr_bwd_gather_kernel(const float* __
     __shared__ int wl[256];
     __shared__ int wsum[4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t b = blockIdx.x / H2;
-    const int y2 = (int)(blockIdx.x - b * H2);
+    // each XCD takes a contiguous range of rows: neighbouring rows stage the
+    // same windows, whose features then come from that XCD's L2
+    const int64_t blk = xcd_logical(blockIdx.x, gridDim.x);
+    const int64_t b = blk / H2;
+    const int y2 = (int)(blk - b * H2);
     const int64_t HW1 = (int64_t)H1 * W1;
     const int64_t nw = (int64_t)N * HW1;  // windows of batch b
     for (int i = tid; i < W2 * C4; i += 256) acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
