@@ -1326,18 +1326,18 @@ __global__ __launch_bounds__(256) void altcorr_bwd_gather_kernel(const float* __
         for (int sub = 0; sub < total; sub += nwin) {
             const int cnt = min(nwin, total - sub);
             for (int j = tid; j < cnt; j += 256) {
-                const int64_t id = s0 + wl[sub + j];  // n * HW1 + pixel
+                const int64_t id = s0 + wl[sub + j];  // n * HW1 + pixel (below 2^31)
                 const int2 o = org[b * nw + id];
                 sX[j] = o.x;
-                sP[j] = (int)(id % HW1);
+                sP[j] = (int)((uint32_t)id % (uint32_t)HW1);
                 const float* tp = taps + (b * nw + id) * NT * NT + (y2 - o.y) * NT;
 #pragma unroll
                 for (int t = 0; t < NT; t++) sG[j * NT + t] = tp[t];
             }
             for (int i = tid; i < cnt * C4; i += 256) {
                 const int j = i / C4, cg = i - j * C4;
-                const int64_t id = s0 + wl[sub + j];
-                sF[i] = *reinterpret_cast<const float4*>(fmap1 + (b * HW1 + id % HW1) * C + 4 * cg);
+                const uint32_t pix = (uint32_t)(s0 + wl[sub + j]) % (uint32_t)HW1;
+                sF[i] = *reinterpret_cast<const float4*>(fmap1 + (b * HW1 + pix) * C + 4 * cg);
             }
             __syncthreads();
             for (int sl = tid; sl < NT * C4; sl += 256) {
@@ -2582,13 +2582,14 @@ int vh_altcorr_backward(int dtype, int64_t B, int N, int H1, int W1, int H2, int
     const bool vec2 = C % 2 == 0 && (reinterpret_cast<uintptr_t>(fmap1) & 7) == 0 &&
                       (reinterpret_cast<uintptr_t>(fmap2) & 7) == 0 && (reinterpret_cast<uintptr_t>(fmap1_grad) & 7) == 0 &&
                       (reinterpret_cast<uintptr_t>(fmap2_grad) & 7) == 0;
-    // the gather path: C in whole float4 groups and a row of fmap2_grad in LDS
+    // the gather path: C in whole float4 groups, a row of fmap2_grad in LDS and
+    // a batch's windows numbered in 31 bits
     // (windows per pass: 64 or 32 if that leaves room for three workgroups per
     // CU, else the most that fit one; fewer than 32 cost more in passes than
     // the occupancy gains)
     int gw = 0;
     if (workspace && C % 4 == 0 && (reinterpret_cast<uintptr_t>(fmap1) & 15) == 0 &&
-        (reinterpret_cast<uintptr_t>(fmap2_grad) & 15) == 0) {
+        (reinterpret_cast<uintptr_t>(fmap2_grad) & 15) == 0 && (int64_t)N * H1 * W1 < (int64_t{1} << 31)) {
         for (int w = kAltWin; w >= 32 && !gw; w /= 2)
             if (altcorr_gather_lds(W2, C, w) <= 53 * 1024) gw = w;
         for (int w = kAltWin; w >= 8 && !gw; w /= 2)
