@@ -140,7 +140,8 @@ int vh_altcorr_forward(int dtype, int64_t B, int N, int H1, int W1, int H2, int 
  * reference returns zeros).  With a workspace of
  * vh_altcorr_backward_workspace_size bytes (device) the f2 gradient is gathered
  * per target row with no atomics, the same on every run (C a multiple of 4,
- * 16-byte aligned fmap1 and fmap2_grad, a row of W2 x C floats within LDS);
+ * 16-byte aligned fmap1 and fmap2_grad, a row of W2 x C floats within LDS,
+ * N x H1 x W1 below 2^31);
  * otherwise (workspace NULL) it is cleared and accumulated with FP32 atomic adds
  * like the reference, whose summation order varies from run to run. */
 size_t vh_altcorr_backward_workspace_size(int64_t B, int N, int H1, int W1);
