@@ -421,6 +421,113 @@ __global__ __launch_bounds__(256) void uo_conv_small_kernel(const _Float16* __re
     }
 }
 
+// The same convolution with 16-byte loads, where the channel counts and offsets
+// are multiples of 8: lane l takes channels 8 (l % 16) .. +7 of a 128-channel
+// group and pixel slot l / 16, so one load instruction reads four pixels' whole
+// channel rows.  A wave takes its 64 pixels 32 at a time (8 per lane), loads
+// every pixel's row of a tap before using any, and sums the 8 x Cout values of
+// each lane over its 16-lane row by halving: lane j of a row ends with value
+// j % (8 Cout), pixel j / Cout of the row's slot, output j % Cout.
+template <int NV>
+__device__ __forceinline__ float uo_row_sum(float (&v)[16], int lane) {
+    if constexpr (NV == 8) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] += __shfl_xor(v[i], 8, 16);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float a = v[i] + __shfl_xor(v[i], 8, 16), b = v[i + 8] + __shfl_xor(v[i + 8], 8, 16);
+            v[i] = (lane & 8) ? b : a;
+        }
+    }
+#pragma unroll
+    for (int m = 4; m >= 1; m >>= 1)
+#pragma unroll
+        for (int i = 0; i < m; i++) {
+            const float a = v[i] + __shfl_xor(v[i], m, 16), b = v[i + m] + __shfl_xor(v[i + m], m, 16);
+            v[i] = (lane & m) ? b : a;
+        }
+    return v[0];
+}
+
+template <int COUT>
+__global__ __launch_bounds__(256) void uo_conv_small_vec_kernel(const _Float16* __restrict__ in, int in_stride,
+                                                                int in_off, int cin, const float* __restrict__ w,
+                                                                const float* __restrict__ bias, int nimg, int H, int W,
+                                                                int KH, int KW, int epi, float* __restrict__ out,
+                                                                int out_stride) {
+    constexpr int PPL = 8;        // pixels per lane and batch
+    constexpr int NV = PPL * COUT;  // values per lane
+    const int lane = threadIdx.x & 63, cg = lane & 15, ps = lane >> 4;
+    const int HW = H * W;
+    const int P = nimg * HW;  // < 2^31, checked by the host
+    const int pw0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    const int ntap = KH * KW;
+    const int ngrp = (cin + 127) / 128;
+    for (int b0 = 0; b0 < 64; b0 += 4 * PPL) {
+        // pixel b0 + 4 k + ps of the wave for k < PPL; one past P gets a row
+        // that fails every bounds test
+        int pb[PPL], py[PPL], px[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; k++) {
+            const int p = pw0 + b0 + 4 * k + ps;
+            const int img = p / HW;
+            const int rem = p - img * HW;
+            pb[k] = img * HW;
+            py[k] = p < P ? rem / W : -(1 << 20);
+            px[k] = rem - (rem / W) * W;
+        }
+        float acc[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[i] = 0.f;
+        for (int g = 0; g < ngrp; g++) {
+            const int c = 128 * g + 8 * cg;
+            const bool cok = c < cin;  // cin is a multiple of 8
+            for (int t = 0; t < ntap; t++) {
+                const int dy = t / KW - KH / 2, dx = t % KW - KW / 2;
+                float4 wq[2 * COUT];  // weights [8 channels][COUT] of the tap
+#pragma unroll
+                for (int u = 0; u < 2 * COUT; u++)
+                    wq[u] = cok ? reinterpret_cast<const float4*>(w + ((int64_t)t * cin + c) * COUT)[u]
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+                const float* wv = reinterpret_cast<const float*>(wq);
+                uint4 x[PPL];
+#pragma unroll
+                for (int k = 0; k < PPL; k++) {
+                    const int yy = py[k] + dy, xx = px[k] + dx;
+                    const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W && cok;
+                    const int64_t o = ok ? (int64_t)(pb[k] + yy * W + xx) * in_stride + in_off + c : 0;
+                    const uint4 v = *reinterpret_cast<const uint4*>(in + o);
+                    x[k] = ok ? v : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int k = 0; k < PPL; k++) {
+                    const unsigned d[4] = {x[k].x, x[k].y, x[k].z, x[k].w};
+#pragma unroll
+                    for (int e2 = 0; e2 < 4; e2++) {
+                        const __half2 h = *reinterpret_cast<const __half2*>(&d[e2]);
+                        const float x0 = __low2float(h), x1 = __high2float(h);
+#pragma unroll
+                        for (int o = 0; o < COUT; o++)
+                            acc[k * COUT + o] += x0 * wv[(2 * e2) * COUT + o] + x1 * wv[(2 * e2 + 1) * COUT + o];
+                    }
+                }
+            }
+        }
+        const float tot = uo_row_sum<NV>(acc, lane);
+        const int j = lane & 15;
+        const int p = pw0 + b0 + 4 * ((j % NV) / COUT) + ps;
+        const int o = j % COUT;
+        if (j < NV && p < P) {
+            float v = tot + bias[o];
+            if (epi == VH_UO_SIGMOID) v = sigmoidf_(v);
+            else if (epi == VH_UO_ETA) v = 0.01f * softplusf_(v);
+            else if (epi == VH_UO_RELU) v = fmaxf(v, 0.f);
+            out[(int64_t)p * out_stride + o] = v;
+        }
+    }
+}
+
 // Layout changes at the module boundary, as register transposes of 8 x 8
 // blocks.  Lane l of a wave takes channel group l % 8 (8 channels) and pixel
 // group l / 8 (8 pixels), so a wave covers 64 channels x 64 pixels: each of its
@@ -727,7 +834,16 @@ int vh_uo_conv_small(const void* in, int in_stride, int in_off, int cin, const f
     const int64_t P = (int64_t)nimg * H * W;
     if (P == 0) return VH_OK;
     if (P + 256 >= (int64_t)1 << 31) return fail(VH_ERR_UNSUPPORTED, "uo_conv_small: %lld pixels", (long long)P);
-    if (Cout == 2)
+    if (cin % 8 == 0 && in_stride % 8 == 0 && in_off % 8 == 0 && ((uintptr_t)weight & 15) == 0) {
+        if (Cout == 2)
+            hipLaunchKernelGGL(uo_conv_small_vec_kernel<2>, dim3(ceil_div(P, 256)), dim3(256), 0,
+                               (hipStream_t)stream, (const _Float16*)in, in_stride, in_off, cin, weight, bias, nimg, H,
+                               W, KH, KW, epilogue, out, out_stride);
+        else
+            hipLaunchKernelGGL(uo_conv_small_vec_kernel<1>, dim3(ceil_div(P, 256)), dim3(256), 0,
+                               (hipStream_t)stream, (const _Float16*)in, in_stride, in_off, cin, weight, bias, nimg, H,
+                               W, KH, KW, epilogue, out, out_stride);
+    } else if (Cout == 2)
         hipLaunchKernelGGL(uo_conv_small_kernel<2>, dim3(ceil_div(P, 256)), dim3(256), 0, (hipStream_t)stream,
                            (const _Float16*)in, in_stride, in_off, cin, weight, bias, nimg, H, W, KH, KW, epilogue,
                            out, out_stride);
