@@ -189,8 +189,9 @@ SMALL_CASES = [
 ]
 
 
+@pytest.mark.parametrize("wshift", [0, 1], ids=["wvec", "wscalar"])
 @pytest.mark.parametrize("case", SMALL_CASES, ids=lambda c: f"k{c[0]}_c{c[1]}@{c[2]}_o{c[4]}_{c[5]}_{c[7]}x{c[8]}")
-def test_small_conv_heads_match_fp64(case):
+def test_small_conv_heads_match_fp64(case, wshift):
     """The 1- and 2-channel heads against an FP64 convolution of the same f16
     inputs: only the FP32 accumulation order differs, so |err| <= 1e-5 |ref|max
     + 1e-5 (sigmoid / eta outputs compared after the activation)."""
@@ -208,7 +209,10 @@ def test_small_conv_heads_match_fp64(case):
     out = torch.full((P, cout + 1), 9.0, device=DEV)
     code = {"none": L.VH_UO_NONE, "relu": L.VH_UO_RELU, "sigmoid": L.VH_UO_SIGMOID, "eta": L.VH_UO_ETA}[epi]
     r = _runner()
-    bd, wd, bb = buf.to(DEV), wp.to(DEV), b.to(DEV)
+    # wshift 1 moves the weights off 16-byte alignment: the kernel with 4-byte loads
+    bd, bb = buf.to(DEV), b.to(DEV)
+    wd = torch.zeros(wp.numel() + wshift, device=DEV)[wshift:]
+    wd.copy_(wp.reshape(-1).to(DEV))
     L.check(r.lib.vh_uo_conv_small(_vp(bd), stride, off, cin, _vp(wd), _vp(bb), nimg, H, W, k, k, cout, code,
                                    _vp(out), cout + 1, r.stream), "uo_conv_small")
     torch.cuda.synchronize()

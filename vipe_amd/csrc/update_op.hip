@@ -19,9 +19,11 @@
 //                         epilogue adds the bias (and a per-image addend) and
 //                         applies the layer's activation, or the ConvGRU gates /
 //                         state update.
-//   uo_conv_small_kernel  convolutions with 1 or 2 output channels (the delta,
-//                         weight and eta heads) on the VALU: lanes along the
-//                         input channels, a cross-lane sum per batch of pixels.
+//   uo_conv_small_vec_kernel / uo_conv_small_kernel  convolutions with 1 or 2
+//                         output channels (the delta, weight and eta heads) on
+//                         the VALU: lanes along the input channels with 16-byte
+//                         loads (4-byte loads where the weights are not 16-byte
+//                         aligned), cross-lane sums per batch of pixels.
 //   uo_to_nhwc / uo_to_nchw_kernel  layout changes at the module boundary.
 //   uo_mean_kernel / uo_fc_kernel   ConvGRU's global context: the per-image mean
 //                         of sigmoid(w(net)) * net and the three 1x1 convolutions
